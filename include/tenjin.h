@@ -1,0 +1,172 @@
+/*
+ * tenjin.h -- C ABI of the MI355X (gfx950) per-cell explicit update.
+ *
+ * The reference has no plugin or FFI layer: its hot path is a set of C++ free
+ * functions and methods called from main_main().  Each entry point below is
+ * the device-side replacement of one of them; the C++ host layer
+ * (tenjin_amd/csrc/host/) keeps the reference's signatures and forwards here.
+ *
+ * Conventions
+ *   - Return 0 on success, a negative TJ_E* code on API misuse; a positive
+ *     status word (TJ_ST_*) is reported by tj_status() for the conditions on
+ *     which the reference prints and calls exit(1) or stops its time loop.
+ *   - All state buffers are caller-owned device memory described by tj_mf;
+ *     the library owns only its scratch (flux planes, THINC history planes,
+ *     the device-resident time-step words).
+ *   - One host thread per context; every call is asynchronous on the
+ *     context's HIP stream unless it returns a host value.
+ */
+#ifndef TENJIN_H
+#define TENJIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TJ_MAX_MATERIALS 4
+
+/* EOS kinds: 0 = MieGruneisenEOS of equationofstate.cpp:5-115 (stiffened gas,
+ * constant-Gamma Mie-Gruneisen with pref/eref); 1 = Us-up Mie-Gruneisen solid
+ * (no reference class; solid-EOS interface of equationofstate.cpp:502-596). */
+typedef struct tj_material {
+    int eos;
+    int solid;    /* MaterialDescriptor::phase == solid (structdefinitions.h:90-98) */
+    int plastic;  /* MaterialDescriptor::plastic */
+    int pad_;
+    double gamma; /* adiabaticIndex; GruneisenGamma = gamma - 1 */
+    double pref, eref, CV;
+    double rho0, c0, s, G0;
+    double yield_stress; /* PlasticEOS::yieldStress[m] (plastic.h:21) */
+} tj_material;
+
+/* Level description: what ParameterStruct (structdefinitions.h:105-144),
+ * the BoxArray of main.cpp:34-36 and the BCRec table of boundary.cpp:3-42
+ * carry for the hot path. */
+typedef struct tj_level_desc {
+    int dim;                 /* AMREX_SPACEDIM: 1, 2 or 3 */
+    int n[3];                /* parameters.n_cells */
+    double L[3];             /* parameters.dimL */
+    int max_grid_size;       /* BoxArray::maxSize chunk (main.cpp:36) */
+    int SOLID, THINC, PLASTIC, MUSCL;
+    double THINCbeta, CFL;
+    int nmat;
+    int bc_lo[3], bc_hi[3];  /* 0 transmissive (foextrap), 1 reflective */
+    tj_material mat[TJ_MAX_MATERIALS];
+} tj_level_desc;
+
+/* Non-owning view of a CellArray on the device: component-major, i fastest.
+ * data points at component 0 of valid cell (0,0,0); strides in doubles, so a
+ * buffer with AMReX ghost cells is described by its own strides. */
+typedef struct tj_mf {
+    double* data;
+    long long jstride, kstride, nstride;
+    int ncomp;
+    int pad_;
+} tj_mf;
+
+typedef struct tj_ctx tj_ctx;
+
+enum {
+    TJ_OK = 0,
+    TJ_EINVAL = -1,
+    TJ_EUNSUPPORTED = -2,
+    TJ_EHIP = -3,
+    TJ_ENOLEVEL = -4
+};
+
+/* status bits (tj_status) */
+enum {
+    TJ_ST_NAN = 1,          /* CellArray::contains_nan (cellarray.cpp:101-117) -> time loop stops */
+    TJ_ST_WAVESPEED = 2,    /* NaN wave speeds in the 5-wave solver (hllc.cpp:169-200, 367-373) */
+    TJ_ST_ALPHA_NANS = 4,   /* "too many Nans" in cleanUpAlpha (boxaccesscellarray.cpp:559-561) */
+    TJ_ST_ALPHA_SUM = 8,    /* total alpha <= 0 in cleanUpAlpha (boxaccesscellarray.cpp:583-588) */
+    TJ_ST_PLASTIC_DIV = 16  /* "Divide by zero error" in boxPlasticUpdate (plastic.cpp:220-224) */
+};
+
+/* precision forms */
+enum { TJ_PREC_FP64 = 0, TJ_PREC_MIXED = 1 };
+
+/* post-step operation bits (tj_post_step) */
+enum {
+    TJ_OP_PLASTIC = 1,      /* PlasticEOS::plasticUpdate (plastic.cpp:227-244) */
+    TJ_OP_CLEAN_V = 2,      /* CellArray::cleanUpV (boxaccesscellarray.cpp:476-528) */
+    TJ_OP_CLEAN_ALPHA = 4,  /* CellArray::cleanUpAlpha (boxaccesscellarray.cpp:530-599) */
+    TJ_OP_DT = 8,           /* TimeStep::getTimeStep (timestep.cpp:36-70) */
+    TJ_OP_NANCHECK = 16     /* CellArray::contains_nan (cellarray.cpp:101-117) */
+};
+
+/* Context on HIP device `device` with its own stream (default priority). */
+int tj_create(tj_ctx** ctx, int device);
+void tj_destroy(tj_ctx* ctx);
+const char* tj_last_error(const tj_ctx* ctx);
+/* The context's stream as a void* (hipStream_t), for callers that order their own work on it. */
+void* tj_stream(tj_ctx* ctx);
+
+/* Binds a level: box table (BoxArray::maxSize cuts), component map of
+ * AccessPattern (accesspattern.cpp:44-79), EOS/material table, BC table,
+ * dx = L/n.  Allocates the scratch (flux planes, THINC history planes).
+ * Replaces the set-up of main.cpp:29-95. */
+int tj_bind_level(tj_ctx* ctx, const tj_level_desc* desc);
+/* Number of components of the bound layout and the first component of a
+ * reference Variable (structdefinitions.h:10-32 numbering); -1 if absent. */
+int tj_ncomp(const tj_ctx* ctx);
+int tj_comp_start(const tj_ctx* ctx, int variable);
+
+/* TimeStep::getTimeStep (timestep.cpp:36-70): getSoundSpeed on the valid
+ * cells of U (writes the SOUNDSPEED component), then CFL * min over cells
+ * and directions of dx_d / (a + |u_d|).  The result stays in the context's
+ * device-resident dt word; if dt_out is non-NULL the call synchronises and
+ * returns it on the host. */
+int tj_compute_dt(tj_ctx* ctx, const tj_mf* U, double* dt_out);
+/* main.cpp:135-143: dt <- min(dt, finalT - t); dt *= 0.2 for step n < 5;
+ * operates on the device-resident words (t, n, dt). */
+int tj_driver_dt(tj_ctx* ctx, double final_time);
+/* Sets the device-resident dt (and resets t, n when reset_clock != 0). */
+int tj_set_dt(tj_ctx* ctx, double dt, int reset_clock);
+/* Reads the device-resident (t, dt, n). Synchronises. */
+int tj_get_clock(tj_ctx* ctx, double* t, double* dt, int* n);
+
+/* One Euler stage of HLLCadvance (hllc.cpp:579-695) with the device dt:
+ *   stage 1: Uout = c2p(Uin + dt L(Uin))
+ *   stage 2: Uout = 1/2 Ubase + 1/2 c2p(Uin + dt L(Uin))   (advance, hllc.cpp:699-707)
+ * Uout may alias Ubase in stage 2.  Only the persistent components are
+ * written (alpha, alphaRho, rho_k, rho, rhoU, u, E, p; V and epsilon,
+ * alphaRhoEpsilon in solid runs); sigma, devH, HJ2, a are derived. */
+int tj_rk_stage(tj_ctx* ctx, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_mf* Uout, int prec);
+
+/* Post-step chain of main.cpp:154-171 (+ the next getTimeStep), fused per cell:
+ * ops is a TJ_OP_* mask, applied in the reference's order. */
+int tj_post_step(tj_ctx* ctx, tj_mf* U, unsigned ops);
+
+/* One full time step of main.cpp:124-171 on S (S^n -> S^{n+1}) using T as the
+ * stage-1 buffer: tj_rk_stage(1), tj_rk_stage(2) with the post-step chain and
+ * the next dt fused into stage 2.  dt is the device-resident dt. */
+int tj_step(tj_ctx* ctx, tj_mf* S, tj_mf* T, int prec);
+
+/* The time loop of main.cpp:121-191 on the device words: dt, clamp, step,
+ * NaN stop.  Returns steps taken in *n_out and final time in *t_out. */
+int tj_run(tj_ctx* ctx, tj_mf* S, tj_mf* T, double final_time, int max_steps, int prec,
+           int check_every, double* t_out, int* n_out);
+
+/* primitiveToConservative over the valid cells (boxaccesscellarray.cpp:124-174),
+ * with the thermal-to-total pressure shift of initialise.cpp:104 when
+ * shift_pressure != 0 (setInitialConditions, initialise.cpp:111-123). */
+int tj_init_from_prims(tj_ctx* ctx, tj_mf* U, int shift_pressure);
+/* conservativeToPrimitive over the valid cells (boxaccesscellarray.cpp:67-122). */
+int tj_c2p(tj_ctx* ctx, tj_mf* U);
+/* Derived components sigma (and devH, HJ2 in solid runs) from the state. */
+int tj_derive(tj_ctx* ctx, tj_mf* U);
+
+/* Status word (TJ_ST_*), OR-ed since the last call with clear != 0. Synchronises. */
+int tj_status(tj_ctx* ctx, uint32_t* status, int clear);
+/* Zeroes the THINC history planes (start of a run, D4 of the oracle). */
+int tj_reset_history(tj_ctx* ctx);
+int tj_synchronize(tj_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENJIN_H */

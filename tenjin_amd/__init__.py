@@ -1,0 +1,258 @@
+"""tenjin_amd: the per-cell explicit update of a diffuse-interface
+multi-material solver on MI355X (gfx950).
+
+The compute path is the HIP library ``libtenjin.so`` (tenjin_amd/csrc, C ABI in
+include/tenjin.h).  This module binds it with ctypes; torch provides device
+memory and nothing else.  There is no CPU fallback: if the library is missing
+or no GPU is visible, the entry points raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import configs  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtenjin.so")
+
+TJ_PREC_FP64 = 0
+TJ_PREC_MIXED = 1
+TJ_ST_NAN, TJ_ST_WAVESPEED, TJ_ST_ALPHA_NANS, TJ_ST_ALPHA_SUM, TJ_ST_PLASTIC_DIV = 1, 2, 4, 8, 16
+
+
+class Material(ctypes.Structure):
+    _fields_ = [
+        ("eos", ctypes.c_int),
+        ("solid", ctypes.c_int),
+        ("plastic", ctypes.c_int),
+        ("pad_", ctypes.c_int),
+        ("gamma", ctypes.c_double),
+        ("pref", ctypes.c_double),
+        ("eref", ctypes.c_double),
+        ("CV", ctypes.c_double),
+        ("rho0", ctypes.c_double),
+        ("c0", ctypes.c_double),
+        ("s", ctypes.c_double),
+        ("G0", ctypes.c_double),
+        ("yield_stress", ctypes.c_double),
+    ]
+
+
+class LevelDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("n", ctypes.c_int * 3),
+        ("L", ctypes.c_double * 3),
+        ("max_grid_size", ctypes.c_int),
+        ("SOLID", ctypes.c_int),
+        ("THINC", ctypes.c_int),
+        ("PLASTIC", ctypes.c_int),
+        ("MUSCL", ctypes.c_int),
+        ("THINCbeta", ctypes.c_double),
+        ("CFL", ctypes.c_double),
+        ("nmat", ctypes.c_int),
+        ("bc_lo", ctypes.c_int * 3),
+        ("bc_hi", ctypes.c_int * 3),
+        ("mat", Material * 4),
+    ]
+
+
+class MFView(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("jstride", ctypes.c_longlong),
+        ("kstride", ctypes.c_longlong),
+        ("nstride", ctypes.c_longlong),
+        ("ncomp", ctypes.c_int),
+        ("pad_", ctypes.c_int),
+    ]
+
+
+# entry points declared in include/tenjin.h
+EXPORTS = [
+    "tj_create", "tj_destroy", "tj_last_error", "tj_stream", "tj_bind_level", "tj_ncomp",
+    "tj_comp_start", "tj_compute_dt", "tj_driver_dt", "tj_set_dt", "tj_get_clock", "tj_rk_stage",
+    "tj_post_step", "tj_step", "tj_run", "tj_init_from_prims", "tj_c2p", "tj_derive", "tj_status",
+    "tj_reset_history", "tj_synchronize",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libtenjin.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python tenjin_amd/build.py`")
+    L = ctypes.CDLL(path)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    mfp = ctypes.POINTER(MFView)
+    L.tj_create.argtypes = [ctypes.POINTER(vp), ip]
+    L.tj_destroy.argtypes = [vp]
+    L.tj_destroy.restype = None
+    L.tj_last_error.argtypes = [vp]
+    L.tj_last_error.restype = ctypes.c_char_p
+    L.tj_stream.argtypes = [vp]
+    L.tj_stream.restype = vp
+    L.tj_bind_level.argtypes = [vp, ctypes.POINTER(LevelDesc)]
+    L.tj_ncomp.argtypes = [vp]
+    L.tj_comp_start.argtypes = [vp, ip]
+    L.tj_compute_dt.argtypes = [vp, mfp, ctypes.POINTER(dp)]
+    L.tj_driver_dt.argtypes = [vp, dp]
+    L.tj_set_dt.argtypes = [vp, dp, ip]
+    L.tj_get_clock.argtypes = [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(ip)]
+    L.tj_rk_stage.argtypes = [vp, ip, mfp, mfp, mfp, ip]
+    L.tj_post_step.argtypes = [vp, mfp, ctypes.c_uint]
+    L.tj_step.argtypes = [vp, mfp, mfp, ip]
+    L.tj_run.argtypes = [vp, mfp, mfp, dp, ip, ip, ip, ctypes.POINTER(dp), ctypes.POINTER(ip)]
+    L.tj_init_from_prims.argtypes = [vp, mfp, ip]
+    L.tj_c2p.argtypes = [vp, mfp]
+    L.tj_derive.argtypes = [vp, mfp]
+    L.tj_status.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ip]
+    L.tj_reset_history.argtypes = [vp]
+    L.tj_synchronize.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def level_desc(pb) -> LevelDesc:
+    d = LevelDesc()
+    d.dim = pb.dim
+    for a in range(3):
+        d.n[a] = pb.n[a] if a < pb.dim else 1
+        d.L[a] = pb.L[a]
+        d.bc_lo[a] = pb.bc_lo[a]
+        d.bc_hi[a] = pb.bc_hi[a]
+    d.max_grid_size = pb.max_grid_size
+    d.SOLID, d.THINC, d.PLASTIC, d.MUSCL = pb.SOLID, pb.THINC, pb.PLASTIC, pb.MUSCL
+    d.THINCbeta = pb.THINCbeta
+    d.CFL = pb.CFL
+    d.nmat = pb.nmat
+    for m, M in enumerate(pb.materials):
+        t = d.mat[m]
+        t.eos, t.solid, t.plastic = M.eos, M.solid, M.plastic
+        t.gamma, t.pref, t.eref, t.CV = M.gamma, M.pref, M.eref, M.CV
+        t.rho0, t.c0, t.s, t.G0, t.yield_stress = M.rho0, M.c0, M.s, M.G0, M.yield_stress
+    return d
+
+
+class TenjinError(RuntimeError):
+    pass
+
+
+class GpuLevel:
+    """A bound level on one GPU with two state buffers (S: the state, T: the
+    stage-1 buffer), each a [ncomp][nz][ny][nx] float64 device array."""
+
+    def __init__(self, pb, device: int = 0, seglen: int | None = None):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise TenjinError("no GPU visible: the tenjin_amd update runs on the GPU only")
+        self.torch = torch
+        self.lib = load_library()
+        self.pb = pb
+        self.ctx = ctypes.c_void_p()
+        self._chk(self.lib.tj_create(ctypes.byref(self.ctx), device), "tj_create")
+        self._desc = level_desc(pb)
+        self._chk(self.lib.tj_bind_level(self.ctx, ctypes.byref(self._desc)), "tj_bind_level")
+        self.ncomp = self.lib.tj_ncomp(self.ctx)
+        self.shape = pb.shape
+        dev = torch.device("cuda", device)
+        self.S = torch.zeros((self.ncomp,) + self.shape, dtype=torch.float64, device=dev)
+        self.T = torch.zeros_like(self.S)
+        self.vS = self._view(self.S)
+        self.vT = self._view(self.T)
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.lib.tj_last_error(self.ctx).decode() if self.ctx else ""
+            raise TenjinError(f"{what} failed ({rc}): {msg}")
+
+    def _view(self, t) -> MFView:
+        nz, ny, nx = self.shape
+        v = MFView()
+        v.data = t.data_ptr()
+        v.jstride = nx
+        v.kstride = nx * ny
+        v.nstride = nx * ny * nz
+        v.ncomp = self.ncomp
+        return v
+
+    def close(self):
+        if self.ctx:
+            self.lib.tj_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state transfer -------------------------------------------------------
+    def set_state(self, arr: np.ndarray):
+        self.torch.cuda.synchronize()
+        self.S.copy_(self.torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)))
+        self.torch.cuda.synchronize()
+
+    def get_state(self) -> np.ndarray:
+        self.sync()
+        return self.S.cpu().numpy()
+
+    def sync(self):
+        self._chk(self.lib.tj_synchronize(self.ctx), "tj_synchronize")
+        self.torch.cuda.synchronize()
+
+    # entry points ----------------------------------------------------------
+    def init_from_prims(self, prims: np.ndarray, shift_pressure: int = 0):
+        self.set_state(prims)
+        self._chk(self.lib.tj_init_from_prims(self.ctx, ctypes.byref(self.vS), shift_pressure), "tj_init_from_prims")
+        self._chk(self.lib.tj_reset_history(self.ctx), "tj_reset_history")
+        self.sync()
+
+    def compute_dt(self) -> float:
+        dt = ctypes.c_double()
+        self._chk(self.lib.tj_compute_dt(self.ctx, ctypes.byref(self.vS), ctypes.byref(dt)), "tj_compute_dt")
+        return dt.value
+
+    def set_dt(self, dt: float, reset_clock: int = 0):
+        self._chk(self.lib.tj_set_dt(self.ctx, dt, reset_clock), "tj_set_dt")
+
+    def step(self, prec: int = TJ_PREC_FP64):
+        self._chk(self.lib.tj_step(self.ctx, ctypes.byref(self.vS), ctypes.byref(self.vT), prec), "tj_step")
+
+    def stage(self, stage: int, prec: int = TJ_PREC_FP64):
+        if stage == 1:
+            rc = self.lib.tj_rk_stage(self.ctx, 1, ctypes.byref(self.vS), ctypes.byref(self.vS), ctypes.byref(self.vT), prec)
+        else:
+            rc = self.lib.tj_rk_stage(self.ctx, 2, ctypes.byref(self.vT), ctypes.byref(self.vS), ctypes.byref(self.vS), prec)
+        self._chk(rc, "tj_rk_stage")
+
+    def post_step(self, ops: int):
+        self._chk(self.lib.tj_post_step(self.ctx, ctypes.byref(self.vS), ops), "tj_post_step")
+
+    def run(self, final_time: float, max_steps: int = 1 << 30, prec: int = TJ_PREC_FP64, check_every: int = 16):
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        self._chk(self.lib.tj_run(self.ctx, ctypes.byref(self.vS), ctypes.byref(self.vT), final_time, max_steps, prec,
+                                  check_every, ctypes.byref(t), ctypes.byref(n)), "tj_run")
+        return n.value, t.value
+
+    def clock(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._chk(self.lib.tj_get_clock(self.ctx, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "tj_get_clock")
+        return t.value, dt.value, n.value
+
+    def status(self, clear: int = 0) -> int:
+        s = ctypes.c_uint32()
+        self._chk(self.lib.tj_status(self.ctx, ctypes.byref(s), clear), "tj_status")
+        return s.value
+
+    def derive(self):
+        self._chk(self.lib.tj_derive(self.ctx, ctypes.byref(self.vS)), "tj_derive")

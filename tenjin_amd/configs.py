@@ -1,0 +1,303 @@
+"""Workload definitions and initial conditions for the per-cell update.
+
+A ``Problem`` carries what the reference reads from its settings file
+(initialise.cpp:310-375: domain, CFL, flags, materials, boundaries) plus the
+initial primitive field.  Initial conditions follow the reference's
+``initial_conditions`` (initialise.cpp:20-109): per cell, volume fractions,
+phase densities, velocity, pressure (and V for solids) are set, optionally the
+thermal pressure is shifted to total pressure (initialise.cpp:104), then
+``primitiveToConservative`` fills the conserved fields.
+
+Stiffened gas maps onto the reference's MieGruneisenEOS as
+``adiabaticIndex = gamma, pref = -gamma * p_inf, eref = 0``
+(equationofstate.cpp:37-64), so a physical pressure is set directly.
+
+Random draws use a 64-bit Mersenne twister with uniforms ``(x >> 11) * 2**-53``
+(numpy's MT19937 bit generator), which C++ ``std::mt19937_64`` reproduces.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+# Variable enum of structdefinitions.h:10-32
+ALPHA, ALPHARHO, RHO_K, RHO, RHOU, TOTAL_E, VELOCITY, P, SOUNDSPEED, USTAR = range(10)
+RHO_MIX, ALPHARHOLAMBDA, LAMBDA, SIGMA, V_TENSOR, VSTAR, DEVH, HJ2, EPSILON, ALPHARHOEPSILON = range(10, 20)
+
+ABSENT_ALPHA = 1e-6
+
+
+class Layout:
+    """Component map of accesspattern.cpp:44-79 (no mixtures)."""
+
+    def __init__(self, nmat: int, solid: bool, plastic: bool):
+        self.start = {}
+        self.names: List[str] = []
+        self.cons: List[tuple] = []
+        self.prim: List[tuple] = []
+        n = 0
+
+        def add(base, kind, var, nm, nr, nc):
+            nonlocal n
+            self.start[var] = n
+            n += nm * nr * nc
+            for m in range(nm):
+                for r in range(nr):
+                    for c in range(nc):
+                        self.names.append(f"{base}{m}_{r}{c}")
+                        if kind in ("cons", "both"):
+                            self.cons.append((var, m, r, c))
+                        if kind in ("prim", "both"):
+                            self.prim.append((var, m, r, c))
+
+        add("alpha", "both", ALPHA, nmat, 1, 1)
+        add("alphaRho", "cons", ALPHARHO, nmat, 1, 1)
+        add("rho_k", "prim", RHO_K, nmat, 1, 1)
+        add("rho", "none", RHO, 1, 1, 1)
+        add("rhoU", "cons", RHOU, 1, 3, 1)
+        add("u", "prim", VELOCITY, 1, 3, 1)
+        add("E", "cons", TOTAL_E, 1, 1, 1)
+        add("p", "prim", P, 1, 1, 1)
+        add("a", "none", SOUNDSPEED, 1, 1, 1)
+        add("uStar", "none", USTAR, 1, 1, 1)
+        add("sigma", "none", SIGMA, 1, 3, 3)
+        if solid:
+            add("V", "both", V_TENSOR, 1, 3, 3)
+            add("VStar", "none", VSTAR, 1, 3, 3)
+            add("devH", "none", DEVH, 1, 3, 3)
+            add("HenckyJ2", "none", HJ2, 1, 1, 1)
+            if plastic:
+                add("epsilon", "prim", EPSILON, nmat, 1, 1)
+                add("alphaRhoEpsilon", "cons", ALPHARHOEPSILON, nmat, 1, 1)
+        self.ncomp = n
+
+    def comp(self, var, m=0, r=0, c=0) -> int:
+        if var in (ALPHA, ALPHARHO, RHO_K, EPSILON, ALPHARHOEPSILON):
+            return self.start[var] + m
+        if var in (RHOU, VELOCITY):
+            return self.start[var] + r
+        if var in (SIGMA, V_TENSOR, VSTAR, DEVH):
+            return self.start[var] + 3 * r + c
+        return self.start[var]
+
+    def cons_comps(self) -> List[int]:
+        return [self.comp(*s) for s in self.cons]
+
+    def prim_comps(self) -> List[int]:
+        return [self.comp(*s) for s in self.prim]
+
+    def state_comps(self) -> List[int]:
+        """Persistent per-cell state: conserved and primitive fields plus rho."""
+        out = sorted(set(self.cons_comps()) | set(self.prim_comps()) | {self.start[RHO]})
+        return out
+
+
+@dataclasses.dataclass
+class Material:
+    """One material: kind 0 = stiffened gas / MieGruneisenEOS, kind 1 = Us-up Mie-Gruneisen solid."""
+
+    gamma: float = 1.4
+    pref: float = 0.0
+    eref: float = 0.0
+    CV: float = 1.0
+    eos: int = 0
+    solid: int = 0
+    plastic: int = 0
+    rho0: float = 1.0
+    c0: float = 0.0
+    s: float = 0.0
+    G0: float = 0.0
+    yield_stress: float = 0.0
+
+    @staticmethod
+    def stiffened(gamma: float, p_inf: float = 0.0) -> "Material":
+        return Material(gamma=gamma, pref=-gamma * p_inf)
+
+    @staticmethod
+    def usup_solid(rho0, Gamma0, c0, s, G, Y, plastic=True) -> "Material":
+        return Material(gamma=Gamma0 + 1.0, eos=1, solid=1, plastic=int(plastic), rho0=rho0,
+                        c0=c0, s=s, G0=G, yield_stress=Y)
+
+
+@dataclasses.dataclass
+class Problem:
+    name: str
+    dim: int
+    n: tuple
+    L: tuple
+    max_grid_size: int
+    materials: List[Material]
+    SOLID: int = 0
+    THINC: int = 0
+    PLASTIC: int = 0
+    MUSCL: int = 1
+    THINCbeta: float = 1.6
+    CFL: float = 0.5
+    bc_lo: tuple = (0, 0, 0)
+    bc_hi: tuple = (0, 0, 0)
+    nghost: int = 2
+    final_time: Optional[float] = None
+    steps: Optional[int] = None
+    shift_pressure: int = 0
+    prims: Optional[np.ndarray] = None  # [ncomp][nz][ny][nx]
+
+    @property
+    def nmat(self) -> int:
+        return len(self.materials)
+
+    @property
+    def layout(self) -> Layout:
+        return Layout(self.nmat, bool(self.SOLID), bool(self.PLASTIC))
+
+    @property
+    def shape(self):
+        nx = self.n[0]
+        ny = self.n[1] if self.dim > 1 else 1
+        nz = self.n[2] if self.dim > 2 else 1
+        return nz, ny, nx
+
+    @property
+    def ncells(self) -> int:
+        nz, ny, nx = self.shape
+        return nx * ny * nz
+
+    def dx(self):
+        return [self.L[a] / self.n[a] if a < self.dim else 1.0 for a in range(3)]
+
+
+def _empty_prims(pb: Problem) -> np.ndarray:
+    return np.zeros((pb.layout.ncomp,) + pb.shape, dtype=np.float64)
+
+
+def _set_two_phase(pb: Problem, U: np.ndarray, mask: np.ndarray, rho0: float, rho1: float,
+                   u: np.ndarray, p: np.ndarray):
+    """mask True -> material 0 present (alpha0 = 1 - 1e-6), else material 1."""
+    L = pb.layout
+    a0 = np.where(mask, 1.0 - ABSENT_ALPHA, ABSENT_ALPHA)
+    U[L.comp(ALPHA, 0)] = a0
+    U[L.comp(ALPHA, 1)] = 1.0 - a0
+    U[L.comp(RHO_K, 0)] = rho0
+    U[L.comp(RHO_K, 1)] = rho1
+    for r in range(3):
+        U[L.comp(VELOCITY, 0, r)] = u[r]
+    U[L.comp(P)] = p
+
+
+def shock_tube_1d(n: int = 1000, interface: float = 0.7, max_grid_size: int = 1 << 30) -> Problem:
+    """Two-material stiffened-gas shock tube (water-like / air), x in [0, 1].
+
+    State rule of initialise.cpp:393-402: cell i takes the left state when
+    i < (int)((interface / L) * n).
+    """
+    mats = [Material.stiffened(4.4, 6e8), Material.stiffened(1.4, 0.0)]
+    pb = Problem("shock_tube_1d", 1, (n, 1, 1), (1.0, 1.0, 1.0), max_grid_size, mats,
+                 final_time=2.4e-4)
+    U = _empty_prims(pb)
+    i = np.arange(n)
+    left = (i < int((interface / 1.0) * n))[None, None, :]
+    zero = np.zeros((1, 1, n))
+    p = np.where(left, 1e9, 1e5)
+    _set_two_phase(pb, U, left, 1000.0, 50.0, [zero, zero, zero], p)
+    pb.prims = U
+    return pb
+
+
+def rankine_hugoniot(gamma: float, rho1: float, p1: float, mach: float):
+    """Post-shock state of a shock of Mach number `mach` moving into gas at rest."""
+    c1 = math.sqrt(gamma * p1 / rho1)
+    m2 = mach * mach
+    p2 = p1 * (1.0 + 2.0 * gamma / (gamma + 1.0) * (m2 - 1.0))
+    rho2 = rho1 * (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0)
+    w = mach * c1
+    u2 = w * (1.0 - rho1 / rho2)
+    return rho2, u2, p2
+
+
+def _mt_uniform(seed: int, count: int) -> np.ndarray:
+    bits = np.random.Generator(np.random.MT19937(seed)).bit_generator.random_raw(count)
+    return (np.asarray(bits, dtype=np.uint64) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+def shock_inclusion_2d(nx: int = 2048, ny: int = 1024, box: int = 64, seed: int = 1,
+                       steps: int = 500, thinc: int = 1, shock_x: float = 0.5,
+                       drift=(0.0, 0.0)) -> Problem:
+    """Mach 1.22 shock in gas (gamma 1.4) hitting a light circular inclusion
+    (gamma 1.67, rho 0.138) whose radius carries 8 seeded Fourier modes.
+
+    Domain [0, 2] x [0, 1] (dx = dy = 1/ny for the full-size config); x
+    transmissive, y reflective.  Cell centres decide the material.  `drift`
+    adds a uniform velocity to every cell (test variants that move the
+    interface from the first step).
+    """
+    mats = [Material.stiffened(1.4, 0.0), Material.stiffened(1.67, 0.0)]
+    Lx, Ly = 2.0, 1.0
+    pb = Problem("shock_inclusion_2d", 2, (nx, ny, 1), (Lx, Ly, 1.0), box, mats,
+                 THINC=thinc, THINCbeta=1.6, bc_lo=(0, 1, 0), bc_hi=(0, 1, 0), steps=steps)
+    U = _empty_prims(pb)
+    x = (np.arange(nx) + 0.5) * (Lx / nx)
+    y = (np.arange(ny) + 0.5) * (Ly / ny)
+    X, Y = np.meshgrid(x, y)  # [ny][nx]
+    uu = _mt_uniform(seed, 16)
+    amp = -0.01 + 0.02 * uu[:8]
+    phase = 2.0 * math.pi * uu[8:]
+    xc, yc, r0 = 0.8, 0.5, 0.2 * Ly
+    th = np.arctan2(Y - yc, X - xc)
+    rr = r0 * (1.0 + sum(amp[k] * np.cos((k + 1) * th + phase[k]) for k in range(8)))
+    inside = np.hypot(X - xc, Y - yc) < rr
+    rho2, u2, p2 = rankine_hugoniot(1.4, 1.0, 1e5, 1.22)
+    shocked = X < shock_x
+    gas_rho = np.where(shocked, rho2, 1.0)
+    ux = np.where(shocked, u2, 0.0) + drift[0]
+    uy = np.zeros_like(X) + drift[1]
+    p = np.where(shocked, p2, 1e5)
+    mask = ~inside  # material 0 (gas) present outside the inclusion
+    zero = np.zeros_like(X)
+    _set_two_phase(pb, U, mask[None], gas_rho[None], 0.138, [ux[None], uy[None], zero[None]], p[None])
+    pb.prims = U
+    return pb
+
+
+COPPER = dict(rho0=8930.0, Gamma0=2.0, c0=3940.0, s=1.49, G=45e9, Y=90e6)
+ALUMINIUM = dict(rho0=2703.0, Gamma0=2.0, c0=5328.0, s=1.338, G=26e9, Y=300e6)
+
+
+def plate_impact(dim: int = 3, n: int = 256, box: int = 64, seed: int = 3, speed: float = 400.0,
+                 steps: int = 300, length: float = 25.6e-3, tilt_deg: float = 5.0,
+                 plastic: bool = True, cu=None, al=None) -> Problem:
+    """Copper flyer on aluminium across a plane interface through the domain
+    centre tilted about z by `tilt_deg` from the x normal; Us-up
+    Mie-Gruneisen solids with perfect plasticity; V = I, thermal p = 0,
+    +-0.1 % seeded density noise.  All boundaries transmissive."""
+    cu = dict(COPPER, **(cu or {}))
+    al = dict(ALUMINIUM, **(al or {}))
+    mats = [Material.usup_solid(cu["rho0"], cu["Gamma0"], cu["c0"], cu["s"], cu["G"], cu["Y"], plastic),
+            Material.usup_solid(al["rho0"], al["Gamma0"], al["c0"], al["s"], al["G"], al["Y"], plastic)]
+    nn = (n, n, n if dim > 2 else 1)
+    pb = Problem(f"plate_impact_{dim}d", dim, nn, (length, length, length), box, mats,
+                 SOLID=1, PLASTIC=int(plastic), steps=steps, shift_pressure=1)
+    L = pb.layout
+    U = _empty_prims(pb)
+    nz, ny, nx = pb.shape
+    h = length / n
+    x = (np.arange(nx) + 0.5) * h
+    y = (np.arange(ny) + 0.5) * h
+    z = (np.arange(nz) + 0.5) * h
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    t = math.radians(tilt_deg)
+    cu_side = (X - 0.5 * length) * math.cos(t) + (Y - 0.5 * length) * math.sin(t) < 0.0
+    noise = _mt_uniform(seed, pb.ncells).reshape(pb.shape)
+    fac = 1.0 + 1e-3 * (2.0 * noise - 1.0)
+    a0 = np.where(cu_side, 1.0 - ABSENT_ALPHA, ABSENT_ALPHA)
+    U[L.comp(ALPHA, 0)] = a0
+    U[L.comp(ALPHA, 1)] = 1.0 - a0
+    U[L.comp(RHO_K, 0)] = cu["rho0"] * fac
+    U[L.comp(RHO_K, 1)] = al["rho0"] * fac
+    U[L.comp(VELOCITY, 0, 0)] = np.where(cu_side, speed, 0.0)
+    U[L.comp(P)] = 0.0
+    for r in range(3):
+        U[L.comp(V_TENSOR, 0, r, r)] = 1.0
+    pb.prims = U
+    return pb

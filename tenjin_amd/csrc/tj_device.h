@@ -1,0 +1,180 @@
+// Device-side data structures and per-cell physics of the gfx950 update.
+//
+// Every expression keeps the operation order of the reference file cited
+// beside it; the library is compiled with -ffp-contract=off, so the FP64 form
+// rounds exactly like the reference's CPU code (IEEE division and square root
+// are correctly rounded on both sides).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tj {
+
+constexpr int MAXM = 4;
+
+// Per-material constants.  Gg, gp1 and xi are the values the reference
+// recomputes on every call (GruneisenGamma, GruneisenGamma + 1.0,
+// 1.0 / GruneisenGamma in equationofstate.cpp:47-75); they are evaluated once
+// on the host with the same operations, so they are bit-identical.
+struct MatC {
+    int eos, solid, plastic, pad;
+    double Gg, gp1, xi;
+    double pref, eref, CV;
+    double rho0, c0, s, G0, yield;
+};
+
+// Variable numbering of structdefinitions.h:10-32
+enum Var {
+    ALPHA, ALPHARHO, RHO_K, RHO, RHOU, TOTAL_E, VELOCITY, PRESSURE, SOUNDSPEED, USTAR,
+    RHO_MIX, ALPHARHOLAMBDA, LAMBDA, SIGMA, V_TENSOR, VSTAR, DEVH, HJ2, EPSILON,
+    ALPHARHOEPSILON, NVAR
+};
+
+struct Level {
+    int dim;
+    int n[3];          // cells per dimension (1 for unused dimensions)
+    double dx[3];
+    double yn[2];      // -1/(8 dx), -1/(8 dy): Youngs-normal prefactors (thinc.cpp:67-68)
+    int nmat;
+    int solid, plastic, thinc;
+    double beta0, cfl;
+    int bc_lo[3], bc_hi[3];
+    const uint8_t* edge[3];  // per coordinate: bit0 = first cell of a box, bit1 = last cell of a box
+    MatC m[MAXM];
+    int cs[NVAR];      // first component of each Variable (AccessPattern), -1 if absent
+    long long ncells;
+};
+
+// Caller-owned CellArray view (component-major, i fastest).
+struct MF {
+    double* p;
+    long long js, ks, ns;
+    __device__ __forceinline__ double& at(int c, int i, int j, int k) const {
+        return p[(long long)c * ns + (long long)k * ks + (long long)j * js + i];
+    }
+};
+
+// Face-centred flux planes of one direction: lower face of cell (i,j,k),
+// extent n_d + 1 along d; component-major.
+struct Faces {
+    double* p;
+    int nx, ny, nz;     // face-grid extents
+    long long cs;       // component stride
+    __device__ __forceinline__ double& at(int c, int i, int j, int k) const {
+        return p[(long long)c * cs + ((long long)k * ny + j) * nx + i];
+    }
+};
+
+// THINC/BVD history planes (faithful form of the reference's stale
+// ULStar/URStar reads, thinc.cpp:57-63 with hllc.cpp:404-406):
+//   urstar  : last THINC right-face alpha written for each cell and material
+//   star_in : star-state alpha of each cell's lower face from the previous
+//             directional flux pass (what ULStar holds in valid cells)
+//   edge_in : star-state alpha of box-high faces from the previous pass in the
+//             same direction (what ULStar holds in a box's high ghost layer)
+struct Hist {
+    double* urstar;
+    const double* star_in;
+    double* star_out;
+    const double* edge_in;
+    double* edge_out;
+    long long pstride;  // plane stride of urstar/star planes (= ncells)
+    long long estride;  // plane stride of the edge planes (= face count of direction d)
+};
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ double sgnd(double v) { return (double)((0.0 < v) - (v < 0.0)); }
+
+// ------------------------------------------------------------------ math
+// Project-written exp/expm1/tanh/cosh/x^(-1/3): Cody-Waite reduction with the
+// fdlibm split of ln2 and Taylor/Horner polynomials; the oracle evaluates the
+// same algorithm with the same constants in the same order.
+__device__ inline double pm_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.782712893384) return __builtin_huge_val();
+    if (x < -745.1332191019412) return 0.0;
+    const double log2e = 1.4426950408889634;
+    const double ln2hi = 6.93147180369123816490e-01;
+    const double ln2lo = 1.90821492927058770002e-10;
+    double kd = __builtin_rint(x * log2e);
+    double r = (x - kd * ln2hi) - kd * ln2lo;
+    double p = 1.0 / 6227020800.0;
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return __builtin_ldexp(p, (int)kd);
+}
+__device__ inline double pm_expm1(double y) {
+    if (__builtin_fabs(y) < 0.3) {
+        double p = 1.0 / 87178291200.0;
+        p = p * y + 1.0 / 6227020800.0;
+        p = p * y + 1.0 / 479001600.0;
+        p = p * y + 1.0 / 39916800.0;
+        p = p * y + 1.0 / 3628800.0;
+        p = p * y + 1.0 / 362880.0;
+        p = p * y + 1.0 / 40320.0;
+        p = p * y + 1.0 / 5040.0;
+        p = p * y + 1.0 / 720.0;
+        p = p * y + 1.0 / 120.0;
+        p = p * y + 1.0 / 24.0;
+        p = p * y + 1.0 / 6.0;
+        p = p * y + 0.5;
+        p = p * y + 1.0;
+        return p * y;
+    }
+    return pm_exp(y) - 1.0;
+}
+__device__ inline double pm_tanh(double x) {
+    double ax = __builtin_fabs(x);
+    if (ax > 20.0) return x > 0.0 ? 1.0 : -1.0;
+    double t = pm_expm1(2.0 * ax);
+    double r = t / (t + 2.0);
+    return x < 0.0 ? -r : r;
+}
+__device__ inline double pm_cosh(double x) {
+    double e = pm_exp(__builtin_fabs(x));
+    return (e + 1.0 / e) * 0.5;
+}
+__device__ inline double pm_rcbrt(double x) {
+    uint64_t bits = __builtin_bit_cast(uint64_t, x);
+    uint64_t g = 0x553ef0ff289dd796ULL - bits / 3;
+    double y = __builtin_bit_cast(double, g);
+    for (int it = 0; it < 5; ++it) {
+        double y3 = (y * y) * y;
+        y = y * ((4.0 - x * y3) / 3.0);
+    }
+    double c = 1.0 / y;
+    c = c - (c - x / (c * c)) / 3.0;
+    return 1.0 / c;
+}
+
+// ------------------------------------------------------------------ boundaries
+// boundary.cpp:3-42: transmissive -> foextrap (nearest valid cell),
+// reflective -> reflect_even, with reflect_odd (sign flip) for the normal
+// velocity and momentum.  Each coordinate is mapped independently.
+__device__ __forceinline__ int bcmap(int c, int n, int rlo, int rhi, bool& flip) {
+    flip = false;
+    int s = c;
+    if (c < 0) {
+        if (rlo) { flip = true; s = -1 - c; }
+        else s = 0;
+    } else if (c > n - 1) {
+        if (rhi) { flip = true; s = 2 * n - 1 - c; }
+        else s = n - 1;
+    }
+    // lanes far outside the domain (unused results) still load in bounds
+    return s < 0 ? 0 : (s > n - 1 ? n - 1 : s);
+}
+
+}  // namespace tj

@@ -1,0 +1,592 @@
+// Kernels of the fluid path: directional flux passes, the fused update
+// (update + c2p [+ RK average + cleanUpAlpha + NaN scan + CFL min]) and the
+// time-step words.
+#pragma once
+#include "tj_fluid.h"
+
+namespace tj {
+
+// Device-resident clock of the time loop (main.cpp:121-143).
+struct Clock {
+    double t, dt, final_time;
+    int n, done;
+    unsigned long long dtmin_bits;  // min over cells of dx/(a+|u|), as ordered bits of a positive double
+    unsigned int status;
+};
+
+// ------------------------------------------------------------------ x pass
+// One wave covers a row segment; lane l holds cell c = f0 - H + l.  Lanes
+// reconstruct their cell (at the boundary-mapped cell s), neighbours' data
+// travel by wave shuffles, and lane l computes the flux of face c (the lower
+// face of cell c).  Faces per wave: 63 (MUSCL) or 61 (MUSCL + THINC/BVD).
+template <int NM, bool TH>
+__global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, Faces F, Hist h, Clock* clk) {
+    if (clk->done) return;
+    constexpr int H = TH ? 2 : 1;
+    constexpr int PER = TH ? 61 : 63;
+    const int lane = threadIdx.x & 63;
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const int nseg = (nx + 1 + PER - 1) / PER;
+    if (wave >= (long long)nseg * ny * nz) return;  // uniform per wave
+    const int seg = (int)(wave % nseg);
+    const long long row = wave / nseg;
+    const int j = (int)(row % ny), k = (int)(row / ny);
+    const int c = seg * PER - H + lane;
+    bool fl;
+    const int s = bcmap(c, nx, lv.bc_lo[0], lv.bc_hi[0], fl);
+    const int lofs = s - c;  // lane offset of the reconstructed cell
+
+    Prim<NM> Pm, Pc, Pp, PL, PR;
+    load_prim<NM>(lv, U, s - 1, j, k, Pm);
+    load_prim<NM>(lv, U, s, j, k, Pc);
+    load_prim<NM>(lv, U, s + 1, j, k, Pp);
+    muscl<NM>(Pm, Pc, Pp, PL, PR);
+
+    unsigned st = 0;
+    const long long cell_s = ((long long)k * ny + j) * nx + s;
+    if (TH) {
+        ARec<NM> A;
+        arec<NM>(lv, U, s, j, k, 0, lv.yn[0], lv.yn[1], A);
+        const uint8_t eflag = lv.edge[0][s];
+        const bool blo = eflag & 1, bhi = eflag & 2;
+        int srcm = lane + lofs - 1, srcp = lane + lofs + 1;
+        srcm = srcm < 0 ? 0 : (srcm > 63 ? 63 : srcm);
+        srcp = srcp < 0 ? 0 : (srcp > 63 ? 63 : srcp);
+        const int mixm = __shfl(A.mixed, srcm, 64);
+        const int mixp = __shfl(A.mixed, srcp, 64);
+        const int mixedFlag = (A.mixed >> (NM - 1)) & 1;
+        double URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            const double uraM = __shfl(A.ura[m], srcm, 64);
+            const double thrM = __shfl(A.thr[m], srcm, 64);
+            const double ulaP = __shfl(A.ula[m], srcp, 64);
+            const double thlP = __shfl(A.thl[m], srcp, 64);
+            if (mixedFlag) {
+                if (blo) {
+                    URl[m] = load_alpha(lv, U, m, s - 1, j, k);
+                    TRl[m] = 0.0;
+                } else {
+                    URl[m] = uraM;
+                    TRl[m] = ((mixm >> m) & 1) ? thrM : h.urstar[m * h.pstride + cell_s - 1];
+                }
+                if (bhi) {
+                    ULr[m] = load_alpha(lv, U, m, s + 1, j, k);
+                    TLr[m] = h.edge_in[m * h.estride + ((long long)k * ny + j) * (nx + 1) + s + 1];
+                } else {
+                    ULr[m] = ulaP;
+                    TLr[m] = ((mixp >> m) & 1) ? thlP : h.star_in[m * h.pstride + cell_s + 1];
+                }
+                if ((A.mixed >> m) & 1) {
+                    thl[m] = A.thl[m];
+                    thr[m] = A.thr[m];
+                } else {
+                    thl[m] = h.star_in[m * h.pstride + cell_s];
+                    thr[m] = h.urstar[m * h.pstride + cell_s];
+                }
+            }
+        }
+        if (mixedFlag) bvd<NM>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
+        st |= cleanup_alpha<NM>(PL.a);
+        st |= cleanup_alpha<NM>(PR.a);
+        // URStar history: the owner lane of a valid cell records its THINC values
+        if (lane >= H && lane < H + PER && c >= 0 && c < nx) {
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                if ((A.mixed >> m) & 1) h.urstar[m * h.pstride + cell_s] = A.thr[m];
+        }
+    }
+
+    FSt<NM> SLs, SRs;
+    p2c_sound<NM>(lv, PL, SLs);
+    p2c_sound<NM>(lv, PR, SRs);
+
+    // left state of face c: UR(c-1) from the neighbouring lane (or the
+    // boundary image of this lane's UR at the domain's low face)
+    FSt<NM> Lf;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        Lf.al[m] = __shfl(SRs.al[m], lane > 0 ? lane - 1 : 0, 64);
+        Lf.ar[m] = __shfl(SRs.ar[m], lane > 0 ? lane - 1 : 0, 64);
+    }
+    Lf.rho = __shfl(SRs.rho, lane > 0 ? lane - 1 : 0, 64);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Lf.u[q] = __shfl(SRs.u[q], lane > 0 ? lane - 1 : 0, 64);
+    Lf.E = __shfl(SRs.E, lane > 0 ? lane - 1 : 0, 64);
+    Lf.p = __shfl(SRs.p, lane > 0 ? lane - 1 : 0, 64);
+    Lf.a = __shfl(SRs.a, lane > 0 ? lane - 1 : 0, 64);
+
+    if (lane >= H && lane < H + PER && c >= 0 && c <= nx) {
+        if (c == 0) {
+            Lf = SRs;
+            if (lv.bc_lo[0]) flip_state<NM>(Lf, 0);
+        }
+        FSt<NM> Rf = SLs;
+        if (c == nx && lv.bc_hi[0]) flip_state<NM>(Rf, 0);
+        double Fv[2 * NM + 5], sa[NM];
+        hllc<NM>(Lf, Rf, 0, Fv, sa);
+#pragma unroll
+        for (int q = 0; q < 2 * NM + 5; ++q) F.at(q, c, j, k) = Fv[q];
+        if (TH) {
+            if (c < nx) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m) h.star_out[m * h.pstride + ((long long)k * ny + j) * nx + c] = sa[m];
+            }
+            if (c == nx || (c > 0 && (lv.edge[0][c - 1] & 2))) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m) h.edge_out[m * h.estride + ((long long)k * ny + j) * (nx + 1) + c] = sa[m];
+            }
+        }
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
+// ------------------------------------------------------------------ y / z pass
+// One wave covers 64 columns (lanes along x, coalesced) and marches along
+// direction D over a segment of faces [f0, f1); the right state of row r-1
+// is carried in registers to the face of row r.  With THINC the alpha
+// reconstruction runs one row ahead so the BVD of row r sees rows r-1 and r+1.
+template <int NM, bool TH, int D>
+__global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, Faces F, Hist h, int seglen, Clock* clk) {
+    if (clk->done) return;
+    const int lane = threadIdx.x & 63;
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const int nd = lv.n[D];
+    const int no = (D == 1) ? nz : ny;
+    const int nstrip = (nx + 63) / 64;
+    const int nseg = (nd + 1 + seglen - 1) / seglen;
+    if (wave >= (long long)nstrip * nseg * no) return;
+    const int strip = (int)(wave % nstrip);
+    const long long rest = wave / nstrip;
+    const int seg = (int)(rest % nseg);
+    const int o = (int)(rest / nseg);
+    const int i0 = strip * 64 + lane;
+    const bool act = i0 < nx;
+    const int i = act ? i0 : nx - 1;
+    const int f0 = seg * seglen;
+    const int f1 = min(f0 + seglen, nd + 1);
+    const int rb = max(f0 - 1, 0), re = min(f1 - 1, nd - 1);
+    auto cj = [&](int r) { return D == 1 ? r : o; };
+    auto ck = [&](int r) { return D == 2 ? r : o; };
+    auto cellidx = [&](int r) { return ((long long)ck(r) * ny + cj(r)) * nx + i; };
+    auto faceidx = [&](int r) {  // lower face r along D
+        return D == 1 ? ((long long)ck(r) * (ny + 1) + r) * nx + i : ((long long)r * ny + cj(r)) * nx + i;
+    };
+
+    Prim<NM> Pm, Pc, Pp;
+    load_prim<NM>(lv, U, i, cj(rb - 1), ck(rb - 1), Pm);
+    load_prim<NM>(lv, U, i, cj(rb), ck(rb), Pc);
+
+    ARec<NM> Aprev, Acur, Anext;
+    bool curValid = false;
+    unsigned st = 0;
+    FSt<NM> prevR;
+
+    for (int r = rb; r <= re; ++r) {
+        load_prim<NM>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
+        Prim<NM> PL, PR;
+        muscl<NM>(Pm, Pc, Pp, PL, PR);
+        if (TH) {
+            const uint8_t ef = lv.edge[D][r];
+            const bool blo = ef & 1, bhi = ef & 2;
+            if (!curValid) {
+                arec<NM>(lv, U, i, cj(r), ck(r), D, lv.yn[0], lv.yn[1], Acur);
+                if (!blo) arec<NM>(lv, U, i, cj(r - 1), ck(r - 1), D, lv.yn[0], lv.yn[1], Aprev);
+            }
+            if (!bhi) arec<NM>(lv, U, i, cj(r + 1), ck(r + 1), D, lv.yn[0], lv.yn[1], Anext);
+            const int mixedFlag = (Acur.mixed >> (NM - 1)) & 1;
+            const long long cr = cellidx(r);
+            const long long cstride = (D == 1) ? nx : (long long)nx * ny;
+            if (mixedFlag) {
+                double URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
+#pragma unroll
+                for (int m = 0; m < NM; ++m) {
+                    if (blo) {
+                        URl[m] = load_alpha(lv, U, m, i, cj(r - 1), ck(r - 1));
+                        TRl[m] = 0.0;
+                    } else {
+                        URl[m] = Aprev.ura[m];
+                        TRl[m] = ((Aprev.mixed >> m) & 1) ? Aprev.thr[m] : h.urstar[m * h.pstride + cr - cstride];
+                    }
+                    if (bhi) {
+                        ULr[m] = load_alpha(lv, U, m, i, cj(r + 1), ck(r + 1));
+                        TLr[m] = h.edge_in[m * h.estride + faceidx(r + 1)];
+                    } else {
+                        ULr[m] = Anext.ula[m];
+                        TLr[m] = ((Anext.mixed >> m) & 1) ? Anext.thl[m] : h.star_in[m * h.pstride + cr + cstride];
+                    }
+                    if ((Acur.mixed >> m) & 1) {
+                        thl[m] = Acur.thl[m];
+                        thr[m] = Acur.thr[m];
+                    } else {
+                        thl[m] = h.star_in[m * h.pstride + cr];
+                        thr[m] = h.urstar[m * h.pstride + cr];
+                    }
+                }
+                bvd<NM>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
+            }
+            st |= cleanup_alpha<NM>(PL.a);
+            st |= cleanup_alpha<NM>(PR.a);
+            if (act && r >= f0) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+                    if ((Acur.mixed >> m) & 1) h.urstar[m * h.pstride + cr] = Acur.thr[m];
+            }
+            Aprev = Acur;
+            Acur = Anext;
+            curValid = !bhi;
+        }
+        FSt<NM> SLs, SRs;
+        p2c_sound<NM>(lv, PL, SLs);
+        p2c_sound<NM>(lv, PR, SRs);
+        auto emit = [&](int face, const FSt<NM>& A, const FSt<NM>& B) {
+            double Fv[2 * NM + 5], sa[NM];
+            hllc<NM>(A, B, D, Fv, sa);
+            if (!act) return;
+            const long long fi = faceidx(face);
+            const long long fcs = F.cs;
+#pragma unroll
+            for (int q = 0; q < 2 * NM + 5; ++q) F.p[q * fcs + fi] = Fv[q];
+            if (TH) {
+                if (face < nd) {
+#pragma unroll
+                    for (int m = 0; m < NM; ++m) h.star_out[m * h.pstride + cellidx(face)] = sa[m];
+                }
+                if (face == nd || (face > 0 && (lv.edge[D][face - 1] & 2))) {
+#pragma unroll
+                    for (int m = 0; m < NM; ++m) h.edge_out[m * h.estride + fi] = sa[m];
+                }
+            }
+        };
+        if (r == 0 && f0 == 0) {
+            FSt<NM> img = SRs;
+            if (lv.bc_lo[D]) flip_state<NM>(img, D);
+            emit(0, img, SLs);
+        } else if (r >= f0) {
+            emit(r, prevR, SLs);
+        }
+        if (r == nd - 1 && f1 == nd + 1) {
+            FSt<NM> img = SLs;
+            if (lv.bc_hi[D]) flip_state<NM>(img, D);
+            emit(nd, SRs, img);
+        }
+        prevR = SRs;
+        Pm = Pc;
+        Pc = Pp;
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
+// ------------------------------------------------------------------ update
+// update (hllc.cpp:493-523) for every direction in x, y, z order, then
+// conservativeToPrimitive (hllc.cpp:689).  Stage 2 adds the RK average of
+// advance (hllc.cpp:705), cleanUpAlpha (main.cpp:169-171), the NaN scan
+// (main.cpp:183-187) and getTimeStep of the next step (main.cpp:128).
+template <int NM, int STAGE>
+__global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, Faces F0, Faces F1, Faces F2,
+                                                Clock* clk, unsigned ops) {
+    if (clk->done) return;
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    double cand = __builtin_huge_val();
+    unsigned st = 0;
+    if (id < ncell) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        const double dt = clk->dt;
+        double al[NM], ar[NM], ru[3], E;
+        double a0[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            a0[m] = Uin.at(lv.cs[ALPHA] + m, i, j, k);
+            al[m] = a0[m];
+            ar[m] = Uin.at(lv.cs[ALPHARHO] + m, i, j, k);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ru[q] = Uin.at(lv.cs[RHOU] + q, i, j, k);
+        E = Uin.at(lv.cs[TOTAL_E], i, j, k);
+        for (int d = 0; d < lv.dim; ++d) {
+            const Faces& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
+            const int i1 = i + (d == 0), j1 = j + (d == 1), k1 = k + (d == 2);
+            const double dtdx = dt / lv.dx[d];
+            const double s0 = Fd.at(2 * NM + 4, i, j, k), s1 = Fd.at(2 * NM + 4, i1, j1, k1);
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                al[m] += dtdx * (Fd.at(m, i, j, k) - Fd.at(m, i1, j1, k1) - a0[m] * (s0 - s1));
+#pragma unroll
+            for (int m = 0; m < NM; ++m) ar[m] += dtdx * (Fd.at(NM + m, i, j, k) - Fd.at(NM + m, i1, j1, k1));
+#pragma unroll
+            for (int q = 0; q < 3; ++q) ru[q] += dtdx * (Fd.at(2 * NM + q, i, j, k) - Fd.at(2 * NM + q, i1, j1, k1));
+            E += dtdx * (Fd.at(2 * NM + 3, i, j, k) - Fd.at(2 * NM + 3, i1, j1, k1));
+        }
+        double rk[NM], rho, u[3], p;
+        c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+        if (STAGE == 2) {
+            // U1 = (U * 1/2) + (U2 * 1/2) over every component
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                al[m] = Ub.at(lv.cs[ALPHA] + m, i, j, k) * (1.0 / 2.0) + al[m] * (1.0 / 2.0);
+                ar[m] = Ub.at(lv.cs[ALPHARHO] + m, i, j, k) * (1.0 / 2.0) + ar[m] * (1.0 / 2.0);
+                rk[m] = Ub.at(lv.cs[RHO_K] + m, i, j, k) * (1.0 / 2.0) + rk[m] * (1.0 / 2.0);
+            }
+            rho = Ub.at(lv.cs[RHO], i, j, k) * (1.0 / 2.0) + rho * (1.0 / 2.0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                ru[q] = Ub.at(lv.cs[RHOU] + q, i, j, k) * (1.0 / 2.0) + ru[q] * (1.0 / 2.0);
+                u[q] = Ub.at(lv.cs[VELOCITY] + q, i, j, k) * (1.0 / 2.0) + u[q] * (1.0 / 2.0);
+            }
+            E = Ub.at(lv.cs[TOTAL_E], i, j, k) * (1.0 / 2.0) + E * (1.0 / 2.0);
+            p = Ub.at(lv.cs[PRESSURE], i, j, k) * (1.0 / 2.0) + p * (1.0 / 2.0);
+            if (ops & 4u) st |= cleanup_alpha<NM>(al);
+            if (ops & 16u) {
+                bool bad = false;
+#pragma unroll
+                for (int m = 0; m < NM; ++m) bad |= __builtin_isnan(al[m]) || __builtin_isnan(ar[m]) || __builtin_isnan(rk[m]);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) bad |= __builtin_isnan(ru[q]) || __builtin_isnan(u[q]);
+                bad |= __builtin_isnan(E) || __builtin_isnan(p);
+                if (bad) st |= 1u;
+            }
+            if (ops & 8u) {
+                const double a = sound<NM>(lv, al, ar, rk, rho, p);
+                Uo.at(lv.cs[SOUNDSPEED], i, j, k) = a;
+                for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(u[d])));
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            Uo.at(lv.cs[ALPHA] + m, i, j, k) = al[m];
+            Uo.at(lv.cs[ALPHARHO] + m, i, j, k) = ar[m];
+            Uo.at(lv.cs[RHO_K] + m, i, j, k) = rk[m];
+        }
+        Uo.at(lv.cs[RHO], i, j, k) = rho;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            Uo.at(lv.cs[RHOU] + q, i, j, k) = ru[q];
+            Uo.at(lv.cs[VELOCITY] + q, i, j, k) = u[q];
+        }
+        Uo.at(lv.cs[TOTAL_E], i, j, k) = E;
+        Uo.at(lv.cs[PRESSURE], i, j, k) = p;
+    }
+    if (STAGE == 2 && (ops & 8u)) {
+        // wave then block min of the positive candidates, one atomic per block
+        unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(b, off, 64);
+            b = o < b ? o : b;
+        }
+        __shared__ unsigned long long red[4];
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long mn = red[0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
+            atomicMin(&clk->dtmin_bits, mn);
+        }
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
+// getSoundSpeed + boxTimeStepFunction + min over the level (timestep.cpp:7-70).
+template <int NM>
+__global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    double cand = __builtin_huge_val();
+    if (id < ncell) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        double al[NM], ar[NM], rk[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+            ar[m] = U.at(lv.cs[ALPHARHO] + m, i, j, k);
+            rk[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
+        }
+        const double rho = U.at(lv.cs[RHO], i, j, k), p = U.at(lv.cs[PRESSURE], i, j, k);
+        const double a = sound<NM>(lv, al, ar, rk, rho, p);
+        U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
+        for (int d = 0; d < lv.dim; ++d)
+            cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(U.at(lv.cs[VELOCITY] + d, i, j, k))));
+    }
+    unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o < b ? o : b;
+    }
+    __shared__ unsigned long long red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long mn = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
+        atomicMin(&clk->dtmin_bits, mn);
+    }
+}
+
+// dt = CFL * min (timestep.cpp:61-69); with `driver` the loop test and the
+// clamps of main.cpp:121, 135-143.  Re-arms the min word.
+__global__ void k_clock_dt(Clock* clk, double cfl, int driver) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (driver) {
+        if (clk->done) return;
+        if (!(clk->t < clk->final_time)) {
+            clk->done = 1;
+            return;
+        }
+    }
+    const double mn = __builtin_bit_cast(double, clk->dtmin_bits);
+    double dt = cfl * mn;
+    if (driver) {
+        if (clk->t + dt > clk->final_time) dt = clk->final_time - clk->t;
+        if (clk->n < 5) dt *= 0.2;
+    }
+    clk->dt = dt;
+    clk->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;  // std::numeric_limits<double>::max()
+}
+// end of a loop iteration: the NaN stop of main.cpp:183-187 (before t, n
+// advance), else t += dt, n++ (main.cpp:121).
+__global__ void k_clock_advance(Clock* clk) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (clk->done) return;
+    if (clk->status & 1u) {
+        clk->done = 1;
+        return;
+    }
+    clk->t += clk->dt;
+    clk->n += 1;
+}
+__global__ void k_clock_final(Clock* clk, double final_time) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) clk->final_time = final_time;
+}
+__global__ void k_clock_clear_status(Clock* clk) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) clk->status = 0;
+}
+
+// stand-alone cleanUpAlpha over the valid cells (boxaccesscellarray.cpp:530-599)
+template <int NM>
+__global__ __launch_bounds__(256) void k_cleanup_alpha(Level lv, MF U, Clock* clk) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    double a[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) a[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+    const unsigned st = cleanup_alpha<NM>(a);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) U.at(lv.cs[ALPHA] + m, i, j, k) = a[m];
+    if (st) atomicOr(&clk->status, st);
+}
+
+// contains_nan over conserved and primitive components (boxaccesscellarray.cpp:601-645)
+template <int NM>
+__global__ __launch_bounds__(256) void k_nancheck(Level lv, MF U, Clock* clk) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    bool bad = false;
+    const int vars[6] = {ALPHA, ALPHARHO, RHO_K, RHOU, VELOCITY, TOTAL_E};
+    const int cnt[6] = {NM, NM, NM, 3, 3, 1};
+    for (int v = 0; v < 6; ++v)
+        for (int q = 0; q < cnt[v]; ++q) bad |= __builtin_isnan(U.at(lv.cs[vars[v]] + q, i, j, k));
+    bad |= __builtin_isnan(U.at(lv.cs[PRESSURE], i, j, k));
+    if (bad) atomicOr(&clk->status, 1u);
+}
+
+// setInitialConditions' pressure shift and primitiveToConservative
+// (initialise.cpp:104, boxaccesscellarray.cpp:124-174) over valid cells.
+template <int NM>
+__global__ __launch_bounds__(256) void k_init_p2c(Level lv, MF U, int shift) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    Prim<NM> P;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        P.a[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+        P.r[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) P.u[q] = U.at(lv.cs[VELOCITY] + q, i, j, k);
+    P.p = U.at(lv.cs[PRESSURE], i, j, k);
+    if (shift) {
+        double ar[NM], G, Ent, Pnt;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) ar[m] = P.a[m] * P.r[m];
+        eos_sums<NM>(lv, P.a, ar, G, Ent, Pnt);
+        P.p += Pnt / G;
+        U.at(lv.cs[PRESSURE], i, j, k) = P.p;
+    }
+    FSt<NM> S;
+    p2c_sound<NM>(lv, P, S);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) U.at(lv.cs[ALPHARHO] + m, i, j, k) = S.ar[m];
+    U.at(lv.cs[RHO], i, j, k) = S.rho;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) U.at(lv.cs[RHOU] + q, i, j, k) = S.u[q] * S.rho;
+    U.at(lv.cs[TOTAL_E], i, j, k) = S.E;
+}
+
+// conservativeToPrimitive over the valid cells.
+template <int NM>
+__global__ __launch_bounds__(256) void k_c2p(Level lv, MF U) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    double al[NM], ar[NM], ru[3], rk[NM], u[3], rho, p;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+        ar[m] = U.at(lv.cs[ALPHARHO] + m, i, j, k);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) ru[q] = U.at(lv.cs[RHOU] + q, i, j, k);
+    const double E = U.at(lv.cs[TOTAL_E], i, j, k);
+    c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) U.at(lv.cs[RHO_K] + m, i, j, k) = rk[m];
+    U.at(lv.cs[RHO], i, j, k) = rho;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) U.at(lv.cs[VELOCITY] + q, i, j, k) = u[q];
+    U.at(lv.cs[PRESSURE], i, j, k) = p;
+}
+
+// sigma = -p I for fluids (stressTensor, boxaccesscellarray.cpp:176-216).
+__global__ __launch_bounds__(256) void k_derive_fluid(Level lv, MF U) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    const double p = U.at(lv.cs[PRESSURE], i, j, k);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) U.at(lv.cs[SIGMA] + 3 * r + c, i, j, k) = -p * (r == c ? 1.0 : 0.0);
+}
+
+// zero-fill for scratch that a caller may capture in a graph (no memset nodes)
+__global__ void k_zero(double* p, long long n) {
+    long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; id < n; id += stride) p[id] = 0.0;
+}
+
+}  // namespace tj

@@ -1,0 +1,532 @@
+// C ABI of the gfx950 per-cell update (include/tenjin.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tenjin.h"
+#include "tj_kernels.h"
+
+using namespace tj;
+
+struct tj_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool bound = false;
+    tj_level_desc desc{};
+    Level lv{};
+    int nmat = 0;
+    uint8_t* d_edge[3] = {nullptr, nullptr, nullptr};
+    double* d_flux[3] = {nullptr, nullptr, nullptr};
+    Faces F[3]{};
+    int nfc = 0;  // flux components per face
+    double* d_urstar = nullptr;
+    double* d_star[2] = {nullptr, nullptr};
+    double* d_estar[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    long long estride[3] = {0, 0, 0};
+    Clock* d_clk = nullptr;
+    unsigned* d_status = nullptr;
+    long long pass = 0;
+    long long dpass[3] = {0, 0, 0};
+    int seglen = 8;
+};
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+            return TJ_EHIP;                                                                 \
+        }                                                                                   \
+    } while (0)
+
+namespace {
+
+// BoxArray::maxSize of one domain box along one dimension: reduce by powers
+// of two, cut into nearly equal blocks, larger blocks first from the high end.
+std::vector<int> chop_dim(int n, int chunk) {
+    std::vector<int> starts;
+    if (n <= chunk) return {0};
+    int ratio = 1, bs = chunk, nlen = n;
+    while (bs % 2 == 0 && nlen % 2 == 0) { ratio *= 2; bs /= 2; nlen /= 2; }
+    int numblk = nlen / bs + (nlen % bs ? 1 : 0);
+    int sz = nlen / numblk, extra = nlen % numblk;
+    int hi = n - 1;
+    for (int k = 0; k < numblk - 1; ++k) {
+        int ksize = (k < extra ? sz + 1 : sz) * ratio;
+        int pos = hi - ksize + 1;
+        starts.push_back(pos);
+        hi = pos - 1;
+    }
+    starts.push_back(0);
+    std::sort(starts.begin(), starts.end());
+    return starts;
+}
+
+// AccessPattern (accesspattern.cpp:44-79), numberOfMixtures = 0.
+int build_layout(const tj_level_desc& d, int cs[NVAR]) {
+    for (int v = 0; v < NVAR; ++v) cs[v] = -1;
+    int n = 0;
+    auto add = [&](Var v, int cnt) { cs[v] = n; n += cnt; };
+    add(ALPHA, d.nmat);
+    add(ALPHARHO, d.nmat);
+    add(RHO_K, d.nmat);
+    add(RHO, 1);
+    add(RHOU, 3);
+    add(VELOCITY, 3);
+    add(TOTAL_E, 1);
+    add(PRESSURE, 1);
+    add(SOUNDSPEED, 1);
+    add(USTAR, 1);
+    add(SIGMA, 9);
+    if (d.SOLID) {
+        add(V_TENSOR, 9);
+        add(VSTAR, 9);
+        add(DEVH, 9);
+        add(HJ2, 1);
+        if (d.PLASTIC) {
+            add(EPSILON, d.nmat);
+            add(ALPHARHOEPSILON, d.nmat);
+        }
+    }
+    return n;
+}
+
+MF view(const tj_mf* m) { return MF{m->data, m->jstride, m->kstride, m->nstride}; }
+
+unsigned grid1(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void free_all(tj_ctx* c) {
+    for (int d = 0; d < 3; ++d) {
+        if (c->d_edge[d]) hipFree(c->d_edge[d]);
+        if (c->d_flux[d]) hipFree(c->d_flux[d]);
+        for (int q = 0; q < 2; ++q)
+            if (c->d_estar[d][q]) hipFree(c->d_estar[d][q]);
+        c->d_edge[d] = nullptr;
+        c->d_flux[d] = nullptr;
+        c->d_estar[d][0] = c->d_estar[d][1] = nullptr;
+    }
+    if (c->d_urstar) hipFree(c->d_urstar);
+    for (int q = 0; q < 2; ++q)
+        if (c->d_star[q]) hipFree(c->d_star[q]);
+    c->d_urstar = c->d_star[0] = c->d_star[1] = nullptr;
+    c->bound = false;
+}
+
+int zero(tj_ctx* c, double* p, long long n) {
+    if (!p || n <= 0) return TJ_OK;
+    unsigned g = grid1(n, 256);
+    if (g > 65535u * 16u) g = 65535u * 16u;
+    hipLaunchKernelGGL(k_zero, dim3(g), dim3(256), 0, c->stream, p, n);
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tj_create(tj_ctx** out, int device) {
+    if (!out) return TJ_EINVAL;
+    tj_ctx* c = new tj_ctx();
+    c->device = device;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return TJ_EHIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TJ_EHIP; }
+    if (hipMalloc(&c->d_clk, sizeof(Clock)) != hipSuccess) { delete c; return TJ_EHIP; }
+    Clock init{};
+    init.final_time = __builtin_huge_val();
+    init.dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
+    if (hipMemcpy(c->d_clk, &init, sizeof(Clock), hipMemcpyHostToDevice) != hipSuccess) { delete c; return TJ_EHIP; }
+    *out = c;
+    return TJ_OK;
+}
+
+void tj_destroy(tj_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_all(c);
+    if (c->d_clk) hipFree(c->d_clk);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* tj_last_error(const tj_ctx* c) { return c ? c->err.c_str() : "null context"; }
+void* tj_stream(tj_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
+    if (!c || !d) return TJ_EINVAL;
+    if (d->dim < 1 || d->dim > 3 || d->nmat < 1 || d->nmat > TJ_MAX_MATERIALS) {
+        c->err = "bind_level: bad dim or nmat";
+        return TJ_EINVAL;
+    }
+    if (d->nmat != 2) {
+        c->err = "bind_level: this build instantiates two-material levels";
+        return TJ_EUNSUPPORTED;
+    }
+    if (d->SOLID) {
+        c->err = "bind_level: solid levels are not available in this build";
+        return TJ_EUNSUPPORTED;
+    }
+    if (d->THINC && d->dim == 3) {
+        c->err = "bind_level: THINC is defined for 1D and 2D levels only (thinc.cpp:151-162)";
+        return TJ_EUNSUPPORTED;
+    }
+    if (!d->MUSCL) {
+        c->err = "bind_level: MUSCL = 0 is not available in this build";
+        return TJ_EUNSUPPORTED;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_all(c);
+    c->desc = *d;
+    Level& lv = c->lv;
+    lv = Level{};
+    lv.dim = d->dim;
+    for (int a = 0; a < 3; ++a) {
+        lv.n[a] = a < d->dim ? d->n[a] : 1;
+        lv.dx[a] = a < d->dim ? d->L[a] / d->n[a] : 1.0;
+        lv.bc_lo[a] = a < d->dim ? d->bc_lo[a] : 0;
+        lv.bc_hi[a] = a < d->dim ? d->bc_hi[a] : 0;
+        if (lv.n[a] < 1) { c->err = "bind_level: n < 1"; return TJ_EINVAL; }
+    }
+    lv.yn[0] = -1.0 / (8.0 * lv.dx[0]);
+    lv.yn[1] = -1.0 / (8.0 * lv.dx[1]);
+    lv.nmat = d->nmat;
+    lv.solid = d->SOLID;
+    lv.plastic = d->PLASTIC;
+    lv.thinc = d->THINC;
+    lv.beta0 = d->THINCbeta;
+    lv.cfl = d->CFL;
+    for (int m = 0; m < d->nmat; ++m) {
+        const tj_material& M = d->mat[m];
+        MatC& C = lv.m[m];
+        C.eos = M.eos;
+        C.solid = M.solid;
+        C.plastic = M.plastic;
+        C.Gg = M.gamma - 1.0;
+        C.gp1 = C.Gg + 1.0;
+        C.xi = 1.0 / C.Gg;
+        C.pref = M.pref;
+        C.eref = M.eref;
+        C.CV = M.CV;
+        C.rho0 = M.rho0;
+        C.c0 = M.c0;
+        C.s = M.s;
+        C.G0 = M.G0;
+        C.yield = M.yield_stress;
+    }
+    build_layout(*d, lv.cs);
+    lv.ncells = (long long)lv.n[0] * lv.n[1] * lv.n[2];
+    c->nmat = d->nmat;
+    for (int a = 0; a < 3; ++a) {
+        std::vector<uint8_t> fl(lv.n[a], 0);
+        std::vector<int> st = a < d->dim ? chop_dim(lv.n[a], d->max_grid_size) : std::vector<int>{0};
+        for (size_t b = 0; b < st.size(); ++b) {
+            int lo = st[b];
+            int hi = (b + 1 < st.size() ? st[b + 1] : lv.n[a]) - 1;
+            fl[lo] |= 1;
+            fl[hi] |= 2;
+        }
+        HIPCHK(c, hipMalloc(&c->d_edge[a], fl.size()));
+        HIPCHK(c, hipMemcpy(c->d_edge[a], fl.data(), fl.size(), hipMemcpyHostToDevice));
+        lv.edge[a] = c->d_edge[a];
+    }
+    c->nfc = 2 * d->nmat + 5;
+    for (int a = 0; a < d->dim; ++a) {
+        Faces& F = c->F[a];
+        F.nx = lv.n[0] + (a == 0);
+        F.ny = lv.n[1] + (a == 1);
+        F.nz = lv.n[2] + (a == 2);
+        F.cs = (long long)F.nx * F.ny * F.nz;
+        HIPCHK(c, hipMalloc(&c->d_flux[a], sizeof(double) * F.cs * c->nfc));
+        F.p = c->d_flux[a];
+        c->estride[a] = F.cs;
+    }
+    if (d->THINC) {
+        const long long pl = lv.ncells * d->nmat;
+        HIPCHK(c, hipMalloc(&c->d_urstar, sizeof(double) * pl));
+        for (int q = 0; q < 2; ++q) HIPCHK(c, hipMalloc(&c->d_star[q], sizeof(double) * pl));
+        for (int a = 0; a < d->dim; ++a)
+            for (int q = 0; q < 2; ++q) HIPCHK(c, hipMalloc(&c->d_estar[a][q], sizeof(double) * c->estride[a] * d->nmat));
+    }
+    c->bound = true;
+    return tj_reset_history(c);
+}
+
+int tj_reset_history(tj_ctx* c) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    c->pass = 0;
+    c->dpass[0] = c->dpass[1] = c->dpass[2] = 0;
+    if (!c->desc.THINC) return TJ_OK;
+    const long long pl = c->lv.ncells * c->nmat;
+    int r;
+    if ((r = zero(c, c->d_urstar, pl))) return r;
+    for (int q = 0; q < 2; ++q)
+        if ((r = zero(c, c->d_star[q], pl))) return r;
+    for (int a = 0; a < c->desc.dim; ++a)
+        for (int q = 0; q < 2; ++q)
+            if ((r = zero(c, c->d_estar[a][q], c->estride[a] * c->nmat))) return r;
+    return TJ_OK;
+}
+
+int tj_ncomp(const tj_ctx* c) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int cs[NVAR];
+    return build_layout(c->desc, cs);
+}
+
+int tj_comp_start(const tj_ctx* c, int v) {
+    if (!c || !c->bound || v < 0 || v >= NVAR) return -1;
+    return c->lv.cs[v];
+}
+
+static int check_mf(tj_ctx* c, const tj_mf* m) {
+    if (!m || !m->data) { c->err = "null tj_mf"; return TJ_EINVAL; }
+    if (m->ncomp < tj_ncomp(c)) { c->err = "tj_mf has fewer components than the bound layout"; return TJ_EINVAL; }
+    if (m->jstride < c->lv.n[0] || m->kstride < m->jstride * c->lv.n[1] || m->nstride < m->kstride * c->lv.n[2]) {
+        c->err = "tj_mf strides do not cover the level";
+        return TJ_EINVAL;
+    }
+    return TJ_OK;
+}
+
+int tj_compute_dt(tj_ctx* c, const tj_mf* U, double* dt_out) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r = check_mf(c, U);
+    if (r) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long long n = c->lv.ncells;
+    hipLaunchKernelGGL(k_sound_dt<2>, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+    hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 0);
+    HIPCHK(c, hipGetLastError());
+    if (dt_out) {
+        Clock h;
+        HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *dt_out = h.dt;
+    }
+    return TJ_OK;
+}
+
+int tj_set_dt(tj_ctx* c, double dt, int reset_clock) {
+    if (!c) return TJ_EINVAL;
+    Clock h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    h.dt = dt;
+    if (reset_clock) {
+        h.t = 0.0;
+        h.n = 0;
+        h.done = 0;
+        h.status = 0;
+        h.final_time = __builtin_huge_val();
+        h.dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_clk, &h, sizeof(Clock), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TJ_OK;
+}
+
+int tj_get_clock(tj_ctx* c, double* t, double* dt, int* n) {
+    if (!c) return TJ_EINVAL;
+    Clock h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t) *t = h.t;
+    if (dt) *dt = h.dt;
+    if (n) *n = h.n;
+    return TJ_OK;
+}
+
+int tj_driver_dt(tj_ctx* c, double final_time) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    hipLaunchKernelGGL(k_clock_final, dim3(1), dim3(64), 0, c->stream, c->d_clk, final_time);
+    hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 1);
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+static int launch_fluxes(tj_ctx* c, const MF& Uin) {
+    const Level& lv = c->lv;
+    const bool th = c->desc.THINC != 0;
+    for (int d = 0; d < lv.dim; ++d) {
+        Hist h{};
+        if (th) {
+            const int q = (int)(c->pass & 1), e = (int)(c->dpass[d] & 1);
+            h.urstar = c->d_urstar;
+            h.star_in = c->d_star[q];
+            h.star_out = c->d_star[q ^ 1];
+            h.edge_in = c->d_estar[d][e];
+            h.edge_out = c->d_estar[d][e ^ 1];
+            h.pstride = lv.ncells;
+            h.estride = c->estride[d];
+        }
+        if (d == 0) {
+            const int per = th ? 61 : 63;
+            const long long nseg = (lv.n[0] + 1 + per - 1) / per;
+            const long long waves = nseg * lv.n[1] * lv.n[2];
+            const unsigned g = grid1(waves, 4);
+            if (th) hipLaunchKernelGGL((k_flux_x<2, true>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[0], h, c->d_clk);
+            else hipLaunchKernelGGL((k_flux_x<2, false>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[0], h, c->d_clk);
+        } else {
+            const long long nstrip = (lv.n[0] + 63) / 64;
+            const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
+            const long long no = d == 1 ? lv.n[2] : lv.n[1];
+            const unsigned g = grid1(nstrip * nseg * no, 4);
+            if (d == 1) {
+                if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk);
+                else hipLaunchKernelGGL((k_flux_march<2, false, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk);
+            } else {
+                hipLaunchKernelGGL((k_flux_march<2, false, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk);
+            }
+        }
+        HIPCHK(c, hipGetLastError());
+        if (th) {
+            c->pass += 1;
+            c->dpass[d] += 1;
+        }
+    }
+    return TJ_OK;
+}
+
+int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_mf* Uout, int prec) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    if (prec != TJ_PREC_FP64) { c->err = "rk_stage: only the FP64 form is available in this build"; return TJ_EUNSUPPORTED; }
+    int r;
+    if ((r = check_mf(c, Uin)) || (r = check_mf(c, Uout))) return r;
+    if (stage == 2 && (r = check_mf(c, Ubase))) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const MF in = view(Uin);
+    if ((r = launch_fluxes(c, in))) return r;
+    const long long n = c->lv.ncells;
+    const MF base = stage == 2 ? view(Ubase) : in;
+    if (stage == 1)
+        hipLaunchKernelGGL((k_update<2, 1>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, in, base, view(Uout),
+                           c->F[0], c->F[1], c->F[2], c->d_clk, 0u);
+    else
+        hipLaunchKernelGGL((k_update<2, 2>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, in, base, view(Uout),
+                           c->F[0], c->F[1], c->F[2], c->d_clk, 0u);
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+int tj_post_step(tj_ctx* c, tj_mf* U, unsigned ops) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r;
+    if ((r = check_mf(c, U))) return r;
+    const long long n = c->lv.ncells;
+    if (ops & TJ_OP_CLEAN_ALPHA)
+        hipLaunchKernelGGL(k_cleanup_alpha<2>, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+    if (ops & TJ_OP_NANCHECK)
+        hipLaunchKernelGGL(k_nancheck<2>, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+    if (ops & TJ_OP_DT) {
+        hipLaunchKernelGGL(k_sound_dt<2>, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+    }
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    if (prec != TJ_PREC_FP64) { c->err = "step: only the FP64 form is available in this build"; return TJ_EUNSUPPORTED; }
+    int r;
+    if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long long n = c->lv.ncells;
+    const MF s = view(S), t = view(T);
+    if ((r = launch_fluxes(c, s))) return r;
+    hipLaunchKernelGGL((k_update<2, 1>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, s, s, t, c->F[0], c->F[1],
+                       c->F[2], c->d_clk, 0u);
+    if ((r = launch_fluxes(c, t))) return r;
+    const unsigned ops = TJ_OP_CLEAN_ALPHA | TJ_OP_DT | TJ_OP_NANCHECK;
+    hipLaunchKernelGGL((k_update<2, 2>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, t, s, s, c->F[0], c->F[1],
+                       c->F[2], c->d_clk, ops);
+    hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, c->stream, c->d_clk);
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+int tj_run(tj_ctx* c, tj_mf* S, tj_mf* T, double final_time, int max_steps, int prec, int check_every,
+           double* t_out, int* n_out) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r;
+    if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
+    if ((r = tj_set_dt(c, 0.0, 1))) return r;
+    // getTimeStep of the initial state feeds the first driver dt
+    hipLaunchKernelGGL(k_sound_dt<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(S), c->d_clk);
+    HIPCHK(c, hipGetLastError());
+    if (check_every < 1) check_every = 1;
+    int steps = 0;
+    while (steps < max_steps) {
+        if ((r = tj_driver_dt(c, final_time))) return r;
+        if ((r = tj_step(c, S, T, prec))) return r;
+        ++steps;
+        if (steps % check_every == 0 || steps == max_steps) {
+            Clock h;
+            HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (h.done) break;
+        }
+    }
+    Clock h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t_out) *t_out = h.t;
+    if (n_out) *n_out = h.n;
+    return TJ_OK;
+}
+
+int tj_init_from_prims(tj_ctx* c, tj_mf* U, int shift) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r;
+    if ((r = check_mf(c, U))) return r;
+    hipLaunchKernelGGL(k_init_p2c<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+int tj_c2p(tj_ctx* c, tj_mf* U) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r;
+    if ((r = check_mf(c, U))) return r;
+    hipLaunchKernelGGL(k_c2p<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+int tj_derive(tj_ctx* c, tj_mf* U) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r;
+    if ((r = check_mf(c, U))) return r;
+    hipLaunchKernelGGL(k_derive_fluid, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+int tj_status(tj_ctx* c, uint32_t* status, int clear) {
+    if (!c) return TJ_EINVAL;
+    Clock h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (status) *status = h.status;
+    if (clear) {
+        hipLaunchKernelGGL(k_clock_clear_status, dim3(1), dim3(64), 0, c->stream, c->d_clk);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return TJ_OK;
+}
+
+int tj_synchronize(tj_ctx* c) {
+    if (!c) return TJ_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TJ_OK;
+}
+
+}  // extern "C"
