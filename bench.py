@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Throughput of the per-cell explicit update on one MI355X.
+
+Workload: the 2D two-material stiffened-gas shock/inclusion interaction on a
+single level of 2048 x 1024 cells in 64 x 64 boxes (MUSCL + THINC/BVD, beta 1.6,
+HLLC, CFL 0.5), synthetic initial data generated here.  One "step" is one full
+time step of the reference's loop body (main.cpp:124-171): CFL dt with the
+driver clamps, two RK stages, the RK average, cleanUpAlpha, the NaN scan and the
+next step's CFL reduction.  The state is resident in HBM before timing starts.
+
+Prints one JSON line:  metric/value (Mcell-updates/s = cells x steps / s),
+roofline of the dominant kernel (HIP-event times on the library's stream over
+the timed region), step roofline (464 B per cell-update) and the CPU baseline
+(the oracle restatement on this host's cores, a bounded sample of the same
+workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell of each kernel (unique state reads + writes,
+# FP64, two materials; halo re-reads and THINC history reads not counted)
+KERNEL_BYTES_PER_CELL = {
+    "flux_x": 64 + 72 + 16,            # 8 primitives in, 9 flux comps out, 2 star alphas out
+    "flux_y": 64 + 72 + 16,
+    "flux_z": 64 + 72,
+    "update_stage1": 64 + 2 * 72 + 120,          # 8 conserved in, 2 faces x 9 comps, 15 comps out
+    "update_stage2_post": 64 + 2 * 72 + 120 + 120 + 8,  # + 15 base comps in, + sound speed out
+}
+STEP_BYTES_PER_CELL_UPDATE = 464  # 8 * (3 * N_s + 2 * N_c), N_s = 14, N_c = 8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--prec", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--box", type=int, default=64)
+    ap.add_argument("--seglen", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a plain 1 GiB device copy first (PMC byte-counter calibration)")
+    return ap.parse_args()
+
+
+def cpu_baseline(pb, budget_s: float):
+    """The oracle (CPU restatement) on this host's cores: first steps of the
+    same workload from its initial condition, until the time budget is used."""
+    from oracle.oracle import Oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = Oracle(pb)
+    o.step(0.2 * o.timestep())  # warm-up step (not timed)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        dt = o.timestep()
+        o.step(dt if n + 1 >= 5 else 0.2 * dt)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 20:
+            break
+    return {
+        "value": pb.ncells * n / el / 1e6,
+        "unit": "Mcell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle restatement, {pb.n[0]}x{pb.n[1]} level, {n} steps after 1 warm-up step, {el:.1f} s, {threads} OpenMP threads over boxes",
+    }
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch of `name` from the committed rocprofv3 PMC summary."""
+    path = os.path.join(HERE, "profiles", "pmc_traffic_shock_inclusion_2d.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == "traffic_bytes_per_launch" and len(parts) >= 3 and parts[1] == name:
+            return float(parts[2])
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tenjin_amd import GpuLevel, TJ_PREC_FP64, TJ_PREC_MIXED, configs
+
+    if args.calib_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    pb = configs.shock_inclusion_2d(nx=args.nx, ny=args.ny, box=args.box)
+    prec = TJ_PREC_FP64 if args.prec == "fp64" else TJ_PREC_MIXED
+    g = GpuLevel(pb)
+    g.set_tuning(args.seglen)
+    g.init_from_prims(pb.prims, pb.shift_pressure)
+    g.begin_run(float("inf"))
+    g.advance(args.warmup, prec)
+    g.sync()
+
+    g.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.advance(args.steps, prec)
+    g.sync()
+    t1 = time.perf_counter()
+    prof = g.prof_report()
+    g.prof_enable(False)
+    status = g.status()
+    _, _, nsteps_done = g.clock()
+    elapsed = t1 - t0
+
+    ncell = pb.ncells
+    value = ncell * args.steps / elapsed / 1e6
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dname, (dcount, dms) = dom
+    davg_s = dms / dcount / 1e3
+    dbytes = KERNEL_BYTES_PER_CELL[dname] * ncell
+    achieved = dbytes / davg_s / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": dname,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": read_traffic(dname),
+        "algorithmic_bytes_per_launch": dbytes,
+        "avg_launch_ms": round(davg_s * 1e3, 4),
+    }
+    kernel_ms = {k: round(v[1] / v[0], 4) for k, v in prof.items()}
+    step_gbs = value * 1e6 * STEP_BYTES_PER_CELL_UPDATE / 1e9
+
+    out = {
+        "metric": "Mcell-updates/s on 1 GPU",
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == TJ_PREC_FP64 else "f32-flux/f64-update",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D two-material shock/inclusion, {pb.n[0]}x{pb.n[1]} cells, {pb.max_grid_size}^2 boxes, "
+                        "MUSCL+THINC(beta 1.6)+HLLC, CFL 0.5",
+            "cells": ncell,
+            "precision": args.prec,
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "step_roofline": {
+            "bytes_per_cell_update": STEP_BYTES_PER_CELL_UPDATE,
+            "achieved": round(step_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        },
+        "kernel_avg_ms": kernel_ms,
+        "status": status,
+        "device_steps": nsteps_done,
+    }
+    del g
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pb, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

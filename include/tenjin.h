@@ -109,6 +109,8 @@ void* tj_stream(tj_ctx* ctx);
  * dx = L/n.  Allocates the scratch (flux planes, THINC history planes).
  * Replaces the set-up of main.cpp:29-95. */
 int tj_bind_level(tj_ctx* ctx, const tj_level_desc* desc);
+/* Rows per marching segment of the y/z flux passes (tuning; default 8). */
+int tj_set_tuning(tj_ctx* ctx, int seglen);
 /* Number of components of the bound layout and the first component of a
  * reference Variable (structdefinitions.h:10-32 numbering); -1 if absent. */
 int tj_ncomp(const tj_ctx* ctx);
@@ -145,6 +147,14 @@ int tj_post_step(tj_ctx* ctx, tj_mf* U, unsigned ops);
  * the next dt fused into stage 2.  dt is the device-resident dt. */
 int tj_step(tj_ctx* ctx, tj_mf* S, tj_mf* T, int prec);
 
+/* Start of the time loop: t = 0, n = 0, status cleared, and the CFL minimum of
+ * the initial state S primed for the first tj_driver_dt. */
+int tj_begin_run(tj_ctx* ctx, const tj_mf* S, double final_time);
+/* nsteps iterations of the loop body of main.cpp:121-191 (tj_driver_dt then
+ * tj_step), queued without synchronisation; iterations after the loop test
+ * fails (t >= final_time) or after a NaN stop are no-ops on the device. */
+int tj_advance_steps(tj_ctx* ctx, tj_mf* S, tj_mf* T, int nsteps, int prec);
+
 /* The time loop of main.cpp:121-191 on the device words: dt, clamp, step,
  * NaN stop.  Returns steps taken in *n_out and final time in *t_out. */
 int tj_run(tj_ctx* ctx, tj_mf* S, tj_mf* T, double final_time, int max_steps, int prec,
@@ -159,9 +169,16 @@ int tj_c2p(tj_ctx* ctx, tj_mf* U);
 /* Derived components sigma (and devH, HJ2 in solid runs) from the state. */
 int tj_derive(tj_ctx* ctx, tj_mf* U);
 
+/* Per-kernel timing with HIP events recorded on the context's stream around
+ * every launch the library makes (no reference counterpart; a measurement
+ * hook).  tj_prof_report writes "name count total_ms\n" lines; synchronises. */
+int tj_prof_enable(tj_ctx* ctx, int on);
+int tj_prof_report(tj_ctx* ctx, char* buf, int len);
+
 /* Status word (TJ_ST_*), OR-ed since the last call with clear != 0. Synchronises. */
 int tj_status(tj_ctx* ctx, uint32_t* status, int clear);
-/* Zeroes the THINC history planes (start of a run, D4 of the oracle). */
+/* Zeroes the THINC history planes.  The reference's ULStar/URStar arrays hold
+ * allocation contents before their first write; zero is used here. */
 int tj_reset_history(tj_ctx* ctx);
 int tj_synchronize(tj_ctx* ctx);
 

@@ -81,6 +81,7 @@ def lib():
         L.tjo_run.restype = ctypes.c_int
         for f in ("tjo_cleanup_v", "tjo_plastic", "tjo_c2p", "tjo_p2c", "tjo_sound"):
             getattr(L, f).argtypes = [vp]
+        L.tjo_reconstruct.argtypes = [vp, ctypes.c_int]
         L.tjo_face_flux.argtypes = [vp, ctypes.c_int, dp, dp, dp]
         L.tjo_vanleer.argtypes = [ctypes.c_double]
         L.tjo_vanleer.restype = ctypes.c_double
@@ -163,6 +164,36 @@ class Oracle:
         t = ctypes.c_double(0.0)
         n = lib().tjo_run(self.h, final_time, max_steps, ctypes.byref(t))
         return n, t.value
+
+    def reconstruct(self, d: int):
+        """UL, UR of direction d from the current state (MUSCL [+THINC/BVD], p2c, sound speed)."""
+        lib().tjo_reconstruct(self.h, d)
+        return self.get(self.UL), self.get(self.UR)
+
+    def face_flux(self, d: int, left: np.ndarray, right: np.ndarray) -> np.ndarray:
+        l = np.ascontiguousarray(left, dtype=np.float64)
+        r = np.ascontiguousarray(right, dtype=np.float64)
+        out = np.zeros(self.ncomp)
+        lib().tjo_face_flux(self.h, d, _dp(l), _dp(r), _dp(out))
+        return out
+
+    def c2p(self):
+        lib().tjo_c2p(self.h)
+
+    def p2c(self):
+        lib().tjo_p2c(self.h)
+
+    def sound(self):
+        lib().tjo_sound(self.h)
+
+    def cleanup_alpha(self) -> int:
+        return lib().tjo_cleanup_alpha(self.h)
+
+    def cleanup_v(self):
+        lib().tjo_cleanup_v(self.h)
+
+    def plastic(self):
+        lib().tjo_plastic(self.h)
 
     def status(self) -> int:
         return lib().tjo_status(self.h)

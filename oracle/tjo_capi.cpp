@@ -195,9 +195,40 @@ void tjo_face_flux(void* h, int d, const double* left, const double* right, doub
     s->sound_cell(ur, i0[0], i0[1], i0[2]);
     s->p2c_cell(ul, i1[0], i1[1], i1[2]);
     s->sound_cell(ul, i1[0], i1[1], i1[2]);
-    if (s->p.SOLID) s->calc_5wave(0, d);
-    else s->calc_fluxes(0, d);
+    if (s->p.SOLID) {
+        double slt = 0.0, srt = 0.0;
+        s->wave5_face(0, d, i1[0], i1[1], i1[2], slt, srt);
+    } else {
+        s->hllc_face(0, d, i1[0], i1[1], i1[2]);
+    }
     for (int n = 0; n < s->L.ncomp; ++n) out[n] = s->flux[d][0](i1[0], i1[1], i1[2], n);
+}
+
+// Reconstruction pass alone (hllc.cpp:589-638 without the flux): UL, UR of
+// direction d from U1, MUSCL then THINC/BVD and cleanUpAlpha when enabled,
+// then primitiveToConservative and getSoundSpeed.  Read back with which = 3/4.
+void tjo_reconstruct(void* h, int d) {
+    auto* s = static_cast<tjo::Solver*>(h);
+    s->fill(s->U1);
+    s->U_in = &s->U1;
+    tjo::Solver::copy(s->UL, s->U1);
+    tjo::Solver::copy(s->UR, s->U1);
+    for (size_t b = 0; b < s->boxes.size(); ++b) {
+        if (s->p.MUSCL) s->muscl((int)b, d);
+        if (s->p.THINC) {
+            s->thinc((int)b, d);
+            s->for_valid((int)b, [&](int i, int j, int k) {
+                s->alpha_fatal |= s->cleanup_alpha_cell(s->UL.f[b], i, j, k);
+                s->alpha_fatal |= s->cleanup_alpha_cell(s->UR.f[b], i, j, k);
+            });
+        }
+        s->for_valid((int)b, [&](int i, int j, int k) {
+            s->p2c_cell(s->UL.f[b], i, j, k);
+            s->p2c_cell(s->UR.f[b], i, j, k);
+            s->sound_cell(s->UL.f[b], i, j, k);
+            s->sound_cell(s->UR.f[b], i, j, k);
+        });
+    }
 }
 
 double tjo_vanleer(double r) { return tjo::Solver::vanleer(r); }

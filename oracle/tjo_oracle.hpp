@@ -25,6 +25,7 @@
 //   D5 exp/cosh/tanh/pow(.,-1/3) come from tjo_math.hpp (math_mode 0).
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -578,6 +579,7 @@ class Solver {
     void fill(CellArray& ca) {
         const int nc = L.ncomp;
         const int uo = L.start[VELOCITY], mo = L.start[RHOU];
+#pragma omp parallel for schedule(dynamic)
         for (size_t b = 0; b < boxes.size(); ++b) {
             Fab& f = ca.f[b];
             const IBox& bx = boxes[b];
@@ -615,6 +617,7 @@ class Solver {
 
     // ------------------------------------------------------------ MultiFab algebra
     static void copy(CellArray& dst, CellArray& src) {  // cellarray.cpp:80-84
+#pragma omp parallel for schedule(static)
         for (size_t b = 0; b < dst.f.size(); ++b) dst.f[b].d = src.f[b].d;
     }
 
@@ -817,52 +820,55 @@ class Solver {
                 uss(V_TENSOR, 0, r, c) += (uss(V_TENSOR, 0, d, c) * (uss(VELOCITY, 0, r) - uk(VELOCITY, 0, r))) / (Ss - SKT);
         }
     }
-    int fatal = 0;  // 1: NaN wave speeds (hllc.cpp:169-200, 367-373)
+    std::atomic<int> fatal{0};  // 1: NaN wave speeds (hllc.cpp:169-200, 367-373)
 
-    // hllc.cpp:384-487
-    void calc_fluxes(int b, int d) {
+    // hllc.cpp:384-487: one face (i,j,k) of box b; the box loop is calc_fluxes
+    void hllc_face(int b, int d, int i, int j, int k) {
         Fab& fl = flux[d][b];
+        int ext[3] = {0, 0, 0};
+        ext[d] = 1;
+        CellV l{this, &UR.f[b], i - ext[0], j - ext[1], k - ext[2]};
+        CellV r{this, &UL.f[b], i, j, k};
+        CellV st{this, &ULStar.f[b], i, j, k};
+        double SR = dmax(std::fabs(l(VELOCITY, 0, d)) + l(SOUNDSPEED), std::fabs(r(VELOCITY, 0, d)) + r(SOUNDSPEED));
+        double SL = -SR;
+        double Ss = sstar(l, r, SL, SR, d);
+        A(fl, i, j, k, USTAR) = Ss;
+        if (SL >= 0.0) {
+            star_state(l, st, SL, Ss, d);
+            for (const Spec& n : L.cons) A(fl, i, j, k, n) = n.var == ALPHA ? pflux(n, st, d) : pflux(n, l, d);
+        } else if (Ss >= 0.0) {
+            star_state(l, st, SL, Ss, d);
+            for (const Spec& n : L.cons)
+                A(fl, i, j, k, n) = n.var == ALPHA ? pflux(n, st, d) : pflux(n, l, d) + SL * (st(n) - l(n));
+        } else if (SR >= 0.0) {
+            star_state(r, st, SR, Ss, d);
+            for (const Spec& n : L.cons)
+                A(fl, i, j, k, n) = n.var == ALPHA ? pflux(n, st, d) : pflux(n, r, d) + SR * (st(n) - r(n));
+        } else {
+            star_state(r, st, SR, Ss, d);
+            for (const Spec& n : L.cons) A(fl, i, j, k, n) = n.var == ALPHA ? pflux(n, st, d) : pflux(n, r, d);
+        }
+    }
+    void calc_fluxes(int b, int d) {
         const IBox& bx = boxes[b];
         int ext[3] = {0, 0, 0};
         ext[d] = 1;
         for (int k = bx.lo[2]; k <= bx.hi[2] + ext[2]; ++k)
             for (int j = bx.lo[1]; j <= bx.hi[1] + ext[1]; ++j)
-                for (int i = bx.lo[0]; i <= bx.hi[0] + ext[0]; ++i) {
-                    CellV l{this, &UR.f[b], i - ext[0], j - ext[1], k - ext[2]};
-                    CellV r{this, &UL.f[b], i, j, k};
-                    CellV st{this, &ULStar.f[b], i, j, k};
-                    double SR = dmax(std::fabs(l(VELOCITY, 0, d)) + l(SOUNDSPEED), std::fabs(r(VELOCITY, 0, d)) + r(SOUNDSPEED));
-                    double SL = -SR;
-                    double Ss = sstar(l, r, SL, SR, d);
-                    A(fl, i, j, k, USTAR) = Ss;
-                    if (SL >= 0.0) {
-                        star_state(l, st, SL, Ss, d);
-                        for (const Spec& n : L.cons) A(fl, i, j, k, n) = n.var == ALPHA ? pflux(n, st, d) : pflux(n, l, d);
-                    } else if (Ss >= 0.0) {
-                        star_state(l, st, SL, Ss, d);
-                        for (const Spec& n : L.cons)
-                            A(fl, i, j, k, n) = n.var == ALPHA ? pflux(n, st, d) : pflux(n, l, d) + SL * (st(n) - l(n));
-                    } else if (SR >= 0.0) {
-                        star_state(r, st, SR, Ss, d);
-                        for (const Spec& n : L.cons)
-                            A(fl, i, j, k, n) = n.var == ALPHA ? pflux(n, st, d) : pflux(n, r, d) + SR * (st(n) - r(n));
-                    } else {
-                        star_state(r, st, SR, Ss, d);
-                        for (const Spec& n : L.cons) A(fl, i, j, k, n) = n.var == ALPHA ? pflux(n, st, d) : pflux(n, r, d);
-                    }
-                }
+                for (int i = bx.lo[0]; i <= bx.hi[0] + ext[0]; ++i) hllc_face(b, d, i, j, k);
     }
 
-    // hllc.cpp:133-380
-    void calc_5wave(int b, int d) {
+    // hllc.cpp:133-380: one face (i,j,k) of box b; returns false on NaN wave
+    // speeds (the reference prints and exits).  SLT/SRT persist across the
+    // faces of a box as in the reference (function-scope variables).
+    bool wave5_face(int b, int d, int i, int j, int k, double& SLT, double& SRT) {
         Fab& fl = flux[d][b];
-        const IBox& bx = boxes[b];
         int ext[3] = {0, 0, 0};
         ext[d] = 1;
-        double SLT = 0.0, SRT = 0.0;
-        for (int k = bx.lo[2]; k <= bx.hi[2] + ext[2]; ++k)
-            for (int j = bx.lo[1]; j <= bx.hi[1] + ext[1]; ++j)
-                for (int i = bx.lo[0]; i <= bx.hi[0] + ext[0]; ++i) {
+        {
+            {
+                {
                     CellV l{this, &UR.f[b], i - ext[0], j - ext[1], k - ext[2]};
                     CellV r{this, &UL.f[b], i, j, k};
                     CellV ls{this, &ULStar.f[b], i, j, k};
@@ -871,7 +877,7 @@ class Solver {
                     double SR = dmax(std::fabs(l(VELOCITY, 0, d)) + l(SOUNDSPEED), std::fabs(r(VELOCITY, 0, d)) + r(SOUNDSPEED));
                     double SL = -SR;
                     double Ss = sstar(l, r, SL, SR, d);
-                    if (std::isnan(SL) || std::isnan(SR) || std::isnan(Ss)) { fatal |= 1; return; }
+                    if (std::isnan(SL) || std::isnan(SR) || std::isnan(Ss)) { fatal |= 1; return false; }
                     A(fl, i, j, k, USTAR) = Ss;
                     auto vstar_from = [&](CellV& c) {
                         for (int rr = 0; rr < 3; ++rr)
@@ -920,9 +926,22 @@ class Solver {
                     }
                     if (std::isnan(SL) || std::isnan(SR) || std::isnan(Ss) || std::isnan(SLT) || std::isnan(SRT)) {
                         fatal |= 1;
-                        return;
+                        return false;
                     }
                 }
+            }
+        }
+        return true;
+    }
+    void calc_5wave(int b, int d) {
+        const IBox& bx = boxes[b];
+        int ext[3] = {0, 0, 0};
+        ext[d] = 1;
+        double SLT = 0.0, SRT = 0.0;
+        for (int k = bx.lo[2]; k <= bx.hi[2] + ext[2]; ++k)
+            for (int j = bx.lo[1]; j <= bx.hi[1] + ext[1]; ++j)
+                for (int i = bx.lo[0]; i <= bx.hi[0] + ext[0]; ++i)
+                    if (!wave5_face(b, d, i, j, k, SLT, SRT)) return;
     }
 
     // hllc.cpp:493-523
@@ -954,7 +973,7 @@ class Solver {
     // ------------------------------------------------------------ stage / advance
     CellArray* U_in = nullptr;
     CellArray* U_out = nullptr;
-    int alpha_fatal = 0;
+    std::atomic<int> alpha_fatal{0};
 
     // hllc.cpp:579-695
     void hllc_advance(CellArray& Uin, CellArray& Uout, double dtv) {
@@ -964,6 +983,7 @@ class Solver {
         for (int d = 0; d < p.dim; ++d) {
             copy(UL, Uin);
             copy(UR, Uin);
+#pragma omp parallel for schedule(dynamic)
             for (size_t b = 0; b < boxes.size(); ++b) {
                 if (p.MUSCL) muscl((int)b, d);
                 if (p.THINC) {
@@ -984,12 +1004,14 @@ class Solver {
             }
             fill(UL);
             fill(UR);
+#pragma omp parallel for schedule(dynamic)
             for (size_t b = 0; b < boxes.size(); ++b) {
                 if (p.SOLID) calc_5wave((int)b, d);
                 else calc_fluxes((int)b, d);
                 update((int)b, d, dtv);
             }
         }
+#pragma omp parallel for schedule(dynamic)
         for (size_t b = 0; b < boxes.size(); ++b)
             for_valid((int)b, [&](int i, int j, int k) { c2p_cell(Uout.f[b], i, j, k); });
         fill(Uout);
@@ -999,6 +1021,7 @@ class Solver {
     void advance(double dtv) {
         hllc_advance(U, U1, dtv);
         hllc_advance(U1, U2, dtv);
+#pragma omp parallel for schedule(static)
         for (size_t b = 0; b < boxes.size(); ++b) {
             std::vector<double>& a = U.f[b].d;
             std::vector<double>& c = U2.f[b].d;
@@ -1016,6 +1039,7 @@ class Solver {
     // plastic.cpp:76-79, 83-244
     void plastic_update() {
         const double s3 = std::sqrt(3.0), s23 = std::sqrt(2.0 / 3.0);
+#pragma omp parallel for schedule(dynamic)
         for (size_t b = 0; b < boxes.size(); ++b) {
             Fab& f = U1.f[b];
             for_valid((int)b, [&](int i, int j, int k) {
@@ -1071,6 +1095,7 @@ class Solver {
     }
     // boxaccesscellarray.cpp:476-528
     void cleanup_v() {
+#pragma omp parallel for schedule(dynamic)
         for (size_t b = 0; b < boxes.size(); ++b) {
             Fab& f = U1.f[b];
             for_valid((int)b, [&](int i, int j, int k) {
@@ -1092,6 +1117,7 @@ class Solver {
         }
     }
     void cleanup_alpha_all(CellArray& ca) {
+#pragma omp parallel for schedule(dynamic)
         for (size_t b = 0; b < boxes.size(); ++b)
             for_valid((int)b, [&](int i, int j, int k) { alpha_fatal |= cleanup_alpha_cell(ca.f[b], i, j, k); });
     }
@@ -1111,6 +1137,7 @@ class Solver {
     // timestep.cpp:7-70
     double get_timestep(CellArray& ca) {
         double dtm = std::numeric_limits<double>::max();
+#pragma omp parallel for schedule(dynamic)
         for (size_t b = 0; b < boxes.size(); ++b) {
             Fab& f = ca.f[b];
             Fab& t = tsfab[b];

@@ -76,7 +76,8 @@ EXPORTS = [
     "tj_create", "tj_destroy", "tj_last_error", "tj_stream", "tj_bind_level", "tj_ncomp",
     "tj_comp_start", "tj_compute_dt", "tj_driver_dt", "tj_set_dt", "tj_get_clock", "tj_rk_stage",
     "tj_post_step", "tj_step", "tj_run", "tj_init_from_prims", "tj_c2p", "tj_derive", "tj_status",
-    "tj_reset_history", "tj_synchronize",
+    "tj_reset_history", "tj_synchronize", "tj_prof_enable", "tj_prof_report", "tj_set_tuning",
+    "tj_begin_run", "tj_advance_steps",
 ]
 
 _lib = None
@@ -116,6 +117,11 @@ def load_library(path: str = LIB_PATH):
     L.tj_status.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ip]
     L.tj_reset_history.argtypes = [vp]
     L.tj_synchronize.argtypes = [vp]
+    L.tj_prof_enable.argtypes = [vp, ip]
+    L.tj_prof_report.argtypes = [vp, ctypes.c_char_p, ip]
+    L.tj_set_tuning.argtypes = [vp, ip]
+    L.tj_begin_run.argtypes = [vp, mfp, dp]
+    L.tj_advance_steps.argtypes = [vp, mfp, mfp, ip, ip]
     _lib = L
     return L
 
@@ -244,6 +250,13 @@ class GpuLevel:
                                   check_every, ctypes.byref(t), ctypes.byref(n)), "tj_run")
         return n.value, t.value
 
+    def begin_run(self, final_time: float = float("inf")):
+        self._chk(self.lib.tj_begin_run(self.ctx, ctypes.byref(self.vS), final_time), "tj_begin_run")
+
+    def advance(self, nsteps: int, prec: int = TJ_PREC_FP64):
+        self._chk(self.lib.tj_advance_steps(self.ctx, ctypes.byref(self.vS), ctypes.byref(self.vT), nsteps, prec),
+                  "tj_advance_steps")
+
     def clock(self):
         t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         self._chk(self.lib.tj_get_clock(self.ctx, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "tj_get_clock")
@@ -253,6 +266,21 @@ class GpuLevel:
         s = ctypes.c_uint32()
         self._chk(self.lib.tj_status(self.ctx, ctypes.byref(s), clear), "tj_status")
         return s.value
+
+    def set_tuning(self, seglen: int):
+        self._chk(self.lib.tj_set_tuning(self.ctx, seglen), "tj_set_tuning")
+
+    def prof_enable(self, on: bool = True):
+        self._chk(self.lib.tj_prof_enable(self.ctx, int(on)), "tj_prof_enable")
+
+    def prof_report(self) -> dict:
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._chk(self.lib.tj_prof_report(self.ctx, buf, len(buf)), "tj_prof_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
 
     def derive(self):
         self._chk(self.lib.tj_derive(self.ctx, ctypes.byref(self.vS)), "tj_derive")

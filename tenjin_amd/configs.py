@@ -260,6 +260,35 @@ def shock_inclusion_2d(nx: int = 2048, ny: int = 1024, box: int = 64, seed: int 
     return pb
 
 
+def shock_inclusion_3d(n=(32, 24, 20), box: int = 8, drift=(80.0, 40.0, 20.0),
+                       bc_lo=(0, 1, 0), bc_hi=(0, 0, 1)) -> Problem:
+    """3D counterpart of the shock/inclusion case for the fluid path (no THINC,
+    which the reference defines in 1D/2D only): a Mach 1.22 shock in gas
+    approaching a light spherical inclusion, with a uniform drift."""
+    mats = [Material.stiffened(1.4, 0.0), Material.stiffened(1.67, 0.0)]
+    Lx = 1.0
+    h = Lx / n[0]
+    pb = Problem("shock_inclusion_3d", 3, tuple(n), (Lx, h * n[1], h * n[2]), box, mats,
+                 bc_lo=bc_lo, bc_hi=bc_hi)
+    U = _empty_prims(pb)
+    nz, ny, nx = pb.shape
+    x = (np.arange(nx) + 0.5) * h
+    y = (np.arange(ny) + 0.5) * h
+    z = (np.arange(nz) + 0.5) * h
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    inside = (X - 0.55) ** 2 + (Y - 0.5 * h * ny) ** 2 + (Z - 0.5 * h * nz) ** 2 < (0.25 * h * ny) ** 2
+    rho2, u2, p2 = rankine_hugoniot(1.4, 1.0, 1e5, 1.22)
+    shocked = X < 0.3
+    gas_rho = np.where(shocked, rho2, 1.0)
+    ux = np.where(shocked, u2, 0.0) + drift[0]
+    uy = np.zeros_like(X) + drift[1]
+    uz = np.zeros_like(X) + drift[2]
+    p = np.where(shocked, p2, 1e5)
+    _set_two_phase(pb, U, ~inside, gas_rho, 0.138, [ux, uy, uz], p)
+    pb.prims = U
+    return pb
+
+
 COPPER = dict(rho0=8930.0, Gamma0=2.0, c0=3940.0, s=1.49, G=45e9, Y=90e6)
 ALUMINIUM = dict(rho0=2703.0, Gamma0=2.0, c0=5328.0, s=1.338, G=26e9, Y=300e6)
 

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,7 +34,47 @@ struct tj_ctx {
     long long pass = 0;
     long long dpass[3] = {0, 0, 0};
     int seglen = 8;
+    // per-kernel event timing
+    bool prof = false;
+    struct Rec { const char* name; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<long long, double>> acc;
 };
+
+namespace {
+hipEvent_t ev_get(tj_ctx* c) {
+    if (!c->pool.empty()) { hipEvent_t e = c->pool.back(); c->pool.pop_back(); return e; }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+// brackets one launch with events when timing is on
+struct Timed {
+    tj_ctx* c;
+    const char* name;
+    hipEvent_t a{}, b{};
+    Timed(tj_ctx* c_, const char* n) : c(c_), name(n) {
+        if (c->prof) { a = ev_get(c); hipEventRecord(a, c->stream); }
+    }
+    ~Timed() {
+        if (c->prof) { b = ev_get(c); hipEventRecord(b, c->stream); c->pending.push_back({name, a, b}); }
+    }
+};
+void prof_drain(tj_ctx* c) {
+    for (auto& r : c->pending) {
+        hipEventSynchronize(r.b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        auto& e = c->acc[r.name];
+        e.first += 1;
+        e.second += ms;
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->pending.clear();
+}
+}  // namespace
 
 #define HIPCHK(ctx, call)                                                                   \
     do {                                                                                    \
@@ -151,6 +192,8 @@ void tj_destroy(tj_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     free_all(c);
+    prof_drain(c);
+    for (hipEvent_t e : c->pool) hipEventDestroy(e);
     if (c->d_clk) hipFree(c->d_clk);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -367,6 +410,8 @@ static int launch_fluxes(tj_ctx* c, const MF& Uin) {
             h.pstride = lv.ncells;
             h.estride = c->estride[d];
         }
+        static const char* names[3] = {"flux_x", "flux_y", "flux_z"};
+        Timed tm(c, names[d]);
         if (d == 0) {
             const int per = th ? 61 : 63;
             const long long nseg = (lv.n[0] + 1 + per - 1) / per;
@@ -441,14 +486,42 @@ int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
     const long long n = c->lv.ncells;
     const MF s = view(S), t = view(T);
     if ((r = launch_fluxes(c, s))) return r;
-    hipLaunchKernelGGL((k_update<2, 1>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, s, s, t, c->F[0], c->F[1],
-                       c->F[2], c->d_clk, 0u);
+    {
+        Timed tm(c, "update_stage1");
+        hipLaunchKernelGGL((k_update<2, 1>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, s, s, t, c->F[0], c->F[1],
+                           c->F[2], c->d_clk, 0u);
+    }
     if ((r = launch_fluxes(c, t))) return r;
     const unsigned ops = TJ_OP_CLEAN_ALPHA | TJ_OP_DT | TJ_OP_NANCHECK;
-    hipLaunchKernelGGL((k_update<2, 2>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, t, s, s, c->F[0], c->F[1],
-                       c->F[2], c->d_clk, ops);
+    {
+        Timed tm(c, "update_stage2_post");
+        hipLaunchKernelGGL((k_update<2, 2>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, t, s, s, c->F[0], c->F[1],
+                           c->F[2], c->d_clk, ops);
+    }
     hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, c->stream, c->d_clk);
     HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+int tj_begin_run(tj_ctx* c, const tj_mf* S, double final_time) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r;
+    if ((r = check_mf(c, S))) return r;
+    if ((r = tj_set_dt(c, 0.0, 1))) return r;
+    hipLaunchKernelGGL(k_clock_final, dim3(1), dim3(64), 0, c->stream, c->d_clk, final_time);
+    // getTimeStep of the initial state feeds the first driver dt
+    hipLaunchKernelGGL(k_sound_dt<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(S), c->d_clk);
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+int tj_advance_steps(tj_ctx* c, tj_mf* S, tj_mf* T, int nsteps, int prec) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r;
+    for (int q = 0; q < nsteps; ++q) {
+        hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 1);
+        if ((r = tj_step(c, S, T, prec))) return r;
+    }
     return TJ_OK;
 }
 
@@ -456,23 +529,18 @@ int tj_run(tj_ctx* c, tj_mf* S, tj_mf* T, double final_time, int max_steps, int 
            double* t_out, int* n_out) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
-    if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
-    if ((r = tj_set_dt(c, 0.0, 1))) return r;
-    // getTimeStep of the initial state feeds the first driver dt
-    hipLaunchKernelGGL(k_sound_dt<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(S), c->d_clk);
-    HIPCHK(c, hipGetLastError());
+    if ((r = check_mf(c, T))) return r;
+    if ((r = tj_begin_run(c, S, final_time))) return r;
     if (check_every < 1) check_every = 1;
     int steps = 0;
     while (steps < max_steps) {
-        if ((r = tj_driver_dt(c, final_time))) return r;
-        if ((r = tj_step(c, S, T, prec))) return r;
-        ++steps;
-        if (steps % check_every == 0 || steps == max_steps) {
-            Clock h;
-            HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            if (h.done) break;
-        }
+        const int chunk = std::min(check_every, max_steps - steps);
+        if ((r = tj_advance_steps(c, S, T, chunk, prec))) return r;
+        steps += chunk;
+        Clock h;
+        HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (h.done) break;
     }
     Clock h;
     HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
@@ -520,6 +588,35 @@ int tj_status(tj_ctx* c, uint32_t* status, int clear) {
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
+    return TJ_OK;
+}
+
+int tj_set_tuning(tj_ctx* c, int seglen) {
+    if (!c || seglen < 1) return TJ_EINVAL;
+    c->seglen = seglen;
+    return TJ_OK;
+}
+
+int tj_prof_enable(tj_ctx* c, int on) {
+    if (!c) return TJ_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_drain(c);
+    c->acc.clear();
+    c->prof = on != 0;
+    return TJ_OK;
+}
+
+int tj_prof_report(tj_ctx* c, char* buf, int len) {
+    if (!c || !buf || len < 1) return TJ_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_drain(c);
+    std::string out;
+    for (auto& kv : c->acc) {
+        char line[256];
+        snprintf(line, sizeof line, "%s %lld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+        out += line;
+    }
+    snprintf(buf, (size_t)len, "%s", out.c_str());
     return TJ_OK;
 }
 

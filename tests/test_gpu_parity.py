@@ -1,0 +1,155 @@
+"""GPU parity: the gfx950 library against the oracle, through the C ABI.
+
+The FP64 form keeps the reference's operation order, so the bar is bit-exact
+equality on the persistent state components (alpha, alphaRho, rho_k, rho,
+rhoU, u, E, p) and on dt; the written tolerance 1e-12 (relative max-norm per
+component) is the contract's and is asserted as well.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.parity_util import exact_fraction, field_errors, max_rel
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _pair(pb):
+    from oracle.oracle import Oracle
+    from tenjin_amd import GpuLevel
+
+    o = Oracle(pb)
+    g = GpuLevel(pb)
+    g.init_from_prims(pb.prims, pb.shift_pressure)
+    return o, g
+
+
+def _stepwise(pb, nsteps, exact=True):
+    print(f"\n[case] {pb.name} n={pb.n[:pb.dim]} box={pb.max_grid_size} thinc={pb.THINC} "
+          f"bc_lo={pb.bc_lo[:pb.dim]} bc_hi={pb.bc_hi[:pb.dim]} steps={nsteps}", flush=True)
+    o, g = _pair(pb)
+    so, sg = o.get(), g.get_state()
+    assert max_rel(pb, so, sg) <= TOL
+    for n in range(nsteps):
+        dto = o.timestep()
+        dtg = g.compute_dt()
+        assert dtg == dto, f"step {n}: dt {dtg!r} vs oracle {dto!r}"
+        dt = 0.2 * dto if n < 5 else dto
+        o.step(dt)
+        g.set_dt(dt, reset_clock=1)
+        g.step()
+        so, sg = o.get(), g.get_state()
+        err = max_rel(pb, so, sg)
+        assert err <= TOL, f"step {n}: {sorted(field_errors(pb, so, sg).items(), key=lambda kv: -kv[1])[:4]}"
+        if exact:
+            assert exact_fraction(pb, so, sg) == 1.0, f"step {n}: not bit-exact (max rel {err:.3e})"
+    assert g.status() == 0
+    return o, g
+
+
+def test_shock_tube_1d_stepwise():
+    from tenjin_amd import configs
+
+    _stepwise(configs.shock_tube_1d(), 60)
+
+
+def test_shock_tube_1d_boxes_reflective():
+    from tenjin_amd import configs
+
+    pb = configs.shock_tube_1d(n=300, max_grid_size=40)
+    pb.bc_lo = (1, 0, 0)
+    pb.bc_hi = (1, 0, 0)
+    pb.THINC = 1
+    _stepwise(pb, 40)
+
+
+def test_shock_tube_1d_full_run():
+    """The device-resident time loop (tj_run: dt, clamps, steps, NaN stop)
+    against the oracle's loop of main.cpp:121-191, to t = 2.4e-4 s."""
+    from tenjin_amd import configs
+
+    pb = configs.shock_tube_1d()
+    print(f"\n[case] {pb.name} full run to {pb.final_time}", flush=True)
+    o, g = _pair(pb)
+    n_o, t_o = o.run(pb.final_time)
+    n_g, t_g = g.run(pb.final_time, check_every=64)
+    assert (n_g, t_g) == (n_o, t_o)
+    so, sg = o.get(), g.get_state()
+    assert max_rel(pb, so, sg) <= TOL
+    assert exact_fraction(pb, so, sg) == 1.0
+
+
+@pytest.mark.parametrize("thinc", [0, 1])
+def test_inclusion_2d_stepwise(thinc):
+    from tenjin_amd import configs
+
+    pb = configs.shock_inclusion_2d(nx=128, ny=64, box=32, thinc=thinc, shock_x=0.55, drift=(150.0, 60.0))
+    _stepwise(pb, 30)
+
+
+def test_inclusion_2d_uneven_boxes():
+    """Box sizes from BoxArray::maxSize that do not divide the domain, THINC on,
+    transmissive/reflective mix."""
+    from tenjin_amd import configs
+
+    pb = configs.shock_inclusion_2d(nx=100, ny=60, box=24, thinc=1, shock_x=0.55, drift=(-120.0, 90.0))
+    pb.bc_lo = (1, 0, 0)
+    pb.bc_hi = (0, 1, 0)
+    _stepwise(pb, 25)
+
+
+def test_inclusion_3d_stepwise():
+    from tenjin_amd import configs
+
+    _stepwise(configs.shock_inclusion_3d(), 12)
+
+
+def test_inclusion_2d_full_size_two_steps():
+    """Full-size workload (2048 x 1024, 64^2 boxes, THINC): two steps from the
+    initial condition against the oracle."""
+    from tenjin_amd import configs
+
+    _stepwise(configs.shock_inclusion_2d(), 2)
+
+
+def test_inflow_balance_full_size():
+    """Size-independent property at the full size (2048 x 1024, 64^2 boxes,
+    THINC): over 40 device-resident steps the cell sums of each material's
+    mass and of the total energy change exactly by the post-shock inflow
+    through the x = 0 boundary (interior fluxes telescope)."""
+    from tenjin_amd import GpuLevel, configs
+    from tests.test_oracle import inflow_balance
+
+    pb = configs.shock_inclusion_2d()
+    print(f"\n[case] inflow balance {pb.n[:2]}", flush=True)
+    g = GpuLevel(pb)
+    g.init_from_prims(pb.prims)
+    s0 = g.get_state()
+    g.begin_run()
+    g.advance(40)
+    s1 = g.get_state()
+    t, _, n = g.clock()
+    assert n == 40
+    for got, expect, total in inflow_balance(pb, s0, s1, t):
+        assert abs(got - expect) <= 1e-13 * abs(total)
+    assert g.status() == 0
+
+
+def test_nan_stops_loop():
+    """A NaN planted in the state (from the host) raises the NaN status bit and
+    stops the device loop before t and n advance (main.cpp:183-187)."""
+    from tenjin_amd import TJ_ST_NAN, GpuLevel, configs
+    from tenjin_amd.configs import TOTAL_E
+
+    pb = configs.shock_tube_1d(n=200)
+    print("\n[case] NaN stop", flush=True)
+    g = GpuLevel(pb)
+    g.init_from_prims(pb.prims)
+    s = g.get_state()
+    s[pb.layout.comp(TOTAL_E), 0, 0, 50] = np.nan
+    g.set_state(s)
+    n, t = g.run(1.0, max_steps=10, check_every=1)
+    assert g.status() & TJ_ST_NAN
+    assert n == 0 and t == 0.0
