@@ -12,8 +12,9 @@ Stiffened gas maps onto the reference's MieGruneisenEOS as
 ``adiabaticIndex = gamma, pref = -gamma * p_inf, eref = 0``
 (equationofstate.cpp:37-64), so a physical pressure is set directly.
 
-Random draws use a 64-bit Mersenne twister with uniforms ``(x >> 11) * 2**-53``
-(numpy's MT19937 bit generator), which C++ ``std::mt19937_64`` reproduces.
+Random draws are 53-bit uniforms from the 32-bit Mersenne twister seeded with
+init_genrand(seed) (numpy's legacy RandomState): u = ((a >> 5) 2^26 + (b >> 6)) 2^-53
+from two consecutive outputs a, b, which C++ ``std::mt19937(seed)`` reproduces.
 """
 from __future__ import annotations
 
@@ -217,8 +218,7 @@ def rankine_hugoniot(gamma: float, rho1: float, p1: float, mach: float):
 
 
 def _mt_uniform(seed: int, count: int) -> np.ndarray:
-    bits = np.random.Generator(np.random.MT19937(seed)).bit_generator.random_raw(count)
-    return (np.asarray(bits, dtype=np.uint64) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return np.random.RandomState(seed).random_sample(count)
 
 
 def shock_inclusion_2d(nx: int = 2048, ny: int = 1024, box: int = 64, seed: int = 1,

@@ -10,6 +10,7 @@
 
 #include "../../include/tenjin.h"
 #include "tj_kernels.h"
+#include "tj_solid_kernels.h"
 
 using namespace tj;
 
@@ -212,8 +213,8 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
         c->err = "bind_level: this build instantiates two-material levels";
         return TJ_EUNSUPPORTED;
     }
-    if (d->SOLID) {
-        c->err = "bind_level: solid levels are not available in this build";
+    if (d->SOLID && d->THINC) {
+        c->err = "bind_level: THINC with SOLID is not available in this build";
         return TJ_EUNSUPPORTED;
     }
     if (d->THINC && d->dim == 3) {
@@ -280,7 +281,7 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
         HIPCHK(c, hipMemcpy(c->d_edge[a], fl.data(), fl.size(), hipMemcpyHostToDevice));
         lv.edge[a] = c->d_edge[a];
     }
-    c->nfc = 2 * d->nmat + 5;
+    c->nfc = d->SOLID ? 3 * d->nmat + 17 : 2 * d->nmat + 5;
     for (int a = 0; a < d->dim; ++a) {
         Faces& F = c->F[a];
         F.nx = lv.n[0] + (a == 0);
@@ -339,13 +340,41 @@ static int check_mf(tj_ctx* c, const tj_mf* m) {
     return TJ_OK;
 }
 
+static void launch_sound_dt(tj_ctx* c, const MF& U) {
+    const unsigned g = grid1(c->lv.ncells, 256);
+    if (c->lv.solid) hipLaunchKernelGGL(k_sound_dt_s<2>, dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
+    else hipLaunchKernelGGL(k_sound_dt<2>, dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
+}
+
+static void launch_update(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops) {
+    const unsigned g = grid1(c->lv.ncells, 256);
+    if (c->lv.solid) {
+        if (stage == 1)
+            hipLaunchKernelGGL((k_update_s<2, 1>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, c->F[0], c->F[1], c->F[2], c->d_clk, ops);
+        else
+            hipLaunchKernelGGL((k_update_s<2, 2>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, c->F[0], c->F[1], c->F[2], c->d_clk, ops);
+    } else {
+        if (stage == 1)
+            hipLaunchKernelGGL((k_update<2, 1>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, c->F[0], c->F[1], c->F[2], c->d_clk, ops);
+        else
+            hipLaunchKernelGGL((k_update<2, 2>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, c->F[0], c->F[1], c->F[2], c->d_clk, ops);
+    }
+}
+
+// post-step operations of the reference's loop body for the bound level
+static unsigned step_ops(const tj_ctx* c) {
+    unsigned ops = TJ_OP_CLEAN_ALPHA | TJ_OP_DT | TJ_OP_NANCHECK;
+    if (c->desc.PLASTIC) ops |= TJ_OP_PLASTIC;
+    if (c->desc.SOLID) ops |= TJ_OP_CLEAN_V;
+    return ops;
+}
+
 int tj_compute_dt(tj_ctx* c, const tj_mf* U, double* dt_out) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r = check_mf(c, U);
     if (r) return r;
     HIPCHK(c, hipSetDevice(c->device));
-    const long long n = c->lv.ncells;
-    hipLaunchKernelGGL(k_sound_dt<2>, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+    launch_sound_dt(c, view(U));
     hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 0);
     HIPCHK(c, hipGetLastError());
     if (dt_out) {
@@ -412,7 +441,22 @@ static int launch_fluxes(tj_ctx* c, const MF& Uin) {
         }
         static const char* names[3] = {"flux_x", "flux_y", "flux_z"};
         Timed tm(c, names[d]);
-        if (d == 0) {
+        if (lv.solid) {
+            if (d == 0) {
+                const long long nseg = (lv.n[0] + 1 + 62) / 63;
+                const unsigned g = grid1(nseg * lv.n[1] * lv.n[2], 4);
+                hipLaunchKernelGGL((k_flux_x_s<2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[0], c->d_clk);
+            } else {
+                const long long nstrip = (lv.n[0] + 63) / 64;
+                const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
+                const long long no = d == 1 ? lv.n[2] : lv.n[1];
+                const unsigned g = grid1(nstrip * nseg * no, 4);
+                if (d == 1)
+                    hipLaunchKernelGGL((k_flux_march_s<2, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], c->seglen, c->d_clk);
+                else
+                    hipLaunchKernelGGL((k_flux_march_s<2, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], c->seglen, c->d_clk);
+            }
+        } else if (d == 0) {
             const int per = th ? 61 : 63;
             const long long nseg = (lv.n[0] + 1 + per - 1) / per;
             const long long waves = nseg * lv.n[1] * lv.n[2];
@@ -449,14 +493,11 @@ int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_m
     HIPCHK(c, hipSetDevice(c->device));
     const MF in = view(Uin);
     if ((r = launch_fluxes(c, in))) return r;
-    const long long n = c->lv.ncells;
     const MF base = stage == 2 ? view(Ubase) : in;
-    if (stage == 1)
-        hipLaunchKernelGGL((k_update<2, 1>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, in, base, view(Uout),
-                           c->F[0], c->F[1], c->F[2], c->d_clk, 0u);
-    else
-        hipLaunchKernelGGL((k_update<2, 2>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, in, base, view(Uout),
-                           c->F[0], c->F[1], c->F[2], c->d_clk, 0u);
+    {
+        Timed tm(c, stage == 1 ? "update_stage1" : "update_stage2");
+        launch_update(c, stage, in, base, view(Uout), 0u);
+    }
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
 }
@@ -465,13 +506,19 @@ int tj_post_step(tj_ctx* c, tj_mf* U, unsigned ops) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
     if ((r = check_mf(c, U))) return r;
-    const long long n = c->lv.ncells;
-    if (ops & TJ_OP_CLEAN_ALPHA)
-        hipLaunchKernelGGL(k_cleanup_alpha<2>, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
-    if (ops & TJ_OP_NANCHECK)
-        hipLaunchKernelGGL(k_nancheck<2>, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
-    if (ops & TJ_OP_DT) {
-        hipLaunchKernelGGL(k_sound_dt<2>, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+    if ((ops & (TJ_OP_PLASTIC | TJ_OP_CLEAN_V)) && !c->lv.solid) {
+        c->err = "post_step: PLASTIC and CLEAN_V apply to solid levels";
+        return TJ_EINVAL;
+    }
+    const unsigned g = grid1(c->lv.ncells, 256);
+    if (c->lv.solid) {
+        hipLaunchKernelGGL(k_post_s<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk, ops);
+    } else {
+        if (ops & TJ_OP_CLEAN_ALPHA)
+            hipLaunchKernelGGL(k_cleanup_alpha<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+        if (ops & TJ_OP_NANCHECK)
+            hipLaunchKernelGGL(k_nancheck<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+        if (ops & TJ_OP_DT) launch_sound_dt(c, view(U));
     }
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
@@ -483,20 +530,16 @@ int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
     int r;
     if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
     HIPCHK(c, hipSetDevice(c->device));
-    const long long n = c->lv.ncells;
     const MF s = view(S), t = view(T);
     if ((r = launch_fluxes(c, s))) return r;
     {
         Timed tm(c, "update_stage1");
-        hipLaunchKernelGGL((k_update<2, 1>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, s, s, t, c->F[0], c->F[1],
-                           c->F[2], c->d_clk, 0u);
+        launch_update(c, 1, s, s, t, 0u);
     }
     if ((r = launch_fluxes(c, t))) return r;
-    const unsigned ops = TJ_OP_CLEAN_ALPHA | TJ_OP_DT | TJ_OP_NANCHECK;
     {
         Timed tm(c, "update_stage2_post");
-        hipLaunchKernelGGL((k_update<2, 2>), dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->lv, t, s, s, c->F[0], c->F[1],
-                           c->F[2], c->d_clk, ops);
+        launch_update(c, 2, t, s, s, step_ops(c));
     }
     hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, c->stream, c->d_clk);
     HIPCHK(c, hipGetLastError());
@@ -510,7 +553,7 @@ int tj_begin_run(tj_ctx* c, const tj_mf* S, double final_time) {
     if ((r = tj_set_dt(c, 0.0, 1))) return r;
     hipLaunchKernelGGL(k_clock_final, dim3(1), dim3(64), 0, c->stream, c->d_clk, final_time);
     // getTimeStep of the initial state feeds the first driver dt
-    hipLaunchKernelGGL(k_sound_dt<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(S), c->d_clk);
+    launch_sound_dt(c, view(S));
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
 }
@@ -554,7 +597,10 @@ int tj_init_from_prims(tj_ctx* c, tj_mf* U, int shift) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
     if ((r = check_mf(c, U))) return r;
-    hipLaunchKernelGGL(k_init_p2c<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
+    if (c->lv.solid)
+        hipLaunchKernelGGL(k_init_p2c_s<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
+    else
+        hipLaunchKernelGGL(k_init_p2c<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
 }
@@ -563,7 +609,10 @@ int tj_c2p(tj_ctx* c, tj_mf* U) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
     if ((r = check_mf(c, U))) return r;
-    hipLaunchKernelGGL(k_c2p<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+    if (c->lv.solid)
+        hipLaunchKernelGGL(k_c2p_s<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+    else
+        hipLaunchKernelGGL(k_c2p<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
 }
@@ -572,7 +621,10 @@ int tj_derive(tj_ctx* c, tj_mf* U) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
     if ((r = check_mf(c, U))) return r;
-    hipLaunchKernelGGL(k_derive_fluid, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+    if (c->lv.solid)
+        hipLaunchKernelGGL(k_derive_s<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+    else
+        hipLaunchKernelGGL(k_derive_fluid, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
 }

@@ -1,0 +1,515 @@
+// Kernels of the elastoplastic path (SOLID = 1): directional 5-wave flux
+// passes and the fused update / post-step kernel.
+#pragma once
+#include "tj_kernels.h"
+#include "tj_solid.h"
+
+namespace tj {
+
+template <int NM>
+__device__ __forceinline__ SSt<NM> shfl_state(const SSt<NM>& S, int src) {
+    SSt<NM> T;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        T.al[m] = __shfl(S.al[m], src, 64);
+        T.ar[m] = __shfl(S.ar[m], src, 64);
+        T.are[m] = __shfl(S.are[m], src, 64);
+    }
+    T.rho = __shfl(S.rho, src, 64);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        T.u[q] = __shfl(S.u[q], src, 64);
+        T.sig[q] = __shfl(S.sig[q], src, 64);
+    }
+    T.E = __shfl(S.E, src, 64);
+    T.p = __shfl(S.p, src, 64);
+    T.a = __shfl(S.a, src, 64);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) T.V[q] = __shfl(S.V[q], src, 64);
+    return T;
+}
+
+// The left image of a reconstructed state across a reflective face:
+// reflect_odd flips the normal velocity and momentum only (boundary.cpp:3-42).
+template <int NM>
+__device__ __forceinline__ void flip_s(SSt<NM>& S, int d) { S.u[d] = -S.u[d]; }
+
+// x pass of the solid path (same wave layout as k_flux_x, no THINC).
+template <int NM>
+__global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, Faces F, Clock* clk) {
+    if (clk->done) return;
+    constexpr int PER = 63;
+    constexpr int NF = 3 * NM + 17;
+    const int lane = threadIdx.x & 63;
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const int nseg = (nx + 1 + PER - 1) / PER;
+    if (wave >= (long long)nseg * ny * nz) return;
+    const int seg = (int)(wave % nseg);
+    const long long row = wave / nseg;
+    const int j = (int)(row % ny), k = (int)(row / ny);
+    const int c = seg * PER - 1 + lane;
+    bool fl;
+    const int s = bcmap(c, nx, lv.bc_lo[0], lv.bc_hi[0], fl);
+    SSt<NM> SLs, SRs;
+    {
+        SPrim<NM> Pm, Pc, Pp, PL, PR;
+        load_sprim<NM>(lv, U, s - 1, j, k, Pm);
+        load_sprim<NM>(lv, U, s, j, k, Pc);
+        load_sprim<NM>(lv, U, s + 1, j, k, Pp);
+        muscl_s<NM>(Pm, Pc, Pp, PL, PR);
+        p2c_sound_s<NM>(lv, PL, 0, SLs);
+        p2c_sound_s<NM>(lv, PR, 0, SRs);
+    }
+    SSt<NM> Lf = shfl_state<NM>(SRs, lane > 0 ? lane - 1 : 0);
+    unsigned st = 0;
+    if (lane >= 1 && c >= 0 && c <= nx) {
+        if (c == 0) {
+            Lf = SRs;
+            if (lv.bc_lo[0]) flip_s<NM>(Lf, 0);
+        }
+        SSt<NM> Rf = SLs;
+        if (c == nx && lv.bc_hi[0]) flip_s<NM>(Rf, 0);
+        double Fv[NF];
+        wave5<NM>(lv, Lf, Rf, 0, Fv, st);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) F.at(q, c, j, k) = Fv[q];
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
+// y / z pass of the solid path: lanes along x, marching along D.
+template <int NM, int D>
+__global__ __launch_bounds__(256) void k_flux_march_s(Level lv, MF U, Faces F, int seglen, Clock* clk) {
+    if (clk->done) return;
+    constexpr int NF = 3 * NM + 17;
+    const int lane = threadIdx.x & 63;
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const int nd = lv.n[D];
+    const int no = (D == 1) ? nz : ny;
+    const int nstrip = (nx + 63) / 64;
+    const int nseg = (nd + 1 + seglen - 1) / seglen;
+    if (wave >= (long long)nstrip * nseg * no) return;
+    const int strip = (int)(wave % nstrip);
+    const long long rest = wave / nstrip;
+    const int seg = (int)(rest % nseg);
+    const int o = (int)(rest / nseg);
+    const int i0 = strip * 64 + lane;
+    const bool act = i0 < nx;
+    const int i = act ? i0 : nx - 1;
+    const int f0 = seg * seglen;
+    const int f1 = min(f0 + seglen, nd + 1);
+    const int rb = max(f0 - 1, 0), re = min(f1 - 1, nd - 1);
+    auto cj = [&](int r) { return D == 1 ? r : o; };
+    auto ck = [&](int r) { return D == 2 ? r : o; };
+    auto faceidx = [&](int r) {
+        return D == 1 ? ((long long)ck(r) * (ny + 1) + r) * nx + i : ((long long)r * ny + cj(r)) * nx + i;
+    };
+    SPrim<NM> Pm, Pc, Pp;
+    load_sprim<NM>(lv, U, i, cj(rb - 1), ck(rb - 1), Pm);
+    load_sprim<NM>(lv, U, i, cj(rb), ck(rb), Pc);
+    SSt<NM> prevR;
+    unsigned st = 0;
+    for (int r = rb; r <= re; ++r) {
+        load_sprim<NM>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
+        SSt<NM> SLs, SRs;
+        {
+            SPrim<NM> PL, PR;
+            muscl_s<NM>(Pm, Pc, Pp, PL, PR);
+            p2c_sound_s<NM>(lv, PL, D, SLs);
+            p2c_sound_s<NM>(lv, PR, D, SRs);
+        }
+        auto emit = [&](int face, const SSt<NM>& A, const SSt<NM>& B) {
+            double Fv[NF];
+            wave5<NM>(lv, A, B, D, Fv, st);
+            if (!act) return;
+            const long long fi = faceidx(face);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) F.p[q * F.cs + fi] = Fv[q];
+        };
+        if (r == 0 && f0 == 0) {
+            SSt<NM> img = SRs;
+            if (lv.bc_lo[D]) flip_s<NM>(img, D);
+            emit(0, img, SLs);
+        } else if (r >= f0) {
+            emit(r, prevR, SLs);
+        }
+        if (r == nd - 1 && f1 == nd + 1) {
+            SSt<NM> img = SLs;
+            if (lv.bc_hi[D]) flip_s<NM>(img, D);
+            emit(nd, SRs, img);
+        }
+        prevR = SRs;
+        Pm = Pc;
+        Pc = Pp;
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
+// Per-cell solid state held by the update kernel.
+template <int NM>
+struct SCell {
+    double al[NM], ar[NM], rk[NM], e[NM], are[NM], rho, ru[3], u[3], E, p, V[9], hj2;
+};
+
+template <int NM>
+__device__ __forceinline__ void load_scell(const Level& lv, const MF& U, int i, int j, int k, SCell<NM>& C) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        C.al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+        C.ar[m] = U.at(lv.cs[ALPHARHO] + m, i, j, k);
+        C.rk[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
+        C.e[m] = lv.plastic ? U.at(lv.cs[EPSILON] + m, i, j, k) : 0.0;
+        C.are[m] = lv.plastic ? U.at(lv.cs[ALPHARHOEPSILON] + m, i, j, k) : 0.0;
+    }
+    C.rho = U.at(lv.cs[RHO], i, j, k);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        C.ru[q] = U.at(lv.cs[RHOU] + q, i, j, k);
+        C.u[q] = U.at(lv.cs[VELOCITY] + q, i, j, k);
+    }
+    C.E = U.at(lv.cs[TOTAL_E], i, j, k);
+    C.p = U.at(lv.cs[PRESSURE], i, j, k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) C.V[q] = U.at(lv.cs[V_TENSOR] + q, i, j, k);
+    C.hj2 = U.at(lv.cs[HJ2], i, j, k);
+}
+template <int NM>
+__device__ __forceinline__ void store_scell(const Level& lv, const MF& U, int i, int j, int k, const SCell<NM>& C) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        U.at(lv.cs[ALPHA] + m, i, j, k) = C.al[m];
+        U.at(lv.cs[ALPHARHO] + m, i, j, k) = C.ar[m];
+        U.at(lv.cs[RHO_K] + m, i, j, k) = C.rk[m];
+        if (lv.plastic) {
+            U.at(lv.cs[EPSILON] + m, i, j, k) = C.e[m];
+            U.at(lv.cs[ALPHARHOEPSILON] + m, i, j, k) = C.are[m];
+        }
+    }
+    U.at(lv.cs[RHO], i, j, k) = C.rho;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        U.at(lv.cs[RHOU] + q, i, j, k) = C.ru[q];
+        U.at(lv.cs[VELOCITY] + q, i, j, k) = C.u[q];
+    }
+    U.at(lv.cs[TOTAL_E], i, j, k) = C.E;
+    U.at(lv.cs[PRESSURE], i, j, k) = C.p;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) U.at(lv.cs[V_TENSOR] + q, i, j, k) = C.V[q];
+    U.at(lv.cs[HJ2], i, j, k) = C.hj2;
+}
+
+// update (hllc.cpp:493-523, with the V rows of :511-514) in x, y, z order,
+// then conservativeToPrimitive.  Stage 2 adds the RK average (hllc.cpp:705),
+// plasticUpdate (main.cpp:154-157, plastic.cpp:227-244), cleanUpV
+// (main.cpp:164-167), cleanUpAlpha (main.cpp:169-171), the NaN scan and the
+// next getTimeStep.
+template <int NM, int STAGE>
+__global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo, Faces F0, Faces F1, Faces F2, Clock* clk,
+                                                  unsigned ops) {
+    if (clk->done) return;
+    constexpr int NC = 3 * NM + 13;
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    double cand = __builtin_huge_val();
+    unsigned st = 0;
+    if (id < ncell) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        const double dt = clk->dt;
+        SCell<NM> C;
+        load_scell<NM>(lv, Uin, i, j, k, C);
+        double a0[NM], V0[9], u0[3];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) a0[m] = C.al[m];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) V0[q] = C.V[q];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) u0[q] = C.u[q];
+        for (int d = 0; d < lv.dim; ++d) {
+            const Faces& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
+            const int i1 = i + (d == 0), j1 = j + (d == 1), k1 = k + (d == 2);
+            const double dtdx = dt / lv.dx[d];
+            const double s0 = Fd.at(NC, i, j, k), s1 = Fd.at(NC, i1, j1, k1);
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                C.al[m] += dtdx * (Fd.at(m, i, j, k) - Fd.at(m, i1, j1, k1) - a0[m] * (s0 - s1));
+                C.ar[m] += dtdx * (Fd.at(NM + m, i, j, k) - Fd.at(NM + m, i1, j1, k1));
+                if (lv.plastic)
+                    C.are[m] += dtdx * (Fd.at(2 * NM + 13 + m, i, j, k) - Fd.at(2 * NM + 13 + m, i1, j1, k1));
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) C.ru[q] += dtdx * (Fd.at(2 * NM + q, i, j, k) - Fd.at(2 * NM + q, i1, j1, k1));
+            C.E += dtdx * (Fd.at(2 * NM + 3, i, j, k) - Fd.at(2 * NM + 3, i1, j1, k1));
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int n = 2 * NM + 4 + r * 3 + c;
+                    const double vs0 = Fd.at(NC + 1 + c, i, j, k), vs1 = Fd.at(NC + 1 + c, i1, j1, k1);
+                    C.V[r * 3 + c] += dtdx * (Fd.at(n, i, j, k) - Fd.at(n, i1, j1, k1) - (2.0 / 3.0) * V0[r * 3 + c] * (s0 - s1) +
+                                              u0[r] * (vs0 - vs1));
+                }
+        }
+        double dH[9];
+        C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+        if (STAGE == 2) {
+            SCell<NM> B;
+            load_scell<NM>(lv, Ub, i, j, k, B);
+            // U1 = (U * 1/2) + (U2 * 1/2) over every component
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                C.al[m] = B.al[m] * (1.0 / 2.0) + C.al[m] * (1.0 / 2.0);
+                C.ar[m] = B.ar[m] * (1.0 / 2.0) + C.ar[m] * (1.0 / 2.0);
+                C.rk[m] = B.rk[m] * (1.0 / 2.0) + C.rk[m] * (1.0 / 2.0);
+                C.e[m] = B.e[m] * (1.0 / 2.0) + C.e[m] * (1.0 / 2.0);
+                C.are[m] = B.are[m] * (1.0 / 2.0) + C.are[m] * (1.0 / 2.0);
+            }
+            C.rho = B.rho * (1.0 / 2.0) + C.rho * (1.0 / 2.0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                C.ru[q] = B.ru[q] * (1.0 / 2.0) + C.ru[q] * (1.0 / 2.0);
+                C.u[q] = B.u[q] * (1.0 / 2.0) + C.u[q] * (1.0 / 2.0);
+            }
+            C.E = B.E * (1.0 / 2.0) + C.E * (1.0 / 2.0);
+            C.p = B.p * (1.0 / 2.0) + C.p * (1.0 / 2.0);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) C.V[q] = B.V[q] * (1.0 / 2.0) + C.V[q] * (1.0 / 2.0);
+            C.hj2 = B.hj2 * (1.0 / 2.0) + C.hj2 * (1.0 / 2.0);
+            if (ops & 1u) {  // plastic relaxation, then conservativeToPrimitive
+                st |= plastic_cell<NM>(lv, C.al, C.ar, C.e, C.are, C.V);
+                C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+            }
+            if (ops & 2u) cleanup_v<NM>(lv, C.al, C.V);
+            if (ops & 4u) st |= cleanup_alpha<NM>(C.al);
+            if (ops & 16u) {
+                bool bad = false;
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+                    bad |= __builtin_isnan(C.al[m]) || __builtin_isnan(C.ar[m]) || __builtin_isnan(C.rk[m]) ||
+                           (lv.plastic && (__builtin_isnan(C.e[m]) || __builtin_isnan(C.are[m])));
+#pragma unroll
+                for (int q = 0; q < 3; ++q) bad |= __builtin_isnan(C.ru[q]) || __builtin_isnan(C.u[q]);
+#pragma unroll
+                for (int q = 0; q < 9; ++q) bad |= __builtin_isnan(C.V[q]);
+                bad |= __builtin_isnan(C.E) || __builtin_isnan(C.p);
+                if (bad) st |= 1u;
+            }
+            if (ops & 8u) {
+                const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
+                Uo.at(lv.cs[SOUNDSPEED], i, j, k) = a;
+                for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
+            }
+        }
+        store_scell<NM>(lv, Uo, i, j, k, C);
+    }
+    if (STAGE == 2 && (ops & 8u)) {
+        unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(b, off, 64);
+            b = o < b ? o : b;
+        }
+        __shared__ unsigned long long red[4];
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long mn = red[0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
+            atomicMin(&clk->dtmin_bits, mn);
+        }
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void k_sound_dt_s(Level lv, MF U, Clock* clk) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    double cand = __builtin_huge_val();
+    if (id < ncell) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        SCell<NM> C;
+        load_scell<NM>(lv, U, i, j, k, C);
+        const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
+        U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
+        for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
+    }
+    unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o < b ? o : b;
+    }
+    __shared__ unsigned long long red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long mn = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
+        atomicMin(&clk->dtmin_bits, mn);
+    }
+}
+
+// setInitialConditions for solid levels: thermal -> total pressure shift
+// (initialise.cpp:104, with the HJ2 component as given) then
+// primitiveToConservative (which computes HJ2 and devH).
+template <int NM>
+__global__ __launch_bounds__(256) void k_init_p2c_s(Level lv, MF U, int shift) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    SPrim<NM> P;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        P.a[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+        P.r[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
+        P.e[m] = lv.plastic ? U.at(lv.cs[EPSILON] + m, i, j, k) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) P.u[q] = U.at(lv.cs[VELOCITY] + q, i, j, k);
+    P.p = U.at(lv.cs[PRESSURE], i, j, k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) P.V[q] = U.at(lv.cs[V_TENSOR] + q, i, j, k);
+    if (shift) {
+        double ar[NM], G, Ent, Pnt;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) ar[m] = P.a[m] * P.r[m];
+        eos_sums_s<NM>(lv, P.a, ar, P.r, U.at(lv.cs[HJ2], i, j, k), G, Ent, Pnt);
+        P.p += Pnt / G;
+        U.at(lv.cs[PRESSURE], i, j, k) = P.p;
+    }
+    SSt<NM> S;
+    p2c_sound_s<NM>(lv, P, 0, S);
+    double dH[9];
+    const double hj2 = hencky(P.V, dH);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        U.at(lv.cs[ALPHARHO] + m, i, j, k) = S.ar[m];
+        if (lv.plastic) U.at(lv.cs[ALPHARHOEPSILON] + m, i, j, k) = S.are[m];
+    }
+    U.at(lv.cs[RHO], i, j, k) = S.rho;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) U.at(lv.cs[RHOU] + q, i, j, k) = S.u[q] * S.rho;
+    U.at(lv.cs[TOTAL_E], i, j, k) = S.E;
+    U.at(lv.cs[HJ2], i, j, k) = hj2;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) U.at(lv.cs[DEVH] + q, i, j, k) = dH[q];
+}
+
+// derived components of a solid state: devH, HJ2 from V and sigma
+// (boxaccesscellarray.cpp:176-216, 326-380)
+template <int NM>
+__global__ __launch_bounds__(256) void k_derive_s(Level lv, MF U) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    double V[9], dH[9], al[NM];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) V[q] = U.at(lv.cs[V_TENSOR] + q, i, j, k);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+    const double hj2 = hencky(V, dH);
+    const double p = U.at(lv.cs[PRESSURE], i, j, k);
+    U.at(lv.cs[HJ2], i, j, k) = hj2;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) U.at(lv.cs[DEVH] + q, i, j, k) = dH[q];
+    for (int d = 0; d < 3; ++d) {
+        double sig[3];
+        stress_col<NM>(lv, al, p, dH, d, sig);
+        for (int r = 0; r < 3; ++r) U.at(lv.cs[SIGMA] + 3 * r + d, i, j, k) = sig[r];
+    }
+}
+
+}  // namespace tj
+
+namespace tj {
+
+// conservativeToPrimitive over the valid cells of a solid level.
+template <int NM>
+__global__ __launch_bounds__(256) void k_c2p_s(Level lv, MF U) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    SCell<NM> C;
+    load_scell<NM>(lv, U, i, j, k, C);
+    double dH[9];
+    C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+    store_scell<NM>(lv, U, i, j, k, C);
+}
+
+// The post-step chain of main.cpp:154-171 on a state, stand-alone
+// (tj_post_step): plastic + c2p, cleanUpV, cleanUpAlpha, NaN scan, sound speed
+// and the CFL minimum, each when its bit is set.
+template <int NM>
+__global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsigned ops) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    double cand = __builtin_huge_val();
+    unsigned st = 0;
+    if (id < ncell) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        SCell<NM> C;
+        load_scell<NM>(lv, U, i, j, k, C);
+        double dH[9];
+        if (ops & 1u) {
+            st |= plastic_cell<NM>(lv, C.al, C.ar, C.e, C.are, C.V);
+            C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+        }
+        if (ops & 2u) cleanup_v<NM>(lv, C.al, C.V);
+        if (ops & 4u) st |= cleanup_alpha<NM>(C.al);
+        if (ops & 16u) {
+            bool bad = false;
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                bad |= __builtin_isnan(C.al[m]) || __builtin_isnan(C.ar[m]) || __builtin_isnan(C.rk[m]) ||
+                       (lv.plastic && (__builtin_isnan(C.e[m]) || __builtin_isnan(C.are[m])));
+#pragma unroll
+            for (int q = 0; q < 3; ++q) bad |= __builtin_isnan(C.ru[q]) || __builtin_isnan(C.u[q]);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) bad |= __builtin_isnan(C.V[q]);
+            bad |= __builtin_isnan(C.E) || __builtin_isnan(C.p);
+            if (bad) st |= 1u;
+        }
+        if (ops & 8u) {
+            const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
+            U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
+            for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
+        }
+        store_scell<NM>(lv, U, i, j, k, C);
+    }
+    if (ops & 8u) {
+        unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(b, off, 64);
+            b = o < b ? o : b;
+        }
+        __shared__ unsigned long long red[4];
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long mn = red[0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
+            atomicMin(&clk->dtmin_bits, mn);
+        }
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
+}  // namespace tj

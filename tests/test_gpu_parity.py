@@ -153,3 +153,38 @@ def test_nan_stops_loop():
     n, t = g.run(1.0, max_steps=10, check_every=1)
     assert g.status() & TJ_ST_NAN
     assert n == 0 and t == 0.0
+
+
+@pytest.mark.parametrize("dim,n,box,steps", [(2, 48, 24, 12), (3, 20, 10, 6)])
+def test_plate_impact_stepwise(dim, n, box, steps):
+    """Elastoplastic path (Us-up EOS, 5-wave flux, radial return, cleanUpV)."""
+    from tenjin_amd import configs
+
+    _stepwise(configs.plate_impact(dim=dim, n=n, box=box), steps)
+
+
+def test_plate_impact_3d_full_size_invariants():
+    """256^3 elastoplastic impact, 3 device-resident steps: no status bit,
+    det V = 1 after cleanUpV, plastic strain non-negative, mass positive."""
+    import torch
+
+    from tenjin_amd import GpuLevel, configs
+    from tenjin_amd.configs import ALPHARHO, EPSILON, V_TENSOR
+
+    pb = configs.plate_impact()
+    print(f"\n[case] plate impact {pb.n} invariants", flush=True)
+    g = GpuLevel(pb)
+    g.init_from_prims(pb.prims, pb.shift_pressure)
+    g.begin_run()
+    g.advance(3)
+    g.sync()
+    assert g.status() == 0
+    L = pb.layout
+    v0 = L.start[V_TENSOR]
+    V = g.S[v0:v0 + 9].reshape(3, 3, -1).permute(2, 0, 1)
+    det = torch.linalg.det(V)
+    assert float((det - 1.0).abs().max()) < 1e-12
+    assert float(g.S[L.comp(EPSILON, 0)].min()) >= 0.0
+    assert float(g.S[L.comp(ALPHARHO, 0)].min()) > 0.0
+    _, _, n = g.clock()
+    assert n == 3
