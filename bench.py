@@ -34,6 +34,10 @@ KERNEL_BYTES_PER_CELL = {
     "flux_z": 64 + 72,
     "update_stage1": 64 + 2 * 72 + 120,          # 8 conserved in, 2 faces x 9 comps, 15 comps out
     "update_stage2_post": 64 + 2 * 72 + 120 + 120 + 8,  # + 15 base comps in, + sound speed out
+    # y pass with the update folded in: 8 primitives + 6 further conserved
+    # comps in, x faces in (9 comps), star alphas out, 15 comps out
+    "flux_y_update1": 112 + 72 + 16 + 120,
+    "flux_y_update2_post": 112 + 72 + 16 + 120 + 120 + 8,  # + base in, + sound speed out
 }
 STEP_BYTES_PER_CELL_UPDATE = 464  # 8 * (3 * N_s + 2 * N_c), N_s = 14, N_c = 8
 
@@ -47,7 +51,9 @@ def parse():
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--box", type=int, default=64)
-    ap.add_argument("--seglen", type=int, default=8)
+    ap.add_argument("--seglen", type=int, default=16)
+    ap.add_argument("--fuse", action="store_true",
+                    help="run the update inside the y pass instead of as a kernel of its own")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calib-copy", action="store_true",
@@ -114,7 +120,7 @@ def main():
     pb = configs.shock_inclusion_2d(nx=args.nx, ny=args.ny, box=args.box)
     prec = TJ_PREC_FP64 if args.prec == "fp64" else TJ_PREC_MIXED
     g = GpuLevel(pb)
-    g.set_tuning(args.seglen)
+    g.set_tuning(args.seglen, 1 if args.fuse else 0)
     g.init_from_prims(pb.prims, pb.shift_pressure)
     g.begin_run(float("inf"))
     g.advance(args.warmup, prec)
@@ -173,6 +179,8 @@ def main():
                         "MUSCL+THINC(beta 1.6)+HLLC, CFL 0.5",
             "cells": ncell,
             "precision": args.prec,
+            "fused_update": bool(args.fuse),
+            "seglen": args.seglen,
             "parallelism": "single GPU",
         },
         "roofline": roofline,

@@ -109,8 +109,13 @@ void* tj_stream(tj_ctx* ctx);
  * dx = L/n.  Allocates the scratch (flux planes, THINC history planes).
  * Replaces the set-up of main.cpp:29-95. */
 int tj_bind_level(tj_ctx* ctx, const tj_level_desc* desc);
-/* Rows per marching segment of the y/z flux passes (tuning; default 8). */
-int tj_set_tuning(tj_ctx* ctx, int seglen);
+/* Tuning: rows per marching segment of the y/z flux passes (default 16),
+ * and whether the update of a fluid level with dim >= 2 runs inside its last
+ * directional pass (fuse = 1) or as a kernel of its own (fuse = 0, default:
+ * faster on gfx950 while the fused pass is limited to one wave per SIMD by
+ * its register use).  Both give identical results; a value < 0 leaves that
+ * setting unchanged. */
+int tj_set_tuning(tj_ctx* ctx, int seglen, int fuse);
 /* Number of components of the bound layout and the first component of a
  * reference Variable (structdefinitions.h:10-32 numbering); -1 if absent. */
 int tj_ncomp(const tj_ctx* ctx);

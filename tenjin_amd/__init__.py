@@ -90,6 +90,9 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with `python tenjin_amd/build.py`")
+    # torch ships its own HIP runtime; loading it first makes the library bind
+    # to that same runtime (two HIP runtimes in one process cannot share a device)
+    import torch  # noqa: F401
     L = ctypes.CDLL(path)
     vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     mfp = ctypes.POINTER(MFView)
@@ -119,7 +122,7 @@ def load_library(path: str = LIB_PATH):
     L.tj_synchronize.argtypes = [vp]
     L.tj_prof_enable.argtypes = [vp, ip]
     L.tj_prof_report.argtypes = [vp, ctypes.c_char_p, ip]
-    L.tj_set_tuning.argtypes = [vp, ip]
+    L.tj_set_tuning.argtypes = [vp, ip, ip]
     L.tj_begin_run.argtypes = [vp, mfp, dp]
     L.tj_advance_steps.argtypes = [vp, mfp, mfp, ip, ip]
     _lib = L
@@ -267,8 +270,10 @@ class GpuLevel:
         self._chk(self.lib.tj_status(self.ctx, ctypes.byref(s), clear), "tj_status")
         return s.value
 
-    def set_tuning(self, seglen: int):
-        self._chk(self.lib.tj_set_tuning(self.ctx, seglen), "tj_set_tuning")
+    def set_tuning(self, seglen: int = -1, fuse: int = -1):
+        """Rows per marching segment (-1: unchanged) and fusion of the update
+        into the last directional pass (1 on, 0 off, -1 unchanged)."""
+        self._chk(self.lib.tj_set_tuning(self.ctx, seglen, fuse), "tj_set_tuning")
 
     def prof_enable(self, on: bool = True):
         self._chk(self.lib.tj_prof_enable(self.ctx, int(on)), "tj_prof_enable")

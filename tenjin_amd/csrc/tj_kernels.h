@@ -142,13 +142,137 @@ __global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, Faces F, Hist h,
     if (st) atomicOr(&clk->status, st);
 }
 
+// ------------------------------------------------------------------ update helpers
+// One direction of update (hllc.cpp:493-523): U1 += dt/dx (F_lo - F_hi), the
+// alpha rows with the non-conservative term -alpha_U (S*_lo - S*_hi).
+template <int NM>
+__device__ __forceinline__ void accum_dir(double* al, double* ar, double* ru, double& E, const double* a0, double dtdx,
+                                          const double* Flo, const double* Fhi) {
+    const double s0 = Flo[2 * NM + 4], s1 = Fhi[2 * NM + 4];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) al[m] += dtdx * (Flo[m] - Fhi[m] - a0[m] * (s0 - s1));
+#pragma unroll
+    for (int m = 0; m < NM; ++m) ar[m] += dtdx * (Flo[NM + m] - Fhi[NM + m]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) ru[q] += dtdx * (Flo[2 * NM + q] - Fhi[2 * NM + q]);
+    E += dtdx * (Flo[2 * NM + 3] - Fhi[2 * NM + 3]);
+}
+template <int NM>
+__device__ __forceinline__ void load_faces(const Faces& Fd, int i, int j, int k, double* Fv) {
+#pragma unroll
+    for (int q = 0; q < 2 * NM + 5; ++q) Fv[q] = Fd.at(q, i, j, k);
+}
+template <int NM>
+__device__ __forceinline__ void load_cons(const Level& lv, const MF& U, int i, int j, int k, double* al, double* ar,
+                                          double* ru, double& E) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+        ar[m] = U.at(lv.cs[ALPHARHO] + m, i, j, k);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) ru[q] = U.at(lv.cs[RHOU] + q, i, j, k);
+    E = U.at(lv.cs[TOTAL_E], i, j, k);
+}
+// conservativeToPrimitive of the updated cell (hllc.cpp:689), and in stage 2
+// the RK average (hllc.cpp:705), cleanUpAlpha (main.cpp:169-171), the NaN scan
+// (main.cpp:183-187) and the sound speed / CFL candidate of the next
+// getTimeStep (main.cpp:128); then the store.
+template <int NM, int STAGE>
+__device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const MF& Uo, int i, int j, int k, double* al,
+                                            double* ar, double* ru, double E, unsigned ops, double& cand, unsigned& st) {
+    double rk[NM], rho, u[3], p;
+    c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+    if (STAGE == 2) {
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            al[m] = Ub.at(lv.cs[ALPHA] + m, i, j, k) * (1.0 / 2.0) + al[m] * (1.0 / 2.0);
+            ar[m] = Ub.at(lv.cs[ALPHARHO] + m, i, j, k) * (1.0 / 2.0) + ar[m] * (1.0 / 2.0);
+            rk[m] = Ub.at(lv.cs[RHO_K] + m, i, j, k) * (1.0 / 2.0) + rk[m] * (1.0 / 2.0);
+        }
+        rho = Ub.at(lv.cs[RHO], i, j, k) * (1.0 / 2.0) + rho * (1.0 / 2.0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            ru[q] = Ub.at(lv.cs[RHOU] + q, i, j, k) * (1.0 / 2.0) + ru[q] * (1.0 / 2.0);
+            u[q] = Ub.at(lv.cs[VELOCITY] + q, i, j, k) * (1.0 / 2.0) + u[q] * (1.0 / 2.0);
+        }
+        E = Ub.at(lv.cs[TOTAL_E], i, j, k) * (1.0 / 2.0) + E * (1.0 / 2.0);
+        p = Ub.at(lv.cs[PRESSURE], i, j, k) * (1.0 / 2.0) + p * (1.0 / 2.0);
+        if (ops & 4u) st |= cleanup_alpha<NM>(al);
+        if (ops & 16u) {
+            bool bad = false;
+#pragma unroll
+            for (int m = 0; m < NM; ++m) bad |= __builtin_isnan(al[m]) || __builtin_isnan(ar[m]) || __builtin_isnan(rk[m]);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) bad |= __builtin_isnan(ru[q]) || __builtin_isnan(u[q]);
+            bad |= __builtin_isnan(E) || __builtin_isnan(p);
+            if (bad) st |= 1u;
+        }
+        if (ops & 8u) {
+            const double a = sound<NM>(lv, al, ar, rk, rho, p);
+            Uo.at(lv.cs[SOUNDSPEED], i, j, k) = a;
+            for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(u[d])));
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        Uo.at(lv.cs[ALPHA] + m, i, j, k) = al[m];
+        Uo.at(lv.cs[ALPHARHO] + m, i, j, k) = ar[m];
+        Uo.at(lv.cs[RHO_K] + m, i, j, k) = rk[m];
+    }
+    Uo.at(lv.cs[RHO], i, j, k) = rho;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        Uo.at(lv.cs[RHOU] + q, i, j, k) = ru[q];
+        Uo.at(lv.cs[VELOCITY] + q, i, j, k) = u[q];
+    }
+    Uo.at(lv.cs[TOTAL_E], i, j, k) = E;
+    Uo.at(lv.cs[PRESSURE], i, j, k) = p;
+}
+// wave min of the positive CFL candidates (as ordered bits)
+__device__ __forceinline__ unsigned long long wave_min_bits(double cand) {
+    unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o < b ? o : b;
+    }
+    return b;
+}
+// one atomic per wave (kernels whose waves each cover many cells)
+__device__ __forceinline__ void wave_min_to(double cand, unsigned long long* dst) {
+    const unsigned long long b = wave_min_bits(cand);
+    if ((threadIdx.x & 63) == 0) atomicMin(dst, b);
+}
+// one atomic per 256-thread block (one-cell-per-thread kernels: keeps the
+// number of same-address atomics low)
+__device__ __forceinline__ void block_min_to(double cand, unsigned long long* dst) {
+    __shared__ unsigned long long red[4];
+    const unsigned long long b = wave_min_bits(cand);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long mn = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
+        atomicMin(dst, mn);
+    }
+}
+
 // ------------------------------------------------------------------ y / z pass
 // One wave covers 64 columns (lanes along x, coalesced) and marches along
 // direction D over a segment of faces [f0, f1); the right state of row r-1
 // is carried in registers to the face of row r.  With THINC the alpha
 // reconstruction runs one row ahead so the BVD of row r sees rows r-1 and r+1.
-template <int NM, bool TH, int D>
-__global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, Faces F, Hist h, int seglen, Clock* clk) {
+// FUSE (last direction of the level): the segment owns rows [f0, f1) and
+// completes each row as soon as its upper face is known (update in x, y, z
+// order with the earlier directions' faces read from their planes, then
+// finish_cell); the faces of this direction stay in registers.
+#ifndef TJ_MARCH_WAVES_PER_EU
+#define TJ_MARCH_WAVES_PER_EU 1
+#endif
+template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WAVES_PER_EU, 8))) void k_flux_march(Level lv, MF U, Faces F, Hist h, int seglen, Clock* clk,
+                                                    MF Ub = MF{}, MF Uo = MF{}, Faces Fx = Faces{}, Faces Fy = Faces{},
+                                                    unsigned ops = 0u) {
     if (clk->done) return;
     const int lane = threadIdx.x & 63;
     const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -156,7 +280,7 @@ __global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, Faces F, His
     const int nd = lv.n[D];
     const int no = (D == 1) ? nz : ny;
     const int nstrip = (nx + 63) / 64;
-    const int nseg = (nd + 1 + seglen - 1) / seglen;
+    const int nseg = FUSE ? (nd + seglen - 1) / seglen : (nd + 1 + seglen - 1) / seglen;
     if (wave >= (long long)nstrip * nseg * no) return;
     const int strip = (int)(wave % nstrip);
     const long long rest = wave / nstrip;
@@ -166,11 +290,15 @@ __global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, Faces F, His
     const bool act = i0 < nx;
     const int i = act ? i0 : nx - 1;
     const int f0 = seg * seglen;
-    const int f1 = min(f0 + seglen, nd + 1);
-    const int rb = max(f0 - 1, 0), re = min(f1 - 1, nd - 1);
+    // faces computed: [f0, f1) (unfused) or [f0, f1] (fused: rows [f0, f1) owned)
+    const int f1 = FUSE ? min(f0 + seglen, nd) : min(f0 + seglen, nd + 1);
+    const int rb = max(f0 - 1, 0), re = FUSE ? min(f1, nd - 1) : min(f1 - 1, nd - 1);
+    const int ftop = FUSE ? f1 : f1 - 1;  // last face of the segment
     auto cj = [&](int r) { return D == 1 ? r : o; };
     auto ck = [&](int r) { return D == 2 ? r : o; };
     auto cellidx = [&](int r) { return ((long long)ck(r) * ny + cj(r)) * nx + i; };
+    double cand = __builtin_huge_val();
+    double Fprev[2 * NM + 5];
     auto faceidx = [&](int r) {  // lower face r along D
         return D == 1 ? ((long long)ck(r) * (ny + 1) + r) * nx + i : ((long long)r * ny + cj(r)) * nx + i;
     };
@@ -229,7 +357,7 @@ __global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, Faces F, His
             }
             st |= cleanup_alpha<NM>(PL.a);
             st |= cleanup_alpha<NM>(PR.a);
-            if (act && r >= f0) {
+            if (act && r >= f0 && r < (FUSE ? f1 : nd)) {
 #pragma unroll
                 for (int m = 0; m < NM; ++m)
                     if ((Acur.mixed >> m) & 1) h.urstar[m * h.pstride + cr] = Acur.thr[m];
@@ -241,14 +369,39 @@ __global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, Faces F, His
         FSt<NM> SLs, SRs;
         p2c_sound<NM>(lv, PL, SLs);
         p2c_sound<NM>(lv, PR, SRs);
-        auto emit = [&](int face, const FSt<NM>& A, const FSt<NM>& B) {
-            double Fv[2 * NM + 5], sa[NM];
+        // complete row q (faces q and q+1 known) in the fused form
+        auto finish_row = [&](int q, const double* Flo, const double* Fhi) {
+            const int jj = cj(q), kk = ck(q);
+            double al[NM], ar[NM], ru[3], E, a0[NM];
+            load_cons<NM>(lv, U, i, jj, kk, al, ar, ru, E);
+#pragma unroll
+            for (int m = 0; m < NM; ++m) a0[m] = al[m];
+            const double dt = clk->dt;
+            {
+                double Fa[2 * NM + 5], Fb[2 * NM + 5];
+                load_faces<NM>(Fx, i, jj, kk, Fa);
+                load_faces<NM>(Fx, i + 1, jj, kk, Fb);
+                accum_dir<NM>(al, ar, ru, E, a0, dt / lv.dx[0], Fa, Fb);
+            }
+            if (D == 2) {
+                double Fa[2 * NM + 5], Fb[2 * NM + 5];
+                load_faces<NM>(Fy, i, jj, kk, Fa);
+                load_faces<NM>(Fy, i, jj + 1, kk, Fb);
+                accum_dir<NM>(al, ar, ru, E, a0, dt / lv.dx[1], Fa, Fb);
+            }
+            accum_dir<NM>(al, ar, ru, E, a0, dt / lv.dx[D], Flo, Fhi);
+            if (act) finish_cell<NM, STAGE>(lv, Ub, Uo, i, jj, kk, al, ar, ru, E, ops, cand, st);
+        };
+        auto emit = [&](int face, const FSt<NM>& A, const FSt<NM>& B, double* Fv) {
+            double sa[NM];
             hllc<NM>(A, B, D, Fv, sa);
             if (!act) return;
             const long long fi = faceidx(face);
-            const long long fcs = F.cs;
+            if (!FUSE) {
+                const long long fcs = F.cs;
 #pragma unroll
-            for (int q = 0; q < 2 * NM + 5; ++q) F.p[q * fcs + fi] = Fv[q];
+                for (int q = 0; q < 2 * NM + 5; ++q) F.p[q * fcs + fi] = Fv[q];
+            }
             if (TH) {
                 if (face < nd) {
 #pragma unroll
@@ -260,22 +413,34 @@ __global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, Faces F, His
                 }
             }
         };
+        double Fcur[2 * NM + 5];
+        bool have = false;
         if (r == 0 && f0 == 0) {
             FSt<NM> img = SRs;
             if (lv.bc_lo[D]) flip_state<NM>(img, D);
-            emit(0, img, SLs);
-        } else if (r >= f0) {
-            emit(r, prevR, SLs);
+            emit(0, img, SLs, Fcur);
+            have = true;
+        } else if (r >= f0 && r <= ftop) {
+            emit(r, prevR, SLs, Fcur);
+            have = true;
         }
-        if (r == nd - 1 && f1 == nd + 1) {
+        if (FUSE && have && r - 1 >= f0) finish_row(r - 1, Fprev, Fcur);
+        if (FUSE && have) {
+#pragma unroll
+            for (int q = 0; q < 2 * NM + 5; ++q) Fprev[q] = Fcur[q];
+        }
+        if (r == nd - 1 && ftop == nd) {
             FSt<NM> img = SLs;
             if (lv.bc_hi[D]) flip_state<NM>(img, D);
-            emit(nd, SRs, img);
+            double Ftop[2 * NM + 5];
+            emit(nd, SRs, img, Ftop);
+            if (FUSE) finish_row(nd - 1, Fprev, Ftop);
         }
         prevR = SRs;
         Pm = Pc;
         Pc = Pp;
     }
+    if (FUSE && STAGE == 2 && (ops & 8u)) wave_min_to(cand, &clk->dtmin_bits);
     if (st) atomicOr(&clk->status, st);
 }
 
@@ -298,96 +463,20 @@ __global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, 
         const int j = (int)((id / nx) % ny);
         const int k = (int)(id / ((long long)nx * ny));
         const double dt = clk->dt;
-        double al[NM], ar[NM], ru[3], E;
-        double a0[NM];
+        double al[NM], ar[NM], ru[3], E, a0[NM];
+        load_cons<NM>(lv, Uin, i, j, k, al, ar, ru, E);
 #pragma unroll
-        for (int m = 0; m < NM; ++m) {
-            a0[m] = Uin.at(lv.cs[ALPHA] + m, i, j, k);
-            al[m] = a0[m];
-            ar[m] = Uin.at(lv.cs[ALPHARHO] + m, i, j, k);
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) ru[q] = Uin.at(lv.cs[RHOU] + q, i, j, k);
-        E = Uin.at(lv.cs[TOTAL_E], i, j, k);
+        for (int m = 0; m < NM; ++m) a0[m] = al[m];
         for (int d = 0; d < lv.dim; ++d) {
             const Faces& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
-            const int i1 = i + (d == 0), j1 = j + (d == 1), k1 = k + (d == 2);
-            const double dtdx = dt / lv.dx[d];
-            const double s0 = Fd.at(2 * NM + 4, i, j, k), s1 = Fd.at(2 * NM + 4, i1, j1, k1);
-#pragma unroll
-            for (int m = 0; m < NM; ++m)
-                al[m] += dtdx * (Fd.at(m, i, j, k) - Fd.at(m, i1, j1, k1) - a0[m] * (s0 - s1));
-#pragma unroll
-            for (int m = 0; m < NM; ++m) ar[m] += dtdx * (Fd.at(NM + m, i, j, k) - Fd.at(NM + m, i1, j1, k1));
-#pragma unroll
-            for (int q = 0; q < 3; ++q) ru[q] += dtdx * (Fd.at(2 * NM + q, i, j, k) - Fd.at(2 * NM + q, i1, j1, k1));
-            E += dtdx * (Fd.at(2 * NM + 3, i, j, k) - Fd.at(2 * NM + 3, i1, j1, k1));
+            double Fa[2 * NM + 5], Fb[2 * NM + 5];
+            load_faces<NM>(Fd, i, j, k, Fa);
+            load_faces<NM>(Fd, i + (d == 0), j + (d == 1), k + (d == 2), Fb);
+            accum_dir<NM>(al, ar, ru, E, a0, dt / lv.dx[d], Fa, Fb);
         }
-        double rk[NM], rho, u[3], p;
-        c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
-        if (STAGE == 2) {
-            // U1 = (U * 1/2) + (U2 * 1/2) over every component
-#pragma unroll
-            for (int m = 0; m < NM; ++m) {
-                al[m] = Ub.at(lv.cs[ALPHA] + m, i, j, k) * (1.0 / 2.0) + al[m] * (1.0 / 2.0);
-                ar[m] = Ub.at(lv.cs[ALPHARHO] + m, i, j, k) * (1.0 / 2.0) + ar[m] * (1.0 / 2.0);
-                rk[m] = Ub.at(lv.cs[RHO_K] + m, i, j, k) * (1.0 / 2.0) + rk[m] * (1.0 / 2.0);
-            }
-            rho = Ub.at(lv.cs[RHO], i, j, k) * (1.0 / 2.0) + rho * (1.0 / 2.0);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                ru[q] = Ub.at(lv.cs[RHOU] + q, i, j, k) * (1.0 / 2.0) + ru[q] * (1.0 / 2.0);
-                u[q] = Ub.at(lv.cs[VELOCITY] + q, i, j, k) * (1.0 / 2.0) + u[q] * (1.0 / 2.0);
-            }
-            E = Ub.at(lv.cs[TOTAL_E], i, j, k) * (1.0 / 2.0) + E * (1.0 / 2.0);
-            p = Ub.at(lv.cs[PRESSURE], i, j, k) * (1.0 / 2.0) + p * (1.0 / 2.0);
-            if (ops & 4u) st |= cleanup_alpha<NM>(al);
-            if (ops & 16u) {
-                bool bad = false;
-#pragma unroll
-                for (int m = 0; m < NM; ++m) bad |= __builtin_isnan(al[m]) || __builtin_isnan(ar[m]) || __builtin_isnan(rk[m]);
-#pragma unroll
-                for (int q = 0; q < 3; ++q) bad |= __builtin_isnan(ru[q]) || __builtin_isnan(u[q]);
-                bad |= __builtin_isnan(E) || __builtin_isnan(p);
-                if (bad) st |= 1u;
-            }
-            if (ops & 8u) {
-                const double a = sound<NM>(lv, al, ar, rk, rho, p);
-                Uo.at(lv.cs[SOUNDSPEED], i, j, k) = a;
-                for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(u[d])));
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < NM; ++m) {
-            Uo.at(lv.cs[ALPHA] + m, i, j, k) = al[m];
-            Uo.at(lv.cs[ALPHARHO] + m, i, j, k) = ar[m];
-            Uo.at(lv.cs[RHO_K] + m, i, j, k) = rk[m];
-        }
-        Uo.at(lv.cs[RHO], i, j, k) = rho;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            Uo.at(lv.cs[RHOU] + q, i, j, k) = ru[q];
-            Uo.at(lv.cs[VELOCITY] + q, i, j, k) = u[q];
-        }
-        Uo.at(lv.cs[TOTAL_E], i, j, k) = E;
-        Uo.at(lv.cs[PRESSURE], i, j, k) = p;
+        finish_cell<NM, STAGE>(lv, Ub, Uo, i, j, k, al, ar, ru, E, ops, cand, st);
     }
-    if (STAGE == 2 && (ops & 8u)) {
-        // wave then block min of the positive candidates, one atomic per block
-        unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
-        for (int off = 32; off > 0; off >>= 1) {
-            unsigned long long o = __shfl_xor(b, off, 64);
-            b = o < b ? o : b;
-        }
-        __shared__ unsigned long long red[4];
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long mn = red[0];
-            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
-            atomicMin(&clk->dtmin_bits, mn);
-        }
-    }
+    if (STAGE == 2 && (ops & 8u)) block_min_to(cand, &clk->dtmin_bits);
     if (st) atomicOr(&clk->status, st);
 }
 
@@ -415,19 +504,7 @@ __global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
         for (int d = 0; d < lv.dim; ++d)
             cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(U.at(lv.cs[VELOCITY] + d, i, j, k))));
     }
-    unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = o < b ? o : b;
-    }
-    __shared__ unsigned long long red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long mn = red[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
-        atomicMin(&clk->dtmin_bits, mn);
-    }
+    block_min_to(cand, &clk->dtmin_bits);
 }
 
 // dt = CFL * min (timestep.cpp:61-69); with `driver` the loop test and the

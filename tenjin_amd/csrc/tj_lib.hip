@@ -34,7 +34,8 @@ struct tj_ctx {
     unsigned* d_status = nullptr;
     long long pass = 0;
     long long dpass[3] = {0, 0, 0};
-    int seglen = 8;
+    int seglen = 16;
+    bool fuse = false;
     // per-kernel event timing
     bool prof = false;
     struct Rec { const char* name; hipEvent_t a, b; };
@@ -424,9 +425,22 @@ int tj_driver_dt(tj_ctx* c, double final_time) {
     return TJ_OK;
 }
 
-static int launch_fluxes(tj_ctx* c, const MF& Uin) {
+// The update folded into the last directional pass: stage, RK base, output
+// and post-step ops of the stage.
+struct FuseArgs {
+    int stage;
+    MF base, out;
+    unsigned ops;
+};
+
+// Directional flux passes of one RK stage.  With fz (fluid levels, dim >= 2,
+// output distinct from the input) the last pass also completes the update and
+// *fused is set; otherwise the caller launches the update kernel.
+static int launch_fluxes(tj_ctx* c, const MF& Uin, const FuseArgs* fz, bool* fused) {
     const Level& lv = c->lv;
     const bool th = c->desc.THINC != 0;
+    const bool fuse = fz && c->fuse && !lv.solid && lv.dim >= 2 && fz->out.p != Uin.p;
+    if (fused) *fused = fuse;
     for (int d = 0; d < lv.dim; ++d) {
         Hist h{};
         if (th) {
@@ -439,8 +453,11 @@ static int launch_fluxes(tj_ctx* c, const MF& Uin) {
             h.pstride = lv.ncells;
             h.estride = c->estride[d];
         }
+        const bool last = fuse && d == lv.dim - 1;
         static const char* names[3] = {"flux_x", "flux_y", "flux_z"};
-        Timed tm(c, names[d]);
+        static const char* fnames[2][3] = {{"", "flux_y_update1", "flux_z_update1"},
+                                           {"", "flux_y_update2_post", "flux_z_update2_post"}};
+        Timed tm(c, last ? fnames[fz->stage - 1][d] : names[d]);
         if (lv.solid) {
             if (d == 0) {
                 const long long nseg = (lv.n[0] + 1 + 62) / 63;
@@ -463,16 +480,34 @@ static int launch_fluxes(tj_ctx* c, const MF& Uin) {
             const unsigned g = grid1(waves, 4);
             if (th) hipLaunchKernelGGL((k_flux_x<2, true>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[0], h, c->d_clk);
             else hipLaunchKernelGGL((k_flux_x<2, false>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[0], h, c->d_clk);
-        } else {
+        } else if (!last) {
             const long long nstrip = (lv.n[0] + 63) / 64;
             const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
             const long long no = d == 1 ? lv.n[2] : lv.n[1];
             const unsigned g = grid1(nstrip * nseg * no, 4);
             if (d == 1) {
-                if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk);
-                else hipLaunchKernelGGL((k_flux_march<2, false, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk);
+                if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, MF{}, MF{}, Faces{}, Faces{}, 0u);
+                else hipLaunchKernelGGL((k_flux_march<2, false, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, MF{}, MF{}, Faces{}, Faces{}, 0u);
             } else {
-                hipLaunchKernelGGL((k_flux_march<2, false, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk);
+                hipLaunchKernelGGL((k_flux_march<2, false, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, MF{}, MF{}, Faces{}, Faces{}, 0u);
+            }
+        } else {
+            // fused: segments own rows [f0, f0 + seglen)
+            const long long nstrip = (lv.n[0] + 63) / 64;
+            const long long nseg = (lv.n[d] + c->seglen - 1) / c->seglen;
+            const long long no = d == 1 ? lv.n[2] : lv.n[1];
+            const unsigned g = grid1(nstrip * nseg * no, 4);
+            const MF b = fz->base, o = fz->out;
+            const unsigned ops = fz->ops;
+            const bool s2 = fz->stage == 2;
+            if (d == 1) {
+                if (th && s2) hipLaunchKernelGGL((k_flux_march<2, true, 1, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+                else if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+                else if (s2) hipLaunchKernelGGL((k_flux_march<2, false, 1, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+                else hipLaunchKernelGGL((k_flux_march<2, false, 1, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+            } else {
+                if (s2) hipLaunchKernelGGL((k_flux_march<2, false, 2, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+                else hipLaunchKernelGGL((k_flux_march<2, false, 2, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
             }
         }
         HIPCHK(c, hipGetLastError());
@@ -492,9 +527,11 @@ int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_m
     if (stage == 2 && (r = check_mf(c, Ubase))) return r;
     HIPCHK(c, hipSetDevice(c->device));
     const MF in = view(Uin);
-    if ((r = launch_fluxes(c, in))) return r;
     const MF base = stage == 2 ? view(Ubase) : in;
-    {
+    const FuseArgs fz{stage, base, view(Uout), 0u};
+    bool fused = false;
+    if ((r = launch_fluxes(c, in, &fz, &fused))) return r;
+    if (!fused) {
         Timed tm(c, stage == 1 ? "update_stage1" : "update_stage2");
         launch_update(c, stage, in, base, view(Uout), 0u);
     }
@@ -531,13 +568,17 @@ int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
     if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
     HIPCHK(c, hipSetDevice(c->device));
     const MF s = view(S), t = view(T);
-    if ((r = launch_fluxes(c, s))) return r;
-    {
+    if (s.p == t.p) { c->err = "step: S and T must be distinct buffers"; return TJ_EINVAL; }
+    bool fused = false;
+    const FuseArgs f1{1, s, t, 0u};
+    if ((r = launch_fluxes(c, s, &f1, &fused))) return r;
+    if (!fused) {
         Timed tm(c, "update_stage1");
         launch_update(c, 1, s, s, t, 0u);
     }
-    if ((r = launch_fluxes(c, t))) return r;
-    {
+    const FuseArgs f2{2, s, s, step_ops(c)};
+    if ((r = launch_fluxes(c, t, &f2, &fused))) return r;
+    if (!fused) {
         Timed tm(c, "update_stage2_post");
         launch_update(c, 2, t, s, s, step_ops(c));
     }
@@ -643,9 +684,10 @@ int tj_status(tj_ctx* c, uint32_t* status, int clear) {
     return TJ_OK;
 }
 
-int tj_set_tuning(tj_ctx* c, int seglen) {
-    if (!c || seglen < 1) return TJ_EINVAL;
-    c->seglen = seglen;
+int tj_set_tuning(tj_ctx* c, int seglen, int fuse) {
+    if (!c || seglen == 0) return TJ_EINVAL;
+    if (seglen > 0) c->seglen = seglen;
+    if (fuse >= 0) c->fuse = fuse != 0;
     return TJ_OK;
 }
 

@@ -307,19 +307,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
         store_scell<NM>(lv, Uo, i, j, k, C);
     }
     if (STAGE == 2 && (ops & 8u)) {
-        unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
-        for (int off = 32; off > 0; off >>= 1) {
-            unsigned long long o = __shfl_xor(b, off, 64);
-            b = o < b ? o : b;
-        }
-        __shared__ unsigned long long red[4];
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long mn = red[0];
-            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
-            atomicMin(&clk->dtmin_bits, mn);
-        }
+        block_min_to(cand, &clk->dtmin_bits);
     }
     if (st) atomicOr(&clk->status, st);
 }
@@ -340,19 +328,7 @@ __global__ __launch_bounds__(256) void k_sound_dt_s(Level lv, MF U, Clock* clk) 
         U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
         for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
     }
-    unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = o < b ? o : b;
-    }
-    __shared__ unsigned long long red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long mn = red[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
-        atomicMin(&clk->dtmin_bits, mn);
-    }
+    block_min_to(cand, &clk->dtmin_bits);
 }
 
 // setInitialConditions for solid levels: thermal -> total pressure shift
@@ -495,19 +471,7 @@ __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsi
         store_scell<NM>(lv, U, i, j, k, C);
     }
     if (ops & 8u) {
-        unsigned long long b = __builtin_bit_cast(unsigned long long, cand);
-        for (int off = 32; off > 0; off >>= 1) {
-            unsigned long long o = __shfl_xor(b, off, 64);
-            b = o < b ? o : b;
-        }
-        __shared__ unsigned long long red[4];
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long mn = red[0];
-            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = red[w] < mn ? red[w] : mn;
-            atomicMin(&clk->dtmin_bits, mn);
-        }
+        block_min_to(cand, &clk->dtmin_bits);
     }
     if (st) atomicOr(&clk->status, st);
 }

@@ -16,20 +16,22 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _pair(pb):
+def _pair(pb, seglen=-1, fuse=-1):
     from oracle.oracle import Oracle
     from tenjin_amd import GpuLevel
 
     o = Oracle(pb)
     g = GpuLevel(pb)
+    g.set_tuning(seglen, fuse)
     g.init_from_prims(pb.prims, pb.shift_pressure)
     return o, g
 
 
-def _stepwise(pb, nsteps, exact=True):
+def _stepwise(pb, nsteps, exact=True, seglen=-1, fuse=-1):
     print(f"\n[case] {pb.name} n={pb.n[:pb.dim]} box={pb.max_grid_size} thinc={pb.THINC} "
-          f"bc_lo={pb.bc_lo[:pb.dim]} bc_hi={pb.bc_hi[:pb.dim]} steps={nsteps}", flush=True)
-    o, g = _pair(pb)
+          f"bc_lo={pb.bc_lo[:pb.dim]} bc_hi={pb.bc_hi[:pb.dim]} steps={nsteps} seglen={seglen} fuse={fuse}",
+          flush=True)
+    o, g = _pair(pb, seglen, fuse)
     so, sg = o.get(), g.get_state()
     assert max_rel(pb, so, sg) <= TOL
     for n in range(nsteps):
@@ -81,29 +83,33 @@ def test_shock_tube_1d_full_run():
     assert exact_fraction(pb, so, sg) == 1.0
 
 
-@pytest.mark.parametrize("thinc", [0, 1])
-def test_inclusion_2d_stepwise(thinc):
+@pytest.mark.parametrize("thinc,fuse", [(0, 1), (1, 1), (0, 0), (1, 0)])
+def test_inclusion_2d_stepwise(thinc, fuse):
+    """fuse=1: the update runs inside the y pass; fuse=0: separate update kernel."""
     from tenjin_amd import configs
 
     pb = configs.shock_inclusion_2d(nx=128, ny=64, box=32, thinc=thinc, shock_x=0.55, drift=(150.0, 60.0))
-    _stepwise(pb, 30)
+    _stepwise(pb, 30, fuse=fuse)
 
 
-def test_inclusion_2d_uneven_boxes():
+@pytest.mark.parametrize("seglen", [5, 16, 64])
+def test_inclusion_2d_uneven_boxes(seglen):
     """Box sizes from BoxArray::maxSize that do not divide the domain, THINC on,
-    transmissive/reflective mix."""
+    transmissive/reflective mix; marching segments that do not divide ny (5),
+    that do (16 does not divide 60 either; 64 covers the column in one)."""
     from tenjin_amd import configs
 
     pb = configs.shock_inclusion_2d(nx=100, ny=60, box=24, thinc=1, shock_x=0.55, drift=(-120.0, 90.0))
     pb.bc_lo = (1, 0, 0)
     pb.bc_hi = (0, 1, 0)
-    _stepwise(pb, 25)
+    _stepwise(pb, 25, seglen=seglen)
 
 
-def test_inclusion_3d_stepwise():
+@pytest.mark.parametrize("seglen,fuse", [(16, 1), (3, 1), (7, 0)])
+def test_inclusion_3d_stepwise(seglen, fuse):
     from tenjin_amd import configs
 
-    _stepwise(configs.shock_inclusion_3d(), 12)
+    _stepwise(configs.shock_inclusion_3d(), 12, seglen=seglen, fuse=fuse)
 
 
 def test_inclusion_2d_full_size_two_steps():
@@ -188,3 +194,33 @@ def test_plate_impact_3d_full_size_invariants():
     assert float(g.S[L.comp(ALPHARHO, 0)].min()) > 0.0
     _, _, n = g.clock()
     assert n == 3
+
+
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_stage_api_matches_step(fuse):
+    """tj_rk_stage(1), tj_rk_stage(2) and tj_post_step (cleanUpAlpha, NaN scan,
+    getTimeStep) reproduce tj_step bit for bit, and both match the oracle."""
+    import torch
+
+    from tenjin_amd import GpuLevel, configs
+
+    pb = configs.shock_inclusion_2d(nx=96, ny=64, box=32, thinc=1, shock_x=0.55, drift=(80.0, -40.0))
+    print(f"\n[case] stage api fuse={fuse}", flush=True)
+    a, b = GpuLevel(pb), GpuLevel(pb)
+    for g in (a, b):
+        g.set_tuning(-1, fuse)
+        g.init_from_prims(pb.prims)
+    for n in range(8):
+        dt = a.compute_dt()
+        assert b.compute_dt() == dt
+        dt = 0.2 * dt if n < 5 else dt
+        for g in (a, b):
+            g.set_dt(dt, reset_clock=1)
+        a.step()
+        b.stage(1)
+        b.stage(2)
+        b.post_step(4 | 16 | 8)
+        a.sync()
+        b.sync()
+        assert torch.equal(a.S, b.S), f"step {n}"
+    assert a.status() == 0 and b.status() == 0
