@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Throughput of the per-cell explicit update on one MI355X.
 
-Workload: the 2D two-material stiffened-gas shock/inclusion interaction on a
-single level of 2048 x 1024 cells in 64 x 64 boxes (MUSCL + THINC/BVD, beta 1.6,
-HLLC, CFL 0.5), synthetic initial data generated here.  One "step" is one full
-time step of the reference's loop body (main.cpp:124-171): CFL dt with the
-driver clamps, two RK stages, the RK average, cleanUpAlpha, the NaN scan and the
-next step's CFL reduction.  The state is resident in HBM before timing starts.
+Default workload: the 2D two-material stiffened-gas shock/inclusion
+interaction on a single level of 2048 x 1024 cells in 64 x 64 boxes (MUSCL +
+THINC/BVD, beta 1.6, HLLC, CFL 0.5), synthetic initial data generated here.
+`--workload plate3d` runs the 3D elastoplastic copper/aluminium plate impact
+on 256^3 cells in 64^3 boxes (Us-up Mie-Gruneisen, 5-wave flux, perfect
+plasticity).  One "step" is one full time step of the reference's loop body
+(main.cpp:124-171): CFL dt with the driver clamps, two RK stages, the RK
+average, plasticUpdate and cleanUpV (solids), cleanUpAlpha, the NaN scan and
+the next step's CFL reduction.  The state is resident in HBM before timing
+starts.
 
 Prints one JSON line:  metric/value (Mcell-updates/s = cells x steps / s),
 roofline of the dominant kernel (HIP-event times on the library's stream over
-the timed region), step roofline (464 B per cell-update) and the CPU baseline
-(the oracle restatement on this host's cores, a bounded sample of the same
-workload).
+the timed region), step roofline (8 (3 N_s + 2 N_c) bytes per cell-update) and
+the CPU baseline (the oracle restatement on this host's cores, a bounded sample
+of the same workload).
 """
 from __future__ import annotations
 
@@ -26,9 +30,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell of each kernel (unique state reads + writes,
-# FP64, two materials; halo re-reads and THINC history reads not counted)
-KERNEL_BYTES_PER_CELL = {
+# Algorithmic bytes per cell of each kernel (unique state reads + writes, FP64,
+# two materials; halo re-reads and THINC history reads not counted).
+FLUID_KERNEL_BYTES = {
     "flux_x": 64 + 72 + 16,            # 8 primitives in, 9 flux comps out, 2 star alphas out
     "flux_y": 64 + 72 + 16,
     "flux_z": 64 + 72,
@@ -39,17 +43,32 @@ KERNEL_BYTES_PER_CELL = {
     "flux_y_update1": 112 + 72 + 16 + 120,
     "flux_y_update2_post": 112 + 72 + 16 + 120 + 120 + 8,  # + base in, + sound speed out
 }
-STEP_BYTES_PER_CELL_UPDATE = 464  # 8 * (3 * N_s + 2 * N_c), N_s = 14, N_c = 8
+SOLID_KERNEL_BYTES = {
+    # 19 primitives in (alpha, rho_k, epsilon, u, p, V), 23 face comps out
+    # (19 conserved fluxes, S*, the V* row of the face)
+    "flux_x": 152 + 184,
+    "flux_y": 152 + 184,
+    "flux_z": 152 + 184,
+    # 19 conserved in, 3 directions x 23 face comps, 29 state comps out
+    "update_stage1": 152 + 3 * 184 + 232,
+    # + 29 base comps in (RK average), + sound speed out
+    "update_stage2_post": 152 + 3 * 184 + 232 + 232 + 8,
+}
+# 8 (3 N_s + 2 N_c): N_s/N_c = 14/8 (two-material fluid), 27/19 (two-material solid)
+STEP_BYTES = {"inclusion2d": 464, "plate3d": 952}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--workload", choices=["inclusion2d", "plate3d"], default="inclusion2d")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 500 for inclusion2d, 40 for plate3d)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--prec", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--n3", type=int, default=256, help="plate3d: cells per side")
     ap.add_argument("--box", type=int, default=64)
     ap.add_argument("--seglen", type=int, default=16)
     ap.add_argument("--fuse", action="store_true",
@@ -58,10 +77,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a plain 1 GiB device copy first (PMC byte-counter calibration)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 500 if args.workload == "inclusion2d" else 40
+    return args
 
 
-def cpu_baseline(pb, budget_s: float):
+def cpu_baseline(pb, budget_s: float, max_steps: int = 20):
     """The oracle (CPU restatement) on this host's cores: first steps of the
     same workload from its initial condition, until the time budget is used."""
     from oracle.oracle import Oracle
@@ -77,20 +99,24 @@ def cpu_baseline(pb, budget_s: float):
         o.step(dt if n + 1 >= 5 else 0.2 * dt)
         n += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 20:
+        if el >= budget_s or n >= max_steps:
             break
     return {
         "value": pb.ncells * n / el / 1e6,
         "unit": "Mcell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle restatement, {pb.n[0]}x{pb.n[1]} level, {n} steps after 1 warm-up step, {el:.1f} s, {threads} OpenMP threads over boxes",
+        "sample": f"oracle restatement, {'x'.join(str(v) for v in pb.n[:pb.dim])} level, {n} steps after 1 warm-up "
+                  f"step, {el:.1f} s, {threads} OpenMP threads over boxes",
     }
 
 
-def read_traffic(name: str):
+TRAFFIC_FILES = {"inclusion2d": "pmc_traffic_shock_inclusion_2d.txt", "plate3d": "pmc_traffic_plate_impact_3d.txt"}
+
+
+def read_traffic(workload: str, name: str):
     """HBM bytes per launch of `name` from the committed rocprofv3 PMC summary."""
-    path = os.path.join(HERE, "profiles", "pmc_traffic_shock_inclusion_2d.txt")
+    path = os.path.join(HERE, "profiles", TRAFFIC_FILES[workload])
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -117,7 +143,16 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    pb = configs.shock_inclusion_2d(nx=args.nx, ny=args.ny, box=args.box)
+    if args.workload == "inclusion2d":
+        pb = configs.shock_inclusion_2d(nx=args.nx, ny=args.ny, box=args.box)
+        kbytes = FLUID_KERNEL_BYTES
+        workload = (f"2D two-material shock/inclusion, {pb.n[0]}x{pb.n[1]} cells, {pb.max_grid_size}^2 boxes, "
+                    "MUSCL+THINC(beta 1.6)+HLLC, CFL 0.5")
+    else:
+        pb = configs.plate_impact(dim=3, n=args.n3, box=args.box)
+        kbytes = SOLID_KERNEL_BYTES
+        workload = (f"3D elastoplastic Cu/Al plate impact, {args.n3}^3 cells, {pb.max_grid_size}^3 boxes, "
+                    "Us-up Mie-Gruneisen, MUSCL+5-wave HLLC, perfect plasticity, CFL 0.5")
     prec = TJ_PREC_FP64 if args.prec == "fp64" else TJ_PREC_MIXED
     g = GpuLevel(pb)
     g.set_tuning(args.seglen, 1 if args.fuse else 0)
@@ -145,7 +180,7 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     dname, (dcount, dms) = dom
     davg_s = dms / dcount / 1e3
-    dbytes = KERNEL_BYTES_PER_CELL[dname] * ncell
+    dbytes = kbytes[dname] * ncell
     achieved = dbytes / davg_s / 1e9
     roofline = {
         "bound": "hbm",
@@ -154,12 +189,13 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": read_traffic(dname),
+        "traffic": read_traffic(args.workload, dname),
         "algorithmic_bytes_per_launch": dbytes,
         "avg_launch_ms": round(davg_s * 1e3, 4),
     }
     kernel_ms = {k: round(v[1] / v[0], 4) for k, v in prof.items()}
-    step_gbs = value * 1e6 * STEP_BYTES_PER_CELL_UPDATE / 1e9
+    step_bytes = STEP_BYTES[args.workload]
+    step_gbs = value * 1e6 * step_bytes / 1e9
 
     out = {
         "metric": "Mcell-updates/s on 1 GPU",
@@ -175,8 +211,7 @@ def main():
         "dtype": "f64" if prec == TJ_PREC_FP64 else "f32-flux/f64-update",
         "data": "synthetic",
         "config": {
-            "workload": f"2D two-material shock/inclusion, {pb.n[0]}x{pb.n[1]} cells, {pb.max_grid_size}^2 boxes, "
-                        "MUSCL+THINC(beta 1.6)+HLLC, CFL 0.5",
+            "workload": workload,
             "cells": ncell,
             "precision": args.prec,
             "fused_update": bool(args.fuse),
@@ -185,7 +220,7 @@ def main():
         },
         "roofline": roofline,
         "step_roofline": {
-            "bytes_per_cell_update": STEP_BYTES_PER_CELL_UPDATE,
+            "bytes_per_cell_update": step_bytes,
             "achieved": round(step_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -197,7 +232,12 @@ def main():
     }
     del g
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pb, args.cpu_seconds)
+        if args.workload == "inclusion2d":
+            out["cpu_baseline"] = cpu_baseline(pb, args.cpu_seconds, 20)
+        else:
+            # bounded sample: the same problem on 128^3 cells (one eighth of the cells)
+            out["cpu_baseline"] = cpu_baseline(configs.plate_impact(dim=3, n=min(args.n3, 128), box=args.box),
+                                               args.cpu_seconds, 3)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
