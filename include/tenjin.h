@@ -171,6 +171,11 @@ int tj_run(tj_ctx* ctx, tj_mf* S, tj_mf* T, double final_time, int max_steps, in
 int tj_init_from_prims(tj_ctx* ctx, tj_mf* U, int shift_pressure);
 /* conservativeToPrimitive over the valid cells (boxaccesscellarray.cpp:67-122). */
 int tj_c2p(tj_ctx* ctx, tj_mf* U);
+/* CellArray RK algebra (cellarray.cpp:80-99, the operators behind
+ * U1 = ((U*(1.0/2.0))+(U2*(1.0/2.0))) of hllc.cpp:705): Z = (X*a) + (Y*b) over
+ * every component of the views and every valid cell, products rounded before
+ * the sum; Y may be NULL (Z = X*a).  Z may alias X or Y. */
+int tj_lincomb(tj_ctx* ctx, const tj_mf* X, double a, const tj_mf* Y, double b, tj_mf* Z);
 /* Derived components sigma (and devH, HJ2 in solid runs) from the state. */
 int tj_derive(tj_ctx* ctx, tj_mf* U);
 

@@ -77,7 +77,7 @@ EXPORTS = [
     "tj_comp_start", "tj_compute_dt", "tj_driver_dt", "tj_set_dt", "tj_get_clock", "tj_rk_stage",
     "tj_post_step", "tj_step", "tj_run", "tj_init_from_prims", "tj_c2p", "tj_derive", "tj_status",
     "tj_reset_history", "tj_synchronize", "tj_prof_enable", "tj_prof_report", "tj_set_tuning",
-    "tj_begin_run", "tj_advance_steps",
+    "tj_begin_run", "tj_advance_steps", "tj_lincomb",
 ]
 
 _lib = None
@@ -125,6 +125,7 @@ def load_library(path: str = LIB_PATH):
     L.tj_set_tuning.argtypes = [vp, ip, ip]
     L.tj_begin_run.argtypes = [vp, mfp, dp]
     L.tj_advance_steps.argtypes = [vp, mfp, mfp, ip, ip]
+    L.tj_lincomb.argtypes = [vp, mfp, dp, mfp, dp, mfp]
     _lib = L
     return L
 

@@ -1,6 +1,8 @@
 """Build step of the gfx950 library: compiles tenjin_amd/csrc/*.hip into
 tenjin_amd/libtenjin.so with hipcc (in-tree, so the built library travels with
-the source tree).
+the source tree), the C++ host interface tenjin_amd/csrc/host/ into
+tenjin_amd/libtenjin_host.so, and the C++ time-loop driver used by the tests
+(tests/host/main_loop.cpp) into tests/host/main_loop.
 
 Flags:
   --offload-arch=gfx950            MI355X (CDNA4) only
@@ -16,8 +18,12 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtenjin.so")
+HOST_LIB = os.path.join(HERE, "libtenjin_host.so")
+DRIVER = os.path.join(ROOT, "tests", "host", "main_loop")
+INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -36,14 +42,37 @@ def sources():
     return [os.path.join(CSRC, "tj_lib.hip")]
 
 
-def build(verbose: bool = True, extra=None) -> str:
-    cmd = [HIPCC] + FLAGS + list(extra or []) + sources() + ["-o", LIB + ".tmp"]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
+
+
+def build(verbose: bool = True, extra=None) -> str:
+    cmd = [HIPCC] + FLAGS + list(extra or []) + sources() + ["-I", INCLUDE, "-o", LIB + ".tmp"]
+    _run(cmd, verbose)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 
 
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
+
+
+def build_host(verbose: bool = True) -> str:
+    """The C++ host interface (host code only; links libtenjin.so)."""
+    src = os.path.join(CSRC, "host", "tenjin_host.cpp")
+    cmd = [HIPCC] + HOST_FLAGS + ["-shared", "-I", INCLUDE, src, "-L", HERE, "-ltenjin",
+                                  "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB + ".tmp"]
+    _run(cmd, verbose)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    src = os.path.join(ROOT, "tests", "host", "main_loop.cpp")
+    cmd = [HIPCC] + HOST_FLAGS + ["-I", INCLUDE, "-I", os.path.join(CSRC, "host"), src, "-L", HERE, "-ltenjin_host",
+                                  "-ltenjin", "-Wl,-rpath,$ORIGIN/../../tenjin_amd", "-o", DRIVER + ".tmp"]
+    _run(cmd, verbose)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return HOST_LIB
+
+
 if __name__ == "__main__":
     build(extra=sys.argv[1:])
+    build_host()

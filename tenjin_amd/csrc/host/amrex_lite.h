@@ -1,0 +1,172 @@
+// amrex_lite.h -- the few AMReX-shaped host types that the reference's hot-path
+// signatures name (Box, BoxArray, DistributionMapping, Geometry, BCRec,
+// MultiFab, MFIter, Array, Vector), for one level on one GPU.
+//
+// These are not AMReX.  They keep AMReX's names and meanings where the
+// reference's code relies on them:
+//   * BoxArray::maxSize cuts the domain as AMReX does (chunks of max_grid_size,
+//     the remainder spread one cell at a time over the leading chunks);
+//   * BCRec holds AMReX BCType codes (reflect_odd -1, reflect_even 1,
+//     foextrap 2);
+//   * a MultiFab's data are component-major with i fastest (FArrayBox order).
+// They differ where a GPU-resident level wants something else: a MultiFab is
+// one device array over the whole level's valid cells (its boxes are views
+// into it) and stores no ghost cells -- ghost values are mapped on the fly by
+// the kernels -- so nGrow() is recorded but not allocated.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#ifndef AMREX_SPACEDIM
+#define AMREX_SPACEDIM 3
+#endif
+
+namespace amrex_lite {
+
+using Real = double;
+template <class T>
+using Vector = std::vector<T>;
+template <class T, std::size_t N>
+using Array = std::array<T, N>;
+
+struct IntVect {
+    int v[3] = {0, 0, 0};
+    IntVect() = default;
+    IntVect(int a, int b, int c) : v{a, b, c} {}
+    int operator[](int d) const { return v[d]; }
+    int& operator[](int d) { return v[d]; }
+};
+
+// Cell-centred index box [lo, hi] (inclusive, AMReX convention).
+struct Box {
+    IntVect lo, hi;
+    Box() = default;
+    Box(const IntVect& l, const IntVect& h) : lo(l), hi(h) {}
+    const IntVect& smallEnd() const { return lo; }
+    const IntVect& bigEnd() const { return hi; }
+    int smallEnd(int d) const { return lo[d]; }
+    int bigEnd(int d) const { return hi[d]; }
+    int length(int d) const { return hi[d] - lo[d] + 1; }
+    long long numPts() const { return (long long)length(0) * length(1) * length(2); }
+};
+
+class BoxArray {
+public:
+    BoxArray() = default;
+    explicit BoxArray(const Box& domain) : boxes_{domain} {}
+    // Chops every box into pieces of at most `chunk` cells per direction.
+    void maxSize(int chunk);
+    int size() const { return (int)boxes_.size(); }
+    const Box& operator[](int b) const { return boxes_[b]; }
+    // the union of the boxes (the level's domain)
+    Box minimalBox() const;
+
+private:
+    Vector<Box> boxes_;
+};
+
+// Single process, single GPU: every box is local.
+class DistributionMapping {
+public:
+    DistributionMapping() = default;
+    explicit DistributionMapping(const BoxArray&) {}
+};
+
+struct RealBox {
+    Real lo[3] = {0, 0, 0};
+    Real hi[3] = {1, 1, 1};
+    RealBox() = default;
+    RealBox(const Array<Real, 3>& l, const Array<Real, 3>& h) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = l[d];
+            hi[d] = h[d];
+        }
+    }
+};
+
+class Geometry {
+public:
+    Geometry() = default;
+    // dim: number of active directions (the reference's AMREX_SPACEDIM)
+    Geometry(const Box& domain, const RealBox& rb, int dim) : domain_(domain), rb_(rb), dim_(dim) {}
+    const Box& Domain() const { return domain_; }
+    Real CellSize(int d) const { return (rb_.hi[d] - rb_.lo[d]) / domain_.length(d); }
+    Real ProbLength(int d) const { return rb_.hi[d] - rb_.lo[d]; }
+    int dim() const { return dim_; }
+
+private:
+    Box domain_;
+    RealBox rb_;
+    int dim_ = AMREX_SPACEDIM;
+};
+
+namespace BCType {
+enum { reflect_odd = -1, int_dir = 0, reflect_even = 1, foextrap = 2, ext_dir = 3 };
+}
+
+class BCRec {
+public:
+    BCRec() = default;
+    void setLo(int d, int v) { lo_[d] = v; }
+    void setHi(int d, int v) { hi_[d] = v; }
+    int lo(int d) const { return lo_[d]; }
+    int hi(int d) const { return hi_[d]; }
+
+private:
+    int lo_[3] = {BCType::foextrap, BCType::foextrap, BCType::foextrap};
+    int hi_[3] = {BCType::foextrap, BCType::foextrap, BCType::foextrap};
+};
+
+// One level's cell data on the GPU: ncomp components over the valid cells of
+// the BoxArray's minimal box, component-major, i fastest.
+class MultiFab {
+public:
+    MultiFab() = default;
+    MultiFab(const BoxArray& ba, const DistributionMapping& dm, int ncomp, int ngrow) { define(ba, dm, ncomp, ngrow); }
+    ~MultiFab();
+    MultiFab(const MultiFab&) = delete;
+    MultiFab& operator=(const MultiFab&) = delete;
+
+    void define(const BoxArray& ba, const DistributionMapping& dm, int ncomp, int ngrow);
+    int nComp() const { return ncomp_; }
+    int nGrow() const { return ngrow_; }
+    const BoxArray& boxArray() const { return ba_; }
+    const Box& domain() const { return dom_; }
+    long long numCells() const { return dom_.numPts(); }
+    // device pointer to component 0 of cell (lo, lo, lo); strides in doubles
+    Real* devicePtr() const { return data_; }
+    long long jstride() const { return dom_.length(0); }
+    long long kstride() const { return (long long)dom_.length(0) * dom_.length(1); }
+    long long nstride() const { return numCells(); }
+
+    // host staging (synchronous): the whole level, same layout
+    void copyToHost(Vector<Real>& h) const;
+    void copyFromHost(const Vector<Real>& h);
+
+    static void Copy(MultiFab& dst, const MultiFab& src, int srccomp, int dstcomp, int numcomp, int nghost);
+
+private:
+    BoxArray ba_;
+    Box dom_;
+    int ncomp_ = 0, ngrow_ = 0;
+    Real* data_ = nullptr;
+};
+
+// Iterates over the boxes of a MultiFab's BoxArray.
+class MFIter {
+public:
+    explicit MFIter(const MultiFab& mf) : ba_(&mf.boxArray()) {}
+    bool isValid() const { return b_ < ba_->size(); }
+    void operator++() { ++b_; }
+    int index() const { return b_; }
+    const Box& validbox() const { return (*ba_)[b_]; }
+
+private:
+    const BoxArray* ba_;
+    int b_ = 0;
+};
+
+}  // namespace amrex_lite

@@ -1,0 +1,393 @@
+// tenjin_host.cpp -- the reference's hot-path C++ interface over the C ABI of
+// the gfx950 library.  Host code only: every computation is a tj_* call.
+#include "tenjin_host.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+
+namespace {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+tenjin_host::Device& dev(ParameterStruct& parameters) {
+    if (!parameters.gpu)
+        throw std::runtime_error("tenjin_host: level not bound (call tenjin_host::bindLevel after the set-up)");
+    return *parameters.gpu;
+}
+
+void tj_check(ParameterStruct& parameters, int rc, const char* what) {
+    if (rc != TJ_OK) {
+        const char* msg = parameters.gpu ? tj_last_error(parameters.gpu->ctx) : "";
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + (msg ? msg : ""));
+    }
+}
+
+// Starts of the pieces of [0, n) cut by BoxArray::maxSize(chunk): common
+// factors of two are taken out of n and chunk, the reduced length is split
+// into ceil(n'/chunk') pieces whose sizes differ by at most one, and the
+// pieces are laid out from the high end, the larger ones first (the rule the
+// library applies in tj_bind_level).
+Vector<int> chop(int n, int chunk) {
+    if (n <= chunk) return {0, n};
+    int ratio = 1, bs = chunk, len = n;
+    while (bs % 2 == 0 && len % 2 == 0) {
+        ratio *= 2;
+        bs /= 2;
+        len /= 2;
+    }
+    const int pieces = len / bs + (len % bs ? 1 : 0);
+    const int sz = len / pieces, extra = len % pieces;
+    Vector<int> starts;
+    int hi = n - 1;
+    for (int p = 0; p < pieces - 1; ++p) {
+        const int size = (p < extra ? sz + 1 : sz) * ratio;
+        starts.push_back(hi - size + 1);
+        hi -= size;
+    }
+    starts.push_back(0);
+    std::sort(starts.begin(), starts.end());
+    starts.push_back(n);
+    return starts;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ amrex_lite
+namespace amrex_lite {
+
+void BoxArray::maxSize(int chunk) {
+    Vector<Box> out;
+    for (const Box& b : boxes_) {
+        Vector<int> cut[3];
+        for (int d = 0; d < 3; ++d) {
+            cut[d] = chop(b.length(d), chunk);
+            for (int& v : cut[d]) v += b.lo[d];
+        }
+        for (size_t k = 0; k + 1 < cut[2].size(); ++k)
+            for (size_t j = 0; j + 1 < cut[1].size(); ++j)
+                for (size_t i = 0; i + 1 < cut[0].size(); ++i)
+                    out.push_back(Box(IntVect(cut[0][i], cut[1][j], cut[2][k]),
+                                      IntVect(cut[0][i + 1] - 1, cut[1][j + 1] - 1, cut[2][k + 1] - 1)));
+    }
+    boxes_ = out;
+}
+
+Box BoxArray::minimalBox() const {
+    Box m = boxes_.at(0);
+    for (const Box& b : boxes_)
+        for (int d = 0; d < 3; ++d) {
+            m.lo[d] = std::min(m.lo[d], b.lo[d]);
+            m.hi[d] = std::max(m.hi[d], b.hi[d]);
+        }
+    return m;
+}
+
+void MultiFab::define(const BoxArray& ba, const DistributionMapping&, int ncomp, int ngrow) {
+    ba_ = ba;
+    dom_ = ba.minimalBox();
+    ncomp_ = ncomp;
+    ngrow_ = ngrow;
+    hip_check(hipMalloc(&data_, sizeof(Real) * (size_t)ncomp * (size_t)numCells()), "MultiFab: hipMalloc");
+    hip_check(hipMemset(data_, 0, sizeof(Real) * (size_t)ncomp * (size_t)numCells()), "MultiFab: hipMemset");
+}
+
+MultiFab::~MultiFab() {
+    if (data_) (void)hipFree(data_);
+}
+
+void MultiFab::copyToHost(Vector<Real>& h) const {
+    h.resize((size_t)ncomp_ * (size_t)numCells());
+    hip_check(hipDeviceSynchronize(), "MultiFab::copyToHost");
+    hip_check(hipMemcpy(h.data(), data_, sizeof(Real) * h.size(), hipMemcpyDeviceToHost), "MultiFab::copyToHost");
+}
+
+void MultiFab::copyFromHost(const Vector<Real>& h) {
+    if (h.size() != (size_t)ncomp_ * (size_t)numCells()) throw std::runtime_error("MultiFab::copyFromHost: size");
+    hip_check(hipDeviceSynchronize(), "MultiFab::copyFromHost");
+    hip_check(hipMemcpy(data_, h.data(), sizeof(Real) * h.size(), hipMemcpyHostToDevice), "MultiFab::copyFromHost");
+}
+
+void MultiFab::Copy(MultiFab& dst, const MultiFab& src, int srccomp, int dstcomp, int numcomp, int) {
+    if (dst.numCells() != src.numCells()) throw std::runtime_error("MultiFab::Copy: different levels");
+    hip_check(hipDeviceSynchronize(), "MultiFab::Copy");
+    hip_check(hipMemcpy(dst.data_ + (size_t)dstcomp * dst.numCells(), src.data_ + (size_t)srccomp * src.numCells(),
+                        sizeof(Real) * (size_t)numcomp * (size_t)src.numCells(), hipMemcpyDeviceToDevice),
+              "MultiFab::Copy");
+}
+
+}  // namespace amrex_lite
+
+// ------------------------------------------------------------------ AccessPattern
+// accesspattern.cpp:44-79: alpha, alphaRho, rho_k (per material), rho,
+// rhoU(3), u(3), E, p, a, uStar, sigma(3x3) [mixtures: none on this path];
+// solids add V(3x3), VStar(3x3), devH(3x3), HenckyJ2; plastic solids add
+// epsilon and alphaRhoEpsilon (per material).  Names are nameBase + material
+// + "_" + row + col (accesspattern.cpp:17).
+AccessPattern::AccessPattern(ParameterStruct& parameters) {
+    int n = 0;
+    auto add = [&](Variable v, const std::string& name, int mats, int rows = 1, int cols = 1) {
+        data[v] = n;
+        for (int m = 0; m < mats; ++m)
+            for (int r = 0; r < rows; ++r)
+                for (int c = 0; c < cols; ++c)
+                    variableNames.push_back(name + std::to_string(m) + "_" + std::to_string(r) + std::to_string(c));
+        n += mats * rows * cols;
+    };
+    const int M = parameters.numberOfMaterials;
+    add(ALPHA, "alpha", M);
+    add(ALPHARHO, "alphaRho", M);
+    add(RHO_K, "rho_k", M);
+    add(RHO, "rho", 1);
+    add(RHOU, "rhoU", 1, 3);
+    add(VELOCITY, "u", 1, 3);
+    add(TOTAL_E, "E", 1);
+    add(P, "p", 1);
+    add(SOUNDSPEED, "a", 1);
+    add(USTAR, "uStar", 1);
+    add(SIGMA, "sigma", 1, 3, 3);
+    if (parameters.SOLID) {
+        add(V_TENSOR, "V", 1, 3, 3);
+        add(VSTAR, "VStar", 1, 3, 3);
+        add(DEVH, "devH", 1, 3, 3);
+        add(HJ2, "HenckyJ2", 1);
+        if (parameters.PLASTIC) {
+            add(EPSILON, "epsilon", M);
+            add(ALPHARHOEPSILON, "alphaRhoEpsilon", M);
+        }
+    }
+    ncomp = n;
+    parameters.Ncomp = n;
+}
+
+// ------------------------------------------------------------------ CellArray
+CellArray::CellArray(BoxArray& ba, DistributionMapping& dm, const int Ncomp, const int Nghost,
+                     AccessPattern& _accessPattern, ParameterStruct& parameters_)
+    : accessPattern(_accessPattern), numberOfMaterials(parameters_.numberOfMaterials), parameters(parameters_) {
+    data.define(ba, dm, Ncomp, Nghost);
+}
+
+tj_mf CellArray::view() {
+    tj_mf v{};
+    v.data = data.devicePtr();
+    v.jstride = data.jstride();
+    v.kstride = data.kstride();
+    v.nstride = data.nstride();
+    v.ncomp = data.nComp();
+    return v;
+}
+
+void CellArray::stage() { data.copyToHost(host); }
+void CellArray::commit() { data.copyFromHost(host); }
+
+// cellarray.cpp:5-18
+void CellArray::conservativeToPrimitive() {
+    tj_mf v = view();
+    tj_check(parameters, tj_c2p(dev(parameters).ctx, &v), "conservativeToPrimitive");
+}
+
+// cellarray.cpp:20-33
+void CellArray::primitiveToConservative() {
+    tj_mf v = view();
+    tj_check(parameters, tj_init_from_prims(dev(parameters).ctx, &v, 0), "primitiveToConservative");
+}
+
+// cellarray.cpp:35-48 (the CFL candidates it also feeds are re-armed by the
+// next getTimeStep)
+void CellArray::getSoundSpeed() {
+    tj_mf v = view();
+    tj_check(parameters, tj_post_step(dev(parameters).ctx, &v, TJ_OP_DT), "getSoundSpeed");
+}
+
+// cellarray.cpp:50-63
+void CellArray::cleanUpV() {
+    tj_mf v = view();
+    tj_check(parameters, tj_post_step(dev(parameters).ctx, &v, TJ_OP_CLEAN_V), "cleanUpV");
+}
+
+// cellarray.cpp:65-78
+void CellArray::cleanUpAlpha() {
+    tj_mf v = view();
+    tj_check(parameters, tj_post_step(dev(parameters).ctx, &v, TJ_OP_CLEAN_ALPHA), "cleanUpAlpha");
+}
+
+// cellarray.cpp:80-85: copy of every component
+void CellArray::operator=(CellArray& U) {
+    tj_mf a = U.view(), b = view();
+    tj_check(parameters, tj_lincomb(dev(parameters).ctx, &a, 1.0, nullptr, 0.0, &b), "CellArray::operator=");
+}
+
+// cellarray.cpp:87-92: scales in place and returns *this
+CellArray& CellArray::operator*(Real d) {
+    tj_mf a = view();
+    tj_check(parameters, tj_lincomb(dev(parameters).ctx, &a, d, nullptr, 0.0, &a), "CellArray::operator*");
+    return *this;
+}
+
+// cellarray.cpp:94-99: adds U in place and returns *this
+CellArray& CellArray::operator+(CellArray& U) {
+    tj_mf a = view(), b = U.view();
+    tj_check(parameters, tj_lincomb(dev(parameters).ctx, &a, 1.0, &b, 1.0, &a), "CellArray::operator+");
+    return *this;
+}
+
+// cellarray.cpp:101-117
+bool CellArray::contains_nan() {
+    tj_mf v = view();
+    tj_check(parameters, tj_post_step(dev(parameters).ctx, &v, TJ_OP_NANCHECK), "contains_nan");
+    return tenjin_host::checkStatus(parameters);
+}
+
+// ------------------------------------------------------------------ BoxAccessCellArray
+BoxAccessCellArray::BoxAccessCellArray(MFIter& mfi, const Box& bx, CellArray& U_)
+    : box(bx), U(U_), accessPattern(U_.accessPattern), numberOfMaterials(U_.numberOfMaterials) {
+    (void)mfi;
+    if (U.host.empty()) throw std::runtime_error("BoxAccessCellArray: stage() the CellArray first");
+}
+
+BoxAccessCellArray::BoxAccessCellArray(const Box& bx, CellArray& U_)
+    : box(bx), U(U_), accessPattern(U_.accessPattern), numberOfMaterials(U_.numberOfMaterials) {
+    if (U.host.empty()) throw std::runtime_error("BoxAccessCellArray: stage() the CellArray first");
+}
+
+// component of (var, mat, row, col) as in boxaccesscellarray.cpp:19-63:
+// per-material variables add mat, vectors add row, 3x3 tensors add 3 row + col
+Real& BoxAccessCellArray::operator()(int i, int j, int k, Variable var, int mat, int row, int col) {
+    const Box& d = U.data.domain();
+    int comp = accessPattern[var];
+    switch (var) {
+        case ALPHA: case ALPHARHO: case RHO_K: case EPSILON: case ALPHARHOEPSILON: comp += mat; break;
+        case RHOU: case VELOCITY: comp += row; break;
+        case SIGMA: case V_TENSOR: case VSTAR: case DEVH: comp += row * 3 + col; break;
+        case RHO: case TOTAL_E: case P: case SOUNDSPEED: case USTAR: case HJ2: break;
+        default: throw std::runtime_error("Incorrect Access variable " + std::to_string((int)var));
+    }
+    const long long cell = ((long long)(k - d.lo[2]) * U.data.kstride()) + (long long)(j - d.lo[1]) * U.data.jstride() +
+                           (i - d.lo[0]);
+    return U.host[(size_t)comp * (size_t)U.data.nstride() + (size_t)cell];
+}
+
+Real& BoxAccessCellArray::operator()(int i, int j, int k, int var, int mat, int row, int col) {
+    return (*this)(i, j, k, (Variable)var, mat, row, col);
+}
+
+// ------------------------------------------------------------------ TimeStep, PlasticEOS, advance
+// timestep.cpp:36-70: CFL * min over the valid cells of dx_d / (a + |u_d|)
+Real TimeStep::getTimeStep(CellArray& U, ParameterStruct& parameters) {
+    tj_mf v = U.view();
+    double dt = 0.0;
+    tj_check(parameters, tj_compute_dt(dev(parameters).ctx, &v, &dt), "getTimeStep");
+    tenjin_host::checkStatus(parameters);
+    return dt;
+}
+
+// plastic.cpp:227-244 (yield stresses come from the materials' EOS objects)
+void PlasticEOS::plasticUpdate(CellArray& U, ParameterStruct& parameters) {
+    tenjin_host::Device& d = dev(parameters);
+    for (size_t m = 0; m < yieldStress.size() && m < (size_t)d.desc.nmat; ++m)
+        if (yieldStress[m] != d.desc.mat[m].yield_stress)
+            throw std::runtime_error("plasticUpdate: PlasticEOS::yieldStress differs from the bound materials");
+    tj_mf v = U.view();
+    tj_check(parameters, tj_post_step(d.ctx, &v, TJ_OP_PLASTIC), "plasticUpdate");
+}
+
+// hllc.cpp:699-707: U1 = 1/2 U + 1/2 HLLCadvance(HLLCadvance(U)) with
+// parameters.dt; as in the reference, U holds the averaged state afterwards.
+void advance(CellArray& U, CellArray& U1, CellArray& U2, CellArray& MUSCLgrad, CellArray& UL, CellArray& UR,
+             CellArray& ULStar, CellArray& URStar, CellArray& UStarStar, Array<MultiFab, AMREX_SPACEDIM>& flux_arr,
+             Geometry const& geom, ParameterStruct& parameters, Vector<BCRec>& bc, THINCArray& THINC) {
+    (void)U2, (void)MUSCLgrad, (void)UL, (void)UR, (void)ULStar, (void)URStar, (void)UStarStar, (void)flux_arr;
+    (void)THINC;
+    if (!parameters.gpu) tenjin_host::bindLevel(parameters, geom, bc);
+    tj_ctx* ctx = parameters.gpu->ctx;
+    tj_check(parameters, tj_set_dt(ctx, parameters.dt, 0), "advance: dt");
+    tj_mf u = U.view(), u1 = U1.view();
+    tj_check(parameters, tj_rk_stage(ctx, 1, &u, &u, &u1, TJ_PREC_FP64), "advance: stage 1");
+    tj_check(parameters, tj_rk_stage(ctx, 2, &u1, &u, &u1, TJ_PREC_FP64), "advance: stage 2");
+    tj_check(parameters, tj_lincomb(ctx, &u1, 1.0, nullptr, 0.0, &u), "advance: U = U1");
+}
+
+// ------------------------------------------------------------------ binding
+namespace tenjin_host {
+
+Device::~Device() {
+    if (ctx) tj_destroy(ctx);
+}
+
+void bindLevel(ParameterStruct& parameters, const Geometry& geom, const Vector<BCRec>& bc, int device) {
+    auto d = std::make_shared<Device>();
+    tj_level_desc& L = d->desc;
+    L.dim = geom.dim();
+    for (int a = 0; a < 3; ++a) {
+        L.n[a] = a < L.dim ? geom.Domain().length(a) : 1;
+        L.L[a] = a < L.dim ? geom.ProbLength(a) : 1.0;
+        const int lo = bc.empty() ? BCType::foextrap : bc[0].lo(a);
+        const int hi = bc.empty() ? BCType::foextrap : bc[0].hi(a);
+        L.bc_lo[a] = lo == BCType::reflect_even || lo == BCType::reflect_odd;
+        L.bc_hi[a] = hi == BCType::reflect_even || hi == BCType::reflect_odd;
+    }
+    L.max_grid_size = parameters.max_grid_size;
+    L.SOLID = parameters.SOLID;
+    L.THINC = parameters.THINC;
+    L.PLASTIC = parameters.PLASTIC;
+    L.MUSCL = parameters.MUSCL;
+    L.THINCbeta = parameters.THINCbeta;
+    L.CFL = parameters.CFL;
+    L.nmat = parameters.numberOfMaterials;
+    if (L.nmat > TJ_MAX_MATERIALS) throw std::runtime_error("bindLevel: too many materials");
+    for (int m = 0; m < L.nmat; ++m) {
+        const MaterialDescriptor& md = parameters.materialInfo.at(m);
+        if (!md.EOS) throw std::runtime_error("bindLevel: material without EOS");
+        tj_material& t = L.mat[m];
+        t.eos = md.EOS->kind();
+        t.solid = md.phase == solid;
+        t.plastic = md.plastic;
+        t.gamma = md.EOS->adiabaticIndex;
+        t.pref = md.EOS->pref;
+        t.eref = md.EOS->eref;
+        t.CV = md.EOS->CV;
+        t.G0 = md.EOS->shearModulus;
+        t.yield_stress = md.EOS->yieldStress;
+        if (const UsUpSolidEOS* s = dynamic_cast<const UsUpSolidEOS*>(md.EOS)) {
+            t.rho0 = s->rho0;
+            t.c0 = s->c0;
+            t.s = s->s;
+        }
+    }
+    if (tj_create(&d->ctx, device) != TJ_OK) throw std::runtime_error("bindLevel: tj_create failed");
+    parameters.gpu = d;
+    tj_check(parameters, tj_bind_level(d->ctx, &L), "tj_bind_level");
+    for (int a = 0; a < L.dim; ++a) parameters.dx[a] = geom.CellSize(a);
+}
+
+void initialiseFromPrimitives(CellArray& U, bool shiftPressure) {
+    tj_mf v = U.view();
+    tj_check(U.parameters, tj_init_from_prims(dev(U.parameters).ctx, &v, shiftPressure ? 1 : 0),
+             "initialiseFromPrimitives");
+}
+
+bool checkStatus(ParameterStruct& parameters) {
+    uint32_t st = 0;
+    tj_check(parameters, tj_status(dev(parameters).ctx, &st, 1), "tj_status");
+    if (st & TJ_ST_PLASTIC_DIV) std::printf("Divide by zero error\n");  // plastic.cpp:191 (printed, not fatal)
+    if (st & TJ_ST_WAVESPEED) {
+        std::printf("Nan in Wavespeeds \n");  // hllc.cpp:169-199, 367-372
+        std::exit(1);
+    }
+    if (st & TJ_ST_ALPHA_NANS) {
+        std::printf("Error in scaling volume fractions, too many Nans\n");  // boxaccesscellarray.cpp:559-561
+        std::exit(1);
+    }
+    if (st & TJ_ST_ALPHA_SUM) {
+        std::printf("Error in scaling volume fractions\n");  // boxaccesscellarray.cpp:585-587
+        std::exit(1);
+    }
+    return (st & TJ_ST_NAN) != 0;
+}
+
+}  // namespace tenjin_host

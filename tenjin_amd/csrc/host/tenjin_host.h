@@ -1,0 +1,219 @@
+// tenjin_host.h -- the reference's hot-path C++ interface, forwarding to the
+// gfx950 library through its C ABI (include/tenjin.h).
+//
+// The names, argument lists and meanings follow the reference:
+//   advance                         hllc.cpp:699-707 (declared simulationheader.h:32)
+//   TimeStep::getTimeStep           timestep.cpp:36-70 (timestep.h:15-16)
+//   PlasticEOS::plasticUpdate       plastic.cpp:227-244 (plastic.h:18-19)
+//   CellArray::{primitiveToConservative, conservativeToPrimitive,
+//     getSoundSpeed, cleanUpV, cleanUpAlpha, operator=, operator*,
+//     operator+, contains_nan}     cellarray.cpp:5-117 (cellarray.h:24-33)
+//   ParameterStruct, MaterialDescriptor, Variable, Material_type
+//                                   structdefinitions.h:10-144
+//   AccessPattern                   accesspattern.cpp:44-79
+// Error behaviour is the reference's: the conditions on which it prints and
+// calls exit(1) print the same message and exit(1) here (checked whenever a
+// call synchronises with the GPU); a NaN makes contains_nan() return true.
+//
+// What differs, and why:
+//   * all cell data live on the GPU (amrex_lite::MultiFab); host access goes
+//     through BoxAccessCellArray over a host staging copy (stage/commit);
+//   * the scratch CellArrays (U2, UL, UR, MUSCLgrad, ULStar, URStar,
+//     UStarStar), the flux MultiFabs and the THINCArray are accepted for
+//     signature compatibility; the library keeps its own scratch, including
+//     the THINC history the reference keeps in ULStar/URStar;
+//   * the level must be bound once (tenjin_host::bindLevel) with the geometry
+//     and BC table before the first call; advance() binds on first use.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+
+#include "amrex_lite.h"
+#include "tenjin.h"
+
+using namespace amrex_lite;
+
+// structdefinitions.h:10-32 (numbering shared with tj_comp_start)
+enum Variable {
+    ALPHA, ALPHARHO, RHO_K, RHO, RHOU, TOTAL_E, VELOCITY, P, SOUNDSPEED, USTAR,
+    RHO_MIX, ALPHARHOLAMBDA, LAMBDA, SIGMA, V_TENSOR, VSTAR, DEVH, HJ2, EPSILON,
+    ALPHARHOEPSILON
+};
+constexpr int NUMBER_OF_VARIABLES = ALPHARHOEPSILON + 1;
+
+enum Material_type { solid, fluid };
+enum Direction_enum { x, y, z };
+
+// MieGruneisenEOS (equationofstate.h:31-60): stiffened gas / constant-Gamma
+// Mie-Gruneisen with reference pressure and energy.
+class MieGruneisenEOS {
+public:
+    MieGruneisenEOS(Real adiabaticIndex_, Real CV_, Real pref_, Real eref_)
+        : adiabaticIndex(adiabaticIndex_), CV(CV_), pref(pref_), eref(eref_) {}
+    virtual ~MieGruneisenEOS() = default;
+    virtual int kind() const { return 0; }
+    Real adiabaticIndex, CV, pref, eref;
+    Real shearModulus = 0.0;
+    Real yieldStress = 0.0;  // the reference keeps it in PlasticEOS::yieldStress
+};
+
+// Us-up Mie-Gruneisen solid (no reference class; the solid-EOS interface of
+// equationofstate.cpp:403-596 with the principal Hugoniot as the cold curve).
+class UsUpSolidEOS : public MieGruneisenEOS {
+public:
+    UsUpSolidEOS(Real rho0_, Real Gamma0, Real c0_, Real s_, Real G, Real Y)
+        : MieGruneisenEOS(Gamma0 + 1.0, 1.0, 0.0, 0.0), rho0(rho0_), c0(c0_), s(s_) {
+        shearModulus = G;
+        yieldStress = Y;
+    }
+    int kind() const override { return 1; }
+    Real rho0, c0, s;
+};
+
+struct MaterialDescriptor {
+    Material_type phase = fluid;
+    bool mixture = 0;
+    int mixtureIndex = 0;
+    int plastic = false;
+    MieGruneisenEOS* EOS = nullptr;
+};
+
+namespace tenjin_host {
+struct Device;
+}
+
+struct ParameterStruct {
+    Vector<Real> dimL = Vector<Real>(AMREX_SPACEDIM);
+    Vector<Real> dx = Vector<Real>(AMREX_SPACEDIM);
+    Vector<int> n_cells = Vector<int>(AMREX_SPACEDIM);
+
+    int SOLID = 0;
+    int THINC = 0;
+    int RADIAL = 0;
+    int PLASTIC = 0;
+    int REACTIVE = 0;
+    int MUSCL = 1;
+
+    int Ncomp = 0;
+    int Nghost = 2;
+    int numberOfMaterials = 0;
+    int numberOfMixtures = 0;
+
+    int max_grid_size = 64;
+
+    Real CFL = 0.5;
+    Real x0 = 0.0;
+    Real dt = 0.0;
+    Real THINCbeta = 1.6;
+
+    Vector<MaterialDescriptor> materialInfo;
+
+    // the GPU context bound to this level (created by tenjin_host::bindLevel)
+    std::shared_ptr<tenjin_host::Device> gpu;
+};
+
+// AccessPattern (accesspattern.cpp:44-79): first component of each variable.
+class AccessPattern {
+public:
+    explicit AccessPattern(ParameterStruct& parameters);
+    int& operator[](Variable var) { return data[var]; }
+    std::map<Variable, int> data;
+    Vector<std::string> variableNames;
+    int ncomp = 0;
+};
+
+class CellArray {
+public:
+    CellArray(BoxArray& ba, DistributionMapping& dm, const int Ncomp, const int Nghost, AccessPattern& _accessPattern,
+              ParameterStruct& parameters);
+
+    void primitiveToConservative();
+    void conservativeToPrimitive();
+    void getSoundSpeed();
+    void cleanUpV();
+    void cleanUpAlpha();
+
+    void operator=(CellArray& U);
+    CellArray& operator*(Real d);
+    CellArray& operator+(CellArray& U);
+    bool contains_nan();
+
+    MultiFab data;
+    AccessPattern& accessPattern;
+    int numberOfMaterials;
+
+    // host staging for BoxAccessCellArray: stage() copies the level to the
+    // host, commit() copies it back
+    void stage();
+    void commit();
+    Vector<Real> host;
+
+    tj_mf view();
+    ParameterStruct& parameters;
+};
+
+// Per-box accessor over a CellArray's host staging copy (cellarray.h:46-107:
+// operator()(i, j, k, Variable, mat, row, col) with i, j, k in level indices).
+class BoxAccessCellArray {
+public:
+    BoxAccessCellArray(MFIter& mfi, const Box& bx, CellArray& U);
+    BoxAccessCellArray(const Box& bx, CellArray& U);
+    const Box& box;
+    CellArray& U;
+    AccessPattern& accessPattern;
+    int numberOfMaterials;
+    Real& operator()(int i, int j, int k, Variable var, int mat = 0, int row = 0, int col = 0);
+    Real& operator()(int i, int j, int k, int var, int mat = 0, int row = 0, int col = 0);
+};
+
+// The reference's THINC flag array; the library keeps the THINC state itself.
+class THINCArray {
+public:
+    THINCArray() = default;
+    THINCArray(BoxArray&, DistributionMapping&, const int, ParameterStruct&) {}
+};
+
+class TimeStep {
+public:
+    TimeStep() = default;
+    TimeStep(BoxArray&, DistributionMapping&, const int, const int) {}
+    Real getTimeStep(CellArray& U, ParameterStruct& parameters);
+};
+
+class PlasticEOS {
+public:
+    PlasticEOS() = default;
+    void plasticUpdate(CellArray& U, ParameterStruct& parameters);
+    Vector<Real> yieldStress;
+};
+
+void advance(CellArray& U, CellArray& U1, CellArray& U2, CellArray& MUSCLgrad, CellArray& UL, CellArray& UR,
+             CellArray& ULStar, CellArray& URStar, CellArray& UStarStar, Array<MultiFab, AMREX_SPACEDIM>& flux_arr,
+             Geometry const& geom, ParameterStruct& parameters, Vector<BCRec>& bc, THINCArray& THINC);
+
+namespace tenjin_host {
+
+// One GPU context bound to one level.
+struct Device {
+    tj_ctx* ctx = nullptr;
+    tj_level_desc desc{};
+    ~Device();
+};
+
+// Binds the level described by parameters/geom/bc to a new GPU context on
+// HIP device `device` and stores it in parameters.gpu (once per run, after
+// the set-up of main.cpp:29-95 and setBoundaryConditions).
+void bindLevel(ParameterStruct& parameters, const Geometry& geom, const Vector<BCRec>& bc, int device = 0);
+
+// setInitialConditions' last steps (initialise.cpp:104-123): primitive
+// variables already in U -> thermal-to-total pressure shift (when
+// shiftPressure) and primitiveToConservative.
+void initialiseFromPrimitives(CellArray& U, bool shiftPressure);
+
+// The reference's "print and exit(1)" for fatal status bits; returns whether
+// the NaN bit was set (and clears the status word).
+bool checkStatus(ParameterStruct& parameters);
+
+}  // namespace tenjin_host

@@ -659,6 +659,23 @@ __global__ __launch_bounds__(256) void k_derive_fluid(Level lv, MF U) {
         for (int c = 0; c < 3; ++c) U.at(lv.cs[SIGMA] + 3 * r + c, i, j, k) = -p * (r == c ? 1.0 : 0.0);
 }
 
+// CellArray RK algebra (cellarray.cpp:80-99): Z = (X * a) + (Y * b) over
+// every component and valid cell, each product rounded before the sum as in
+// MultiFab::mult followed by MultiFab::Add.  Without Y: Z = X * a.
+__global__ __launch_bounds__(256) void k_lincomb(MF X, double a, MF Y, double b, int hasY, MF Z, int ncomp, int nx,
+                                                 int ny, int nz) {
+    const long long ncell = (long long)nx * ny * nz;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell * ncomp) return;
+    const int c = (int)(id / ncell);
+    const long long cell = id % ncell;
+    const int i = (int)(cell % nx);
+    const int j = (int)((cell / nx) % ny);
+    const int k = (int)(cell / ((long long)nx * ny));
+    const double xa = X.at(c, i, j, k) * a;
+    Z.at(c, i, j, k) = hasY ? xa + Y.at(c, i, j, k) * b : xa;
+}
+
 // zero-fill for scratch that a caller may capture in a graph (no memset nodes)
 __global__ void k_zero(double* p, long long n) {
     long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;

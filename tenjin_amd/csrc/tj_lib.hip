@@ -658,6 +658,21 @@ int tj_c2p(tj_ctx* c, tj_mf* U) {
     return TJ_OK;
 }
 
+int tj_lincomb(tj_ctx* c, const tj_mf* X, double a, const tj_mf* Y, double b, tj_mf* Z) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    int r;
+    if ((r = check_mf(c, X)) || (r = check_mf(c, Z))) return r;
+    if (Y && (r = check_mf(c, Y))) return r;
+    const int nc = std::min(X->ncomp, Z->ncomp);
+    if (Y && Y->ncomp < nc) { c->err = "lincomb: Y has fewer components than X and Z"; return TJ_EINVAL; }
+    const long long n = c->lv.ncells * nc;
+    const MF y = Y ? view(Y) : view(X);
+    hipLaunchKernelGGL(k_lincomb, dim3(grid1(n, 256)), dim3(256), 0, c->stream, view(X), a, y, b, Y ? 1 : 0, view(Z), nc,
+                       c->lv.n[0], c->lv.n[1], c->lv.n[2]);
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
 int tj_derive(tj_ctx* c, tj_mf* U) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;

@@ -1,0 +1,203 @@
+// Drives the reference's time loop (main.cpp:121-191) through the C++ host
+// interface (tenjin_host.h), on a problem read from a file written by the
+// tests: the level description, the primitive initial state, finalT and a step
+// cap.  Writes (t, n) and the final state of U1.
+//
+//   main_loop <input> <output>
+//
+// input : "TJIN" | int32 1 | tj_level_desc | double finalT | int32 max_steps
+//         | int32 shift_pressure | int64 ncomp | int64 ncells | ncomp*ncells doubles
+// output: "TJOU" | double t | int32 n | int32 nan_stop | int64 count | count doubles
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+
+#include "tenjin_host.h"
+
+namespace {
+
+template <class T>
+void get(std::ifstream& f, T& v) {
+    f.read(reinterpret_cast<char*>(&v), sizeof(T));
+    if (!f) throw std::runtime_error("main_loop: short input");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: main_loop <input> <output>\n");
+        return 2;
+    }
+    std::ifstream in(argv[1], std::ios::binary);
+    char magic[4];
+    in.read(magic, 4);
+    int32_t version = 0;
+    get(in, version);
+    if (std::memcmp(magic, "TJIN", 4) != 0 || version != 1) {
+        std::fprintf(stderr, "main_loop: bad input header\n");
+        return 2;
+    }
+    tj_level_desc desc;
+    get(in, desc);
+    double finalT;
+    int32_t max_steps, shift;
+    int64_t ncomp, ncells;
+    get(in, finalT);
+    get(in, max_steps);
+    get(in, shift);
+    get(in, ncomp);
+    get(in, ncells);
+    Vector<Real> prims((size_t)(ncomp * ncells));
+    in.read(reinterpret_cast<char*>(prims.data()), sizeof(Real) * prims.size());
+    if (!in) throw std::runtime_error("main_loop: short state");
+
+    // ---- set-up (main.cpp:29-119 with the settings of the input)
+    ParameterStruct parameters;
+    for (int d = 0; d < AMREX_SPACEDIM; ++d) {
+        parameters.n_cells[d] = desc.n[d];
+        parameters.dimL[d] = desc.L[d];
+    }
+    parameters.SOLID = desc.SOLID;
+    parameters.THINC = desc.THINC;
+    parameters.PLASTIC = desc.PLASTIC;
+    parameters.MUSCL = desc.MUSCL;
+    parameters.CFL = desc.CFL;
+    parameters.THINCbeta = desc.THINCbeta;
+    parameters.max_grid_size = desc.max_grid_size;
+    parameters.numberOfMaterials = desc.nmat;
+    std::vector<std::unique_ptr<MieGruneisenEOS>> eos;
+    for (int m = 0; m < desc.nmat; ++m) {
+        const tj_material& t = desc.mat[m];
+        if (t.eos == 1)
+            eos.emplace_back(new UsUpSolidEOS(t.rho0, t.gamma - 1.0, t.c0, t.s, t.G0, t.yield_stress));
+        else {
+            eos.emplace_back(new MieGruneisenEOS(t.gamma, t.CV, t.pref, t.eref));
+            eos.back()->shearModulus = t.G0;
+            eos.back()->yieldStress = t.yield_stress;
+        }
+        MaterialDescriptor md;
+        md.phase = t.solid ? solid : fluid;
+        md.plastic = t.plastic;
+        md.EOS = eos.back().get();
+        parameters.materialInfo.push_back(md);
+    }
+
+    Box domain(IntVect(0, 0, 0), IntVect(desc.n[0] - 1, desc.n[1] - 1, desc.n[2] - 1));
+    BoxArray ba(domain);
+    ba.maxSize(parameters.max_grid_size);
+    DistributionMapping dm(ba);
+    Geometry geom(domain, RealBox({0.0, 0.0, 0.0}, {desc.L[0], desc.L[1], desc.L[2]}), desc.dim);
+    Vector<BCRec> bc(1);
+    for (int d = 0; d < 3; ++d) {
+        bc[0].setLo(d, desc.bc_lo[d] ? BCType::reflect_even : BCType::foextrap);
+        bc[0].setHi(d, desc.bc_hi[d] ? BCType::reflect_even : BCType::foextrap);
+    }
+
+    AccessPattern accessPattern(parameters);
+    const int Ncomp = parameters.Ncomp;
+    const int Nghost = parameters.Nghost;
+    if (Ncomp != ncomp) {
+        std::fprintf(stderr, "main_loop: input has %lld components, layout %d\n", (long long)ncomp, Ncomp);
+        return 2;
+    }
+    CellArray U(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    CellArray U1(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    CellArray U2(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    CellArray UL(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    CellArray UR(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    CellArray MUSCLgrad(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    CellArray ULStar(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    CellArray URStar(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    CellArray UStarStar(ba, dm, Ncomp, Nghost, accessPattern, parameters);
+    Array<MultiFab, AMREX_SPACEDIM> flux_arr;
+    THINCArray THINCArr(ba, dm, Nghost, parameters);
+    TimeStep timeStep(ba, dm, AMREX_SPACEDIM, Nghost);
+    PlasticEOS plastic;
+
+    tenjin_host::bindLevel(parameters, geom, bc);
+
+    // initial conditions: the primitive variables cell by cell through the
+    // per-box accessor, then the pressure shift and primitiveToConservative
+    U1.stage();
+    const Box& dom = U1.data.domain();
+    const long long nc = U1.data.nstride();
+    for (MFIter mfi(U1.data); mfi.isValid(); ++mfi) {
+        const Box& bx = mfi.validbox();
+        BoxAccessCellArray baca(mfi, bx, U1);
+        for (int k = bx.smallEnd(2); k <= bx.bigEnd(2); ++k)
+            for (int j = bx.smallEnd(1); j <= bx.bigEnd(1); ++j)
+                for (int i = bx.smallEnd(0); i <= bx.bigEnd(0); ++i) {
+                    const long long cell = ((long long)(k - dom.lo[2]) * U1.data.kstride()) +
+                                           (long long)(j - dom.lo[1]) * U1.data.jstride() + (i - dom.lo[0]);
+                    for (int c = 0; c < Ncomp; ++c) {
+                        // the variable and (mat | row, col) of component c
+                        Variable var = ALPHA;
+                        int first = -1;
+                        for (auto& kv : accessPattern.data)
+                            if (kv.second <= c && kv.second > first) {
+                                first = kv.second;
+                                var = kv.first;
+                            }
+                        const int off = c - first;
+                        const bool tensor = var == SIGMA || var == V_TENSOR || var == VSTAR || var == DEVH;
+                        const bool vec = var == RHOU || var == VELOCITY;
+                        Real& ref = tensor ? baca(i, j, k, var, 0, off / 3, off % 3)
+                                           : (vec ? baca(i, j, k, var, 0, off) : baca(i, j, k, var, off));
+                        ref = prims[(size_t)c * (size_t)nc + (size_t)cell];
+                    }
+                }
+    }
+    U1.commit();
+    tenjin_host::initialiseFromPrimitives(U1, shift != 0);
+
+    // ---- the time loop of main.cpp:121-191
+    int n = 0;
+    Real t = 0.0;
+    int nan_stop = 0;
+    for (t = 0.0; t < finalT && n < max_steps; t += parameters.dt, n++) {
+        parameters.dt = timeStep.getTimeStep(U1, parameters);
+
+        if (t + parameters.dt > finalT) {
+            parameters.dt = finalT - t;
+        }
+        if (n < 5) {
+            parameters.dt *= 0.2;
+        }
+
+        U = U1;
+
+        advance(U, U1, U2, MUSCLgrad, UL, UR, ULStar, URStar, UStarStar, flux_arr, geom, parameters, bc, THINCArr);
+
+        if (parameters.PLASTIC) {
+            plastic.plasticUpdate(U1, parameters);
+        }
+        if (parameters.SOLID) {
+            U1.cleanUpV();
+        }
+        {
+            U1.cleanUpAlpha();
+        }
+        if (U1.contains_nan()) {
+            std::cout << "Nan found in U1" << std::endl;
+            nan_stop = 1;
+            break;
+        }
+    }
+
+    U1.stage();
+    std::ofstream out(argv[2], std::ios::binary);
+    out.write("TJOU", 4);
+    out.write(reinterpret_cast<const char*>(&t), sizeof t);
+    int32_t n32 = n, ns = nan_stop;
+    out.write(reinterpret_cast<const char*>(&n32), sizeof n32);
+    out.write(reinterpret_cast<const char*>(&ns), sizeof ns);
+    int64_t count = (int64_t)U1.host.size();
+    out.write(reinterpret_cast<const char*>(&count), sizeof count);
+    out.write(reinterpret_cast<const char*>(U1.host.data()), sizeof(Real) * U1.host.size());
+    std::printf("main_loop: n=%d t=%.17g\n", n, t);
+    return 0;
+}

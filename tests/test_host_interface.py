@@ -1,0 +1,62 @@
+"""The reference's C++ hot-path interface (tenjin_host.h: advance,
+TimeStep::getTimeStep, PlasticEOS::plasticUpdate, CellArray clean-ups and
+contains_nan, over amrex_lite types) driving the time loop of main.cpp:121-191
+from C++ (tests/host/main_loop.cpp), against the oracle's loop: same number of
+steps, same final time, bit-identical state."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from tests.parity_util import exact_fraction, max_rel
+
+TOL = 1e-12
+
+
+def test_driver_and_host_library_built():
+    from tests.host_driver import DRIVER
+
+    assert os.path.exists(DRIVER)
+    assert os.path.exists(os.path.join(os.path.dirname(DRIVER), "..", "..", "tenjin_amd", "libtenjin_host.so"))
+
+
+def _check(pb, final_time, max_steps, tmp_path):
+    from oracle.oracle import Oracle
+    from tests import host_driver
+
+    print(f"\n[case] host interface {pb.name} n={pb.n[:pb.dim]} thinc={pb.THINC} solid={pb.SOLID} "
+          f"final={final_time} max_steps={max_steps}", flush=True)
+    o = Oracle(pb)
+    n_o, t_o = o.run(final_time, max_steps)
+    t_g, n_g, nan_stop, sg = host_driver.run(pb, final_time, max_steps, str(tmp_path))
+    assert nan_stop == 0
+    assert (n_g, t_g) == (n_o, t_o)
+    so = o.get()
+    assert max_rel(pb, so, sg) <= TOL
+    assert exact_fraction(pb, so, sg) == 1.0
+
+
+@pytest.mark.gpu
+def test_host_interface_shock_tube_full_run(tmp_path):
+    from tenjin_amd import configs
+
+    pb = configs.shock_tube_1d()
+    _check(pb, pb.final_time, 1 << 30, tmp_path)
+
+
+@pytest.mark.gpu
+def test_host_interface_inclusion_2d_thinc(tmp_path):
+    from tenjin_amd import configs
+
+    pb = configs.shock_inclusion_2d(nx=128, ny=64, box=32, thinc=1, shock_x=0.55, drift=(150.0, 60.0))
+    _check(pb, 1.0, 25, tmp_path)
+
+
+@pytest.mark.gpu
+def test_host_interface_plate_impact_2d(tmp_path):
+    from tenjin_amd import configs
+
+    pb = configs.plate_impact(dim=2, n=48, box=24)
+    _check(pb, 1.0, 10, tmp_path)
