@@ -42,6 +42,12 @@ FLUID_KERNEL_BYTES = {
     # comps in, x faces in (9 comps), star alphas out, 15 comps out
     "flux_y_update1": 112 + 72 + 16 + 120,
     "flux_y_update2_post": 112 + 72 + 16 + 120 + 120 + 8,  # + base in, + sound speed out
+    # mixed form: FP32 flux planes (9 comps x 4 B), FP64 state
+    "flux_x_mixed": 64 + 36 + 16,
+    "flux_y_mixed": 64 + 36 + 16,
+    "flux_z_mixed": 64 + 36,
+    "update_stage1_mixed": 64 + 2 * 36 + 120,
+    "update_stage2_post_mixed": 64 + 2 * 36 + 120 + 120 + 8,
 }
 SOLID_KERNEL_BYTES = {
     # 19 primitives in (alpha, rho_k, epsilon, u, p, V), 23 face comps out
@@ -53,6 +59,12 @@ SOLID_KERNEL_BYTES = {
     "update_stage1": 152 + 3 * 184 + 232,
     # + 29 base comps in (RK average), + sound speed out
     "update_stage2_post": 152 + 3 * 184 + 232 + 232 + 8,
+    # mixed form: FP32 flux planes (23 comps x 4 B), FP64 state
+    "flux_x_mixed": 152 + 92,
+    "flux_y_mixed": 152 + 92,
+    "flux_z_mixed": 152 + 92,
+    "update_stage1_mixed": 152 + 3 * 92 + 232,
+    "update_stage2_post_mixed": 152 + 3 * 92 + 232 + 232 + 8,
 }
 # 8 (3 N_s + 2 N_c): N_s/N_c = 14/8 (two-material fluid), 27/19 (two-material solid)
 STEP_BYTES = {"inclusion2d": 464, "plate3d": 952}
@@ -65,7 +77,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 500 for inclusion2d, 40 for plate3d)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--prec", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--prec", choices=["fp64", "mixed", "both"], default="both",
+                    help="both: FP64 is the headline value, the mixed form is timed beside it with its L1 error")
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--n3", type=int, default=256, help="plate3d: cells per side")
@@ -153,47 +166,105 @@ def main():
         kbytes = SOLID_KERNEL_BYTES
         workload = (f"3D elastoplastic Cu/Al plate impact, {args.n3}^3 cells, {pb.max_grid_size}^3 boxes, "
                     "Us-up Mie-Gruneisen, MUSCL+5-wave HLLC, perfect plasticity, CFL 0.5")
-    prec = TJ_PREC_FP64 if args.prec == "fp64" else TJ_PREC_MIXED
-    g = GpuLevel(pb)
-    g.set_tuning(args.seglen, 1 if args.fuse else 0)
-    g.init_from_prims(pb.prims, pb.shift_pressure)
-    g.begin_run(float("inf"))
-    g.advance(args.warmup, prec)
-    g.sync()
-
-    g.prof_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    g.advance(args.steps, prec)
-    g.sync()
-    t1 = time.perf_counter()
-    prof = g.prof_report()
-    g.prof_enable(False)
-    status = g.status()
-    _, _, nsteps_done = g.clock()
-    elapsed = t1 - t0
-
     ncell = pb.ncells
-    value = ncell * args.steps / elapsed / 1e6
-    ms_per_step = 1e3 * elapsed / args.steps
 
-    dom = max(prof.items(), key=lambda kv: kv[1][1])
-    dname, (dcount, dms) = dom
-    davg_s = dms / dcount / 1e3
-    dbytes = kbytes[dname] * ncell
-    achieved = dbytes / davg_s / 1e9
-    roofline = {
-        "bound": "hbm",
-        "kernel": dname,
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": read_traffic(args.workload, dname),
-        "algorithmic_bytes_per_launch": dbytes,
-        "avg_launch_ms": round(davg_s * 1e3, 4),
-    }
-    kernel_ms = {k: round(v[1] / v[0], 4) for k, v in prof.items()}
+    def timed(prec):
+        """warm-up steps, then exactly args.steps timed steps of the device loop"""
+        g = GpuLevel(pb)
+        g.set_tuning(args.seglen, 1 if args.fuse else 0)
+        g.init_from_prims(pb.prims, pb.shift_pressure)
+        g.begin_run(float("inf"))
+        g.advance(args.warmup, prec)
+        g.sync()
+        g.prof_enable(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.advance(args.steps, prec)
+        g.sync()
+        t1 = time.perf_counter()
+        prof = g.prof_report()
+        g.prof_enable(False)
+        status = g.status()
+        t_end, _, nsteps_done = g.clock()
+        el = t1 - t0
+        dom = max(prof.items(), key=lambda kv: kv[1][1])
+        dname, (dcount, dms) = dom
+        davg_s = dms / dcount / 1e3
+        dbytes = kbytes[dname] * ncell
+        achieved = dbytes / davg_s / 1e9
+        res = {
+            "value": ncell * args.steps / el / 1e6,
+            "ms_per_step": 1e3 * el / args.steps,
+            "status": status,
+            "device_steps": nsteps_done,
+            "t_end": t_end,
+            "kernel_avg_ms": {k: round(v[1] / v[0], 4) for k, v in prof.items()},
+            "roofline": {
+                "bound": "hbm",
+                "kernel": dname,
+                "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(args.workload, dname),
+                "algorithmic_bytes_per_launch": dbytes,
+                "avg_launch_ms": round(davg_s * 1e3, 4),
+            },
+        }
+        return g, res
+
+    def rel_l1(a, b):
+        """relative L1 error of density, pressure and the first volume fraction"""
+        L = pb.layout
+        out = {}
+        for name, c in (("rho", L.comp(configs.RHO)), ("p", L.comp(configs.P)), ("alpha", L.comp(configs.ALPHA, 0))):
+            x, y = a[c].double(), b[c].double()
+            out[name] = float((x - y).abs().sum() / y.abs().sum().clamp_min(1e-300))
+        return out
+
+    mixed = None
+    if args.prec in ("fp64", "both"):
+        g, main_res = timed(TJ_PREC_FP64)
+        prec = TJ_PREC_FP64
+        if args.prec == "both":
+            ref_state = g.S.clone()
+            t_ref, _, n_ref = g.clock()
+            del g
+            torch.cuda.empty_cache()
+            gm, mres = timed(TJ_PREC_MIXED)
+            del gm
+            torch.cuda.empty_cache()
+            # accuracy: the mixed form from the same initial state to the FP64 run's final time
+            ge = GpuLevel(pb)
+            ge.init_from_prims(pb.prims, pb.shift_pressure)
+            n_mix, t_mix = ge.run(t_ref, max_steps=n_ref + 1000, prec=TJ_PREC_MIXED, check_every=64)
+            errs = rel_l1(ge.S, ref_state)
+            del ge, ref_state
+            torch.cuda.empty_cache()
+            mixed = {
+                "value": round(mres["value"], 2),
+                "ms_per_step": round(mres["ms_per_step"], 4),
+                "speedup_vs_fp64": round(mres["value"] / main_res["value"], 3),
+                "dtype": "f32-flux/f64-update",
+                "l1_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+                "compared_at": {"t": t_ref, "fp64_steps": n_ref, "mixed_steps": n_mix, "mixed_t": t_mix},
+                "roofline": mres["roofline"],
+                "kernel_avg_ms": mres["kernel_avg_ms"],
+                "status": mres["status"],
+            }
+        else:
+            del g
+    else:
+        g, main_res = timed(TJ_PREC_MIXED)
+        prec = TJ_PREC_MIXED
+        del g
+
+    value = main_res["value"]
+    ms_per_step = main_res["ms_per_step"]
+    roofline = main_res["roofline"]
+    kernel_ms = main_res["kernel_avg_ms"]
+    status = main_res["status"]
+    nsteps_done = main_res["device_steps"]
     step_bytes = STEP_BYTES[args.workload]
     step_gbs = value * 1e6 * step_bytes / 1e9
 
@@ -213,7 +284,7 @@ def main():
         "config": {
             "workload": workload,
             "cells": ncell,
-            "precision": args.prec,
+            "precision": "fp64" if prec == TJ_PREC_FP64 else "mixed",
             "fused_update": bool(args.fuse),
             "seglen": args.seglen,
             "parallelism": "single GPU",
@@ -230,7 +301,8 @@ def main():
         "status": status,
         "device_steps": nsteps_done,
     }
-    del g
+    if mixed is not None:
+        out["mixed"] = mixed
     if not args.no_cpu:
         if args.workload == "inclusion2d":
             out["cpu_baseline"] = cpu_baseline(pb, args.cpu_seconds, 20)

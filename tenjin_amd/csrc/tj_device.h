@@ -15,13 +15,16 @@ constexpr int MAXM = 4;
 // Per-material constants.  Gg, gp1 and xi are the values the reference
 // recomputes on every call (GruneisenGamma, GruneisenGamma + 1.0,
 // 1.0 / GruneisenGamma in equationofstate.cpp:47-75); they are evaluated once
-// on the host with the same operations, so they are bit-identical.
-struct MatC {
+// on the host with the same operations, so they are bit-identical.  The
+// mixed-precision form reads an FP32 copy (MatT<float>) in its FP32 parts.
+template <typename R>
+struct MatT {
     int eos, solid, plastic, pad;
-    double Gg, gp1, xi;
-    double pref, eref, CV;
-    double rho0, c0, s, G0, yield;
+    R Gg, gp1, xi;
+    R pref, eref, CV;
+    R rho0, c0, s, G0, yield;
 };
+using MatC = MatT<double>;
 
 // Variable numbering of structdefinitions.h:10-32
 enum Var {
@@ -41,9 +44,17 @@ struct Level {
     int bc_lo[3], bc_hi[3];
     const uint8_t* edge[3];  // per coordinate: bit0 = first cell of a box, bit1 = last cell of a box
     MatC m[MAXM];
+    MatT<float> mf[MAXM];  // FP32 copy of m for the mixed-precision form
     int cs[NVAR];      // first component of each Variable (AccessPattern), -1 if absent
     long long ncells;
 };
+
+template <typename R>
+__device__ __forceinline__ const MatT<R>& mat(const Level& lv, int m);
+template <>
+__device__ __forceinline__ const MatT<double>& mat<double>(const Level& lv, int m) { return lv.m[m]; }
+template <>
+__device__ __forceinline__ const MatT<float>& mat<float>(const Level& lv, int m) { return lv.mf[m]; }
 
 // Caller-owned CellArray view (component-major, i fastest).
 struct MF {
@@ -55,15 +66,18 @@ struct MF {
 };
 
 // Face-centred flux planes of one direction: lower face of cell (i,j,k),
-// extent n_d + 1 along d; component-major.
-struct Faces {
-    double* p;
+// extent n_d + 1 along d; component-major.  FP64, or FP32 in the
+// mixed-precision form.
+template <typename T>
+struct FacesT {
+    T* p;
     int nx, ny, nz;     // face-grid extents
     long long cs;       // component stride
-    __device__ __forceinline__ double& at(int c, int i, int j, int k) const {
+    __device__ __forceinline__ T& at(int c, int i, int j, int k) const {
         return p[(long long)c * cs + ((long long)k * ny + j) * nx + i];
     }
 };
+using Faces = FacesT<double>;
 
 // THINC/BVD history planes (faithful form of the reference's stale
 // ULStar/URStar reads, thinc.cpp:57-63 with hllc.cpp:404-406):
@@ -85,6 +99,23 @@ struct Hist {
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ double sgnd(double v) { return (double)((0.0 < v) - (v < 0.0)); }
+
+// Real-type generic forms used by the reconstruction and flux code, which runs
+// in FP64 (R = double: the same operations as above) or FP32 (mixed form).
+template <typename R>
+__device__ __forceinline__ R tmin(R a, R b) { return (b < a) ? b : a; }
+template <typename R>
+__device__ __forceinline__ R tmax(R a, R b) { return (a < b) ? b : a; }
+template <typename R>
+__device__ __forceinline__ R tsgn(R v) { return (R)((R(0) < v) - (v < R(0))); }
+__device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ bool tisnan(double x) { return __builtin_isnan(x); }
+__device__ __forceinline__ bool tisnan(float x) { return __builtin_isnan(x); }
+__device__ __forceinline__ bool tisinf(double x) { return __builtin_isinf(x); }
+__device__ __forceinline__ bool tisinf(float x) { return __builtin_isinf(x); }
 
 // ------------------------------------------------------------------ math
 // Project-written exp/expm1/tanh/cosh/x^(-1/3): Cody-Waite reduction with the
@@ -158,6 +189,15 @@ __device__ inline double pm_rcbrt(double x) {
     c = c - (c - x / (c * c)) / 3.0;
     return 1.0 / c;
 }
+
+// exp/tanh/cosh by real type: the project-written FP64 routines above, and
+// the device library's single-precision functions for the mixed form.
+__device__ __forceinline__ double texp(double x) { return pm_exp(x); }
+__device__ __forceinline__ double ttanh(double x) { return pm_tanh(x); }
+__device__ __forceinline__ double tcosh(double x) { return pm_cosh(x); }
+__device__ __forceinline__ float texp(float x) { return expf(x); }
+__device__ __forceinline__ float ttanh(float x) { return tanhf(x); }
+__device__ __forceinline__ float tcosh(float x) { return coshf(x); }
 
 // ------------------------------------------------------------------ boundaries
 // boundary.cpp:3-42: transmissive -> foextrap (nearest valid cell),

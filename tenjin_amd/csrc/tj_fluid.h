@@ -6,22 +6,24 @@
 namespace tj {
 
 // ------------------------------------------------------------------ states
-template <int NM>
+// R: the real type of reconstruction and flux work -- double in the FP64
+// form, float in the mixed-precision form.
+template <int NM, typename R = double>
 struct Prim {  // primitive variables reconstructed by MUSCL (accesspattern.cpp:44-79)
-    double a[NM], r[NM], u[3], p;
+    R a[NM], r[NM], u[3], p;
 };
 
-template <int NM>
+template <int NM, typename R = double>
 struct FSt {  // face state after primitiveToConservative + getSoundSpeed
-    double al[NM], ar[NM], rho, u[3], E, p, a;
+    R al[NM], ar[NM], rho, u[3], E, p, a;
 };
 
 // α-reconstruction of one cell for THINC/BVD (thinc.cpp:67-186)
-template <int NM>
+template <int NM, typename R = double>
 struct ARec {
-    double ula[NM], ura[NM];  // MUSCL face alphas
-    double thl[NM], thr[NM];  // THINC face alphas (valid where mixed)
-    int mixed;                // bit m: mixedCell for material m
+    R ula[NM], ura[NM];  // MUSCL face alphas
+    R thl[NM], thr[NM];  // THINC face alphas (valid where mixed)
+    int mixed;           // bit m: mixedCell for material m
 };
 
 // ------------------------------------------------------------------ loads
@@ -31,47 +33,51 @@ __device__ __forceinline__ void map3(const Level& lv, int& i, int& j, int& k, bo
     if (lv.dim > 2) k = bcmap(k, lv.n[2], lv.bc_lo[2], lv.bc_hi[2], f[2]); else f[2] = false;
 }
 
-template <int NM>
-__device__ __forceinline__ void load_prim(const Level& lv, const MF& U, int i, int j, int k, Prim<NM>& P) {
+template <int NM, typename R>
+__device__ __forceinline__ void load_prim(const Level& lv, const MF& U, int i, int j, int k, Prim<NM, R>& P) {
     bool f[3];
     map3(lv, i, j, k, f);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        P.a[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
-        P.r[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
+        P.a[m] = (R)U.at(lv.cs[ALPHA] + m, i, j, k);
+        P.r[m] = (R)U.at(lv.cs[RHO_K] + m, i, j, k);
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        double v = U.at(lv.cs[VELOCITY] + r, i, j, k);
+        R v = (R)U.at(lv.cs[VELOCITY] + r, i, j, k);
         P.u[r] = f[r] ? -v : v;
     }
-    P.p = U.at(lv.cs[PRESSURE], i, j, k);
+    P.p = (R)U.at(lv.cs[PRESSURE], i, j, k);
 }
 
-__device__ __forceinline__ double load_alpha(const Level& lv, const MF& U, int m, int i, int j, int k) {
+template <typename R = double>
+__device__ __forceinline__ R load_alpha(const Level& lv, const MF& U, int m, int i, int j, int k) {
     bool f[3];
     map3(lv, i, j, k, f);
-    return U.at(lv.cs[ALPHA] + m, i, j, k);
+    return (R)U.at(lv.cs[ALPHA] + m, i, j, k);
 }
 
 // ------------------------------------------------------------------ MUSCL
 // hllc.cpp:527-575.  min(2/(1+r), 2r/(1+r)) is evaluated with one division:
 // both quotients share the divisor, and 2r < 2 exactly when r < 1, so the
 // smaller quotient is the one with the smaller numerator (ties are equal).
-__device__ __forceinline__ double vanleer(double r) {
-    if (r <= 0.0) return 0.0;
-    double num = (r < 1.0) ? 2.0 * r : 2.0;
-    return num / (1.0 + r);
+template <typename R>
+__device__ __forceinline__ R vanleer(R r) {
+    if (r <= R(0)) return R(0);
+    R num = (r < R(1)) ? R(2) * r : R(2);
+    return num / (R(1) + r);
 }
-__device__ __forceinline__ void muscl1(double um, double uc, double up, double& L, double& R) {
-    double r = (uc - um) / (up - uc);
-    if (__builtin_isinf(r) || __builtin_isnan(r)) r = 0.0;
-    double g = vanleer(r) * 0.5 * (up - um);
-    L = uc - 0.5 * g;
-    R = uc + 0.5 * g;
+template <typename R>
+__device__ __forceinline__ void muscl1(R um, R uc, R up, R& L, R& Rt) {
+    R r = (uc - um) / (up - uc);
+    if (tisinf(r) || tisnan(r)) r = R(0);
+    R g = vanleer(r) * R(0.5) * (up - um);
+    L = uc - R(0.5) * g;
+    Rt = uc + R(0.5) * g;
 }
-template <int NM>
-__device__ __forceinline__ void muscl(const Prim<NM>& m_, const Prim<NM>& c, const Prim<NM>& p_, Prim<NM>& L, Prim<NM>& R) {
+template <int NM, typename Rt>
+__device__ __forceinline__ void muscl(const Prim<NM, Rt>& m_, const Prim<NM, Rt>& c, const Prim<NM, Rt>& p_,
+                                      Prim<NM, Rt>& L, Prim<NM, Rt>& R) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) muscl1(m_.a[m], c.a[m], p_.a[m], L.a[m], R.a[m]);
 #pragma unroll
@@ -83,51 +89,51 @@ __device__ __forceinline__ void muscl(const Prim<NM>& m_, const Prim<NM>& c, con
 
 // ------------------------------------------------------------------ EOS (kind 0)
 // equationofstate.cpp:37-64 and boxaccesscellarray.cpp:218-252.
-template <int NM>
-__device__ __forceinline__ void eos_sums(const Level& lv, const double* a, const double* ar, double& G, double& Ent, double& Pnt) {
-    G = 0.0;
-    Ent = 0.0;
-    Pnt = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ void eos_sums(const Level& lv, const R* a, const R* ar, R& G, R& Ent, R& Pnt) {
+    G = R(0);
+    Ent = R(0);
+    Pnt = R(0);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) G += a[m] / lv.m[m].Gg;
+    for (int m = 0; m < NM; ++m) G += a[m] / mat<R>(lv, m).Gg;
 #pragma unroll
-    for (int m = 0; m < NM; ++m) Ent += (lv.m[m].eref * ar[m] + 0.0 * a[m]);
+    for (int m = 0; m < NM; ++m) Ent += (mat<R>(lv, m).eref * ar[m] + R(0) * a[m]);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) Pnt += lv.m[m].pref * a[m] / lv.m[m].Gg + 0.0 * a[m];
+    for (int m = 0; m < NM; ++m) Pnt += mat<R>(lv, m).pref * a[m] / mat<R>(lv, m).Gg + R(0) * a[m];
 }
 
 // primitiveToConservative + getSoundSpeed of one reconstructed state
 // (boxaccesscellarray.cpp:124-174, 254-285).
-template <int NM>
-__device__ __forceinline__ void p2c_sound(const Level& lv, const Prim<NM>& P, FSt<NM>& S) {
-    double rho = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ void p2c_sound(const Level& lv, const Prim<NM, R>& P, FSt<NM, R>& S) {
+    R rho = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         S.al[m] = P.a[m];
         S.ar[m] = P.a[m] * P.r[m];
         rho += S.ar[m];
     }
-    double ke = 0.0;
+    R ke = R(0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         S.u[q] = P.u[q];
-        ke += 0.5 * rho * P.u[q] * P.u[q];
+        ke += R(0.5) * rho * P.u[q] * P.u[q];
     }
-    double G, Ent, Pnt;
-    eos_sums<NM>(lv, S.al, S.ar, G, Ent, Pnt);
+    R G, Ent, Pnt;
+    eos_sums<NM, R>(lv, S.al, S.ar, G, Ent, Pnt);
     S.E = P.p * G + Ent - Pnt + ke;
     S.p = P.p;
     S.rho = rho;
-    double a = 0.0, xt = 0.0;
+    R a = R(0), xt = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        const MatC& M = lv.m[m];
-        double c2 = P.p * M.gp1 / P.r[m] - M.pref / P.r[m];
-        a += dmax(0.0, M.xi * c2 * S.ar[m] / rho);
+        const MatT<R>& M = mat<R>(lv, m);
+        R c2 = P.p * M.gp1 / P.r[m] - M.pref / P.r[m];
+        a += tmax(R(0), M.xi * c2 * S.ar[m] / rho);
         xt += M.xi * S.al[m];
     }
-    if (a <= 0.0) a = 1E-6;
-    S.a = __builtin_sqrt(a / xt);
+    if (a <= R(0)) a = R(1E-6);
+    S.a = tsqrt(a / xt);
 }
 
 // conservativeToPrimitive (boxaccesscellarray.cpp:67-122), fluid part.
@@ -147,7 +153,7 @@ __device__ __forceinline__ void c2p(const Level& lv, const double* al, const dou
         ke += 0.5 * rho * u[q] * u[q];
     }
     double G, Ent, Pnt;
-    eos_sums<NM>(lv, al, ar, G, Ent, Pnt);
+    eos_sums<NM, double>(lv, al, ar, G, Ent, Pnt);
     p = (E - ke - Ent + Pnt) / G;
 }
 
@@ -169,132 +175,133 @@ __device__ __forceinline__ double sound(const Level& lv, const double* al, const
 // ------------------------------------------------------------------ HLLC
 // hllc.cpp:5-73 (getSstar, getStarState), 384-487 (calc_fluxes), flux.cpp:5-47.
 // F layout: [0,NM) alpha, [NM,2NM) alphaRho, [2NM,2NM+3) rhoU, 2NM+3 E, 2NM+4 S*.
-template <int NM>
-__device__ __forceinline__ void hllc(const FSt<NM>& L, const FSt<NM>& R, int d, double* F, double* sa) {
-    const double sigL = -L.p * 1.0, sigR = -R.p * 1.0;  // sigma_dd = -p delta_dd
-    double ruL[3], ruR[3];
+template <int NM, typename R>
+__device__ __forceinline__ void hllc(const FSt<NM, R>& L, const FSt<NM, R>& Rs, int d, R* F, R* sa) {
+    const FSt<NM, R>& Rr = Rs;
+    const R sigL = -L.p * R(1), sigR = -Rr.p * R(1);  // sigma_dd = -p delta_dd
+    R ruL[3], ruR[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         ruL[q] = L.u[q] * L.rho;
-        ruR[q] = R.u[q] * R.rho;
+        ruR[q] = Rr.u[q] * Rr.rho;
     }
-    const double SR = dmax(__builtin_fabs(L.u[d]) + L.a, __builtin_fabs(R.u[d]) + R.a);
-    const double SL = -SR;
-    const double Ss = (-sigR + sigL + ruL[d] * (SL - L.u[d]) - ruR[d] * (SR - R.u[d])) /
-                      (L.rho * (SL - L.u[d]) - R.rho * (SR - R.u[d]));
+    const R SR = tmax(tabs(L.u[d]) + L.a, tabs(Rr.u[d]) + Rr.a);
+    const R SL = -SR;
+    const R Ss = (-sigR + sigL + ruL[d] * (SL - L.u[d]) - ruR[d] * (SR - Rr.u[d])) /
+                 (L.rho * (SL - L.u[d]) - Rr.rho * (SR - Rr.u[d]));
     F[2 * NM + 4] = Ss;
     // branch selection of calc_fluxes; K is the state the star state is built from
-    const bool useL = (SL >= 0.0) || (Ss >= 0.0);
-    const bool addStar = !(SL >= 0.0) && ((Ss >= 0.0) || (SR >= 0.0));
-    const FSt<NM>& K = useL ? L : R;
-    const double* ruK = useL ? ruL : ruR;
-    const double SK = useL ? SL : SR;
-    const double sigK = useL ? sigL : sigR;
+    const bool useL = (SL >= R(0)) || (Ss >= R(0));
+    const bool addStar = !(SL >= R(0)) && ((Ss >= R(0)) || (SR >= R(0)));
+    const FSt<NM, R>& K = useL ? L : Rr;
+    const R* ruK = useL ? ruL : ruR;
+    const R SK = useL ? SL : SR;
+    const R sigK = useL ? sigL : sigR;
     // star state
-    const double mult = (__builtin_fabs((Ss - K.u[d]) / Ss) < 1E-20) ? 1.0 : (SK - K.u[d]) / (SK - Ss);
-    double arS[NM], rhoS = 0.0;
+    const R mult = (tabs((Ss - K.u[d]) / Ss) < R(1E-20)) ? R(1) : (SK - K.u[d]) / (SK - Ss);
+    R arS[NM], rhoS = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         arS[m] = mult * K.ar[m];
         rhoS += arS[m];
     }
-    double ruS[3];
+    R ruS[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) ruS[q] = (q == d) ? mult * K.rho * Ss : mult * ruK[q];
-    const double uSd = ruS[d] / rhoS;
-    const double ES = mult * K.rho * (K.E / K.rho + (Ss - K.u[d]) * (Ss - (sigK) / (K.rho * (SK - K.u[d]))));
+    const R uSd = ruS[d] / rhoS;
+    const R ES = mult * K.rho * (K.E / K.rho + (Ss - K.u[d]) * (Ss - (sigK) / (K.rho * (SK - K.u[d]))));
     // physical flux of K (flux.cpp:33-47) with sigma = -p I
-    double vds = 0.0;
+    R vds = R(0);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) vds += K.u[q] * (-K.p * (q == d ? 1.0 : 0.0));
+    for (int q = 0; q < 3; ++q) vds += K.u[q] * (-K.p * (q == d ? R(1) : R(0)));
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         F[m] = K.al[m] * uSd;
         sa[m] = K.al[m];
-        double f = K.ar[m] * K.u[d];
+        R f = K.ar[m] * K.u[d];
         F[NM + m] = addStar ? f + SK * (arS[m] - K.ar[m]) : f;
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        double f = ruK[q] * K.u[d] - (-K.p * (q == d ? 1.0 : 0.0));
+        R f = ruK[q] * K.u[d] - (-K.p * (q == d ? R(1) : R(0)));
         F[2 * NM + q] = addStar ? f + SK * (ruS[q] - ruK[q]) : f;
     }
     {
-        double f = K.E * K.u[d] - vds;
+        R f = K.E * K.u[d] - vds;
         F[2 * NM + 3] = addStar ? f + SK * (ES - K.E) : f;
     }
 }
 
-template <int NM>
-__device__ __forceinline__ void flip_state(FSt<NM>& S, int d) { S.u[d] = -S.u[d]; }
+template <int NM, typename R>
+__device__ __forceinline__ void flip_state(FSt<NM, R>& S, int d) { S.u[d] = -S.u[d]; }
 
 // ------------------------------------------------------------------ THINC
 // thinc.cpp:54-88 (mixedCell, Youngs normal in 2D), 130-186 (pass 1).
 // cell (i,j,k) along direction d; alphas are read from U with the boundary map.
-template <int NM>
-__device__ inline void arec(const Level& lv, const MF& U, int i, int j, int k, int d, double ynx, double yny,
-                            ARec<NM>& A) {
+template <int NM, typename R>
+__device__ inline void arec(const Level& lv, const MF& U, int i, int j, int k, int d, R ynx, R yny, ARec<NM, R>& A) {
     int di = d == 0, dj = d == 1, dk = d == 2;
     A.mixed = 0;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        const double aL = load_alpha(lv, U, m, i - di, j - dj, k - dk);
-        const double aC = load_alpha(lv, U, m, i, j, k);
-        const double aR = load_alpha(lv, U, m, i + di, j + dj, k + dk);
+        const R aL = load_alpha<R>(lv, U, m, i - di, j - dj, k - dk);
+        const R aC = load_alpha<R>(lv, U, m, i, j, k);
+        const R aR = load_alpha<R>(lv, U, m, i + di, j + dj, k + dk);
         muscl1(aL, aC, aR, A.ula[m], A.ura[m]);
-        const double mn = dmin(aL, aR);
-        const double mx = dmax(aL, aR) - mn;
-        const double theta = sgnd(aR - aL);
-        const double eps = 1E-20;
-        const double C = (mx < eps) ? 0.0 : (aC - mn + eps) / (mx + eps);
-        const double dl = 1E-4;
-        const bool mixed = (C > dl) && (C < (0.999999 - dl)) && ((aR - aC) * (aC - aL) >= 0.0);
-        A.thl[m] = 0.0;
-        A.thr[m] = 0.0;
+        const R mn = tmin(aL, aR);
+        const R mx = tmax(aL, aR) - mn;
+        const R theta = tsgn(aR - aL);
+        const R eps = R(1E-20);
+        const R C = (mx < eps) ? R(0) : (aC - mn + eps) / (mx + eps);
+        const R dl = R(1E-4);
+        const bool mixed = (C > dl) && (C < (R(0.999999) - dl)) && ((aR - aC) * (aC - aL) >= R(0));
+        A.thl[m] = R(0);
+        A.thr[m] = R(0);
         if (mixed) {
             A.mixed |= 1 << m;
-            double nvc = 1.0;
+            R nvc = R(1);
             if (lv.dim == 2) {
-                auto a = [&](int ddi, int ddj) { return load_alpha(lv, U, m, i + ddi, j + ddj, k); };
-                const double nx = ynx * (a(1, 1) + 2.0 * a(1, 0) + a(1, -1) - a(-1, 1) - 2.0 * a(-1, 0) - a(-1, -1));
-                const double ny = yny * (a(1, 1) + 2.0 * a(0, 1) + a(-1, 1) - a(-1, -1) - 2.0 * a(0, -1) - a(1, -1));
-                const double norm = __builtin_sqrt(nx * nx + ny * ny);
-                if (norm < 1E-20) nvc = 0;
-                else nvc = (d == 0) ? __builtin_fabs(nx / norm) : __builtin_fabs(ny / norm);
+                auto a = [&](int ddi, int ddj) { return load_alpha<R>(lv, U, m, i + ddi, j + ddj, k); };
+                const R nx = ynx * (a(1, 1) + R(2) * a(1, 0) + a(1, -1) - a(-1, 1) - R(2) * a(-1, 0) - a(-1, -1));
+                const R ny = yny * (a(1, 1) + R(2) * a(0, 1) + a(-1, 1) - a(-1, -1) - R(2) * a(0, -1) - a(1, -1));
+                const R norm = tsqrt(nx * nx + ny * ny);
+                if (norm < R(1E-20)) nvc = R(0);
+                else nvc = (d == 0) ? tabs(nx / norm) : tabs(ny / norm);
             }
-            const double beta = nvc * lv.beta0 + 0.01;
-            const double coshB = pm_cosh(beta), tanhB = pm_tanh(beta);
-            const double B = pm_exp(theta * beta * ((2.0 * C) - 1.0));
-            const double Ac = ((B / coshB) - 1.0) / tanhB;
-            A.thr[m] = mn + (mx / 2.0) * (1.0 + theta * ((tanhB + Ac) / (1.0 + Ac * tanhB)));
-            A.thl[m] = mn + (mx / 2.0) * (1.0 + theta * Ac);
+            const R beta = nvc * R(lv.beta0) + R(0.01);
+            const R coshB = tcosh(beta), tanhB = ttanh(beta);
+            const R B = texp(theta * beta * ((R(2) * C) - R(1)));
+            const R Ac = ((B / coshB) - R(1)) / tanhB;
+            A.thr[m] = mn + (mx / R(2)) * (R(1) + theta * ((tanhB + Ac) / (R(1) + Ac * tanhB)));
+            A.thl[m] = mn + (mx / R(2)) * (R(1) + theta * Ac);
         }
     }
 }
 
 // TBV of thinc.cpp:57-63 for one material.
-__device__ __forceinline__ double tbv(double URl, double TRl, double ULr, double TLr, double PL, double PR) {
-    return dmin(dmin(__builtin_fabs(URl - PL) + __builtin_fabs(PR - ULr), __builtin_fabs(TRl - PL) + __builtin_fabs(PR - TLr)),
-                dmin(__builtin_fabs(URl - PL) + __builtin_fabs(PR - TLr), __builtin_fabs(TRl - PL) + __builtin_fabs(PR - ULr)));
+template <typename R>
+__device__ __forceinline__ R tbv(R URl, R TRl, R ULr, R TLr, R PL, R PR) {
+    return tmin(tmin(tabs(URl - PL) + tabs(PR - ULr), tabs(TRl - PL) + tabs(PR - TLr)),
+                tmin(tabs(URl - PL) + tabs(PR - TLr), tabs(TRl - PL) + tabs(PR - ULr)));
 }
 
 // cleanUpAlpha of one cell (boxaccesscellarray.cpp:530-599), the Nan-index
 // quirk of :553-556 included.  Returns TJ status bits.
-template <int NM>
-__device__ __forceinline__ unsigned cleanup_alpha(double* a) {
-    double tot = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ unsigned cleanup_alpha(R* a) {
+    R tot = R(0);
     int nan = 0;
     unsigned st = 0;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        if (__builtin_isnan(a[m])) {
-            a[m] = 0.0;
+        if (tisnan(a[m])) {
+            a[m] = R(0);
             if (nan == 0) nan = m;
             else st |= 4u;
-        } else if (a[m] < 0.0) {
-            a[m] = 1E-6;
-        } else if (a[m] > 1.0) {
-            a[m] = 1.0;
+        } else if (a[m] < R(0)) {
+            a[m] = R(1E-6);
+        } else if (a[m] > R(1)) {
+            a[m] = R(1);
         }
         tot += a[m];
     }
@@ -302,26 +309,26 @@ __device__ __forceinline__ unsigned cleanup_alpha(double* a) {
 #pragma unroll
         for (int m = 0; m < NM; ++m)
             if (m == nan) {
-                a[m] = 1.0 - tot;
+                a[m] = R(1) - tot;
                 tot += a[m];
             }
     }
-    if (tot <= 0.0) st |= 8u;
+    if (tot <= R(0)) st |= 8u;
 #pragma unroll
-    for (int m = 0; m < NM; ++m) a[m] *= 1.0 / tot;
+    for (int m = 0; m < NM; ++m) a[m] *= R(1) / tot;
     return st;
 }
 
 // BVD selection for one cell (thinc.cpp:191-238) given the neighbour data.
 //   nb*: neighbour values with the reference's box-edge semantics already applied
 //   own thl/thr: THINC values of this cell where mixed, else the stale history
-template <int NM>
-__device__ __forceinline__ void bvd(int mixedFlag, const double* URl, const double* TRl, const double* ULr, const double* TLr,
-                                    const double* thl, const double* thr, double* ula, double* ura) {
+template <int NM, typename R>
+__device__ __forceinline__ void bvd(int mixedFlag, const R* URl, const R* TRl, const R* ULr, const R* TLr, const R* thl,
+                                    const R* thr, R* ula, R* ura) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        const double tm = tbv(URl[m], TRl[m], ULr[m], TLr[m], ula[m], ura[m]);
-        const double tt = tbv(URl[m], TRl[m], ULr[m], TLr[m], thl[m], thr[m]);
+        const R tm = tbv(URl[m], TRl[m], ULr[m], TLr[m], ula[m], ura[m]);
+        const R tt = tbv(URl[m], TRl[m], ULr[m], TLr[m], thl[m], thr[m]);
         if (mixedFlag && tt < tm) {
             ula[m] = thl[m];
             ura[m] = thr[m];

@@ -19,8 +19,8 @@ struct Clock {
 // reconstruct their cell (at the boundary-mapped cell s), neighbours' data
 // travel by wave shuffles, and lane l computes the flux of face c (the lower
 // face of cell c).  Faces per wave: 63 (MUSCL) or 61 (MUSCL + THINC/BVD).
-template <int NM, bool TH>
-__global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, Faces F, Hist h, Clock* clk) {
+template <int NM, bool TH, typename R = double>
+__global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, FacesT<R> F, Hist h, Clock* clk) {
     if (clk->done) return;
     constexpr int H = TH ? 2 : 1;
     constexpr int PER = TH ? 61 : 63;
@@ -37,17 +37,17 @@ __global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, Faces F, Hist h,
     const int s = bcmap(c, nx, lv.bc_lo[0], lv.bc_hi[0], fl);
     const int lofs = s - c;  // lane offset of the reconstructed cell
 
-    Prim<NM> Pm, Pc, Pp, PL, PR;
-    load_prim<NM>(lv, U, s - 1, j, k, Pm);
-    load_prim<NM>(lv, U, s, j, k, Pc);
-    load_prim<NM>(lv, U, s + 1, j, k, Pp);
-    muscl<NM>(Pm, Pc, Pp, PL, PR);
+    Prim<NM, R> Pm, Pc, Pp, PL, PR;
+    load_prim<NM, R>(lv, U, s - 1, j, k, Pm);
+    load_prim<NM, R>(lv, U, s, j, k, Pc);
+    load_prim<NM, R>(lv, U, s + 1, j, k, Pp);
+    muscl<NM, R>(Pm, Pc, Pp, PL, PR);
 
     unsigned st = 0;
     const long long cell_s = ((long long)k * ny + j) * nx + s;
     if (TH) {
-        ARec<NM> A;
-        arec<NM>(lv, U, s, j, k, 0, lv.yn[0], lv.yn[1], A);
+        ARec<NM, R> A;
+        arec<NM, R>(lv, U, s, j, k, 0, (R)lv.yn[0], (R)lv.yn[1], A);
         const uint8_t eflag = lv.edge[0][s];
         const bool blo = eflag & 1, bhi = eflag & 2;
         int srcm = lane + lofs - 1, srcp = lane + lofs + 1;
@@ -56,55 +56,55 @@ __global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, Faces F, Hist h,
         const int mixm = __shfl(A.mixed, srcm, 64);
         const int mixp = __shfl(A.mixed, srcp, 64);
         const int mixedFlag = (A.mixed >> (NM - 1)) & 1;
-        double URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
+        R URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
-            const double uraM = __shfl(A.ura[m], srcm, 64);
-            const double thrM = __shfl(A.thr[m], srcm, 64);
-            const double ulaP = __shfl(A.ula[m], srcp, 64);
-            const double thlP = __shfl(A.thl[m], srcp, 64);
+            const R uraM = __shfl(A.ura[m], srcm, 64);
+            const R thrM = __shfl(A.thr[m], srcm, 64);
+            const R ulaP = __shfl(A.ula[m], srcp, 64);
+            const R thlP = __shfl(A.thl[m], srcp, 64);
             if (mixedFlag) {
                 if (blo) {
-                    URl[m] = load_alpha(lv, U, m, s - 1, j, k);
-                    TRl[m] = 0.0;
+                    URl[m] = load_alpha<R>(lv, U, m, s - 1, j, k);
+                    TRl[m] = R(0);
                 } else {
                     URl[m] = uraM;
-                    TRl[m] = ((mixm >> m) & 1) ? thrM : h.urstar[m * h.pstride + cell_s - 1];
+                    TRl[m] = ((mixm >> m) & 1) ? thrM : (R)h.urstar[m * h.pstride + cell_s - 1];
                 }
                 if (bhi) {
-                    ULr[m] = load_alpha(lv, U, m, s + 1, j, k);
-                    TLr[m] = h.edge_in[m * h.estride + ((long long)k * ny + j) * (nx + 1) + s + 1];
+                    ULr[m] = load_alpha<R>(lv, U, m, s + 1, j, k);
+                    TLr[m] = (R)h.edge_in[m * h.estride + ((long long)k * ny + j) * (nx + 1) + s + 1];
                 } else {
                     ULr[m] = ulaP;
-                    TLr[m] = ((mixp >> m) & 1) ? thlP : h.star_in[m * h.pstride + cell_s + 1];
+                    TLr[m] = ((mixp >> m) & 1) ? thlP : (R)h.star_in[m * h.pstride + cell_s + 1];
                 }
                 if ((A.mixed >> m) & 1) {
                     thl[m] = A.thl[m];
                     thr[m] = A.thr[m];
                 } else {
-                    thl[m] = h.star_in[m * h.pstride + cell_s];
-                    thr[m] = h.urstar[m * h.pstride + cell_s];
+                    thl[m] = (R)h.star_in[m * h.pstride + cell_s];
+                    thr[m] = (R)h.urstar[m * h.pstride + cell_s];
                 }
             }
         }
-        if (mixedFlag) bvd<NM>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
-        st |= cleanup_alpha<NM>(PL.a);
-        st |= cleanup_alpha<NM>(PR.a);
+        if (mixedFlag) bvd<NM, R>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
+        st |= cleanup_alpha<NM, R>(PL.a);
+        st |= cleanup_alpha<NM, R>(PR.a);
         // URStar history: the owner lane of a valid cell records its THINC values
         if (lane >= H && lane < H + PER && c >= 0 && c < nx) {
 #pragma unroll
             for (int m = 0; m < NM; ++m)
-                if ((A.mixed >> m) & 1) h.urstar[m * h.pstride + cell_s] = A.thr[m];
+                if ((A.mixed >> m) & 1) h.urstar[m * h.pstride + cell_s] = (double)A.thr[m];
         }
     }
 
-    FSt<NM> SLs, SRs;
-    p2c_sound<NM>(lv, PL, SLs);
-    p2c_sound<NM>(lv, PR, SRs);
+    FSt<NM, R> SLs, SRs;
+    p2c_sound<NM, R>(lv, PL, SLs);
+    p2c_sound<NM, R>(lv, PR, SRs);
 
     // left state of face c: UR(c-1) from the neighbouring lane (or the
     // boundary image of this lane's UR at the domain's low face)
-    FSt<NM> Lf;
+    FSt<NM, R> Lf;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         Lf.al[m] = __shfl(SRs.al[m], lane > 0 ? lane - 1 : 0, 64);
@@ -120,22 +120,23 @@ __global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, Faces F, Hist h,
     if (lane >= H && lane < H + PER && c >= 0 && c <= nx) {
         if (c == 0) {
             Lf = SRs;
-            if (lv.bc_lo[0]) flip_state<NM>(Lf, 0);
+            if (lv.bc_lo[0]) flip_state<NM, R>(Lf, 0);
         }
-        FSt<NM> Rf = SLs;
-        if (c == nx && lv.bc_hi[0]) flip_state<NM>(Rf, 0);
-        double Fv[2 * NM + 5], sa[NM];
-        hllc<NM>(Lf, Rf, 0, Fv, sa);
+        FSt<NM, R> Rf = SLs;
+        if (c == nx && lv.bc_hi[0]) flip_state<NM, R>(Rf, 0);
+        R Fv[2 * NM + 5], sa[NM];
+        hllc<NM, R>(Lf, Rf, 0, Fv, sa);
 #pragma unroll
         for (int q = 0; q < 2 * NM + 5; ++q) F.at(q, c, j, k) = Fv[q];
         if (TH) {
             if (c < nx) {
 #pragma unroll
-                for (int m = 0; m < NM; ++m) h.star_out[m * h.pstride + ((long long)k * ny + j) * nx + c] = sa[m];
+                for (int m = 0; m < NM; ++m) h.star_out[m * h.pstride + ((long long)k * ny + j) * nx + c] = (double)sa[m];
             }
             if (c == nx || (c > 0 && (lv.edge[0][c - 1] & 2))) {
 #pragma unroll
-                for (int m = 0; m < NM; ++m) h.edge_out[m * h.estride + ((long long)k * ny + j) * (nx + 1) + c] = sa[m];
+                for (int m = 0; m < NM; ++m)
+                    h.edge_out[m * h.estride + ((long long)k * ny + j) * (nx + 1) + c] = (double)sa[m];
             }
         }
     }
@@ -145,20 +146,22 @@ __global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, Faces F, Hist h,
 // ------------------------------------------------------------------ update helpers
 // One direction of update (hllc.cpp:493-523): U1 += dt/dx (F_lo - F_hi), the
 // alpha rows with the non-conservative term -alpha_U (S*_lo - S*_hi).
-template <int NM>
+// The face values are FP64, or FP32 in the mixed form (widened here: the
+// accumulation stays FP64).
+template <int NM, typename T>
 __device__ __forceinline__ void accum_dir(double* al, double* ar, double* ru, double& E, const double* a0, double dtdx,
-                                          const double* Flo, const double* Fhi) {
-    const double s0 = Flo[2 * NM + 4], s1 = Fhi[2 * NM + 4];
+                                          const T* Flo, const T* Fhi) {
+    const double s0 = (double)Flo[2 * NM + 4], s1 = (double)Fhi[2 * NM + 4];
 #pragma unroll
-    for (int m = 0; m < NM; ++m) al[m] += dtdx * (Flo[m] - Fhi[m] - a0[m] * (s0 - s1));
+    for (int m = 0; m < NM; ++m) al[m] += dtdx * ((double)Flo[m] - (double)Fhi[m] - a0[m] * (s0 - s1));
 #pragma unroll
-    for (int m = 0; m < NM; ++m) ar[m] += dtdx * (Flo[NM + m] - Fhi[NM + m]);
+    for (int m = 0; m < NM; ++m) ar[m] += dtdx * ((double)Flo[NM + m] - (double)Fhi[NM + m]);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) ru[q] += dtdx * (Flo[2 * NM + q] - Fhi[2 * NM + q]);
-    E += dtdx * (Flo[2 * NM + 3] - Fhi[2 * NM + 3]);
+    for (int q = 0; q < 3; ++q) ru[q] += dtdx * ((double)Flo[2 * NM + q] - (double)Fhi[2 * NM + q]);
+    E += dtdx * ((double)Flo[2 * NM + 3] - (double)Fhi[2 * NM + 3]);
 }
-template <int NM>
-__device__ __forceinline__ void load_faces(const Faces& Fd, int i, int j, int k, double* Fv) {
+template <int NM, typename T>
+__device__ __forceinline__ void load_faces(const FacesT<T>& Fd, int i, int j, int k, T* Fv) {
 #pragma unroll
     for (int q = 0; q < 2 * NM + 5; ++q) Fv[q] = Fd.at(q, i, j, k);
 }
@@ -211,7 +214,9 @@ __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const
         if (ops & 8u) {
             const double a = sound<NM>(lv, al, ar, rk, rho, p);
             Uo.at(lv.cs[SOUNDSPEED], i, j, k) = a;
-            for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(u[d])));
+            #pragma unroll
+            for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
+                if (d < lv.dim) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(u[d])));
         }
     }
 #pragma unroll
@@ -269,10 +274,10 @@ __device__ __forceinline__ void block_min_to(double cand, unsigned long long* ds
 #ifndef TJ_MARCH_WAVES_PER_EU
 #define TJ_MARCH_WAVES_PER_EU 1
 #endif
-template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WAVES_PER_EU, 8))) void k_flux_march(Level lv, MF U, Faces F, Hist h, int seglen, Clock* clk,
-                                                    MF Ub = MF{}, MF Uo = MF{}, Faces Fx = Faces{}, Faces Fy = Faces{},
-                                                    unsigned ops = 0u) {
+template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1, typename R = double>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WAVES_PER_EU, 8))) void k_flux_march(
+    Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk, MF Ub = MF{}, MF Uo = MF{}, FacesT<R> Fx = FacesT<R>{},
+    FacesT<R> Fy = FacesT<R>{}, unsigned ops = 0u) {
     if (clk->done) return;
     const int lane = threadIdx.x & 63;
     const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -298,79 +303,80 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WA
     auto ck = [&](int r) { return D == 2 ? r : o; };
     auto cellidx = [&](int r) { return ((long long)ck(r) * ny + cj(r)) * nx + i; };
     double cand = __builtin_huge_val();
-    double Fprev[2 * NM + 5];
+    R Fprev[2 * NM + 5];
     auto faceidx = [&](int r) {  // lower face r along D
         return D == 1 ? ((long long)ck(r) * (ny + 1) + r) * nx + i : ((long long)r * ny + cj(r)) * nx + i;
     };
 
-    Prim<NM> Pm, Pc, Pp;
-    load_prim<NM>(lv, U, i, cj(rb - 1), ck(rb - 1), Pm);
-    load_prim<NM>(lv, U, i, cj(rb), ck(rb), Pc);
+    Prim<NM, R> Pm, Pc, Pp;
+    load_prim<NM, R>(lv, U, i, cj(rb - 1), ck(rb - 1), Pm);
+    load_prim<NM, R>(lv, U, i, cj(rb), ck(rb), Pc);
 
-    ARec<NM> Aprev, Acur, Anext;
+    ARec<NM, R> Aprev, Acur, Anext;
     bool curValid = false;
     unsigned st = 0;
-    FSt<NM> prevR;
+    FSt<NM, R> prevR;
+    const R ynx = (R)lv.yn[0], yny = (R)lv.yn[1];
 
     for (int r = rb; r <= re; ++r) {
-        load_prim<NM>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
-        Prim<NM> PL, PR;
-        muscl<NM>(Pm, Pc, Pp, PL, PR);
+        load_prim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
+        Prim<NM, R> PL, PR;
+        muscl<NM, R>(Pm, Pc, Pp, PL, PR);
         if (TH) {
             const uint8_t ef = lv.edge[D][r];
             const bool blo = ef & 1, bhi = ef & 2;
             if (!curValid) {
-                arec<NM>(lv, U, i, cj(r), ck(r), D, lv.yn[0], lv.yn[1], Acur);
-                if (!blo) arec<NM>(lv, U, i, cj(r - 1), ck(r - 1), D, lv.yn[0], lv.yn[1], Aprev);
+                arec<NM, R>(lv, U, i, cj(r), ck(r), D, ynx, yny, Acur);
+                if (!blo) arec<NM, R>(lv, U, i, cj(r - 1), ck(r - 1), D, ynx, yny, Aprev);
             }
-            if (!bhi) arec<NM>(lv, U, i, cj(r + 1), ck(r + 1), D, lv.yn[0], lv.yn[1], Anext);
+            if (!bhi) arec<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), D, ynx, yny, Anext);
             const int mixedFlag = (Acur.mixed >> (NM - 1)) & 1;
             const long long cr = cellidx(r);
             const long long cstride = (D == 1) ? nx : (long long)nx * ny;
             if (mixedFlag) {
-                double URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
+                R URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
 #pragma unroll
                 for (int m = 0; m < NM; ++m) {
                     if (blo) {
-                        URl[m] = load_alpha(lv, U, m, i, cj(r - 1), ck(r - 1));
-                        TRl[m] = 0.0;
+                        URl[m] = load_alpha<R>(lv, U, m, i, cj(r - 1), ck(r - 1));
+                        TRl[m] = R(0);
                     } else {
                         URl[m] = Aprev.ura[m];
-                        TRl[m] = ((Aprev.mixed >> m) & 1) ? Aprev.thr[m] : h.urstar[m * h.pstride + cr - cstride];
+                        TRl[m] = ((Aprev.mixed >> m) & 1) ? Aprev.thr[m] : (R)h.urstar[m * h.pstride + cr - cstride];
                     }
                     if (bhi) {
-                        ULr[m] = load_alpha(lv, U, m, i, cj(r + 1), ck(r + 1));
-                        TLr[m] = h.edge_in[m * h.estride + faceidx(r + 1)];
+                        ULr[m] = load_alpha<R>(lv, U, m, i, cj(r + 1), ck(r + 1));
+                        TLr[m] = (R)h.edge_in[m * h.estride + faceidx(r + 1)];
                     } else {
                         ULr[m] = Anext.ula[m];
-                        TLr[m] = ((Anext.mixed >> m) & 1) ? Anext.thl[m] : h.star_in[m * h.pstride + cr + cstride];
+                        TLr[m] = ((Anext.mixed >> m) & 1) ? Anext.thl[m] : (R)h.star_in[m * h.pstride + cr + cstride];
                     }
                     if ((Acur.mixed >> m) & 1) {
                         thl[m] = Acur.thl[m];
                         thr[m] = Acur.thr[m];
                     } else {
-                        thl[m] = h.star_in[m * h.pstride + cr];
-                        thr[m] = h.urstar[m * h.pstride + cr];
+                        thl[m] = (R)h.star_in[m * h.pstride + cr];
+                        thr[m] = (R)h.urstar[m * h.pstride + cr];
                     }
                 }
-                bvd<NM>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
+                bvd<NM, R>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
             }
-            st |= cleanup_alpha<NM>(PL.a);
-            st |= cleanup_alpha<NM>(PR.a);
+            st |= cleanup_alpha<NM, R>(PL.a);
+            st |= cleanup_alpha<NM, R>(PR.a);
             if (act && r >= f0 && r < (FUSE ? f1 : nd)) {
 #pragma unroll
                 for (int m = 0; m < NM; ++m)
-                    if ((Acur.mixed >> m) & 1) h.urstar[m * h.pstride + cr] = Acur.thr[m];
+                    if ((Acur.mixed >> m) & 1) h.urstar[m * h.pstride + cr] = (double)Acur.thr[m];
             }
             Aprev = Acur;
             Acur = Anext;
             curValid = !bhi;
         }
-        FSt<NM> SLs, SRs;
-        p2c_sound<NM>(lv, PL, SLs);
-        p2c_sound<NM>(lv, PR, SRs);
+        FSt<NM, R> SLs, SRs;
+        p2c_sound<NM, R>(lv, PL, SLs);
+        p2c_sound<NM, R>(lv, PR, SRs);
         // complete row q (faces q and q+1 known) in the fused form
-        auto finish_row = [&](int q, const double* Flo, const double* Fhi) {
+        auto finish_row = [&](int q, const R* Flo, const R* Fhi) {
             const int jj = cj(q), kk = ck(q);
             double al[NM], ar[NM], ru[3], E, a0[NM];
             load_cons<NM>(lv, U, i, jj, kk, al, ar, ru, E);
@@ -378,23 +384,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WA
             for (int m = 0; m < NM; ++m) a0[m] = al[m];
             const double dt = clk->dt;
             {
-                double Fa[2 * NM + 5], Fb[2 * NM + 5];
-                load_faces<NM>(Fx, i, jj, kk, Fa);
-                load_faces<NM>(Fx, i + 1, jj, kk, Fb);
-                accum_dir<NM>(al, ar, ru, E, a0, dt / lv.dx[0], Fa, Fb);
+                R Fa[2 * NM + 5], Fb[2 * NM + 5];
+                load_faces<NM, R>(Fx, i, jj, kk, Fa);
+                load_faces<NM, R>(Fx, i + 1, jj, kk, Fb);
+                accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[0], Fa, Fb);
             }
             if (D == 2) {
-                double Fa[2 * NM + 5], Fb[2 * NM + 5];
-                load_faces<NM>(Fy, i, jj, kk, Fa);
-                load_faces<NM>(Fy, i, jj + 1, kk, Fb);
-                accum_dir<NM>(al, ar, ru, E, a0, dt / lv.dx[1], Fa, Fb);
+                R Fa[2 * NM + 5], Fb[2 * NM + 5];
+                load_faces<NM, R>(Fy, i, jj, kk, Fa);
+                load_faces<NM, R>(Fy, i, jj + 1, kk, Fb);
+                accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[1], Fa, Fb);
             }
-            accum_dir<NM>(al, ar, ru, E, a0, dt / lv.dx[D], Flo, Fhi);
+            accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[D], Flo, Fhi);
             if (act) finish_cell<NM, STAGE>(lv, Ub, Uo, i, jj, kk, al, ar, ru, E, ops, cand, st);
         };
-        auto emit = [&](int face, const FSt<NM>& A, const FSt<NM>& B, double* Fv) {
-            double sa[NM];
-            hllc<NM>(A, B, D, Fv, sa);
+        auto emit = [&](int face, const FSt<NM, R>& A, const FSt<NM, R>& B, R* Fv) {
+            R sa[NM];
+            hllc<NM, R>(A, B, D, Fv, sa);
             if (!act) return;
             const long long fi = faceidx(face);
             if (!FUSE) {
@@ -405,19 +411,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WA
             if (TH) {
                 if (face < nd) {
 #pragma unroll
-                    for (int m = 0; m < NM; ++m) h.star_out[m * h.pstride + cellidx(face)] = sa[m];
+                    for (int m = 0; m < NM; ++m) h.star_out[m * h.pstride + cellidx(face)] = (double)sa[m];
                 }
                 if (face == nd || (face > 0 && (lv.edge[D][face - 1] & 2))) {
 #pragma unroll
-                    for (int m = 0; m < NM; ++m) h.edge_out[m * h.estride + fi] = sa[m];
+                    for (int m = 0; m < NM; ++m) h.edge_out[m * h.estride + fi] = (double)sa[m];
                 }
             }
         };
-        double Fcur[2 * NM + 5];
+        R Fcur[2 * NM + 5];
         bool have = false;
         if (r == 0 && f0 == 0) {
-            FSt<NM> img = SRs;
-            if (lv.bc_lo[D]) flip_state<NM>(img, D);
+            FSt<NM, R> img = SRs;
+            if (lv.bc_lo[D]) flip_state<NM, R>(img, D);
             emit(0, img, SLs, Fcur);
             have = true;
         } else if (r >= f0 && r <= ftop) {
@@ -430,9 +436,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WA
             for (int q = 0; q < 2 * NM + 5; ++q) Fprev[q] = Fcur[q];
         }
         if (r == nd - 1 && ftop == nd) {
-            FSt<NM> img = SLs;
-            if (lv.bc_hi[D]) flip_state<NM>(img, D);
-            double Ftop[2 * NM + 5];
+            FSt<NM, R> img = SLs;
+            if (lv.bc_hi[D]) flip_state<NM, R>(img, D);
+            R Ftop[2 * NM + 5];
             emit(nd, SRs, img, Ftop);
             if (FUSE) finish_row(nd - 1, Fprev, Ftop);
         }
@@ -449,9 +455,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WA
 // conservativeToPrimitive (hllc.cpp:689).  Stage 2 adds the RK average of
 // advance (hllc.cpp:705), cleanUpAlpha (main.cpp:169-171), the NaN scan
 // (main.cpp:183-187) and getTimeStep of the next step (main.cpp:128).
-template <int NM, int STAGE>
-__global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, Faces F0, Faces F1, Faces F2,
-                                                Clock* clk, unsigned ops) {
+template <int NM, int STAGE, typename T = double>
+__global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, FacesT<T> F0, FacesT<T> F1,
+                                                FacesT<T> F2, Clock* clk, unsigned ops) {
     if (clk->done) return;
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -468,11 +474,11 @@ __global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, 
 #pragma unroll
         for (int m = 0; m < NM; ++m) a0[m] = al[m];
         for (int d = 0; d < lv.dim; ++d) {
-            const Faces& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
-            double Fa[2 * NM + 5], Fb[2 * NM + 5];
-            load_faces<NM>(Fd, i, j, k, Fa);
-            load_faces<NM>(Fd, i + (d == 0), j + (d == 1), k + (d == 2), Fb);
-            accum_dir<NM>(al, ar, ru, E, a0, dt / lv.dx[d], Fa, Fb);
+            const FacesT<T>& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
+            T Fa[2 * NM + 5], Fb[2 * NM + 5];
+            load_faces<NM, T>(Fd, i, j, k, Fa);
+            load_faces<NM, T>(Fd, i + (d == 0), j + (d == 1), k + (d == 2), Fb);
+            accum_dir<NM, T>(al, ar, ru, E, a0, dt / lv.dx[d], Fa, Fb);
         }
         finish_cell<NM, STAGE>(lv, Ub, Uo, i, j, k, al, ar, ru, E, ops, cand, st);
     }

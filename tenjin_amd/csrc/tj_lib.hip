@@ -25,6 +25,9 @@ struct tj_ctx {
     uint8_t* d_edge[3] = {nullptr, nullptr, nullptr};
     double* d_flux[3] = {nullptr, nullptr, nullptr};
     Faces F[3]{};
+    // FP32 flux planes of the mixed-precision form (allocated on first use)
+    float* d_fluxf[3] = {nullptr, nullptr, nullptr};
+    FacesT<float> Ff[3]{};
     int nfc = 0;  // flux components per face
     double* d_urstar = nullptr;
     double* d_star[2] = {nullptr, nullptr};
@@ -147,10 +150,12 @@ void free_all(tj_ctx* c) {
     for (int d = 0; d < 3; ++d) {
         if (c->d_edge[d]) hipFree(c->d_edge[d]);
         if (c->d_flux[d]) hipFree(c->d_flux[d]);
+        if (c->d_fluxf[d]) hipFree(c->d_fluxf[d]);
         for (int q = 0; q < 2; ++q)
             if (c->d_estar[d][q]) hipFree(c->d_estar[d][q]);
         c->d_edge[d] = nullptr;
         c->d_flux[d] = nullptr;
+        c->d_fluxf[d] = nullptr;
         c->d_estar[d][0] = c->d_estar[d][1] = nullptr;
     }
     if (c->d_urstar) hipFree(c->d_urstar);
@@ -265,6 +270,21 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
         C.s = M.s;
         C.G0 = M.G0;
         C.yield = M.yield_stress;
+        MatT<float>& F = lv.mf[m];
+        F.eos = C.eos;
+        F.solid = C.solid;
+        F.plastic = C.plastic;
+        F.Gg = (float)C.Gg;
+        F.gp1 = (float)C.gp1;
+        F.xi = (float)C.xi;
+        F.pref = (float)C.pref;
+        F.eref = (float)C.eref;
+        F.CV = (float)C.CV;
+        F.rho0 = (float)C.rho0;
+        F.c0 = (float)C.c0;
+        F.s = (float)C.s;
+        F.G0 = (float)C.G0;
+        F.yield = (float)C.yield;
     }
     build_layout(*d, lv.cs);
     lv.ncells = (long long)lv.n[0] * lv.n[1] * lv.n[2];
@@ -347,19 +367,49 @@ static void launch_sound_dt(tj_ctx* c, const MF& U) {
     else hipLaunchKernelGGL(k_sound_dt<2>, dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
 }
 
-static void launch_update(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops) {
+extern "C++" {
+template <typename T>
+static void launch_update_t(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops,
+                            const FacesT<T>* F) {
     const unsigned g = grid1(c->lv.ncells, 256);
     if (c->lv.solid) {
         if (stage == 1)
-            hipLaunchKernelGGL((k_update_s<2, 1>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, c->F[0], c->F[1], c->F[2], c->d_clk, ops);
+            hipLaunchKernelGGL((k_update_s<2, 1, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
         else
-            hipLaunchKernelGGL((k_update_s<2, 2>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, c->F[0], c->F[1], c->F[2], c->d_clk, ops);
+            hipLaunchKernelGGL((k_update_s<2, 2, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
     } else {
         if (stage == 1)
-            hipLaunchKernelGGL((k_update<2, 1>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, c->F[0], c->F[1], c->F[2], c->d_clk, ops);
+            hipLaunchKernelGGL((k_update<2, 1, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
         else
-            hipLaunchKernelGGL((k_update<2, 2>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, c->F[0], c->F[1], c->F[2], c->d_clk, ops);
+            hipLaunchKernelGGL((k_update<2, 2, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
     }
+}
+
+}  // extern "C++"
+
+static void launch_update(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops, int prec) {
+    if (prec == TJ_PREC_MIXED) launch_update_t<float>(c, stage, in, base, out, ops, c->Ff);
+    else launch_update_t<double>(c, stage, in, base, out, ops, c->F);
+}
+
+// FP32 flux planes for the mixed-precision form, allocated on first use.
+static int ensure_mixed(tj_ctx* c) {
+    if (c->d_fluxf[0]) return TJ_OK;
+    for (int a = 0; a < c->lv.dim; ++a) {
+        FacesT<float>& F = c->Ff[a];
+        F.nx = c->F[a].nx;
+        F.ny = c->F[a].ny;
+        F.nz = c->F[a].nz;
+        F.cs = c->F[a].cs;
+        HIPCHK(c, hipMalloc(&c->d_fluxf[a], sizeof(float) * F.cs * c->nfc));
+        F.p = c->d_fluxf[a];
+    }
+    return TJ_OK;
+}
+
+static int check_prec(tj_ctx* c, int prec) {
+    if (prec != TJ_PREC_FP64 && prec != TJ_PREC_MIXED) { c->err = "unknown precision form"; return TJ_EINVAL; }
+    return prec == TJ_PREC_MIXED ? ensure_mixed(c) : TJ_OK;
 }
 
 // post-step operations of the reference's loop body for the bound level
@@ -433,85 +483,109 @@ struct FuseArgs {
     unsigned ops;
 };
 
-// Directional flux passes of one RK stage.  With fz (fluid levels, dim >= 2,
-// output distinct from the input) the last pass also completes the update and
-// *fused is set; otherwise the caller launches the update kernel.
-static int launch_fluxes(tj_ctx* c, const MF& Uin, const FuseArgs* fz, bool* fused) {
+// THINC history planes of direction d for the current pass.
+static Hist hist_for(tj_ctx* c, int d) {
+    Hist h{};
+    if (!c->desc.THINC) return h;
+    const int q = (int)(c->pass & 1), e = (int)(c->dpass[d] & 1);
+    h.urstar = c->d_urstar;
+    h.star_in = c->d_star[q];
+    h.star_out = c->d_star[q ^ 1];
+    h.edge_in = c->d_estar[d][e];
+    h.edge_out = c->d_estar[d][e ^ 1];
+    h.pstride = c->lv.ncells;
+    h.estride = c->estride[d];
+    return h;
+}
+
+// One directional flux pass (not fused) with face values of type R.
+extern "C++" {
+template <typename R>
+static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, const Hist& h) {
     const Level& lv = c->lv;
     const bool th = c->desc.THINC != 0;
-    const bool fuse = fz && c->fuse && !lv.solid && lv.dim >= 2 && fz->out.p != Uin.p;
-    if (fused) *fused = fuse;
-    for (int d = 0; d < lv.dim; ++d) {
-        Hist h{};
-        if (th) {
-            const int q = (int)(c->pass & 1), e = (int)(c->dpass[d] & 1);
-            h.urstar = c->d_urstar;
-            h.star_in = c->d_star[q];
-            h.star_out = c->d_star[q ^ 1];
-            h.edge_in = c->d_estar[d][e];
-            h.edge_out = c->d_estar[d][e ^ 1];
-            h.pstride = lv.ncells;
-            h.estride = c->estride[d];
-        }
-        const bool last = fuse && d == lv.dim - 1;
-        static const char* names[3] = {"flux_x", "flux_y", "flux_z"};
-        static const char* fnames[2][3] = {{"", "flux_y_update1", "flux_z_update1"},
-                                           {"", "flux_y_update2_post", "flux_z_update2_post"}};
-        Timed tm(c, last ? fnames[fz->stage - 1][d] : names[d]);
-        if (lv.solid) {
-            if (d == 0) {
-                const long long nseg = (lv.n[0] + 1 + 62) / 63;
-                const unsigned g = grid1(nseg * lv.n[1] * lv.n[2], 4);
-                hipLaunchKernelGGL((k_flux_x_s<2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[0], c->d_clk);
-            } else {
-                const long long nstrip = (lv.n[0] + 63) / 64;
-                const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
-                const long long no = d == 1 ? lv.n[2] : lv.n[1];
-                const unsigned g = grid1(nstrip * nseg * no, 4);
-                if (d == 1)
-                    hipLaunchKernelGGL((k_flux_march_s<2, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], c->seglen, c->d_clk);
-                else
-                    hipLaunchKernelGGL((k_flux_march_s<2, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], c->seglen, c->d_clk);
-            }
-        } else if (d == 0) {
-            const int per = th ? 61 : 63;
-            const long long nseg = (lv.n[0] + 1 + per - 1) / per;
-            const long long waves = nseg * lv.n[1] * lv.n[2];
-            const unsigned g = grid1(waves, 4);
-            if (th) hipLaunchKernelGGL((k_flux_x<2, true>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[0], h, c->d_clk);
-            else hipLaunchKernelGGL((k_flux_x<2, false>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[0], h, c->d_clk);
-        } else if (!last) {
+    if (lv.solid) {
+        if (d == 0) {
+            const long long nseg = (lv.n[0] + 1 + 62) / 63;
+            const unsigned g = grid1(nseg * lv.n[1] * lv.n[2], 4);
+            hipLaunchKernelGGL((k_flux_x_s<2, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], c->d_clk);
+        } else {
             const long long nstrip = (lv.n[0] + 63) / 64;
             const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
             const long long no = d == 1 ? lv.n[2] : lv.n[1];
             const unsigned g = grid1(nstrip * nseg * no, 4);
-            if (d == 1) {
-                if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, MF{}, MF{}, Faces{}, Faces{}, 0u);
-                else hipLaunchKernelGGL((k_flux_march<2, false, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, MF{}, MF{}, Faces{}, Faces{}, 0u);
-            } else {
-                hipLaunchKernelGGL((k_flux_march<2, false, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, MF{}, MF{}, Faces{}, Faces{}, 0u);
-            }
-        } else {
-            // fused: segments own rows [f0, f0 + seglen)
-            const long long nstrip = (lv.n[0] + 63) / 64;
-            const long long nseg = (lv.n[d] + c->seglen - 1) / c->seglen;
-            const long long no = d == 1 ? lv.n[2] : lv.n[1];
-            const unsigned g = grid1(nstrip * nseg * no, 4);
-            const MF b = fz->base, o = fz->out;
-            const unsigned ops = fz->ops;
-            const bool s2 = fz->stage == 2;
-            if (d == 1) {
-                if (th && s2) hipLaunchKernelGGL((k_flux_march<2, true, 1, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-                else if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-                else if (s2) hipLaunchKernelGGL((k_flux_march<2, false, 1, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-                else hipLaunchKernelGGL((k_flux_march<2, false, 1, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-            } else {
-                if (s2) hipLaunchKernelGGL((k_flux_march<2, false, 2, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-                else hipLaunchKernelGGL((k_flux_march<2, false, 2, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-            }
+            if (d == 1)
+                hipLaunchKernelGGL((k_flux_march_s<2, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], c->seglen, c->d_clk);
+            else
+                hipLaunchKernelGGL((k_flux_march_s<2, 2, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[2], c->seglen, c->d_clk);
         }
+    } else if (d == 0) {
+        const int per = th ? 61 : 63;
+        const long long nseg = (lv.n[0] + 1 + per - 1) / per;
+        const unsigned g = grid1(nseg * lv.n[1] * lv.n[2], 4);
+        if (th) hipLaunchKernelGGL((k_flux_x<2, true, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], h, c->d_clk);
+        else hipLaunchKernelGGL((k_flux_x<2, false, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], h, c->d_clk);
+    } else {
+        const long long nstrip = (lv.n[0] + 63) / 64;
+        const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
+        const long long no = d == 1 ? lv.n[2] : lv.n[1];
+        const unsigned g = grid1(nstrip * nseg * no, 4);
+        const MF z{};
+        const FacesT<R> zf{};
+        if (d == 1) {
+            if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
+            else hipLaunchKernelGGL((k_flux_march<2, false, 1, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
+        } else {
+            hipLaunchKernelGGL((k_flux_march<2, false, 2, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[2], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
+        }
+    }
+}
+
+}  // extern "C++"
+
+// The last fluid pass with the update folded in (FP64 form).
+static void launch_fused(tj_ctx* c, const MF& Uin, int d, const Hist& h, const FuseArgs* fz) {
+    const Level& lv = c->lv;
+    const bool th = c->desc.THINC != 0;
+    const long long nstrip = (lv.n[0] + 63) / 64;
+    const long long nseg = (lv.n[d] + c->seglen - 1) / c->seglen;  // segments own rows [f0, f0 + seglen)
+    const long long no = d == 1 ? lv.n[2] : lv.n[1];
+    const unsigned g = grid1(nstrip * nseg * no, 4);
+    const MF b = fz->base, o = fz->out;
+    const unsigned ops = fz->ops;
+    const bool s2 = fz->stage == 2;
+    if (d == 1) {
+        if (th && s2) hipLaunchKernelGGL((k_flux_march<2, true, 1, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+        else if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+        else if (s2) hipLaunchKernelGGL((k_flux_march<2, false, 1, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+        else hipLaunchKernelGGL((k_flux_march<2, false, 1, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+    } else {
+        if (s2) hipLaunchKernelGGL((k_flux_march<2, false, 2, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+        else hipLaunchKernelGGL((k_flux_march<2, false, 2, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+    }
+}
+
+// Directional flux passes of one RK stage.  With fz (FP64 form, fluid
+// levels, dim >= 2, output distinct from the input, fusion switched on) the
+// last pass also completes the update and *fused is set; otherwise the caller
+// launches the update kernel.
+static int launch_fluxes(tj_ctx* c, const MF& Uin, const FuseArgs* fz, bool* fused, int prec) {
+    const Level& lv = c->lv;
+    const bool mixed = prec == TJ_PREC_MIXED;
+    const bool fuse = fz && c->fuse && !mixed && !lv.solid && lv.dim >= 2 && fz->out.p != Uin.p;
+    if (fused) *fused = fuse;
+    for (int d = 0; d < lv.dim; ++d) {
+        const Hist h = hist_for(c, d);
+        const bool last = fuse && d == lv.dim - 1;
+        static const char* names[2][3] = {{"flux_x", "flux_y", "flux_z"}, {"flux_x_mixed", "flux_y_mixed", "flux_z_mixed"}};
+        static const char* fnames[2][3] = {{"", "flux_y_update1", "flux_z_update1"},
+                                           {"", "flux_y_update2_post", "flux_z_update2_post"}};
+        Timed tm(c, last ? fnames[fz->stage - 1][d] : names[mixed][d]);
+        if (last) launch_fused(c, Uin, d, h, fz);
+        else if (mixed) launch_pass<float>(c, Uin, d, c->Ff, h);
+        else launch_pass<double>(c, Uin, d, c->F, h);
         HIPCHK(c, hipGetLastError());
-        if (th) {
+        if (c->desc.THINC) {
             c->pass += 1;
             c->dpass[d] += 1;
         }
@@ -521,8 +595,8 @@ static int launch_fluxes(tj_ctx* c, const MF& Uin, const FuseArgs* fz, bool* fus
 
 int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_mf* Uout, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
-    if (prec != TJ_PREC_FP64) { c->err = "rk_stage: only the FP64 form is available in this build"; return TJ_EUNSUPPORTED; }
     int r;
+    if ((r = check_prec(c, prec))) return r;
     if ((r = check_mf(c, Uin)) || (r = check_mf(c, Uout))) return r;
     if (stage == 2 && (r = check_mf(c, Ubase))) return r;
     HIPCHK(c, hipSetDevice(c->device));
@@ -530,10 +604,11 @@ int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_m
     const MF base = stage == 2 ? view(Ubase) : in;
     const FuseArgs fz{stage, base, view(Uout), 0u};
     bool fused = false;
-    if ((r = launch_fluxes(c, in, &fz, &fused))) return r;
+    if ((r = launch_fluxes(c, in, &fz, &fused, prec))) return r;
     if (!fused) {
-        Timed tm(c, stage == 1 ? "update_stage1" : "update_stage2");
-        launch_update(c, stage, in, base, view(Uout), 0u);
+        static const char* names[2][2] = {{"update_stage1", "update_stage2"}, {"update_stage1_mixed", "update_stage2_mixed"}};
+        Timed tm(c, names[prec == TJ_PREC_MIXED][stage - 1]);
+        launch_update(c, stage, in, base, view(Uout), 0u, prec);
     }
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
@@ -563,24 +638,25 @@ int tj_post_step(tj_ctx* c, tj_mf* U, unsigned ops) {
 
 int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
-    if (prec != TJ_PREC_FP64) { c->err = "step: only the FP64 form is available in this build"; return TJ_EUNSUPPORTED; }
     int r;
+    if ((r = check_prec(c, prec))) return r;
     if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
     HIPCHK(c, hipSetDevice(c->device));
     const MF s = view(S), t = view(T);
     if (s.p == t.p) { c->err = "step: S and T must be distinct buffers"; return TJ_EINVAL; }
+    const bool mixed = prec == TJ_PREC_MIXED;
     bool fused = false;
     const FuseArgs f1{1, s, t, 0u};
-    if ((r = launch_fluxes(c, s, &f1, &fused))) return r;
+    if ((r = launch_fluxes(c, s, &f1, &fused, prec))) return r;
     if (!fused) {
-        Timed tm(c, "update_stage1");
-        launch_update(c, 1, s, s, t, 0u);
+        Timed tm(c, mixed ? "update_stage1_mixed" : "update_stage1");
+        launch_update(c, 1, s, s, t, 0u, prec);
     }
     const FuseArgs f2{2, s, s, step_ops(c)};
-    if ((r = launch_fluxes(c, t, &f2, &fused))) return r;
+    if ((r = launch_fluxes(c, t, &f2, &fused, prec))) return r;
     if (!fused) {
-        Timed tm(c, "update_stage2_post");
-        launch_update(c, 2, t, s, s, step_ops(c));
+        Timed tm(c, mixed ? "update_stage2_post_mixed" : "update_stage2_post");
+        launch_update(c, 2, t, s, s, step_ops(c), prec);
     }
     hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, c->stream, c->d_clk);
     HIPCHK(c, hipGetLastError());

@@ -11,24 +11,29 @@ namespace tj {
 // ------------------------------------------------------------------ 3x3 algebra
 // tensor.cpp:81-115 (products: k = 0,1,2 summed left to right), 140-150
 // (trace), 187-233 (adjugate inverse), 296-333 (cofactor determinant).
-__device__ __forceinline__ double det2(double a, double b, double c, double d) { return a * d - b * c; }
-__device__ __forceinline__ double det3(const double* M) {
-    double det = 0.0;
+// R: double (FP64 form and every per-cell update) or float (face states and
+// fluxes of the mixed-precision form).
+template <typename R>
+__device__ __forceinline__ R det2(R a, R b, R c, R d) { return a * d - b * c; }
+template <typename R>
+__device__ __forceinline__ R det3(const R* M) {
+    R det = R(0);
     det += M[0] * det2(M[4], M[5], M[7], M[8]);
     det += (-M[1]) * det2(M[3], M[5], M[6], M[8]);
     det += M[2] * det2(M[3], M[4], M[6], M[7]);
     return det;
 }
-__device__ __forceinline__ void inv3(const double* M, double* Mi) {
-    const double D = det3(M);
-    double cof[9];
+template <typename R>
+__device__ __forceinline__ void inv3(const R* M, R* Mi) {
+    const R D = det3(M);
+    R cof[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const int k0 = i == 0 ? 1 : 0, k1 = i == 2 ? 1 : 2;
             const int m0 = j == 0 ? 1 : 0, m1 = j == 2 ? 1 : 2;
-            const double sign = ((i + j) % 2) ? -1.0 : 1.0;
+            const R sign = ((i + j) % 2) ? R(-1) : R(1);
             cof[i * 3 + j] = sign * det2(M[k0 * 3 + m0], M[k0 * 3 + m1], M[k1 * 3 + m0], M[k1 * 3 + m1]);
         }
 #pragma unroll
@@ -36,14 +41,16 @@ __device__ __forceinline__ void inv3(const double* M, double* Mi) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) Mi[i * 3 + j] = cof[j * 3 + i] / D;
 }
-__device__ __forceinline__ void mm3(const double* A, const double* B, double* C) {  // C = A B
+template <typename R>
+__device__ __forceinline__ void mm3(const R* A, const R* B, R* C) {  // C = A B
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             C[r * 3 + c] = (A[r * 3] * B[c] + A[r * 3 + 1] * B[3 + c]) + A[r * 3 + 2] * B[6 + c];
 }
-__device__ __forceinline__ void mmT3(const double* A, const double* B, double* C) {  // C = A B^T
+template <typename R>
+__device__ __forceinline__ void mmT3(const R* A, const R* B, R* C) {  // C = A B^T
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -53,13 +60,14 @@ __device__ __forceinline__ void mmT3(const double* A, const double* B, double* C
 
 // getDeviatoricHenckyStrain + getHenckyJ2 (boxaccesscellarray.cpp:326-380):
 // devH = (G - G^-1)/4 with G = V V^T, HJ2 = tr(devH devH^T).
-__device__ __forceinline__ double hencky(const double* V, double* dH) {
-    double G[9], Gi[9];
+template <typename R>
+__device__ __forceinline__ R hencky(const R* V, R* dH) {
+    R G[9], Gi[9];
     mmT3(V, V, G);
     inv3(G, Gi);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) dH[q] = 0.5 * 0.5 * (G[q] - Gi[q]);
-    double t = 0.0;
+    for (int q = 0; q < 9; ++q) dH[q] = R(0.5) * R(0.5) * (G[q] - Gi[q]);
+    R t = R(0);
 #pragma unroll
     for (int r = 0; r < 3; ++r) t += (dH[r * 3] * dH[r * 3] + dH[r * 3 + 1] * dH[r * 3 + 1]) + dH[r * 3 + 2] * dH[r * 3 + 2];
     return t;
@@ -117,121 +125,128 @@ __device__ inline void sqrtm_spd(const double* Ain, double* S) {
 // ------------------------------------------------------------------ Us-up solid EOS
 // Hugoniot reference curve p_H = rho0 c0^2 eta/(1 - s eta)^2, e_H = p_H eta/(2 rho0),
 // eta = 1 - rho0/rho, with the Mie-Gruneisen form p = p_H + Gamma rho (e - e_H).
-__device__ __forceinline__ double pH(const MatC& M, double rho) {
-    const double eta = 1.0 - M.rho0 / rho;
-    const double den = 1.0 - M.s * eta;
+template <typename R>
+__device__ __forceinline__ R pH(const MatT<R>& M, R rho) {
+    const R eta = R(1) - M.rho0 / rho;
+    const R den = R(1) - M.s * eta;
     return M.rho0 * M.c0 * M.c0 * eta / (den * den);
 }
-__device__ __forceinline__ double eH(const MatC& M, double rho) {
-    const double eta = 1.0 - M.rho0 / rho;
-    return pH(M, rho) * eta / (2.0 * M.rho0);
+template <typename R>
+__device__ __forceinline__ R eH(const MatT<R>& M, R rho) {
+    const R eta = R(1) - M.rho0 / rho;
+    return pH(M, rho) * eta / (R(2) * M.rho0);
 }
-__device__ __forceinline__ double dpH(const MatC& M, double rho) {
-    const double eta = 1.0 - M.rho0 / rho;
-    const double den = 1.0 - M.s * eta;
-    const double detadrho = M.rho0 / (rho * rho);
-    return M.rho0 * M.c0 * M.c0 * (1.0 + M.s * eta) / (den * den * den) * detadrho;
+template <typename R>
+__device__ __forceinline__ R dpH(const MatT<R>& M, R rho) {
+    const R eta = R(1) - M.rho0 / rho;
+    const R den = R(1) - M.s * eta;
+    const R detadrho = M.rho0 / (rho * rho);
+    return M.rho0 * M.c0 * M.c0 * (R(1) + M.s * eta) / (den * den * den) * detadrho;
 }
-__device__ __forceinline__ double deH(const MatC& M, double rho) {
-    const double eta = 1.0 - M.rho0 / rho;
-    const double detadrho = M.rho0 / (rho * rho);
-    return (dpH(M, rho) * eta + pH(M, rho) * detadrho) / (2.0 * M.rho0);
+template <typename R>
+__device__ __forceinline__ R deH(const MatT<R>& M, R rho) {
+    const R eta = R(1) - M.rho0 / rho;
+    const R detadrho = M.rho0 / (rho * rho);
+    return (dpH(M, rho) * eta + pH(M, rho) * detadrho) / (R(2) * M.rho0);
 }
-__device__ __forceinline__ double shearG(const MatC& M) { return M.eos == 1 ? M.G0 : 0.0; }
+template <typename R>
+__device__ __forceinline__ R shearG(const MatT<R>& M) { return M.eos == 1 ? M.G0 : R(0); }
 
 // effective sums of boxaccesscellarray.cpp:218-252 with the solid terms
-template <int NM>
-__device__ __forceinline__ void eos_sums_s(const Level& lv, const double* a, const double* ar, const double* rk, double hj2,
-                                           double& G, double& Ent, double& Pnt) {
-    G = 0.0;
-    Ent = 0.0;
-    Pnt = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ void eos_sums_s(const Level& lv, const R* a, const R* ar, const R* rk, R hj2, R& G, R& Ent,
+                                           R& Pnt) {
+    G = R(0);
+    Ent = R(0);
+    Pnt = R(0);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) G += a[m] / lv.m[m].Gg;
+    for (int m = 0; m < NM; ++m) G += a[m] / mat<R>(lv, m).Gg;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        const MatC& M = lv.m[m];
+        const MatT<R>& M = mat<R>(lv, m);
         if (M.eos == 1) Ent += (ar[m] * eH(M, rk[m]) + a[m] * (shearG(M) * hj2));
-        else Ent += (M.eref * ar[m] + 0.0 * a[m]);
+        else Ent += (M.eref * ar[m] + R(0) * a[m]);
     }
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        const MatC& M = lv.m[m];
-        if (M.eos == 1) Pnt += (a[m] / M.Gg) * pH(M, rk[m]) + (a[m] / M.Gg) * ((rk[m] * 0.0 - shearG(M)) * hj2);
-        else Pnt += M.pref * a[m] / M.Gg + 0.0 * a[m];
+        const MatT<R>& M = mat<R>(lv, m);
+        if (M.eos == 1) Pnt += (a[m] / M.Gg) * pH(M, rk[m]) + (a[m] / M.Gg) * ((rk[m] * R(0) - shearG(M)) * hj2);
+        else Pnt += M.pref * a[m] / M.Gg + R(0) * a[m];
     }
 }
 // getSoundSpeedContribution: stiffened gas (equationofstate.cpp:61-64) or the
 // Us-up solid with c^2 = dp/drho|_s + 4G/(3 rho)
-__device__ __forceinline__ double c2_s(const MatC& M, double p, double rk, double hj2) {
+template <typename R>
+__device__ __forceinline__ R c2_s(const MatT<R>& M, R p, R rk, R hj2) {
     if (M.eos == 1)
-        return (p * M.gp1 - pH(M, rk) + M.gp1 * M.G0 * hj2) / rk + dpH(M, rk) - M.Gg * rk * deH(M, rk) + (4.0 / 3.0) * M.G0 / rk;
+        return (p * M.gp1 - pH(M, rk) + M.gp1 * M.G0 * hj2) / rk + dpH(M, rk) - M.Gg * rk * deH(M, rk) +
+               (R(4.0) / R(3.0)) * M.G0 / rk;
     return p * M.gp1 / rk - M.pref / rk;
 }
-template <int NM>
-__device__ __forceinline__ double sound_s(const Level& lv, const double* al, const double* ar, const double* rk, double rho,
-                                          double p, double hj2) {
-    double a = 0.0, xt = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ R sound_s(const Level& lv, const R* al, const R* ar, const R* rk, R rho, R p, R hj2) {
+    R a = R(0), xt = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        const MatC& M = lv.m[m];
-        a += dmax(0.0, M.xi * c2_s(M, p, rk[m], hj2) * ar[m] / rho);
+        const MatT<R>& M = mat<R>(lv, m);
+        a += tmax(R(0), M.xi * c2_s(M, p, rk[m], hj2) * ar[m] / rho);
         xt += M.xi * al[m];
     }
-    if (a <= 0.0) a = 1E-6;
-    return __builtin_sqrt(a / xt);
+    if (a <= R(0)) a = R(1E-6);
+    return tsqrt(a / xt);
 }
 // stressTensor column d (boxaccesscellarray.cpp:176-216)
-template <int NM>
-__device__ __forceinline__ void stress_col(const Level& lv, const double* al, double p, const double* dH, int d, double* sig) {
-    double Gt = 0.0, Gi = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ void stress_col(const Level& lv, const R* al, R p, const R* dH, int d, R* sig) {
+    R Gt = R(0), Gi = R(0);
     int solid = 0;
 #pragma unroll
     for (int m = 0; m < NM; ++m)
-        if (lv.m[m].solid) {
+        if (mat<R>(lv, m).solid) {
             solid = 1;
-            Gt += shearG(lv.m[m]) * (al[m] / lv.m[m].Gg);
+            Gt += shearG(mat<R>(lv, m)) * (al[m] / mat<R>(lv, m).Gg);
         }
 #pragma unroll
-    for (int m = 0; m < NM; ++m) Gi += al[m] / lv.m[m].Gg;
+    for (int m = 0; m < NM; ++m) Gi += al[m] / mat<R>(lv, m).Gg;
     Gt = Gt / Gi;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
-        sig[r] = solid ? -p * (r == d ? 1.0 : 0.0) + 2.0 * Gt * dH[r * 3 + d] : -p * (r == d ? 1.0 : 0.0);
+        sig[r] = solid ? -p * (r == d ? R(1) : R(0)) + R(2) * Gt * dH[r * 3 + d] : -p * (r == d ? R(1) : R(0));
 }
 
 // ------------------------------------------------------------------ states
-template <int NM>
+template <int NM, typename R = double>
 struct SPrim {
-    double a[NM], r[NM], u[3], p, V[9], e[NM];
+    R a[NM], r[NM], u[3], p, V[9], e[NM];
 };
-template <int NM>
+template <int NM, typename R = double>
 struct SSt {  // face state along one direction d
-    double al[NM], ar[NM], rho, u[3], E, p, a, sig[3], V[9], are[NM];
+    R al[NM], ar[NM], rho, u[3], E, p, a, sig[3], V[9], are[NM];
 };
 
-template <int NM>
-__device__ __forceinline__ void load_sprim(const Level& lv, const MF& U, int i, int j, int k, SPrim<NM>& P) {
+template <int NM, typename R>
+__device__ __forceinline__ void load_sprim(const Level& lv, const MF& U, int i, int j, int k, SPrim<NM, R>& P) {
     bool f[3];
     map3(lv, i, j, k, f);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        P.a[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
-        P.r[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
-        P.e[m] = lv.plastic ? U.at(lv.cs[EPSILON] + m, i, j, k) : 0.0;
+        P.a[m] = (R)U.at(lv.cs[ALPHA] + m, i, j, k);
+        P.r[m] = (R)U.at(lv.cs[RHO_K] + m, i, j, k);
+        P.e[m] = lv.plastic ? (R)U.at(lv.cs[EPSILON] + m, i, j, k) : R(0);
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        const double v = U.at(lv.cs[VELOCITY] + r, i, j, k);
+        const R v = (R)U.at(lv.cs[VELOCITY] + r, i, j, k);
         P.u[r] = f[r] ? -v : v;
     }
-    P.p = U.at(lv.cs[PRESSURE], i, j, k);
+    P.p = (R)U.at(lv.cs[PRESSURE], i, j, k);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) P.V[q] = U.at(lv.cs[V_TENSOR] + q, i, j, k);
+    for (int q = 0; q < 9; ++q) P.V[q] = (R)U.at(lv.cs[V_TENSOR] + q, i, j, k);
 }
 
-template <int NM>
-__device__ __forceinline__ void muscl_s(const SPrim<NM>& m_, const SPrim<NM>& c, const SPrim<NM>& p_, SPrim<NM>& L, SPrim<NM>& R) {
+template <int NM, typename Rt>
+__device__ __forceinline__ void muscl_s(const SPrim<NM, Rt>& m_, const SPrim<NM, Rt>& c, const SPrim<NM, Rt>& p_,
+                                        SPrim<NM, Rt>& L, SPrim<NM, Rt>& R) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         muscl1(m_.a[m], c.a[m], p_.a[m], L.a[m], R.a[m]);
@@ -247,33 +262,33 @@ __device__ __forceinline__ void muscl_s(const SPrim<NM>& m_, const SPrim<NM>& c,
 
 // primitiveToConservative + getSoundSpeed of a reconstructed solid state
 // (boxaccesscellarray.cpp:124-174, 254-285), keeping sigma column d.
-template <int NM>
-__device__ __forceinline__ void p2c_sound_s(const Level& lv, const SPrim<NM>& P, int d, SSt<NM>& S) {
-    double rho = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ void p2c_sound_s(const Level& lv, const SPrim<NM, R>& P, int d, SSt<NM, R>& S) {
+    R rho = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         S.al[m] = P.a[m];
         S.ar[m] = P.a[m] * P.r[m];
         rho += S.ar[m];
-        S.are[m] = lv.m[m].plastic ? P.e[m] * S.ar[m] : 0.0;
+        S.are[m] = mat<R>(lv, m).plastic ? P.e[m] * S.ar[m] : R(0);
     }
-    double dH[9];
-    const double hj2 = hencky(P.V, dH);
-    double ke = 0.0;
+    R dH[9];
+    const R hj2 = hencky(P.V, dH);
+    R ke = R(0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         S.u[q] = P.u[q];
-        ke += 0.5 * rho * P.u[q] * P.u[q];
+        ke += R(0.5) * rho * P.u[q] * P.u[q];
     }
-    double G, Ent, Pnt;
-    eos_sums_s<NM>(lv, S.al, S.ar, P.r, hj2, G, Ent, Pnt);
+    R G, Ent, Pnt;
+    eos_sums_s<NM, R>(lv, S.al, S.ar, P.r, hj2, G, Ent, Pnt);
     S.E = P.p * G + Ent - Pnt + ke;
     S.p = P.p;
     S.rho = rho;
-    stress_col<NM>(lv, S.al, P.p, dH, d, S.sig);
+    stress_col<NM, R>(lv, S.al, P.p, dH, d, S.sig);
 #pragma unroll
     for (int q = 0; q < 9; ++q) S.V[q] = P.V[q];
-    S.a = sound_s<NM>(lv, S.al, S.ar, P.r, rho, P.p, hj2);
+    S.a = sound_s<NM, R>(lv, S.al, S.ar, P.r, rho, P.p, hj2);
 }
 
 // conservativeToPrimitive (boxaccesscellarray.cpp:67-122) of a solid cell;
@@ -297,43 +312,43 @@ __device__ __forceinline__ double c2p_s(const Level& lv, const double* al, const
         ke += 0.5 * rho * u[q] * u[q];
     }
     double G, Ent, Pnt;
-    eos_sums_s<NM>(lv, al, ar, rk, hj2, G, Ent, Pnt);
+    eos_sums_s<NM, double>(lv, al, ar, rk, hj2, G, Ent, Pnt);
     p = (E - ke - Ent + Pnt) / G;
     return hj2;
 }
 
 // transverseWaveSpeed of a star state (boxaccesscellarray.cpp:287-324)
-template <int NM>
-__device__ __forceinline__ double transverse(const Level& lv, const double* al, const double* rk) {
-    double b = 0.0, g = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ R transverse(const Level& lv, const R* al, const R* rk) {
+    R b = R(0), g = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m)
-        if (lv.m[m].solid) b += (al[m] / lv.m[m].Gg) * shearG(lv.m[m]) / rk[m];
-    if (__builtin_isnan(b) || b < 0.0) b = 0.0;
+        if (mat<R>(lv, m).solid) b += (al[m] / mat<R>(lv, m).Gg) * shearG(mat<R>(lv, m)) / rk[m];
+    if (tisnan(b) || b < R(0)) b = R(0);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) g += al[m] / lv.m[m].Gg;
-    double t = __builtin_sqrt(b / g);
-    if (__builtin_isnan(t)) t = 0.0;
+    for (int m = 0; m < NM; ++m) g += al[m] / mat<R>(lv, m).Gg;
+    R t = tsqrt(b / g);
+    if (tisnan(t)) t = R(0);
     return t;
 }
 
 // ------------------------------------------------------------------ 5-wave flux
 // Star state of hllc.cpp:12-73 (solid form).
-template <int NM>
+template <int NM, typename R = double>
 struct SStar {
-    double al[NM], ar[NM], rk[NM], rho, ru[3], u[3], E, V[9], are[NM];
+    R al[NM], ar[NM], rk[NM], rho, ru[3], u[3], E, V[9], are[NM];
 };
-template <int NM>
-__device__ __forceinline__ void star_s(const Level& lv, const SSt<NM>& K, const double* ruK, double SK, double Ss, int d,
-                                       SStar<NM>& T) {
-    const double mult = (__builtin_fabs((Ss - K.u[d]) / Ss) < 1E-20) ? 1.0 : (SK - K.u[d]) / (SK - Ss);
-    T.rho = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ void star_s(const Level& lv, const SSt<NM, R>& K, const R* ruK, R SK, R Ss, int d,
+                                       SStar<NM, R>& T) {
+    const R mult = (tabs((Ss - K.u[d]) / Ss) < R(1E-20)) ? R(1) : (SK - K.u[d]) / (SK - Ss);
+    T.rho = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         T.al[m] = K.al[m];
         T.ar[m] = mult * K.ar[m];
         T.rk[m] = T.ar[m] / T.al[m];
-        T.are[m] = lv.m[m].plastic ? mult * K.are[m] : 0.0;
+        T.are[m] = mat<R>(lv, m).plastic ? mult * K.are[m] : R(0);
         T.rho += T.ar[m];
     }
 #pragma unroll
@@ -354,8 +369,8 @@ struct SCons {
 };
 
 // Physical flux (flux.cpp:5-47) of the conserved variables of state K.
-template <int NM>
-__device__ __forceinline__ void pflux_s(const SSt<NM>& K, const double* ruK, int d, double* f) {
+template <int NM, typename R>
+__device__ __forceinline__ void pflux_s(const SSt<NM, R>& K, const R* ruK, int d, R* f) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         f[m] = K.al[m] * K.u[d];
@@ -364,7 +379,7 @@ __device__ __forceinline__ void pflux_s(const SSt<NM>& K, const double* ruK, int
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) f[2 * NM + q] = ruK[q] * K.u[d] - K.sig[q];
-    double vds = 0.0;
+    R vds = R(0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) vds += K.u[q] * K.sig[q];
     f[2 * NM + 3] = K.E * K.u[d] - vds;
@@ -376,71 +391,72 @@ __device__ __forceinline__ void pflux_s(const SSt<NM>& K, const double* ruK, int
 
 // calc_5Wave_fluxes for one face (hllc.cpp:75-380).  F layout: the conserved
 // vector (3 NM + 13), then S*, then VSTAR row d (3).
-template <int NM>
-__device__ inline void wave5(const Level& lv, const SSt<NM>& L, const SSt<NM>& R, int d, double* F, unsigned& st) {
+template <int NM, typename Rt>
+__device__ inline void wave5(const Level& lv, const SSt<NM, Rt>& L, const SSt<NM, Rt>& R, int d, Rt* F,
+                             unsigned& st) {
     constexpr int NC = 3 * NM + 13;
-    double ruL[3], ruR[3];
+    Rt ruL[3], ruR[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         ruL[q] = L.u[q] * L.rho;
         ruR[q] = R.u[q] * R.rho;
     }
-    const double SR = dmax(__builtin_fabs(L.u[d]) + L.a, __builtin_fabs(R.u[d]) + R.a);
-    const double SL = -SR;
-    const double Ss = (-R.sig[d] + L.sig[d] + ruL[d] * (SL - L.u[d]) - ruR[d] * (SR - R.u[d])) /
+    const Rt SR = tmax(tabs(L.u[d]) + L.a, tabs(R.u[d]) + R.a);
+    const Rt SL = -SR;
+    const Rt Ss = (-R.sig[d] + L.sig[d] + ruL[d] * (SL - L.u[d]) - ruR[d] * (SR - R.u[d])) /
                       (L.rho * (SL - L.u[d]) - R.rho * (SR - R.u[d]));
-    if (__builtin_isnan(SL) || __builtin_isnan(SR) || __builtin_isnan(Ss)) st |= 2u;
+    if (tisnan(SL) || tisnan(SR) || tisnan(Ss)) st |= 2u;
     F[NC] = Ss;
 
-    SStar<NM> Ls, Rs;
-    double SLT = 0.0, SRT = 0.0;
+    SStar<NM, Rt> Ls, Rs;
+    Rt SLT = Rt(0), SRT = Rt(0);
     // which: 0 flux of K only, 1 K + K*, 2 K + K* + K**; left/right K
     int kind;
     bool left;
-    if (SL >= 0.0) {
-        star_s<NM>(lv, L, ruL, SL, Ss, d, Ls);
+    if (SL >= Rt(0)) {
+        star_s<NM, Rt>(lv, L, ruL, SL, Ss, d, Ls);
         kind = 0;
         left = true;
-    } else if (Ss >= 0.0) {
-        star_s<NM>(lv, L, ruL, SL, Ss, d, Ls);
-        SLT = Ss - transverse<NM>(lv, Ls.al, Ls.rk);
+    } else if (Ss >= Rt(0)) {
+        star_s<NM, Rt>(lv, L, ruL, SL, Ss, d, Ls);
+        SLT = Ss - transverse<NM, Rt>(lv, Ls.al, Ls.rk);
         left = true;
-        if (SLT >= 0.0) {
+        if (SLT >= Rt(0)) {
             kind = 1;
         } else {
-            star_s<NM>(lv, R, ruR, SR, Ss, d, Rs);
-            SRT = Ss + transverse<NM>(lv, Rs.al, Rs.rk);
+            star_s<NM, Rt>(lv, R, ruR, SR, Ss, d, Rs);
+            SRT = Ss + transverse<NM, Rt>(lv, Rs.al, Rs.rk);
             kind = 2;
         }
-    } else if (SR >= 0.0) {
-        star_s<NM>(lv, R, ruR, SR, Ss, d, Rs);
-        SRT = Ss + transverse<NM>(lv, Rs.al, Rs.rk);
+    } else if (SR >= Rt(0)) {
+        star_s<NM, Rt>(lv, R, ruR, SR, Ss, d, Rs);
+        SRT = Ss + transverse<NM, Rt>(lv, Rs.al, Rs.rk);
         left = false;
-        if (SRT >= 0.0) {
-            star_s<NM>(lv, L, ruL, SL, Ss, d, Ls);
-            SLT = Ss - transverse<NM>(lv, Ls.al, Ls.rk);
+        if (SRT >= Rt(0)) {
+            star_s<NM, Rt>(lv, L, ruL, SL, Ss, d, Ls);
+            SLT = Ss - transverse<NM, Rt>(lv, Ls.al, Ls.rk);
             kind = 2;
         } else {
             kind = 1;
         }
     } else {
-        star_s<NM>(lv, R, ruR, SR, Ss, d, Rs);
+        star_s<NM, Rt>(lv, R, ruR, SR, Ss, d, Rs);
         kind = 0;
         left = false;
     }
-    const SSt<NM>& K = left ? L : R;
-    const double* ruK = left ? ruL : ruR;
-    const SStar<NM>& Ks = left ? Ls : Rs;
-    const double SK = left ? SL : SR;
-    const double SKT = left ? SLT : SRT;
+    const SSt<NM, Rt>& K = left ? L : R;
+    const Rt* ruK = left ? ruL : ruR;
+    const SStar<NM, Rt>& Ks = left ? Ls : Rs;
+    const Rt SK = left ? SL : SR;
+    const Rt SKT = left ? SLT : SRT;
 
-    double f[NC];
-    pflux_s<NM>(K, ruK, d, f);
+    Rt f[NC];
+    pflux_s<NM, Rt>(K, ruK, d, f);
     // alpha: flux of the star state
 #pragma unroll
     for (int m = 0; m < NM; ++m) F[m] = Ks.al[m] * Ks.u[d];
     // conserved vectors of K and K*
-    double uK[NC], uS[NC];
+    Rt uK[NC], uS[NC];
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         uK[m] = K.al[m];
@@ -474,17 +490,17 @@ __device__ inline void wave5(const Level& lv, const SSt<NM>& L, const SSt<NM>& R
         for (int c = 0; c < 3; ++c) F[NC + 1 + c] = Ks.V[d * 3 + c];
     } else {
         // getStarStarState (hllc.cpp:87-129) with getSigmaStar (hllc.cpp:75-83)
-        double sigS[3];
-        const double aL = Ls.rho * (Ss - SLT), aR = Rs.rho * (Ss - SRT);
+        Rt sigS[3];
+        const Rt aL = Ls.rho * (Ss - SLT), aR = Rs.rho * (Ss - SRT);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             sigS[i] = (Ls.rho * (Ss - SLT) * Rs.rho * (Ss - SRT) * (L.u[i] - R.u[i]) + Ls.rho * (Ss - SLT) * R.sig[i] -
                        Rs.rho * (Ss - SRT) * L.sig[i]) /
                       (aL - aR);
-        double uss[NC];
+        Rt uss[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n) uss[n] = uS[n];
-        double ussu[3] = {Ks.u[0], Ks.u[1], Ks.u[2]};
+        Rt ussu[3] = {Ks.u[0], Ks.u[1], Ks.u[2]};
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             if (r == d) continue;
@@ -504,7 +520,7 @@ __device__ inline void wave5(const Level& lv, const SSt<NM>& L, const SSt<NM>& R
 #pragma unroll
         for (int c = 0; c < 3; ++c) F[NC + 1 + c] = uss[2 * NM + 4 + d * 3 + c];
     }
-    if (__builtin_isnan(SL) || __builtin_isnan(SR) || __builtin_isnan(Ss) || __builtin_isnan(SLT) || __builtin_isnan(SRT)) st |= 2u;
+    if (tisnan(SL) || tisnan(SR) || tisnan(Ss) || tisnan(SLT) || tisnan(SRT)) st |= 2u;
 }
 
 // ------------------------------------------------------------------ plastic

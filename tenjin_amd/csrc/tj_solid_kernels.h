@@ -6,9 +6,9 @@
 
 namespace tj {
 
-template <int NM>
-__device__ __forceinline__ SSt<NM> shfl_state(const SSt<NM>& S, int src) {
-    SSt<NM> T;
+template <int NM, typename R>
+__device__ __forceinline__ SSt<NM, R> shfl_state(const SSt<NM, R>& S, int src) {
+    SSt<NM, R> T;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         T.al[m] = __shfl(S.al[m], src, 64);
@@ -31,12 +31,12 @@ __device__ __forceinline__ SSt<NM> shfl_state(const SSt<NM>& S, int src) {
 
 // The left image of a reconstructed state across a reflective face:
 // reflect_odd flips the normal velocity and momentum only (boundary.cpp:3-42).
-template <int NM>
-__device__ __forceinline__ void flip_s(SSt<NM>& S, int d) { S.u[d] = -S.u[d]; }
+template <int NM, typename R>
+__device__ __forceinline__ void flip_s(SSt<NM, R>& S, int d) { S.u[d] = -S.u[d]; }
 
 // x pass of the solid path (same wave layout as k_flux_x, no THINC).
-template <int NM>
-__global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, Faces F, Clock* clk) {
+template <int NM, typename R = double>
+__global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, FacesT<R> F, Clock* clk) {
     if (clk->done) return;
     constexpr int PER = 63;
     constexpr int NF = 3 * NM + 17;
@@ -51,27 +51,27 @@ __global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, Faces F, Clock
     const int c = seg * PER - 1 + lane;
     bool fl;
     const int s = bcmap(c, nx, lv.bc_lo[0], lv.bc_hi[0], fl);
-    SSt<NM> SLs, SRs;
+    SSt<NM, R> SLs, SRs;
     {
-        SPrim<NM> Pm, Pc, Pp, PL, PR;
-        load_sprim<NM>(lv, U, s - 1, j, k, Pm);
-        load_sprim<NM>(lv, U, s, j, k, Pc);
-        load_sprim<NM>(lv, U, s + 1, j, k, Pp);
-        muscl_s<NM>(Pm, Pc, Pp, PL, PR);
-        p2c_sound_s<NM>(lv, PL, 0, SLs);
-        p2c_sound_s<NM>(lv, PR, 0, SRs);
+        SPrim<NM, R> Pm, Pc, Pp, PL, PR;
+        load_sprim<NM, R>(lv, U, s - 1, j, k, Pm);
+        load_sprim<NM, R>(lv, U, s, j, k, Pc);
+        load_sprim<NM, R>(lv, U, s + 1, j, k, Pp);
+        muscl_s<NM, R>(Pm, Pc, Pp, PL, PR);
+        p2c_sound_s<NM, R>(lv, PL, 0, SLs);
+        p2c_sound_s<NM, R>(lv, PR, 0, SRs);
     }
-    SSt<NM> Lf = shfl_state<NM>(SRs, lane > 0 ? lane - 1 : 0);
+    SSt<NM, R> Lf = shfl_state<NM, R>(SRs, lane > 0 ? lane - 1 : 0);
     unsigned st = 0;
     if (lane >= 1 && c >= 0 && c <= nx) {
         if (c == 0) {
             Lf = SRs;
-            if (lv.bc_lo[0]) flip_s<NM>(Lf, 0);
+            if (lv.bc_lo[0]) flip_s<NM, R>(Lf, 0);
         }
-        SSt<NM> Rf = SLs;
-        if (c == nx && lv.bc_hi[0]) flip_s<NM>(Rf, 0);
-        double Fv[NF];
-        wave5<NM>(lv, Lf, Rf, 0, Fv, st);
+        SSt<NM, R> Rf = SLs;
+        if (c == nx && lv.bc_hi[0]) flip_s<NM, R>(Rf, 0);
+        R Fv[NF];
+        wave5<NM, R>(lv, Lf, Rf, 0, Fv, st);
 #pragma unroll
         for (int q = 0; q < NF; ++q) F.at(q, c, j, k) = Fv[q];
     }
@@ -79,8 +79,8 @@ __global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, Faces F, Clock
 }
 
 // y / z pass of the solid path: lanes along x, marching along D.
-template <int NM, int D>
-__global__ __launch_bounds__(256) void k_flux_march_s(Level lv, MF U, Faces F, int seglen, Clock* clk) {
+template <int NM, int D, typename R = double>
+__global__ __launch_bounds__(256) void k_flux_march_s(Level lv, MF U, FacesT<R> F, int seglen, Clock* clk) {
     if (clk->done) return;
     constexpr int NF = 3 * NM + 17;
     const int lane = threadIdx.x & 63;
@@ -106,38 +106,38 @@ __global__ __launch_bounds__(256) void k_flux_march_s(Level lv, MF U, Faces F, i
     auto faceidx = [&](int r) {
         return D == 1 ? ((long long)ck(r) * (ny + 1) + r) * nx + i : ((long long)r * ny + cj(r)) * nx + i;
     };
-    SPrim<NM> Pm, Pc, Pp;
-    load_sprim<NM>(lv, U, i, cj(rb - 1), ck(rb - 1), Pm);
-    load_sprim<NM>(lv, U, i, cj(rb), ck(rb), Pc);
-    SSt<NM> prevR;
+    SPrim<NM, R> Pm, Pc, Pp;
+    load_sprim<NM, R>(lv, U, i, cj(rb - 1), ck(rb - 1), Pm);
+    load_sprim<NM, R>(lv, U, i, cj(rb), ck(rb), Pc);
+    SSt<NM, R> prevR;
     unsigned st = 0;
     for (int r = rb; r <= re; ++r) {
-        load_sprim<NM>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
-        SSt<NM> SLs, SRs;
+        load_sprim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
+        SSt<NM, R> SLs, SRs;
         {
-            SPrim<NM> PL, PR;
-            muscl_s<NM>(Pm, Pc, Pp, PL, PR);
-            p2c_sound_s<NM>(lv, PL, D, SLs);
-            p2c_sound_s<NM>(lv, PR, D, SRs);
+            SPrim<NM, R> PL, PR;
+            muscl_s<NM, R>(Pm, Pc, Pp, PL, PR);
+            p2c_sound_s<NM, R>(lv, PL, D, SLs);
+            p2c_sound_s<NM, R>(lv, PR, D, SRs);
         }
-        auto emit = [&](int face, const SSt<NM>& A, const SSt<NM>& B) {
-            double Fv[NF];
-            wave5<NM>(lv, A, B, D, Fv, st);
+        auto emit = [&](int face, const SSt<NM, R>& A, const SSt<NM, R>& B) {
+            R Fv[NF];
+            wave5<NM, R>(lv, A, B, D, Fv, st);
             if (!act) return;
             const long long fi = faceidx(face);
 #pragma unroll
             for (int q = 0; q < NF; ++q) F.p[q * F.cs + fi] = Fv[q];
         };
         if (r == 0 && f0 == 0) {
-            SSt<NM> img = SRs;
-            if (lv.bc_lo[D]) flip_s<NM>(img, D);
+            SSt<NM, R> img = SRs;
+            if (lv.bc_lo[D]) flip_s<NM, R>(img, D);
             emit(0, img, SLs);
         } else if (r >= f0) {
             emit(r, prevR, SLs);
         }
         if (r == nd - 1 && f1 == nd + 1) {
-            SSt<NM> img = SLs;
-            if (lv.bc_hi[D]) flip_s<NM>(img, D);
+            SSt<NM, R> img = SLs;
+            if (lv.bc_hi[D]) flip_s<NM, R>(img, D);
             emit(nd, SRs, img);
         }
         prevR = SRs;
@@ -205,9 +205,9 @@ __device__ __forceinline__ void store_scell(const Level& lv, const MF& U, int i,
 // plasticUpdate (main.cpp:154-157, plastic.cpp:227-244), cleanUpV
 // (main.cpp:164-167), cleanUpAlpha (main.cpp:169-171), the NaN scan and the
 // next getTimeStep.
-template <int NM, int STAGE>
-__global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo, Faces F0, Faces F1, Faces F2, Clock* clk,
-                                                  unsigned ops) {
+template <int NM, int STAGE, typename T = double>
+__global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo, FacesT<T> F0, FacesT<T> F1, FacesT<T> F2,
+                                                  Clock* clk, unsigned ops) {
     if (clk->done) return;
     constexpr int NC = 3 * NM + 13;
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
@@ -230,27 +230,27 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
 #pragma unroll
         for (int q = 0; q < 3; ++q) u0[q] = C.u[q];
         for (int d = 0; d < lv.dim; ++d) {
-            const Faces& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
+            const FacesT<T>& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
             const int i1 = i + (d == 0), j1 = j + (d == 1), k1 = k + (d == 2);
             const double dtdx = dt / lv.dx[d];
-            const double s0 = Fd.at(NC, i, j, k), s1 = Fd.at(NC, i1, j1, k1);
+            const double s0 = (double)Fd.at(NC, i, j, k), s1 = (double)Fd.at(NC, i1, j1, k1);
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
-                C.al[m] += dtdx * (Fd.at(m, i, j, k) - Fd.at(m, i1, j1, k1) - a0[m] * (s0 - s1));
-                C.ar[m] += dtdx * (Fd.at(NM + m, i, j, k) - Fd.at(NM + m, i1, j1, k1));
+                C.al[m] += dtdx * ((double)Fd.at(m, i, j, k) - (double)Fd.at(m, i1, j1, k1) - a0[m] * (s0 - s1));
+                C.ar[m] += dtdx * ((double)Fd.at(NM + m, i, j, k) - (double)Fd.at(NM + m, i1, j1, k1));
                 if (lv.plastic)
-                    C.are[m] += dtdx * (Fd.at(2 * NM + 13 + m, i, j, k) - Fd.at(2 * NM + 13 + m, i1, j1, k1));
+                    C.are[m] += dtdx * ((double)Fd.at(2 * NM + 13 + m, i, j, k) - (double)Fd.at(2 * NM + 13 + m, i1, j1, k1));
             }
 #pragma unroll
-            for (int q = 0; q < 3; ++q) C.ru[q] += dtdx * (Fd.at(2 * NM + q, i, j, k) - Fd.at(2 * NM + q, i1, j1, k1));
-            C.E += dtdx * (Fd.at(2 * NM + 3, i, j, k) - Fd.at(2 * NM + 3, i1, j1, k1));
+            for (int q = 0; q < 3; ++q) C.ru[q] += dtdx * ((double)Fd.at(2 * NM + q, i, j, k) - (double)Fd.at(2 * NM + q, i1, j1, k1));
+            C.E += dtdx * ((double)Fd.at(2 * NM + 3, i, j, k) - (double)Fd.at(2 * NM + 3, i1, j1, k1));
 #pragma unroll
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const int n = 2 * NM + 4 + r * 3 + c;
-                    const double vs0 = Fd.at(NC + 1 + c, i, j, k), vs1 = Fd.at(NC + 1 + c, i1, j1, k1);
-                    C.V[r * 3 + c] += dtdx * (Fd.at(n, i, j, k) - Fd.at(n, i1, j1, k1) - (2.0 / 3.0) * V0[r * 3 + c] * (s0 - s1) +
+                    const double vs0 = (double)Fd.at(NC + 1 + c, i, j, k), vs1 = (double)Fd.at(NC + 1 + c, i1, j1, k1);
+                    C.V[r * 3 + c] += dtdx * ((double)Fd.at(n, i, j, k) - (double)Fd.at(n, i1, j1, k1) - (2.0 / 3.0) * V0[r * 3 + c] * (s0 - s1) +
                                               u0[r] * (vs0 - vs1));
                 }
         }
@@ -301,7 +301,9 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
             if (ops & 8u) {
                 const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
                 Uo.at(lv.cs[SOUNDSPEED], i, j, k) = a;
-                for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
+                #pragma unroll
+                for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
+                    if (d < lv.dim) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
             }
         }
         store_scell<NM>(lv, Uo, i, j, k, C);
@@ -326,7 +328,9 @@ __global__ __launch_bounds__(256) void k_sound_dt_s(Level lv, MF U, Clock* clk) 
         load_scell<NM>(lv, U, i, j, k, C);
         const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
         U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
-        for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
+        #pragma unroll
+        for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
+            if (d < lv.dim) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
     }
     block_min_to(cand, &clk->dtmin_bits);
 }
@@ -466,7 +470,9 @@ __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsi
         if (ops & 8u) {
             const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
             U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
-            for (int d = 0; d < lv.dim; ++d) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
+            #pragma unroll
+            for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
+                if (d < lv.dim) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
         }
         store_scell<NM>(lv, U, i, j, k, C);
     }
