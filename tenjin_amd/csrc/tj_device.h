@@ -108,8 +108,13 @@ template <typename R>
 __device__ __forceinline__ R tmax(R a, R b) { return (a < b) ? b : a; }
 template <typename R>
 __device__ __forceinline__ R tsgn(R v) { return (R)((R(0) < v) - (v < R(0))); }
+// Division and square root: IEEE (correctly rounded) in FP64; in the FP32
+// parts of the mixed form the hardware reciprocal and square root (1 ulp),
+// whose error is far below that form's accuracy bar.
+__device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
 __device__ __forceinline__ bool tisnan(double x) { return __builtin_isnan(x); }

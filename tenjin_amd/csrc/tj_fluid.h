@@ -65,11 +65,11 @@ template <typename R>
 __device__ __forceinline__ R vanleer(R r) {
     if (r <= R(0)) return R(0);
     R num = (r < R(1)) ? R(2) * r : R(2);
-    return num / (R(1) + r);
+    return tdiv(num, R(1) + r);
 }
 template <typename R>
 __device__ __forceinline__ void muscl1(R um, R uc, R up, R& L, R& Rt) {
-    R r = (uc - um) / (up - uc);
+    R r = tdiv(uc - um, up - uc);
     if (tisinf(r) || tisnan(r)) r = R(0);
     R g = vanleer(r) * R(0.5) * (up - um);
     L = uc - R(0.5) * g;
@@ -95,11 +95,11 @@ __device__ __forceinline__ void eos_sums(const Level& lv, const R* a, const R* a
     Ent = R(0);
     Pnt = R(0);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) G += a[m] / mat<R>(lv, m).Gg;
+    for (int m = 0; m < NM; ++m) G += tdiv(a[m], mat<R>(lv, m).Gg);
 #pragma unroll
     for (int m = 0; m < NM; ++m) Ent += (mat<R>(lv, m).eref * ar[m] + R(0) * a[m]);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) Pnt += mat<R>(lv, m).pref * a[m] / mat<R>(lv, m).Gg + R(0) * a[m];
+    for (int m = 0; m < NM; ++m) Pnt += tdiv(mat<R>(lv, m).pref * a[m], mat<R>(lv, m).Gg) + R(0) * a[m];
 }
 
 // primitiveToConservative + getSoundSpeed of one reconstructed state
@@ -128,12 +128,12 @@ __device__ __forceinline__ void p2c_sound(const Level& lv, const Prim<NM, R>& P,
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         const MatT<R>& M = mat<R>(lv, m);
-        R c2 = P.p * M.gp1 / P.r[m] - M.pref / P.r[m];
-        a += tmax(R(0), M.xi * c2 * S.ar[m] / rho);
+        R c2 = tdiv(P.p * M.gp1, P.r[m]) - tdiv(M.pref, P.r[m]);
+        a += tmax(R(0), tdiv(M.xi * c2 * S.ar[m], rho));
         xt += M.xi * S.al[m];
     }
     if (a <= R(0)) a = R(1E-6);
-    S.a = tsqrt(a / xt);
+    S.a = tsqrt(tdiv(a, xt));
 }
 
 // conservativeToPrimitive (boxaccesscellarray.cpp:67-122), fluid part.
@@ -187,8 +187,8 @@ __device__ __forceinline__ void hllc(const FSt<NM, R>& L, const FSt<NM, R>& Rs, 
     }
     const R SR = tmax(tabs(L.u[d]) + L.a, tabs(Rr.u[d]) + Rr.a);
     const R SL = -SR;
-    const R Ss = (-sigR + sigL + ruL[d] * (SL - L.u[d]) - ruR[d] * (SR - Rr.u[d])) /
-                 (L.rho * (SL - L.u[d]) - Rr.rho * (SR - Rr.u[d]));
+    const R Ss = tdiv(-sigR + sigL + ruL[d] * (SL - L.u[d]) - ruR[d] * (SR - Rr.u[d]),
+                      L.rho * (SL - L.u[d]) - Rr.rho * (SR - Rr.u[d]));
     F[2 * NM + 4] = Ss;
     // branch selection of calc_fluxes; K is the state the star state is built from
     const bool useL = (SL >= R(0)) || (Ss >= R(0));
@@ -198,7 +198,7 @@ __device__ __forceinline__ void hllc(const FSt<NM, R>& L, const FSt<NM, R>& Rs, 
     const R SK = useL ? SL : SR;
     const R sigK = useL ? sigL : sigR;
     // star state
-    const R mult = (tabs((Ss - K.u[d]) / Ss) < R(1E-20)) ? R(1) : (SK - K.u[d]) / (SK - Ss);
+    const R mult = (tabs(tdiv(Ss - K.u[d], Ss)) < R(1E-20)) ? R(1) : tdiv(SK - K.u[d], SK - Ss);
     R arS[NM], rhoS = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
@@ -208,8 +208,8 @@ __device__ __forceinline__ void hllc(const FSt<NM, R>& L, const FSt<NM, R>& Rs, 
     R ruS[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) ruS[q] = (q == d) ? mult * K.rho * Ss : mult * ruK[q];
-    const R uSd = ruS[d] / rhoS;
-    const R ES = mult * K.rho * (K.E / K.rho + (Ss - K.u[d]) * (Ss - (sigK) / (K.rho * (SK - K.u[d]))));
+    const R uSd = tdiv(ruS[d], rhoS);
+    const R ES = mult * K.rho * (tdiv(K.E, K.rho) + (Ss - K.u[d]) * (Ss - tdiv(sigK, K.rho * (SK - K.u[d]))));
     // physical flux of K (flux.cpp:33-47) with sigma = -p I
     R vds = R(0);
 #pragma unroll
@@ -252,7 +252,7 @@ __device__ inline void arec(const Level& lv, const MF& U, int i, int j, int k, i
         const R mx = tmax(aL, aR) - mn;
         const R theta = tsgn(aR - aL);
         const R eps = R(1E-20);
-        const R C = (mx < eps) ? R(0) : (aC - mn + eps) / (mx + eps);
+        const R C = (mx < eps) ? R(0) : tdiv(aC - mn + eps, mx + eps);
         const R dl = R(1E-4);
         const bool mixed = (C > dl) && (C < (R(0.999999) - dl)) && ((aR - aC) * (aC - aL) >= R(0));
         A.thl[m] = R(0);
@@ -266,14 +266,14 @@ __device__ inline void arec(const Level& lv, const MF& U, int i, int j, int k, i
                 const R ny = yny * (a(1, 1) + R(2) * a(0, 1) + a(-1, 1) - a(-1, -1) - R(2) * a(0, -1) - a(1, -1));
                 const R norm = tsqrt(nx * nx + ny * ny);
                 if (norm < R(1E-20)) nvc = R(0);
-                else nvc = (d == 0) ? tabs(nx / norm) : tabs(ny / norm);
+                else nvc = (d == 0) ? tabs(tdiv(nx, norm)) : tabs(tdiv(ny, norm));
             }
             const R beta = nvc * R(lv.beta0) + R(0.01);
             const R coshB = tcosh(beta), tanhB = ttanh(beta);
             const R B = texp(theta * beta * ((R(2) * C) - R(1)));
-            const R Ac = ((B / coshB) - R(1)) / tanhB;
-            A.thr[m] = mn + (mx / R(2)) * (R(1) + theta * ((tanhB + Ac) / (R(1) + Ac * tanhB)));
-            A.thl[m] = mn + (mx / R(2)) * (R(1) + theta * Ac);
+            const R Ac = tdiv(tdiv(B, coshB) - R(1), tanhB);
+            A.thr[m] = mn + tdiv(mx, R(2)) * (R(1) + theta * tdiv(tanhB + Ac, R(1) + Ac * tanhB));
+            A.thl[m] = mn + tdiv(mx, R(2)) * (R(1) + theta * Ac);
         }
     }
 }
@@ -315,7 +315,7 @@ __device__ __forceinline__ unsigned cleanup_alpha(R* a) {
     }
     if (tot <= R(0)) st |= 8u;
 #pragma unroll
-    for (int m = 0; m < NM; ++m) a[m] *= R(1) / tot;
+    for (int m = 0; m < NM; ++m) a[m] *= tdiv(R(1), tot);
     return st;
 }
 

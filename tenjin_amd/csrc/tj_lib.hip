@@ -14,6 +14,7 @@
 
 using namespace tj;
 
+
 struct tj_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -515,9 +516,9 @@ static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, con
             const long long no = d == 1 ? lv.n[2] : lv.n[1];
             const unsigned g = grid1(nstrip * nseg * no, 4);
             if (d == 1)
-                hipLaunchKernelGGL((k_flux_march_s<2, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], c->seglen, c->d_clk);
+                hipLaunchKernelGGL((k_flux_lmarch_s<2, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], c->seglen, c->d_clk);
             else
-                hipLaunchKernelGGL((k_flux_march_s<2, 2, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[2], c->seglen, c->d_clk);
+                hipLaunchKernelGGL((k_flux_lmarch_s<2, 2, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[2], c->seglen, c->d_clk);
         }
     } else if (d == 0) {
         const int per = th ? 61 : 63;

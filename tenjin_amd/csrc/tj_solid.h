@@ -39,7 +39,7 @@ __device__ __forceinline__ void inv3(const R* M, R* Mi) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) Mi[i * 3 + j] = cof[j * 3 + i] / D;
+        for (int j = 0; j < 3; ++j) Mi[i * 3 + j] = tdiv(cof[j * 3 + i], D);
 }
 template <typename R>
 __device__ __forceinline__ void mm3(const R* A, const R* B, R* C) {  // C = A B
@@ -127,27 +127,27 @@ __device__ inline void sqrtm_spd(const double* Ain, double* S) {
 // eta = 1 - rho0/rho, with the Mie-Gruneisen form p = p_H + Gamma rho (e - e_H).
 template <typename R>
 __device__ __forceinline__ R pH(const MatT<R>& M, R rho) {
-    const R eta = R(1) - M.rho0 / rho;
+    const R eta = R(1) - tdiv(M.rho0, rho);
     const R den = R(1) - M.s * eta;
-    return M.rho0 * M.c0 * M.c0 * eta / (den * den);
+    return tdiv(M.rho0 * M.c0 * M.c0 * eta, den * den);
 }
 template <typename R>
 __device__ __forceinline__ R eH(const MatT<R>& M, R rho) {
-    const R eta = R(1) - M.rho0 / rho;
-    return pH(M, rho) * eta / (R(2) * M.rho0);
+    const R eta = R(1) - tdiv(M.rho0, rho);
+    return tdiv(pH(M, rho) * eta, R(2) * M.rho0);
 }
 template <typename R>
 __device__ __forceinline__ R dpH(const MatT<R>& M, R rho) {
-    const R eta = R(1) - M.rho0 / rho;
+    const R eta = R(1) - tdiv(M.rho0, rho);
     const R den = R(1) - M.s * eta;
-    const R detadrho = M.rho0 / (rho * rho);
-    return M.rho0 * M.c0 * M.c0 * (R(1) + M.s * eta) / (den * den * den) * detadrho;
+    const R detadrho = tdiv(M.rho0, rho * rho);
+    return tdiv(M.rho0 * M.c0 * M.c0 * (R(1) + M.s * eta), den * den * den) * detadrho;
 }
 template <typename R>
 __device__ __forceinline__ R deH(const MatT<R>& M, R rho) {
-    const R eta = R(1) - M.rho0 / rho;
-    const R detadrho = M.rho0 / (rho * rho);
-    return (dpH(M, rho) * eta + pH(M, rho) * detadrho) / (R(2) * M.rho0);
+    const R eta = R(1) - tdiv(M.rho0, rho);
+    const R detadrho = tdiv(M.rho0, rho * rho);
+    return tdiv(dpH(M, rho) * eta + pH(M, rho) * detadrho, R(2) * M.rho0);
 }
 template <typename R>
 __device__ __forceinline__ R shearG(const MatT<R>& M) { return M.eos == 1 ? M.G0 : R(0); }
@@ -160,7 +160,7 @@ __device__ __forceinline__ void eos_sums_s(const Level& lv, const R* a, const R*
     Ent = R(0);
     Pnt = R(0);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) G += a[m] / mat<R>(lv, m).Gg;
+    for (int m = 0; m < NM; ++m) G += tdiv(a[m], mat<R>(lv, m).Gg);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         const MatT<R>& M = mat<R>(lv, m);
@@ -170,8 +170,8 @@ __device__ __forceinline__ void eos_sums_s(const Level& lv, const R* a, const R*
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         const MatT<R>& M = mat<R>(lv, m);
-        if (M.eos == 1) Pnt += (a[m] / M.Gg) * pH(M, rk[m]) + (a[m] / M.Gg) * ((rk[m] * R(0) - shearG(M)) * hj2);
-        else Pnt += M.pref * a[m] / M.Gg + R(0) * a[m];
+        if (M.eos == 1) Pnt += tdiv(a[m], M.Gg) * pH(M, rk[m]) + tdiv(a[m], M.Gg) * ((rk[m] * R(0) - shearG(M)) * hj2);
+        else Pnt += tdiv(M.pref * a[m], M.Gg) + R(0) * a[m];
     }
 }
 // getSoundSpeedContribution: stiffened gas (equationofstate.cpp:61-64) or the
@@ -179,9 +179,9 @@ __device__ __forceinline__ void eos_sums_s(const Level& lv, const R* a, const R*
 template <typename R>
 __device__ __forceinline__ R c2_s(const MatT<R>& M, R p, R rk, R hj2) {
     if (M.eos == 1)
-        return (p * M.gp1 - pH(M, rk) + M.gp1 * M.G0 * hj2) / rk + dpH(M, rk) - M.Gg * rk * deH(M, rk) +
-               (R(4.0) / R(3.0)) * M.G0 / rk;
-    return p * M.gp1 / rk - M.pref / rk;
+        return tdiv(p * M.gp1 - pH(M, rk) + M.gp1 * M.G0 * hj2, rk) + dpH(M, rk) - M.Gg * rk * deH(M, rk) +
+               tdiv((R(4.0) / R(3.0)) * M.G0, rk);
+    return tdiv(p * M.gp1, rk) - tdiv(M.pref, rk);
 }
 template <int NM, typename R>
 __device__ __forceinline__ R sound_s(const Level& lv, const R* al, const R* ar, const R* rk, R rho, R p, R hj2) {
@@ -189,11 +189,11 @@ __device__ __forceinline__ R sound_s(const Level& lv, const R* al, const R* ar, 
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         const MatT<R>& M = mat<R>(lv, m);
-        a += tmax(R(0), M.xi * c2_s(M, p, rk[m], hj2) * ar[m] / rho);
+        a += tmax(R(0), tdiv(M.xi * c2_s(M, p, rk[m], hj2) * ar[m], rho));
         xt += M.xi * al[m];
     }
     if (a <= R(0)) a = R(1E-6);
-    return tsqrt(a / xt);
+    return tsqrt(tdiv(a, xt));
 }
 // stressTensor column d (boxaccesscellarray.cpp:176-216)
 template <int NM, typename R>
@@ -204,11 +204,11 @@ __device__ __forceinline__ void stress_col(const Level& lv, const R* al, R p, co
     for (int m = 0; m < NM; ++m)
         if (mat<R>(lv, m).solid) {
             solid = 1;
-            Gt += shearG(mat<R>(lv, m)) * (al[m] / mat<R>(lv, m).Gg);
+            Gt += shearG(mat<R>(lv, m)) * tdiv(al[m], mat<R>(lv, m).Gg);
         }
 #pragma unroll
-    for (int m = 0; m < NM; ++m) Gi += al[m] / mat<R>(lv, m).Gg;
-    Gt = Gt / Gi;
+    for (int m = 0; m < NM; ++m) Gi += tdiv(al[m], mat<R>(lv, m).Gg);
+    Gt = tdiv(Gt, Gi);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
         sig[r] = solid ? -p * (r == d ? R(1) : R(0)) + R(2) * Gt * dH[r * 3 + d] : -p * (r == d ? R(1) : R(0));
@@ -323,11 +323,11 @@ __device__ __forceinline__ R transverse(const Level& lv, const R* al, const R* r
     R b = R(0), g = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m)
-        if (mat<R>(lv, m).solid) b += (al[m] / mat<R>(lv, m).Gg) * shearG(mat<R>(lv, m)) / rk[m];
+        if (mat<R>(lv, m).solid) b += tdiv(tdiv(al[m], mat<R>(lv, m).Gg) * shearG(mat<R>(lv, m)), rk[m]);
     if (tisnan(b) || b < R(0)) b = R(0);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) g += al[m] / mat<R>(lv, m).Gg;
-    R t = tsqrt(b / g);
+    for (int m = 0; m < NM; ++m) g += tdiv(al[m], mat<R>(lv, m).Gg);
+    R t = tsqrt(tdiv(b, g));
     if (tisnan(t)) t = R(0);
     return t;
 }
@@ -341,24 +341,24 @@ struct SStar {
 template <int NM, typename R>
 __device__ __forceinline__ void star_s(const Level& lv, const SSt<NM, R>& K, const R* ruK, R SK, R Ss, int d,
                                        SStar<NM, R>& T) {
-    const R mult = (tabs((Ss - K.u[d]) / Ss) < R(1E-20)) ? R(1) : (SK - K.u[d]) / (SK - Ss);
+    const R mult = (tabs(tdiv(Ss - K.u[d], Ss)) < R(1E-20)) ? R(1) : tdiv(SK - K.u[d], SK - Ss);
     T.rho = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         T.al[m] = K.al[m];
         T.ar[m] = mult * K.ar[m];
-        T.rk[m] = T.ar[m] / T.al[m];
+        T.rk[m] = tdiv(T.ar[m], T.al[m]);
         T.are[m] = mat<R>(lv, m).plastic ? mult * K.are[m] : R(0);
         T.rho += T.ar[m];
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         T.ru[r] = (r == d) ? mult * K.rho * Ss : mult * ruK[r];
-        T.u[r] = T.ru[r] / T.rho;
+        T.u[r] = tdiv(T.ru[r], T.rho);
 #pragma unroll
         for (int c = 0; c < 3; ++c) T.V[r * 3 + c] = (r == d) ? K.V[r * 3 + c] : mult * K.V[r * 3 + c];
     }
-    T.E = mult * K.rho * (K.E / K.rho + (Ss - K.u[d]) * (Ss - (K.sig[d]) / (K.rho * (SK - K.u[d]))));
+    T.E = mult * K.rho * (tdiv(K.E, K.rho) + (Ss - K.u[d]) * (Ss - tdiv(K.sig[d], K.rho * (SK - K.u[d]))));
 }
 
 // conservative-variable vector of a state (order of the flux layout):
@@ -403,8 +403,8 @@ __device__ inline void wave5(const Level& lv, const SSt<NM, Rt>& L, const SSt<NM
     }
     const Rt SR = tmax(tabs(L.u[d]) + L.a, tabs(R.u[d]) + R.a);
     const Rt SL = -SR;
-    const Rt Ss = (-R.sig[d] + L.sig[d] + ruL[d] * (SL - L.u[d]) - ruR[d] * (SR - R.u[d])) /
-                      (L.rho * (SL - L.u[d]) - R.rho * (SR - R.u[d]));
+    const Rt Ss = tdiv(-R.sig[d] + L.sig[d] + ruL[d] * (SL - L.u[d]) - ruR[d] * (SR - R.u[d]),
+                       L.rho * (SL - L.u[d]) - R.rho * (SR - R.u[d]));
     if (tisnan(SL) || tisnan(SR) || tisnan(Ss)) st |= 2u;
     F[NC] = Ss;
 
@@ -494,9 +494,9 @@ __device__ inline void wave5(const Level& lv, const SSt<NM, Rt>& L, const SSt<NM
         const Rt aL = Ls.rho * (Ss - SLT), aR = Rs.rho * (Ss - SRT);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            sigS[i] = (Ls.rho * (Ss - SLT) * Rs.rho * (Ss - SRT) * (L.u[i] - R.u[i]) + Ls.rho * (Ss - SLT) * R.sig[i] -
-                       Rs.rho * (Ss - SRT) * L.sig[i]) /
-                      (aL - aR);
+            sigS[i] = tdiv(Ls.rho * (Ss - SLT) * Rs.rho * (Ss - SRT) * (L.u[i] - R.u[i]) + Ls.rho * (Ss - SLT) * R.sig[i] -
+                               Rs.rho * (Ss - SRT) * L.sig[i],
+                           aL - aR);
         Rt uss[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n) uss[n] = uS[n];
@@ -504,16 +504,16 @@ __device__ inline void wave5(const Level& lv, const SSt<NM, Rt>& L, const SSt<NM
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             if (r == d) continue;
-            uss[2 * NM + r] += (sigS[r] - K.sig[r]) / (Ss - SKT);
-            ussu[r] = uss[2 * NM + r] / Ks.rho;
+            uss[2 * NM + r] += tdiv(sigS[r] - K.sig[r], Ss - SKT);
+            ussu[r] = tdiv(uss[2 * NM + r], Ks.rho);
         }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             if (r == d) continue;
-            uss[2 * NM + 3] += (ussu[r] * sigS[r] - K.u[r] * K.sig[r]) / (Ss - SKT);
+            uss[2 * NM + 3] += tdiv(ussu[r] * sigS[r] - K.u[r] * K.sig[r], Ss - SKT);
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                uss[2 * NM + 4 + r * 3 + c] += (uss[2 * NM + 4 + d * 3 + c] * (ussu[r] - K.u[r])) / (Ss - SKT);
+                uss[2 * NM + 4 + r * 3 + c] += tdiv(uss[2 * NM + 4 + d * 3 + c] * (ussu[r] - K.u[r]), Ss - SKT);
         }
 #pragma unroll
         for (int n = NM; n < NC; ++n) F[n] = f[n] + SK * (uS[n] - uK[n]) + SKT * (uss[n] - uS[n]);

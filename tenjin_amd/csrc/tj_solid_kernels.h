@@ -78,19 +78,74 @@ __global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, FacesT<R> F, C
     if (st) atomicOr(&clk->status, st);
 }
 
-// y / z pass of the solid path: lanes along x, marching along D.
+// A solid face state in LDS, one slot per field, lane-minor (conflict-free).
+template <int NM, typename R>
+struct SStSlots {
+    static constexpr int N = 5 * NM + 15;
+};
+template <int NM, typename R>
+__device__ __forceinline__ void sst_store(R* sh, int lane, const SSt<NM, R>& S) {
+    int q = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        sh[(q++) * 64 + lane] = S.al[m];
+        sh[(q++) * 64 + lane] = S.ar[m];
+        sh[(q++) * 64 + lane] = S.are[m];
+    }
+    sh[(q++) * 64 + lane] = S.rho;
+    sh[(q++) * 64 + lane] = S.E;
+    sh[(q++) * 64 + lane] = S.p;
+    sh[(q++) * 64 + lane] = S.a;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        sh[(q++) * 64 + lane] = S.u[c];
+        sh[(q++) * 64 + lane] = S.sig[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) sh[(q++) * 64 + lane] = S.V[c];
+}
+template <int NM, typename R>
+__device__ __forceinline__ void sst_load(const R* sh, int lane, SSt<NM, R>& S) {
+    int q = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        S.al[m] = sh[(q++) * 64 + lane];
+        S.ar[m] = sh[(q++) * 64 + lane];
+        S.are[m] = sh[(q++) * 64 + lane];
+    }
+    S.rho = sh[(q++) * 64 + lane];
+    S.E = sh[(q++) * 64 + lane];
+    S.p = sh[(q++) * 64 + lane];
+    S.a = sh[(q++) * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        S.u[c] = sh[(q++) * 64 + lane];
+        S.sig[c] = sh[(q++) * 64 + lane];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) S.V[c] = sh[(q++) * 64 + lane];
+}
+
+// y / z pass of the solid path: lanes along x (coalesced), marching along D
+// over a segment of faces.  The right state of row r-1 waits for face r in
+// the wave's own LDS slice (not in registers), and the three primitive rows
+// of each row are reloaded from cache: one reconstruction per cell as in a
+// register march, with the register footprint of the x pass.
 template <int NM, int D, typename R = double>
-__global__ __launch_bounds__(256) void k_flux_march_s(Level lv, MF U, FacesT<R> F, int seglen, Clock* clk) {
+__global__ __launch_bounds__(256) void k_flux_lmarch_s(Level lv, MF U, FacesT<R> F, int seglen, Clock* clk) {
     if (clk->done) return;
     constexpr int NF = 3 * NM + 17;
+    constexpr int NS = SStSlots<NM, R>::N;
+    __shared__ R shm[4 * NS * 64];
     const int lane = threadIdx.x & 63;
+    R* sh = shm + (threadIdx.x >> 6) * NS * 64;
     const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const int nd = lv.n[D];
     const int no = (D == 1) ? nz : ny;
     const int nstrip = (nx + 63) / 64;
     const int nseg = (nd + 1 + seglen - 1) / seglen;
-    if (wave >= (long long)nstrip * nseg * no) return;
+    if (wave >= (long long)nstrip * nseg * no) return;  // uniform per wave; no block barrier below
     const int strip = (int)(wave % nstrip);
     const long long rest = wave / nstrip;
     const int seg = (int)(rest % nseg);
@@ -106,43 +161,45 @@ __global__ __launch_bounds__(256) void k_flux_march_s(Level lv, MF U, FacesT<R> 
     auto faceidx = [&](int r) {
         return D == 1 ? ((long long)ck(r) * (ny + 1) + r) * nx + i : ((long long)r * ny + cj(r)) * nx + i;
     };
-    SPrim<NM, R> Pm, Pc, Pp;
-    load_sprim<NM, R>(lv, U, i, cj(rb - 1), ck(rb - 1), Pm);
-    load_sprim<NM, R>(lv, U, i, cj(rb), ck(rb), Pc);
-    SSt<NM, R> prevR;
     unsigned st = 0;
-    for (int r = rb; r <= re; ++r) {
+    // reconstruct row r (MUSCL on rows r-1, r, r+1; p2c and sound speed)
+    auto recon = [&](int r, SSt<NM, R>& SL, SSt<NM, R>& SR) {
+        SPrim<NM, R> Pm, Pc, Pp, PL, PR;
+        load_sprim<NM, R>(lv, U, i, cj(r - 1), ck(r - 1), Pm);
+        load_sprim<NM, R>(lv, U, i, cj(r), ck(r), Pc);
         load_sprim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
-        SSt<NM, R> SLs, SRs;
-        {
-            SPrim<NM, R> PL, PR;
-            muscl_s<NM, R>(Pm, Pc, Pp, PL, PR);
-            p2c_sound_s<NM, R>(lv, PL, D, SLs);
-            p2c_sound_s<NM, R>(lv, PR, D, SRs);
-        }
-        auto emit = [&](int face, const SSt<NM, R>& A, const SSt<NM, R>& B) {
-            R Fv[NF];
-            wave5<NM, R>(lv, A, B, D, Fv, st);
-            if (!act) return;
-            const long long fi = faceidx(face);
+        muscl_s<NM, R>(Pm, Pc, Pp, PL, PR);
+        p2c_sound_s<NM, R>(lv, PL, D, SL);
+        p2c_sound_s<NM, R>(lv, PR, D, SR);
+    };
+    auto emit = [&](int face, const SSt<NM, R>& A, const SSt<NM, R>& B) {
+        R Fv[NF];
+        wave5<NM, R>(lv, A, B, D, Fv, st);
+        if (!act) return;
+        const long long fi = faceidx(face);
 #pragma unroll
-            for (int q = 0; q < NF; ++q) F.p[q * F.cs + fi] = Fv[q];
-        };
-        if (r == 0 && f0 == 0) {
-            SSt<NM, R> img = SRs;
-            if (lv.bc_lo[D]) flip_s<NM, R>(img, D);
-            emit(0, img, SLs);
-        } else if (r >= f0) {
-            emit(r, prevR, SLs);
+        for (int q = 0; q < NF; ++q) F.p[q * F.cs + fi] = Fv[q];
+    };
+    for (int r = rb; r <= re; ++r) {
+        SSt<NM, R> SLs, SRs;
+        recon(r, SLs, SRs);
+        if (r >= f0) {  // face r: left UR(r-1) (the image of UR(0) at the domain face), right UL(r)
+            SSt<NM, R> A;
+            if (r == 0) {
+                A = SRs;
+                if (lv.bc_lo[D]) flip_s<NM, R>(A, D);
+            } else {
+                sst_load<NM, R>(sh, lane, A);
+            }
+            emit(r, A, SLs);
         }
-        if (r == nd - 1 && f1 == nd + 1) {
-            SSt<NM, R> img = SLs;
-            if (lv.bc_hi[D]) flip_s<NM, R>(img, D);
-            emit(nd, SRs, img);
-        }
-        prevR = SRs;
-        Pm = Pc;
-        Pc = Pp;
+        sst_store<NM, R>(sh, lane, SRs);
+    }
+    if (f1 == nd + 1) {  // domain face nd: UR(nd-1) and the image of UL(nd-1)
+        SSt<NM, R> SLs, SRs;
+        recon(nd - 1, SLs, SRs);
+        if (lv.bc_hi[D]) flip_s<NM, R>(SLs, D);
+        emit(nd, SRs, SLs);
     }
     if (st) atomicOr(&clk->status, st);
 }
