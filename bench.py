@@ -67,13 +67,13 @@ SOLID_KERNEL_BYTES = {
     "update_stage2_post_mixed": 152 + 3 * 92 + 232 + 232 + 8,
 }
 # 8 (3 N_s + 2 N_c): N_s/N_c = 14/8 (two-material fluid), 27/19 (two-material solid)
-STEP_BYTES = {"inclusion2d": 464, "plate3d": 952}
+STEP_BYTES = {"inclusion2d": 464, "plate3d": 952, "tube1d": 464}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["inclusion2d", "plate3d"], default="inclusion2d")
+    ap.add_argument("--workload", choices=["inclusion2d", "plate3d", "tube1d"], default="inclusion2d")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 500 for inclusion2d, 40 for plate3d)")
     ap.add_argument("--warmup", type=int, default=10)
@@ -92,7 +92,7 @@ def parse():
                     help="run a plain 1 GiB device copy first (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 500 if args.workload == "inclusion2d" else 40
+        args.steps = {"inclusion2d": 500, "plate3d": 40, "tube1d": 1280}[args.workload]
     return args
 
 
@@ -124,7 +124,8 @@ def cpu_baseline(pb, budget_s: float, max_steps: int = 20):
     }
 
 
-TRAFFIC_FILES = {"inclusion2d": "pmc_traffic_shock_inclusion_2d.txt", "plate3d": "pmc_traffic_plate_impact_3d.txt"}
+TRAFFIC_FILES = {"inclusion2d": "pmc_traffic_shock_inclusion_2d.txt", "plate3d": "pmc_traffic_plate_impact_3d.txt",
+                 "tube1d": "pmc_traffic_shock_tube_1d.txt"}
 
 
 def read_traffic(workload: str, name: str):
@@ -137,6 +138,68 @@ def read_traffic(workload: str, name: str):
         if len(parts) >= 2 and parts[0] == "traffic_bytes_per_launch" and len(parts) >= 3 and parts[1] == name:
             return float(parts[2])
     return None
+
+
+def bench_tube1d(args):
+    """The 1D shock tube (1000 cells): latency-bound, so the loop bodies run as
+    replays of one captured hipGraph of 64 bodies; the eager device loop is
+    timed beside it."""
+    import torch
+    from tenjin_amd import GpuLevel, TJ_PREC_FP64, configs
+
+    pb = configs.shock_tube_1d()
+    group = 64
+    nrep = max(1, args.steps // group)
+    steps = nrep * group
+
+    def run(graph):
+        g = GpuLevel(pb)
+        g.init_from_prims(pb.prims, pb.shift_pressure)
+        g.begin_run(float("inf"))
+        if graph:
+            g.graph_capture(group, TJ_PREC_FP64)
+            g.graph_launch()  # warm-up
+        else:
+            g.advance(group, TJ_PREC_FP64)
+        g.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            if graph:
+                g.graph_launch()
+            else:
+                g.advance(group, TJ_PREC_FP64)
+        g.sync()
+        el = time.perf_counter() - t0
+        assert g.status() == 0
+        del g
+        return el
+
+    el_graph = run(True)
+    el_eager = run(False)
+    value = pb.ncells * steps / el_graph / 1e6
+    out = {
+        "metric": "Mcell-updates/s on 1 GPU",
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": group,
+        "ms_per_step": round(1e3 * el_graph / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "1D two-material stiffened-gas shock tube, 1000 cells, MUSCL+HLLC, CFL 0.5",
+                   "cells": pb.ncells, "precision": "fp64", "launch": f"hipGraph of {group} loop bodies",
+                   "parallelism": "single GPU"},
+        "eager_ms_per_step": round(1e3 * el_eager / steps, 5),
+        "graph_speedup": round(el_eager / el_graph, 3),
+        "roofline": None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(pb, args.cpu_seconds, 1000),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -156,6 +219,9 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.workload == "tube1d":
+        bench_tube1d(args)
+        return
     if args.workload == "inclusion2d":
         pb = configs.shock_inclusion_2d(nx=args.nx, ny=args.ny, box=args.box)
         kbytes = FLUID_KERNEL_BYTES

@@ -160,6 +160,16 @@ int tj_begin_run(tj_ctx* ctx, const tj_mf* S, double final_time);
  * fails (t >= final_time) or after a NaN stop are no-ops on the device. */
 int tj_advance_steps(tj_ctx* ctx, tj_mf* S, tj_mf* T, int nsteps, int prec);
 
+/* hipGraph form of tj_advance_steps (the loop body of main.cpp:121-191 as one
+ * graph): tj_graph_capture records nsteps loop bodies on S and T into a
+ * graph owned by the context (replacing any earlier one); each
+ * tj_graph_launch replays them on the context's stream.  The graph holds S
+ * and T by address: they must stay allocated, in place, until the next
+ * capture or tj_destroy.  With THINC the replay must start from the history
+ * phase it was captured at (true after whole steps; TJ_EINVAL otherwise). */
+int tj_graph_capture(tj_ctx* ctx, tj_mf* S, tj_mf* T, int nsteps, int prec);
+int tj_graph_launch(tj_ctx* ctx);
+
 /* The time loop of main.cpp:121-191 on the device words: dt, clamp, step,
  * NaN stop.  Returns steps taken in *n_out and final time in *t_out. */
 int tj_run(tj_ctx* ctx, tj_mf* S, tj_mf* T, double final_time, int max_steps, int prec,

@@ -77,7 +77,7 @@ EXPORTS = [
     "tj_comp_start", "tj_compute_dt", "tj_driver_dt", "tj_set_dt", "tj_get_clock", "tj_rk_stage",
     "tj_post_step", "tj_step", "tj_run", "tj_init_from_prims", "tj_c2p", "tj_derive", "tj_status",
     "tj_reset_history", "tj_synchronize", "tj_prof_enable", "tj_prof_report", "tj_set_tuning",
-    "tj_begin_run", "tj_advance_steps", "tj_lincomb",
+    "tj_begin_run", "tj_advance_steps", "tj_lincomb", "tj_graph_capture", "tj_graph_launch",
 ]
 
 _lib = None
@@ -126,6 +126,8 @@ def load_library(path: str = LIB_PATH):
     L.tj_begin_run.argtypes = [vp, mfp, dp]
     L.tj_advance_steps.argtypes = [vp, mfp, mfp, ip, ip]
     L.tj_lincomb.argtypes = [vp, mfp, dp, mfp, dp, mfp]
+    L.tj_graph_capture.argtypes = [vp, mfp, mfp, ip, ip]
+    L.tj_graph_launch.argtypes = [vp]
     _lib = L
     return L
 
@@ -260,6 +262,14 @@ class GpuLevel:
     def advance(self, nsteps: int, prec: int = TJ_PREC_FP64):
         self._chk(self.lib.tj_advance_steps(self.ctx, ctypes.byref(self.vS), ctypes.byref(self.vT), nsteps, prec),
                   "tj_advance_steps")
+
+    def graph_capture(self, nsteps: int, prec: int = TJ_PREC_FP64):
+        """Records nsteps loop bodies (dt, clamps, step, NaN stop) as one hipGraph."""
+        self._chk(self.lib.tj_graph_capture(self.ctx, ctypes.byref(self.vS), ctypes.byref(self.vT), nsteps, prec),
+                  "tj_graph_capture")
+
+    def graph_launch(self):
+        self._chk(self.lib.tj_graph_launch(self.ctx), "tj_graph_launch")
 
     def clock(self):
         t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()

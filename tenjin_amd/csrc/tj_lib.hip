@@ -40,6 +40,12 @@ struct tj_ctx {
     long long dpass[3] = {0, 0, 0};
     int seglen = 16;
     bool fuse = false;
+    // a captured run of loop bodies (tj_graph_capture) and the THINC pass
+    // counters it advances each launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    long long g_pass0 = 0, g_dpass0[3] = {0, 0, 0};
+    long long g_npass = 0, g_ndpass[3] = {0, 0, 0};
     // per-kernel event timing
     bool prof = false;
     struct Rec { const char* name; hipEvent_t a, b; };
@@ -195,10 +201,18 @@ int tj_create(tj_ctx** out, int device) {
     return TJ_OK;
 }
 
+static void graph_clear(tj_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
 void tj_destroy(tj_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    graph_clear(c);
     free_all(c);
     prof_drain(c);
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
@@ -683,6 +697,58 @@ int tj_advance_steps(tj_ctx* c, tj_mf* S, tj_mf* T, int nsteps, int prec) {
         hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 1);
         if ((r = tj_step(c, S, T, prec))) return r;
     }
+    return TJ_OK;
+}
+
+int tj_graph_capture(tj_ctx* c, tj_mf* S, tj_mf* T, int nsteps, int prec) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    if (nsteps < 1) { c->err = "graph_capture: nsteps < 1"; return TJ_EINVAL; }
+    int r;
+    if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
+    if ((r = check_prec(c, prec))) return r;  // scratch allocations happen before the capture
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    graph_clear(c);
+    const bool prof = c->prof;
+    c->prof = false;
+    const long long p0 = c->pass, d0[3] = {c->dpass[0], c->dpass[1], c->dpass[2]};
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rr = tj_advance_steps(c, S, T, nsteps, prec);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->prof = prof;
+    c->g_pass0 = p0;
+    c->g_npass = c->pass - p0;
+    for (int a = 0; a < 3; ++a) {
+        c->g_dpass0[a] = d0[a];
+        c->g_ndpass[a] = c->dpass[a] - d0[a];
+        c->dpass[a] = d0[a];
+    }
+    c->pass = p0;  // the counters advance when the graph is launched
+    if (rr) {
+        if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+        return rr;
+    }
+    HIPCHK(c, e);
+    c->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    return TJ_OK;
+}
+
+int tj_graph_launch(tj_ctx* c) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    if (!c->gexec) { c->err = "graph_launch: nothing captured"; return TJ_EINVAL; }
+    // the captured passes read and write the THINC history planes of a fixed
+    // ping-pong phase: the level must be at that phase again
+    bool same = ((c->pass - c->g_pass0) & 1) == 0;
+    for (int a = 0; a < 3; ++a) same = same && ((c->dpass[a] - c->g_dpass0[a]) & 1) == 0;
+    if (c->desc.THINC && !same) {
+        c->err = "graph_launch: THINC history phase differs from the captured one (capture again)";
+        return TJ_EINVAL;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    c->pass += c->g_npass;
+    for (int a = 0; a < 3; ++a) c->dpass[a] += c->g_ndpass[a];
     return TJ_OK;
 }
 
