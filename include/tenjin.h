@@ -202,6 +202,42 @@ int tj_status(tj_ctx* ctx, uint32_t* status, int clear);
 int tj_reset_history(tj_ctx* ctx);
 int tj_synchronize(tj_ctx* ctx);
 
+/* ---- Batched ensemble -------------------------------------------------
+ * Many independent fluid levels (1D and 2D, uneven sizes, each with its own
+ * materials, boundary conditions and time step) advanced together: each
+ * problem runs the loop body of main.cpp:121-191 with its own device clock,
+ * and every pass of a step is one launch over all problems of a dimension
+ * group.  A problem's results are bit-identical to running it alone with
+ * tj_run.  The reference has no batched driver; this serves ensembles of
+ * small problems that would each leave most of the GPU idle.
+ * Restrictions: fluid levels (SOLID = 0), two materials, one THINC setting
+ * for the whole batch.  S[p] and T[p] are caller-owned device views (state
+ * and stage buffer) of problem p, held by address until tj_batch_destroy.
+ * On an error from tj_batch_create, *batch may hold a batch for
+ * tj_batch_last_error; destroy it. */
+typedef struct tj_batch tj_batch;
+int tj_batch_create(tj_batch** batch, int device, int nprob, const tj_level_desc* descs, const tj_mf* S,
+                    const tj_mf* T);
+void tj_batch_destroy(tj_batch* batch);
+const char* tj_batch_last_error(const tj_batch* batch);
+/* Zeroes every problem's THINC history planes (as tj_reset_history). */
+int tj_batch_reset_history(tj_batch* batch);
+/* Rows per marching segment of problem p's y pass (tuning; default 16). */
+int tj_batch_set_seglen(tj_batch* batch, int prob, int seglen);
+/* tj_init_from_prims for every problem. */
+int tj_batch_init_from_prims(tj_batch* batch, int shift_pressure);
+/* tj_begin_run for every problem (one final time for all). */
+int tj_batch_begin_run(tj_batch* batch, double final_time);
+/* nsteps loop bodies for every problem (problems past their final time or
+ * stopped by a NaN are no-ops on the device). */
+int tj_batch_advance(tj_batch* batch, int nsteps, int prec);
+/* hipGraph form of tj_batch_advance (as tj_graph_capture / tj_graph_launch). */
+int tj_batch_graph_capture(tj_batch* batch, int nsteps, int prec);
+int tj_batch_graph_launch(tj_batch* batch);
+/* Per-problem t, n and status words (arrays of nprob; any may be NULL). Synchronises. */
+int tj_batch_clocks(tj_batch* batch, double* t, int* n, uint32_t* status);
+int tj_batch_synchronize(tj_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,9 @@ EXPORTS = [
     "tj_post_step", "tj_step", "tj_run", "tj_init_from_prims", "tj_c2p", "tj_derive", "tj_status",
     "tj_reset_history", "tj_synchronize", "tj_prof_enable", "tj_prof_report", "tj_set_tuning",
     "tj_begin_run", "tj_advance_steps", "tj_lincomb", "tj_graph_capture", "tj_graph_launch",
+    "tj_batch_create", "tj_batch_destroy", "tj_batch_last_error", "tj_batch_reset_history", "tj_batch_set_seglen",
+    "tj_batch_init_from_prims", "tj_batch_begin_run", "tj_batch_advance", "tj_batch_graph_capture",
+    "tj_batch_graph_launch", "tj_batch_clocks", "tj_batch_synchronize",
 ]
 
 _lib = None
@@ -128,6 +131,20 @@ def load_library(path: str = LIB_PATH):
     L.tj_lincomb.argtypes = [vp, mfp, dp, mfp, dp, mfp]
     L.tj_graph_capture.argtypes = [vp, mfp, mfp, ip, ip]
     L.tj_graph_launch.argtypes = [vp]
+    L.tj_batch_create.argtypes = [ctypes.POINTER(vp), ip, ip, ctypes.POINTER(LevelDesc), mfp, mfp]
+    L.tj_batch_destroy.argtypes = [vp]
+    L.tj_batch_destroy.restype = None
+    L.tj_batch_last_error.argtypes = [vp]
+    L.tj_batch_last_error.restype = ctypes.c_char_p
+    L.tj_batch_reset_history.argtypes = [vp]
+    L.tj_batch_set_seglen.argtypes = [vp, ip, ip]
+    L.tj_batch_init_from_prims.argtypes = [vp, ip]
+    L.tj_batch_begin_run.argtypes = [vp, dp]
+    L.tj_batch_advance.argtypes = [vp, ip, ip]
+    L.tj_batch_graph_capture.argtypes = [vp, ip, ip]
+    L.tj_batch_graph_launch.argtypes = [vp]
+    L.tj_batch_clocks.argtypes = [vp, ctypes.POINTER(dp), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_uint32)]
+    L.tj_batch_synchronize.argtypes = [vp]
     _lib = L
     return L
 
@@ -300,3 +317,99 @@ class GpuLevel:
 
     def derive(self):
         self._chk(self.lib.tj_derive(self.ctx, ctypes.byref(self.vS)), "tj_derive")
+
+
+class GpuBatch:
+    """An ensemble of independent fluid problems (1D and 2D) advanced together
+    on one GPU through the tj_batch_* entry points; S[p], T[p] are torch
+    float64 device tensors [ncomp][nz][ny][nx] per problem."""
+
+    def __init__(self, problems, device: int = 0):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise TenjinError("no GPU visible: the tenjin_amd update runs on the GPU only")
+        self.torch = torch
+        self.lib = load_library()
+        self.problems = list(problems)
+        n = len(self.problems)
+        dev = torch.device("cuda", device)
+        self.S, self.T = [], []
+        self._descs = (LevelDesc * n)()
+        vS, vT = (MFView * n)(), (MFView * n)()
+        for p, pb in enumerate(self.problems):
+            self._descs[p] = level_desc(pb)
+            nc = pb.layout.ncomp
+            S = torch.zeros((nc,) + pb.shape, dtype=torch.float64, device=dev)
+            T = torch.zeros_like(S)
+            self.S.append(S)
+            self.T.append(T)
+            for v, t in ((vS, S), (vT, T)):
+                nz, ny, nx = pb.shape
+                v[p].data = t.data_ptr()
+                v[p].jstride = nx
+                v[p].kstride = nx * ny
+                v[p].nstride = nx * ny * nz
+                v[p].ncomp = nc
+        self._vS, self._vT = vS, vT
+        self.b = ctypes.c_void_p()
+        rc = self.lib.tj_batch_create(ctypes.byref(self.b), device, n, self._descs, vS, vT)
+        if rc != 0:
+            msg = self.lib.tj_batch_last_error(self.b).decode() if self.b else ""
+            self.close()
+            raise TenjinError(f"tj_batch_create failed ({rc}): {msg}")
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.lib.tj_batch_last_error(self.b).decode() if self.b else ""
+            raise TenjinError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if self.b:
+            self.lib.tj_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_from_prims(self, shift_pressure: int = 0):
+        self.torch.cuda.synchronize()
+        for S, pb in zip(self.S, self.problems):
+            S.copy_(self.torch.from_numpy(np.ascontiguousarray(pb.prims, dtype=np.float64)))
+        self.torch.cuda.synchronize()
+        self._chk(self.lib.tj_batch_init_from_prims(self.b, shift_pressure), "tj_batch_init_from_prims")
+        self._chk(self.lib.tj_batch_reset_history(self.b), "tj_batch_reset_history")
+
+    def set_seglen(self, prob: int, seglen: int):
+        self._chk(self.lib.tj_batch_set_seglen(self.b, prob, seglen), "tj_batch_set_seglen")
+
+    def begin_run(self, final_time: float = float("inf")):
+        self._chk(self.lib.tj_batch_begin_run(self.b, final_time), "tj_batch_begin_run")
+
+    def advance(self, nsteps: int, prec: int = TJ_PREC_FP64):
+        self._chk(self.lib.tj_batch_advance(self.b, nsteps, prec), "tj_batch_advance")
+
+    def graph_capture(self, nsteps: int, prec: int = TJ_PREC_FP64):
+        self._chk(self.lib.tj_batch_graph_capture(self.b, nsteps, prec), "tj_batch_graph_capture")
+
+    def graph_launch(self):
+        self._chk(self.lib.tj_batch_graph_launch(self.b), "tj_batch_graph_launch")
+
+    def sync(self):
+        self._chk(self.lib.tj_batch_synchronize(self.b), "tj_batch_synchronize")
+        self.torch.cuda.synchronize()
+
+    def clocks(self):
+        n = len(self.problems)
+        t = (ctypes.c_double * n)()
+        k = (ctypes.c_int * n)()
+        st = (ctypes.c_uint32 * n)()
+        self._chk(self.lib.tj_batch_clocks(self.b, t, k, st), "tj_batch_clocks")
+        return list(t), list(k), list(st)
+
+    def state(self, p: int) -> np.ndarray:
+        self.sync()
+        return self.S[p].cpu().numpy()

@@ -330,3 +330,43 @@ def plate_impact(dim: int = 3, n: int = 256, box: int = 64, seed: int = 3, speed
         U[L.comp(V_TENSOR, 0, r, r)] = 1.0
     pb.prims = U
     return pb
+
+
+def ensemble(seed: int = 6, n1d: int = 128, n2d: int = 128, n1d_range=(500, 8000), side_range=(64, 512),
+             thinc: int = 1, steps: int = 200) -> List[Problem]:
+    """Ensemble of independent two-material stiffened-gas shock tubes: n1d 1D
+    tubes with N ~ U{n1d_range} cells and n2d 2D square tubes with side
+    ~ U{side_range} (planar interface at x = 0.5 with a one-mode sinusoidal
+    perturbation in y).  Per problem: gamma ~ U[1.2, 5] and p_inf ~ U[0, 1e9]
+    for each material, density ratio ~ logU[10, 1000] and pressure ratio
+    ~ logU[10, 1e4] (the dense, high-pressure side on the left), unit domain,
+    64-cell boxes, transmissive ends; 2D tubes reflective in y.  Draws come from
+    numpy's RandomState(seed) (MT19937)."""
+    rs = np.random.RandomState(seed)
+    out = []
+    for q in range(n1d + n2d):
+        dim = 1 if q < n1d else 2
+        n = int(rs.randint(n1d_range[0], n1d_range[1] + 1)) if dim == 1 else int(
+            rs.randint(side_range[0], side_range[1] + 1))
+        g0, g1 = rs.uniform(1.2, 5.0, 2)
+        pi0, pi1 = rs.uniform(0.0, 1e9, 2)
+        drat = float(np.exp(rs.uniform(np.log(10.0), np.log(1000.0))))
+        prat = float(np.exp(rs.uniform(np.log(10.0), np.log(1e4))))
+        amp = float(rs.uniform(0.0, 0.05))
+        mats = [Material.stiffened(float(g0), float(pi0)), Material.stiffened(float(g1), float(pi1))]
+        nn = (n, n if dim == 2 else 1, 1)
+        pb = Problem(f"ensemble_{q}_{dim}d", dim, nn, (1.0, 1.0, 1.0), 64, mats, THINC=thinc, steps=steps,
+                     bc_lo=(0, 1, 0), bc_hi=(0, 1, 0))
+        U = _empty_prims(pb)
+        nz, ny, nx = pb.shape
+        x = (np.arange(nx) + 0.5) / nx
+        y = (np.arange(ny) + 0.5) / ny
+        Y, X = np.meshgrid(y, x, indexing="ij")
+        xi = 0.5 + (amp * np.sin(2.0 * np.pi * Y) if dim == 2 else 0.0)
+        left = (X < xi)[None, :, :]
+        zero = np.zeros(pb.shape)
+        _set_two_phase(pb, U, left, drat * 1.0, 1.0, [zero, zero, zero],
+                       np.where(left, 1e5 * prat, 1e5))
+        pb.prims = U
+        out.append(pb)
+    return out

@@ -19,13 +19,13 @@ struct Clock {
 // reconstruct their cell (at the boundary-mapped cell s), neighbours' data
 // travel by wave shuffles, and lane l computes the flux of face c (the lower
 // face of cell c).  Faces per wave: 63 (MUSCL) or 61 (MUSCL + THINC/BVD).
-template <int NM, bool TH, typename R = double>
-__global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, FacesT<R> F, Hist h, Clock* clk) {
+template <int NM, bool TH, typename R>
+__device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const FacesT<R>& F, const Hist& h, Clock* clk,
+                                            long long wave) {
     if (clk->done) return;
     constexpr int H = TH ? 2 : 1;
     constexpr int PER = TH ? 61 : 63;
     const int lane = threadIdx.x & 63;
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const int nseg = (nx + 1 + PER - 1) / PER;
     if (wave >= (long long)nseg * ny * nz) return;  // uniform per wave
@@ -141,6 +141,10 @@ __global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, FacesT<R> F, His
         }
     }
     if (st) atomicOr(&clk->status, st);
+}
+template <int NM, bool TH, typename R = double>
+__global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, FacesT<R> F, Hist h, Clock* clk) {
+    flux_x_body<NM, TH, R>(lv, U, F, h, clk, (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
 // ------------------------------------------------------------------ update helpers
@@ -274,13 +278,12 @@ __device__ __forceinline__ void block_min_to(double cand, unsigned long long* ds
 #ifndef TJ_MARCH_WAVES_PER_EU
 #define TJ_MARCH_WAVES_PER_EU 1
 #endif
-template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1, typename R = double>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WAVES_PER_EU, 8))) void k_flux_march(
-    Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk, MF Ub = MF{}, MF Uo = MF{}, FacesT<R> Fx = FacesT<R>{},
-    FacesT<R> Fy = FacesT<R>{}, unsigned ops = 0u) {
+template <int NM, bool TH, int D, bool FUSE, int STAGE, typename R>
+__device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, const FacesT<R>& F, const Hist& h, int seglen,
+                                                Clock* clk, const MF& Ub, const MF& Uo, const FacesT<R>& Fx,
+                                                const FacesT<R>& Fy, unsigned ops, long long wave) {
     if (clk->done) return;
     const int lane = threadIdx.x & 63;
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const int nd = lv.n[D];
     const int no = (D == 1) ? nz : ny;
@@ -449,18 +452,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WA
     if (FUSE && STAGE == 2 && (ops & 8u)) wave_min_to(cand, &clk->dtmin_bits);
     if (st) atomicOr(&clk->status, st);
 }
+template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1, typename R = double>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WAVES_PER_EU, 8))) void k_flux_march(
+    Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk, MF Ub = MF{}, MF Uo = MF{}, FacesT<R> Fx = FacesT<R>{},
+    FacesT<R> Fy = FacesT<R>{}, unsigned ops = 0u) {
+    flux_march_body<NM, TH, D, FUSE, STAGE, R>(lv, U, F, h, seglen, clk, Ub, Uo, Fx, Fy, ops,
+                                               (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
 
 // ------------------------------------------------------------------ update
 // update (hllc.cpp:493-523) for every direction in x, y, z order, then
 // conservativeToPrimitive (hllc.cpp:689).  Stage 2 adds the RK average of
 // advance (hllc.cpp:705), cleanUpAlpha (main.cpp:169-171), the NaN scan
 // (main.cpp:183-187) and getTimeStep of the next step (main.cpp:128).
-template <int NM, int STAGE, typename T = double>
-__global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, FacesT<T> F0, FacesT<T> F1,
-                                                FacesT<T> F2, Clock* clk, unsigned ops) {
+// (one block of 256 cells: `block` is the block's index within the level)
+template <int NM, int STAGE, typename T>
+__device__ __forceinline__ void update_body(const Level& lv, const MF& Uin, const MF& Ub, const MF& Uo, const FacesT<T>& F0,
+                                            const FacesT<T>& F1, const FacesT<T>& F2, Clock* clk, unsigned ops,
+                                            long long block) {
     if (clk->done) return;
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long id = block * blockDim.x + threadIdx.x;
     const long long ncell = (long long)nx * ny * nz;
     double cand = __builtin_huge_val();
     unsigned st = 0;
@@ -485,12 +497,17 @@ __global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, 
     if (STAGE == 2 && (ops & 8u)) block_min_to(cand, &clk->dtmin_bits);
     if (st) atomicOr(&clk->status, st);
 }
+template <int NM, int STAGE, typename T = double>
+__global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, FacesT<T> F0, FacesT<T> F1,
+                                                FacesT<T> F2, Clock* clk, unsigned ops) {
+    update_body<NM, STAGE, T>(lv, Uin, Ub, Uo, F0, F1, F2, clk, ops, blockIdx.x);
+}
 
 // getSoundSpeed + boxTimeStepFunction + min over the level (timestep.cpp:7-70).
 template <int NM>
-__global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
+__device__ __forceinline__ void sound_dt_body(const Level& lv, const MF& U, Clock* clk, long long block) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long id = block * blockDim.x + threadIdx.x;
     const long long ncell = (long long)nx * ny * nz;
     double cand = __builtin_huge_val();
     if (id < ncell) {
@@ -511,6 +528,10 @@ __global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
             cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(U.at(lv.cs[VELOCITY] + d, i, j, k))));
     }
     block_min_to(cand, &clk->dtmin_bits);
+}
+template <int NM>
+__global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
+    sound_dt_body<NM>(lv, U, clk, blockIdx.x);
 }
 
 // dt = CFL * min (timestep.cpp:61-69); with `driver` the loop test and the
@@ -591,9 +612,9 @@ __global__ __launch_bounds__(256) void k_nancheck(Level lv, MF U, Clock* clk) {
 // setInitialConditions' pressure shift and primitiveToConservative
 // (initialise.cpp:104, boxaccesscellarray.cpp:124-174) over valid cells.
 template <int NM>
-__global__ __launch_bounds__(256) void k_init_p2c(Level lv, MF U, int shift) {
+__device__ __forceinline__ void init_p2c_body(const Level& lv, const MF& U, int shift, long long block) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long id = block * blockDim.x + threadIdx.x;
     if (id >= (long long)nx * ny * nz) return;
     const int i = (int)(id % nx);
     const int j = (int)((id / nx) % ny);
@@ -623,6 +644,10 @@ __global__ __launch_bounds__(256) void k_init_p2c(Level lv, MF U, int shift) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) U.at(lv.cs[RHOU] + q, i, j, k) = S.u[q] * S.rho;
     U.at(lv.cs[TOTAL_E], i, j, k) = S.E;
+}
+template <int NM>
+__global__ __launch_bounds__(256) void k_init_p2c(Level lv, MF U, int shift) {
+    init_p2c_body<NM>(lv, U, shift, blockIdx.x);
 }
 
 // conservativeToPrimitive over the valid cells.

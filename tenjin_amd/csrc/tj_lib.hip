@@ -224,33 +224,31 @@ void tj_destroy(tj_ctx* c) {
 const char* tj_last_error(const tj_ctx* c) { return c ? c->err.c_str() : "null context"; }
 void* tj_stream(tj_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
-int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
-    if (!c || !d) return TJ_EINVAL;
+// Validates a level description and fills the device-side Level (without
+// the edge-flag pointers); edges[a] receives the box-edge flags of
+// coordinate a (bit 0: first cell of a box, bit 1: last cell of a box).
+static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::vector<uint8_t> edges[3]) {
+    if (!d) return TJ_EINVAL;
     if (d->dim < 1 || d->dim > 3 || d->nmat < 1 || d->nmat > TJ_MAX_MATERIALS) {
-        c->err = "bind_level: bad dim or nmat";
+        err = "bind_level: bad dim or nmat";
         return TJ_EINVAL;
     }
     if (d->nmat != 2) {
-        c->err = "bind_level: this build instantiates two-material levels";
+        err = "bind_level: this build instantiates two-material levels";
         return TJ_EUNSUPPORTED;
     }
     if (d->SOLID && d->THINC) {
-        c->err = "bind_level: THINC with SOLID is not available in this build";
+        err = "bind_level: THINC with SOLID is not available in this build";
         return TJ_EUNSUPPORTED;
     }
     if (d->THINC && d->dim == 3) {
-        c->err = "bind_level: THINC is defined for 1D and 2D levels only (thinc.cpp:151-162)";
+        err = "bind_level: THINC is defined for 1D and 2D levels only (thinc.cpp:151-162)";
         return TJ_EUNSUPPORTED;
     }
     if (!d->MUSCL) {
-        c->err = "bind_level: MUSCL = 0 is not available in this build";
+        err = "bind_level: MUSCL = 0 is not available in this build";
         return TJ_EUNSUPPORTED;
     }
-    HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    free_all(c);
-    c->desc = *d;
-    Level& lv = c->lv;
     lv = Level{};
     lv.dim = d->dim;
     for (int a = 0; a < 3; ++a) {
@@ -258,7 +256,7 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
         lv.dx[a] = a < d->dim ? d->L[a] / d->n[a] : 1.0;
         lv.bc_lo[a] = a < d->dim ? d->bc_lo[a] : 0;
         lv.bc_hi[a] = a < d->dim ? d->bc_hi[a] : 0;
-        if (lv.n[a] < 1) { c->err = "bind_level: n < 1"; return TJ_EINVAL; }
+        if (lv.n[a] < 1) { err = "bind_level: n < 1"; return TJ_EINVAL; }
     }
     lv.yn[0] = -1.0 / (8.0 * lv.dx[0]);
     lv.yn[1] = -1.0 / (8.0 * lv.dx[1]);
@@ -303,9 +301,9 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
     }
     build_layout(*d, lv.cs);
     lv.ncells = (long long)lv.n[0] * lv.n[1] * lv.n[2];
-    c->nmat = d->nmat;
     for (int a = 0; a < 3; ++a) {
-        std::vector<uint8_t> fl(lv.n[a], 0);
+        std::vector<uint8_t>& fl = edges[a];
+        fl.assign(lv.n[a], 0);
         std::vector<int> st = a < d->dim ? chop_dim(lv.n[a], d->max_grid_size) : std::vector<int>{0};
         for (size_t b = 0; b < st.size(); ++b) {
             int lo = st[b];
@@ -313,6 +311,25 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
             fl[lo] |= 1;
             fl[hi] |= 2;
         }
+    }
+    return TJ_OK;
+}
+
+int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
+    if (!c || !d) return TJ_EINVAL;
+    Level lvn{};
+    std::vector<uint8_t> edges[3];
+    int r;
+    if ((r = make_level(c->err, d, lvn, edges))) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_all(c);
+    c->desc = *d;
+    Level& lv = c->lv;
+    lv = lvn;
+    c->nmat = d->nmat;
+    for (int a = 0; a < 3; ++a) {
+        const std::vector<uint8_t>& fl = edges[a];
         HIPCHK(c, hipMalloc(&c->d_edge[a], fl.size()));
         HIPCHK(c, hipMemcpy(c->d_edge[a], fl.data(), fl.size(), hipMemcpyHostToDevice));
         lv.edge[a] = c->d_edge[a];
@@ -879,3 +896,5 @@ int tj_synchronize(tj_ctx* c) {
 }
 
 }  // extern "C"
+
+#include "tj_batch.h"
