@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_b_flux_x(BTab t, int which, int q, int 
 
 // y pass (marching) over all problems of a 2D group
 template <bool TH, typename R>
-__global__ __launch_bounds__(256) void k_b_flux_y(BTab t, int which, int q, int e) {
+__device__ __forceinline__ void b_flux_y(const BTab& t, int which, int q, int e) {
     const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (gw >= t.start[t.np]) return;
     const int p = bfind(t.start, t.np, gw);
@@ -82,6 +82,16 @@ __global__ __launch_bounds__(256) void k_b_flux_y(BTab t, int which, int q, int 
     const FacesT<R> zf{};
     flux_march_body<2, TH, 1, false, 1, R>(P.lv, which ? P.T : P.S, bfaces<R>(P, 1), bhist(P, 1, q, e), P.seglen,
                                            P.clk, z, z, zf, zf, 0u, gw - t.start[p]);
+}
+template <bool TH, typename R>
+__global__ __launch_bounds__(256) void k_b_flux_y(BTab t, int which, int q, int e) {
+    b_flux_y<TH, R>(t, which, q, e);
+}
+// two waves per SIMD for the FP64 THINC pass, as k_flux_march_w2
+template <bool TH, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_b_flux_y_w2(BTab t, int which, int q,
+                                                                                              int e) {
+    b_flux_y<TH, R>(t, which, q, e);
 }
 
 // stage 1: T = c2p(S + dt L(S)); stage 2: S = 1/2 S + 1/2 c2p(T + dt L(T)) + post-step ops
@@ -290,7 +300,7 @@ int batch_step(tj_batch* b, int prec) {
                 const unsigned gy = grid1(b->tot[g][1], 4);
                 const BTab t = tab(b, g, 1);
                 if (b->thinc && mixed) hipLaunchKernelGGL((k_b_flux_y<true, float>), dim3(gy), dim3(256), 0, b->stream, t, which, q, e);
-                else if (b->thinc) hipLaunchKernelGGL((k_b_flux_y<true, double>), dim3(gy), dim3(256), 0, b->stream, t, which, q, e);
+                else if (b->thinc) hipLaunchKernelGGL((k_b_flux_y_w2<true, double>), dim3(gy), dim3(256), 0, b->stream, t, which, q, e);
                 else if (mixed) hipLaunchKernelGGL((k_b_flux_y<false, float>), dim3(gy), dim3(256), 0, b->stream, t, which, q, e);
                 else hipLaunchKernelGGL((k_b_flux_y<false, double>), dim3(gy), dim3(256), 0, b->stream, t, which, q, e);
                 if (b->thinc) {

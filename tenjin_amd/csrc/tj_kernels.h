@@ -275,21 +275,99 @@ __device__ __forceinline__ void block_min_to(double cand, unsigned long long* ds
 // completes each row as soon as its upper face is known (update in x, y, z
 // order with the earlier directions' faces read from their planes, then
 // finish_cell); the faces of this direction stay in registers.
-#ifndef TJ_MARCH_WAVES_PER_EU
-#define TJ_MARCH_WAVES_PER_EU 1
-#endif
+// Per-wave LDS slices of the marching pass, slot-major and lane-minor: the
+// right state of row r-1 (2 NM + 7 slots), the alpha reconstructions of rows
+// r-1 and r (THINC, 4 NM + 1 slots each) and the lower-face flux of the row
+// being completed (fused form, 2 NM + 5 slots).
+template <int NM, bool TH, bool FUSE>
+struct MarchSlots {
+    static constexpr int FST = 2 * NM + 7;
+    static constexpr int AREC = 4 * NM + 1;
+    static constexpr int FLX = 2 * NM + 5;
+    static constexpr int N = FST + (TH ? 2 * AREC : 0) + (FUSE ? FLX : 0);
+};
+template <int NM, typename R>
+__device__ __forceinline__ void fst_put(R* sh, int lane, const FSt<NM, R>& S) {
+    int q = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        sh[(q++) * 64 + lane] = S.al[m];
+        sh[(q++) * 64 + lane] = S.ar[m];
+    }
+    sh[(q++) * 64 + lane] = S.rho;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sh[(q++) * 64 + lane] = S.u[c];
+    sh[(q++) * 64 + lane] = S.E;
+    sh[(q++) * 64 + lane] = S.p;
+    sh[(q++) * 64 + lane] = S.a;
+}
+template <int NM, typename R>
+__device__ __forceinline__ void fst_get(const R* sh, int lane, FSt<NM, R>& S) {
+    int q = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        S.al[m] = sh[(q++) * 64 + lane];
+        S.ar[m] = sh[(q++) * 64 + lane];
+    }
+    S.rho = sh[(q++) * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) S.u[c] = sh[(q++) * 64 + lane];
+    S.E = sh[(q++) * 64 + lane];
+    S.p = sh[(q++) * 64 + lane];
+    S.a = sh[(q++) * 64 + lane];
+}
+template <int NM, typename R>
+__device__ __forceinline__ void arec_put(R* sh, int lane, const ARec<NM, R>& A) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        sh[(4 * m + 0) * 64 + lane] = A.ula[m];
+        sh[(4 * m + 1) * 64 + lane] = A.ura[m];
+        sh[(4 * m + 2) * 64 + lane] = A.thl[m];
+        sh[(4 * m + 3) * 64 + lane] = A.thr[m];
+    }
+    sh[(4 * NM) * 64 + lane] = (R)A.mixed;  // a small integer: exact in either type
+}
+template <int NM, typename R>
+__device__ __forceinline__ void arec_get(const R* sh, int lane, ARec<NM, R>& A) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        A.ula[m] = sh[(4 * m + 0) * 64 + lane];
+        A.ura[m] = sh[(4 * m + 1) * 64 + lane];
+        A.thl[m] = sh[(4 * m + 2) * 64 + lane];
+        A.thr[m] = sh[(4 * m + 3) * 64 + lane];
+    }
+    A.mixed = (int)sh[(4 * NM) * 64 + lane];
+}
+
+// One wave covers 64 columns (lanes along x, coalesced) and marches along
+// direction D over a segment of faces [f0, f1).  What passes from row r-1 to
+// row r -- the right face state, the THINC lookahead reconstructions, the
+// fused form's lower face -- waits in the wave's own LDS slice rather than in
+// registers, and the primitive rows are reloaded from cache, so the pass
+// keeps the register footprint of a single face.  With THINC the alpha
+// reconstruction runs one row ahead so the BVD of row r sees rows r-1 and
+// r+1.  FUSE (last direction of the level): the segment owns rows [f0, f1)
+// and completes each row as soon as its upper face is known (update in
+// x, y, z order with the earlier directions' faces read from their planes,
+// then finish_cell); the faces of this direction never reach HBM.
 template <int NM, bool TH, int D, bool FUSE, int STAGE, typename R>
 __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, const FacesT<R>& F, const Hist& h, int seglen,
                                                 Clock* clk, const MF& Ub, const MF& Uo, const FacesT<R>& Fx,
                                                 const FacesT<R>& Fy, unsigned ops, long long wave) {
+    using Slots = MarchSlots<NM, TH, FUSE>;
+    __shared__ R shm[4 * Slots::N * 64];
     if (clk->done) return;
     const int lane = threadIdx.x & 63;
+    R* const shR = shm + (threadIdx.x >> 6) * Slots::N * 64;  // right state of row r-1
+    R* const shAp = shR + Slots::FST * 64;                      // alpha reconstruction of row r-1
+    R* const shAc = shAp + Slots::AREC * 64;                    // alpha reconstruction of row r
+    R* const shF = shR + (Slots::FST + (TH ? 2 * Slots::AREC : 0)) * 64;  // lower face flux (fused)
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const int nd = lv.n[D];
     const int no = (D == 1) ? nz : ny;
     const int nstrip = (nx + 63) / 64;
     const int nseg = FUSE ? (nd + seglen - 1) / seglen : (nd + 1 + seglen - 1) / seglen;
-    if (wave >= (long long)nstrip * nseg * no) return;
+    if (wave >= (long long)nstrip * nseg * no) return;  // uniform per wave; no block barrier below
     const int strip = (int)(wave % nstrip);
     const long long rest = wave / nstrip;
     const int seg = (int)(rest % nseg);
@@ -305,38 +383,91 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
     auto cj = [&](int r) { return D == 1 ? r : o; };
     auto ck = [&](int r) { return D == 2 ? r : o; };
     auto cellidx = [&](int r) { return ((long long)ck(r) * ny + cj(r)) * nx + i; };
-    double cand = __builtin_huge_val();
-    R Fprev[2 * NM + 5];
     auto faceidx = [&](int r) {  // lower face r along D
         return D == 1 ? ((long long)ck(r) * (ny + 1) + r) * nx + i : ((long long)r * ny + cj(r)) * nx + i;
     };
-
-    Prim<NM, R> Pm, Pc, Pp;
-    load_prim<NM, R>(lv, U, i, cj(rb - 1), ck(rb - 1), Pm);
-    load_prim<NM, R>(lv, U, i, cj(rb), ck(rb), Pc);
-
-    ARec<NM, R> Aprev, Acur, Anext;
-    bool curValid = false;
+    double cand = __builtin_huge_val();
     unsigned st = 0;
-    FSt<NM, R> prevR;
     const R ynx = (R)lv.yn[0], yny = (R)lv.yn[1];
 
+    // complete row q (faces q and q+1 known) in the fused form
+    auto finish_row = [&](int q, const R* Flo, const R* Fhi) {
+        const int jj = cj(q), kk = ck(q);
+        double al[NM], ar[NM], ru[3], E, a0[NM];
+        load_cons<NM>(lv, U, i, jj, kk, al, ar, ru, E);
+#pragma unroll
+        for (int m = 0; m < NM; ++m) a0[m] = al[m];
+        const double dt = clk->dt;
+        {
+            R Fa[2 * NM + 5], Fb[2 * NM + 5];
+            load_faces<NM, R>(Fx, i, jj, kk, Fa);
+            load_faces<NM, R>(Fx, i + 1, jj, kk, Fb);
+            accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[0], Fa, Fb);
+        }
+        if (D == 2) {
+            R Fa[2 * NM + 5], Fb[2 * NM + 5];
+            load_faces<NM, R>(Fy, i, jj, kk, Fa);
+            load_faces<NM, R>(Fy, i, jj + 1, kk, Fb);
+            accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[1], Fa, Fb);
+        }
+        accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[D], Flo, Fhi);
+        if (act) finish_cell<NM, STAGE>(lv, Ub, Uo, i, jj, kk, al, ar, ru, E, ops, cand, st);
+    };
+    auto emit = [&](int face, const FSt<NM, R>& A, const FSt<NM, R>& B, R* Fv) {
+        R sa[NM];
+        hllc<NM, R>(A, B, D, Fv, sa);
+        if (!act) return;
+        const long long fi = faceidx(face);
+        if (!FUSE) {
+            const long long fcs = F.cs;
+#pragma unroll
+            for (int q = 0; q < 2 * NM + 5; ++q) F.p[q * fcs + fi] = Fv[q];
+        }
+        if (TH) {
+            if (face < nd) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m) h.star_out[m * h.pstride + cellidx(face)] = (double)sa[m];
+            }
+            if (face == nd || (face > 0 && (lv.edge[D][face - 1] & 2))) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m) h.edge_out[m * h.estride + fi] = (double)sa[m];
+            }
+        }
+    };
+
+    bool curValid = false;
+    FSt<NM, R> SLs, SRs;
     for (int r = rb; r <= re; ++r) {
-        load_prim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
         Prim<NM, R> PL, PR;
-        muscl<NM, R>(Pm, Pc, Pp, PL, PR);
+        {
+            Prim<NM, R> Pm, Pc, Pp;
+            load_prim<NM, R>(lv, U, i, cj(r - 1), ck(r - 1), Pm);
+            load_prim<NM, R>(lv, U, i, cj(r), ck(r), Pc);
+            load_prim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
+            muscl<NM, R>(Pm, Pc, Pp, PL, PR);
+        }
         if (TH) {
             const uint8_t ef = lv.edge[D][r];
             const bool blo = ef & 1, bhi = ef & 2;
             if (!curValid) {
-                arec<NM, R>(lv, U, i, cj(r), ck(r), D, ynx, yny, Acur);
-                if (!blo) arec<NM, R>(lv, U, i, cj(r - 1), ck(r - 1), D, ynx, yny, Aprev);
+                ARec<NM, R> A;
+                arec<NM, R>(lv, U, i, cj(r), ck(r), D, ynx, yny, A);
+                arec_put<NM, R>(shAc, lane, A);
+                if (!blo) {
+                    arec<NM, R>(lv, U, i, cj(r - 1), ck(r - 1), D, ynx, yny, A);
+                    arec_put<NM, R>(shAp, lane, A);
+                }
             }
+            ARec<NM, R> Anext;
             if (!bhi) arec<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), D, ynx, yny, Anext);
+            ARec<NM, R> Acur;
+            arec_get<NM, R>(shAc, lane, Acur);
             const int mixedFlag = (Acur.mixed >> (NM - 1)) & 1;
             const long long cr = cellidx(r);
             const long long cstride = (D == 1) ? nx : (long long)nx * ny;
             if (mixedFlag) {
+                ARec<NM, R> Aprev;
+                if (!blo) arec_get<NM, R>(shAp, lane, Aprev);
                 R URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
 #pragma unroll
                 for (int m = 0; m < NM; ++m) {
@@ -371,93 +502,65 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
                 for (int m = 0; m < NM; ++m)
                     if ((Acur.mixed >> m) & 1) h.urstar[m * h.pstride + cr] = (double)Acur.thr[m];
             }
-            Aprev = Acur;
-            Acur = Anext;
+            // rows move up by one: Aprev <- Acur, Acur <- Anext
+            arec_put<NM, R>(shAp, lane, Acur);
+            if (!bhi) arec_put<NM, R>(shAc, lane, Anext);
             curValid = !bhi;
         }
-        FSt<NM, R> SLs, SRs;
         p2c_sound<NM, R>(lv, PL, SLs);
         p2c_sound<NM, R>(lv, PR, SRs);
-        // complete row q (faces q and q+1 known) in the fused form
-        auto finish_row = [&](int q, const R* Flo, const R* Fhi) {
-            const int jj = cj(q), kk = ck(q);
-            double al[NM], ar[NM], ru[3], E, a0[NM];
-            load_cons<NM>(lv, U, i, jj, kk, al, ar, ru, E);
-#pragma unroll
-            for (int m = 0; m < NM; ++m) a0[m] = al[m];
-            const double dt = clk->dt;
-            {
-                R Fa[2 * NM + 5], Fb[2 * NM + 5];
-                load_faces<NM, R>(Fx, i, jj, kk, Fa);
-                load_faces<NM, R>(Fx, i + 1, jj, kk, Fb);
-                accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[0], Fa, Fb);
+        if (r >= f0 && r <= ftop) {  // face r: left UR(r-1) (the image of UR(0) at the domain face), right UL(r)
+            FSt<NM, R> A;
+            if (r == 0) {
+                A = SRs;
+                if (lv.bc_lo[D]) flip_state<NM, R>(A, D);
+            } else {
+                fst_get<NM, R>(shR, lane, A);
             }
-            if (D == 2) {
-                R Fa[2 * NM + 5], Fb[2 * NM + 5];
-                load_faces<NM, R>(Fy, i, jj, kk, Fa);
-                load_faces<NM, R>(Fy, i, jj + 1, kk, Fb);
-                accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[1], Fa, Fb);
-            }
-            accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[D], Flo, Fhi);
-            if (act) finish_cell<NM, STAGE>(lv, Ub, Uo, i, jj, kk, al, ar, ru, E, ops, cand, st);
-        };
-        auto emit = [&](int face, const FSt<NM, R>& A, const FSt<NM, R>& B, R* Fv) {
-            R sa[NM];
-            hllc<NM, R>(A, B, D, Fv, sa);
-            if (!act) return;
-            const long long fi = faceidx(face);
-            if (!FUSE) {
-                const long long fcs = F.cs;
+            R Fcur[2 * NM + 5];
+            emit(r, A, SLs, Fcur);
+            if (FUSE) {
+                if (r - 1 >= f0) {
+                    R Flo[2 * NM + 5];
 #pragma unroll
-                for (int q = 0; q < 2 * NM + 5; ++q) F.p[q * fcs + fi] = Fv[q];
-            }
-            if (TH) {
-                if (face < nd) {
-#pragma unroll
-                    for (int m = 0; m < NM; ++m) h.star_out[m * h.pstride + cellidx(face)] = (double)sa[m];
+                    for (int q = 0; q < 2 * NM + 5; ++q) Flo[q] = shF[q * 64 + lane];
+                    finish_row(r - 1, Flo, Fcur);
                 }
-                if (face == nd || (face > 0 && (lv.edge[D][face - 1] & 2))) {
 #pragma unroll
-                    for (int m = 0; m < NM; ++m) h.edge_out[m * h.estride + fi] = (double)sa[m];
-                }
+                for (int q = 0; q < 2 * NM + 5; ++q) shF[q * 64 + lane] = Fcur[q];
             }
-        };
-        R Fcur[2 * NM + 5];
-        bool have = false;
-        if (r == 0 && f0 == 0) {
-            FSt<NM, R> img = SRs;
-            if (lv.bc_lo[D]) flip_state<NM, R>(img, D);
-            emit(0, img, SLs, Fcur);
-            have = true;
-        } else if (r >= f0 && r <= ftop) {
-            emit(r, prevR, SLs, Fcur);
-            have = true;
         }
-        if (FUSE && have && r - 1 >= f0) finish_row(r - 1, Fprev, Fcur);
-        if (FUSE && have) {
+        fst_put<NM, R>(shR, lane, SRs);
+    }
+    if (ftop == nd) {  // domain face nd: UR(nd-1) and the image of UL(nd-1), from the last row
+        if (lv.bc_hi[D]) flip_state<NM, R>(SLs, D);
+        R Ftop[2 * NM + 5];
+        emit(nd, SRs, SLs, Ftop);
+        if (FUSE) {
+            R Flo[2 * NM + 5];
 #pragma unroll
-            for (int q = 0; q < 2 * NM + 5; ++q) Fprev[q] = Fcur[q];
+            for (int q = 0; q < 2 * NM + 5; ++q) Flo[q] = shF[q * 64 + lane];
+            finish_row(nd - 1, Flo, Ftop);
         }
-        if (r == nd - 1 && ftop == nd) {
-            FSt<NM, R> img = SLs;
-            if (lv.bc_hi[D]) flip_state<NM, R>(img, D);
-            R Ftop[2 * NM + 5];
-            emit(nd, SRs, img, Ftop);
-            if (FUSE) finish_row(nd - 1, Fprev, Ftop);
-        }
-        prevR = SRs;
-        Pm = Pc;
-        Pc = Pp;
     }
     if (FUSE && STAGE == 2 && (ops & 8u)) wave_min_to(cand, &clk->dtmin_bits);
     if (st) atomicOr(&clk->status, st);
 }
 template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1, typename R = double>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TJ_MARCH_WAVES_PER_EU, 8))) void k_flux_march(
-    Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk, MF Ub = MF{}, MF Uo = MF{}, FacesT<R> Fx = FacesT<R>{},
-    FacesT<R> Fy = FacesT<R>{}, unsigned ops = 0u) {
+__global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk,
+                                                    MF Ub = MF{}, MF Uo = MF{}, FacesT<R> Fx = FacesT<R>{},
+                                                    FacesT<R> Fy = FacesT<R>{}, unsigned ops = 0u) {
     flux_march_body<NM, TH, D, FUSE, STAGE, R>(lv, U, F, h, seglen, clk, Ub, Uo, Fx, Fy, ops,
                                                (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
+// The same pass held to two waves per SIMD (the FP64 THINC pass: its few
+// registers beyond 256 spill to scratch on rarely taken paths, and the second
+// wave hides the FP64 latency; measured faster than one wave per SIMD).
+template <int NM, bool TH, int D, typename R = double>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_flux_march_w2(
+    Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk) {
+    flux_march_body<NM, TH, D, false, 1, R>(lv, U, F, h, seglen, clk, MF{}, MF{}, FacesT<R>{}, FacesT<R>{}, 0u,
+                                            (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
 // ------------------------------------------------------------------ update

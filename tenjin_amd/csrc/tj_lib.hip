@@ -546,10 +546,11 @@ static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, con
             const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
             const long long no = d == 1 ? lv.n[2] : lv.n[1];
             const unsigned g = grid1(nstrip * nseg * no, 4);
-            if (d == 1)
+            if (d == 1) {
                 hipLaunchKernelGGL((k_flux_lmarch_s<2, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], c->seglen, c->d_clk);
-            else
+            } else {
                 hipLaunchKernelGGL((k_flux_lmarch_s<2, 2, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[2], c->seglen, c->d_clk);
+            }
         }
     } else if (d == 0) {
         const int per = th ? 61 : 63;
@@ -565,7 +566,8 @@ static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, con
         const MF z{};
         const FacesT<R> zf{};
         if (d == 1) {
-            if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
+            if (th && sizeof(R) == 8) hipLaunchKernelGGL((k_flux_march_w2<2, true, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk);
+            else if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
             else hipLaunchKernelGGL((k_flux_march<2, false, 1, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
         } else {
             hipLaunchKernelGGL((k_flux_march<2, false, 2, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[2], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);

@@ -131,8 +131,8 @@ __device__ __forceinline__ void sst_load(const R* sh, int lane, SSt<NM, R>& S) {
 // the wave's own LDS slice (not in registers), and the three primitive rows
 // of each row are reloaded from cache: one reconstruction per cell as in a
 // register march, with the register footprint of the x pass.
-template <int NM, int D, typename R = double>
-__global__ __launch_bounds__(256) void k_flux_lmarch_s(Level lv, MF U, FacesT<R> F, int seglen, Clock* clk) {
+template <int NM, int D, typename R>
+__device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U, const FacesT<R>& F, int seglen, Clock* clk) {
     if (clk->done) return;
     constexpr int NF = 3 * NM + 17;
     constexpr int NS = SStSlots<NM, R>::N;
@@ -202,6 +202,10 @@ __global__ __launch_bounds__(256) void k_flux_lmarch_s(Level lv, MF U, FacesT<R>
         emit(nd, SRs, SLs);
     }
     if (st) atomicOr(&clk->status, st);
+}
+template <int NM, int D, typename R = double>
+__global__ __launch_bounds__(256) void k_flux_lmarch_s(Level lv, MF U, FacesT<R> F, int seglen, Clock* clk) {
+    flux_lmarch_s_body<NM, D, R>(lv, U, F, seglen, clk);
 }
 
 // Per-cell solid state held by the update kernel.
