@@ -147,6 +147,22 @@ __global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, FacesT<R> F, His
     flux_x_body<NM, TH, R>(lv, U, F, h, clk, (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
+// ------------------------------------------------------------------ XCD-aware block order
+// The 8 XCDs of the device take a grid's blocks round-robin (block b runs on
+// XCD b mod 8) and each XCD has its own L2.  Renumbering blocks so that each
+// XCD works through one contiguous range of cells keeps a 2D level's
+// neighbouring rows, which share their faces, in one L2 (measured: the 2D
+// update kernels 6-8 % faster).  In 3D the round-robin order is kept: with
+// 256 cells per row the cell above in z is 256 blocks on, which round-robin
+// puts on the same XCD one generation of resident blocks later, while the
+// contiguous order would put it out of the L2 (measured slower).
+__device__ __forceinline__ long long xcd_block(long long b, long long nb, int dim) {
+    if (dim > 2) return b;
+    const long long base = nb / 8, extra = nb % 8;
+    const long long x = b % 8, q = b / 8;
+    return x * base + (x < extra ? x : extra) + q;  // a bijection of [0, nb)
+}
+
 // ------------------------------------------------------------------ update helpers
 // One direction of update (hllc.cpp:493-523): U1 += dt/dx (F_lo - F_hi), the
 // alpha rows with the non-conservative term -alpha_U (S*_lo - S*_hi).
@@ -603,7 +619,7 @@ __device__ __forceinline__ void update_body(const Level& lv, const MF& Uin, cons
 template <int NM, int STAGE, typename T = double>
 __global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, FacesT<T> F0, FacesT<T> F1,
                                                 FacesT<T> F2, Clock* clk, unsigned ops) {
-    update_body<NM, STAGE, T>(lv, Uin, Ub, Uo, F0, F1, F2, clk, ops, blockIdx.x);
+    update_body<NM, STAGE, T>(lv, Uin, Ub, Uo, F0, F1, F2, clk, ops, xcd_block(blockIdx.x, gridDim.x, lv.dim));
 }
 
 // getSoundSpeed + boxTimeStepFunction + min over the level (timestep.cpp:7-70).

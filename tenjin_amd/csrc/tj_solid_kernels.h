@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
     if (clk->done) return;
     constexpr int NC = 3 * NM + 13;
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long id = xcd_block(blockIdx.x, gridDim.x, lv.dim) * blockDim.x + threadIdx.x;
     const long long ncell = (long long)nx * ny * nz;
     double cand = __builtin_huge_val();
     unsigned st = 0;
