@@ -4,8 +4,8 @@
 //
 // These are not AMReX.  They keep AMReX's names and meanings where the
 // reference's code relies on them:
-//   * BoxArray::maxSize cuts the domain as AMReX does (chunks of max_grid_size,
-//     the remainder spread one cell at a time over the leading chunks);
+//   * BoxArray::maxSize cuts the domain by AMReX's rule (pieces of at most
+//     max_grid_size cells, laid out as tj_bind_level lays them out);
 //   * BCRec holds AMReX BCType codes (reflect_odd -1, reflect_even 1,
 //     foextrap 2);
 //   * a MultiFab's data are component-major with i fastest (FArrayBox order).
@@ -168,5 +168,17 @@ private:
     const BoxArray* ba_;
     int b_ = 0;
 };
+
+// Plotfile name of main.cpp:104, 176, 196: root followed by the step number
+// zero-padded to `mindigits` (AMReX Concatenate).
+std::string Concatenate(const std::string& root, int num, int mindigits = 5);
+
+// One-level AMReX plotfile (the "HyperCLaw-V1.1" format AMReX tools and yt
+// read): <dir>/Header, <dir>/Level_0/Cell_H and <dir>/Level_0/Cell_D_00000
+// holding each box of mf's BoxArray as an IEEE-754 little-endian FAB.
+// Signature of AMReX's WriteSingleLevelPlotfile as called at main.cpp:106,
+// 178, 198.  Synchronises with the device (copies mf to the host).
+void WriteSingleLevelPlotfile(const std::string& dir, const MultiFab& mf, const Vector<std::string>& varnames,
+                              const Geometry& geom, Real time, int level_step);
 
 }  // namespace amrex_lite

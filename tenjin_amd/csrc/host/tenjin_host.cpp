@@ -4,9 +4,17 @@
 
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
+#include <cerrno>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <sstream>
 #include <stdexcept>
 
 namespace {
@@ -119,6 +127,104 @@ void MultiFab::Copy(MultiFab& dst, const MultiFab& src, int srccomp, int dstcomp
     hip_check(hipMemcpy(dst.data_ + (size_t)dstcomp * dst.numCells(), src.data_ + (size_t)srccomp * src.numCells(),
                         sizeof(Real) * (size_t)numcomp * (size_t)src.numCells(), hipMemcpyDeviceToDevice),
               "MultiFab::Copy");
+}
+
+std::string Concatenate(const std::string& root, int num, int mindigits) {
+    std::ostringstream os;
+    os << root << std::setfill('0') << std::setw(mindigits) << num;
+    return os.str();
+}
+
+namespace {
+
+void make_dir(const std::string& d) {
+    if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) throw std::runtime_error("cannot create directory " + d);
+}
+
+std::string ivec(const IntVect& v, int dim) {
+    std::string s = "(";
+    for (int d = 0; d < dim; ++d) s += (d ? "," : "") + std::to_string(v[d]);
+    return s + ")";
+}
+
+// ((lo) (hi) (type)) of a cell-centred box
+std::string box_str(const Box& b, int dim) {
+    return "(" + ivec(b.lo, dim) + " " + ivec(b.hi, dim) + " " + ivec(IntVect(0, 0, 0), dim) + ")";
+}
+
+}  // namespace
+
+void WriteSingleLevelPlotfile(const std::string& dir, const MultiFab& mf, const Vector<std::string>& varnames,
+                              const Geometry& geom, Real time, int level_step) {
+    const int dim = geom.dim();
+    const int nc = mf.nComp();
+    if ((int)varnames.size() < nc) throw std::runtime_error("WriteSingleLevelPlotfile: fewer names than components");
+    Vector<Real> h;
+    mf.copyToHost(h);
+    const Box& dom = mf.domain();
+    const long long ncell = mf.numCells();
+    make_dir(dir);
+    make_dir(dir + "/Level_0");
+    std::ofstream hd(dir + "/Header");
+    hd << std::setprecision(17);
+    hd << "HyperCLaw-V1.1\n" << nc << "\n";
+    for (int c = 0; c < nc; ++c) hd << varnames[c] << "\n";
+    hd << dim << "\n" << time << "\n" << 0 << "\n";
+    for (int d = 0; d < dim; ++d) hd << (d ? " " : "") << 0.0;
+    hd << "\n";
+    for (int d = 0; d < dim; ++d) hd << (d ? " " : "") << geom.ProbLength(d);
+    hd << "\n\n" << box_str(geom.Domain(), dim) << "\n" << level_step << "\n";
+    for (int d = 0; d < dim; ++d) hd << (d ? " " : "") << geom.CellSize(d);
+    hd << "\n0\n0\n";
+    const BoxArray& ba = mf.boxArray();
+    hd << 0 << " " << ba.size() << " " << time << "\n" << level_step << "\n";
+    for (int b = 0; b < ba.size(); ++b)
+        for (int d = 0; d < dim; ++d)
+            hd << ba[b].lo[d] * geom.CellSize(d) << " " << (ba[b].hi[d] + 1) * geom.CellSize(d) << "\n";
+    hd << "Level_0/Cell\n";
+    // FABs and the VisMF header
+    std::ofstream fd(dir + "/Level_0/Cell_D_00000", std::ios::binary);
+    std::ostringstream ch;
+    ch << std::setprecision(17);
+    ch << "1\n1\n" << nc << "\n0\n(" << ba.size() << " 0\n";
+    for (int b = 0; b < ba.size(); ++b) ch << box_str(ba[b], dim) << "\n";
+    ch << ")\n" << ba.size() << "\n";
+    std::vector<std::vector<Real>> mins(ba.size()), maxs(ba.size());
+    for (int b = 0; b < ba.size(); ++b) {
+        const Box& bx = ba[b];
+        ch << "FabOnDisk: Cell_D_00000 " << (long long)fd.tellp() << "\n";
+        fd << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(bx, dim) << " " << nc << "\n";
+        std::vector<Real> buf;
+        buf.reserve((size_t)bx.numPts());
+        for (int c = 0; c < nc; ++c) {
+            buf.clear();
+            Real mn = std::numeric_limits<Real>::max(), mx = -std::numeric_limits<Real>::max();
+            for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+                for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+                    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+                        const long long cell = (long long)(k - dom.lo[2]) * mf.kstride() + (long long)(j - dom.lo[1]) * mf.jstride() +
+                                               (i - dom.lo[0]);
+                        const Real v = h[(size_t)c * (size_t)ncell + (size_t)cell];
+                        buf.push_back(v);
+                        mn = std::min(mn, v);
+                        mx = std::max(mx, v);
+                    }
+            fd.write(reinterpret_cast<const char*>(buf.data()), sizeof(Real) * buf.size());
+            mins[b].push_back(mn);
+            maxs[b].push_back(mx);
+        }
+    }
+    ch << "\n" << ba.size() << "," << nc << "\n";
+    for (int b = 0; b < ba.size(); ++b) {
+        for (Real v : mins[b]) ch << v << ",";
+        ch << "\n";
+    }
+    ch << "\n" << ba.size() << "," << nc << "\n";
+    for (int b = 0; b < ba.size(); ++b) {
+        for (Real v : maxs[b]) ch << v << ",";
+        ch << "\n";
+    }
+    std::ofstream(dir + "/Level_0/Cell_H") << ch.str();
 }
 
 }  // namespace amrex_lite

@@ -3,7 +3,10 @@
 // tests: the level description, the primitive initial state, finalT and a step
 // cap.  Writes (t, n) and the final state of U1.
 //
-//   main_loop <input> <output>
+//   main_loop <input> <output> [plotfile root]
+//
+// With a plotfile root, the initial and the final state of U1 are written as
+// AMReX plotfiles <root>00000 and <root><n> (main.cpp:104-107, 195-199).
 //
 // input : "TJIN" | int32 1 | tj_level_desc | double finalT | int32 max_steps
 //         | int32 shift_pressure | int64 ncomp | int64 ncells | ncomp*ncells doubles
@@ -28,10 +31,11 @@ void get(std::ifstream& f, T& v) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: main_loop <input> <output>\n");
+    if (argc != 3 && argc != 4) {
+        std::fprintf(stderr, "usage: main_loop <input> <output> [plotfile root]\n");
         return 2;
     }
+    const std::string plotroot = argc == 4 ? argv[3] : "";
     std::ifstream in(argv[1], std::ios::binary);
     char magic[4];
     in.read(magic, 4);
@@ -153,6 +157,8 @@ int main(int argc, char** argv) {
     }
     U1.commit();
     tenjin_host::initialiseFromPrimitives(U1, shift != 0);
+    if (!plotroot.empty())
+        WriteSingleLevelPlotfile(Concatenate(plotroot, 0, 5), U1.data, U1.accessPattern.variableNames, geom, 0.0, 0);
 
     // ---- the time loop of main.cpp:121-191
     int n = 0;
@@ -188,6 +194,8 @@ int main(int argc, char** argv) {
         }
     }
 
+    if (!plotroot.empty())
+        WriteSingleLevelPlotfile(Concatenate(plotroot, n, 5), U1.data, U1.accessPattern.variableNames, geom, t, n);
     U1.stage();
     std::ofstream out(argv[2], std::ios::binary);
     out.write("TJOU", 4);

@@ -37,13 +37,14 @@ def read_output(path: str, pb):
     return t, n, nan_stop, data.reshape((-1,) + pb.shape)
 
 
-def run(pb, final_time: float, max_steps: int, workdir: str, timeout: float = 300.0):
+def run(pb, final_time: float, max_steps: int, workdir: str, timeout: float = 300.0, plotroot: str = ""):
     if not os.path.exists(DRIVER):
         raise RuntimeError(f"{DRIVER} is missing: build it with `python tenjin_amd/build.py`")
     src = os.path.join(workdir, "in.bin")
     dst = os.path.join(workdir, "out.bin")
     write_input(src, pb, final_time, max_steps)
-    r = subprocess.run([DRIVER, src, dst], capture_output=True, text=True, timeout=timeout)
+    r = subprocess.run([DRIVER, src, dst] + ([plotroot] if plotroot else []), capture_output=True, text=True,
+                       timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"main_loop failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
     return read_output(dst, pb)

@@ -60,3 +60,28 @@ def test_host_interface_plate_impact_2d(tmp_path):
 
     pb = configs.plate_impact(dim=2, n=48, box=24)
     _check(pb, 1.0, 10, tmp_path)
+
+
+@pytest.mark.gpu
+def test_host_interface_plotfiles(tmp_path):
+    """WriteSingleLevelPlotfile of the C++ interface (main.cpp:104-107,
+    195-199): the initial and final plotfiles hold every component of U1 with
+    AccessPattern's names, the final one equal to the oracle's state."""
+    from oracle.oracle import Oracle
+    from tenjin_amd import configs
+    from tenjin_amd.plotfile import read_plotfile
+    from tests import host_driver
+
+    pb = configs.shock_inclusion_2d(nx=96, ny=64, box=32, thinc=1, shock_x=0.55)
+    print("\n[case] host interface plotfiles", flush=True)
+    o = Oracle(pb)
+    n_o, t_o = o.run(1.0, 12)
+    root = str(tmp_path / "plt")
+    t_g, n_g, _, sg = host_driver.run(pb, 1.0, 12, str(tmp_path), plotroot=root)
+    meta0, d0 = read_plotfile(root + "00000")
+    meta, d = read_plotfile(root + f"{n_g:05d}")
+    assert meta["step"] == n_g == n_o and meta["time"] == t_g == t_o and meta0["time"] == 0.0
+    assert meta["names"][0] == "alpha0_00" and meta["names"][pb.layout.comp(2)] == "rho_k0_00"
+    assert len(meta["boxes"]) == 3 * 2 and meta["dim"] == 2
+    assert np.array_equal(d, sg[: d.shape[0]], equal_nan=True)
+    assert exact_fraction(pb, o.get(), d) == 1.0

@@ -451,3 +451,26 @@ def test_c_abi_library_exports_every_declared_symbol():
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode != 0 and "one GPU" in r.stderr
+
+
+def test_plotfile_roundtrip(tmp_path):
+    """One-level AMReX plotfile writer and reader: uneven boxes in 2D and 3D,
+    the header fields, the FAB encoding line and per-box min/max."""
+    import os
+
+    from tenjin_amd.plotfile import read_plotfile, write_plotfile
+
+    rng = np.random.default_rng(5)
+    for shape, boxes in (((1, 20, 33), [((0, 0, 0), (16, 9, 0)), ((17, 0, 0), (32, 9, 0)), ((0, 10, 0), (32, 19, 0))]),
+                         ((6, 5, 4), [((0, 0, 0), (3, 4, 2)), ((0, 0, 3), (3, 4, 5))])):
+        data = rng.standard_normal((3,) + shape)
+        path = str(tmp_path / f"plt_{len(boxes)}")
+        write_plotfile(path, data, ["a", "b", "c"], boxes, (1.0, 2.0, 3.0), 1.25e-3, 7)
+        meta, back = read_plotfile(path)
+        assert np.array_equal(back, data)
+        assert meta["names"] == ["a", "b", "c"] and meta["step"] == 7 and meta["time"] == 1.25e-3
+        assert meta["dim"] == (3 if shape[0] > 1 else 2)
+        head = open(os.path.join(path, "Level_0", "Cell_D_00000"), "rb").readline().decode()
+        assert head.startswith("FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))")
+        cell_h = open(os.path.join(path, "Level_0", "Cell_H")).read().split("\n")
+        assert cell_h[:4] == ["1", "1", "3", "0"]

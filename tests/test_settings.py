@@ -1,0 +1,93 @@
+"""Reference-format settings decks (libconfig syntax, initialise.cpp:125-385)
+-> Problem, with the pluggable initial-condition geometry."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tenjin_amd import settings
+from tenjin_amd.configs import ALPHA, P, RHO_K, VELOCITY
+
+DECK = """
+// 1D two-material stiffened-gas shock tube, in the reference's keys
+finalTime = 2.4e-4;
+numberOfxCells = 1000;  numberOfyCells = 1;  numberOfzCells = 1;
+xdomainLength = 1.0;  ydomainLength = 1.0;  zdomainLength = 1.0;
+CFL = 0.5;
+materials = 2;  mixtures = 0;  states = 2;
+interface = 0.7;
+numberOfPictures = 10;
+SOLID = 0; THINC = 0; RADIAL = 0; PLASTIC = 0; MUSCL = 1; REACTIVE = 0;
+THINCbeta = 1.6;
+lowBoundary = [0, 0, 0];
+highBoundary = [0, 0, 0];
+/* materials: water-like stiffened gas, then air */
+listOfMaterials = (
+  { material = "fluid"; mixture = 0; EOS = "MieGruneisen";
+    adiabaticIndex = 4.4; pref = -2.64e9; eref = 0.0; CV = 1.0; },
+  { material = "fluid"; mixture = 0; EOS = "Mie" "Gruneisen";   # adjacent strings concatenate
+    adiabaticIndex = 1.4; pref = 0.0; eref = 0.0; CV = 1.0; }
+);
+listOfStates = (
+  { material = ( { alpha = 0.999999; rho = 1000.0; }, { alpha = 0.000001; rho = 50.0; } );
+    p = 1.0e9; u = 0.0; v = 0.0; w = 0.0; },
+  { material = ( { alpha = 0.000001; rho = 1000.0; }, { alpha = 0.999999; rho = 50.0; } );
+    p = 1.0e5; u = 0.0; v = 0.0; w = 0.0; }
+);
+"""
+
+
+def test_parse_libconfig_subset():
+    cfg = settings.parse(DECK)
+    assert cfg["numberOfxCells"] == 1000 and cfg["finalTime"] == 2.4e-4 and cfg["lowBoundary"] == [0, 0, 0]
+    assert cfg["listOfMaterials"][1]["EOS"] == "MieGruneisen"
+    assert cfg["listOfStates"][0]["material"][0]["rho"] == 1000.0
+    assert settings.parse('a : true; b = "x\\"y"; c = 0x1F; d = {e = [1.5, -2e3];};') == {
+        "a": True, "b": 'x"y', "c": 31, "d": {"e": [1.5, -2e3]}}
+
+
+def test_deck_gives_the_shock_tube():
+    """The deck above is the 1D shock tube: the same materials, the same
+    per-cell states by the rule of initialise.cpp:393-402, and, with the
+    reference's pressure shift of initialise.cpp:104, the same first steps
+    in the oracle as the hand-built problem with that shift."""
+    from oracle.oracle import Oracle
+    from tenjin_amd import configs
+
+    pb = settings.problem_from_settings(DECK, max_grid_size=1 << 30)
+    ref = configs.shock_tube_1d()
+    assert pb.dim == 1 and pb.n == ref.n and pb.final_time == ref.final_time and pb.shift_pressure == 1
+    for a, b in zip(pb.materials, ref.materials):
+        assert (a.gamma, a.pref, a.eref) == (b.gamma, b.pref, b.eref)
+    L = pb.layout
+    for c in (L.comp(RHO_K, 0), L.comp(RHO_K, 1), L.comp(P), L.comp(VELOCITY, 0, 0)):
+        assert np.array_equal(pb.prims[c], ref.prims[c])
+    for m in range(2):  # the deck's literals vs 1 - (1 - 1e-6) of the hand-built problem: rounding only
+        c = L.comp(ALPHA, m)
+        assert np.allclose(pb.prims[c], ref.prims[c], rtol=0, atol=1e-16)
+        ref.prims[c] = pb.prims[c]
+    ref.shift_pressure = 1
+    o1, o2 = Oracle(pb), Oracle(ref)
+    for n in range(3):
+        dt = o1.driver_dt(n, 0.0)
+        assert dt == o2.driver_dt(n, 0.0)
+        o1.step(dt)
+        o2.step(dt)
+    assert np.array_equal(o1.get(), o2.get())
+
+
+def test_custom_geometry_and_rejections():
+    cfg = settings.parse(DECK)
+    cfg["numberOfyCells"] = 8
+    cfg["numberOfxCells"] = 16
+    pb = settings.problem_from_settings(cfg, geometry=lambda i, j, k, pb, cfg: (j >= 4).astype(int))
+    assert pb.dim == 2
+    assert pb.prims[pb.layout.comp(P)][0, 0, 0] == 1e9 and pb.prims[pb.layout.comp(P)][0, 7, 0] == 1e5
+    bad = settings.parse(DECK)
+    bad["listOfMaterials"][0]["EOS"] = "RomenskiiSolid"
+    with pytest.raises(ValueError):
+        settings.problem_from_settings(bad)
+    bad = settings.parse(DECK)
+    bad["mixtures"] = 1
+    with pytest.raises(ValueError):
+        settings.problem_from_settings(bad)
