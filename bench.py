@@ -6,7 +6,8 @@ interaction on a single level of 2048 x 1024 cells in 64 x 64 boxes (MUSCL +
 THINC/BVD, beta 1.6, HLLC, CFL 0.5), synthetic initial data generated here.
 `--workload plate3d` runs the 3D elastoplastic copper/aluminium plate impact
 on 256^3 cells in 64^3 boxes (Us-up Mie-Gruneisen, 5-wave flux, perfect
-plasticity).  One "step" is one full time step of the reference's loop body
+plasticity); `--workload explosion_rz` an axisymmetric (r, z) underwater
+explosion on 1448^2 cells with the cylindrical geometric source.  One "step" is one full time step of the reference's loop body
 (main.cpp:124-171): CFL dt with the driver clamps, two RK stages, the RK
 average, plasticUpdate and cleanUpV (solids), cleanUpAlpha, the NaN scan and
 the next step's CFL reduction.  The state is resident in HBM before timing
@@ -67,13 +68,14 @@ SOLID_KERNEL_BYTES = {
     "update_stage2_post_mixed": 152 + 3 * 92 + 232 + 232 + 8,
 }
 # 8 (3 N_s + 2 N_c): N_s/N_c = 14/8 (two-material fluid), 27/19 (two-material solid)
-STEP_BYTES = {"inclusion2d": 464, "plate3d": 952, "tube1d": 464, "ensemble": 464}
+STEP_BYTES = {"inclusion2d": 464, "plate3d": 952, "tube1d": 464, "ensemble": 464, "explosion_rz": 464}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["inclusion2d", "plate3d", "tube1d", "ensemble"], default="inclusion2d")
+    ap.add_argument("--workload", choices=["inclusion2d", "plate3d", "tube1d", "ensemble", "explosion_rz"],
+                    default="inclusion2d")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 500 for inclusion2d, 40 for plate3d)")
     ap.add_argument("--warmup", type=int, default=10)
@@ -82,8 +84,9 @@ def parse():
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--n3", type=int, default=256, help="plate3d: cells per side")
+    ap.add_argument("--nrz", type=int, default=1448, help="explosion_rz: cells per side")
     ap.add_argument("--box", type=int, default=64)
-    ap.add_argument("--seglen", type=int, default=16)
+    ap.add_argument("--seglen", type=int, default=0, help="rows per marching segment (0: chosen per pass by the library)")
     ap.add_argument("--fuse", action="store_true",
                     help="run the update inside the y pass instead of as a kernel of its own")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -92,7 +95,8 @@ def parse():
                     help="run a plain 1 GiB device copy first (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"inclusion2d": 500, "plate3d": 40, "tube1d": 1280, "ensemble": 200}[args.workload]
+        args.steps = {"inclusion2d": 500, "plate3d": 40, "tube1d": 1280, "ensemble": 200,
+                      "explosion_rz": 500}[args.workload]
     return args
 
 
@@ -125,7 +129,7 @@ def cpu_baseline(pb, budget_s: float, max_steps: int = 20):
 
 
 TRAFFIC_FILES = {"inclusion2d": "pmc_traffic_shock_inclusion_2d.txt", "plate3d": "pmc_traffic_plate_impact_3d.txt",
-                 "tube1d": "pmc_traffic_shock_tube_1d.txt"}
+                 "tube1d": "pmc_traffic_shock_tube_1d.txt", "explosion_rz": "pmc_traffic_explosion_rz.txt"}
 
 
 def read_traffic(workload: str, name: str):
@@ -293,6 +297,11 @@ def main():
         kbytes = FLUID_KERNEL_BYTES
         workload = (f"2D two-material shock/inclusion, {pb.n[0]}x{pb.n[1]} cells, {pb.max_grid_size}^2 boxes, "
                     "MUSCL+THINC(beta 1.6)+HLLC, CFL 0.5")
+    elif args.workload == "explosion_rz":
+        pb = configs.underwater_explosion(2, args.nrz, box=args.box, thinc=1)
+        kbytes = FLUID_KERNEL_BYTES
+        workload = (f"2D axisymmetric (r, z) underwater explosion, {pb.n[0]}x{pb.n[1]} cells, "
+                    f"{pb.max_grid_size}^2 boxes, MUSCL+THINC(beta 1.6)+HLLC, cylindrical geometric source, CFL 0.5")
     else:
         pb = configs.plate_impact(dim=3, n=args.n3, box=args.box)
         kbytes = SOLID_KERNEL_BYTES
@@ -436,7 +445,7 @@ def main():
     if mixed is not None:
         out["mixed"] = mixed
     if not args.no_cpu:
-        if args.workload == "inclusion2d":
+        if args.workload in ("inclusion2d", "explosion_rz"):
             out["cpu_baseline"] = cpu_baseline(pb, args.cpu_seconds, 20)
         else:
             # bounded sample: the same problem on 128^3 cells (one eighth of the cells)

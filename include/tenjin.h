@@ -54,6 +54,11 @@ typedef struct tj_level_desc {
     int nmat;
     int bc_lo[3], bc_hi[3];  /* 0 transmissive (foextrap), 1 reflective */
     tj_material mat[TJ_MAX_MATERIALS];
+    int RADIAL;              /* parameters.RADIAL: geometricSourceTerm each step (main.cpp:159-162);
+                                x is the radius, spherical in 1D, cylindrical otherwise.  On solid
+                                levels it needs PLASTIC (the source reads the stress of the state
+                                that plasticUpdate's conservativeToPrimitive leaves). */
+    int pad_;
 } tj_level_desc;
 
 /* Non-owning view of a CellArray on the device: component-major, i fastest.
@@ -94,7 +99,9 @@ enum {
     TJ_OP_CLEAN_V = 2,      /* CellArray::cleanUpV (boxaccesscellarray.cpp:476-528) */
     TJ_OP_CLEAN_ALPHA = 4,  /* CellArray::cleanUpAlpha (boxaccesscellarray.cpp:530-599) */
     TJ_OP_DT = 8,           /* TimeStep::getTimeStep (timestep.cpp:36-70) */
-    TJ_OP_NANCHECK = 16     /* CellArray::contains_nan (cellarray.cpp:101-117) */
+    TJ_OP_NANCHECK = 16,    /* CellArray::contains_nan (cellarray.cpp:101-117) */
+    TJ_OP_RADIAL = 32       /* geometricSourceTerm (radial.cpp:37-54) with the dt of tj_set_dt;
+                               applied after PLASTIC and before CLEAN_V */
 };
 
 /* Context on HIP device `device` with its own stream (default priority). */
@@ -109,12 +116,14 @@ void* tj_stream(tj_ctx* ctx);
  * dx = L/n.  Allocates the scratch (flux planes, THINC history planes).
  * Replaces the set-up of main.cpp:29-95. */
 int tj_bind_level(tj_ctx* ctx, const tj_level_desc* desc);
-/* Tuning: rows per marching segment of the y/z flux passes (default 16),
- * and whether the update of a fluid level with dim >= 2 runs inside its last
- * directional pass (fuse = 1) or as a kernel of its own (fuse = 0, default:
- * faster on gfx950 while the fused pass is limited to one wave per SIMD by
- * its register use).  Both give identical results; a value < 0 leaves that
- * setting unchanged. */
+/* Tuning: rows per marching segment of the y/z flux passes (seglen > 0
+ * fixes it; 0, the default, picks it per pass from the level's shape and the
+ * kernel's occupancy so that the segments fill the device's wave slots
+ * evenly), and whether the update of a fluid level with dim >= 2 runs inside
+ * its last directional pass (fuse = 1) or as a kernel of its own (fuse = 0,
+ * default: faster on gfx950 while the fused pass is limited to one wave per
+ * SIMD by its register use).  Every choice gives identical results; a value
+ * < 0 leaves that setting unchanged. */
 int tj_set_tuning(tj_ctx* ctx, int seglen, int fuse);
 /* Number of components of the bound layout and the first component of a
  * reference Variable (structdefinitions.h:10-32 numbering); -1 if absent. */

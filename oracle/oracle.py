@@ -43,6 +43,7 @@ class Params(ctypes.Structure):
         ("s", ctypes.c_double * 4),
         ("G0", ctypes.c_double * 4),
         ("yield_", ctypes.c_double * 4),
+        ("RADIAL", ctypes.c_int),
     ]
 
 
@@ -77,6 +78,7 @@ def lib():
         L.tjo_timestep.argtypes = [vp]
         L.tjo_timestep.restype = ctypes.c_double
         L.tjo_step.argtypes = [vp, ctypes.c_double]
+        L.tjo_geometric_source.argtypes = [vp, ctypes.c_double]
         L.tjo_run.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp]
         L.tjo_run.restype = ctypes.c_int
         for f in ("tjo_cleanup_v", "tjo_plastic", "tjo_c2p", "tjo_p2c", "tjo_sound"):
@@ -112,6 +114,7 @@ def make_params(pb, math_mode: int = 0) -> Params:
     q.max_grid_size = pb.max_grid_size
     q.nghost = pb.nghost
     q.SOLID, q.THINC, q.PLASTIC, q.MUSCL = pb.SOLID, pb.THINC, pb.PLASTIC, pb.MUSCL
+    q.RADIAL = pb.RADIAL
     q.THINCbeta = pb.THINCbeta
     q.CFL = pb.CFL
     q.nmat = pb.nmat
@@ -194,6 +197,10 @@ class Oracle:
 
     def plastic(self):
         lib().tjo_plastic(self.h)
+
+    def geometric_source(self, dt: float):
+        """geometricSourceTerm (radial.cpp:37-54) on U1 with step dt."""
+        lib().tjo_geometric_source(self.h, dt)
 
     def status(self) -> int:
         return lib().tjo_status(self.h)

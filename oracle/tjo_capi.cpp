@@ -24,6 +24,7 @@ struct tjo_params {
     int solid[4];
     int plastic[4];
     double gamma[4], pref[4], eref[4], CV[4], rho0[4], c0[4], s[4], G0[4], yield[4];
+    int RADIAL;
 };
 
 static tjo::Params to_params(const tjo_params* q) {
@@ -40,6 +41,7 @@ static tjo::Params to_params(const tjo_params* q) {
     p.SOLID = q->SOLID;
     p.THINC = q->THINC;
     p.PLASTIC = q->PLASTIC;
+    p.RADIAL = q->RADIAL;
     p.MUSCL = q->MUSCL;
     p.THINCbeta = q->THINCbeta;
     p.CFL = q->CFL;
@@ -161,6 +163,7 @@ int tjo_cleanup_alpha(void* h) {
 }
 void tjo_cleanup_v(void* h) { static_cast<tjo::Solver*>(h)->cleanup_v(); }
 void tjo_plastic(void* h) { static_cast<tjo::Solver*>(h)->plastic_update(); }
+void tjo_geometric_source(void* h, double dt) { static_cast<tjo::Solver*>(h)->geometric_source(dt); }
 void tjo_c2p(void* h) {
     auto* s = static_cast<tjo::Solver*>(h);
     for (size_t b = 0; b < s->boxes.size(); ++b)

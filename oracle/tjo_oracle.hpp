@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,7 @@ struct Params {
     int max_grid_size = 1 << 30;
     int nghost = 2;
     int SOLID = 0, THINC = 0, PLASTIC = 0, MUSCL = 1;
+    int RADIAL = 0;  // geometric source of radial.cpp (structdefinitions.h:113)
     double THINCbeta = 1.6;
     double CFL = 0.5;
     int nmat = 2;
@@ -1155,6 +1157,43 @@ class Solver {
         return p.CFL * dtm;
     }
 
+    // flux.cpp:51-77 geometricFlux: the radial flux of conservative variable
+    // n in cell c (the stress is the cell's stored sigma)
+    double gflux(const Spec& n, CellV& c) {
+        switch (n.var) {
+            case ALPHA: return 0.0;
+            case ALPHARHO: case ALPHARHOLAMBDA: case ALPHARHOEPSILON:
+                return c(n) * c(VELOCITY, 0, 0);
+            case RHOU:
+                if (n.row == 0) return c(n) * c(VELOCITY, 0, 0) - c(SIGMA, 0, 0, 0) + c(SIGMA, 0, 2, 2);
+                if (n.row == 1) return c(n) * c(VELOCITY, 0, 0) - c(SIGMA, 0, 0, 1);
+                return -2.0 * c(SIGMA, 0, 0, 2);
+            case TOTAL_E: return c(n) * c(VELOCITY, 0, 0) - vdots(c, 0);
+            case V_TENSOR:
+                if (n.row == 0 || n.row == 1) return (1.0 / 3.0) * c(n) * c(VELOCITY, 0, 0);
+                return -(2.0 / 3.0) * c(n) * c(VELOCITY, 0, 0);
+            default: throw std::runtime_error("oracle: bad radial flux variable");
+        }
+    }
+    // radial.cpp:5-54 geometricSourceTerm on U1: per conservative variable,
+    // U += dt * (multiplier / ((i + 1/2) dx)) * geometricFlux, multiplier -2
+    // in one dimension (spherical) and -1 otherwise (cylindrical); then
+    // conservativeToPrimitive box by box.
+    void geometric_source(double dtv) {
+        const double mult = (p.dim == 1 ? -2.0 : -1.0);
+#pragma omp parallel for schedule(dynamic)
+        for (size_t b = 0; b < boxes.size(); ++b) {
+            Fab& f = U1.f[b];
+            for (const Spec& n : L.cons)
+                for_valid((int)b, [&](int i, int j, int k) {
+                    CellV c{this, &f, i, j, k};
+                    const double inverse_r = mult / (((double)i + 0.5) * dx[0]);
+                    A(f, i, j, k, n) += dtv * inverse_r * gflux(n, c);
+                });
+            for_valid((int)b, [&](int i, int j, int k) { c2p_cell(f, i, j, k); });
+        }
+    }
+
     // main.cpp:124-171: one time step starting from U1 (ghosts filled here)
     // with a caller-chosen dt (main.cpp:128-143 chooses it).
     void step(double dtv) {
@@ -1162,6 +1201,7 @@ class Solver {
         copy(U, U1);
         advance(dtv);
         if (p.PLASTIC) plastic_update();
+        if (p.RADIAL) geometric_source(dtv);
         if (p.SOLID) cleanup_v();
         cleanup_alpha_all(U1);
     }

@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libtenjin.so")
 TJ_PREC_FP64 = 0
 TJ_PREC_MIXED = 1
 TJ_ST_NAN, TJ_ST_WAVESPEED, TJ_ST_ALPHA_NANS, TJ_ST_ALPHA_SUM, TJ_ST_PLASTIC_DIV = 1, 2, 4, 8, 16
+TJ_OP_PLASTIC, TJ_OP_CLEAN_V, TJ_OP_CLEAN_ALPHA, TJ_OP_DT, TJ_OP_NANCHECK, TJ_OP_RADIAL = 1, 2, 4, 8, 16, 32
 
 
 class Material(ctypes.Structure):
@@ -57,6 +58,8 @@ class LevelDesc(ctypes.Structure):
         ("bc_lo", ctypes.c_int * 3),
         ("bc_hi", ctypes.c_int * 3),
         ("mat", Material * 4),
+        ("RADIAL", ctypes.c_int),
+        ("pad_", ctypes.c_int),
     ]
 
 
@@ -159,6 +162,7 @@ def level_desc(pb) -> LevelDesc:
         d.bc_hi[a] = pb.bc_hi[a]
     d.max_grid_size = pb.max_grid_size
     d.SOLID, d.THINC, d.PLASTIC, d.MUSCL = pb.SOLID, pb.THINC, pb.PLASTIC, pb.MUSCL
+    d.RADIAL = pb.RADIAL
     d.THINCbeta = pb.THINCbeta
     d.CFL = pb.CFL
     d.nmat = pb.nmat

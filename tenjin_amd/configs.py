@@ -135,6 +135,7 @@ class Problem:
     THINC: int = 0
     PLASTIC: int = 0
     MUSCL: int = 1
+    RADIAL: int = 0  # geometric (cylindrical / spherical) source, radial.cpp
     THINCbeta: float = 1.6
     CFL: float = 0.5
     bc_lo: tuple = (0, 0, 0)
@@ -328,6 +329,69 @@ def plate_impact(dim: int = 3, n: int = 256, box: int = 64, seed: int = 3, speed
     U[L.comp(P)] = 0.0
     for r in range(3):
         U[L.comp(V_TENSOR, 0, r, r)] = 1.0
+    pb.prims = U
+    return pb
+
+
+def underwater_explosion(dim: int = 1, n: int = 800, box: int = 1 << 30, radius: float = 0.2,
+                         thinc: int = 0) -> Problem:
+    """High-pressure gas bubble (gamma 1.4, rho 1270, p 8.29e8) in water
+    (stiffened gas 4.4 / 6e8, rho 1000, p 1e5) about the origin of a radial
+    coordinate, with the geometric source on (RADIAL):
+
+    * dim 1: a sphere, x = r in [0, 1] (multiplier -2 in radial.cpp:11);
+    * dim 2: (x, y) = (r, z) in [0, 1] x [0, 1], the bubble centred on the
+      axis at z = 1/2 (cylindrical, multiplier -1).
+
+    Reflective at r = 0 (x low), transmissive elsewhere.  Cell centres decide
+    the material."""
+    mats = [Material.stiffened(4.4, 6e8), Material.stiffened(1.4, 0.0)]
+    nn = (n, n if dim > 1 else 1, 1)
+    pb = Problem(f"underwater_explosion_{dim}d", dim, nn, (1.0, 1.0, 1.0), box, mats, THINC=thinc,
+                 bc_lo=(1, 0, 0), bc_hi=(0, 0, 0), final_time=2e-4, RADIAL=1)
+    U = _empty_prims(pb)
+    nz, ny, nx = pb.shape
+    r = (np.arange(nx) + 0.5) / nx
+    z = (np.arange(ny) + 0.5) / ny if dim > 1 else np.zeros(1)
+    Z, R = np.meshgrid(z, r, indexing="ij")  # [ny][nx]
+    inside = (np.hypot(R, Z - 0.5) if dim > 1 else R) < radius
+    water = ~inside[None]
+    zero = np.zeros((1, ny, nx))
+    _set_two_phase(pb, U, water, 1000.0, 1270.0, [zero, zero, zero], np.where(water, 1e5, 8.29e8))
+    pb.prims = U
+    return pb
+
+
+def rod_impact_axisymmetric(n: int = 96, box: int = 32, speed: float = 300.0, radius: float = 0.3,
+                            length: float = 10e-3, seed: int = 5) -> Problem:
+    """(x, y) = (r, z) axisymmetric impact with the geometric source on: a
+    copper rod (r < radius L, z > L/2) moving at -speed along z into
+    aluminium at rest, which fills the rest of the domain; Us-up
+    Mie-Gruneisen solids with perfect plasticity, V = I, thermal p = 0,
+    +-0.1 % seeded density noise.  Reflective at r = 0, transmissive
+    elsewhere."""
+    cu, al = COPPER, ALUMINIUM
+    mats = [Material.usup_solid(cu["rho0"], cu["Gamma0"], cu["c0"], cu["s"], cu["G"], cu["Y"], True),
+            Material.usup_solid(al["rho0"], al["Gamma0"], al["c0"], al["s"], al["G"], al["Y"], True)]
+    pb = Problem("rod_impact_axisymmetric", 2, (n, n, 1), (length, length, 1.0), box, mats,
+                 SOLID=1, PLASTIC=1, RADIAL=1, bc_lo=(1, 0, 0), steps=60, shift_pressure=1)
+    L = pb.layout
+    U = _empty_prims(pb)
+    h = length / n
+    c = (np.arange(n) + 0.5) * h
+    Z, R = np.meshgrid(c, c, indexing="ij")
+    rod = (R < radius * length) & (Z > 0.5 * length)
+    noise = _mt_uniform(seed, pb.ncells).reshape(pb.shape)
+    fac = 1.0 + 1e-3 * (2.0 * noise - 1.0)
+    a0 = np.where(rod, 1.0 - ABSENT_ALPHA, ABSENT_ALPHA)[None]
+    U[L.comp(ALPHA, 0)] = a0
+    U[L.comp(ALPHA, 1)] = 1.0 - a0
+    U[L.comp(RHO_K, 0)] = cu["rho0"] * fac
+    U[L.comp(RHO_K, 1)] = al["rho0"] * fac
+    U[L.comp(VELOCITY, 0, 1)] = np.where(rod, -speed, 0.0)[None]
+    U[L.comp(P)] = 0.0
+    for q in range(3):
+        U[L.comp(V_TENSOR, 0, q, q)] = 1.0
     pb.prims = U
     return pb
 

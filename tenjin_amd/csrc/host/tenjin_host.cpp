@@ -402,6 +402,14 @@ void PlasticEOS::plasticUpdate(CellArray& U, ParameterStruct& parameters) {
     tj_check(parameters, tj_post_step(d.ctx, &v, TJ_OP_PLASTIC), "plasticUpdate");
 }
 
+// radial.cpp:37-54 with parameters.dt
+void geometricSourceTerm(CellArray& U, ParameterStruct& parameters) {
+    tj_ctx* ctx = dev(parameters).ctx;
+    tj_check(parameters, tj_set_dt(ctx, parameters.dt, 0), "geometricSourceTerm: dt");
+    tj_mf v = U.view();
+    tj_check(parameters, tj_post_step(ctx, &v, TJ_OP_RADIAL), "geometricSourceTerm");
+}
+
 // hllc.cpp:699-707: U1 = 1/2 U + 1/2 HLLCadvance(HLLCadvance(U)) with
 // parameters.dt; as in the reference, U holds the averaged state afterwards.
 void advance(CellArray& U, CellArray& U1, CellArray& U2, CellArray& MUSCLgrad, CellArray& UL, CellArray& UR,
@@ -442,6 +450,9 @@ void bindLevel(ParameterStruct& parameters, const Geometry& geom, const Vector<B
     L.THINC = parameters.THINC;
     L.PLASTIC = parameters.PLASTIC;
     L.MUSCL = parameters.MUSCL;
+    L.RADIAL = parameters.RADIAL;
+    if (parameters.REACTIVE)
+        throw std::runtime_error("bindLevel: REACTIVE (reactiveUpdate, mixture EOS) is not part of this library");
     L.THINCbeta = parameters.THINCbeta;
     L.CFL = parameters.CFL;
     L.nmat = parameters.numberOfMaterials;

@@ -5,6 +5,7 @@
 //   advance                         hllc.cpp:699-707 (declared simulationheader.h:32)
 //   TimeStep::getTimeStep           timestep.cpp:36-70 (timestep.h:15-16)
 //   PlasticEOS::plasticUpdate       plastic.cpp:227-244 (plastic.h:18-19)
+//   geometricSourceTerm             radial.cpp:37-54 (simulationheader.h:40)
 //   CellArray::{primitiveToConservative, conservativeToPrimitive,
 //     getSoundSpeed, cleanUpV, cleanUpAlpha, operator=, operator*,
 //     operator+, contains_nan}     cellarray.cpp:5-117 (cellarray.h:24-33)
@@ -192,6 +193,12 @@ public:
 void advance(CellArray& U, CellArray& U1, CellArray& U2, CellArray& MUSCLgrad, CellArray& UL, CellArray& UR,
              CellArray& ULStar, CellArray& URStar, CellArray& UStarStar, Array<MultiFab, AMREX_SPACEDIM>& flux_arr,
              Geometry const& geom, ParameterStruct& parameters, Vector<BCRec>& bc, THINCArray& THINC);
+
+// The cylindrical (2D, 3D) or spherical (1D) source about x = 0 with
+// parameters.dt, then conservativeToPrimitive.  On solid levels the stress is
+// that of the state, as plasticUpdate leaves it; RADIAL solid levels must be
+// PLASTIC (bindLevel checks).
+void geometricSourceTerm(CellArray& U, ParameterStruct& parameters);
 
 namespace tenjin_host {
 

@@ -361,8 +361,8 @@ int tj_batch_create(tj_batch** out, int device, int nprob, const tj_level_desc* 
         const int p = b->order[r];
         const tj_level_desc& d = descs[p];
         b->descs[r] = d;
-        if (d.SOLID || d.THINC != b->thinc) {
-            b->err = "batch: fluid problems with one THINC setting";
+        if (d.SOLID || d.THINC != b->thinc || d.RADIAL) {
+            b->err = "batch: fluid problems with one THINC setting and no geometric source";
             *out = b;
             return TJ_EUNSUPPORTED;
         }

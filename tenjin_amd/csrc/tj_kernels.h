@@ -197,13 +197,48 @@ __device__ __forceinline__ void load_cons(const Level& lv, const MF& U, int i, i
     for (int q = 0; q < 3; ++q) ru[q] = U.at(lv.cs[RHOU] + q, i, j, k);
     E = U.at(lv.cs[TOTAL_E], i, j, k);
 }
+// geometricSourceTerm (radial.cpp:5-31): the factor dt * multiplier /
+// ((i + 1/2) dx) of cell i, multiplier -2 in one dimension (spherical) and -1
+// otherwise (cylindrical about the x = 0 axis).
+__device__ __forceinline__ double radial_factor(const Level& lv, int i, double dt) {
+    const double multiplier = lv.dim == 1 ? -2.0 : -1.0;
+    const double inverse_r = multiplier / (((double)i + 0.5) * lv.dx[0]);
+    return dt * inverse_r;
+}
+// The geometric fluxes of rhoU and E (flux.cpp:19-29, 51-77) added with
+// factor f: s0 is stress column 0 (sigma_r0, r = 0..2); s01, s02, s22 the
+// other stress entries the fluxes read.  Each conservative variable reads
+// only its own old value and the primitives, so the in-place order of
+// radial.cpp:13-28 does not matter.
+__device__ __forceinline__ void radial_momentum_energy(double f, const double* u, const double* s0, double s01,
+                                                       double s02, double s22, double* ru, double& E) {
+    ru[0] += f * (ru[0] * u[0] - s0[0] + s22);
+    ru[1] += f * (ru[1] * u[0] - s01);
+    ru[2] += f * (-2.0 * s02);
+    double vs = 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) vs += u[r] * s0[r];
+    E += f * (E * u[0] - vs);
+}
+// alpha has no geometric flux (flux.cpp:55): x + f * 0 leaves every x as it is.
+template <int NM>
+__device__ __forceinline__ void radial_fluid(double f, const double* u, double p, double* ar, double* ru, double& E) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) ar[m] += f * (ar[m] * u[0]);
+    // sigma = -p I of a fluid cell (stressTensor, boxaccesscellarray.cpp:205-214)
+    const double s0[3] = {-p * 1.0, -p * 0.0, -p * 0.0};
+    radial_momentum_energy(f, u, s0, -p * 0.0, -p * 0.0, -p * 1.0, ru, E);
+}
+
 // conservativeToPrimitive of the updated cell (hllc.cpp:689), and in stage 2
-// the RK average (hllc.cpp:705), cleanUpAlpha (main.cpp:169-171), the NaN scan
-// (main.cpp:183-187) and the sound speed / CFL candidate of the next
-// getTimeStep (main.cpp:128); then the store.
+// the RK average (hllc.cpp:705), the geometric source (main.cpp:159-162),
+// cleanUpAlpha (main.cpp:169-171), the NaN scan (main.cpp:183-187) and the
+// sound speed / CFL candidate of the next getTimeStep (main.cpp:128); then the
+// store.
 template <int NM, int STAGE>
 __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const MF& Uo, int i, int j, int k, double* al,
-                                            double* ar, double* ru, double E, unsigned ops, double& cand, unsigned& st) {
+                                            double* ar, double* ru, double E, double dt, unsigned ops, double& cand,
+                                            unsigned& st) {
     double rk[NM], rho, u[3], p;
     c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
     if (STAGE == 2) {
@@ -221,6 +256,10 @@ __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const
         }
         E = Ub.at(lv.cs[TOTAL_E], i, j, k) * (1.0 / 2.0) + E * (1.0 / 2.0);
         p = Ub.at(lv.cs[PRESSURE], i, j, k) * (1.0 / 2.0) + p * (1.0 / 2.0);
+        if (ops & 32u) {  // the averaged sigma is -p I with the averaged p
+            radial_fluid<NM>(radial_factor(lv, i, dt), u, p, ar, ru, E);
+            c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+        }
         if (ops & 4u) st |= cleanup_alpha<NM>(al);
         if (ops & 16u) {
             bool bad = false;
@@ -427,7 +466,7 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
             accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[1], Fa, Fb);
         }
         accum_dir<NM, R>(al, ar, ru, E, a0, dt / lv.dx[D], Flo, Fhi);
-        if (act) finish_cell<NM, STAGE>(lv, Ub, Uo, i, jj, kk, al, ar, ru, E, ops, cand, st);
+        if (act) finish_cell<NM, STAGE>(lv, Ub, Uo, i, jj, kk, al, ar, ru, E, dt, ops, cand, st);
     };
     auto emit = [&](int face, const FSt<NM, R>& A, const FSt<NM, R>& B, R* Fv) {
         R sa[NM];
@@ -611,7 +650,7 @@ __device__ __forceinline__ void update_body(const Level& lv, const MF& Uin, cons
             load_faces<NM, T>(Fd, i + (d == 0), j + (d == 1), k + (d == 2), Fb);
             accum_dir<NM, T>(al, ar, ru, E, a0, dt / lv.dx[d], Fa, Fb);
         }
-        finish_cell<NM, STAGE>(lv, Ub, Uo, i, j, k, al, ar, ru, E, ops, cand, st);
+        finish_cell<NM, STAGE>(lv, Ub, Uo, i, j, k, al, ar, ru, E, dt, ops, cand, st);
     }
     if (STAGE == 2 && (ops & 8u)) block_min_to(cand, &clk->dtmin_bits);
     if (st) atomicOr(&clk->status, st);
@@ -708,6 +747,38 @@ __global__ __launch_bounds__(256) void k_cleanup_alpha(Level lv, MF U, Clock* cl
 #pragma unroll
     for (int m = 0; m < NM; ++m) U.at(lv.cs[ALPHA] + m, i, j, k) = a[m];
     if (st) atomicOr(&clk->status, st);
+}
+
+// stand-alone geometricSourceTerm of a fluid level (radial.cpp:37-54) with the
+// clock's dt: the source, then conservativeToPrimitive
+template <int NM>
+__global__ __launch_bounds__(256) void k_radial_fluid(Level lv, MF U, Clock* clk) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    double al[NM], ar[NM], ru[3], E, rk[NM], rho, u[3], p;
+    load_cons<NM>(lv, U, i, j, k, al, ar, ru, E);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) u[q] = U.at(lv.cs[VELOCITY] + q, i, j, k);
+    p = U.at(lv.cs[PRESSURE], i, j, k);
+    radial_fluid<NM>(radial_factor(lv, i, clk->dt), u, p, ar, ru, E);
+    c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        U.at(lv.cs[ALPHARHO] + m, i, j, k) = ar[m];
+        U.at(lv.cs[RHO_K] + m, i, j, k) = rk[m];
+    }
+    U.at(lv.cs[RHO], i, j, k) = rho;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        U.at(lv.cs[RHOU] + q, i, j, k) = ru[q];
+        U.at(lv.cs[VELOCITY] + q, i, j, k) = u[q];
+    }
+    U.at(lv.cs[TOTAL_E], i, j, k) = E;
+    U.at(lv.cs[PRESSURE], i, j, k) = p;
 }
 
 // contains_nan over conserved and primitive components (boxaccesscellarray.cpp:601-645)

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <map>
+#include <queue>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -38,7 +39,11 @@ struct tj_ctx {
     unsigned* d_status = nullptr;
     long long pass = 0;
     long long dpass[3] = {0, 0, 0};
-    int seglen = 16;
+    // rows per marching segment: 0 = chosen per pass from the launch shape
+    // (pick_seglen, cached per kernel and direction), > 0 = fixed by the caller
+    int seglen = 0;
+    int ncu = 0;  // compute units of the device
+    std::map<std::pair<const void*, int>, int> seg_cache;
     bool fuse = false;
     // a captured run of loop bodies (tj_graph_capture) and the THINC pass
     // counters it advances each launch
@@ -193,6 +198,8 @@ int tj_create(tj_ctx** out, int device) {
     if (hipSetDevice(device) != hipSuccess) { delete c; return TJ_EHIP; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TJ_EHIP; }
     if (hipMalloc(&c->d_clk, sizeof(Clock)) != hipSuccess) { delete c; return TJ_EHIP; }
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->ncu < 1)
+        c->ncu = 1;
     Clock init{};
     init.final_time = __builtin_huge_val();
     init.dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
@@ -247,6 +254,13 @@ static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::
     }
     if (!d->MUSCL) {
         err = "bind_level: MUSCL = 0 is not available in this build";
+        return TJ_EUNSUPPORTED;
+    }
+    if (d->RADIAL && d->SOLID && !d->PLASTIC) {
+        // the source reads the stored stress, which on an elastic level is the
+        // RK average of two states' stresses (main.cpp:149-162), not the stress
+        // of the averaged state this library can recompute
+        err = "bind_level: RADIAL on a solid level needs PLASTIC in this build";
         return TJ_EUNSUPPORTED;
     }
     lv = Level{};
@@ -324,6 +338,7 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_all(c);
+    c->seg_cache.clear();
     c->desc = *d;
     Level& lv = c->lv;
     lv = lvn;
@@ -449,6 +464,7 @@ static unsigned step_ops(const tj_ctx* c) {
     unsigned ops = TJ_OP_CLEAN_ALPHA | TJ_OP_DT | TJ_OP_NANCHECK;
     if (c->desc.PLASTIC) ops |= TJ_OP_PLASTIC;
     if (c->desc.SOLID) ops |= TJ_OP_CLEAN_V;
+    if (c->desc.RADIAL) ops |= TJ_OP_RADIAL;
     return ops;
 }
 
@@ -530,6 +546,99 @@ static Hist hist_for(tj_ctx* c, int d) {
     return h;
 }
 
+// Rows per segment of a marching pass along d.  The pass's waves are numbered
+// strip fastest, then segment, then the outer index, four to a block, and a
+// block holds one of `slots` block slots: compute units x blocks per unit at
+// the kernel's occupancy, at most two (eight waves per unit; measured on
+// gfx950, these latency-bound passes gain nothing from more resident waves).
+// A wave's time is taken as its segment length + 2 (the row above the segment
+// and its set-up), a block's as its slowest wave's.  Of the seglen values
+// 8..24 the one with the shortest estimated pass time is returned (ties: the
+// one closest to 16):
+//  * up to two blocks per slot at seglen 16, max(total work / slots, longest
+//    block): blocks of unequal work (interface cells) let a short tail round
+//    fill in behind the first, and a level too small for the device gets
+//    shorter segments so that more waves run;
+//  * beyond that, the makespan of list scheduling in launch order, which
+//    favours segment counts that divide into whole rounds;
+//  * a pass much larger than the device keeps 16.
+// The choice changes no value a pass computes, only how rows are split among
+// waves.
+static int pick_seglen(long long nstrip, int units, long long no, long long slots) {
+    const long long nb16 = (nstrip * ((units + 15) / 16) * no + 3) / 4;
+    if (nb16 > 64 * slots) return 16;
+    const bool few_rounds = nb16 <= 2 * slots;
+    int best = 16;
+    double best_t = -1.0;
+    for (int sl = 8; sl <= 24; ++sl) {
+        const long long nseg = (units + sl - 1) / sl;
+        const long long nw = nstrip * nseg * no;
+        const long long nb = (nw + 3) / 4;
+        auto work = [&](long long b) {
+            long long w = 0;
+            for (long long q = 4 * b; q < std::min(4 * b + 4, nw); ++q) {
+                const long long seg = (q / nstrip) % nseg;
+                w = std::max(w, std::min<long long>(sl, units - seg * sl) + 2);
+            }
+            return w;
+        };
+        double t = 0.0;
+        if (few_rounds) {
+            long long total = 0, longest = 0;
+            for (long long b = 0; b < nb; ++b) {
+                const long long w = work(b);
+                total += w;
+                longest = std::max(longest, w);
+            }
+            t = std::max((double)total / (double)slots, (double)longest);
+        } else {
+            std::priority_queue<long long, std::vector<long long>, std::greater<long long>> free_at;
+            for (long long q = 0; q < slots; ++q) free_at.push(0);
+            long long end_max = 0;
+            for (long long b = 0; b < nb; ++b) {
+                const long long start = free_at.top();
+                free_at.pop();
+                const long long end = start + work(b);
+                end_max = std::max(end_max, end);
+                free_at.push(end);
+            }
+            t = (double)end_max;
+        }
+        if (best_t < 0.0 || t < best_t || (t == best_t && std::abs(sl - 16) < std::abs(best - 16))) {
+            best = sl;
+            best_t = t;
+        }
+    }
+    return best;
+}
+
+// seglen of one marching kernel along d: the caller's fixed value, or
+// pick_seglen at the kernel's occupancy (blocks of 256 threads, static LDS).
+static int seg_for(tj_ctx* c, const void* kernel, int d, bool fused) {
+    if (c->seglen > 0) return c->seglen;
+    const auto key = std::make_pair(kernel, d * 2 + (fused ? 1 : 0));
+    const auto it = c->seg_cache.find(key);
+    if (it != c->seg_cache.end()) return it->second;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const Level& lv = c->lv;
+    const long long nstrip = (lv.n[0] + 63) / 64;
+    const long long no = d == 1 ? lv.n[2] : lv.n[1];
+    const int units = fused ? lv.n[d] : lv.n[d] + 1;  // rows owned (fused) or faces computed
+    const int sl = pick_seglen(nstrip, units, no, (long long)c->ncu * std::min(per_cu, 2));
+    c->seg_cache[key] = sl;
+    return sl;
+}
+// grid of a marching pass along d with segments of sl rows (faces, or rows
+// owned when the update is fused)
+static unsigned march_grid(const Level& lv, int d, int sl, bool fused) {
+    const long long nstrip = (lv.n[0] + 63) / 64;
+    const long long nseg = fused ? (lv.n[d] + sl - 1) / sl : (lv.n[d] + 1 + sl - 1) / sl;
+    const long long no = d == 1 ? lv.n[2] : lv.n[1];
+    return grid1(nstrip * nseg * no, 4);
+}
+#define TJ_KPTR(...) reinterpret_cast<const void*>(&__VA_ARGS__)
+
 // One directional flux pass (not fused) with face values of type R.
 extern "C++" {
 template <typename R>
@@ -541,16 +650,14 @@ static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, con
             const long long nseg = (lv.n[0] + 1 + 62) / 63;
             const unsigned g = grid1(nseg * lv.n[1] * lv.n[2], 4);
             hipLaunchKernelGGL((k_flux_x_s<2, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], c->d_clk);
+        } else if (d == 1) {
+            const int sl = seg_for(c, TJ_KPTR(k_flux_lmarch_s<2, 1, R>), 1, false);
+            hipLaunchKernelGGL((k_flux_lmarch_s<2, 1, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0, c->stream,
+                               lv, Uin, F[1], sl, c->d_clk);
         } else {
-            const long long nstrip = (lv.n[0] + 63) / 64;
-            const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
-            const long long no = d == 1 ? lv.n[2] : lv.n[1];
-            const unsigned g = grid1(nstrip * nseg * no, 4);
-            if (d == 1) {
-                hipLaunchKernelGGL((k_flux_lmarch_s<2, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], c->seglen, c->d_clk);
-            } else {
-                hipLaunchKernelGGL((k_flux_lmarch_s<2, 2, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[2], c->seglen, c->d_clk);
-            }
+            const int sl = seg_for(c, TJ_KPTR(k_flux_lmarch_s<2, 2, R>), 2, false);
+            hipLaunchKernelGGL((k_flux_lmarch_s<2, 2, R>), dim3(march_grid(lv, 2, sl, false)), dim3(256), 0, c->stream,
+                               lv, Uin, F[2], sl, c->d_clk);
         }
     } else if (d == 0) {
         const int per = th ? 61 : 63;
@@ -559,18 +666,24 @@ static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, con
         if (th) hipLaunchKernelGGL((k_flux_x<2, true, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], h, c->d_clk);
         else hipLaunchKernelGGL((k_flux_x<2, false, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], h, c->d_clk);
     } else {
-        const long long nstrip = (lv.n[0] + 63) / 64;
-        const long long nseg = (lv.n[d] + 1 + c->seglen - 1) / c->seglen;
-        const long long no = d == 1 ? lv.n[2] : lv.n[1];
-        const unsigned g = grid1(nstrip * nseg * no, 4);
         const MF z{};
         const FacesT<R> zf{};
-        if (d == 1) {
-            if (th && sizeof(R) == 8) hipLaunchKernelGGL((k_flux_march_w2<2, true, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk);
-            else if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
-            else hipLaunchKernelGGL((k_flux_march<2, false, 1, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[1], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
+        if (d == 1 && th && sizeof(R) == 8) {
+            const int sl = seg_for(c, TJ_KPTR(k_flux_march_w2<2, true, 1, R>), 1, false);
+            hipLaunchKernelGGL((k_flux_march_w2<2, true, 1, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
+                               c->stream, lv, Uin, F[1], h, sl, c->d_clk);
+        } else if (d == 1 && th) {
+            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, true, 1, false, 1, R>), 1, false);
+            hipLaunchKernelGGL((k_flux_march<2, true, 1, false, 1, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
+                               c->stream, lv, Uin, F[1], h, sl, c->d_clk, z, z, zf, zf, 0u);
+        } else if (d == 1) {
+            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, false, 1, false, 1, R>), 1, false);
+            hipLaunchKernelGGL((k_flux_march<2, false, 1, false, 1, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
+                               c->stream, lv, Uin, F[1], h, sl, c->d_clk, z, z, zf, zf, 0u);
         } else {
-            hipLaunchKernelGGL((k_flux_march<2, false, 2, false, 1, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[2], h, c->seglen, c->d_clk, z, z, zf, zf, 0u);
+            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, false, 2, false, 1, R>), 2, false);
+            hipLaunchKernelGGL((k_flux_march<2, false, 2, false, 1, R>), dim3(march_grid(lv, 2, sl, false)), dim3(256), 0,
+                               c->stream, lv, Uin, F[2], h, sl, c->d_clk, z, z, zf, zf, 0u);
         }
     }
 }
@@ -581,22 +694,25 @@ static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, con
 static void launch_fused(tj_ctx* c, const MF& Uin, int d, const Hist& h, const FuseArgs* fz) {
     const Level& lv = c->lv;
     const bool th = c->desc.THINC != 0;
-    const long long nstrip = (lv.n[0] + 63) / 64;
-    const long long nseg = (lv.n[d] + c->seglen - 1) / c->seglen;  // segments own rows [f0, f0 + seglen)
-    const long long no = d == 1 ? lv.n[2] : lv.n[1];
-    const unsigned g = grid1(nstrip * nseg * no, 4);
     const MF b = fz->base, o = fz->out;
     const unsigned ops = fz->ops;
     const bool s2 = fz->stage == 2;
+#define TJ_FUSED(TH, D, ST)                                                                                     \
+    do {                                                                                                        \
+        const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, TH, D, true, ST>), D, true);                          \
+        hipLaunchKernelGGL((k_flux_march<2, TH, D, true, ST>), dim3(march_grid(lv, D, sl, true)), dim3(256), 0, \
+                           c->stream, lv, Uin, c->F[D], h, sl, c->d_clk, b, o, c->F[0], c->F[1], ops);          \
+    } while (0)
     if (d == 1) {
-        if (th && s2) hipLaunchKernelGGL((k_flux_march<2, true, 1, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-        else if (th) hipLaunchKernelGGL((k_flux_march<2, true, 1, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-        else if (s2) hipLaunchKernelGGL((k_flux_march<2, false, 1, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-        else hipLaunchKernelGGL((k_flux_march<2, false, 1, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[1], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+        if (th && s2) TJ_FUSED(true, 1, 2);
+        else if (th) TJ_FUSED(true, 1, 1);
+        else if (s2) TJ_FUSED(false, 1, 2);
+        else TJ_FUSED(false, 1, 1);
     } else {
-        if (s2) hipLaunchKernelGGL((k_flux_march<2, false, 2, true, 2>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
-        else hipLaunchKernelGGL((k_flux_march<2, false, 2, true, 1>), dim3(g), dim3(256), 0, c->stream, lv, Uin, c->F[2], h, c->seglen, c->d_clk, b, o, c->F[0], c->F[1], ops);
+        if (s2) TJ_FUSED(false, 2, 2);
+        else TJ_FUSED(false, 2, 1);
     }
+#undef TJ_FUSED
 }
 
 // Directional flux passes of one RK stage.  With fz (FP64 form, fluid
@@ -656,10 +772,16 @@ int tj_post_step(tj_ctx* c, tj_mf* U, unsigned ops) {
         c->err = "post_step: PLASTIC and CLEAN_V apply to solid levels";
         return TJ_EINVAL;
     }
+    if ((ops & TJ_OP_RADIAL) && c->lv.solid && !c->lv.plastic) {
+        c->err = "post_step: RADIAL on a solid level needs a PLASTIC level";
+        return TJ_EINVAL;
+    }
     const unsigned g = grid1(c->lv.ncells, 256);
     if (c->lv.solid) {
         hipLaunchKernelGGL(k_post_s<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk, ops);
     } else {
+        if (ops & TJ_OP_RADIAL)
+            hipLaunchKernelGGL(k_radial_fluid<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
         if (ops & TJ_OP_CLEAN_ALPHA)
             hipLaunchKernelGGL(k_cleanup_alpha<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
         if (ops & TJ_OP_NANCHECK)
@@ -862,8 +984,8 @@ int tj_status(tj_ctx* c, uint32_t* status, int clear) {
 }
 
 int tj_set_tuning(tj_ctx* c, int seglen, int fuse) {
-    if (!c || seglen == 0) return TJ_EINVAL;
-    if (seglen > 0) c->seglen = seglen;
+    if (!c) return TJ_EINVAL;
+    if (seglen >= 0) c->seglen = seglen;
     if (fuse >= 0) c->fuse = fuse != 0;
     return TJ_OK;
 }

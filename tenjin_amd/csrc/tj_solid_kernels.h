@@ -261,11 +261,42 @@ __device__ __forceinline__ void store_scell(const Level& lv, const MF& U, int i,
     U.at(lv.cs[HJ2], i, j, k) = C.hj2;
 }
 
+// geometricSourceTerm (radial.cpp:5-31, flux.cpp:51-77) of a solid cell
+// whose stored stress is that of its own state, as it is after the
+// conservativeToPrimitive that ends plasticUpdate: sigma from p and the
+// deviatoric Hencky strain dH (stressTensor, boxaccesscellarray.cpp:176-203).
+// V rows 0 and 1 take +1/3 V u_x, row 2 -2/3 V u_x.
+template <int NM>
+__device__ __forceinline__ void radial_solid(const Level& lv, double f, SCell<NM>& C, const double* dH) {
+    double s0[3], s1[3], s2[3];
+    stress_col<NM>(lv, C.al, C.p, dH, 0, s0);
+    stress_col<NM>(lv, C.al, C.p, dH, 1, s1);
+    stress_col<NM>(lv, C.al, C.p, dH, 2, s2);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        C.ar[m] += f * (C.ar[m] * C.u[0]);
+        if (lv.plastic) C.are[m] += f * (C.are[m] * C.u[0]);
+    }
+    radial_momentum_energy(f, C.u, s0, s1[0], s2[0], s2[2], C.ru, C.E);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) C.V[q] += f * ((1.0 / 3.0) * C.V[q] * C.u[0]);
+#pragma unroll
+    for (int q = 6; q < 9; ++q) C.V[q] += f * (-(2.0 / 3.0) * C.V[q] * C.u[0]);
+}
+// plasticUpdate's conservativeToPrimitive leaves p and dH of the state; a
+// stand-alone source recomputes dH from V (hencky does what getHenckyJ2 does).
+template <int NM>
+__device__ __forceinline__ void radial_solid_c2p(const Level& lv, double f, SCell<NM>& C, double* dH, bool have_dH) {
+    if (!have_dH) hencky(C.V, dH);
+    radial_solid<NM>(lv, f, C, dH);
+    C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+}
+
 // update (hllc.cpp:493-523, with the V rows of :511-514) in x, y, z order,
 // then conservativeToPrimitive.  Stage 2 adds the RK average (hllc.cpp:705),
-// plasticUpdate (main.cpp:154-157, plastic.cpp:227-244), cleanUpV
-// (main.cpp:164-167), cleanUpAlpha (main.cpp:169-171), the NaN scan and the
-// next getTimeStep.
+// plasticUpdate (main.cpp:154-157, plastic.cpp:227-244), the geometric source
+// (main.cpp:159-162), cleanUpV (main.cpp:164-167), cleanUpAlpha
+// (main.cpp:169-171), the NaN scan and the next getTimeStep.
 template <int NM, int STAGE, typename T = double>
 __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo, FacesT<T> F0, FacesT<T> F1, FacesT<T> F2,
                                                   Clock* clk, unsigned ops) {
@@ -344,6 +375,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
                 st |= plastic_cell<NM>(lv, C.al, C.ar, C.e, C.are, C.V);
                 C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
             }
+            if (ops & 32u) radial_solid_c2p<NM>(lv, radial_factor(lv, i, dt), C, dH, (ops & 1u) != 0);
             if (ops & 2u) cleanup_v<NM>(lv, C.al, C.V);
             if (ops & 4u) st |= cleanup_alpha<NM>(C.al);
             if (ops & 16u) {
@@ -493,8 +525,9 @@ __global__ __launch_bounds__(256) void k_c2p_s(Level lv, MF U) {
 }
 
 // The post-step chain of main.cpp:154-171 on a state, stand-alone
-// (tj_post_step): plastic + c2p, cleanUpV, cleanUpAlpha, NaN scan, sound speed
-// and the CFL minimum, each when its bit is set.
+// (tj_post_step): plastic + c2p, the geometric source + c2p (with the clock's
+// dt), cleanUpV, cleanUpAlpha, NaN scan, sound speed and the CFL minimum, each
+// when its bit is set.
 template <int NM>
 __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsigned ops) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
@@ -513,6 +546,7 @@ __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsi
             st |= plastic_cell<NM>(lv, C.al, C.ar, C.e, C.are, C.V);
             C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
         }
+        if (ops & 32u) radial_solid_c2p<NM>(lv, radial_factor(lv, i, clk->dt), C, dH, (ops & 1u) != 0);
         if (ops & 2u) cleanup_v<NM>(lv, C.al, C.V);
         if (ops & 4u) st |= cleanup_alpha<NM>(C.al);
         if (ops & 16u) {

@@ -6,7 +6,8 @@ geometry in place of the reference's hard-coded chooseStateBasedOnInitialConditi
 Supported: fluid materials with EOS "MieGruneisen" (stiffened gas /
 constant-Gamma Mie-Gruneisen: adiabaticIndex, pref, eref, CV) and solid
 materials with EOS "UsUpSolid" (rho0, Gamma0, c0, s, G0, yieldStress; this
-project's Us-up Mie-Gruneisen solid, which has no reference class).  Mixtures
+project's Us-up Mie-Gruneisen solid, which has no reference class); RADIAL
+(initialise.cpp:332) switches the geometric source on.  Mixtures, REACTIVE
 and the Romenskii and Wilkins solids are rejected with an error.  Host-side
 set-up only.
 """
@@ -148,6 +149,8 @@ def problem_from_settings(cfg, max_grid_size: int = 64, dim: Optional[int] = Non
     nm = int(cfg["materials"])
     if int(cfg.get("mixtures", 0)) != 0:
         raise ValueError("settings: mixtures are not part of this path")
+    if int(cfg.get("REACTIVE", 0)) != 0:
+        raise ValueError("settings: REACTIVE (reactiveUpdate with mixture EOS) is not part of this path")
     solid_run = int(cfg.get("SOLID", 0))
     plastic_run = int(cfg.get("PLASTIC", 0))
     mats = []
@@ -171,7 +174,8 @@ def problem_from_settings(cfg, max_grid_size: int = 64, dim: Optional[int] = Non
     pb = Problem(name, dim, nn, tuple(L), max_grid_size, mats, SOLID=solid_run, THINC=int(cfg.get("THINC", 0)),
                  PLASTIC=plastic_run, MUSCL=int(cfg.get("MUSCL", 1)), THINCbeta=float(cfg.get("THINCbeta", 1.6)),
                  CFL=float(cfg.get("CFL", 0.5)), bc_lo=tuple(low[:3]), bc_hi=tuple(high[:3]),
-                 final_time=float(cfg["finalTime"]) if "finalTime" in cfg else None, shift_pressure=1)
+                 final_time=float(cfg["finalTime"]) if "finalTime" in cfg else None, shift_pressure=1,
+                 RADIAL=int(cfg.get("RADIAL", 0)))
     # states (initialise.cpp:236-263) and the per-cell choice
     states = cfg["listOfStates"][: int(cfg.get("states", len(cfg["listOfStates"])))]
     nz, ny, nx = pb.shape

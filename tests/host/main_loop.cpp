@@ -68,6 +68,7 @@ int main(int argc, char** argv) {
     parameters.SOLID = desc.SOLID;
     parameters.THINC = desc.THINC;
     parameters.PLASTIC = desc.PLASTIC;
+    parameters.RADIAL = desc.RADIAL;
     parameters.MUSCL = desc.MUSCL;
     parameters.CFL = desc.CFL;
     parameters.THINCbeta = desc.THINCbeta;
@@ -180,6 +181,9 @@ int main(int argc, char** argv) {
 
         if (parameters.PLASTIC) {
             plastic.plasticUpdate(U1, parameters);
+        }
+        if (parameters.RADIAL) {
+            geometricSourceTerm(U1, parameters);
         }
         if (parameters.SOLID) {
             U1.cleanUpV();

@@ -1,0 +1,178 @@
+"""The geometric (radial) source term: the oracle's restatement of
+geometricSourceTerm (radial.cpp:5-54, flux.cpp:51-77) pinned against the
+textbook axisymmetric / spherically symmetric balance laws, and the settings
+switch.  GPU parity for the source is in test_gpu_radial.py.
+
+The textbook forms are written from the equations, not from the reference's
+code: with x = r, the conservation laws div(F) in cylindrical (k = 1) or
+spherical (k = 2) symmetry differ from their Cartesian form by the source
+-(k / r) * [alpha rho u_r, T_rr - T_tt, T_rz, 2 T_rt, (E u - sigma u)_r],
+T = rho u (x) u - sigma the momentum flux tensor, t the hoop direction (row 2).
+The reference assumes no swirl (u_t = 0), which the test states keep.  The
+stretch tensor V of the solid model follows the deviatoric part of the hoop
+stretching, dV = dt (u_r / r) dev(e_t (x) e_t) V, which keeps det V = 1 to
+first order.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from tenjin_amd import configs as C
+
+
+def _random_state(dim, solid, n=24, seed=0):
+    rng = np.random.RandomState(seed)
+    if solid:
+        cu, al = C.COPPER, C.ALUMINIUM
+        mats = [C.Material.usup_solid(cu["rho0"], cu["Gamma0"], cu["c0"], cu["s"], cu["G"], cu["Y"], True),
+                C.Material.usup_solid(al["rho0"], al["Gamma0"], al["c0"], al["s"], al["G"], al["Y"], True)]
+    else:
+        mats = [C.Material.stiffened(4.4, 6e8), C.Material.stiffened(1.4, 0.0)]
+    nn = (n, n if dim > 1 else 1, 1)
+    pb = C.Problem("radial_unit", dim, nn, (1.0, 1.0, 1.0), 8, mats, SOLID=int(solid), PLASTIC=int(solid),
+                   RADIAL=1)
+    L = pb.layout
+    U = np.zeros((L.ncomp,) + pb.shape)
+    a0 = rng.uniform(0.1, 0.9, pb.shape)
+    U[L.comp(C.ALPHA, 0)] = a0
+    U[L.comp(C.ALPHA, 1)] = 1.0 - a0
+    if solid:
+        U[L.comp(C.RHO_K, 0)] = C.COPPER["rho0"] * rng.uniform(0.98, 1.03, pb.shape)
+        U[L.comp(C.RHO_K, 1)] = C.ALUMINIUM["rho0"] * rng.uniform(0.98, 1.03, pb.shape)
+        U[L.comp(C.VELOCITY, 0, 0)] = rng.uniform(-300.0, 300.0, pb.shape)
+        U[L.comp(C.VELOCITY, 0, 1)] = rng.uniform(-300.0, 300.0, pb.shape)
+        U[L.comp(C.P)] = rng.uniform(1e8, 1e9, pb.shape)
+        # symmetric positive stretch near the identity, det 1
+        Vs = np.empty(pb.shape + (3, 3))
+        for idx in np.ndindex(*pb.shape):
+            A = rng.uniform(-0.02, 0.02, (3, 3))
+            V = np.eye(3) + 0.5 * (A + A.T)
+            Vs[idx] = V / np.cbrt(np.linalg.det(V))
+        for r in range(3):
+            for c in range(3):
+                U[L.comp(C.V_TENSOR, 0, r, c)] = Vs[..., r, c]
+    else:
+        U[L.comp(C.RHO_K, 0)] = rng.uniform(900.0, 1100.0, pb.shape)
+        U[L.comp(C.RHO_K, 1)] = rng.uniform(1.0, 50.0, pb.shape)
+        U[L.comp(C.VELOCITY, 0, 0)] = rng.uniform(-100.0, 100.0, pb.shape)
+        U[L.comp(C.VELOCITY, 0, 1)] = rng.uniform(-100.0, 100.0, pb.shape) if dim > 1 else 0.0
+        U[L.comp(C.P)] = rng.uniform(1e5, 1e8, pb.shape)
+    pb.prims = U
+    return pb
+
+
+def _radius(pb):
+    nx = pb.shape[2]
+    return ((np.arange(nx) + 0.5) * (pb.L[0] / pb.n[0]))[None, None, :]
+
+
+def _check(delta, expect, what, rel=1e-9):
+    scale = np.max(np.abs(expect))
+    assert scale > 0.0, what
+    err = np.max(np.abs(delta - expect)) / scale
+    assert err < rel, f"{what}: relative error {err:.3e}"
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_fluid_source_is_the_textbook_one(dim):
+    pb = _random_state(dim, solid=False)
+    L = pb.layout
+    o = Oracle(pb)
+    s0 = o.get()
+    dt = 1e-5
+    o.geometric_source(dt)
+    s1 = o.get()
+    k = 2.0 if dim == 1 else 1.0
+    g = -dt * k / _radius(pb)
+    u = s0[L.comp(C.VELOCITY, 0, 0)]
+    p = s0[L.comp(C.P)]
+    for m in range(pb.nmat):
+        assert np.array_equal(s1[L.comp(C.ALPHA, m)], s0[L.comp(C.ALPHA, m)])
+        c = L.comp(C.ALPHARHO, m)
+        _check(s1[c] - s0[c], g * s0[c] * u, f"alphaRho{m}")
+    for q in range(2 if dim > 1 else 1):
+        c = L.comp(C.RHOU, 0, q)
+        _check(s1[c] - s0[c], g * s0[c] * u, f"rhoU{q}")  # T_rr - T_tt = rho u^2 for a fluid
+    c = L.comp(C.RHOU, 0, 2)
+    assert np.max(np.abs(s1[c] - s0[c])) == 0.0
+    c = L.comp(C.TOTAL_E)
+    _check(s1[c] - s0[c], g * (s0[c] + p) * u, "E")
+    # primitives follow from the new conserved state (conservativeToPrimitive)
+    rho = sum(s1[L.comp(C.ALPHARHO, m)] for m in range(pb.nmat))
+    assert np.allclose(s1[L.comp(C.VELOCITY, 0, 0)], s1[L.comp(C.RHOU, 0, 0)] / rho, rtol=1e-14, atol=0.0)
+
+
+def test_solid_source_is_the_textbook_one():
+    pb = _random_state(2, solid=True)
+    L = pb.layout
+    o = Oracle(pb)
+    s0 = o.get()
+    dt = 2e-6
+    o.geometric_source(dt)
+    s1 = o.get()
+    r = _radius(pb)
+    g = -dt / r
+    sh = pb.shape
+    u = np.stack([s0[L.comp(C.VELOCITY, 0, q)] for q in range(3)], -1)  # [..., 3]
+    sig = np.empty(sh + (3, 3))
+    V0 = np.empty(sh + (3, 3))
+    V1 = np.empty(sh + (3, 3))
+    for a in range(3):
+        for b in range(3):
+            sig[..., a, b] = s0[L.comp(C.SIGMA, 0, a, b)]
+            V0[..., a, b] = s0[L.comp(C.V_TENSOR, 0, a, b)]
+            V1[..., a, b] = s1[L.comp(C.V_TENSOR, 0, a, b)]
+    rho = s0[L.comp(C.RHO)]
+    T = rho[..., None, None] * u[..., :, None] * u[..., None, :] - sig
+    ur = u[..., 0]
+    mom = [T[..., 0, 0] - T[..., 2, 2], T[..., 0, 1], 2.0 * T[..., 0, 2]]
+    for q in range(3):
+        c = L.comp(C.RHOU, 0, q)
+        _check(s1[c] - s0[c], g * mom[q], f"rhoU{q}", rel=1e-8)
+    c = L.comp(C.TOTAL_E)
+    sig_u_r = np.einsum("...a,...ar->...r", u, sig)[..., 0]
+    _check(s1[c] - s0[c], g * (s0[c] * ur - sig_u_r), "E", rel=1e-8)
+    for m in range(pb.nmat):
+        for var in (C.ALPHARHO, C.ALPHARHOEPSILON):
+            c = L.comp(var, m)
+            if np.any(s0[c] != 0.0):
+                _check(s1[c] - s0[c], g * s0[c] * ur, f"var{var} mat{m}")
+    hoop = np.diag([-1.0, -1.0, 2.0]) / 3.0
+    dV = (dt * ur / r)[..., None, None] * np.einsum("ab,...bc->...ac", hoop, V0)
+    _check(V1 - V0, dV, "V")
+    # det V = 1 kept to first order in the step
+    eps = np.max(np.abs(dt * ur / r))
+    assert np.max(np.abs(np.linalg.det(V1) - np.linalg.det(V0))) < 4.0 * eps * eps + 1e-13
+
+
+def test_rest_state_is_kept_by_the_source():
+    """u = 0: every geometric flux vanishes for a fluid (sigma_rr = sigma_tt)."""
+    pb = C.underwater_explosion(2, 16, 8)
+    L = pb.layout
+    o = Oracle(pb)
+    s0 = o.get()
+    o.geometric_source(1e-5)
+    s1 = o.get()
+    for c in L.cons_comps():
+        assert np.array_equal(s1[c], s0[c]), c
+
+
+def test_settings_deck_radial_switch():
+    from tenjin_amd.settings import problem_from_settings
+
+    deck = """
+    numberOfxCells = 64; xdomainLength = 1.0; finalTime = 1e-4; materials = 2; states = 2;
+    RADIAL = 1; THINC = 0;
+    lowBoundary = [1]; highBoundary = [0];
+    listOfMaterials = ( { EOS = "MieGruneisen"; material = "fluid"; adiabaticIndex = 4.4; pref = -2.64e9; eref = 0.0; CV = 1.0; },
+                        { EOS = "MieGruneisen"; material = "fluid"; adiabaticIndex = 1.4; pref = 0.0; eref = 0.0; CV = 1.0; } );
+    interface = 0.2;
+    listOfStates = ( { p = 8.29e8; u = 0.0; material = ( { alpha = 1e-6; rho = 1000.0; }, { alpha = 0.999999; rho = 1270.0; } ); },
+                     { p = 1e5;    u = 0.0; material = ( { alpha = 0.999999; rho = 1000.0; }, { alpha = 1e-6; rho = 1270.0; } ); } );
+    """
+    pb = problem_from_settings(deck)
+    assert pb.RADIAL == 1 and pb.dim == 1
+    with pytest.raises(ValueError, match="REACTIVE"):
+        problem_from_settings(deck.replace("RADIAL = 1;", "RADIAL = 1; REACTIVE = 1;"))
