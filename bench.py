@@ -7,7 +7,9 @@ THINC/BVD, beta 1.6, HLLC, CFL 0.5), synthetic initial data generated here.
 `--workload plate3d` runs the 3D elastoplastic copper/aluminium plate impact
 on 256^3 cells in 64^3 boxes (Us-up Mie-Gruneisen, 5-wave flux, perfect
 plasticity); `--workload explosion_rz` an axisymmetric (r, z) underwater
-explosion on 1448^2 cells with the cylindrical geometric source.  One "step" is one full time step of the reference's loop body
+explosion on 1448^2 cells with the cylindrical geometric source;
+`--workload reactive2d` a reactive mixture (MixtureEOS with rootFind in every
+conservativeToPrimitive, pressure-based burn) on 2048 x 1024 cells.  One "step" is one full time step of the reference's loop body
 (main.cpp:124-171): CFL dt with the driver clamps, two RK stages, the RK
 average, plasticUpdate and cleanUpV (solids), cleanUpAlpha, the NaN scan and
 the next step's CFL reduction.  The state is resident in HBM before timing
@@ -50,6 +52,18 @@ FLUID_KERNEL_BYTES = {
     "update_stage1_mixed": 64 + 2 * 36 + 120,
     "update_stage2_post_mixed": 64 + 2 * 36 + 120 + 120 + 8,
 }
+# two-material fluid with one mixture material: 9 primitives (lambda), 10
+# flux comps (alpha rho lambda), 19 persistent comps written by the update
+MIX_KERNEL_BYTES = {
+    "flux_x": 72 + 80 + 16,
+    "flux_y": 72 + 80 + 16,
+    "update_stage1": 72 + 2 * 80 + 152,
+    "update_stage2_post": 72 + 2 * 80 + 152 + 152 + 8,
+    "flux_x_mixed": 72 + 40 + 16,
+    "flux_y_mixed": 72 + 40 + 16,
+    "update_stage1_mixed": 72 + 2 * 40 + 152,
+    "update_stage2_post_mixed": 72 + 2 * 40 + 152 + 152 + 8,
+}
 SOLID_KERNEL_BYTES = {
     # 19 primitives in (alpha, rho_k, epsilon, u, p, V), 23 face comps out
     # (19 conserved fluxes, S*, the V* row of the face)
@@ -68,14 +82,16 @@ SOLID_KERNEL_BYTES = {
     "update_stage2_post_mixed": 152 + 3 * 92 + 232 + 232 + 8,
 }
 # 8 (3 N_s + 2 N_c): N_s/N_c = 14/8 (two-material fluid), 27/19 (two-material solid)
-STEP_BYTES = {"inclusion2d": 464, "plate3d": 952, "tube1d": 464, "ensemble": 464, "explosion_rz": 464}
+# reactive2d: N_s = 14 + 4 (alpha rho lambda, lambda, two phase densities), N_c = 9
+STEP_BYTES = {"inclusion2d": 464, "plate3d": 952, "tube1d": 464, "ensemble": 464, "explosion_rz": 464,
+              "reactive2d": 8 * (3 * 18 + 2 * 9)}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["inclusion2d", "plate3d", "tube1d", "ensemble", "explosion_rz"],
-                    default="inclusion2d")
+    ap.add_argument("--workload", choices=["inclusion2d", "plate3d", "tube1d", "ensemble", "explosion_rz",
+                                           "reactive2d"], default="inclusion2d")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 500 for inclusion2d, 40 for plate3d)")
     ap.add_argument("--warmup", type=int, default=10)
@@ -96,7 +112,7 @@ def parse():
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"inclusion2d": 500, "plate3d": 40, "tube1d": 1280, "ensemble": 200,
-                      "explosion_rz": 500}[args.workload]
+                      "explosion_rz": 500, "reactive2d": 200}[args.workload]
     return args
 
 
@@ -129,7 +145,8 @@ def cpu_baseline(pb, budget_s: float, max_steps: int = 20):
 
 
 TRAFFIC_FILES = {"inclusion2d": "pmc_traffic_shock_inclusion_2d.txt", "plate3d": "pmc_traffic_plate_impact_3d.txt",
-                 "tube1d": "pmc_traffic_shock_tube_1d.txt", "explosion_rz": "pmc_traffic_explosion_rz.txt"}
+                 "tube1d": "pmc_traffic_shock_tube_1d.txt", "explosion_rz": "pmc_traffic_explosion_rz.txt",
+                 "reactive2d": "pmc_traffic_reactive_2d.txt"}
 
 
 def read_traffic(workload: str, name: str):
@@ -297,6 +314,11 @@ def main():
         kbytes = FLUID_KERNEL_BYTES
         workload = (f"2D two-material shock/inclusion, {pb.n[0]}x{pb.n[1]} cells, {pb.max_grid_size}^2 boxes, "
                     "MUSCL+THINC(beta 1.6)+HLLC, CFL 0.5")
+    elif args.workload == "reactive2d":
+        pb = configs.reactive_tube(2, args.nx, box=args.box, thinc=1)
+        kbytes = MIX_KERNEL_BYTES
+        workload = (f"2D reactive mixture (MixtureEOS, pressure-based burn) beside air, {pb.n[0]}x{pb.n[1]} cells, "
+                    f"{pb.max_grid_size}^2 boxes, MUSCL+THINC(beta 1.6)+HLLC, CFL 0.5")
     elif args.workload == "explosion_rz":
         pb = configs.underwater_explosion(2, args.nrz, box=args.box, thinc=1)
         kbytes = FLUID_KERNEL_BYTES
@@ -445,7 +467,7 @@ def main():
     if mixed is not None:
         out["mixed"] = mixed
     if not args.no_cpu:
-        if args.workload in ("inclusion2d", "explosion_rz"):
+        if args.workload in ("inclusion2d", "explosion_rz", "reactive2d"):
             out["cpu_baseline"] = cpu_baseline(pb, args.cpu_seconds, 20)
         else:
             # bounded sample: the same problem on 128^3 cells (one eighth of the cells)

@@ -34,11 +34,15 @@ typedef struct tj_material {
     int eos;
     int solid;    /* MaterialDescriptor::phase == solid (structdefinitions.h:90-98) */
     int plastic;  /* MaterialDescriptor::plastic */
-    int pad_;
+    int mixture;  /* MaterialDescriptor::mixture: MixtureEOS (equationofstate.cpp:116-376) of the
+                     MieGruneisen reactant (gamma, pref, eref, CV) and products (*_mix below);
+                     at most one mixture material, on fluid levels */
     double gamma; /* adiabaticIndex; GruneisenGamma = gamma - 1 */
     double pref, eref, CV;
     double rho0, c0, s, G0;
     double yield_stress; /* PlasticEOS::yieldStress[m] (plastic.h:21) */
+    double gamma_mix, pref_mix, eref_mix, CV_mix; /* the products' EOS (adiabaticIndexmix, prefmix,
+                                                     erefmix, CVmix of initialise.cpp:189-192) */
 } tj_material;
 
 /* Level description: what ParameterStruct (structdefinitions.h:105-144),
@@ -58,7 +62,8 @@ typedef struct tj_level_desc {
                                 x is the radius, spherical in 1D, cylindrical otherwise.  On solid
                                 levels it needs PLASTIC (the source reads the stress of the state
                                 that plasticUpdate's conservativeToPrimitive leaves). */
-    int pad_;
+    int REACTIVE;            /* parameters.REACTIVE: reactiveUpdate each step (main.cpp:149-152) on
+                                the mixture material */
 } tj_level_desc;
 
 /* Non-owning view of a CellArray on the device: component-major, i fastest.
@@ -87,7 +92,10 @@ enum {
     TJ_ST_WAVESPEED = 2,    /* NaN wave speeds in the 5-wave solver (hllc.cpp:169-200, 367-373) */
     TJ_ST_ALPHA_NANS = 4,   /* "too many Nans" in cleanUpAlpha (boxaccesscellarray.cpp:559-561) */
     TJ_ST_ALPHA_SUM = 8,    /* total alpha <= 0 in cleanUpAlpha (boxaccesscellarray.cpp:583-588) */
-    TJ_ST_PLASTIC_DIV = 16  /* "Divide by zero error" in boxPlasticUpdate (plastic.cpp:220-224) */
+    TJ_ST_PLASTIC_DIV = 16, /* "Divide by zero error" in boxPlasticUpdate (plastic.cpp:220-224) */
+    TJ_ST_MIX_BRACKET = 32, /* "Error in root Bisection" in MixtureEOS::rootFind (equationofstate.cpp:313-321) */
+    TJ_ST_MIX_ITER = 64     /* rootFind's bisection reached its bound of 2000 passes (the reference
+                               loops without one, equationofstate.cpp:324-353) */
 };
 
 /* precision forms */
@@ -100,8 +108,10 @@ enum {
     TJ_OP_CLEAN_ALPHA = 4,  /* CellArray::cleanUpAlpha (boxaccesscellarray.cpp:530-599) */
     TJ_OP_DT = 8,           /* TimeStep::getTimeStep (timestep.cpp:36-70) */
     TJ_OP_NANCHECK = 16,    /* CellArray::contains_nan (cellarray.cpp:101-117) */
-    TJ_OP_RADIAL = 32       /* geometricSourceTerm (radial.cpp:37-54) with the dt of tj_set_dt;
+    TJ_OP_RADIAL = 32,      /* geometricSourceTerm (radial.cpp:37-54) with the dt of tj_set_dt;
                                applied after PLASTIC and before CLEAN_V */
+    TJ_OP_REACTIVE = 64     /* reactiveUpdate (reactive.cpp:60-103) with the dt of tj_set_dt;
+                               applied first, before PLASTIC */
 };
 
 /* Context on HIP device `device` with its own stream (default priority). */

@@ -44,6 +44,12 @@ class Params(ctypes.Structure):
         ("G0", ctypes.c_double * 4),
         ("yield_", ctypes.c_double * 4),
         ("RADIAL", ctypes.c_int),
+        ("REACTIVE", ctypes.c_int),
+        ("mixture", ctypes.c_int * 4),
+        ("gamma2", ctypes.c_double * 4),
+        ("pref2", ctypes.c_double * 4),
+        ("eref2", ctypes.c_double * 4),
+        ("CV2", ctypes.c_double * 4),
     ]
 
 
@@ -79,6 +85,7 @@ def lib():
         L.tjo_timestep.restype = ctypes.c_double
         L.tjo_step.argtypes = [vp, ctypes.c_double]
         L.tjo_geometric_source.argtypes = [vp, ctypes.c_double]
+        L.tjo_reactive.argtypes = [vp, ctypes.c_double]
         L.tjo_run.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp]
         L.tjo_run.restype = ctypes.c_int
         for f in ("tjo_cleanup_v", "tjo_plastic", "tjo_c2p", "tjo_p2c", "tjo_sound"):
@@ -115,6 +122,7 @@ def make_params(pb, math_mode: int = 0) -> Params:
     q.nghost = pb.nghost
     q.SOLID, q.THINC, q.PLASTIC, q.MUSCL = pb.SOLID, pb.THINC, pb.PLASTIC, pb.MUSCL
     q.RADIAL = pb.RADIAL
+    q.REACTIVE = pb.REACTIVE
     q.THINCbeta = pb.THINCbeta
     q.CFL = pb.CFL
     q.nmat = pb.nmat
@@ -123,6 +131,8 @@ def make_params(pb, math_mode: int = 0) -> Params:
         q.eos[m], q.solid[m], q.plastic[m] = M.eos, M.solid, M.plastic
         q.gamma[m], q.pref[m], q.eref[m], q.CV[m] = M.gamma, M.pref, M.eref, M.CV
         q.rho0[m], q.c0[m], q.s[m], q.G0[m], q.yield_[m] = M.rho0, M.c0, M.s, M.G0, M.yield_stress
+        q.mixture[m] = M.mixture
+        q.gamma2[m], q.pref2[m], q.eref2[m], q.CV2[m] = M.gamma2, M.pref2, M.eref2, M.CV2
     return q
 
 
@@ -197,6 +207,10 @@ class Oracle:
 
     def plastic(self):
         lib().tjo_plastic(self.h)
+
+    def reactive(self, dt: float):
+        """reactiveUpdate (reactive.cpp:60-103) on U1 with step dt."""
+        lib().tjo_reactive(self.h, dt)
 
     def geometric_source(self, dt: float):
         """geometricSourceTerm (radial.cpp:37-54) on U1 with step dt."""

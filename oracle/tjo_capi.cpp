@@ -25,6 +25,9 @@ struct tjo_params {
     int plastic[4];
     double gamma[4], pref[4], eref[4], CV[4], rho0[4], c0[4], s[4], G0[4], yield[4];
     int RADIAL;
+    int REACTIVE;
+    int mixture[4];
+    double gamma2[4], pref2[4], eref2[4], CV2[4];
 };
 
 static tjo::Params to_params(const tjo_params* q) {
@@ -42,6 +45,7 @@ static tjo::Params to_params(const tjo_params* q) {
     p.THINC = q->THINC;
     p.PLASTIC = q->PLASTIC;
     p.RADIAL = q->RADIAL;
+    p.REACTIVE = q->REACTIVE;
     p.MUSCL = q->MUSCL;
     p.THINCbeta = q->THINCbeta;
     p.CFL = q->CFL;
@@ -61,6 +65,12 @@ static tjo::Params to_params(const tjo_params* q) {
         M.s = q->s[m];
         M.G0 = q->G0[m];
         M.yield = q->yield[m];
+        M.mixture = q->mixture[m];
+        M.adiabaticIndex2 = q->gamma2[m];
+        M.pref2 = q->pref2[m];
+        M.eref2 = q->eref2[m];
+        M.CV2 = q->CV2[m];
+        if (m < p.nmat && M.mixture) p.nmix = 1;
     }
     return p;
 }
@@ -152,6 +162,7 @@ int tjo_status(void* h) {
     if (s->contains_nan(s->U1)) st |= 1;
     if (s->fatal) st |= 2;
     st |= (s->alpha_fatal & 7) << 2;
+    st |= (s->mix_fatal & 3) << 5;
     return st;
 }
 // Stand-alone cleanUpAlpha on U1 (boxaccesscellarray.cpp:530-599)
@@ -164,6 +175,7 @@ int tjo_cleanup_alpha(void* h) {
 void tjo_cleanup_v(void* h) { static_cast<tjo::Solver*>(h)->cleanup_v(); }
 void tjo_plastic(void* h) { static_cast<tjo::Solver*>(h)->plastic_update(); }
 void tjo_geometric_source(void* h, double dt) { static_cast<tjo::Solver*>(h)->geometric_source(dt); }
+void tjo_reactive(void* h, double dt) { static_cast<tjo::Solver*>(h)->reactive_update(dt); }
 void tjo_c2p(void* h) {
     auto* s = static_cast<tjo::Solver*>(h);
     for (size_t b = 0; b < s->boxes.size(); ++b)
@@ -242,6 +254,9 @@ double tjo_math(int kind, int mode, double x) {
         case 0: return m.exp(x);
         case 1: return m.cosh(x);
         case 2: return m.tanh(x);
+        case 4: return m.log(x);
+        case 5: return m.pow(x, 0.5);
+        case 6: return m.pow(x, 1.2);
         default: return m.pow_m13(x);
     }
 }

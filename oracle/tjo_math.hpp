@@ -19,6 +19,11 @@
 //  rcbrt : x^(-1/3) by bit-level initial guess plus Newton steps on
 //          y <- y (4 - x y^3)/3, finished with one step of the cube-root
 //          Newton iteration on 1/y.
+//  log   : x = 2^e m with m in [sqrt(2)/2, sqrt(2)), s = (m - 1)/(m + 1) and
+//          log m = 2 atanh(s) = 2s + s z (2/3 + 2/5 z + ... + 2/25 z^11),
+//          z = s^2 (Horner), then e ln2_hi + (log m + e ln2_lo).
+//  pow   : exp(y log x) (std::pow in reactive.cpp:17): x < 0 gives NaN
+//          through the log, x = 0 gives 0 for y > 0.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -103,6 +108,54 @@ inline double pm_rcbrt(double x) {
     return 1.0 / c;
 }
 
+inline double pm_log(double x) {
+    if (x != x || x < 0.0) return std::nan("");
+    if (x == 0.0) return -HUGE_VAL;
+    if (x == HUGE_VAL) return x;
+    uint64_t bits;
+    std::memcpy(&bits, &x, 8);
+    int e = 0;
+    if ((bits >> 52) == 0) {  // subnormal: scale by 2^54
+        x *= 18014398509481984.0;
+        std::memcpy(&bits, &x, 8);
+        e = -54;
+    }
+    e += (int)((bits >> 52) & 0x7ff) - 1023;
+    bits = (bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+    double m;
+    std::memcpy(&m, &bits, 8);
+    if (m > 1.4142135623730951) {
+        m *= 0.5;
+        e += 1;
+    }
+    const double ln2hi = 6.93147180369123816490e-01;
+    const double ln2lo = 1.90821492927058770002e-10;
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double q = 2.0 / 25.0;
+    q = q * z + 2.0 / 23.0;
+    q = q * z + 2.0 / 21.0;
+    q = q * z + 2.0 / 19.0;
+    q = q * z + 2.0 / 17.0;
+    q = q * z + 2.0 / 15.0;
+    q = q * z + 2.0 / 13.0;
+    q = q * z + 2.0 / 11.0;
+    q = q * z + 2.0 / 9.0;
+    q = q * z + 2.0 / 7.0;
+    q = q * z + 2.0 / 5.0;
+    q = q * z + 2.0 / 3.0;
+    const double lm = 2.0 * s + s * (z * q);
+    const double ed = (double)e;
+    return ed * ln2hi + (lm + ed * ln2lo);
+}
+
+inline double pm_pow(double x, double y) {
+    if (y == 0.0) return 1.0;
+    if (x == 0.0) return y > 0.0 ? 0.0 : HUGE_VAL;
+    return pm_exp(y * pm_log(x));
+}
+
 // math_mode 0: project functions (bit-identical with the device);
 // math_mode 1: libm, as the reference calls it.
 struct Math {
@@ -111,6 +164,8 @@ struct Math {
     double cosh(double x) const { return mode ? std::cosh(x) : pm_cosh(x); }
     double tanh(double x) const { return mode ? std::tanh(x) : pm_tanh(x); }
     double pow_m13(double x) const { return mode ? std::pow(x, -1.0 / 3.0) : pm_rcbrt(x); }
+    double log(double x) const { return mode ? std::log(x) : pm_log(x); }
+    double pow(double x, double y) const { return mode ? std::pow(x, y) : pm_pow(x, y); }
 };
 
 }  // namespace tjo

@@ -60,7 +60,12 @@ struct Material {
     double adiabaticIndex = 1.4;  // gamma; GruneisenGamma = gamma - 1
     double pref = 0.0, eref = 0.0, CV = 1.0;
     double rho0 = 1.0, c0 = 0.0, s = 0.0, G0 = 0.0, yield = 0.0;
+    // MixtureEOS (equationofstate.cpp:116-376): `first` is the MieGruneisen
+    // EOS above (reactant a), `second` the one below (products b)
+    int mixture = 0;
+    double adiabaticIndex2 = 1.4, pref2 = 0.0, eref2 = 0.0, CV2 = 1.0;
     double GG() const { return adiabaticIndex - 1.0; }
+    double GG2() const { return adiabaticIndex2 - 1.0; }
 };
 
 struct Params {
@@ -71,6 +76,8 @@ struct Params {
     int nghost = 2;
     int SOLID = 0, THINC = 0, PLASTIC = 0, MUSCL = 1;
     int RADIAL = 0;  // geometric source of radial.cpp (structdefinitions.h:113)
+    int REACTIVE = 0;  // reactiveUpdate of reactive.cpp (structdefinitions.h:115)
+    int nmix = 0;      // parameters.numberOfMixtures (0 or 1: every mixture uses mixtureIndex 0)
     double THINCbeta = 1.6;
     double CFL = 0.5;
     int nmat = 2;
@@ -80,7 +87,7 @@ struct Params {
 };
 
 // ---------------------------------------------------------------- layout
-// accesspattern.cpp:44-79 (mixtures not instantiated: numberOfMixtures = 0)
+// accesspattern.cpp:44-79 (numberOfMixtures 0 or 1)
 struct Layout {
     int start[NVAR];
     int ncomp = 0;
@@ -115,6 +122,9 @@ struct Layout {
         add(n, "a", 3, SOUNDSPEED, 1, 1, 1);
         add(n, "uStar", 3, USTAR, 1, 1, 1);
         add(n, "sigma", 3, SIGMA, 1, 3, 3);
+        add(n, "rhomix", 3, RHO_MIX, p.nmix, 2, 1);
+        add(n, "lambda", 0, LAMBDA, p.nmix, 1, 1);
+        add(n, "alpharholambda", 1, ALPHARHOLAMBDA, p.nmix, 1, 1);
         if (p.SOLID) {
             add(n, "V", 2, V_TENSOR, 1, 3, 3);
             add(n, "VStar", 3, VSTAR, 1, 3, 3);
@@ -136,7 +146,9 @@ struct Layout {
                 return start[v] + r;
             case SIGMA: case V_TENSOR: case VSTAR: case DEVH:
                 return start[v] + r * 3 + c;
-            default:
+            case RHO_MIX:  // + mixtureIndex (0 for every mixture, initialise.cpp:187) + row
+                return start[v] + r;
+            default:  // LAMBDA, ALPHARHOLAMBDA: + mixtureIndex (0)
                 return start[v];
         }
     }
@@ -386,12 +398,29 @@ class Solver {
     double coldE(Fab& f, int i, int j, int k, int m) {
         const Material& M = p.mat[m];
         if (M.eos == 1) return A(f, i, j, k, ALPHARHO, m) * eH(M, A(f, i, j, k, RHO_K, m));
+        if (M.mixture) {  // equationofstate.cpp:158-162
+            const double lam = A(f, i, j, k, LAMBDA, m);
+            return lam * (M.eref * A(f, i, j, k, ALPHARHO, m)) + (1.0 - lam) * (M.eref2 * A(f, i, j, k, ALPHARHO, m));
+        }
         return M.eref * A(f, i, j, k, ALPHARHO, m);
     }
     double coldP(Fab& f, int i, int j, int k, int m) {
         const Material& M = p.mat[m];
         if (M.eos == 1) return (A(f, i, j, k, ALPHA, m) / M.GG()) * pH(M, A(f, i, j, k, RHO_K, m));
+        if (M.mixture) {  // equationofstate.cpp:164-168
+            const double lam = A(f, i, j, k, LAMBDA, m), rk = A(f, i, j, k, RHO_K, m);
+            const double pa = M.pref * A(f, i, j, k, ALPHA, m) / M.GG();
+            const double pb = M.pref2 * A(f, i, j, k, ALPHA, m) / M.GG2();
+            return (pa)*lam * rk / (A(f, i, j, k, RHO_MIX, m, 0)) + (pb) * (1.0 - lam) * rk / (A(f, i, j, k, RHO_MIX, m, 1));
+        }
         return M.pref * A(f, i, j, k, ALPHA, m) / M.GG();
+    }
+    // equationofstate.cpp:170-175, 233-236: the mixture's Gruneisen coefficient
+    // from the CV-weighted adiabatic index
+    double mixGamma(Fab& f, int i, int j, int k, int m) {
+        const Material& M = p.mat[m];
+        const double lam = A(f, i, j, k, LAMBDA, m);
+        return (lam * M.adiabaticIndex * M.CV + (1.0 - lam) * M.adiabaticIndex2 * M.CV2) / (lam * M.CV + (1.0 - lam) * M.CV2) - 1.0;
     }
     double shearG(int m) const {
         const Material& M = p.mat[m];
@@ -407,12 +436,25 @@ class Solver {
             return (A(f, i, j, k, ALPHA, m) / M.GG()) * ((A(f, i, j, k, RHO_K, m) * 0.0 - shearG(m)) * A(f, i, j, k, HJ2));
         return 0.0 * A(f, i, j, k, ALPHA, m);
     }
-    double invGr(Fab& f, int i, int j, int k, int m) { return A(f, i, j, k, ALPHA, m) / p.mat[m].GG(); }
-    double xi(int m) const { return 1.0 / p.mat[m].GG(); }
+    double invGr(Fab& f, int i, int j, int k, int m) {
+        if (p.mat[m].mixture) return A(f, i, j, k, ALPHA, m) / (mixGamma(f, i, j, k, m));
+        return A(f, i, j, k, ALPHA, m) / p.mat[m].GG();
+    }
+    double xi(Fab& f, int i, int j, int k, int m) {
+        if (p.mat[m].mixture) return 1.0 / (mixGamma(f, i, j, k, m));
+        return 1.0 / p.mat[m].GG();
+    }
     double ssContrib(Fab& f, int i, int j, int k, int m) {
         const Material& M = p.mat[m];
         double rk = A(f, i, j, k, RHO_K, m);
         double pp = A(f, i, j, k, P);
+        if (M.mixture) {  // mixtureSoundSpeed, equationofstate.cpp:212-226 (constant pref terms)
+            const double lam = A(f, i, j, k, LAMBDA, m);
+            double dedrho = (lam) * ((M.pref - pp) / (M.GG() * rk * rk));
+            dedrho += (1.0 - lam) * ((M.pref2 - pp) / (M.GG2() * rk * rk));
+            return (pp / (rk * rk) - dedrho) /
+                   (lam / (A(f, i, j, k, RHO_MIX, m, 0) * M.GG()) + (1.0 - lam) / (A(f, i, j, k, RHO_MIX, m, 1) * M.GG2()));
+        }
         if (M.eos == 1) {
             double gp1 = M.GG() + 1.0;
             double hj2 = A(f, i, j, k, HJ2);
@@ -477,6 +519,95 @@ class Solver {
                     A(f, i, j, k, SIGMA, 0, r, c) = -A(f, i, j, k, P) * delta<double>(r, c);
             }
     }
+    // ------------------------------------------------------------ mixture EOS
+    // equationofstate.cpp:239-247 rhobFunc: the products' density from the
+    // reactants' at the cell's mixture density and mass fraction
+    double rhob_of(Fab& f, int i, int j, int k, int m, double rhoa) {
+        return (1.0 - A(f, i, j, k, LAMBDA, m)) / (1.0 / A(f, i, j, k, RHO_K, m) - A(f, i, j, k, LAMBDA, m) / rhoa);
+    }
+    // :249-262 pressureFunction: difference of the two phases' temperatures
+    double pressure_fn(Fab& f, int i, int j, int k, int m, double pp) {
+        const Material& M = p.mat[m];
+        return ((pp - M.pref) / (A(f, i, j, k, RHO_MIX, m, 0) * M.GG() * M.CV)) -
+               ((pp - M.pref2) / (A(f, i, j, k, RHO_MIX, m, 1) * M.GG2() * M.CV2));
+    }
+    // :264-273 bisectionFunction (stores the trial densities in RHO_MIX)
+    double bisection_fn(Fab& f, int i, int j, int k, int m, double rhoTry, double ke) {
+        A(f, i, j, k, RHO_MIX, m, 0) = rhoTry;
+        A(f, i, j, k, RHO_MIX, m, 1) = rhob_of(f, i, j, k, m, rhoTry);
+        const double pp = (A(f, i, j, k, TOTAL_E) - ke - effNTE(f, i, j, k) + effNTP(f, i, j, k)) / (effInvG(f, i, j, k));
+        return pressure_fn(f, i, j, k, m, pp);
+    }
+    static int sgn(double v) { return (0.0 < v) - (v < 0.0); }
+    // Bisection bound: the reference loops until convergence with no bound
+    // (equationofstate.cpp:324-353); the interval halves each pass, so a
+    // finite bracket converges in under 100 passes.
+    static constexpr int kMixMaxIter = 2000;
+    std::atomic<int> mix_fatal{0};  // 1: no sign change ("Error in root Bisection", exit); 2: bound hit
+    // :275-354 rootFind: thermal equilibrium of the two phases by bisection on
+    // the reactants' density
+    void root_find(Fab& f, int i, int j, int k, int m, double ke) {
+        const Material& M = p.mat[m];
+        const double toleranceForSinglePhaseTreatment = 1E-3;
+        const double toleranceForConvergence = 1E-7;
+        const double toleranceForBeingNearRoot = 1E-4;
+        double& ra = A(f, i, j, k, RHO_MIX, m, 0);
+        double& rb = A(f, i, j, k, RHO_MIX, m, 1);
+        const double lam = A(f, i, j, k, LAMBDA, m), rk = A(f, i, j, k, RHO_K, m);
+        if (M.pref == 0.0 && M.pref2 == 0.0) {
+            ra = rk * (lam + (1.0 - lam) * (M.GG2() * M.CV2) / (M.GG() * M.CV));
+            rb = rhob_of(f, i, j, k, m, ra);
+            return;
+        }
+        if (lam > 1.0 - toleranceForSinglePhaseTreatment) {
+            ra = rk;
+            rb = rk;
+            return;
+        } else if (lam < toleranceForSinglePhaseTreatment) {
+            rb = rk;
+            ra = rk;
+            return;
+        }
+        double a = lam * rk + 1E-10;
+        double b = 1000000.0;
+        double mid = 0.5 * (a + b);
+        if (std::fabs(bisection_fn(f, i, j, k, m, a, ke)) <= toleranceForBeingNearRoot) {
+            ra = a;
+            rb = rhob_of(f, i, j, k, m, ra);
+            return;
+        } else if (std::fabs(bisection_fn(f, i, j, k, m, b, ke)) <= toleranceForBeingNearRoot) {
+            ra = b;
+            rb = rhob_of(f, i, j, k, m, ra);
+            return;
+        }
+        const int sa = sgn(bisection_fn(f, i, j, k, m, a, ke));
+        const int sb = sgn(bisection_fn(f, i, j, k, m, b, ke));
+        if (sa == sb) {  // fatal in the reference; the trial densities of b stay
+            mix_fatal |= 1;
+            return;
+        }
+        for (int n = 0;; n++) {
+            const int smid = sgn(bisection_fn(f, i, j, k, m, mid, ke));
+            if (smid == sgn(bisection_fn(f, i, j, k, m, a, ke))) a = mid;
+            else b = mid;
+            mid = 0.5 * (a + b);
+            if (std::fabs(a - b) / (a + b) < toleranceForConvergence || n >= kMixMaxIter) {
+                if (n >= kMixMaxIter) mix_fatal |= 2;
+                ra = mid;
+                rb = rhob_of(f, i, j, k, m, ra);
+                return;
+            }
+        }
+    }
+    // :364-370 defineMixtureDensities: the phase densities of a primitive state
+    void define_mix(Fab& f, int i, int j, int k, int m) {
+        const Material& M = p.mat[m];
+        const double lam = A(f, i, j, k, LAMBDA, m), pp = A(f, i, j, k, P);
+        A(f, i, j, k, RHO_MIX, m, 0) =
+            A(f, i, j, k, RHO_K, m) * (lam + (1.0 - lam) * ((pp - M.pref) / (pp - M.pref2)) * ((M.CV2 * M.GG2()) / (M.CV * M.GG())));
+        A(f, i, j, k, RHO_MIX, m, 1) = rhob_of(f, i, j, k, m, A(f, i, j, k, RHO_MIX, m, 0));
+    }
+
     // boxaccesscellarray.cpp:67-122
     void c2p_cell(Fab& f, int i, int j, int k) {
         A(f, i, j, k, RHO) = 0;
@@ -492,6 +623,11 @@ class Solver {
             A(f, i, j, k, VELOCITY, 0, r) = A(f, i, j, k, RHOU, 0, r) / A(f, i, j, k, RHO);
             ke += 0.5 * A(f, i, j, k, RHO) * A(f, i, j, k, VELOCITY, 0, r) * A(f, i, j, k, VELOCITY, 0, r);
         }
+        for (int m = 0; m < p.nmat; ++m)
+            if (p.mat[m].mixture) {
+                A(f, i, j, k, LAMBDA, m) = A(f, i, j, k, ALPHARHOLAMBDA, m) / A(f, i, j, k, ALPHARHO, m);
+                root_find(f, i, j, k, m, ke);
+            }
         A(f, i, j, k, P) = (A(f, i, j, k, TOTAL_E) - ke - effNTE(f, i, j, k) + effNTP(f, i, j, k)) / effInvG(f, i, j, k);
         stress(f, i, j, k);
     }
@@ -501,6 +637,10 @@ class Solver {
         for (int m = 0; m < p.nmat; ++m) {
             A(f, i, j, k, ALPHARHO, m) = A(f, i, j, k, ALPHA, m) * A(f, i, j, k, RHO_K, m);
             A(f, i, j, k, RHO) += A(f, i, j, k, ALPHARHO, m);
+            if (p.mat[m].mixture) {
+                A(f, i, j, k, ALPHARHOLAMBDA, m) = A(f, i, j, k, LAMBDA, m) * A(f, i, j, k, ALPHA, m) * A(f, i, j, k, RHO_K, m);
+                define_mix(f, i, j, k, m);
+            }
             if (p.mat[m].plastic)
                 A(f, i, j, k, ALPHARHOEPSILON, m) = A(f, i, j, k, EPSILON, m) * A(f, i, j, k, ALPHARHO, m);
         }
@@ -517,8 +657,8 @@ class Solver {
     void sound_cell(Fab& f, int i, int j, int k) {
         double a = 0.0, xt = 0.0;
         for (int m = 0; m < p.nmat; ++m) {
-            a += dmax(0.0, xi(m) * ssContrib(f, i, j, k, m) * A(f, i, j, k, ALPHARHO, m) / A(f, i, j, k, RHO));
-            xt += xi(m) * A(f, i, j, k, ALPHA, m);
+            a += dmax(0.0, xi(f, i, j, k, m) * ssContrib(f, i, j, k, m) * A(f, i, j, k, ALPHARHO, m) / A(f, i, j, k, RHO));
+            xt += xi(f, i, j, k, m) * A(f, i, j, k, ALPHA, m);
         }
         if (a <= 0.0) a = 1E-6;
         A(f, i, j, k, SOUNDSPEED) = std::sqrt(a / xt);
@@ -754,6 +894,7 @@ class Solver {
             case ALPHA: return c(ALPHA, n.mat) * c(VELOCITY, 0, d);
             case ALPHARHO: return c(ALPHARHO, n.mat) * c(VELOCITY, 0, d);
             case ALPHARHOEPSILON: return c(ALPHARHOEPSILON, n.mat) * c(VELOCITY, 0, d);
+            case ALPHARHOLAMBDA: return c(ALPHARHOLAMBDA, n.mat) * c(VELOCITY, 0, d);
             case RHOU: return c(RHOU, 0, n.row) * c(VELOCITY, 0, d) - c(SIGMA, 0, n.row, d);
             case TOTAL_E: return c(TOTAL_E) * c(VELOCITY, 0, d) - vdots(c, d);
             case V_TENSOR:
@@ -775,6 +916,7 @@ class Solver {
             us(ALPHA, m) = u(ALPHA, m);
             us(ALPHARHO, m) = mult * u(ALPHARHO, m);
             us(RHO_K, m) = us(ALPHARHO, m) / us(ALPHA, m);
+            if (p.mat[m].mixture) us(ALPHARHOLAMBDA, m) = mult * u(ALPHARHOLAMBDA, m);
             if (p.mat[m].plastic) us(ALPHARHOEPSILON, m) = mult * u(ALPHARHOEPSILON, m);
             us(RHO) += us(ALPHARHO, m);
         }
@@ -794,7 +936,8 @@ class Solver {
     void copy_cell(CellV& dst, CellV& src) {
         for (int n = 0; n < L.ncomp; ++n) {
             bool cellvar = !(n >= L.start[DEVH] && L.start[DEVH] >= 0 && n < L.start[DEVH] + 9) &&
-                           !(L.start[HJ2] >= 0 && n == L.start[HJ2]);
+                           !(L.start[HJ2] >= 0 && n == L.start[HJ2]) &&
+                           !(L.start[RHO_MIX] >= 0 && n >= L.start[RHO_MIX] && n < L.start[RHO_MIX] + 2);
             if (cellvar) (*dst.f)(dst.i, dst.j, dst.k, n) = (*src.f)(src.i, src.j, src.k, n);
         }
     }
@@ -1194,12 +1337,47 @@ class Solver {
         }
     }
 
+    // reactive.cpp:3-24 pressureBased_UpdateMassFraction: one explicit step of
+    // the pressure-dependent burn rate from state u into o (cell, material m)
+    void burn(Fab& fu, Fab& fo, int i, int j, int k, int m, double dtv) {
+        const double sigma = 20.0, nu = 0.5, n = 1.2;
+        if (A(fu, i, j, k, ALPHA, m) < 0.5) {
+            A(fo, i, j, k, ALPHARHOLAMBDA, m) = A(fu, i, j, k, ALPHARHOLAMBDA, m);
+            return;
+        }
+        A(fo, i, j, k, ALPHARHOLAMBDA, m) =
+            A(fu, i, j, k, ALPHARHOLAMBDA, m) - (dtv) * (sigma * mth.pow(A(fu, i, j, k, ALPHARHO, m), 1.0 - nu) *
+                                                         mth.pow(A(fu, i, j, k, ALPHARHOLAMBDA, m), nu) *
+                                                         mth.pow((A(fu, i, j, k, P) / 1E6), n));
+        if (A(fo, i, j, k, ALPHARHOLAMBDA, m) < 0.0) A(fo, i, j, k, ALPHARHOLAMBDA, m) = 0.0;
+    }
+    // reactive.cpp:60-103 reactiveUpdate: Heun's RK2 of the burn on the mixture
+    // materials of U1, then conservativeToPrimitive box by box
+    void reactive_update(double dtv) {
+        if (p.nmix <= 0) return;
+        copy(U, U1);
+#pragma omp parallel for schedule(dynamic)
+        for (size_t b = 0; b < boxes.size(); ++b) {
+            Fab &fu = U.f[b], &f1 = U1.f[b], &f2 = U2.f[b];
+            for (int m = 0; m < p.nmat; ++m)
+                if (p.mat[m].mixture)
+                    for_valid((int)b, [&](int i, int j, int k) {
+                        burn(fu, f1, i, j, k, m, dtv);
+                        burn(f1, f2, i, j, k, m, dtv);
+                        A(f1, i, j, k, ALPHARHOLAMBDA, m) =
+                            ((A(fu, i, j, k, ALPHARHOLAMBDA, m) * (1.0 / 2.0)) + (A(f2, i, j, k, ALPHARHOLAMBDA, m) * (1.0 / 2.0)));
+                    });
+            for_valid((int)b, [&](int i, int j, int k) { c2p_cell(f1, i, j, k); });
+        }
+    }
+
     // main.cpp:124-171: one time step starting from U1 (ghosts filled here)
     // with a caller-chosen dt (main.cpp:128-143 chooses it).
     void step(double dtv) {
         fill(U1);
         copy(U, U1);
         advance(dtv);
+        if (p.REACTIVE) reactive_update(dtv);
         if (p.PLASTIC) plastic_update();
         if (p.RADIAL) geometric_source(dtv);
         if (p.SOLID) cleanup_v();
@@ -1215,7 +1393,7 @@ class Solver {
             if (t + dt > finalT) dt = finalT - t;
             if (n < 5) dt *= 0.2;
             step(dt);
-            if (contains_nan(U1) || fatal || alpha_fatal) break;  // main.cpp:183-187
+            if (contains_nan(U1) || fatal || alpha_fatal || mix_fatal) break;  // main.cpp:183-187
         }
         if (t_out) *t_out = t;
         return n;

@@ -21,7 +21,8 @@ LIB_PATH = os.path.join(HERE, "libtenjin.so")
 TJ_PREC_FP64 = 0
 TJ_PREC_MIXED = 1
 TJ_ST_NAN, TJ_ST_WAVESPEED, TJ_ST_ALPHA_NANS, TJ_ST_ALPHA_SUM, TJ_ST_PLASTIC_DIV = 1, 2, 4, 8, 16
-TJ_OP_PLASTIC, TJ_OP_CLEAN_V, TJ_OP_CLEAN_ALPHA, TJ_OP_DT, TJ_OP_NANCHECK, TJ_OP_RADIAL = 1, 2, 4, 8, 16, 32
+TJ_OP_PLASTIC, TJ_OP_CLEAN_V, TJ_OP_CLEAN_ALPHA, TJ_OP_DT, TJ_OP_NANCHECK, TJ_OP_RADIAL, TJ_OP_REACTIVE = 1, 2, 4, 8, 16, 32, 64
+TJ_ST_MIX_BRACKET, TJ_ST_MIX_ITER = 32, 64
 
 
 class Material(ctypes.Structure):
@@ -29,7 +30,7 @@ class Material(ctypes.Structure):
         ("eos", ctypes.c_int),
         ("solid", ctypes.c_int),
         ("plastic", ctypes.c_int),
-        ("pad_", ctypes.c_int),
+        ("mixture", ctypes.c_int),
         ("gamma", ctypes.c_double),
         ("pref", ctypes.c_double),
         ("eref", ctypes.c_double),
@@ -39,6 +40,10 @@ class Material(ctypes.Structure):
         ("s", ctypes.c_double),
         ("G0", ctypes.c_double),
         ("yield_stress", ctypes.c_double),
+        ("gamma_mix", ctypes.c_double),
+        ("pref_mix", ctypes.c_double),
+        ("eref_mix", ctypes.c_double),
+        ("CV_mix", ctypes.c_double),
     ]
 
 
@@ -59,7 +64,7 @@ class LevelDesc(ctypes.Structure):
         ("bc_hi", ctypes.c_int * 3),
         ("mat", Material * 4),
         ("RADIAL", ctypes.c_int),
-        ("pad_", ctypes.c_int),
+        ("REACTIVE", ctypes.c_int),
     ]
 
 
@@ -163,6 +168,7 @@ def level_desc(pb) -> LevelDesc:
     d.max_grid_size = pb.max_grid_size
     d.SOLID, d.THINC, d.PLASTIC, d.MUSCL = pb.SOLID, pb.THINC, pb.PLASTIC, pb.MUSCL
     d.RADIAL = pb.RADIAL
+    d.REACTIVE = pb.REACTIVE
     d.THINCbeta = pb.THINCbeta
     d.CFL = pb.CFL
     d.nmat = pb.nmat
@@ -171,6 +177,8 @@ def level_desc(pb) -> LevelDesc:
         t.eos, t.solid, t.plastic = M.eos, M.solid, M.plastic
         t.gamma, t.pref, t.eref, t.CV = M.gamma, M.pref, M.eref, M.CV
         t.rho0, t.c0, t.s, t.G0, t.yield_stress = M.rho0, M.c0, M.s, M.G0, M.yield_stress
+        t.mixture = M.mixture
+        t.gamma_mix, t.pref_mix, t.eref_mix, t.CV_mix = M.gamma2, M.pref2, M.eref2, M.CV2
     return d
 
 
@@ -321,6 +329,10 @@ class GpuLevel:
 
     def derive(self):
         self._chk(self.lib.tj_derive(self.ctx, ctypes.byref(self.vS)), "tj_derive")
+
+    def c2p(self):
+        """conservativeToPrimitive of S (tj_c2p)."""
+        self._chk(self.lib.tj_c2p(self.ctx, ctypes.byref(self.vS)), "tj_c2p")
 
 
 class GpuBatch:

@@ -32,9 +32,9 @@ ABSENT_ALPHA = 1e-6
 
 
 class Layout:
-    """Component map of accesspattern.cpp:44-79 (no mixtures)."""
+    """Component map of accesspattern.cpp:44-79 (numberOfMixtures 0 or 1)."""
 
-    def __init__(self, nmat: int, solid: bool, plastic: bool):
+    def __init__(self, nmat: int, solid: bool, plastic: bool, nmix: int = 0):
         self.start = {}
         self.names: List[str] = []
         self.cons: List[tuple] = []
@@ -65,6 +65,11 @@ class Layout:
         add("a", "none", SOUNDSPEED, 1, 1, 1)
         add("uStar", "none", USTAR, 1, 1, 1)
         add("sigma", "none", SIGMA, 1, 3, 3)
+        if nmix:
+            add("rhomix", "none", RHO_MIX, nmix, 2, 1)
+            add("lambda", "prim", LAMBDA, nmix, 1, 1)
+            add("alpharholambda", "cons", ALPHARHOLAMBDA, nmix, 1, 1)
+        self.nmix = nmix
         if solid:
             add("V", "both", V_TENSOR, 1, 3, 3)
             add("VStar", "none", VSTAR, 1, 3, 3)
@@ -82,6 +87,8 @@ class Layout:
             return self.start[var] + r
         if var in (SIGMA, V_TENSOR, VSTAR, DEVH):
             return self.start[var] + 3 * r + c
+        if var == RHO_MIX:  # every mixture uses mixtureIndex 0 (initialise.cpp:187)
+            return self.start[var] + r
         return self.start[var]
 
     def cons_comps(self) -> List[int]:
@@ -91,9 +98,12 @@ class Layout:
         return [self.comp(*s) for s in self.prim]
 
     def state_comps(self) -> List[int]:
-        """Persistent per-cell state: conserved and primitive fields plus rho."""
-        out = sorted(set(self.cons_comps()) | set(self.prim_comps()) | {self.start[RHO]})
-        return out
+        """Persistent per-cell state: conserved and primitive fields plus rho
+        (and the phase densities of a mixture, which the next sound speed reads)."""
+        extra = {self.start[RHO]}
+        if self.nmix:
+            extra |= {self.start[RHO_MIX], self.start[RHO_MIX] + 1}
+        return sorted(set(self.cons_comps()) | set(self.prim_comps()) | extra)
 
 
 @dataclasses.dataclass
@@ -112,10 +122,23 @@ class Material:
     s: float = 0.0
     G0: float = 0.0
     yield_stress: float = 0.0
+    # MixtureEOS (equationofstate.cpp:116-376): the fields above are the
+    # reactants' MieGruneisen EOS ("first"), these the products' ("second")
+    mixture: int = 0
+    gamma2: float = 1.4
+    pref2: float = 0.0
+    eref2: float = 0.0
+    CV2: float = 1.0
 
     @staticmethod
     def stiffened(gamma: float, p_inf: float = 0.0) -> "Material":
         return Material(gamma=gamma, pref=-gamma * p_inf)
+
+    @staticmethod
+    def mixture_mg(a: tuple, b: tuple) -> "Material":
+        """Reactant/product mixture; a, b = (adiabaticIndex, pref, eref, CV)."""
+        return Material(gamma=a[0], pref=a[1], eref=a[2], CV=a[3], mixture=1,
+                        gamma2=b[0], pref2=b[1], eref2=b[2], CV2=b[3])
 
     @staticmethod
     def usup_solid(rho0, Gamma0, c0, s, G, Y, plastic=True) -> "Material":
@@ -136,6 +159,7 @@ class Problem:
     PLASTIC: int = 0
     MUSCL: int = 1
     RADIAL: int = 0  # geometric (cylindrical / spherical) source, radial.cpp
+    REACTIVE: int = 0  # pressure-based burn of the mixture materials, reactive.cpp
     THINCbeta: float = 1.6
     CFL: float = 0.5
     bc_lo: tuple = (0, 0, 0)
@@ -152,7 +176,11 @@ class Problem:
 
     @property
     def layout(self) -> Layout:
-        return Layout(self.nmat, bool(self.SOLID), bool(self.PLASTIC))
+        return Layout(self.nmat, bool(self.SOLID), bool(self.PLASTIC), self.nmix)
+
+    @property
+    def nmix(self) -> int:
+        return 1 if any(m.mixture for m in self.materials) else 0
 
     @property
     def shape(self):
@@ -392,6 +420,44 @@ def rod_impact_axisymmetric(n: int = 96, box: int = 32, speed: float = 300.0, ra
     U[L.comp(P)] = 0.0
     for q in range(3):
         U[L.comp(V_TENSOR, 0, q, q)] = 1.0
+    pb.prims = U
+    return pb
+
+
+EXPLOSIVE_REACTANT = (4.4, -4.4 * 6e8, 0.0, 2000.0)  # adiabaticIndex, pref, eref, CV
+EXPLOSIVE_PRODUCTS = (2.0, -2.0 * 2e8, 0.0, 1000.0)
+
+
+def reactive_tube(dim: int = 1, n: int = 800, box: int = 1 << 30, thinc: int = 0, seed: int = 7) -> Problem:
+    """A reactive mixture (stiffened-gas reactant and products,
+    MixtureEOS) next to air, with the pressure-based burn of reactive.cpp:3-24
+    (REACTIVE).  x in [0, 1]: an ignition region x < 0.1 of mostly burnt,
+    high-pressure mixture (lambda 0.2, p 2e9), unburnt explosive (lambda
+    seeded in [0.5, 1)) up to x = 0.6, air beyond.  In 2D the interface with
+    the air is a seeded wavy line across y in [0, 1/2] (reflective in y).
+    Cell centres decide the region; all x boundaries transmissive."""
+    mats = [Material.mixture_mg(EXPLOSIVE_REACTANT, EXPLOSIVE_PRODUCTS), Material.stiffened(1.4, 0.0)]
+    nn = (n, n // 2 if dim > 1 else 1, 1)
+    pb = Problem(f"reactive_tube_{dim}d", dim, nn, (1.0, 0.5, 1.0), box, mats, THINC=thinc, REACTIVE=1,
+                 bc_lo=(0, 1, 0), bc_hi=(0, 1, 0), final_time=1e-4)
+    L = pb.layout
+    U = _empty_prims(pb)
+    nz, ny, nx = pb.shape
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) * (0.5 / ny)
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    uu = _mt_uniform(seed, 8 + pb.ncells)
+    edge = 0.6 + (0.02 * np.sin(2.0 * math.pi * (4.0 * Y + uu[0])) if dim > 1 else 0.0)
+    explosive = (X < edge)[None]
+    ignition = (X < 0.1)[None]
+    lam = np.where(ignition, 0.2, 0.5 + 0.5 * uu[8:].reshape(pb.shape))
+    zero = np.zeros((1, ny, nx))
+    _set_two_phase(pb, U, explosive, 1000.0, 1.2, [zero, zero, zero],
+                   np.where(ignition, 2e9, 1e5))
+    U[L.comp(LAMBDA)] = lam
+    # initialise.cpp:66-71: both phase densities start at the mixture's
+    U[L.comp(RHO_MIX, 0, 0)] = U[L.comp(RHO_K, 0)]
+    U[L.comp(RHO_MIX, 0, 1)] = U[L.comp(RHO_K, 0)]
     pb.prims = U
     return pb
 

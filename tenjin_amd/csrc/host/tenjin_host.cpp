@@ -257,6 +257,11 @@ AccessPattern::AccessPattern(ParameterStruct& parameters) {
     add(SOUNDSPEED, "a", 1);
     add(USTAR, "uStar", 1);
     add(SIGMA, "sigma", 1, 3, 3);
+    if (parameters.numberOfMixtures > 0) {
+        add(RHO_MIX, "rhomix", parameters.numberOfMixtures, 2);
+        add(LAMBDA, "lambda", parameters.numberOfMixtures);
+        add(ALPHARHOLAMBDA, "alpharholambda", parameters.numberOfMixtures);
+    }
     if (parameters.SOLID) {
         add(V_TENSOR, "V", 1, 3, 3);
         add(VSTAR, "VStar", 1, 3, 3);
@@ -368,6 +373,9 @@ Real& BoxAccessCellArray::operator()(int i, int j, int k, Variable var, int mat,
     int comp = accessPattern[var];
     switch (var) {
         case ALPHA: case ALPHARHO: case RHO_K: case EPSILON: case ALPHARHOEPSILON: comp += mat; break;
+        // + mixtureIndex, which is 0 for every mixture (initialise.cpp:187)
+        case RHO_MIX: comp += row; break;
+        case LAMBDA: case ALPHARHOLAMBDA: break;
         case RHOU: case VELOCITY: comp += row; break;
         case SIGMA: case V_TENSOR: case VSTAR: case DEVH: comp += row * 3 + col; break;
         case RHO: case TOTAL_E: case P: case SOUNDSPEED: case USTAR: case HJ2: break;
@@ -400,6 +408,17 @@ void PlasticEOS::plasticUpdate(CellArray& U, ParameterStruct& parameters) {
             throw std::runtime_error("plasticUpdate: PlasticEOS::yieldStress differs from the bound materials");
     tj_mf v = U.view();
     tj_check(parameters, tj_post_step(d.ctx, &v, TJ_OP_PLASTIC), "plasticUpdate");
+}
+
+// reactive.cpp:60-103 with parameters.dt
+void reactiveUpdate(CellArray& U, CellArray& U1, CellArray& U2, ParameterStruct& parameters) {
+    (void)U2;
+    tj_ctx* ctx = dev(parameters).ctx;
+    if (parameters.numberOfMixtures <= 0) return;
+    tj_mf u = U.view(), u1 = U1.view();
+    tj_check(parameters, tj_lincomb(ctx, &u1, 1.0, nullptr, 0.0, &u), "reactiveUpdate: U = U1");
+    tj_check(parameters, tj_set_dt(ctx, parameters.dt, 0), "reactiveUpdate: dt");
+    tj_check(parameters, tj_post_step(ctx, &u1, TJ_OP_REACTIVE), "reactiveUpdate");
 }
 
 // radial.cpp:37-54 with parameters.dt
@@ -451,8 +470,7 @@ void bindLevel(ParameterStruct& parameters, const Geometry& geom, const Vector<B
     L.PLASTIC = parameters.PLASTIC;
     L.MUSCL = parameters.MUSCL;
     L.RADIAL = parameters.RADIAL;
-    if (parameters.REACTIVE)
-        throw std::runtime_error("bindLevel: REACTIVE (reactiveUpdate, mixture EOS) is not part of this library");
+    L.REACTIVE = parameters.REACTIVE;
     L.THINCbeta = parameters.THINCbeta;
     L.CFL = parameters.CFL;
     L.nmat = parameters.numberOfMaterials;
@@ -461,6 +479,22 @@ void bindLevel(ParameterStruct& parameters, const Geometry& geom, const Vector<B
         const MaterialDescriptor& md = parameters.materialInfo.at(m);
         if (!md.EOS) throw std::runtime_error("bindLevel: material without EOS");
         tj_material& t = L.mat[m];
+        if (md.mixture) {
+            const MixtureEOS* mx = dynamic_cast<const MixtureEOS*>(md.EOS);
+            if (!mx) throw std::runtime_error("bindLevel: a mixture material needs a MixtureEOS");
+            t.eos = 0;
+            t.mixture = 1;
+            t.gamma = mx->first.adiabaticIndex;
+            t.pref = mx->first.pref;
+            t.eref = mx->first.eref;
+            t.CV = mx->first.CV;
+            t.gamma_mix = mx->second.adiabaticIndex;
+            t.pref_mix = mx->second.pref;
+            t.eref_mix = mx->second.eref;
+            t.CV_mix = mx->second.CV;
+            t.solid = md.phase == solid;
+            continue;
+        }
         t.eos = md.EOS->kind();
         t.solid = md.phase == solid;
         t.plastic = md.plastic;

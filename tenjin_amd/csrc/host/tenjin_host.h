@@ -6,6 +6,7 @@
 //   TimeStep::getTimeStep           timestep.cpp:36-70 (timestep.h:15-16)
 //   PlasticEOS::plasticUpdate       plastic.cpp:227-244 (plastic.h:18-19)
 //   geometricSourceTerm             radial.cpp:37-54 (simulationheader.h:40)
+//   reactiveUpdate                  reactive.cpp:60-103 (simulationheader.h:42)
 //   CellArray::{primitiveToConservative, conservativeToPrimitive,
 //     getSoundSpeed, cleanUpV, cleanUpAlpha, operator=, operator*,
 //     operator+, contains_nan}     cellarray.cpp:5-117 (cellarray.h:24-33)
@@ -49,15 +50,42 @@ enum Direction_enum { x, y, z };
 
 // MieGruneisenEOS (equationofstate.h:31-60): stiffened gas / constant-Gamma
 // Mie-Gruneisen with reference pressure and energy.
+// MieGruneisenEOS (equationofstate.h:33-71): the constructor's argument order
+// and defaults are the reference's (equationofstate.cpp:5-12).
 class MieGruneisenEOS {
 public:
-    MieGruneisenEOS(Real adiabaticIndex_, Real CV_, Real pref_, Real eref_)
-        : adiabaticIndex(adiabaticIndex_), CV(CV_), pref(pref_), eref(eref_) {}
+    MieGruneisenEOS() = default;
+    MieGruneisenEOS(Real gamma, Real _pref = 0.0, Real _eref = 0.0, Real _CV = 0.0)
+        : adiabaticIndex(gamma), GruneisenGamma(gamma - 1.0), pref(_pref), eref(_eref), CV(_CV) {}
     virtual ~MieGruneisenEOS() = default;
+    // define(params) of equationofstate.cpp:14-26: adiabaticIndex, pref, eref, CV
+    virtual void define(const Vector<Real>& params) {
+        adiabaticIndex = params.at(0);
+        GruneisenGamma = adiabaticIndex - 1.0;
+        pref = params.at(1);
+        eref = params.at(2);
+        CV = params.at(3);
+    }
     virtual int kind() const { return 0; }
-    Real adiabaticIndex, CV, pref, eref;
+    Real adiabaticIndex = 1.4, GruneisenGamma = 0.4, pref = 0.0, eref = 0.0, CV = 0.0;
     Real shearModulus = 0.0;
     Real yieldStress = 0.0;  // the reference keeps it in PlasticEOS::yieldStress
+};
+
+// MixtureEOS (equationofstate.h:73-106): a reactant (first) and its products
+// (second), each a MieGruneisenEOS, mixed by the mass fraction lambda with
+// the phases in thermal equilibrium.
+class MixtureEOS : public MieGruneisenEOS {
+public:
+    MixtureEOS() = default;
+    MixtureEOS(const MieGruneisenEOS& a, const MieGruneisenEOS& b) : first(a), second(b) {}
+    // equationofstate.cpp:177-200: the first four entries the reactant's, the rest the products'
+    void define(const Vector<Real>& params) override {
+        first.define(Vector<Real>(params.begin(), params.begin() + 4));
+        second.define(Vector<Real>(params.begin() + 4, params.end()));
+    }
+    int kind() const override { return 2; }
+    MieGruneisenEOS first, second;
 };
 
 // Us-up Mie-Gruneisen solid (no reference class; the solid-EOS interface of
@@ -65,7 +93,7 @@ public:
 class UsUpSolidEOS : public MieGruneisenEOS {
 public:
     UsUpSolidEOS(Real rho0_, Real Gamma0, Real c0_, Real s_, Real G, Real Y)
-        : MieGruneisenEOS(Gamma0 + 1.0, 1.0, 0.0, 0.0), rho0(rho0_), c0(c0_), s(s_) {
+        : MieGruneisenEOS(Gamma0 + 1.0, 0.0, 0.0, 1.0), rho0(rho0_), c0(c0_), s(s_) {
         shearModulus = G;
         yieldStress = Y;
     }
@@ -193,6 +221,12 @@ public:
 void advance(CellArray& U, CellArray& U1, CellArray& U2, CellArray& MUSCLgrad, CellArray& UL, CellArray& UR,
              CellArray& ULStar, CellArray& URStar, CellArray& UStarStar, Array<MultiFab, AMREX_SPACEDIM>& flux_arr,
              Geometry const& geom, ParameterStruct& parameters, Vector<BCRec>& bc, THINCArray& THINC);
+
+// reactive.cpp:60-103: Heun's RK2 of the pressure-based burn of the mixture
+// material's alpha rho lambda with parameters.dt, then conservativeToPrimitive
+// of U1; U is left a copy of U1's state before the burn, as in the reference.
+// U2 is not used (the library keeps no scratch state for it).
+void reactiveUpdate(CellArray& U, CellArray& U1, CellArray& U2, ParameterStruct& parameters);
 
 // The cylindrical (2D, 3D) or spherical (1D) source about x = 0 with
 // parameters.dt, then conservativeToPrimitive.  On solid levels the stress is

@@ -361,8 +361,10 @@ int tj_batch_create(tj_batch** out, int device, int nprob, const tj_level_desc* 
         const int p = b->order[r];
         const tj_level_desc& d = descs[p];
         b->descs[r] = d;
-        if (d.SOLID || d.THINC != b->thinc || d.RADIAL) {
-            b->err = "batch: fluid problems with one THINC setting and no geometric source";
+        bool mix = false;
+        for (int m = 0; m < d.nmat && m < TJ_MAX_MATERIALS; ++m) mix |= d.mat[m].mixture != 0;
+        if (d.SOLID || d.THINC != b->thinc || d.RADIAL || mix) {
+            b->err = "batch: fluid problems with one THINC setting, no geometric source and no mixture";
             *out = b;
             return TJ_EUNSUPPORTED;
         }

@@ -19,10 +19,14 @@ constexpr int MAXM = 4;
 // mixed-precision form reads an FP32 copy (MatT<float>) in its FP32 parts.
 template <typename R>
 struct MatT {
-    int eos, solid, plastic, pad;
+    int eos, solid, plastic, mix;
     R Gg, gp1, xi;
     R pref, eref, CV;
     R rho0, c0, s, G0, yield;
+    // MixtureEOS (equationofstate.cpp:116-376): ga = the reactant's
+    // adiabaticIndex as given (the mixture formulas read it, not Gg + 1);
+    // g2, Gg2, pref2, eref2, CV2 the products' MieGruneisen EOS
+    R ga, g2, Gg2, pref2, eref2, CV2;
 };
 using MatC = MatT<double>;
 
@@ -47,6 +51,7 @@ struct Level {
     MatT<float> mf[MAXM];  // FP32 copy of m for the mixed-precision form
     int cs[NVAR];      // first component of each Variable (AccessPattern), -1 if absent
     long long ncells;
+    int mixm;          // the mixture material (MixtureEOS), -1 if none
 };
 
 template <typename R>
@@ -193,6 +198,54 @@ __device__ inline double pm_rcbrt(double x) {
     double c = 1.0 / y;
     c = c - (c - x / (c * c)) / 3.0;
     return 1.0 / c;
+}
+
+// log (x = 2^e m, m in [sqrt(2)/2, sqrt(2)), the atanh series of
+// s = (m - 1)/(m + 1) to s^25) and pow = exp(y log x), for std::pow of
+// reactive.cpp:17; the oracle evaluates the same steps.
+__device__ inline double pm_log(double x) {
+    if (x != x || x < 0.0) return __builtin_nan("");
+    if (x == 0.0) return -__builtin_huge_val();
+    if (x == __builtin_huge_val()) return x;
+    uint64_t bits = __builtin_bit_cast(uint64_t, x);
+    int e = 0;
+    if ((bits >> 52) == 0) {
+        x *= 18014398509481984.0;
+        bits = __builtin_bit_cast(uint64_t, x);
+        e = -54;
+    }
+    e += (int)((bits >> 52) & 0x7ff) - 1023;
+    bits = (bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+    double m = __builtin_bit_cast(double, bits);
+    if (m > 1.4142135623730951) {
+        m *= 0.5;
+        e += 1;
+    }
+    const double ln2hi = 6.93147180369123816490e-01;
+    const double ln2lo = 1.90821492927058770002e-10;
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double q = 2.0 / 25.0;
+    q = q * z + 2.0 / 23.0;
+    q = q * z + 2.0 / 21.0;
+    q = q * z + 2.0 / 19.0;
+    q = q * z + 2.0 / 17.0;
+    q = q * z + 2.0 / 15.0;
+    q = q * z + 2.0 / 13.0;
+    q = q * z + 2.0 / 11.0;
+    q = q * z + 2.0 / 9.0;
+    q = q * z + 2.0 / 7.0;
+    q = q * z + 2.0 / 5.0;
+    q = q * z + 2.0 / 3.0;
+    const double lm = 2.0 * s + s * (z * q);
+    const double ed = (double)e;
+    return ed * ln2hi + (lm + ed * ln2lo);
+}
+__device__ inline double pm_pow(double x, double y) {
+    if (y == 0.0) return 1.0;
+    if (x == 0.0) return y > 0.0 ? 0.0 : __builtin_huge_val();
+    return pm_exp(y * pm_log(x));
 }
 
 // exp/tanh/cosh by real type: the project-written FP64 routines above, and

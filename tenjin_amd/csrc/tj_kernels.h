@@ -19,9 +19,10 @@ struct Clock {
 // reconstruct their cell (at the boundary-mapped cell s), neighbours' data
 // travel by wave shuffles, and lane l computes the flux of face c (the lower
 // face of cell c).  Faces per wave: 63 (MUSCL) or 61 (MUSCL + THINC/BVD).
-template <int NM, bool TH, typename R>
+template <int NM, bool TH, typename R, bool MX = false>
 __device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const FacesT<R>& F, const Hist& h, Clock* clk,
                                             long long wave) {
+    constexpr int NF = 2 * NM + 5 + (MX ? 1 : 0);  // flux components per face
     if (clk->done) return;
     constexpr int H = TH ? 2 : 1;
     constexpr int PER = TH ? 61 : 63;
@@ -38,10 +39,10 @@ __device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const 
     const int lofs = s - c;  // lane offset of the reconstructed cell
 
     Prim<NM, R> Pm, Pc, Pp, PL, PR;
-    load_prim<NM, R>(lv, U, s - 1, j, k, Pm);
-    load_prim<NM, R>(lv, U, s, j, k, Pc);
-    load_prim<NM, R>(lv, U, s + 1, j, k, Pp);
-    muscl<NM, R>(Pm, Pc, Pp, PL, PR);
+    load_prim<NM, R, MX>(lv, U, s - 1, j, k, Pm);
+    load_prim<NM, R, MX>(lv, U, s, j, k, Pc);
+    load_prim<NM, R, MX>(lv, U, s + 1, j, k, Pp);
+    muscl<NM, R, MX>(Pm, Pc, Pp, PL, PR);
 
     unsigned st = 0;
     const long long cell_s = ((long long)k * ny + j) * nx + s;
@@ -99,8 +100,8 @@ __device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const 
     }
 
     FSt<NM, R> SLs, SRs;
-    p2c_sound<NM, R>(lv, PL, SLs);
-    p2c_sound<NM, R>(lv, PR, SRs);
+    p2c_sound<NM, R, MX>(lv, PL, SLs);
+    p2c_sound<NM, R, MX>(lv, PR, SRs);
 
     // left state of face c: UR(c-1) from the neighbouring lane (or the
     // boundary image of this lane's UR at the domain's low face)
@@ -116,6 +117,7 @@ __device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const 
     Lf.E = __shfl(SRs.E, lane > 0 ? lane - 1 : 0, 64);
     Lf.p = __shfl(SRs.p, lane > 0 ? lane - 1 : 0, 64);
     Lf.a = __shfl(SRs.a, lane > 0 ? lane - 1 : 0, 64);
+    if (MX) Lf.arl = __shfl(SRs.arl, lane > 0 ? lane - 1 : 0, 64);
 
     if (lane >= H && lane < H + PER && c >= 0 && c <= nx) {
         if (c == 0) {
@@ -124,10 +126,10 @@ __device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const 
         }
         FSt<NM, R> Rf = SLs;
         if (c == nx && lv.bc_hi[0]) flip_state<NM, R>(Rf, 0);
-        R Fv[2 * NM + 5], sa[NM];
-        hllc<NM, R>(Lf, Rf, 0, Fv, sa);
+        R Fv[NF], sa[NM];
+        hllc<NM, R, MX>(Lf, Rf, 0, Fv, sa);
 #pragma unroll
-        for (int q = 0; q < 2 * NM + 5; ++q) F.at(q, c, j, k) = Fv[q];
+        for (int q = 0; q < NF; ++q) F.at(q, c, j, k) = Fv[q];
         if (TH) {
             if (c < nx) {
 #pragma unroll
@@ -142,9 +144,9 @@ __device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const 
     }
     if (st) atomicOr(&clk->status, st);
 }
-template <int NM, bool TH, typename R = double>
+template <int NM, bool TH, typename R = double, bool MX = false>
 __global__ __launch_bounds__(256) void k_flux_x(Level lv, MF U, FacesT<R> F, Hist h, Clock* clk) {
-    flux_x_body<NM, TH, R>(lv, U, F, h, clk, (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    flux_x_body<NM, TH, R, MX>(lv, U, F, h, clk, (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
 // ------------------------------------------------------------------ XCD-aware block order
@@ -168,9 +170,9 @@ __device__ __forceinline__ long long xcd_block(long long b, long long nb, int di
 // alpha rows with the non-conservative term -alpha_U (S*_lo - S*_hi).
 // The face values are FP64, or FP32 in the mixed form (widened here: the
 // accumulation stays FP64).
-template <int NM, typename T>
+template <int NM, typename T, bool MX = false>
 __device__ __forceinline__ void accum_dir(double* al, double* ar, double* ru, double& E, const double* a0, double dtdx,
-                                          const T* Flo, const T* Fhi) {
+                                          const T* Flo, const T* Fhi, double* arl = nullptr) {
     const double s0 = (double)Flo[2 * NM + 4], s1 = (double)Fhi[2 * NM + 4];
 #pragma unroll
     for (int m = 0; m < NM; ++m) al[m] += dtdx * ((double)Flo[m] - (double)Fhi[m] - a0[m] * (s0 - s1));
@@ -179,15 +181,17 @@ __device__ __forceinline__ void accum_dir(double* al, double* ar, double* ru, do
 #pragma unroll
     for (int q = 0; q < 3; ++q) ru[q] += dtdx * ((double)Flo[2 * NM + q] - (double)Fhi[2 * NM + q]);
     E += dtdx * ((double)Flo[2 * NM + 3] - (double)Fhi[2 * NM + 3]);
+    if (MX) *arl += dtdx * ((double)Flo[2 * NM + 5] - (double)Fhi[2 * NM + 5]);
 }
-template <int NM, typename T>
+template <int NM, typename T, bool MX = false>
 __device__ __forceinline__ void load_faces(const FacesT<T>& Fd, int i, int j, int k, T* Fv) {
 #pragma unroll
-    for (int q = 0; q < 2 * NM + 5; ++q) Fv[q] = Fd.at(q, i, j, k);
+    for (int q = 0; q < 2 * NM + 5 + (MX ? 1 : 0); ++q) Fv[q] = Fd.at(q, i, j, k);
 }
-template <int NM>
+template <int NM, bool MX = false>
 __device__ __forceinline__ void load_cons(const Level& lv, const MF& U, int i, int j, int k, double* al, double* ar,
-                                          double* ru, double& E) {
+                                          double* ru, double& E, double* arl = nullptr) {
+    if (MX) *arl = U.at(lv.cs[ALPHARHOLAMBDA], i, j, k);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
@@ -221,26 +225,51 @@ __device__ __forceinline__ void radial_momentum_energy(double f, const double* u
     E += f * (E * u[0] - vs);
 }
 // alpha has no geometric flux (flux.cpp:55): x + f * 0 leaves every x as it is.
-template <int NM>
-__device__ __forceinline__ void radial_fluid(double f, const double* u, double p, double* ar, double* ru, double& E) {
+// A mixture's alpha rho lambda takes alpha rho lambda u_x (flux.cpp:57).
+template <int NM, bool MX = false>
+__device__ __forceinline__ void radial_fluid(double f, const double* u, double p, double* ar, double* ru, double& E,
+                                             double* arl = nullptr) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) ar[m] += f * (ar[m] * u[0]);
+    if (MX) *arl += f * (*arl * u[0]);
     // sigma = -p I of a fluid cell (stressTensor, boxaccesscellarray.cpp:205-214)
     const double s0[3] = {-p * 1.0, -p * 0.0, -p * 0.0};
     radial_momentum_energy(f, u, s0, -p * 0.0, -p * 0.0, -p * 1.0, ru, E);
 }
 
+// pressureBased_UpdateMassFraction (reactive.cpp:3-24): one explicit step of
+// the burn of alpha rho lambda y at alpha, alpha rho and p; no burn where
+// alpha < 0.5, clamped at 0.
+__device__ __forceinline__ double burn_step(double alpha, double ar, double y, double p, double dt) {
+    const double sigma = 20.0, nu = 0.5, n = 1.2;
+    if (alpha < 0.5) return y;
+    double y1 = y - (dt) * (sigma * pm_pow(ar, 1.0 - nu) * pm_pow(y, nu) * pm_pow((p / 1E6), n));
+    if (y1 < 0.0) y1 = 0.0;
+    return y1;
+}
+// reactiveUpdate (reactive.cpp:60-103) of one cell: Heun's RK2 of the burn
+// (the second stage reads the first's alpha rho lambda, the cell's alpha,
+// alpha rho and p unchanged), then the average.
+__device__ __forceinline__ double burn_rk2(double alpha, double ar, double y, double p, double dt) {
+    const double y1 = burn_step(alpha, ar, y, p, dt);
+    const double y2 = burn_step(alpha, ar, y1, p, dt);
+    return ((y * (1.0 / 2.0)) + (y2 * (1.0 / 2.0)));
+}
+
 // conservativeToPrimitive of the updated cell (hllc.cpp:689), and in stage 2
-// the RK average (hllc.cpp:705), the geometric source (main.cpp:159-162),
-// cleanUpAlpha (main.cpp:169-171), the NaN scan (main.cpp:183-187) and the
-// sound speed / CFL candidate of the next getTimeStep (main.cpp:128); then the
-// store.
-template <int NM, int STAGE>
+// the RK average (hllc.cpp:705), the reactive source (main.cpp:149-152), the
+// geometric source (main.cpp:159-162), cleanUpAlpha (main.cpp:169-171), the
+// NaN scan (main.cpp:183-187) and the sound speed / CFL candidate of the next
+// getTimeStep (main.cpp:128); then the store.  MX: the mixture's
+// alpha rho lambda (arl) rides along, lambda and the phase densities come
+// from rootFind in each c2p and are averaged with the rest.
+template <int NM, int STAGE, bool MX = false>
 __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const MF& Uo, int i, int j, int k, double* al,
                                             double* ar, double* ru, double E, double dt, unsigned ops, double& cand,
-                                            unsigned& st) {
+                                            unsigned& st, double arl = 0.0) {
     double rk[NM], rho, u[3], p;
-    c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+    MixV<double> x{};
+    st |= c2p<NM, MX>(lv, al, ar, ru, E, rk, rho, u, p, arl, &x);
     if (STAGE == 2) {
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
@@ -256,9 +285,21 @@ __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const
         }
         E = Ub.at(lv.cs[TOTAL_E], i, j, k) * (1.0 / 2.0) + E * (1.0 / 2.0);
         p = Ub.at(lv.cs[PRESSURE], i, j, k) * (1.0 / 2.0) + p * (1.0 / 2.0);
+        if (MX) {
+            arl = Ub.at(lv.cs[ALPHARHOLAMBDA], i, j, k) * (1.0 / 2.0) + arl * (1.0 / 2.0);
+            x.lam = Ub.at(lv.cs[LAMBDA], i, j, k) * (1.0 / 2.0) + x.lam * (1.0 / 2.0);
+            x.ra = Ub.at(lv.cs[RHO_MIX], i, j, k) * (1.0 / 2.0) + x.ra * (1.0 / 2.0);
+            x.rb = Ub.at(lv.cs[RHO_MIX] + 1, i, j, k) * (1.0 / 2.0) + x.rb * (1.0 / 2.0);
+            if (ops & 64u) {  // the burn on the averaged state, then c2p
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+                    if (m == lv.mixm) arl = burn_rk2(al[m], ar[m], arl, p, dt);
+                st |= c2p<NM, MX>(lv, al, ar, ru, E, rk, rho, u, p, arl, &x);
+            }
+        }
         if (ops & 32u) {  // the averaged sigma is -p I with the averaged p
-            radial_fluid<NM>(radial_factor(lv, i, dt), u, p, ar, ru, E);
-            c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+            radial_fluid<NM, MX>(radial_factor(lv, i, dt), u, p, ar, ru, E, &arl);
+            st |= c2p<NM, MX>(lv, al, ar, ru, E, rk, rho, u, p, arl, &x);
         }
         if (ops & 4u) st |= cleanup_alpha<NM>(al);
         if (ops & 16u) {
@@ -268,10 +309,16 @@ __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const
 #pragma unroll
             for (int q = 0; q < 3; ++q) bad |= __builtin_isnan(ru[q]) || __builtin_isnan(u[q]);
             bad |= __builtin_isnan(E) || __builtin_isnan(p);
+            if (MX) bad |= __builtin_isnan(arl) || __builtin_isnan(x.lam);
             if (bad) st |= 1u;
         }
         if (ops & 8u) {
-            const double a = sound<NM>(lv, al, ar, rk, rho, p);
+            if (MX) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+                    if (m == lv.mixm) x.rk = rk[m];
+            }
+            const double a = sound<NM, MX>(lv, al, ar, rk, rho, p, x);
             Uo.at(lv.cs[SOUNDSPEED], i, j, k) = a;
             #pragma unroll
             for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
@@ -292,6 +339,12 @@ __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const
     }
     Uo.at(lv.cs[TOTAL_E], i, j, k) = E;
     Uo.at(lv.cs[PRESSURE], i, j, k) = p;
+    if (MX) {
+        Uo.at(lv.cs[ALPHARHOLAMBDA], i, j, k) = arl;
+        Uo.at(lv.cs[LAMBDA], i, j, k) = x.lam;
+        Uo.at(lv.cs[RHO_MIX], i, j, k) = x.ra;
+        Uo.at(lv.cs[RHO_MIX] + 1, i, j, k) = x.rb;
+    }
 }
 // wave min of the positive CFL candidates (as ordered bits)
 __device__ __forceinline__ unsigned long long wave_min_bits(double cand) {
@@ -334,14 +387,14 @@ __device__ __forceinline__ void block_min_to(double cand, unsigned long long* ds
 // right state of row r-1 (2 NM + 7 slots), the alpha reconstructions of rows
 // r-1 and r (THINC, 4 NM + 1 slots each) and the lower-face flux of the row
 // being completed (fused form, 2 NM + 5 slots).
-template <int NM, bool TH, bool FUSE>
+template <int NM, bool TH, bool FUSE, bool MX = false>
 struct MarchSlots {
-    static constexpr int FST = 2 * NM + 7;
+    static constexpr int FST = 2 * NM + 7 + (MX ? 1 : 0);
     static constexpr int AREC = 4 * NM + 1;
     static constexpr int FLX = 2 * NM + 5;
     static constexpr int N = FST + (TH ? 2 * AREC : 0) + (FUSE ? FLX : 0);
 };
-template <int NM, typename R>
+template <int NM, typename R, bool MX = false>
 __device__ __forceinline__ void fst_put(R* sh, int lane, const FSt<NM, R>& S) {
     int q = 0;
 #pragma unroll
@@ -355,8 +408,9 @@ __device__ __forceinline__ void fst_put(R* sh, int lane, const FSt<NM, R>& S) {
     sh[(q++) * 64 + lane] = S.E;
     sh[(q++) * 64 + lane] = S.p;
     sh[(q++) * 64 + lane] = S.a;
+    if (MX) sh[(q++) * 64 + lane] = S.arl;
 }
-template <int NM, typename R>
+template <int NM, typename R, bool MX = false>
 __device__ __forceinline__ void fst_get(const R* sh, int lane, FSt<NM, R>& S) {
     int q = 0;
 #pragma unroll
@@ -370,6 +424,7 @@ __device__ __forceinline__ void fst_get(const R* sh, int lane, FSt<NM, R>& S) {
     S.E = sh[(q++) * 64 + lane];
     S.p = sh[(q++) * 64 + lane];
     S.a = sh[(q++) * 64 + lane];
+    if (MX) S.arl = sh[(q++) * 64 + lane];
 }
 template <int NM, typename R>
 __device__ __forceinline__ void arec_put(R* sh, int lane, const ARec<NM, R>& A) {
@@ -405,11 +460,13 @@ __device__ __forceinline__ void arec_get(const R* sh, int lane, ARec<NM, R>& A) 
 // and completes each row as soon as its upper face is known (update in
 // x, y, z order with the earlier directions' faces read from their planes,
 // then finish_cell); the faces of this direction never reach HBM.
-template <int NM, bool TH, int D, bool FUSE, int STAGE, typename R>
+template <int NM, bool TH, int D, bool FUSE, int STAGE, typename R, bool MX = false>
 __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, const FacesT<R>& F, const Hist& h, int seglen,
                                                 Clock* clk, const MF& Ub, const MF& Uo, const FacesT<R>& Fx,
                                                 const FacesT<R>& Fy, unsigned ops, long long wave) {
-    using Slots = MarchSlots<NM, TH, FUSE>;
+    static_assert(!(FUSE && MX), "the fused update has no mixture form");
+    constexpr int NF = 2 * NM + 5 + (MX ? 1 : 0);  // flux components per face
+    using Slots = MarchSlots<NM, TH, FUSE, MX>;
     __shared__ R shm[4 * Slots::N * 64];
     if (clk->done) return;
     const int lane = threadIdx.x & 63;
@@ -470,13 +527,13 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
     };
     auto emit = [&](int face, const FSt<NM, R>& A, const FSt<NM, R>& B, R* Fv) {
         R sa[NM];
-        hllc<NM, R>(A, B, D, Fv, sa);
+        hllc<NM, R, MX>(A, B, D, Fv, sa);
         if (!act) return;
         const long long fi = faceidx(face);
         if (!FUSE) {
             const long long fcs = F.cs;
 #pragma unroll
-            for (int q = 0; q < 2 * NM + 5; ++q) F.p[q * fcs + fi] = Fv[q];
+            for (int q = 0; q < NF; ++q) F.p[q * fcs + fi] = Fv[q];
         }
         if (TH) {
             if (face < nd) {
@@ -496,10 +553,10 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
         Prim<NM, R> PL, PR;
         {
             Prim<NM, R> Pm, Pc, Pp;
-            load_prim<NM, R>(lv, U, i, cj(r - 1), ck(r - 1), Pm);
-            load_prim<NM, R>(lv, U, i, cj(r), ck(r), Pc);
-            load_prim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
-            muscl<NM, R>(Pm, Pc, Pp, PL, PR);
+            load_prim<NM, R, MX>(lv, U, i, cj(r - 1), ck(r - 1), Pm);
+            load_prim<NM, R, MX>(lv, U, i, cj(r), ck(r), Pc);
+            load_prim<NM, R, MX>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
+            muscl<NM, R, MX>(Pm, Pc, Pp, PL, PR);
         }
         if (TH) {
             const uint8_t ef = lv.edge[D][r];
@@ -562,17 +619,17 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
             if (!bhi) arec_put<NM, R>(shAc, lane, Anext);
             curValid = !bhi;
         }
-        p2c_sound<NM, R>(lv, PL, SLs);
-        p2c_sound<NM, R>(lv, PR, SRs);
+        p2c_sound<NM, R, MX>(lv, PL, SLs);
+        p2c_sound<NM, R, MX>(lv, PR, SRs);
         if (r >= f0 && r <= ftop) {  // face r: left UR(r-1) (the image of UR(0) at the domain face), right UL(r)
             FSt<NM, R> A;
             if (r == 0) {
                 A = SRs;
                 if (lv.bc_lo[D]) flip_state<NM, R>(A, D);
             } else {
-                fst_get<NM, R>(shR, lane, A);
+                fst_get<NM, R, MX>(shR, lane, A);
             }
-            R Fcur[2 * NM + 5];
+            R Fcur[NF];
             emit(r, A, SLs, Fcur);
             if (FUSE) {
                 if (r - 1 >= f0) {
@@ -585,11 +642,11 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
                 for (int q = 0; q < 2 * NM + 5; ++q) shF[q * 64 + lane] = Fcur[q];
             }
         }
-        fst_put<NM, R>(shR, lane, SRs);
+        fst_put<NM, R, MX>(shR, lane, SRs);
     }
     if (ftop == nd) {  // domain face nd: UR(nd-1) and the image of UL(nd-1), from the last row
         if (lv.bc_hi[D]) flip_state<NM, R>(SLs, D);
-        R Ftop[2 * NM + 5];
+        R Ftop[NF];
         emit(nd, SRs, SLs, Ftop);
         if (FUSE) {
             R Flo[2 * NM + 5];
@@ -601,21 +658,21 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
     if (FUSE && STAGE == 2 && (ops & 8u)) wave_min_to(cand, &clk->dtmin_bits);
     if (st) atomicOr(&clk->status, st);
 }
-template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1, typename R = double>
+template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1, typename R = double, bool MX = false>
 __global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk,
                                                     MF Ub = MF{}, MF Uo = MF{}, FacesT<R> Fx = FacesT<R>{},
                                                     FacesT<R> Fy = FacesT<R>{}, unsigned ops = 0u) {
-    flux_march_body<NM, TH, D, FUSE, STAGE, R>(lv, U, F, h, seglen, clk, Ub, Uo, Fx, Fy, ops,
-                                               (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    flux_march_body<NM, TH, D, FUSE, STAGE, R, MX>(lv, U, F, h, seglen, clk, Ub, Uo, Fx, Fy, ops,
+                                                   (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 // The same pass held to two waves per SIMD (the FP64 THINC pass: its few
 // registers beyond 256 spill to scratch on rarely taken paths, and the second
 // wave hides the FP64 latency; measured faster than one wave per SIMD).
-template <int NM, bool TH, int D, typename R = double>
+template <int NM, bool TH, int D, typename R = double, bool MX = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_flux_march_w2(
     Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk) {
-    flux_march_body<NM, TH, D, false, 1, R>(lv, U, F, h, seglen, clk, MF{}, MF{}, FacesT<R>{}, FacesT<R>{}, 0u,
-                                            (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    flux_march_body<NM, TH, D, false, 1, R, MX>(lv, U, F, h, seglen, clk, MF{}, MF{}, FacesT<R>{}, FacesT<R>{}, 0u,
+                                                (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
 // ------------------------------------------------------------------ update
@@ -624,10 +681,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
 // advance (hllc.cpp:705), cleanUpAlpha (main.cpp:169-171), the NaN scan
 // (main.cpp:183-187) and getTimeStep of the next step (main.cpp:128).
 // (one block of 256 cells: `block` is the block's index within the level)
-template <int NM, int STAGE, typename T>
+template <int NM, int STAGE, typename T, bool MX = false>
 __device__ __forceinline__ void update_body(const Level& lv, const MF& Uin, const MF& Ub, const MF& Uo, const FacesT<T>& F0,
                                             const FacesT<T>& F1, const FacesT<T>& F2, Clock* clk, unsigned ops,
                                             long long block) {
+    constexpr int NF = 2 * NM + 5 + (MX ? 1 : 0);
     if (clk->done) return;
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = block * blockDim.x + threadIdx.x;
@@ -639,30 +697,32 @@ __device__ __forceinline__ void update_body(const Level& lv, const MF& Uin, cons
         const int j = (int)((id / nx) % ny);
         const int k = (int)(id / ((long long)nx * ny));
         const double dt = clk->dt;
-        double al[NM], ar[NM], ru[3], E, a0[NM];
-        load_cons<NM>(lv, Uin, i, j, k, al, ar, ru, E);
+        double al[NM], ar[NM], ru[3], E, a0[NM], arl = 0.0;
+        load_cons<NM, MX>(lv, Uin, i, j, k, al, ar, ru, E, &arl);
 #pragma unroll
         for (int m = 0; m < NM; ++m) a0[m] = al[m];
         for (int d = 0; d < lv.dim; ++d) {
             const FacesT<T>& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
-            T Fa[2 * NM + 5], Fb[2 * NM + 5];
-            load_faces<NM, T>(Fd, i, j, k, Fa);
-            load_faces<NM, T>(Fd, i + (d == 0), j + (d == 1), k + (d == 2), Fb);
-            accum_dir<NM, T>(al, ar, ru, E, a0, dt / lv.dx[d], Fa, Fb);
+            T Fa[NF], Fb[NF];
+            load_faces<NM, T, MX>(Fd, i, j, k, Fa);
+            load_faces<NM, T, MX>(Fd, i + (d == 0), j + (d == 1), k + (d == 2), Fb);
+            accum_dir<NM, T, MX>(al, ar, ru, E, a0, dt / lv.dx[d], Fa, Fb, &arl);
         }
-        finish_cell<NM, STAGE>(lv, Ub, Uo, i, j, k, al, ar, ru, E, dt, ops, cand, st);
+        finish_cell<NM, STAGE, MX>(lv, Ub, Uo, i, j, k, al, ar, ru, E, dt, ops, cand, st, arl);
     }
     if (STAGE == 2 && (ops & 8u)) block_min_to(cand, &clk->dtmin_bits);
     if (st) atomicOr(&clk->status, st);
 }
-template <int NM, int STAGE, typename T = double>
+template <int NM, int STAGE, typename T = double, bool MX = false>
 __global__ __launch_bounds__(256) void k_update(Level lv, MF Uin, MF Ub, MF Uo, FacesT<T> F0, FacesT<T> F1,
                                                 FacesT<T> F2, Clock* clk, unsigned ops) {
-    update_body<NM, STAGE, T>(lv, Uin, Ub, Uo, F0, F1, F2, clk, ops, xcd_block(blockIdx.x, gridDim.x, lv.dim));
+    update_body<NM, STAGE, T, MX>(lv, Uin, Ub, Uo, F0, F1, F2, clk, ops, xcd_block(blockIdx.x, gridDim.x, lv.dim));
 }
 
 // getSoundSpeed + boxTimeStepFunction + min over the level (timestep.cpp:7-70).
-template <int NM>
+// A mixture reads its stored lambda and phase densities (MixtureEOS::
+// mixtureSoundSpeed, equationofstate.cpp:212-226).
+template <int NM, bool MX = false>
 __device__ __forceinline__ void sound_dt_body(const Level& lv, const MF& U, Clock* clk, long long block) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = block * blockDim.x + threadIdx.x;
@@ -680,16 +740,25 @@ __device__ __forceinline__ void sound_dt_body(const Level& lv, const MF& U, Cloc
             rk[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
         }
         const double rho = U.at(lv.cs[RHO], i, j, k), p = U.at(lv.cs[PRESSURE], i, j, k);
-        const double a = sound<NM>(lv, al, ar, rk, rho, p);
+        MixV<double> x{};
+        if (MX) {
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                if (m == lv.mixm) x.rk = rk[m];
+            x.lam = U.at(lv.cs[LAMBDA], i, j, k);
+            x.ra = U.at(lv.cs[RHO_MIX], i, j, k);
+            x.rb = U.at(lv.cs[RHO_MIX] + 1, i, j, k);
+        }
+        const double a = sound<NM, MX>(lv, al, ar, rk, rho, p, x);
         U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
         for (int d = 0; d < lv.dim; ++d)
             cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(U.at(lv.cs[VELOCITY] + d, i, j, k))));
     }
     block_min_to(cand, &clk->dtmin_bits);
 }
-template <int NM>
+template <int NM, bool MX = false>
 __global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
-    sound_dt_body<NM>(lv, U, clk, blockIdx.x);
+    sound_dt_body<NM, MX>(lv, U, clk, blockIdx.x);
 }
 
 // dt = CFL * min (timestep.cpp:61-69); with `driver` the loop test and the
@@ -749,23 +818,28 @@ __global__ __launch_bounds__(256) void k_cleanup_alpha(Level lv, MF U, Clock* cl
     if (st) atomicOr(&clk->status, st);
 }
 
-// stand-alone geometricSourceTerm of a fluid level (radial.cpp:37-54) with the
-// clock's dt: the source, then conservativeToPrimitive
+// Mixture variables of a cell (MixtureEOS): alpha rho lambda, lambda, RHO_MIX.
 template <int NM>
-__global__ __launch_bounds__(256) void k_radial_fluid(Level lv, MF U, Clock* clk) {
-    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long long)nx * ny * nz) return;
-    const int i = (int)(id % nx);
-    const int j = (int)((id / nx) % ny);
-    const int k = (int)(id / ((long long)nx * ny));
-    double al[NM], ar[NM], ru[3], E, rk[NM], rho, u[3], p;
-    load_cons<NM>(lv, U, i, j, k, al, ar, ru, E);
+__device__ __forceinline__ void load_mix(const Level& lv, const MF& U, int i, int j, int k, const double* rk, double& arl,
+                                         MixV<double>& x) {
+    arl = U.at(lv.cs[ALPHARHOLAMBDA], i, j, k);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) u[q] = U.at(lv.cs[VELOCITY] + q, i, j, k);
-    p = U.at(lv.cs[PRESSURE], i, j, k);
-    radial_fluid<NM>(radial_factor(lv, i, clk->dt), u, p, ar, ru, E);
-    c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+    for (int m = 0; m < NM; ++m)
+        if (m == lv.mixm) x.rk = rk[m];
+    x.lam = U.at(lv.cs[LAMBDA], i, j, k);
+    x.ra = U.at(lv.cs[RHO_MIX], i, j, k);
+    x.rb = U.at(lv.cs[RHO_MIX] + 1, i, j, k);
+}
+__device__ __forceinline__ void store_mix(const Level& lv, const MF& U, int i, int j, int k, double arl, const MixV<double>& x) {
+    U.at(lv.cs[ALPHARHOLAMBDA], i, j, k) = arl;
+    U.at(lv.cs[LAMBDA], i, j, k) = x.lam;
+    U.at(lv.cs[RHO_MIX], i, j, k) = x.ra;
+    U.at(lv.cs[RHO_MIX] + 1, i, j, k) = x.rb;
+}
+// a fluid cell after c2p: the conserved and primitive variables it changed
+template <int NM>
+__device__ __forceinline__ void store_c2p(const Level& lv, const MF& U, int i, int j, int k, const double* ar, const double* rk,
+                                          double rho, const double* ru, const double* u, double E, double p) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         U.at(lv.cs[ALPHARHO] + m, i, j, k) = ar[m];
@@ -781,8 +855,54 @@ __global__ __launch_bounds__(256) void k_radial_fluid(Level lv, MF U, Clock* clk
     U.at(lv.cs[PRESSURE], i, j, k) = p;
 }
 
-// contains_nan over conserved and primitive components (boxaccesscellarray.cpp:601-645)
+// stand-alone geometricSourceTerm of a fluid level (radial.cpp:37-54) with the
+// clock's dt: the source, then conservativeToPrimitive
+template <int NM, bool MX = false>
+__global__ __launch_bounds__(256) void k_radial_fluid(Level lv, MF U, Clock* clk) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    double al[NM], ar[NM], ru[3], E, rk[NM], rho, u[3], p, arl = 0.0;
+    load_cons<NM, MX>(lv, U, i, j, k, al, ar, ru, E, &arl);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) u[q] = U.at(lv.cs[VELOCITY] + q, i, j, k);
+    p = U.at(lv.cs[PRESSURE], i, j, k);
+    radial_fluid<NM, MX>(radial_factor(lv, i, clk->dt), u, p, ar, ru, E, &arl);
+    MixV<double> x{};
+    const unsigned st = c2p<NM, MX>(lv, al, ar, ru, E, rk, rho, u, p, arl, &x);
+    store_c2p<NM>(lv, U, i, j, k, ar, rk, rho, ru, u, E, p);
+    if (MX) store_mix(lv, U, i, j, k, arl, x);
+    if (st) atomicOr(&clk->status, st);
+}
+
+// stand-alone reactiveUpdate of a level with a mixture (reactive.cpp:60-103)
+// with the clock's dt: Heun's RK2 of the burn, then conservativeToPrimitive
 template <int NM>
+__global__ __launch_bounds__(256) void k_reactive_fluid(Level lv, MF U, Clock* clk) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    double al[NM], ar[NM], ru[3], E, rk[NM], rho, u[3], p, arl = 0.0;
+    load_cons<NM, true>(lv, U, i, j, k, al, ar, ru, E, &arl);
+    p = U.at(lv.cs[PRESSURE], i, j, k);
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+        if (m == lv.mixm) arl = burn_rk2(al[m], ar[m], arl, p, clk->dt);
+    MixV<double> x{};
+    const unsigned st = c2p<NM, true>(lv, al, ar, ru, E, rk, rho, u, p, arl, &x);
+    store_c2p<NM>(lv, U, i, j, k, ar, rk, rho, ru, u, E, p);
+    store_mix(lv, U, i, j, k, arl, x);
+    if (st) atomicOr(&clk->status, st);
+}
+
+// contains_nan over conserved and primitive components (boxaccesscellarray.cpp:601-645)
+template <int NM, bool MX = false>
 __global__ __launch_bounds__(256) void k_nancheck(Level lv, MF U, Clock* clk) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -796,12 +916,15 @@ __global__ __launch_bounds__(256) void k_nancheck(Level lv, MF U, Clock* clk) {
     for (int v = 0; v < 6; ++v)
         for (int q = 0; q < cnt[v]; ++q) bad |= __builtin_isnan(U.at(lv.cs[vars[v]] + q, i, j, k));
     bad |= __builtin_isnan(U.at(lv.cs[PRESSURE], i, j, k));
+    if (MX) bad |= __builtin_isnan(U.at(lv.cs[LAMBDA], i, j, k)) || __builtin_isnan(U.at(lv.cs[ALPHARHOLAMBDA], i, j, k));
     if (bad) atomicOr(&clk->status, 1u);
 }
 
 // setInitialConditions' pressure shift and primitiveToConservative
-// (initialise.cpp:104, boxaccesscellarray.cpp:124-174) over valid cells.
-template <int NM>
+// (initialise.cpp:104, boxaccesscellarray.cpp:124-174) over valid cells.  A
+// mixture's pressure shift reads the phase densities the initial state holds
+// (initialise.cpp:66-71); p2c then defines them from the shifted p.
+template <int NM, bool MX = false>
 __device__ __forceinline__ void init_p2c_body(const Level& lv, const MF& U, int shift, long long block) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = block * blockDim.x + threadIdx.x;
@@ -818,53 +941,58 @@ __device__ __forceinline__ void init_p2c_body(const Level& lv, const MF& U, int 
 #pragma unroll
     for (int q = 0; q < 3; ++q) P.u[q] = U.at(lv.cs[VELOCITY] + q, i, j, k);
     P.p = U.at(lv.cs[PRESSURE], i, j, k);
+    if (MX) P.lam = U.at(lv.cs[LAMBDA], i, j, k);
     if (shift) {
-        double ar[NM], G, Ent, Pnt;
+        double ar[NM], G, Ent, Pnt, arl;
 #pragma unroll
         for (int m = 0; m < NM; ++m) ar[m] = P.a[m] * P.r[m];
-        eos_sums<NM>(lv, P.a, ar, G, Ent, Pnt);
+        MixV<double> x{};
+        if (MX) load_mix<NM>(lv, U, i, j, k, P.r, arl, x);
+        eos_sums<NM, double, MX>(lv, P.a, ar, G, Ent, Pnt, x);
         P.p += Pnt / G;
         U.at(lv.cs[PRESSURE], i, j, k) = P.p;
     }
     FSt<NM> S;
-    p2c_sound<NM>(lv, P, S);
+    MixV<double> xo{};
+    p2c_sound<NM, double, MX>(lv, P, S, &xo);
 #pragma unroll
     for (int m = 0; m < NM; ++m) U.at(lv.cs[ALPHARHO] + m, i, j, k) = S.ar[m];
     U.at(lv.cs[RHO], i, j, k) = S.rho;
 #pragma unroll
     for (int q = 0; q < 3; ++q) U.at(lv.cs[RHOU] + q, i, j, k) = S.u[q] * S.rho;
     U.at(lv.cs[TOTAL_E], i, j, k) = S.E;
+    if (MX) {
+        U.at(lv.cs[ALPHARHOLAMBDA], i, j, k) = S.arl;
+        U.at(lv.cs[RHO_MIX], i, j, k) = xo.ra;
+        U.at(lv.cs[RHO_MIX] + 1, i, j, k) = xo.rb;
+    }
 }
-template <int NM>
+template <int NM, bool MX = false>
 __global__ __launch_bounds__(256) void k_init_p2c(Level lv, MF U, int shift) {
-    init_p2c_body<NM>(lv, U, shift, blockIdx.x);
+    init_p2c_body<NM, MX>(lv, U, shift, blockIdx.x);
 }
 
 // conservativeToPrimitive over the valid cells.
-template <int NM>
-__global__ __launch_bounds__(256) void k_c2p(Level lv, MF U) {
+template <int NM, bool MX = false>
+__global__ __launch_bounds__(256) void k_c2p(Level lv, MF U, Clock* clk) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)nx * ny * nz) return;
     const int i = (int)(id % nx);
     const int j = (int)((id / nx) % ny);
     const int k = (int)(id / ((long long)nx * ny));
-    double al[NM], ar[NM], ru[3], rk[NM], u[3], rho, p;
-#pragma unroll
-    for (int m = 0; m < NM; ++m) {
-        al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
-        ar[m] = U.at(lv.cs[ALPHARHO] + m, i, j, k);
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) ru[q] = U.at(lv.cs[RHOU] + q, i, j, k);
-    const double E = U.at(lv.cs[TOTAL_E], i, j, k);
-    c2p<NM>(lv, al, ar, ru, E, rk, rho, u, p);
+    double al[NM], ar[NM], ru[3], rk[NM], u[3], rho, p, E, arl = 0.0;
+    load_cons<NM, MX>(lv, U, i, j, k, al, ar, ru, E, &arl);
+    MixV<double> x{};
+    const unsigned st = c2p<NM, MX>(lv, al, ar, ru, E, rk, rho, u, p, arl, &x);
 #pragma unroll
     for (int m = 0; m < NM; ++m) U.at(lv.cs[RHO_K] + m, i, j, k) = rk[m];
     U.at(lv.cs[RHO], i, j, k) = rho;
 #pragma unroll
     for (int q = 0; q < 3; ++q) U.at(lv.cs[VELOCITY] + q, i, j, k) = u[q];
     U.at(lv.cs[PRESSURE], i, j, k) = p;
+    if (MX) store_mix(lv, U, i, j, k, arl, x);
+    if (st) atomicOr(&clk->status, st);
 }
 
 // sigma = -p I for fluids (stressTensor, boxaccesscellarray.cpp:176-216).

@@ -141,6 +141,13 @@ int build_layout(const tj_level_desc& d, int cs[NVAR]) {
     add(SOUNDSPEED, 1);
     add(USTAR, 1);
     add(SIGMA, 9);
+    int nmix = 0;
+    for (int m = 0; m < d.nmat; ++m) nmix |= d.mat[m].mixture != 0;
+    if (nmix) {  // every mixture uses mixtureIndex 0 (initialise.cpp:187)
+        add(RHO_MIX, 2);
+        add(LAMBDA, 1);
+        add(ALPHARHOLAMBDA, 1);
+    }
     if (d.SOLID) {
         add(V_TENSOR, 9);
         add(VSTAR, 9);
@@ -256,6 +263,18 @@ static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::
         err = "bind_level: MUSCL = 0 is not available in this build";
         return TJ_EUNSUPPORTED;
     }
+    int nmixm = 0;
+    for (int m = 0; m < d->nmat; ++m) nmixm += d->mat[m].mixture != 0;
+    if (nmixm > 1) {
+        // the reference addresses every mixture's lambda and phase densities
+        // at mixtureIndex 0, so a second mixture would alias the first
+        err = "bind_level: at most one mixture material (all mixtures share mixtureIndex 0, initialise.cpp:187)";
+        return TJ_EUNSUPPORTED;
+    }
+    if (nmixm && d->SOLID) {
+        err = "bind_level: mixtures are available on fluid levels";
+        return TJ_EUNSUPPORTED;
+    }
     if (d->RADIAL && d->SOLID && !d->PLASTIC) {
         // the source reads the stored stress, which on an elastic level is the
         // RK average of two states' stresses (main.cpp:149-162), not the stress
@@ -280,6 +299,7 @@ static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::
     lv.thinc = d->THINC;
     lv.beta0 = d->THINCbeta;
     lv.cfl = d->CFL;
+    lv.mixm = -1;
     for (int m = 0; m < d->nmat; ++m) {
         const tj_material& M = d->mat[m];
         MatC& C = lv.m[m];
@@ -297,6 +317,14 @@ static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::
         C.s = M.s;
         C.G0 = M.G0;
         C.yield = M.yield_stress;
+        C.mix = M.mixture != 0;
+        C.ga = M.gamma;
+        C.g2 = M.gamma_mix;
+        C.Gg2 = M.gamma_mix - 1.0;
+        C.pref2 = M.pref_mix;
+        C.eref2 = M.eref_mix;
+        C.CV2 = M.CV_mix;
+        if (C.mix) lv.mixm = m;
         MatT<float>& F = lv.mf[m];
         F.eos = C.eos;
         F.solid = C.solid;
@@ -312,6 +340,13 @@ static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::
         F.s = (float)C.s;
         F.G0 = (float)C.G0;
         F.yield = (float)C.yield;
+        F.mix = C.mix;
+        F.ga = (float)C.ga;
+        F.g2 = (float)C.g2;
+        F.Gg2 = (float)C.Gg2;
+        F.pref2 = (float)C.pref2;
+        F.eref2 = (float)C.eref2;
+        F.CV2 = (float)C.CV2;
     }
     build_layout(*d, lv.cs);
     lv.ncells = (long long)lv.n[0] * lv.n[1] * lv.n[2];
@@ -349,7 +384,7 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
         HIPCHK(c, hipMemcpy(c->d_edge[a], fl.data(), fl.size(), hipMemcpyHostToDevice));
         lv.edge[a] = c->d_edge[a];
     }
-    c->nfc = d->SOLID ? 3 * d->nmat + 17 : 2 * d->nmat + 5;
+    c->nfc = d->SOLID ? 3 * d->nmat + 17 : 2 * d->nmat + 5 + (lv.mixm >= 0 ? 1 : 0);
     for (int a = 0; a < d->dim; ++a) {
         Faces& F = c->F[a];
         F.nx = lv.n[0] + (a == 0);
@@ -411,6 +446,7 @@ static int check_mf(tj_ctx* c, const tj_mf* m) {
 static void launch_sound_dt(tj_ctx* c, const MF& U) {
     const unsigned g = grid1(c->lv.ncells, 256);
     if (c->lv.solid) hipLaunchKernelGGL(k_sound_dt_s<2>, dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
+    else if (c->lv.mixm >= 0) hipLaunchKernelGGL((k_sound_dt<2, true>), dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
     else hipLaunchKernelGGL(k_sound_dt<2>, dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
 }
 
@@ -424,6 +460,11 @@ static void launch_update_t(tj_ctx* c, int stage, const MF& in, const MF& base, 
             hipLaunchKernelGGL((k_update_s<2, 1, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
         else
             hipLaunchKernelGGL((k_update_s<2, 2, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
+    } else if (c->lv.mixm >= 0) {
+        if (stage == 1)
+            hipLaunchKernelGGL((k_update<2, 1, T, true>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
+        else
+            hipLaunchKernelGGL((k_update<2, 2, T, true>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
     } else {
         if (stage == 1)
             hipLaunchKernelGGL((k_update<2, 1, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
@@ -465,6 +506,7 @@ static unsigned step_ops(const tj_ctx* c) {
     if (c->desc.PLASTIC) ops |= TJ_OP_PLASTIC;
     if (c->desc.SOLID) ops |= TJ_OP_CLEAN_V;
     if (c->desc.RADIAL) ops |= TJ_OP_RADIAL;
+    if (c->desc.REACTIVE && c->lv.mixm >= 0) ops |= TJ_OP_REACTIVE;  // reactive.cpp:64
     return ops;
 }
 
@@ -641,8 +683,8 @@ static unsigned march_grid(const Level& lv, int d, int sl, bool fused) {
 
 // One directional flux pass (not fused) with face values of type R.
 extern "C++" {
-template <typename R>
-static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, const Hist& h) {
+template <typename R, bool MX>
+static void launch_pass_t(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, const Hist& h) {
     const Level& lv = c->lv;
     const bool th = c->desc.THINC != 0;
     if (lv.solid) {
@@ -663,29 +705,36 @@ static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, con
         const int per = th ? 61 : 63;
         const long long nseg = (lv.n[0] + 1 + per - 1) / per;
         const unsigned g = grid1(nseg * lv.n[1] * lv.n[2], 4);
-        if (th) hipLaunchKernelGGL((k_flux_x<2, true, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], h, c->d_clk);
-        else hipLaunchKernelGGL((k_flux_x<2, false, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], h, c->d_clk);
+        if (th) hipLaunchKernelGGL((k_flux_x<2, true, R, MX>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], h, c->d_clk);
+        else hipLaunchKernelGGL((k_flux_x<2, false, R, MX>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], h, c->d_clk);
     } else {
         const MF z{};
         const FacesT<R> zf{};
         if (d == 1 && th && sizeof(R) == 8) {
-            const int sl = seg_for(c, TJ_KPTR(k_flux_march_w2<2, true, 1, R>), 1, false);
-            hipLaunchKernelGGL((k_flux_march_w2<2, true, 1, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
+            const int sl = seg_for(c, TJ_KPTR(k_flux_march_w2<2, true, 1, R, MX>), 1, false);
+            hipLaunchKernelGGL((k_flux_march_w2<2, true, 1, R, MX>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
                                c->stream, lv, Uin, F[1], h, sl, c->d_clk);
         } else if (d == 1 && th) {
-            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, true, 1, false, 1, R>), 1, false);
-            hipLaunchKernelGGL((k_flux_march<2, true, 1, false, 1, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
+            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, true, 1, false, 1, R, MX>), 1, false);
+            hipLaunchKernelGGL((k_flux_march<2, true, 1, false, 1, R, MX>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
                                c->stream, lv, Uin, F[1], h, sl, c->d_clk, z, z, zf, zf, 0u);
         } else if (d == 1) {
-            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, false, 1, false, 1, R>), 1, false);
-            hipLaunchKernelGGL((k_flux_march<2, false, 1, false, 1, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
+            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, false, 1, false, 1, R, MX>), 1, false);
+            hipLaunchKernelGGL((k_flux_march<2, false, 1, false, 1, R, MX>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0,
                                c->stream, lv, Uin, F[1], h, sl, c->d_clk, z, z, zf, zf, 0u);
         } else {
-            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, false, 2, false, 1, R>), 2, false);
-            hipLaunchKernelGGL((k_flux_march<2, false, 2, false, 1, R>), dim3(march_grid(lv, 2, sl, false)), dim3(256), 0,
+            const int sl = seg_for(c, TJ_KPTR(k_flux_march<2, false, 2, false, 1, R, MX>), 2, false);
+            hipLaunchKernelGGL((k_flux_march<2, false, 2, false, 1, R, MX>), dim3(march_grid(lv, 2, sl, false)), dim3(256), 0,
                                c->stream, lv, Uin, F[2], h, sl, c->d_clk, z, z, zf, zf, 0u);
         }
     }
+}
+
+// the pass for the level's material set (MX: a mixture material)
+template <typename R>
+static void launch_pass(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, const Hist& h) {
+    if (c->lv.mixm >= 0) launch_pass_t<R, true>(c, Uin, d, F, h);
+    else launch_pass_t<R, false>(c, Uin, d, F, h);
 }
 
 }  // extern "C++"
@@ -722,7 +771,7 @@ static void launch_fused(tj_ctx* c, const MF& Uin, int d, const Hist& h, const F
 static int launch_fluxes(tj_ctx* c, const MF& Uin, const FuseArgs* fz, bool* fused, int prec) {
     const Level& lv = c->lv;
     const bool mixed = prec == TJ_PREC_MIXED;
-    const bool fuse = fz && c->fuse && !mixed && !lv.solid && lv.dim >= 2 && fz->out.p != Uin.p;
+    const bool fuse = fz && c->fuse && !mixed && !lv.solid && lv.mixm < 0 && lv.dim >= 2 && fz->out.p != Uin.p;
     if (fused) *fused = fuse;
     for (int d = 0; d < lv.dim; ++d) {
         const Hist h = hist_for(c, d);
@@ -780,12 +829,20 @@ int tj_post_step(tj_ctx* c, tj_mf* U, unsigned ops) {
     if (c->lv.solid) {
         hipLaunchKernelGGL(k_post_s<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk, ops);
     } else {
-        if (ops & TJ_OP_RADIAL)
+        const bool mx = c->lv.mixm >= 0;
+        if ((ops & TJ_OP_REACTIVE) && mx)  // reactiveUpdate acts on mixtures only (reactive.cpp:64)
+            hipLaunchKernelGGL(k_reactive_fluid<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+        if ((ops & TJ_OP_RADIAL) && mx)
+            hipLaunchKernelGGL((k_radial_fluid<2, true>), dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+        else if (ops & TJ_OP_RADIAL)
             hipLaunchKernelGGL(k_radial_fluid<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
         if (ops & TJ_OP_CLEAN_ALPHA)
             hipLaunchKernelGGL(k_cleanup_alpha<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
         if (ops & TJ_OP_NANCHECK)
-            hipLaunchKernelGGL(k_nancheck<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+        {
+            if (mx) hipLaunchKernelGGL((k_nancheck<2, true>), dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+            else hipLaunchKernelGGL(k_nancheck<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+        }
         if (ops & TJ_OP_DT) launch_sound_dt(c, view(U));
     }
     HIPCHK(c, hipGetLastError());
@@ -925,7 +982,10 @@ int tj_init_from_prims(tj_ctx* c, tj_mf* U, int shift) {
     if (c->lv.solid)
         hipLaunchKernelGGL(k_init_p2c_s<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
     else
-        hipLaunchKernelGGL(k_init_p2c<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
+        if (c->lv.mixm >= 0)
+            hipLaunchKernelGGL((k_init_p2c<2, true>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
+        else
+            hipLaunchKernelGGL(k_init_p2c<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
 }
@@ -937,7 +997,10 @@ int tj_c2p(tj_ctx* c, tj_mf* U) {
     if (c->lv.solid)
         hipLaunchKernelGGL(k_c2p_s<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
     else
-        hipLaunchKernelGGL(k_c2p<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+        if (c->lv.mixm >= 0)
+            hipLaunchKernelGGL((k_c2p<2, true>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
+        else
+            hipLaunchKernelGGL(k_c2p<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
 }

@@ -74,19 +74,26 @@ int main(int argc, char** argv) {
     parameters.THINCbeta = desc.THINCbeta;
     parameters.max_grid_size = desc.max_grid_size;
     parameters.numberOfMaterials = desc.nmat;
+    parameters.REACTIVE = desc.REACTIVE;
     std::vector<std::unique_ptr<MieGruneisenEOS>> eos;
     for (int m = 0; m < desc.nmat; ++m) {
         const tj_material& t = desc.mat[m];
-        if (t.eos == 1)
+        if (t.eos == 1) {
             eos.emplace_back(new UsUpSolidEOS(t.rho0, t.gamma - 1.0, t.c0, t.s, t.G0, t.yield_stress));
-        else {
-            eos.emplace_back(new MieGruneisenEOS(t.gamma, t.CV, t.pref, t.eref));
+        } else if (t.mixture) {  // initialise.cpp:183-193: MixtureEOS::define with the eight numbers
+            auto* mx = new MixtureEOS();
+            mx->define({t.gamma, t.pref, t.eref, t.CV, t.gamma_mix, t.pref_mix, t.eref_mix, t.CV_mix});
+            eos.emplace_back(mx);
+            parameters.numberOfMixtures = 1;
+        } else {
+            eos.emplace_back(new MieGruneisenEOS(t.gamma, t.pref, t.eref, t.CV));
             eos.back()->shearModulus = t.G0;
             eos.back()->yieldStress = t.yield_stress;
         }
         MaterialDescriptor md;
         md.phase = t.solid ? solid : fluid;
         md.plastic = t.plastic;
+        md.mixture = t.mixture != 0;
         md.EOS = eos.back().get();
         parameters.materialInfo.push_back(md);
     }
@@ -149,7 +156,7 @@ int main(int argc, char** argv) {
                             }
                         const int off = c - first;
                         const bool tensor = var == SIGMA || var == V_TENSOR || var == VSTAR || var == DEVH;
-                        const bool vec = var == RHOU || var == VELOCITY;
+                        const bool vec = var == RHOU || var == VELOCITY || var == RHO_MIX;
                         Real& ref = tensor ? baca(i, j, k, var, 0, off / 3, off % 3)
                                            : (vec ? baca(i, j, k, var, 0, off) : baca(i, j, k, var, off));
                         ref = prims[(size_t)c * (size_t)nc + (size_t)cell];
@@ -178,6 +185,10 @@ int main(int argc, char** argv) {
         U = U1;
 
         advance(U, U1, U2, MUSCLgrad, UL, UR, ULStar, URStar, UStarStar, flux_arr, geom, parameters, bc, THINCArr);
+
+        if (parameters.REACTIVE) {
+            reactiveUpdate(U, U1, U2, parameters);
+        }
 
         if (parameters.PLASTIC) {
             plastic.plasticUpdate(U1, parameters);
