@@ -4,11 +4,14 @@ geometry in place of the reference's hard-coded chooseStateBasedOnInitialConditi
 (initialise.cpp:387-581).
 
 Supported: fluid materials with EOS "MieGruneisen" (stiffened gas /
-constant-Gamma Mie-Gruneisen: adiabaticIndex, pref, eref, CV) and solid
-materials with EOS "UsUpSolid" (rho0, Gamma0, c0, s, G0, yieldStress; this
-project's Us-up Mie-Gruneisen solid, which has no reference class); RADIAL
-(initialise.cpp:332) switches the geometric source on.  Mixtures, REACTIVE
-and the Romenskii and Wilkins solids are rejected with an error.  Host-side
+constant-Gamma Mie-Gruneisen: adiabaticIndex, pref, eref, CV), one of them
+optionally a mixture (mixture = 1: MixtureEOS with the products'
+adiabaticIndexmix, prefmix, erefmix, CVmix, initialise.cpp:183-193, and per
+state lambda and rhoa, :245-249); solid materials with EOS "UsUpSolid" (rho0,
+Gamma0, c0, s, G0, yieldStress; this project's Us-up Mie-Gruneisen solid,
+which has no reference class); RADIAL and REACTIVE (initialise.cpp:332-335)
+switch the geometric and the reactive source on.  A second mixture and the
+Romenskii and Wilkins solids are rejected with an error.  Host-side
 set-up only.
 """
 from __future__ import annotations
@@ -19,7 +22,8 @@ from typing import Callable, Optional
 
 import numpy as np
 
-from .configs import (ALPHA, EPSILON, P, RHO_K, V_TENSOR, VELOCITY, Material, Problem, _empty_prims)
+from .configs import (ALPHA, EPSILON, LAMBDA, P, RHO_K, RHO_MIX, V_TENSOR, VELOCITY, Material, Problem,
+                      _empty_prims)
 
 # ---------------------------------------------------------------- libconfig subset
 _TOKEN = re.compile(r"""
@@ -147,20 +151,22 @@ def problem_from_settings(cfg, max_grid_size: int = 64, dim: Optional[int] = Non
     if dim is None:
         dim = 3 if n[2] > 1 else (2 if n[1] > 1 else 1)
     nm = int(cfg["materials"])
-    if int(cfg.get("mixtures", 0)) != 0:
-        raise ValueError("settings: mixtures are not part of this path")
-    if int(cfg.get("REACTIVE", 0)) != 0:
-        raise ValueError("settings: REACTIVE (reactiveUpdate with mixture EOS) is not part of this path")
+    if int(cfg.get("mixtures", 0)) > 1:
+        raise ValueError("settings: at most one mixture (every mixture shares mixtureIndex 0, initialise.cpp:187)")
     solid_run = int(cfg.get("SOLID", 0))
     plastic_run = int(cfg.get("PLASTIC", 0))
     mats = []
     for m in range(nm):
         md = cfg["listOfMaterials"][m]
-        if int(md.get("mixture", 0)):
-            raise ValueError("settings: mixtures are not part of this path")
         eos = md.get("EOS", "")
         phase = md.get("material", "fluid")
-        if eos == "MieGruneisen" and phase == "fluid":
+        if int(md.get("mixture", 0)):
+            if eos != "MieGruneisen" or phase != "fluid":
+                raise ValueError(f"settings: material {m}: only a MieGruneisen fluid can be a mixture")
+            mats.append(Material.mixture_mg(
+                (float(md["adiabaticIndex"]), float(md["pref"]), float(md["eref"]), float(md["CV"])),
+                (float(md["adiabaticIndexmix"]), float(md["prefmix"]), float(md["erefmix"]), float(md["CVmix"]))))
+        elif eos == "MieGruneisen" and phase == "fluid":
             mats.append(Material(gamma=float(md["adiabaticIndex"]), pref=float(md["pref"]), eref=float(md["eref"]),
                                  CV=float(md["CV"])))
         elif eos == "UsUpSolid" and phase == "solid":
@@ -175,7 +181,9 @@ def problem_from_settings(cfg, max_grid_size: int = 64, dim: Optional[int] = Non
                  PLASTIC=plastic_run, MUSCL=int(cfg.get("MUSCL", 1)), THINCbeta=float(cfg.get("THINCbeta", 1.6)),
                  CFL=float(cfg.get("CFL", 0.5)), bc_lo=tuple(low[:3]), bc_hi=tuple(high[:3]),
                  final_time=float(cfg["finalTime"]) if "finalTime" in cfg else None, shift_pressure=1,
-                 RADIAL=int(cfg.get("RADIAL", 0)))
+                 RADIAL=int(cfg.get("RADIAL", 0)), REACTIVE=int(cfg.get("REACTIVE", 0)))
+    if sum(mt.mixture for mt in mats) > 1:
+        raise ValueError("settings: at most one mixture material")
     # states (initialise.cpp:236-263) and the per-cell choice
     states = cfg["listOfStates"][: int(cfg.get("states", len(cfg["listOfStates"])))]
     nz, ny, nx = pb.shape
@@ -194,6 +202,11 @@ def problem_from_settings(cfg, max_grid_size: int = 64, dim: Optional[int] = Non
             if mats[m].solid:
                 F = np.asarray(sm["F"], dtype=np.float64).reshape(3, 3)
                 U[Lay.comp(RHO_K, m)][mask] = mats[m].rho0 / np.linalg.det(F)  # setRhoFromDeformationTensor
+            elif mats[m].mixture:  # initialise.cpp:66-71: rho_k and both phase densities = rhoa
+                U[Lay.comp(RHO_K, m)][mask] = float(sm["rhoa"])
+                U[Lay.comp(RHO_MIX, m, 0)][mask] = float(sm["rhoa"])
+                U[Lay.comp(RHO_MIX, m, 1)][mask] = float(sm["rhoa"])
+                U[Lay.comp(LAMBDA, m)][mask] = float(sm["lambda"])
             else:
                 U[Lay.comp(RHO_K, m)][mask] = float(sm["rho"])
             if plastic_run and solid_run:

@@ -160,3 +160,35 @@ def test_reactive_tube_runs_and_burns():
     s = o.get()
     assert o.status() == 0 and np.isfinite(s[L.state_comps()]).all()
     assert s[L.comp(C.LAMBDA)].mean() < lam0.mean()
+
+
+def test_settings_deck_with_a_reactive_mixture():
+    """A reference-format deck with mixtures = 1, REACTIVE = 1, the products'
+    *mix constants and per-state lambda / rhoa gives the same problem as the
+    configs set-up, and the oracle runs it."""
+    from tenjin_amd.settings import problem_from_settings
+
+    a, b = REACT, PROD
+    deck = f"""
+    numberOfxCells = 200; xdomainLength = 1.0; finalTime = 1e-5; materials = 2; states = 2; mixtures = 1;
+    REACTIVE = 1; THINC = 0; lowBoundary = [0]; highBoundary = [0]; interface = 0.1;
+    listOfMaterials = ( {{ EOS = "MieGruneisen"; material = "fluid"; mixture = 1;
+                           adiabaticIndex = {a[0]}; pref = {a[1]}; eref = {a[2]}; CV = {a[3]};
+                           adiabaticIndexmix = {b[0]}; prefmix = {b[1]}; erefmix = {b[2]}; CVmix = {b[3]}; }},
+                        {{ EOS = "MieGruneisen"; material = "fluid"; adiabaticIndex = 1.4; pref = 0.0; eref = 0.0; CV = 1.0; }} );
+    listOfStates = ( {{ p = 2e9; u = 0.0; material = ( {{ alpha = 0.999999; lambda = 0.2; rhoa = 1000.0; }}, {{ alpha = 1e-6; rho = 1.2; }} ); }},
+                     {{ p = 1e5; u = 0.0; material = ( {{ alpha = 0.999999; lambda = 0.9999; rhoa = 1000.0; }}, {{ alpha = 1e-6; rho = 1.2; }} ); }} );
+    """
+    # (lambda = 1 exactly gives the products' density 0/0 in defineMixtureDensities,
+    # equationofstate.cpp:364-370, as it does in the reference)
+    pb = problem_from_settings(deck)
+    L = pb.layout
+    assert pb.REACTIVE == 1 and pb.nmix == 1 and pb.materials[0].mixture == 1
+    assert pb.materials[0].pref2 == b[1] and pb.materials[0].CV2 == b[3]
+    lam = pb.prims[L.comp(C.LAMBDA)][0, 0]
+    assert lam[0] == 0.2 and lam[-1] == 0.9999
+    assert np.all(pb.prims[L.comp(C.RHO_MIX, 0, 1)] == 1000.0)
+    pb.shift_pressure = 0
+    o = Oracle(pb)
+    n, t = o.run(1e-5, 50)
+    assert o.status() == 0 and n > 0

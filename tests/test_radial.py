@@ -173,6 +173,4 @@ def test_settings_deck_radial_switch():
                      { p = 1e5;    u = 0.0; material = ( { alpha = 0.999999; rho = 1000.0; }, { alpha = 1e-6; rho = 1270.0; } ); } );
     """
     pb = problem_from_settings(deck)
-    assert pb.RADIAL == 1 and pb.dim == 1
-    with pytest.raises(ValueError, match="REACTIVE"):
-        problem_from_settings(deck.replace("RADIAL = 1;", "RADIAL = 1; REACTIVE = 1;"))
+    assert pb.RADIAL == 1 and pb.dim == 1 and pb.REACTIVE == 0

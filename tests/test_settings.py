@@ -88,6 +88,10 @@ def test_custom_geometry_and_rejections():
     with pytest.raises(ValueError):
         settings.problem_from_settings(bad)
     bad = settings.parse(DECK)
-    bad["mixtures"] = 1
+    bad["mixtures"] = 2  # every mixture would share mixtureIndex 0
     with pytest.raises(ValueError):
+        settings.problem_from_settings(bad)
+    bad = settings.parse(DECK)
+    bad["listOfMaterials"][1]["mixture"] = 1  # a mixture needs the products' *mix constants
+    with pytest.raises(KeyError):
         settings.problem_from_settings(bad)
