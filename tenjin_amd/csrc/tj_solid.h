@@ -76,45 +76,49 @@ __device__ __forceinline__ R hencky(const R* V, R* dH) {
 // Principal square root of an SPD 3x3 matrix by cyclic Jacobi rotations
 // (Golub & Van Loan, Matrix Computations, 8.5), six sweeps; the oracle
 // evaluates the same sequence.
-__device__ inline void sqrtm_spd(const double* Ain, double* S) {
-    double a[9], q[9];
+template <typename R = double>
+__device__ inline void sqrtm_spd(const R* Ain, R* S) {
+    R a[9], q[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         a[i] = Ain[i];
-        q[i] = (i / 3 == i % 3) ? 1.0 : 0.0;
+        q[i] = (i / 3 == i % 3) ? R(1) : R(0);
     }
     for (int sweep = 0; sweep < 6; ++sweep) {
 #pragma unroll
         for (int rr = 0; rr < 3; ++rr) {
             const int p = rr == 2 ? 1 : 0, qq = rr == 0 ? 1 : 2;
-            const double apq = a[p * 3 + qq];
-            if (apq == 0.0) continue;
-            const double app = a[p * 3 + p], aqq = a[qq * 3 + qq];
-            const double theta = (aqq - app) / (2.0 * apq);
-            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (__builtin_fabs(theta) + __builtin_sqrt(theta * theta + 1.0));
-            const double c = 1.0 / __builtin_sqrt(t * t + 1.0);
-            const double s = t * c;
+            const R apq = a[p * 3 + qq];
+            if (apq == R(0)) continue;
+            const R app = a[p * 3 + p], aqq = a[qq * 3 + qq];
+            // FP32: an off-diagonal below the diagonal's resolution rotates by
+            // nothing representable, and its reciprocal can overflow
+            if (sizeof(R) == 4 && tabs(apq) < R(1e-12) * (tabs(app) + tabs(aqq))) continue;
+            const R theta = tdiv(aqq - app, R(2) * apq);
+            const R t = tdiv(theta >= R(0) ? R(1) : R(-1), tabs(theta) + tsqrt(theta * theta + R(1)));
+            const R c = tdiv(R(1), tsqrt(t * t + R(1)));
+            const R s = t * c;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const double akp = a[k * 3 + p], akq = a[k * 3 + qq];
+                const R akp = a[k * 3 + p], akq = a[k * 3 + qq];
                 a[k * 3 + p] = c * akp - s * akq;
                 a[k * 3 + qq] = s * akp + c * akq;
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const double apk = a[p * 3 + k], aqk = a[qq * 3 + k];
+                const R apk = a[p * 3 + k], aqk = a[qq * 3 + k];
                 a[p * 3 + k] = c * apk - s * aqk;
                 a[qq * 3 + k] = s * apk + c * aqk;
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const double qkp = q[k * 3 + p], qkq = q[k * 3 + qq];
+                const R qkp = q[k * 3 + p], qkq = q[k * 3 + qq];
                 q[k * 3 + p] = c * qkp - s * qkq;
                 q[k * 3 + qq] = s * qkp + c * qkq;
             }
         }
     }
-    const double sl0 = __builtin_sqrt(a[0]), sl1 = __builtin_sqrt(a[4]), sl2 = __builtin_sqrt(a[8]);
+    const R sl0 = tsqrt(a[0]), sl1 = tsqrt(a[4]), sl2 = tsqrt(a[8]);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -585,23 +589,29 @@ __device__ inline unsigned plastic_cell(const Level& lv, const double* al, const
     return st;
 }
 
-// cleanUpV of one cell (boxaccesscellarray.cpp:476-528)
-template <int NM>
+// cleanUpV of one cell (boxaccesscellarray.cpp:476-528).  R = double in the
+// FP64 form; the mixed form evaluates it in FP32 (the square root's Jacobi
+// sweeps are its cost; the stored V stays FP64).
+__device__ __forceinline__ double trcbrt(double x) { return pm_rcbrt(x); }
+__device__ __forceinline__ float trcbrt(float x) { return __builtin_amdgcn_rcpf(cbrtf(x)); }
+template <int NM, typename R = double>
 __device__ inline void cleanup_v(const Level& lv, const double* al, double* V) {
-    double ta = 0.0, tsd = 0.0;
+    R ta = R(0), tsd = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        ta += al[m];
-        if (lv.m[m].solid) tsd += al[m];
+        ta += (R)al[m];
+        if (lv.m[m].solid) tsd += (R)al[m];
     }
-    double G[9], S[9];
-    mmT3(V, V, G);
-    sqrtm_spd(G, S);
-    const double norm = pm_rcbrt(det3(S));
+    R Vr[9], G[9], S[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Vr[q] = (R)V[q];
+    mmT3(Vr, Vr, G);
+    sqrtm_spd<R>(G, S);
+    const R norm = trcbrt(det3(S));
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        const double dl = (q / 3 == q % 3) ? 1.0 : 0.0;
-        V[q] = ((S[q] * norm * tsd) + dl * (ta - tsd)) / ta;
+        const R dl = (q / 3 == q % 3) ? R(1) : R(0);
+        V[q] = (double)tdiv((S[q] * norm * tsd) + dl * (ta - tsd), ta);
     }
 }
 

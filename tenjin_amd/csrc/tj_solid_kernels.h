@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
                 C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
             }
             if (ops & 32u) radial_solid_c2p<NM>(lv, radial_factor(lv, i, dt), C, dH, (ops & 1u) != 0);
-            if (ops & 2u) cleanup_v<NM>(lv, C.al, C.V);
+            if (ops & 2u) cleanup_v<NM, T>(lv, C.al, C.V);  // FP32 in the mixed form
             if (ops & 4u) st |= cleanup_alpha<NM>(C.al);
             if (ops & 16u) {
                 bool bad = false;
