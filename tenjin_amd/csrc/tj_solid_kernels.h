@@ -128,9 +128,12 @@ __device__ __forceinline__ void sst_load(const R* sh, int lane, SSt<NM, R>& S) {
 
 // y / z pass of the solid path: lanes along x (coalesced), marching along D
 // over a segment of faces.  The right state of row r-1 waits for face r in
-// the wave's own LDS slice (not in registers), and the three primitive rows
-// of each row are reloaded from cache: one reconstruction per cell as in a
-// register march, with the register footprint of the x pass.
+// the wave's own LDS slice (not in registers).  The FP64 form reloads the
+// three primitive rows of each row from cache, keeping the register
+// footprint of the x pass; the FP32 (mixed) form slides them -- rows r-1 and
+// r stay in registers and row r+1 is the one load per row -- because its
+// reloads miss the L2 (a 4 MB L2 per XCD against the rows of all resident
+// waves) and made the pass bandwidth-bound.
 template <int NM, int D, typename R>
 __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U, const FacesT<R>& F, int seglen, Clock* clk) {
     if (clk->done) return;
@@ -180,9 +183,25 @@ __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U,
 #pragma unroll
         for (int q = 0; q < NF; ++q) F.p[q * F.cs + fi] = Fv[q];
     };
+    constexpr bool SLIDE = sizeof(R) == 4;
+    SPrim<NM, R> W0, W1;  // rows r-1 and r (sliding form)
+    if (SLIDE) {
+        load_sprim<NM, R>(lv, U, i, cj(rb - 1), ck(rb - 1), W0);
+        load_sprim<NM, R>(lv, U, i, cj(rb), ck(rb), W1);
+    }
     for (int r = rb; r <= re; ++r) {
         SSt<NM, R> SLs, SRs;
-        recon(r, SLs, SRs);
+        if (SLIDE) {
+            SPrim<NM, R> W2, PL, PR;
+            load_sprim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), W2);
+            muscl_s<NM, R>(W0, W1, W2, PL, PR);
+            p2c_sound_s<NM, R>(lv, PL, D, SLs);
+            p2c_sound_s<NM, R>(lv, PR, D, SRs);
+            W0 = W1;
+            W1 = W2;
+        } else {
+            recon(r, SLs, SRs);
+        }
         if (r >= f0) {  // face r: left UR(r-1) (the image of UR(0) at the domain face), right UL(r)
             SSt<NM, R> A;
             if (r == 0) {
