@@ -128,12 +128,12 @@ __device__ __forceinline__ void sst_load(const R* sh, int lane, SSt<NM, R>& S) {
 
 // y / z pass of the solid path: lanes along x (coalesced), marching along D
 // over a segment of faces.  The right state of row r-1 waits for face r in
-// the wave's own LDS slice (not in registers).  The FP64 form reloads the
-// three primitive rows of each row from cache, keeping the register
-// footprint of the x pass; the FP32 (mixed) form slides them -- rows r-1 and
-// r stay in registers and row r+1 is the one load per row -- because its
-// reloads miss the L2 (a 4 MB L2 per XCD against the rows of all resident
-// waves) and made the pass bandwidth-bound.
+// the wave's own LDS slice (not in registers).  The primitive rows slide:
+// rows r-1 and r stay in registers and row r+1 is the one load per row.
+// Reloading all three rows from cache missed the L2 (a 4 MB L2 per XCD
+// against the rows of all resident waves): 565 B/cell of HBM traffic against
+// 336, which made the FP32 (mixed) pass bandwidth-bound (measured: FP32 pass
+// 1.6 -> 1.1 ms, FP64 3.2 -> 3.0 ms on 256^3).
 template <int NM, int D, typename R>
 __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U, const FacesT<R>& F, int seglen, Clock* clk) {
     if (clk->done) return;
@@ -183,7 +183,7 @@ __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U,
 #pragma unroll
         for (int q = 0; q < NF; ++q) F.p[q * F.cs + fi] = Fv[q];
     };
-    constexpr bool SLIDE = sizeof(R) == 4;
+    constexpr bool SLIDE = true;
     SPrim<NM, R> W0, W1;  // rows r-1 and r (sliding form)
     if (SLIDE) {
         load_sprim<NM, R>(lv, U, i, cj(rb - 1), ck(rb - 1), W0);
