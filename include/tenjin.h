@@ -94,8 +94,10 @@ enum {
     TJ_ST_ALPHA_SUM = 8,    /* total alpha <= 0 in cleanUpAlpha (boxaccesscellarray.cpp:583-588) */
     TJ_ST_PLASTIC_DIV = 16, /* "Divide by zero error" in boxPlasticUpdate (plastic.cpp:220-224) */
     TJ_ST_MIX_BRACKET = 32, /* "Error in root Bisection" in MixtureEOS::rootFind (equationofstate.cpp:313-321) */
-    TJ_ST_MIX_ITER = 64     /* rootFind's bisection reached its bound of 2000 passes (the reference
+    TJ_ST_MIX_ITER = 64,    /* rootFind's bisection reached its bound of 2000 passes (the reference
                                loops without one, equationofstate.cpp:324-353) */
+    TJ_ST_TANGENT_NAN = 128 /* a NaN derivative part in the tangent-linear form (no reference
+                               counterpart; the run goes on) */
 };
 
 /* precision forms */
@@ -256,6 +258,42 @@ int tj_batch_graph_launch(tj_batch* batch);
 /* Per-problem t, n and status words (arrays of nprob; any may be NULL). Synchronises. */
 int tj_batch_clocks(tj_batch* batch, double* t, int* n, uint32_t* status);
 int tj_batch_synchronize(tj_batch* batch);
+
+/* ---- Tangent-linear (differentiated) form -----------------------------
+ * Forward-mode derivatives of the solid path's time loop (main.cpp:121-191
+ * with advance, plasticUpdate, cleanUpV, cleanUpAlpha and getTimeStep, as
+ * tj_run) with respect to per-material parameters of the Us-up solid EOS and
+ * the perfect-plastic return: every value is the FP64 form's, bit for bit,
+ * and each direction carries d(state)/d(parameter) through init, flux, update,
+ * plastic, V clean-up and the CFL time step.  The reference has no
+ * differentiated form; this serves gradient-based calibration of material
+ * parameters against probe traces.
+ * Restrictions: the bound level of ctx is SOLID (no RADIAL); directions name
+ * Us-up solid materials (TJ_TP_YIELD: plastic ones).  Directions run in
+ * groups of `chunk` (1, 2, 3 or 6; 0 = all in one group up to 6): each group
+ * re-runs the trajectory from the initial primitives.
+ * On an error from tj_tangent_create, *tg is NULL. */
+enum {
+    TJ_TP_GAMMA0 = 0, /* Gruneisen coefficient Gamma0 (adiabaticIndex - 1) */
+    TJ_TP_S = 1,      /* Us-up Hugoniot slope s */
+    TJ_TP_YIELD = 2   /* yield stress (plastic.cpp:132-135) */
+};
+typedef struct tj_tangent tj_tangent;
+int tj_tangent_create(tj_ctx* ctx, int ndir, const int* mat, const int* param, int chunk, tj_tangent** tg);
+void tj_tangent_destroy(tj_tangent* tg);
+/* From the primitive state P0 (as given to tj_init_from_prims, shift_pressure
+ * alike): for each group, copy P0 to S, initialise, begin a run to final_time
+ * and advance nsteps loop bodies (S state, T stage buffer; both caller-owned
+ * views of the bound layout).  After every step the value and derivatives of
+ * component probe_comp of cell (pi, pj, pk) go to row n of trace (host,
+ * nsteps x (1 + ndir) doubles: value, then one column per direction; may be
+ * NULL).  *steps_done: loop bodies run (fewer than nsteps after the final
+ * time or a NaN stop).  S ends in the final state. */
+int tj_tangent_run(tj_tangent* tg, const tj_mf* P0, int shift_pressure, tj_mf* S, tj_mf* T, double final_time,
+                   int nsteps, int probe_comp, int pi, int pj, int pk, double* trace, int* steps_done);
+/* The final-state derivative of direction dir (of the last group run) into
+ * the caller's device view D of the bound layout. */
+int tj_tangent_state(tj_tangent* tg, int dir, tj_mf* D);
 
 #ifdef __cplusplus
 }

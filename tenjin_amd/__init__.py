@@ -23,6 +23,8 @@ TJ_PREC_MIXED = 1
 TJ_ST_NAN, TJ_ST_WAVESPEED, TJ_ST_ALPHA_NANS, TJ_ST_ALPHA_SUM, TJ_ST_PLASTIC_DIV = 1, 2, 4, 8, 16
 TJ_OP_PLASTIC, TJ_OP_CLEAN_V, TJ_OP_CLEAN_ALPHA, TJ_OP_DT, TJ_OP_NANCHECK, TJ_OP_RADIAL, TJ_OP_REACTIVE = 1, 2, 4, 8, 16, 32, 64
 TJ_ST_MIX_BRACKET, TJ_ST_MIX_ITER = 32, 64
+TJ_ST_TANGENT_NAN = 128
+TJ_TP_GAMMA0, TJ_TP_S, TJ_TP_YIELD = 0, 1, 2
 
 
 class Material(ctypes.Structure):
@@ -153,6 +155,11 @@ def load_library(path: str = LIB_PATH):
     L.tj_batch_graph_launch.argtypes = [vp]
     L.tj_batch_clocks.argtypes = [vp, ctypes.POINTER(dp), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_uint32)]
     L.tj_batch_synchronize.argtypes = [vp]
+    L.tj_tangent_create.argtypes = [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip), ip, ctypes.POINTER(vp)]
+    L.tj_tangent_destroy.argtypes = [vp]
+    L.tj_tangent_destroy.restype = None
+    L.tj_tangent_run.argtypes = [vp, mfp, ip, mfp, mfp, dp, ip, ip, ip, ip, ip, ctypes.POINTER(dp), ctypes.POINTER(ip)]
+    L.tj_tangent_state.argtypes = [vp, ip, mfp]
     _lib = L
     return L
 

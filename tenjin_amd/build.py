@@ -1,4 +1,5 @@
-"""Build step of the gfx950 library: compiles tenjin_amd/csrc/*.hip into
+"""Build step of the gfx950 library: compiles tenjin_amd/csrc/*.hip (the
+FP64 / mixed forms and the tangent-linear form) into
 tenjin_amd/libtenjin.so with hipcc (in-tree, so the built library travels with
 the source tree), the C++ host interface tenjin_amd/csrc/host/ into
 tenjin_amd/libtenjin_host.so, and the C++ time-loop driver used by the tests
@@ -39,7 +40,7 @@ FLAGS = [
 
 
 def sources():
-    return [os.path.join(CSRC, "tj_lib.hip")]
+    return [os.path.join(CSRC, "tj_lib.hip"), os.path.join(CSRC, "tj_tangent.hip")]
 
 
 def _run(cmd, verbose):
@@ -49,8 +50,24 @@ def _run(cmd, verbose):
 
 
 def build(verbose: bool = True, extra=None) -> str:
-    cmd = [HIPCC] + FLAGS + list(extra or []) + sources() + ["-I", INCLUDE, "-o", LIB + ".tmp"]
-    _run(cmd, verbose)
+    """Each translation unit compiles in its own process (objects in a
+    temporary folder), then one link makes the shared library."""
+    import tempfile
+
+    comp = [f for f in FLAGS if f != "-shared"] + list(extra or [])
+    with tempfile.TemporaryDirectory(prefix="tenjin_build_") as tmp:
+        objs, procs = [], []
+        for src in sources():
+            obj = os.path.join(tmp, os.path.basename(src) + ".o")
+            cmd = [HIPCC] + comp + ["-c", src, "-I", INCLUDE, "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd)))
+            objs.append(obj)
+        failed = [cmd for cmd, p in procs if p.wait() != 0]
+        if failed:
+            raise subprocess.CalledProcessError(1, failed[0])
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -361,6 +361,46 @@ def plate_impact(dim: int = 3, n: int = 256, box: int = 64, seed: int = 3, speed
     return pb
 
 
+PLATE_PARAMS = ("Gamma0", "s", "Y")  # per material: Us-up Gruneisen coefficient, Hugoniot slope, yield stress
+
+
+def plate_params(cu=None, al=None) -> list:
+    """[Gamma0, s, Y] of copper then aluminium (the differentiated parameters)."""
+    cu = dict(COPPER, **(cu or {}))
+    al = dict(ALUMINIUM, **(al or {}))
+    return [cu[k] for k in PLATE_PARAMS] + [al[k] for k in PLATE_PARAMS]
+
+
+def plate_impact_differentiated(n: int = 512, box: int = 64, seed: int = 4, theta=None, steps: int = 200) -> Problem:
+    """The 2D elastoplastic plate impact of the differentiated form: the
+    copper/aluminium geometry of plate_impact in 2D, n x n cells, impact speed
+    U(200, 600) from `seed` (the density noise uses the same seed).  theta:
+    [Gamma0, s, Y] of copper then aluminium (default the tabulated values).
+    The probe is u_x of the cell two in from the aluminium's far (x high)
+    face at mid-height, recorded after every step."""
+    speed = 200.0 + 400.0 * float(_mt_uniform(seed, 1)[0])
+    th = plate_params() if theta is None else [float(v) for v in theta]
+    cu = dict(zip(PLATE_PARAMS, th[:3]))
+    al = dict(zip(PLATE_PARAMS, th[3:6]))
+    pb = plate_impact(dim=2, n=n, box=box, seed=seed, speed=speed, steps=steps, cu=cu, al=al)
+    pb.name = "plate_impact_differentiated_2d"
+    pb.speed = speed
+    return pb
+
+
+def plate_probe(pb: Problem):
+    """(component, i, j, k) of the differentiated plate impact's probe."""
+    nz, ny, nx = pb.shape
+    return pb.layout.comp(VELOCITY, 0, 0), nx - 3, ny // 2, 0
+
+
+def plate_target_theta(seed: int = 5) -> list:
+    """The parameters of the target trace: each tabulated value times
+    1 + 0.05 U(-1, 1) from `seed`."""
+    u = _mt_uniform(seed, 6)
+    return [v * (1.0 + 0.05 * (2.0 * w - 1.0)) for v, w in zip(plate_params(), u)]
+
+
 def underwater_explosion(dim: int = 1, n: int = 800, box: int = 1 << 30, radius: float = 0.2,
                          thinc: int = 0) -> Problem:
     """High-pressure gas bubble (gamma 1.4, rho 1270, p 8.29e8) in water

@@ -1,0 +1,80 @@
+// Library context shared by the translation units of the C ABI (the FP64 /
+// mixed forms in tj_lib.hip, the tangent-linear form in tj_tangent.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tenjin.h"
+#include "tj_kernels.h"
+#include "tj_solid_kernels.h"
+
+using namespace tj;
+
+struct tj_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool bound = false;
+    tj_level_desc desc{};
+    Level lv{};
+    int nmat = 0;
+    uint8_t* d_edge[3] = {nullptr, nullptr, nullptr};
+    double* d_flux[3] = {nullptr, nullptr, nullptr};
+    Faces F[3]{};
+    // FP32 flux planes of the mixed-precision form (allocated on first use)
+    float* d_fluxf[3] = {nullptr, nullptr, nullptr};
+    FacesT<float> Ff[3]{};
+    int nfc = 0;  // flux components per face
+    double* d_urstar = nullptr;
+    double* d_star[2] = {nullptr, nullptr};
+    double* d_estar[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    long long estride[3] = {0, 0, 0};
+    Clock* d_clk = nullptr;
+    unsigned* d_status = nullptr;
+    long long pass = 0;
+    long long dpass[3] = {0, 0, 0};
+    // rows per marching segment: 0 = chosen per pass from the launch shape
+    // (pick_seglen, cached per kernel and direction), > 0 = fixed by the caller
+    int seglen = 0;
+    int ncu = 0;  // compute units of the device
+    std::map<std::pair<const void*, int>, int> seg_cache;
+    bool fuse = false;
+    // a captured run of loop bodies (tj_graph_capture) and the THINC pass
+    // counters it advances each launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    long long g_pass0 = 0, g_dpass0[3] = {0, 0, 0};
+    long long g_npass = 0, g_ndpass[3] = {0, 0, 0};
+    // per-kernel event timing
+    bool prof = false;
+    struct Rec { const char* name; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<long long, double>> acc;
+};
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+            return TJ_EHIP;                                                                 \
+        }                                                                                   \
+    } while (0)
+
+inline MF view(const tj_mf* m) { return MF{m->data, m->jstride, m->kstride, m->nstride}; }
+
+inline unsigned grid1(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+inline int check_mf(tj_ctx* c, const tj_mf* m) {
+    if (!m || !m->data) { c->err = "null tj_mf"; return TJ_EINVAL; }
+    if (m->ncomp < tj_ncomp(c)) { c->err = "tj_mf has fewer components than the bound layout"; return TJ_EINVAL; }
+    if (m->jstride < c->lv.n[0] || m->kstride < m->jstride * c->lv.n[1] || m->nstride < m->kstride * c->lv.n[2]) {
+        c->err = "tj_mf strides do not cover the level";
+        return TJ_EINVAL;
+    }
+    return TJ_OK;
+}

@@ -52,10 +52,15 @@ struct Level {
     int cs[NVAR];      // first component of each Variable (AccessPattern), -1 if absent
     long long ncells;
     int mixm;          // the mixture material (MixtureEOS), -1 if none
+    // materials with derivative parts (MatT<Dual<K>>[MAXM], device memory) for
+    // the tangent-linear form; null elsewhere
+    const void* md;
 };
 
 template <typename R>
-__device__ __forceinline__ const MatT<R>& mat(const Level& lv, int m);
+__device__ __forceinline__ const MatT<R>& mat(const Level& lv, int m) {
+    return static_cast<const MatT<R>*>(lv.md)[m];
+}
 template <>
 __device__ __forceinline__ const MatT<double>& mat<double>(const Level& lv, int m) { return lv.m[m]; }
 template <>

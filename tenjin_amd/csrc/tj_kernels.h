@@ -761,6 +761,7 @@ __global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
     sound_dt_body<NM, MX>(lv, U, clk, blockIdx.x);
 }
 
+#ifndef TJ_TEMPLATE_KERNELS_ONLY  // non-template kernels: one definition, in tj_lib.hip
 // dt = CFL * min (timestep.cpp:61-69); with `driver` the loop test and the
 // clamps of main.cpp:121, 135-143.  Re-arms the min word.
 __global__ void k_clock_dt(Clock* clk, double cfl, int driver) {
@@ -799,6 +800,7 @@ __global__ void k_clock_final(Clock* clk, double final_time) {
 __global__ void k_clock_clear_status(Clock* clk) {
     if (threadIdx.x == 0 && blockIdx.x == 0) clk->status = 0;
 }
+#endif
 
 // stand-alone cleanUpAlpha over the valid cells (boxaccesscellarray.cpp:530-599)
 template <int NM>
@@ -995,6 +997,7 @@ __global__ __launch_bounds__(256) void k_c2p(Level lv, MF U, Clock* clk) {
     if (st) atomicOr(&clk->status, st);
 }
 
+#ifndef TJ_TEMPLATE_KERNELS_ONLY  // non-template kernels: one definition, in tj_lib.hip
 // sigma = -p I for fluids (stressTensor, boxaccesscellarray.cpp:176-216).
 __global__ __launch_bounds__(256) void k_derive_fluid(Level lv, MF U) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
@@ -1031,5 +1034,6 @@ __global__ void k_zero(double* p, long long n) {
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; id < n; id += stride) p[id] = 0.0;
 }
+#endif
 
 }  // namespace tj

@@ -5,14 +5,15 @@
 // files cited beside each function.
 #pragma once
 #include "tj_fluid.h"
+#include "tj_dual.h"
 
 namespace tj {
 
 // ------------------------------------------------------------------ 3x3 algebra
 // tensor.cpp:81-115 (products: k = 0,1,2 summed left to right), 140-150
 // (trace), 187-233 (adjugate inverse), 296-333 (cofactor determinant).
-// R: double (FP64 form and every per-cell update) or float (face states and
-// fluxes of the mixed-precision form).
+// R: double (FP64 form and every per-cell update), float (face states and
+// fluxes of the mixed-precision form) or Dual<K> (tangent-linear form).
 template <typename R>
 __device__ __forceinline__ R det2(R a, R b, R c, R d) { return a * d - b * c; }
 template <typename R>
@@ -76,9 +77,10 @@ __device__ __forceinline__ R hencky(const R* V, R* dH) {
 // Principal square root of an SPD 3x3 matrix by cyclic Jacobi rotations
 // (Golub & Van Loan, Matrix Computations, 8.5), six sweeps; the oracle
 // evaluates the same sequence.
-template <typename R = double>
-__device__ inline void sqrtm_spd(const R* Ain, R* S) {
-    R a[9], q[9];
+// jacobi_spd leaves A = Q diag(a) Q^T (a[0], a[4], a[8]) with the rotations
+// accumulated in q; sqrtm_spd assembles Q diag(sqrt(a)) Q^T.
+template <typename R>
+__device__ __forceinline__ void jacobi_spd(const R* Ain, R* a, R* q) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         a[i] = Ain[i];
@@ -118,12 +120,59 @@ __device__ inline void sqrtm_spd(const R* Ain, R* S) {
             }
         }
     }
+}
+template <typename R = double>
+__device__ inline void sqrtm_spd(const R* Ain, R* S) {
+    R a[9], q[9];
+    jacobi_spd<R>(Ain, a, q);
     const R sl0 = tsqrt(a[0]), sl1 = tsqrt(a[4]), sl2 = tsqrt(a[8]);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             S[r * 3 + c] = (q[r * 3] * sl0 * q[c * 3] + q[r * 3 + 1] * sl1 * q[c * 3 + 1]) + q[r * 3 + 2] * sl2 * q[c * 3 + 2];
+}
+// Tangent form: the value by the FP64 sweeps above; the derivative from
+// S dS + dS S = dA (differentiating S S = A), solved in the eigenbasis of the
+// sweeps: dS = Q [ (Q^T dA Q)_rc / (sqrt(a_r) + sqrt(a_c)) ] Q^T.  Rotating
+// the derivative parts through the sweeps instead loses them to cancellation
+// when eigenvalues nearly coincide (V close to I).
+template <int K>
+__device__ inline void sqrtm_spd(const Dual<K>* Ain, Dual<K>* S) {
+    double Av[9], a[9], q[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Av[i] = Ain[i].v;
+    jacobi_spd<double>(Av, a, q);
+    const double sl[3] = {__builtin_sqrt(a[0]), __builtin_sqrt(a[4]), __builtin_sqrt(a[8])};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            S[r * 3 + c].v = (q[r * 3] * sl[0] * q[c * 3] + q[r * 3 + 1] * sl[1] * q[c * 3 + 1]) + q[r * 3 + 2] * sl[2] * q[c * 3 + 2];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        double T1[9], M[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                T1[r * 3 + c] = Ain[r * 3].d[t] * q[c] + Ain[r * 3 + 1].d[t] * q[3 + c] + Ain[r * 3 + 2].d[t] * q[6 + c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                M[r * 3 + c] = (q[r] * T1[c] + q[3 + r] * T1[3 + c] + q[6 + r] * T1[6 + c]) / (sl[r] + sl[c]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                T1[r * 3 + c] = q[r * 3] * M[c] + q[r * 3 + 1] * M[3 + c] + q[r * 3 + 2] * M[6 + c];  // Q M
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                S[r * 3 + c].d[t] = T1[r * 3] * q[c * 3] + T1[r * 3 + 1] * q[c * 3 + 1] + T1[r * 3 + 2] * q[c * 3 + 2];  // (Q M) Q^T
+    }
 }
 
 // ------------------------------------------------------------------ Us-up solid EOS
@@ -297,27 +346,26 @@ __device__ __forceinline__ void p2c_sound_s(const Level& lv, const SPrim<NM, R>&
 
 // conservativeToPrimitive (boxaccesscellarray.cpp:67-122) of a solid cell;
 // returns HJ2 and the deviatoric Hencky strain in dH.
-template <int NM>
-__device__ __forceinline__ double c2p_s(const Level& lv, const double* al, const double* ar, const double* are, const double* ru,
-                                        double E, const double* V, double* rk, double* e, double& rho, double* u, double& p,
-                                        double* dH) {
-    rho = 0.0;
+template <int NM, typename R>
+__device__ __forceinline__ R c2p_s(const Level& lv, const R* al, const R* ar, const R* are, const R* ru, R E, const R* V,
+                                   R* rk, R* e, R& rho, R* u, R& p, R* dH) {
+    rho = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        rk[m] = ar[m] / al[m];
+        rk[m] = tdiv(ar[m], al[m]);
         rho += ar[m];
-        e[m] = lv.m[m].plastic ? are[m] / ar[m] : 0.0;
+        e[m] = mat<R>(lv, m).plastic ? tdiv(are[m], ar[m]) : R(0);
     }
-    const double hj2 = hencky(V, dH);
-    double ke = 0.0;
+    const R hj2 = hencky(V, dH);
+    R ke = R(0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        u[q] = ru[q] / rho;
-        ke += 0.5 * rho * u[q] * u[q];
+        u[q] = tdiv(ru[q], rho);
+        ke += R(0.5) * rho * u[q] * u[q];
     }
-    double G, Ent, Pnt;
-    eos_sums_s<NM, double>(lv, al, ar, rk, hj2, G, Ent, Pnt);
-    p = (E - ke - Ent + Pnt) / G;
+    R G, Ent, Pnt;
+    eos_sums_s<NM, R>(lv, al, ar, rk, hj2, G, Ent, Pnt);
+    p = tdiv(E - ke - Ent + Pnt, G);
     return hj2;
 }
 
@@ -530,57 +578,57 @@ __device__ inline void wave5(const Level& lv, const SSt<NM, Rt>& L, const SSt<NM
 // ------------------------------------------------------------------ plastic
 // boxPlasticUpdate for one cell (plastic.cpp:83-225); V is overwritten with
 // the Cayley update even without yield.  Returns status bits.
-template <int NM>
-__device__ inline unsigned plastic_cell(const Level& lv, const double* al, const double* ar, double* e, double* are, double* V) {
-    double ts = 0.0;
+template <int NM, typename R>
+__device__ inline unsigned plastic_cell(const Level& lv, const R* al, const R* ar, R* e, R* are, R* V) {
+    R ts = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m)
-        if (lv.m[m].solid) ts += al[m];
+        if (mat<R>(lv, m).solid) ts += al[m];
     if (ts <= 0.5) {
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
-            e[m] = 0.0;
-            are[m] = 0.0;
+            e[m] = R(0);
+            are[m] = R(0);
         }
         return 0u;
     }
     const double s3 = __builtin_sqrt(3.0), s23 = __builtin_sqrt(2.0 / 3.0);
-    double dH[9];
-    const double hj2 = hencky(V, dH);
-    double JT = 0.0, JB = 0.0, norm = 0.0;
+    R dH[9];
+    const R hj2 = hencky(V, dH);
+    R JT = R(0), JB = R(0), norm = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        const MatC& M = lv.m[m];
-        double J, Jn;
+        const MatT<R>& M = mat<R>(lv, m);
+        R J, Jn;
         if (M.solid) {
-            J = __builtin_sqrt(hj2);
-            if (shearG(M) * J * s3 > M.yield) Jn = M.yield / (s3 * shearG(M));
+            J = tsqrt(hj2);
+            if (shearG(M) * J * s3 > M.yield) Jn = tdiv(M.yield, s3 * shearG(M));
             else Jn = J;
             e[m] += s23 * (J - Jn);
             are[m] = e[m] * ar[m];
         } else {
-            J = 0.0;
-            Jn = 0.0;
-            e[m] = 0.0;
-            are[m] = 0.0;
+            J = R(0);
+            Jn = R(0);
+            e[m] = R(0);
+            are[m] = R(0);
         }
-        JB += (al[m] / M.Gg) * J;
-        JT += (al[m] / M.Gg) * Jn;
-        norm += (al[m] / M.Gg);
+        JB += tdiv(al[m], M.Gg) * J;
+        JT += tdiv(al[m], M.Gg) * Jn;
+        norm += tdiv(al[m], M.Gg);
     }
-    JB *= 1.0 / norm;
-    JT *= 1.0 / norm;
+    JB *= tdiv(R(1), norm);
+    JT *= tdiv(R(1), norm);
     if (JB == 0.0 && JT == 0.0) {
-        JB = 1.0;
-        JT = 1.0;
+        JB = R(1);
+        JT = R(1);
     }
     unsigned st = 0;
-    double t1[9], t2[9], t2i[9];
+    R t1[9], t2[9], t2i[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
         const double dl = (q / 3 == q % 3) ? 1.0 : 0.0;
         if (dH[q] != 0.0 && JB == 0.0 && JT != 0.0) st |= 16u;
-        else dH[q] *= JT / JB;
+        else dH[q] *= tdiv(JT, JB);
         t1[q] = dl + 0.5 * dH[q];
         t2[q] = dl - 0.5 * dH[q];
     }
@@ -594,24 +642,24 @@ __device__ inline unsigned plastic_cell(const Level& lv, const double* al, const
 // sweeps are its cost; the stored V stays FP64).
 __device__ __forceinline__ double trcbrt(double x) { return pm_rcbrt(x); }
 __device__ __forceinline__ float trcbrt(float x) { return __builtin_amdgcn_rcpf(cbrtf(x)); }
-template <int NM, typename R = double>
-__device__ inline void cleanup_v(const Level& lv, const double* al, double* V) {
+template <int NM, typename R = double, typename S>
+__device__ inline void cleanup_v(const Level& lv, const S* al, S* V) {
     R ta = R(0), tsd = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         ta += (R)al[m];
         if (lv.m[m].solid) tsd += (R)al[m];
     }
-    R Vr[9], G[9], S[9];
+    R Vr[9], G[9], Sq[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) Vr[q] = (R)V[q];
     mmT3(Vr, Vr, G);
-    sqrtm_spd<R>(G, S);
-    const R norm = trcbrt(det3(S));
+    sqrtm_spd(G, Sq);
+    const R norm = trcbrt(det3(Sq));
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
         const R dl = (q / 3 == q % 3) ? R(1) : R(0);
-        V[q] = (double)tdiv((S[q] * norm * tsd) + dl * (ta - tsd), ta);
+        V[q] = (S)tdiv((Sq[q] * norm * tsd) + dl * (ta - tsd), ta);
     }
 }
 

@@ -1,0 +1,465 @@
+// Tangent-linear (forward-mode differentiated) form of the solid path: the
+// FP64 step of the solid kernels evaluated on Dual<K> numbers, carrying K
+// directional derivatives of the whole state with respect to per-material
+// EOS / plasticity parameters (Gamma0, s, yield stress).  Each kernel mirrors
+// its FP64 counterpart (k_init_p2c_s, k_sound_dt_s, k_flux_x_s /
+// k_flux_lmarch_s, k_update_s, k_clock_dt, k_clock_advance) operation for
+// operation, so the value parts equal the FP64 trajectory bit for bit.
+//
+// Layout: a state's values are the caller's CellArray; its derivative parts
+// are K further blocks of the same components, compact (i fastest).  Face
+// planes hold the NF flux components of the values, then NF per direction.
+// The time step is differentiated through dt = CFL min(dx / (a + |u|)) along
+// the cell that attains the minimum (the lowest cell index on ties).
+#pragma once
+#include "tj_dual.h"
+#include "tj_solid_kernels.h"
+
+namespace tj {
+
+template <int K>
+struct DMF {
+    MF u;           // values
+    MF dd;          // derivative block 0 (compact strides)
+    long long bs;   // stride between derivative blocks
+    __device__ __forceinline__ Dual<K> get(int c, int i, int j, int k) const {
+        Dual<K> x;
+        x.v = u.at(c, i, j, k);
+        const long long o = (long long)c * dd.ns + (long long)k * dd.ks + (long long)j * dd.js + i;
+#pragma unroll
+        for (int t = 0; t < K; ++t) x.d[t] = dd.p[t * bs + o];
+        return x;
+    }
+    __device__ __forceinline__ void put(int c, int i, int j, int k, const Dual<K>& x) const {
+        u.at(c, i, j, k) = x.v;
+        const long long o = (long long)c * dd.ns + (long long)k * dd.ks + (long long)j * dd.js + i;
+#pragma unroll
+        for (int t = 0; t < K; ++t) dd.p[t * bs + o] = x.d[t];
+    }
+};
+
+template <int NM, int K>
+__device__ __forceinline__ void load_tprim(const Level& lv, const DMF<K>& U, int i, int j, int k,
+                                           SPrim<NM, Dual<K>>& P) {
+    bool f[3];
+    map3(lv, i, j, k, f);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        P.a[m] = U.get(lv.cs[ALPHA] + m, i, j, k);
+        P.r[m] = U.get(lv.cs[RHO_K] + m, i, j, k);
+        P.e[m] = lv.plastic ? U.get(lv.cs[EPSILON] + m, i, j, k) : Dual<K>(0.0);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const Dual<K> v = U.get(lv.cs[VELOCITY] + r, i, j, k);
+        P.u[r] = f[r] ? -v : v;
+    }
+    P.p = U.get(lv.cs[PRESSURE], i, j, k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) P.V[q] = U.get(lv.cs[V_TENSOR] + q, i, j, k);
+}
+
+// One face per thread along D: left state UR(f-1), right state UL(f); at the
+// domain faces the image of the boundary cell's own reconstruction (flipped
+// normal velocity when reflective), as the FP64 passes do.
+template <int NM, int K, int D>
+__global__ __launch_bounds__(256) void k_tflux_s(Level lv, DMF<K> U, Faces F, Clock* clk) {
+    if (clk->done) return;
+    constexpr int NF = 3 * NM + 17;
+    using T = Dual<K>;
+    const int fx = F.nx, fy = F.ny, fz = F.nz;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned st = 0;
+    if (id < (long long)fx * fy * fz) {
+        const int i = (int)(id % fx);
+        const int j = (int)((id / fx) % fy);
+        const int k = (int)(id / ((long long)fx * fy));
+        const int f = D == 0 ? i : (D == 1 ? j : k);
+        const int nd = lv.n[D];
+        const int cl = f == 0 ? 0 : f - 1, cr = f == nd ? nd - 1 : f;
+        auto at = [&](int c, SPrim<NM, T>& P) {
+            load_tprim<NM, K>(lv, U, D == 0 ? c : i, D == 1 ? c : j, D == 2 ? c : k, P);
+        };
+        SSt<NM, T> A, B;
+        {
+            SPrim<NM, T> Pm, Pc, Pp, PL, PR;
+            at(cl - 1, Pm);
+            at(cl, Pc);
+            at(cl + 1, Pp);
+            muscl_s<NM, T>(Pm, Pc, Pp, PL, PR);
+            p2c_sound_s<NM, T>(lv, PR, D, A);
+        }
+        {
+            SPrim<NM, T> Pm, Pc, Pp, PL, PR;
+            at(cr - 1, Pm);
+            at(cr, Pc);
+            at(cr + 1, Pp);
+            muscl_s<NM, T>(Pm, Pc, Pp, PL, PR);
+            p2c_sound_s<NM, T>(lv, PL, D, B);
+        }
+        if (f == 0 && lv.bc_lo[D]) flip_s<NM, T>(A, D);
+        if (f == nd && lv.bc_hi[D]) flip_s<NM, T>(B, D);
+        T Fv[NF];
+        wave5<NM, T>(lv, A, B, D, Fv, st);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            F.p[q * F.cs + id] = Fv[q].v;
+#pragma unroll
+            for (int t = 0; t < K; ++t) F.p[(NF * (t + 1) + q) * F.cs + id] = Fv[q].d[t];
+        }
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
+template <int NM, int K>
+struct TCell {
+    Dual<K> al[NM], ar[NM], rk[NM], e[NM], are[NM], rho, ru[3], u[3], E, p, V[9], hj2;
+};
+template <int NM, int K>
+__device__ __forceinline__ void load_tcell(const Level& lv, const DMF<K>& U, int i, int j, int k, TCell<NM, K>& C) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        C.al[m] = U.get(lv.cs[ALPHA] + m, i, j, k);
+        C.ar[m] = U.get(lv.cs[ALPHARHO] + m, i, j, k);
+        C.rk[m] = U.get(lv.cs[RHO_K] + m, i, j, k);
+        C.e[m] = lv.plastic ? U.get(lv.cs[EPSILON] + m, i, j, k) : Dual<K>(0.0);
+        C.are[m] = lv.plastic ? U.get(lv.cs[ALPHARHOEPSILON] + m, i, j, k) : Dual<K>(0.0);
+    }
+    C.rho = U.get(lv.cs[RHO], i, j, k);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        C.ru[q] = U.get(lv.cs[RHOU] + q, i, j, k);
+        C.u[q] = U.get(lv.cs[VELOCITY] + q, i, j, k);
+    }
+    C.E = U.get(lv.cs[TOTAL_E], i, j, k);
+    C.p = U.get(lv.cs[PRESSURE], i, j, k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) C.V[q] = U.get(lv.cs[V_TENSOR] + q, i, j, k);
+    C.hj2 = U.get(lv.cs[HJ2], i, j, k);
+}
+template <int NM, int K>
+__device__ __forceinline__ void store_tcell(const Level& lv, const DMF<K>& U, int i, int j, int k, const TCell<NM, K>& C) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        U.put(lv.cs[ALPHA] + m, i, j, k, C.al[m]);
+        U.put(lv.cs[ALPHARHO] + m, i, j, k, C.ar[m]);
+        U.put(lv.cs[RHO_K] + m, i, j, k, C.rk[m]);
+        if (lv.plastic) {
+            U.put(lv.cs[EPSILON] + m, i, j, k, C.e[m]);
+            U.put(lv.cs[ALPHARHOEPSILON] + m, i, j, k, C.are[m]);
+        }
+    }
+    U.put(lv.cs[RHO], i, j, k, C.rho);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        U.put(lv.cs[RHOU] + q, i, j, k, C.ru[q]);
+        U.put(lv.cs[VELOCITY] + q, i, j, k, C.u[q]);
+    }
+    U.put(lv.cs[TOTAL_E], i, j, k, C.E);
+    U.put(lv.cs[PRESSURE], i, j, k, C.p);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) U.put(lv.cs[V_TENSOR] + q, i, j, k, C.V[q]);
+    U.put(lv.cs[HJ2], i, j, k, C.hj2);
+}
+
+// getSoundSpeed + the CFL candidate of one cell; the candidate's value and
+// derivative parts go to cand (K + 1 planes of ncell).
+template <int NM, int K>
+__device__ __forceinline__ double tsound_cand(const Level& lv, const DMF<K>& U, const TCell<NM, K>& C, int i, int j,
+                                              int k, long long id, double* cand) {
+    using T = Dual<K>;
+    const T a = sound_s<NM, T>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
+    U.put(lv.cs[SOUNDSPEED], i, j, k, a);
+    T c = T(__builtin_huge_val());
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (d < lv.dim) c = tmin(c, tdiv(lv.dx[d], a + tabs(C.u[d])));
+    const long long nc = lv.ncells;
+    cand[id] = c.v;
+#pragma unroll
+    for (int t = 0; t < K; ++t) cand[(t + 1) * nc + id] = c.d[t];
+    return c.v;
+}
+
+// update + conservativeToPrimitive; stage 2 the RK average and the post-step
+// chain (plastic + c2p, cleanUpV, cleanUpAlpha, NaN scan, CFL candidate).
+// dt: value clk->dt, derivatives tdt[1..K].
+template <int NM, int STAGE, int K>
+__global__ __launch_bounds__(256) void k_tupdate_s(Level lv, DMF<K> Uin, DMF<K> Ub, DMF<K> Uo, Faces F0, Faces F1,
+                                                   Faces F2, Clock* clk, const double* tdt, unsigned ops, double* cand) {
+    if (clk->done) return;
+    using T = Dual<K>;
+    constexpr int NC = 3 * NM + 13, NF = NC + 4;
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    double cv = __builtin_huge_val();
+    unsigned st = 0;
+    if (id < ncell) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        T dt;
+        dt.v = clk->dt;
+#pragma unroll
+        for (int t = 0; t < K; ++t) dt.d[t] = tdt[1 + t];
+        TCell<NM, K> C;
+        load_tcell<NM, K>(lv, Uin, i, j, k, C);
+        T a0[NM], V0[9], u0[3];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) a0[m] = C.al[m];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) V0[q] = C.V[q];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) u0[q] = C.u[q];
+        for (int d = 0; d < lv.dim; ++d) {
+            const Faces& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
+            const long long lo = ((long long)k * Fd.ny + j) * Fd.nx + i;
+            const long long hi = ((long long)(k + (d == 2)) * Fd.ny + (j + (d == 1))) * Fd.nx + (i + (d == 0));
+            auto fv = [&](int q, long long o) {
+                T x;
+                x.v = Fd.p[q * Fd.cs + o];
+#pragma unroll
+                for (int t = 0; t < K; ++t) x.d[t] = Fd.p[(NF * (t + 1) + q) * Fd.cs + o];
+                return x;
+            };
+            const T dtdx = tdiv(dt, lv.dx[d]);
+            const T s0 = fv(NC, lo), s1 = fv(NC, hi);
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                C.al[m] += dtdx * (fv(m, lo) - fv(m, hi) - a0[m] * (s0 - s1));
+                C.ar[m] += dtdx * (fv(NM + m, lo) - fv(NM + m, hi));
+                if (lv.plastic) C.are[m] += dtdx * (fv(2 * NM + 13 + m, lo) - fv(2 * NM + 13 + m, hi));
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) C.ru[q] += dtdx * (fv(2 * NM + q, lo) - fv(2 * NM + q, hi));
+            C.E += dtdx * (fv(2 * NM + 3, lo) - fv(2 * NM + 3, hi));
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int n = 2 * NM + 4 + r * 3 + c;
+                    const T vs0 = fv(NC + 1 + c, lo), vs1 = fv(NC + 1 + c, hi);
+                    C.V[r * 3 + c] += dtdx * (fv(n, lo) - fv(n, hi) - (2.0 / 3.0) * V0[r * 3 + c] * (s0 - s1) +
+                                              u0[r] * (vs0 - vs1));
+                }
+        }
+        T dH[9];
+        C.hj2 = c2p_s<NM, T>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+        if (STAGE == 2) {
+            TCell<NM, K> B;
+            load_tcell<NM, K>(lv, Ub, i, j, k, B);
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                C.al[m] = B.al[m] * (1.0 / 2.0) + C.al[m] * (1.0 / 2.0);
+                C.ar[m] = B.ar[m] * (1.0 / 2.0) + C.ar[m] * (1.0 / 2.0);
+                C.rk[m] = B.rk[m] * (1.0 / 2.0) + C.rk[m] * (1.0 / 2.0);
+                C.e[m] = B.e[m] * (1.0 / 2.0) + C.e[m] * (1.0 / 2.0);
+                C.are[m] = B.are[m] * (1.0 / 2.0) + C.are[m] * (1.0 / 2.0);
+            }
+            C.rho = B.rho * (1.0 / 2.0) + C.rho * (1.0 / 2.0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                C.ru[q] = B.ru[q] * (1.0 / 2.0) + C.ru[q] * (1.0 / 2.0);
+                C.u[q] = B.u[q] * (1.0 / 2.0) + C.u[q] * (1.0 / 2.0);
+            }
+            C.E = B.E * (1.0 / 2.0) + C.E * (1.0 / 2.0);
+            C.p = B.p * (1.0 / 2.0) + C.p * (1.0 / 2.0);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) C.V[q] = B.V[q] * (1.0 / 2.0) + C.V[q] * (1.0 / 2.0);
+            C.hj2 = B.hj2 * (1.0 / 2.0) + C.hj2 * (1.0 / 2.0);
+            if (ops & 1u) {
+                st |= plastic_cell<NM, T>(lv, C.al, C.ar, C.e, C.are, C.V);
+                C.hj2 = c2p_s<NM, T>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+            }
+            if (ops & 2u) cleanup_v<NM, T>(lv, C.al, C.V);
+            if (ops & 4u) st |= cleanup_alpha<NM, T>(C.al);
+            if (ops & 16u) {
+                bool bad = false, tbad = false;
+#pragma unroll
+                for (int m = 0; m < NM; ++m) {
+                    bad |= tisnan(C.al[m]) || tisnan(C.ar[m]) || tisnan(C.rk[m]) ||
+                           (lv.plastic && (tisnan(C.e[m]) || tisnan(C.are[m])));
+                    tbad |= tan_nan(C.al[m]) || tan_nan(C.ar[m]) || tan_nan(C.rk[m]) || tan_nan(C.e[m]) ||
+                            tan_nan(C.are[m]);
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    bad |= tisnan(C.ru[q]) || tisnan(C.u[q]);
+                    tbad |= tan_nan(C.ru[q]) || tan_nan(C.u[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    bad |= tisnan(C.V[q]);
+                    tbad |= tan_nan(C.V[q]);
+                }
+                bad |= tisnan(C.E) || tisnan(C.p);
+                tbad |= tan_nan(C.E) || tan_nan(C.p);
+                if (bad) st |= 1u;
+                if (tbad) st |= 128u;
+            }
+            if (ops & 8u) cv = tsound_cand<NM, K>(lv, Uo, C, i, j, k, id, cand);
+        }
+        store_tcell<NM, K>(lv, Uo, i, j, k, C);
+    }
+    if (STAGE == 2 && (ops & 8u)) block_min_to(cv, &clk->dtmin_bits);
+    if (st) atomicOr(&clk->status, st);
+}
+
+// getTimeStep of a state (k_sound_dt_s) with the candidates' derivatives.
+template <int NM, int K>
+__global__ __launch_bounds__(256) void k_tsound_dt_s(Level lv, DMF<K> U, Clock* clk, double* cand) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double cv = __builtin_huge_val();
+    if (id < (long long)nx * ny * nz) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        TCell<NM, K> C;
+        load_tcell<NM, K>(lv, U, i, j, k, C);
+        cv = tsound_cand<NM, K>(lv, U, C, i, j, k, id, cand);
+    }
+    block_min_to(cv, &clk->dtmin_bits);
+}
+
+// setInitialConditions of a solid level (k_init_p2c_s) with derivatives: the
+// thermal -> total pressure shift and primitiveToConservative depend on the
+// EOS parameters through the materials' derivative parts.
+template <int NM, int K>
+__global__ __launch_bounds__(256) void k_tinit_p2c_s(Level lv, DMF<K> U, int shift) {
+    using T = Dual<K>;
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    SPrim<NM, T> P;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        P.a[m] = U.get(lv.cs[ALPHA] + m, i, j, k);
+        P.r[m] = U.get(lv.cs[RHO_K] + m, i, j, k);
+        P.e[m] = lv.plastic ? U.get(lv.cs[EPSILON] + m, i, j, k) : T(0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) P.u[q] = U.get(lv.cs[VELOCITY] + q, i, j, k);
+    P.p = U.get(lv.cs[PRESSURE], i, j, k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) P.V[q] = U.get(lv.cs[V_TENSOR] + q, i, j, k);
+    if (shift) {
+        T ar[NM], G, Ent, Pnt;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) ar[m] = P.a[m] * P.r[m];
+        eos_sums_s<NM, T>(lv, P.a, ar, P.r, U.get(lv.cs[HJ2], i, j, k), G, Ent, Pnt);
+        P.p += tdiv(Pnt, G);
+        U.put(lv.cs[PRESSURE], i, j, k, P.p);
+    }
+    SSt<NM, T> S;
+    p2c_sound_s<NM, T>(lv, P, 0, S);
+    T dH[9];
+    const T hj2 = hencky(P.V, dH);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        U.put(lv.cs[ALPHARHO] + m, i, j, k, S.ar[m]);
+        if (lv.plastic) U.put(lv.cs[ALPHARHOEPSILON] + m, i, j, k, S.are[m]);
+    }
+    U.put(lv.cs[RHO], i, j, k, S.rho);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) U.put(lv.cs[RHOU] + q, i, j, k, S.u[q] * S.rho);
+    U.put(lv.cs[TOTAL_E], i, j, k, S.E);
+    U.put(lv.cs[HJ2], i, j, k, hj2);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) U.put(lv.cs[DEVH] + q, i, j, k, dH[q]);
+}
+
+// The lowest cell index whose candidate attains the level minimum.
+__global__ __launch_bounds__(256) void k_targmin(const Clock* clk, const double* cand, long long ncell,
+                                                 unsigned long long* arg) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < ncell && __builtin_bit_cast(unsigned long long, cand[id]) == clk->dtmin_bits)
+        atomicMin(arg, (unsigned long long)id);
+}
+
+// k_clock_dt (driver form) with dt's derivatives: CFL times the minimising
+// cell's candidate derivatives; minus the derivatives of t where dt is
+// clipped to the final time; times 0.2 over the first five steps.
+template <int K>
+__global__ void k_tclock_dt(Clock* clk, double cfl, const double* cand, long long ncell, unsigned long long* arg,
+                            double* tdt, const double* tt) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (clk->done) return;
+    if (!(clk->t < clk->final_time)) {
+        clk->done = 1;
+        return;
+    }
+    const double mn = __builtin_bit_cast(double, clk->dtmin_bits);
+    double dt = cfl * mn;
+    const unsigned long long a = *arg;
+    double g[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) g[t] = a < (unsigned long long)ncell ? cfl * cand[(t + 1) * ncell + (long long)a] : 0.0;
+    if (clk->t + dt > clk->final_time) {
+        dt = clk->final_time - clk->t;
+#pragma unroll
+        for (int t = 0; t < K; ++t) g[t] = -tt[t];
+    }
+    if (clk->n < 5) {
+        dt *= 0.2;
+#pragma unroll
+        for (int t = 0; t < K; ++t) g[t] *= 0.2;
+    }
+    clk->dt = dt;
+    tdt[0] = dt;
+#pragma unroll
+    for (int t = 0; t < K; ++t) tdt[1 + t] = g[t];
+    clk->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
+    *arg = ~0ull;
+}
+// start of a run (k_clock_final of tj_begin_run): the final time, no
+// minimising cell yet, t's derivatives zero
+template <int K>
+__global__ void k_tclock_begin(Clock* clk, double final_time, unsigned long long* arg, double* tt) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    clk->final_time = final_time;
+    *arg = ~0ull;
+#pragma unroll
+    for (int t = 0; t < K; ++t) tt[t] = 0.0;
+}
+// k_clock_advance with t's derivatives.
+template <int K>
+__global__ void k_tclock_advance(Clock* clk, const double* tdt, double* tt) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (clk->done) return;
+    if (clk->status & 1u) {
+        clk->done = 1;
+        return;
+    }
+    clk->t += clk->dt;
+    clk->n += 1;
+#pragma unroll
+    for (int t = 0; t < K; ++t) tt[t] += tdt[1 + t];
+}
+// Z = X over ncomp components of the valid cells (views of any strides)
+__global__ __launch_bounds__(256) void k_copy_mf(MF X, MF Z, int ncomp, int nx, int ny, int nz) {
+    const long long ncell = (long long)nx * ny * nz;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell * ncomp) return;
+    const int c = (int)(id / ncell);
+    const long long cell = id % ncell;
+    const int i = (int)(cell % nx);
+    const int j = (int)((cell / nx) % ny);
+    const int k = (int)(cell / ((long long)nx * ny));
+    Z.at(c, i, j, k) = X.at(c, i, j, k);
+}
+// value and derivatives of one component of one cell into a trace row
+template <int K>
+__global__ void k_tprobe(DMF<K> U, int c, int i, int j, int k, double* row) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const Dual<K> x = U.get(c, i, j, k);
+    row[0] = x.v;
+#pragma unroll
+    for (int t = 0; t < K; ++t) row[1 + t] = x.d[t];
+}
+
+}  // namespace tj

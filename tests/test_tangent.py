@@ -1,0 +1,93 @@
+"""CPU side of the tangent-linear form: the differentiated plate impact's
+configuration, and the finite-difference reference gradient of the oracle
+pinned against the closed-form derivative of the Us-up solid EOS.
+
+After setInitialConditions with V = I (HJ2 = 0) and the thermal -> total
+pressure shift p = sum p_nt / sum(alpha / Gamma) (initialise.cpp:104), the
+total energy is E = sum_m alphaRho_m e_H(rho_m) + KE, e_H = p_H eta / (2 rho0),
+p_H = rho0 c0^2 eta / (1 - s eta)^2, eta = 1 - rho0 / rho: so
+dE/ds = sum_m alphaRho_m eta / (2 rho0) * 2 rho0 c0^2 eta^2 / (1 - s eta)^3 and
+dE/dGamma0 = 0.  The GPU test of the same closed form for the tangent run is
+in test_gpu_tangent.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tenjin_amd import configs
+
+
+def _closed_form_dE_ds(pb, m_diff):
+    L = pb.layout
+    U = pb.prims
+    M = pb.materials[m_diff]
+    rho = U[L.comp(configs.RHO_K, m_diff)]
+    alpha = U[L.comp(configs.ALPHA, m_diff)]
+    eta = 1.0 - M.rho0 / rho
+    dpH = 2.0 * M.rho0 * M.c0 ** 2 * eta ** 2 / (1.0 - M.s * eta) ** 3
+    return alpha * rho * eta / (2.0 * M.rho0) * dpH
+
+
+def test_differentiated_plate_impact_config():
+    pb = configs.plate_impact_differentiated(n=64)
+    assert pb.dim == 2 and pb.shape == (1, 64, 64) and pb.SOLID and pb.PLASTIC
+    assert 200.0 <= pb.speed <= 600.0
+    assert configs.plate_params() == [2.0, 1.49, 90e6, 2.0, 1.338, 300e6]
+    comp, i, j, k = configs.plate_probe(pb)
+    assert (i, j, k) == (61, 32, 0) and comp == pb.layout.comp(configs.VELOCITY, 0, 0)
+    # the probe cell is aluminium
+    assert pb.prims[pb.layout.comp(configs.ALPHA, 1), k, j, i] > 0.99
+    th = configs.plate_target_theta()
+    for a, b in zip(th, configs.plate_params()):
+        assert abs(a / b - 1.0) <= 0.05 and a != b
+    # theta reaches the materials
+    pb2 = configs.plate_impact_differentiated(n=16, theta=[2.1, 1.5, 1e8, 1.9, 1.3, 2e8])
+    assert pb2.materials[0].gamma == 3.1 and pb2.materials[1].s == 1.3 and pb2.materials[1].yield_stress == 2e8
+
+
+@pytest.mark.parametrize("m", [0, 1])
+def test_oracle_fd_of_initial_energy_matches_closed_form(m):
+    from oracle.oracle import Oracle
+
+    th = configs.plate_params()
+    p = 3 * m + 1  # s of material m
+    # E carries the flyer's kinetic energy (~1e9): a step of 1e-3 s keeps the
+    # difference of rounded energies well above their rounding; the quotient's
+    # truncation error is O(h^2 eta^2) with eta ~ 1e-3, far below the bar
+    h = 1e-3 * th[p]
+    E = []
+    for sgn in (1.0, -1.0):
+        t = list(th)
+        t[p] += sgn * h
+        pb = configs.plate_impact_differentiated(n=24, theta=t)
+        E.append(Oracle(pb).get()[pb.layout.comp(configs.TOTAL_E)])
+    fd = (E[0] - E[1]) / (2 * h)
+    pb = configs.plate_impact_differentiated(n=24)
+    ref = _closed_form_dE_ds(pb, m)
+    err = np.max(np.abs(fd - ref)) / np.max(np.abs(ref))
+    assert np.max(np.abs(ref)) > 0.0 and err < 1e-5
+    # Gamma0 does not enter E at V = I with the pressure shift
+    t = list(th)
+    t[3 * m] += 1e-3
+    pb2 = configs.plate_impact_differentiated(n=24, theta=t)
+    E2 = Oracle(pb2).get()[pb2.layout.comp(configs.TOTAL_E)]
+    E0 = Oracle(pb).get()[pb.layout.comp(configs.TOTAL_E)]
+    assert np.max(np.abs(E2 - E0)) <= 1e-9 * np.max(np.abs(E0))
+
+
+def test_tangent_run_needs_the_gpu():
+    import torch
+
+    from tenjin_amd import TenjinError, tangent
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(TenjinError, match="GPU"):
+        tangent.TangentRun(configs.plate_impact_differentiated(n=16), tangent.ALL_DIRECTIONS)
+
+
+def test_probe_loss():
+    from tenjin_amd import tangent
+
+    assert tangent.probe_loss([1.0, 2.0, 3.0], [1.0, 1.0, 1.0]) == 5.0
