@@ -9,7 +9,10 @@ on 256^3 cells in 64^3 boxes (Us-up Mie-Gruneisen, 5-wave flux, perfect
 plasticity); `--workload explosion_rz` an axisymmetric (r, z) underwater
 explosion on 1448^2 cells with the cylindrical geometric source;
 `--workload reactive2d` a reactive mixture (MixtureEOS with rootFind in every
-conservativeToPrimitive, pressure-based burn) on 2048 x 1024 cells.  One "step" is one full time step of the reference's loop body
+conservativeToPrimitive, pressure-based burn) on 2048 x 1024 cells;
+`--workload plate2d_grad` the gradient of a probe-trace loss of the 2D plate
+impact (512^2, 200 steps) with respect to six material parameters by the
+tangent-linear form, against the forward run.  One "step" is one full time step of the reference's loop body
 (main.cpp:124-171): CFL dt with the driver clamps, two RK stages, the RK
 average, plasticUpdate and cleanUpV (solids), cleanUpAlpha, the NaN scan and
 the next step's CFL reduction.  The state is resident in HBM before timing
@@ -91,7 +94,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=["inclusion2d", "plate3d", "tube1d", "ensemble", "explosion_rz",
-                                           "reactive2d"], default="inclusion2d")
+                                           "reactive2d", "plate2d_grad"], default="inclusion2d")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 500 for inclusion2d, 40 for plate3d)")
     ap.add_argument("--warmup", type=int, default=10)
@@ -101,6 +104,7 @@ def parse():
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--n3", type=int, default=256, help="plate3d: cells per side")
     ap.add_argument("--nrz", type=int, default=1448, help="explosion_rz: cells per side")
+    ap.add_argument("--n2g", type=int, default=512, help="plate2d_grad: cells per side")
     ap.add_argument("--box", type=int, default=64)
     ap.add_argument("--seglen", type=int, default=0, help="rows per marching segment (0: chosen per pass by the library)")
     ap.add_argument("--fuse", action="store_true",
@@ -112,7 +116,7 @@ def parse():
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"inclusion2d": 500, "plate3d": 40, "tube1d": 1280, "ensemble": 200,
-                      "explosion_rz": 500, "reactive2d": 200}[args.workload]
+                      "explosion_rz": 500, "reactive2d": 200, "plate2d_grad": 200}[args.workload]
     return args
 
 
@@ -286,6 +290,128 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_plate2d_grad(args):
+    """The differentiated 2D plate impact (512^2 Cu/Al, 200 steps, probe u_x
+    at the aluminium's far face): the loss sum_n (u_n - u*_n)^2 against a
+    target trace from perturbed parameters, and its gradient with respect to
+    Gamma0, s and Y of both materials by the tangent-linear form (6
+    directions; the direction-group size is tuned first).  Reports the
+    gradient run's throughput (cells x steps / s of the gradient), the
+    forward run's, their ratio, and the gradient against central differences
+    of the FP64 form (h = 1e-6 |theta| and 1e-4 |theta|)."""
+    import numpy as np
+    import torch
+    from tenjin_amd import GpuLevel, configs, tangent
+
+    n, steps = args.n2g, args.steps
+    theta = configs.plate_params()
+    pb = configs.plate_impact_differentiated(n=n, steps=steps)
+    probe = configs.plate_probe(pb)
+    target = tangent.forward_trace(configs.plate_impact_differentiated(n=n, theta=configs.plate_target_theta(),
+                                                                       steps=steps), steps, probe)
+
+    P0 = torch.from_numpy(pb.prims).to("cuda")
+
+    def forward_once():
+        """initialisation from the device-resident primitives + the time loop
+        (what each direction group of the tangent run repeats)"""
+        g = GpuLevel(pb)
+        g.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.init_from_device(P0, pb.shift_pressure)
+        g.begin_run(float("inf"))
+        g.advance(steps)
+        g.sync()
+        el = time.perf_counter() - t0
+        st = g.status()
+        g.close()
+        return el, st
+
+    forward_once()  # warm-up
+    t_fwd, st_fwd = min(forward_once() for _ in range(3))
+    # direction-group size: 10-step trial runs of each
+    trial = {}
+    for chunk in (1, 2, 3):
+        tr = tangent.TangentRun(pb, tangent.ALL_DIRECTIONS, chunk=chunk)
+        tr.run(2, probe)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr.run(10, probe)
+        trial[chunk] = time.perf_counter() - t0
+        tr.close()
+    chunk = min(trial, key=trial.get)
+    tr = tangent.TangentRun(pb, tangent.ALL_DIRECTIONS, chunk=chunk)
+    tr.run(2, probe)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    trace, done = tr.run(steps, probe)
+    t_grad = time.perf_counter() - t0
+    st_grad = tr.status()
+    tr.close()
+    r = trace[:, 0] - target
+    loss = float(r @ r)
+    grad = 2.0 * (r @ trace[:, 1:])
+    # central differences of the FP64 form along each parameter, at two steps:
+    # the loss (~1e-5 here) is a sum of squares of probe values that the wave
+    # has barely reached, so the quotient's rounding noise grows as h shrinks
+    fd_err = {}
+    for rel in (1e-6, 1e-4):
+        fd = np.zeros(6)
+        for p in range(6):
+            h = rel * abs(theta[p])
+            lpm = []
+            for sgn in (1.0, -1.0):
+                th = list(theta)
+                th[p] += sgn * h
+                u = tangent.forward_trace(configs.plate_impact_differentiated(n=n, theta=th, steps=steps), steps,
+                                          probe)
+                lpm.append(tangent.probe_loss(u, target))
+            fd[p] = (lpm[0] - lpm[1]) / (2 * h)
+        fd_err[f"h={rel:.0e}|theta|"] = [float(f"{abs(a - b) / max(abs(b), 1e-300):.3e}") for a, b in zip(grad, fd)]
+    value = pb.ncells * steps / t_grad / 1e6
+    out = {
+        "metric": "Mcell-updates/s on 1 GPU",
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 2,
+        "ms_per_step": round(1e3 * t_grad / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"differentiated 2D elastoplastic Cu/Al plate impact, {n}^2 cells, {steps} steps, "
+                               f"impact {pb.speed:.1f} m/s; loss = sum of squared probe-trace misfits; gradient "
+                               "w.r.t. Gamma0, s, Y of both materials by the tangent-linear form",
+                   "cells": pb.ncells, "precision": "fp64", "directions": 6, "chunk": chunk,
+                   "chunk_trial_s": {str(k): round(v, 4) for k, v in trial.items()},
+                   "parallelism": "single GPU"},
+        "adjoint_over_forward": round(t_grad / t_fwd, 3),
+        "per_direction_over_forward": round(t_grad / t_fwd / 6, 3),
+        "forward_value": round(pb.ncells * steps / t_fwd / 1e6, 2),
+        "forward_ms": round(1e3 * t_fwd, 3),
+        "gradient_ms": round(1e3 * t_grad, 3),
+        "loss": loss,
+        "gradient": [float(f"{v:.9e}") for v in grad],
+        "gradient_rel_err_vs_central_differences": fd_err,
+        "status": [st_fwd, st_grad],
+        "steps_done": done,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        # the gradient by central differences on the CPU costs 13 runs of the
+        # oracle: its rate is the oracle's forward rate / 13
+        cb = cpu_baseline(configs.plate_impact_differentiated(n=min(n, 256), steps=steps), args.cpu_seconds, 40)
+        cb["value"] = cb["value"] / 13.0
+        cb["sample"] = "gradient by central differences (13 forward runs): " + cb["sample"]
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -308,6 +434,9 @@ def main():
         return
     if args.workload == "ensemble":
         bench_ensemble(args)
+        return
+    if args.workload == "plate2d_grad":
+        bench_plate2d_grad(args)
         return
     if args.workload == "inclusion2d":
         pb = configs.shock_inclusion_2d(nx=args.nx, ny=args.ny, box=args.box)

@@ -264,6 +264,15 @@ class GpuLevel:
         self._chk(self.lib.tj_reset_history(self.ctx), "tj_reset_history")
         self.sync()
 
+    def init_from_device(self, prims, shift_pressure: int = 0):
+        """init_from_prims from a device tensor shaped like S (a device-to-device
+        copy; no host transfer and no synchronisation after the launch)."""
+        self.torch.cuda.synchronize()
+        self.S.copy_(prims)
+        self.torch.cuda.synchronize()
+        self._chk(self.lib.tj_init_from_prims(self.ctx, ctypes.byref(self.vS), shift_pressure), "tj_init_from_prims")
+        self._chk(self.lib.tj_reset_history(self.ctx), "tj_reset_history")
+
     def compute_dt(self) -> float:
         dt = ctypes.c_double()
         self._chk(self.lib.tj_compute_dt(self.ctx, ctypes.byref(self.vS), ctypes.byref(dt)), "tj_compute_dt")

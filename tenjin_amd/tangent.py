@@ -65,7 +65,8 @@ class TangentRun:
 
     def state_derivative(self, d: int) -> np.ndarray:
         """d(final state)/d(parameter of direction d) (direction of the last group)."""
-        out = self.g.torch.zeros_like(self.g.S)
+        out = self.g.torch.empty_like(self.g.S)
+        self.g.torch.cuda.synchronize()  # the allocation is torch's; the copy runs on the library's stream
         v = self.g._view(out)
         self.g._chk(self.lib.tj_tangent_state(self.tg, d, ctypes.byref(v)), "tj_tangent_state")
         return out.cpu().numpy()
