@@ -193,6 +193,107 @@ def test_initial_energy_derivative_closed_form(m):
     assert np.max(np.abs(dE_dg)) <= 1e-12 * np.max(np.abs(E))
 
 
+def _oracle_run_to(pb, final_time, probe):
+    """The driver loop of main.cpp:121-191 to a final time (dt clipped to it)."""
+    from oracle.oracle import Oracle
+
+    o = Oracle(pb)
+    comp, i, j, k = probe
+    t, n, out = 0.0, 0, []
+    while t < final_time:
+        dt = o.driver_dt(n, t, final_time)
+        o.step(dt)
+        t += dt
+        n += 1
+        out.append(o.get()[comp, k, j, i])
+    return np.array(out), o.get(), n
+
+
+def test_final_time_clip_derivative():
+    """A run to a fixed final time: the last dt is clipped to it, so its
+    derivative is minus the accumulated derivative of t, and the final state's
+    derivative is the one at that fixed time (central differences of the
+    oracle's runs to the same final time)."""
+    from tenjin_amd import configs, tangent
+
+    from oracle.oracle import Oracle
+
+    pb = _problem(steps=40)
+    probe = configs.plate_probe(pb)
+    # a final time halfway through step 26 of the unclipped run
+    o = Oracle(pb)
+    t = 0.0
+    for q in range(25):
+        dt = o.driver_dt(q, t)
+        o.step(dt)
+        t += dt
+    T = t + 0.5 * o.driver_dt(25, t)
+    print(f"\n[case] tangent run to final time {T:.6e}", flush=True)
+    tr = tangent.TangentRun(pb, [(1, "s"), (0, "Gamma0")], chunk=2)
+    trace, done = tr.run(40, probe, final_time=T)
+    dS = [tr.state_derivative(0), tr.state_derivative(1)]
+    S = tr.state()
+    tr.close()
+    ov, So, n_o = _oracle_run_to(pb, T, probe)
+    assert done == n_o == 26
+    assert np.array_equal(trace[:done, 0], ov)
+    from tests.parity_util import exact_fraction
+
+    assert exact_fraction(pb, So, S) == 1.0
+    th = configs.plate_params()
+    L = pb.layout
+    for col, p in ((0, 4), (1, 0)):
+        h = 1e-6 * abs(th[p])
+        st = []
+        for sgn in (1.0, -1.0):
+            tp = list(th)
+            tp[p] += sgn * h
+            _, Sx, nx_ = _oracle_run_to(_problem(tp, steps=40), T, probe)
+            assert nx_ == n_o
+            st.append(Sx)
+        fd = (st[0] - st[1]) / (2 * h)
+        for c in (L.comp(configs.TOTAL_E), L.comp(configs.RHOU, 0, 0), L.comp(configs.P)):
+            scale = np.max(np.abs(fd[c]))
+            err = np.max(np.abs(dS[col][c] - fd[c])) / scale
+            print(f"[case] direction {p} comp {L.names[c]}: {err:.3e}", flush=True)
+            assert err <= FD_BAR
+
+
+def test_reflective_boundaries():
+    """Reflective x-low and y-high faces (mirrored states, flipped normal
+    velocity): values bit-exact, derivatives against the oracle."""
+    from tenjin_amd import configs, tangent
+
+    def refl(theta=None):
+        pb = _problem(theta, steps=40)
+        pb.bc_lo = (1, 0, 0)
+        pb.bc_hi = (0, 1, 0)
+        return pb
+
+    steps = 40
+    pb = refl()
+    comp = pb.layout.comp(configs.VELOCITY, 0, 0)
+    probe = (comp, 3, N - 2, 0)  # next to both reflective faces
+    print(f"\n[case] tangent with reflective x-low / y-high, steps={steps}", flush=True)
+    tr = tangent.TangentRun(pb, [(0, "s"), (1, "Gamma0")], chunk=2)
+    trace, done = tr.run(steps, probe)
+    tr.close()
+    ov, _ = _oracle_trace(pb, steps, probe)
+    assert done == steps and np.array_equal(trace[:, 0], ov)
+    th = configs.plate_params()
+    for col, p in ((1, 1), (2, 3)):
+        h = 1e-6 * abs(th[p])
+        us = []
+        for sgn in (1.0, -1.0):
+            tp = list(th)
+            tp[p] += sgn * h
+            us.append(_oracle_trace(refl(tp), steps, probe)[0])
+        fd = (us[0] - us[1]) / (2 * h)
+        err = np.max(np.abs(trace[:, col] - fd)) / np.max(np.abs(fd))
+        print(f"[case] reflective direction {p}: {err:.3e}", flush=True)
+        assert err <= FD_BAR
+
+
 def test_three_dimensional_probe_derivative():
     from tenjin_amd import configs, tangent
 
