@@ -4,6 +4,13 @@
 // in the same order, so a tangent run reproduces the FP64 trajectory bit for
 // bit; the derivative parts follow the chain rule of each operation.
 //
+// The derivative parts are not bound to the value's rounding: they use fused
+// multiply-adds and a reciprocal refined from the hardware estimate (two
+// Newton steps, a few ulp) instead of a second IEEE division.  The one
+// quotient whose denominator reaches the subnormals, the MUSCL ratio of
+// differences of exponentially small precursor values ahead of a wave (where
+// the estimate overflows), divides (muscl1 below).
+//
 // Non-smooth points take one one-sided derivative: |x| and max/min follow the
 // branch the value takes, sqrt at 0 has derivative 0 (it is reached only at a
 // minimum of its argument, HJ2 = 0 of an undeformed cell), and the MUSCL
@@ -28,6 +35,15 @@ template <typename T>
 struct is_dual { static constexpr bool value = false; };
 template <int K>
 struct is_dual<Dual<K>> { static constexpr bool value = true; };
+
+// 1/b for derivative parts: v_rcp_f64's estimate and two Newton steps
+__device__ __forceinline__ double drcp(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
 
 #define TJ_DUAL_FN template <int K> __device__ __forceinline__
 
@@ -78,7 +94,7 @@ TJ_DUAL_FN Dual<K> operator*(const Dual<K>& a, const Dual<K>& b) {
     Dual<K> r;
     r.v = a.v * b.v;
 #pragma unroll
-    for (int t = 0; t < K; ++t) r.d[t] = a.d[t] * b.v + a.v * b.d[t];
+    for (int t = 0; t < K; ++t) r.d[t] = __builtin_fma(a.d[t], b.v, a.v * b.d[t]);
     return r;
 }
 TJ_DUAL_FN Dual<K> operator*(const Dual<K>& a, double b) {
@@ -98,22 +114,25 @@ TJ_DUAL_FN Dual<K> operator*(double a, const Dual<K>& b) {
 TJ_DUAL_FN Dual<K> tdiv(const Dual<K>& a, const Dual<K>& b) {
     Dual<K> r;
     r.v = a.v / b.v;
+    const double rb = drcp(b.v);
 #pragma unroll
-    for (int t = 0; t < K; ++t) r.d[t] = (a.d[t] - r.v * b.d[t]) / b.v;
+    for (int t = 0; t < K; ++t) r.d[t] = __builtin_fma(-r.v, b.d[t], a.d[t]) * rb;
     return r;
 }
 TJ_DUAL_FN Dual<K> tdiv(const Dual<K>& a, double b) {
     Dual<K> r;
     r.v = a.v / b;
+    const double rb = drcp(b);
 #pragma unroll
-    for (int t = 0; t < K; ++t) r.d[t] = a.d[t] / b;
+    for (int t = 0; t < K; ++t) r.d[t] = a.d[t] * rb;
     return r;
 }
 TJ_DUAL_FN Dual<K> tdiv(double a, const Dual<K>& b) {
     Dual<K> r;
     r.v = a / b.v;
+    const double g = -r.v * drcp(b.v);
 #pragma unroll
-    for (int t = 0; t < K; ++t) r.d[t] = -(r.v * b.d[t]) / b.v;
+    for (int t = 0; t < K; ++t) r.d[t] = g * b.d[t];
     return r;
 }
 TJ_DUAL_FN Dual<K> operator/(const Dual<K>& a, const Dual<K>& b) { return tdiv(a, b); }
@@ -138,7 +157,7 @@ TJ_DUAL_CMP(!=)
 TJ_DUAL_FN Dual<K> tsqrt(const Dual<K>& a) {
     Dual<K> r;
     r.v = __builtin_sqrt(a.v);
-    const double h = r.v > 0.0 ? 0.5 / r.v : 0.0;
+    const double h = r.v > 0.0 ? 0.5 * drcp(r.v) : 0.0;
 #pragma unroll
     for (int t = 0; t < K; ++t) r.d[t] = a.d[t] * h;
     return r;
@@ -157,10 +176,24 @@ TJ_DUAL_FN bool tisinf(const Dual<K>& a) { return __builtin_isinf(a.v); }
 TJ_DUAL_FN Dual<K> trcbrt(const Dual<K>& a) {
     Dual<K> r;
     r.v = pm_rcbrt(a.v);
-    const double g = -(r.v / a.v) / 3.0;
+    const double g = -(1.0 / 3.0) * r.v * drcp(a.v);
 #pragma unroll
     for (int t = 0; t < K; ++t) r.d[t] = g * a.d[t];
     return r;
+}
+// MUSCL with van Leer's limiter (hllc.cpp:527-575) on dual numbers: the
+// value parts as muscl1 of tj_fluid.h; the ratio's derivative parts by IEEE
+// division (its denominator can be subnormal)
+TJ_DUAL_FN void muscl1(Dual<K> um, Dual<K> uc, Dual<K> up, Dual<K>& L, Dual<K>& Rt) {
+    const Dual<K> num = uc - um, den = up - uc;
+    Dual<K> r;
+    r.v = num.v / den.v;
+#pragma unroll
+    for (int t = 0; t < K; ++t) r.d[t] = (num.d[t] - r.v * den.d[t]) / den.v;
+    if (tisinf(r) || tisnan(r)) r = Dual<K>(0.0);
+    Dual<K> g = vanleer(r) * 0.5 * (up - um);
+    L = uc - 0.5 * g;
+    Rt = uc + 0.5 * g;
 }
 // true when any derivative part is NaN (the value part is checked separately)
 TJ_DUAL_FN bool tan_nan(const Dual<K>& a) {

@@ -315,6 +315,23 @@ def test_three_dimensional_probe_derivative():
         assert err <= FD_BAR
 
 
+def test_full_size_run_has_finite_derivatives():
+    """C4's size (512^2, 200 steps): ahead of the wave the state's
+    differences decay to the subnormals, where the MUSCL ratio's derivative
+    must stay finite (no TJ_ST_TANGENT_NAN)."""
+    from tenjin_amd import configs, tangent
+
+    pb = configs.plate_impact_differentiated(n=512, steps=200)
+    print("\n[case] tangent run 512^2, 200 steps, two directions", flush=True)
+    tr = tangent.TangentRun(pb, [(0, "Y"), (1, "s")], chunk=2)
+    trace, done = tr.run(200, configs.plate_probe(pb))
+    st = tr.status()
+    dS = tr.state_derivative(1)
+    tr.close()
+    assert done == 200 and st == 0
+    assert np.all(np.isfinite(trace)) and np.all(np.isfinite(dS[pb.layout.state_comps()]))
+
+
 def test_autograd_wrapper():
     import torch
 
