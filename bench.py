@@ -297,11 +297,12 @@ def bench_plate2d_grad(args):
     Gamma0, s and Y of both materials by the tangent-linear form (6
     directions; the direction-group size is tuned first).  Reports the
     gradient run's throughput (cells x steps / s of the gradient), the
-    forward run's, their ratio, and the gradient against central differences
-    of the FP64 form (h = 1e-6 |theta| and 1e-4 |theta|)."""
+    forward run's, their ratio, the gradient against central differences of
+    the FP64 form (h = 1e-6 |theta| and 1e-4 |theta|), and the mixed forward
+    run's time and error against the FP64 one."""
     import numpy as np
     import torch
-    from tenjin_amd import GpuLevel, configs, tangent
+    from tenjin_amd import GpuLevel, TJ_PREC_FP64, TJ_PREC_MIXED, configs, tangent
 
     n, steps = args.n2g, args.steps
     theta = configs.plate_params()
@@ -330,6 +331,42 @@ def bench_plate2d_grad(args):
 
     forward_once()  # warm-up
     t_fwd, st_fwd = min(forward_once() for _ in range(3))
+
+    # the mixed forward beside it: time, and relative L1 of rho, p, alpha and
+    # of the probe trace against the FP64 run after the same steps
+    def run_state(prec, trace=False):
+        g = GpuLevel(pb)
+        g.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.init_from_device(P0, pb.shift_pressure)
+        g.begin_run(float("inf"))
+        tr = []
+        if trace:
+            comp, i, j, k = probe
+            for _ in range(steps):
+                g.advance(1, prec)
+                g.sync()
+                tr.append(float(g.S[comp, k, j, i].item()))
+        else:
+            g.advance(steps, prec)
+        g.sync()
+        el = time.perf_counter() - t0
+        S = g.S.clone()
+        st = g.status()
+        g.close()
+        return el, S, np.array(tr), st
+
+    run_state(TJ_PREC_MIXED)  # warm-up
+    t_mix = min(run_state(TJ_PREC_MIXED)[0] for _ in range(3))
+    _, S64, tr64, _ = run_state(TJ_PREC_FP64, trace=True)
+    _, S32, tr32, st_mix = run_state(TJ_PREC_MIXED, trace=True)
+    L = pb.layout
+    mixed_err = {}
+    for name, c in (("rho", L.comp(configs.RHO)), ("p", L.comp(configs.P)), ("alpha", L.comp(configs.ALPHA, 0))):
+        mixed_err[name] = float(f"{float((S32[c] - S64[c]).abs().sum() / S64[c].abs().sum()):.3e}")
+    mixed_err["probe_trace"] = float(f"{np.abs(tr32 - tr64).sum() / max(np.abs(tr64).sum(), 1e-300):.3e}")
+    del S64, S32
     # direction-group size: 10-step trial runs of each
     trial = {}
     for chunk in (1, 2, 3):
@@ -393,6 +430,9 @@ def bench_plate2d_grad(args):
         "per_direction_over_forward": round(t_grad / t_fwd / 6, 3),
         "forward_value": round(pb.ncells * steps / t_fwd / 1e6, 2),
         "forward_ms": round(1e3 * t_fwd, 3),
+        "mixed_forward": {"value": round(pb.ncells * steps / t_mix / 1e6, 2), "ms": round(1e3 * t_mix, 3),
+                          "speedup_vs_fp64": round(t_fwd / t_mix, 3), "l1_rel_err_vs_fp64": mixed_err,
+                          "status": st_mix},
         "gradient_ms": round(1e3 * t_grad, 3),
         "loss": loss,
         "gradient": [float(f"{v:.9e}") for v in grad],
