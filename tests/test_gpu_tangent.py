@@ -316,7 +316,7 @@ def test_three_dimensional_probe_derivative():
 
 
 def test_full_size_run_has_finite_derivatives():
-    """C4's size (512^2, 200 steps): ahead of the wave the state's
+    """The differentiated plate impact at full size (512^2, 200 steps): ahead of the wave the state's
     differences decay to the subnormals, where the MUSCL ratio's derivative
     must stay finite (no TJ_ST_TANGENT_NAN)."""
     from tenjin_amd import configs, tangent
