@@ -288,7 +288,8 @@ void tj_tangent_destroy(tj_tangent* tg);
  * component probe_comp of cell (pi, pj, pk) go to row n of trace (host,
  * nsteps x (1 + ndir) doubles: value, then one column per direction; may be
  * NULL).  *steps_done: loop bodies run (fewer than nsteps after the final
- * time or a NaN stop).  S ends in the final state. */
+ * time or a NaN stop).  S ends in the final state.  P0, S and T are three
+ * distinct buffers. */
 int tj_tangent_run(tj_tangent* tg, const tj_mf* P0, int shift_pressure, tj_mf* S, tj_mf* T, double final_time,
                    int nsteps, int probe_comp, int pi, int pj, int pk, double* trace, int* steps_done);
 /* The final-state derivative of direction dir (of the last group run) into

@@ -3,7 +3,9 @@ FP64 / mixed forms and the tangent-linear form) into
 tenjin_amd/libtenjin.so with hipcc (in-tree, so the built library travels with
 the source tree), the C++ host interface tenjin_amd/csrc/host/ into
 tenjin_amd/libtenjin_host.so, and the C++ time-loop driver used by the tests
-(tests/host/main_loop.cpp) into tests/host/main_loop.
+(tests/host/main_loop.cpp) into tests/host/main_loop, and the libtorch
+autograd Function over the tangent-linear form (csrc/torch/) into
+tenjin_amd/tenjin_torch.so.
 
 Flags:
   --offload-arch=gfx950            MI355X (CDNA4) only
@@ -72,6 +74,33 @@ def build(verbose: bool = True, extra=None) -> str:
     return LIB
 
 
+TORCH_LIB = os.path.join(HERE, "tenjin_torch.so")
+
+
+def build_torch(verbose: bool = True) -> str:
+    """The libtorch autograd Function over the tangent-linear form
+    (csrc/torch/tj_autograd.cpp -> tenjin_amd/tenjin_torch.so, a Python
+    extension module that links libtenjin.so from its own folder), with the
+    flags torch.utils.cpp_extension uses for a C++ extension."""
+    import sysconfig
+
+    from torch.utils import cpp_extension as ce
+
+    src = os.path.join(CSRC, "torch", "tj_autograd.cpp")
+    cmd = ["c++", "-O2", "-std=c++17", "-fPIC", "-shared", "-DTORCH_EXTENSION_NAME=tenjin_torch",
+           "-DTORCH_API_INCLUDE_EXTENSION_H", "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", "-DHIPBLAS_V2"]
+    for p in ce.include_paths():
+        cmd += ["-isystem", p]
+    cmd += ["-isystem", sysconfig.get_paths()["include"], "-I", INCLUDE, src]
+    for p in ce.library_paths():
+        cmd += ["-L", p]
+    cmd += ["-lc10", "-ltorch_cpu", "-ltorch", "-ltorch_python", "-L", HERE, "-ltenjin", "-Wl,-rpath,$ORIGIN",
+            "-o", TORCH_LIB + ".tmp"]
+    _run(cmd, verbose)
+    os.replace(TORCH_LIB + ".tmp", TORCH_LIB)
+    return TORCH_LIB
+
+
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
 
 
@@ -93,3 +122,4 @@ def build_host(verbose: bool = True) -> str:
 if __name__ == "__main__":
     build(extra=sys.argv[1:])
     build_host()
+    build_torch()

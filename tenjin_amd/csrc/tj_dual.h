@@ -31,11 +31,6 @@ struct Dual {
     }
 };
 
-template <typename T>
-struct is_dual { static constexpr bool value = false; };
-template <int K>
-struct is_dual<Dual<K>> { static constexpr bool value = true; };
-
 // 1/b for derivative parts: v_rcp_f64's estimate and two Newton steps
 __device__ __forceinline__ double drcp(double b) {
     double r = __builtin_amdgcn_rcp(b);

@@ -69,7 +69,8 @@ std::vector<int> chop_dim(int n, int chunk) {
     return starts;
 }
 
-// AccessPattern (accesspattern.cpp:44-79), numberOfMixtures = 0.
+// AccessPattern (accesspattern.cpp:44-79); the mixture block (RHO_MIX,
+// LAMBDA, ALPHARHOLAMBDA) when a material is a MixtureEOS.
 int build_layout(const tj_level_desc& d, int cs[NVAR]) {
     for (int v = 0; v < NVAR; ++v) cs[v] = -1;
     int n = 0;

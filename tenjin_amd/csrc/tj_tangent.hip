@@ -256,7 +256,10 @@ int tj_tangent_run(tj_tangent* tg, const tj_mf* P0, int shift, tj_mf* S, tj_mf* 
     tj_ctx* c = tg->c;
     int r;
     if ((r = check_mf(c, P0)) || (r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
-    if (S->data == T->data) { c->err = "tangent: S and T must be distinct buffers"; return TJ_EINVAL; }
+    if (S->data == T->data || P0->data == S->data || P0->data == T->data) {
+        c->err = "tangent: P0, S and T must be distinct buffers (every group restarts from P0)";
+        return TJ_EINVAL;
+    }
     if (nsteps < 0) { c->err = "tangent: nsteps < 0"; return TJ_EINVAL; }
     const Level& lv = c->lv;
     if (probe_comp < 0 || probe_comp >= tg->ncomp || pi < 0 || pi >= lv.n[0] || pj < 0 || pj >= lv.n[1] || pk < 0 ||

@@ -151,6 +151,37 @@ def _autograd_function():
     return ProbeLoss
 
 
+def libtorch_module():
+    """The libtorch extension (tenjin_amd/tenjin_torch.so, built by
+    tenjin_amd.build.build_torch): ProbeLoss as a C++ torch::autograd::Function."""
+    import importlib
+
+    import torch  # noqa: F401  (the extension links libtorch)
+
+    return importlib.import_module("tenjin_amd.tenjin_torch")
+
+
+def probe_loss_libtorch(theta, target: np.ndarray, n: int = 512, steps: int = 200, chunk: int = 1, seed: int = 4):
+    """probe_loss_torch through the C++ torch::autograd::Function of the
+    libtorch extension: the level, its initial primitives and the probe come
+    from the differentiated plate impact; theta (a torch tensor of 6 values)
+    sets its materials inside the Function, so loss.backward() fills
+    theta.grad."""
+    import ctypes as ct
+
+    import torch
+
+    from . import level_desc
+
+    pb = configs.plate_impact_differentiated(n=n, seed=seed, steps=steps)
+    d = level_desc(pb)
+    desc = torch.frombuffer(bytearray(ct.string_at(ct.addressof(d), ct.sizeof(d))), dtype=torch.uint8)
+    prims = torch.from_numpy(np.ascontiguousarray(pb.prims, dtype=np.float64)).to("cuda")
+    tgt = torch.as_tensor(np.asarray(target, dtype=np.float64))
+    probe = list(configs.plate_probe(pb))
+    return libtorch_module().probe_loss(theta, prims, tgt, desc, steps, probe, chunk, pb.shift_pressure)
+
+
 def probe_loss_torch(theta, target: np.ndarray, n: int = 512, steps: int = 200, chunk: int = 1):
     """The calibration loss as a differentiable function of a torch tensor
     theta (6 values); loss.backward() fills theta.grad."""

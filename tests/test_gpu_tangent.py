@@ -351,6 +351,26 @@ def test_autograd_wrapper():
     assert np.all(np.isfinite(ref_grad)) and np.any(ref_grad != 0.0)
 
 
+def test_libtorch_autograd_function():
+    """The C++ torch::autograd::Function gives the loss and gradient of the
+    Python path (same tangent run; the sums differ in order only)."""
+    import torch
+
+    from tenjin_amd import configs, tangent
+
+    steps = 20
+    pb = _problem(steps=steps)
+    probe = configs.plate_probe(pb)
+    target = tangent.forward_trace(_problem(configs.plate_target_theta(), steps=steps), steps, probe)
+    theta = torch.tensor(configs.plate_params(), dtype=torch.float64, requires_grad=True)
+    print("\n[case] libtorch ProbeLoss", flush=True)
+    loss = tangent.probe_loss_libtorch(theta, target, n=N, steps=steps, chunk=2)
+    (3.0 * loss).backward()
+    ref_loss, ref_grad = tangent.loss_and_grad(configs.plate_params(), target, n=N, steps=steps, chunk=1)
+    assert abs(float(loss) - ref_loss) <= 1e-12 * ref_loss
+    assert np.allclose(theta.grad.numpy(), 3.0 * ref_grad, rtol=1e-10, atol=1e-12 * np.max(np.abs(3.0 * ref_grad)))
+
+
 def test_refusals():
     from tenjin_amd import TenjinError, configs, tangent
 
@@ -366,4 +386,7 @@ def test_refusals():
     tr = tangent.TangentRun(_problem(n=16), [(0, "s")])
     with pytest.raises(TenjinError, match="probe"):
         tr.run(2, (10, 16, 0, 0))
+    tr.vP0 = tr.g.vS  # the initial primitives must not alias the state
+    with pytest.raises(TenjinError, match="distinct"):
+        tr.run(2, (10, 3, 3, 0))
     tr.close()

@@ -91,3 +91,24 @@ def test_probe_loss():
     from tenjin_amd import tangent
 
     assert tangent.probe_loss([1.0, 2.0, 3.0], [1.0, 1.0, 1.0]) == 5.0
+
+
+def test_libtorch_extension_loads_and_checks_its_inputs():
+    """The C++ torch::autograd::Function (tenjin_amd/tenjin_torch.so) loads
+    without a GPU and refuses host-resident primitives before touching one."""
+    import ctypes as ct
+
+    import torch
+
+    from tenjin_amd import level_desc, tangent
+
+    mod = tangent.libtorch_module()
+    pb = configs.plate_impact_differentiated(n=8, steps=2)
+    d = level_desc(pb)
+    desc = torch.frombuffer(bytearray(ct.string_at(ct.addressof(d), ct.sizeof(d))), dtype=torch.uint8)
+    theta = torch.tensor(configs.plate_params(), dtype=torch.float64, requires_grad=True)
+    prims = torch.from_numpy(pb.prims)
+    with pytest.raises(RuntimeError, match="GPU"):
+        mod.probe_loss(theta, prims, torch.zeros(2, dtype=torch.float64), desc, 2, [10, 5, 4, 0], 1, 1)
+    with pytest.raises(RuntimeError, match="theta"):
+        mod.probe_loss(theta[:5], prims, torch.zeros(2, dtype=torch.float64), desc, 2, [10, 5, 4, 0], 1, 1)
