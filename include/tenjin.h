@@ -260,23 +260,23 @@ int tj_batch_clocks(tj_batch* batch, double* t, int* n, uint32_t* status);
 int tj_batch_synchronize(tj_batch* batch);
 
 /* ---- Tangent-linear (differentiated) form -----------------------------
- * Forward-mode derivatives of the solid path's time loop (main.cpp:121-191
- * with advance, plasticUpdate, cleanUpV, cleanUpAlpha and getTimeStep, as
- * tj_run) with respect to per-material parameters of the Us-up solid EOS and
- * the perfect-plastic return: every value is the FP64 form's, bit for bit,
- * and each direction carries d(state)/d(parameter) through init, flux, update,
- * plastic, V clean-up and the CFL time step.  The reference has no
- * differentiated form; this serves gradient-based calibration of material
- * parameters against probe traces.
- * Restrictions: the bound level of ctx is SOLID (no RADIAL); directions name
- * Us-up solid materials (TJ_TP_YIELD: plastic ones).  Directions run in
- * groups of `chunk` (1, 2, 3 or 6; 0 = all in one group up to 6): each group
- * re-runs the trajectory from the initial primitives.
+ * Forward-mode derivatives of the time loop (main.cpp:121-191 with advance,
+ * plasticUpdate, cleanUpV, cleanUpAlpha and getTimeStep, as tj_run) with
+ * respect to per-material EOS and plasticity parameters: every value is the
+ * FP64 form's, bit for bit, and each direction carries d(state)/d(parameter)
+ * through init, flux, update, plastic, V clean-up and the CFL time step.  The
+ * reference has no differentiated form; this serves gradient-based
+ * calibration of material parameters against probe traces.
+ * Levels: SOLID levels (5-wave path, no RADIAL), and fluid levels with MUSCL
+ * + HLLC (no THINC, no mixture, no RADIAL).  Directions run in groups of
+ * `chunk` (1, 2, 3 or 6; 0 = all in one group up to 6): each group re-runs
+ * the trajectory from the initial primitives.
  * On an error from tj_tangent_create, *tg is NULL. */
 enum {
-    TJ_TP_GAMMA0 = 0, /* Gruneisen coefficient Gamma0 (adiabaticIndex - 1) */
-    TJ_TP_S = 1,      /* Us-up Hugoniot slope s */
-    TJ_TP_YIELD = 2   /* yield stress (plastic.cpp:132-135) */
+    TJ_TP_GAMMA0 = 0, /* Gruneisen coefficient adiabaticIndex - 1 (Gamma0 of a Us-up solid) */
+    TJ_TP_S = 1,      /* Us-up Hugoniot slope s (Us-up solids) */
+    TJ_TP_YIELD = 2,  /* yield stress (plastic.cpp:132-135; plastic Us-up solids) */
+    TJ_TP_PREF = 3    /* pref of a MieGruneisen (stiffened-gas) material (equationofstate.cpp:37-115) */
 };
 typedef struct tj_tangent tj_tangent;
 int tj_tangent_create(tj_ctx* ctx, int ndir, const int* mat, const int* param, int chunk, tj_tangent** tg);

@@ -24,7 +24,7 @@ TJ_ST_NAN, TJ_ST_WAVESPEED, TJ_ST_ALPHA_NANS, TJ_ST_ALPHA_SUM, TJ_ST_PLASTIC_DIV
 TJ_OP_PLASTIC, TJ_OP_CLEAN_V, TJ_OP_CLEAN_ALPHA, TJ_OP_DT, TJ_OP_NANCHECK, TJ_OP_RADIAL, TJ_OP_REACTIVE = 1, 2, 4, 8, 16, 32, 64
 TJ_ST_MIX_BRACKET, TJ_ST_MIX_ITER = 32, 64
 TJ_ST_TANGENT_NAN = 128
-TJ_TP_GAMMA0, TJ_TP_S, TJ_TP_YIELD = 0, 1, 2
+TJ_TP_GAMMA0, TJ_TP_S, TJ_TP_YIELD, TJ_TP_PREF = 0, 1, 2, 3
 
 
 class Material(ctypes.Structure):

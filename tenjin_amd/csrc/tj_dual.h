@@ -6,10 +6,10 @@
 //
 // The derivative parts are not bound to the value's rounding: they use fused
 // multiply-adds and a reciprocal refined from the hardware estimate (two
-// Newton steps, a few ulp) instead of a second IEEE division.  The one
-// quotient whose denominator reaches the subnormals, the MUSCL ratio of
-// differences of exponentially small precursor values ahead of a wave (where
-// the estimate overflows), divides (muscl1 below).
+// Newton steps, a few ulp) instead of a second IEEE division.  The MUSCL
+// slope takes its derivative from the limiter's closed form (muscl1 below),
+// not through the ratio of differences, which ahead of a wave is a ratio of
+// exponentially small precursor values whose derivative overflows.
 //
 // Non-smooth points take one one-sided derivative: |x| and max/min follow the
 // branch the value takes, sqrt at 0 has derivative 0 (it is reached only at a
@@ -17,6 +17,7 @@
 // limiter's inf/NaN reset zeroes the derivative with the value.
 #pragma once
 #include "tj_device.h"
+#include "tj_fluid.h"
 
 namespace tj {
 
@@ -176,19 +177,24 @@ TJ_DUAL_FN Dual<K> trcbrt(const Dual<K>& a) {
     for (int t = 0; t < K; ++t) r.d[t] = g * a.d[t];
     return r;
 }
-// MUSCL with van Leer's limiter (hllc.cpp:527-575) on dual numbers: the
-// value parts as muscl1 of tj_fluid.h; the ratio's derivative parts by IEEE
-// division (its denominator can be subnormal)
+// MUSCL with van Leer's limiter (hllc.cpp:527-575) on dual numbers.  The
+// value parts are muscl1 of tj_fluid.h, operation for operation.  With
+// a = uc - um, b = up - uc and r = a / b, the limited slope
+// phi(r) (a + b) / 2 with phi = min(2r, 2) / (1 + r) for r > 0 is exactly a
+// for 0 < r < 1 and b for r >= 1 (0 for r <= 0): its derivative is da or db
+// along the branch the value takes.
 TJ_DUAL_FN void muscl1(Dual<K> um, Dual<K> uc, Dual<K> up, Dual<K>& L, Dual<K>& Rt) {
-    const Dual<K> num = uc - um, den = up - uc;
-    Dual<K> r;
-    r.v = num.v / den.v;
+    double r = tdiv(uc.v - um.v, up.v - uc.v);
+    if (tisinf(r) || tisnan(r)) r = 0.0;
+    const double g = vanleer(r) * 0.5 * (up.v - um.v);
+    L.v = uc.v - 0.5 * g;
+    Rt.v = uc.v + 0.5 * g;
 #pragma unroll
-    for (int t = 0; t < K; ++t) r.d[t] = (num.d[t] - r.v * den.d[t]) / den.v;
-    if (tisinf(r) || tisnan(r)) r = Dual<K>(0.0);
-    Dual<K> g = vanleer(r) * 0.5 * (up - um);
-    L = uc - 0.5 * g;
-    Rt = uc + 0.5 * g;
+    for (int t = 0; t < K; ++t) {
+        const double gd = r <= 0.0 ? 0.0 : (r < 1.0 ? uc.d[t] - um.d[t] : up.d[t] - uc.d[t]);
+        L.d[t] = uc.d[t] - 0.5 * gd;
+        Rt.d[t] = uc.d[t] + 0.5 * gd;
+    }
 }
 // true when any derivative part is NaN (the value part is checked separately)
 TJ_DUAL_FN bool tan_nan(const Dual<K>& a) {

@@ -1,7 +1,8 @@
-// Tangent-linear (forward-mode differentiated) form of the solid path: the
-// FP64 step of the solid kernels evaluated on Dual<K> numbers, carrying K
-// directional derivatives of the whole state with respect to per-material
-// EOS / plasticity parameters (Gamma0, s, yield stress).  Each kernel mirrors
+// Tangent-linear (forward-mode differentiated) form: the FP64 step of the
+// solid kernels, and of the MUSCL + HLLC fluid kernels, evaluated on Dual<K>
+// numbers, carrying K directional derivatives of the whole state with respect
+// to per-material EOS / plasticity parameters (Gamma0, s, yield stress; for
+// fluids the Gruneisen coefficient adiabaticIndex - 1 and pref).  Each kernel mirrors
 // its FP64 counterpart (k_init_p2c_s, k_sound_dt_s, k_flux_x_s /
 // k_flux_lmarch_s, k_update_s, k_clock_dt, k_clock_advance) operation for
 // operation, so the value parts equal the FP64 trajectory bit for bit.
@@ -440,6 +441,294 @@ __global__ void k_tclock_advance(Clock* clk, const double* tdt, double* tt) {
 #pragma unroll
     for (int t = 0; t < K; ++t) tt[t] += tdt[1 + t];
 }
+// ------------------------------------------------------------------ fluid levels
+// The MUSCL + HLLC path of fluid levels (no THINC, no mixture, no geometric
+// source), mirroring k_flux_x / k_flux_march (MUSCL only), update_body with
+// finish_cell, init_p2c_body and sound_dt_body.
+template <int NM, int K>
+__device__ __forceinline__ void load_tprim_f(const Level& lv, const DMF<K>& U, int i, int j, int k,
+                                             Prim<NM, Dual<K>>& P) {
+    bool f[3];
+    map3(lv, i, j, k, f);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        P.a[m] = U.get(lv.cs[ALPHA] + m, i, j, k);
+        P.r[m] = U.get(lv.cs[RHO_K] + m, i, j, k);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const Dual<K> v = U.get(lv.cs[VELOCITY] + r, i, j, k);
+        P.u[r] = f[r] ? -v : v;
+    }
+    P.p = U.get(lv.cs[PRESSURE], i, j, k);
+}
+
+template <int NM, int K, int D>
+__global__ __launch_bounds__(256) void k_tflux_f(Level lv, DMF<K> U, Faces F, Clock* clk) {
+    if (clk->done) return;
+    constexpr int NF = 2 * NM + 5;
+    using T = Dual<K>;
+    const int fx = F.nx, fy = F.ny, fz = F.nz;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)fx * fy * fz) return;
+    const int i = (int)(id % fx);
+    const int j = (int)((id / fx) % fy);
+    const int k = (int)(id / ((long long)fx * fy));
+    const int f = D == 0 ? i : (D == 1 ? j : k);
+    const int nd = lv.n[D];
+    const int cl = f == 0 ? 0 : f - 1, cr = f == nd ? nd - 1 : f;
+    auto at = [&](int c, Prim<NM, T>& P) {
+        load_tprim_f<NM, K>(lv, U, D == 0 ? c : i, D == 1 ? c : j, D == 2 ? c : k, P);
+    };
+    FSt<NM, T> A, B;
+    {
+        Prim<NM, T> Pm, Pc, Pp, PL, PR;
+        at(cl - 1, Pm);
+        at(cl, Pc);
+        at(cl + 1, Pp);
+        muscl<NM, T>(Pm, Pc, Pp, PL, PR);
+        p2c_sound<NM, T>(lv, PR, A);
+    }
+    {
+        Prim<NM, T> Pm, Pc, Pp, PL, PR;
+        at(cr - 1, Pm);
+        at(cr, Pc);
+        at(cr + 1, Pp);
+        muscl<NM, T>(Pm, Pc, Pp, PL, PR);
+        p2c_sound<NM, T>(lv, PL, B);
+    }
+    if (f == 0 && lv.bc_lo[D]) flip_state<NM, T>(A, D);
+    if (f == nd && lv.bc_hi[D]) flip_state<NM, T>(B, D);
+    T Fv[NF], sa[NM];
+    hllc<NM, T>(A, B, D, Fv, sa);
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        F.p[q * F.cs + id] = Fv[q].v;
+#pragma unroll
+        for (int t = 0; t < K; ++t) F.p[(NF * (t + 1) + q) * F.cs + id] = Fv[q].d[t];
+    }
+}
+
+// conservativeToPrimitive (c2p of tj_fluid.h, no mixture) on dual numbers
+template <int NM, int K>
+__device__ __forceinline__ void tc2p_f(const Level& lv, const Dual<K>* al, const Dual<K>* ar, const Dual<K>* ru,
+                                       Dual<K> E, Dual<K>* rk, Dual<K>& rho, Dual<K>* u, Dual<K>& p) {
+    using T = Dual<K>;
+    rho = T(0.0);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        rk[m] = tdiv(ar[m], al[m]);
+        rho += ar[m];
+    }
+    T ke = T(0.0);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        u[q] = tdiv(ru[q], rho);
+        ke += 0.5 * rho * u[q] * u[q];
+    }
+    T G, Ent, Pnt;
+    eos_sums<NM, T>(lv, al, ar, G, Ent, Pnt);
+    p = tdiv(E - ke - Ent + Pnt, G);
+}
+// getSoundSpeed (sound of tj_fluid.h, no mixture) on dual numbers
+template <int NM, int K>
+__device__ __forceinline__ Dual<K> tsound_f(const Level& lv, const Dual<K>* al, const Dual<K>* ar, const Dual<K>* rk,
+                                            Dual<K> rho, Dual<K> p) {
+    using T = Dual<K>;
+    T a = T(0.0), xt = T(0.0);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        const MatT<T>& M = mat<T>(lv, m);
+        const T c2 = tdiv(p * M.gp1, rk[m]) - tdiv(M.pref, rk[m]);
+        a += tmax(T(0.0), tdiv(M.xi * c2 * ar[m], rho));
+        xt += M.xi * al[m];
+    }
+    if (a <= 0.0) a = T(1E-6);
+    return tsqrt(tdiv(a, xt));
+}
+template <int NM, int K>
+__device__ __forceinline__ double tcand_f(const Level& lv, const DMF<K>& U, Dual<K> a, const Dual<K>* u, int i, int j,
+                                          int k, long long id, double* cand) {
+    using T = Dual<K>;
+    U.put(lv.cs[SOUNDSPEED], i, j, k, a);
+    T c = T(__builtin_huge_val());
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (d < lv.dim) c = tmin(c, tdiv(lv.dx[d], a + tabs(u[d])));
+    const long long nc = lv.ncells;
+    cand[id] = c.v;
+#pragma unroll
+    for (int t = 0; t < K; ++t) cand[(t + 1) * nc + id] = c.d[t];
+    return c.v;
+}
+
+template <int NM, int STAGE, int K>
+__global__ __launch_bounds__(256) void k_tupdate_f(Level lv, DMF<K> Uin, DMF<K> Ub, DMF<K> Uo, Faces F0, Faces F1,
+                                                   Faces F2, Clock* clk, const double* tdt, unsigned ops, double* cand) {
+    if (clk->done) return;
+    using T = Dual<K>;
+    constexpr int NF = 2 * NM + 5;
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)nx * ny * nz;
+    double cv = __builtin_huge_val();
+    unsigned st = 0;
+    if (id < ncell) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        T dt;
+        dt.v = clk->dt;
+#pragma unroll
+        for (int t = 0; t < K; ++t) dt.d[t] = tdt[1 + t];
+        T al[NM], ar[NM], ru[3], E, a0[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            al[m] = Uin.get(lv.cs[ALPHA] + m, i, j, k);
+            ar[m] = Uin.get(lv.cs[ALPHARHO] + m, i, j, k);
+            a0[m] = al[m];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ru[q] = Uin.get(lv.cs[RHOU] + q, i, j, k);
+        E = Uin.get(lv.cs[TOTAL_E], i, j, k);
+        for (int d = 0; d < lv.dim; ++d) {
+            const Faces& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
+            const long long lo = ((long long)k * Fd.ny + j) * Fd.nx + i;
+            const long long hi = ((long long)(k + (d == 2)) * Fd.ny + (j + (d == 1))) * Fd.nx + (i + (d == 0));
+            auto fv = [&](int q, long long o) {
+                T x;
+                x.v = Fd.p[q * Fd.cs + o];
+#pragma unroll
+                for (int t = 0; t < K; ++t) x.d[t] = Fd.p[(NF * (t + 1) + q) * Fd.cs + o];
+                return x;
+            };
+            const T dtdx = tdiv(dt, lv.dx[d]);
+            const T s0 = fv(2 * NM + 4, lo), s1 = fv(2 * NM + 4, hi);
+#pragma unroll
+            for (int m = 0; m < NM; ++m) al[m] += dtdx * (fv(m, lo) - fv(m, hi) - a0[m] * (s0 - s1));
+#pragma unroll
+            for (int m = 0; m < NM; ++m) ar[m] += dtdx * (fv(NM + m, lo) - fv(NM + m, hi));
+#pragma unroll
+            for (int q = 0; q < 3; ++q) ru[q] += dtdx * (fv(2 * NM + q, lo) - fv(2 * NM + q, hi));
+            E += dtdx * (fv(2 * NM + 3, lo) - fv(2 * NM + 3, hi));
+        }
+        T rk[NM], rho, u[3], p;
+        tc2p_f<NM, K>(lv, al, ar, ru, E, rk, rho, u, p);
+        if (STAGE == 2) {
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                al[m] = Ub.get(lv.cs[ALPHA] + m, i, j, k) * (1.0 / 2.0) + al[m] * (1.0 / 2.0);
+                ar[m] = Ub.get(lv.cs[ALPHARHO] + m, i, j, k) * (1.0 / 2.0) + ar[m] * (1.0 / 2.0);
+                rk[m] = Ub.get(lv.cs[RHO_K] + m, i, j, k) * (1.0 / 2.0) + rk[m] * (1.0 / 2.0);
+            }
+            rho = Ub.get(lv.cs[RHO], i, j, k) * (1.0 / 2.0) + rho * (1.0 / 2.0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                ru[q] = Ub.get(lv.cs[RHOU] + q, i, j, k) * (1.0 / 2.0) + ru[q] * (1.0 / 2.0);
+                u[q] = Ub.get(lv.cs[VELOCITY] + q, i, j, k) * (1.0 / 2.0) + u[q] * (1.0 / 2.0);
+            }
+            E = Ub.get(lv.cs[TOTAL_E], i, j, k) * (1.0 / 2.0) + E * (1.0 / 2.0);
+            p = Ub.get(lv.cs[PRESSURE], i, j, k) * (1.0 / 2.0) + p * (1.0 / 2.0);
+            if (ops & 4u) st |= cleanup_alpha<NM, T>(al);
+            if (ops & 16u) {
+                bool bad = false, tbad = false;
+#pragma unroll
+                for (int m = 0; m < NM; ++m) {
+                    bad |= tisnan(al[m]) || tisnan(ar[m]) || tisnan(rk[m]);
+                    tbad |= tan_nan(al[m]) || tan_nan(ar[m]) || tan_nan(rk[m]);
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    bad |= tisnan(ru[q]) || tisnan(u[q]);
+                    tbad |= tan_nan(ru[q]) || tan_nan(u[q]);
+                }
+                bad |= tisnan(E) || tisnan(p);
+                tbad |= tan_nan(E) || tan_nan(p);
+                if (bad) st |= 1u;
+                if (tbad) st |= 128u;
+            }
+            if (ops & 8u) cv = tcand_f<NM, K>(lv, Uo, tsound_f<NM, K>(lv, al, ar, rk, rho, p), u, i, j, k, id, cand);
+        }
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            Uo.put(lv.cs[ALPHA] + m, i, j, k, al[m]);
+            Uo.put(lv.cs[ALPHARHO] + m, i, j, k, ar[m]);
+            Uo.put(lv.cs[RHO_K] + m, i, j, k, rk[m]);
+        }
+        Uo.put(lv.cs[RHO], i, j, k, rho);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            Uo.put(lv.cs[RHOU] + q, i, j, k, ru[q]);
+            Uo.put(lv.cs[VELOCITY] + q, i, j, k, u[q]);
+        }
+        Uo.put(lv.cs[TOTAL_E], i, j, k, E);
+        Uo.put(lv.cs[PRESSURE], i, j, k, p);
+    }
+    if (STAGE == 2 && (ops & 8u)) block_min_to(cv, &clk->dtmin_bits);
+    if (st) atomicOr(&clk->status, st);
+}
+
+template <int NM, int K>
+__global__ __launch_bounds__(256) void k_tsound_dt_f(Level lv, DMF<K> U, Clock* clk, double* cand) {
+    using T = Dual<K>;
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double cv = __builtin_huge_val();
+    if (id < (long long)nx * ny * nz) {
+        const int i = (int)(id % nx);
+        const int j = (int)((id / nx) % ny);
+        const int k = (int)(id / ((long long)nx * ny));
+        T al[NM], ar[NM], rk[NM], u[3];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            al[m] = U.get(lv.cs[ALPHA] + m, i, j, k);
+            ar[m] = U.get(lv.cs[ALPHARHO] + m, i, j, k);
+            rk[m] = U.get(lv.cs[RHO_K] + m, i, j, k);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) u[q] = U.get(lv.cs[VELOCITY] + q, i, j, k);
+        const T a = tsound_f<NM, K>(lv, al, ar, rk, U.get(lv.cs[RHO], i, j, k), U.get(lv.cs[PRESSURE], i, j, k));
+        cv = tcand_f<NM, K>(lv, U, a, u, i, j, k, id, cand);
+    }
+    block_min_to(cv, &clk->dtmin_bits);
+}
+
+template <int NM, int K>
+__global__ __launch_bounds__(256) void k_tinit_p2c_f(Level lv, DMF<K> U, int shift) {
+    using T = Dual<K>;
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const int k = (int)(id / ((long long)nx * ny));
+    Prim<NM, T> P;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        P.a[m] = U.get(lv.cs[ALPHA] + m, i, j, k);
+        P.r[m] = U.get(lv.cs[RHO_K] + m, i, j, k);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) P.u[q] = U.get(lv.cs[VELOCITY] + q, i, j, k);
+    P.p = U.get(lv.cs[PRESSURE], i, j, k);
+    if (shift) {
+        T ar[NM], G, Ent, Pnt;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) ar[m] = P.a[m] * P.r[m];
+        eos_sums<NM, T>(lv, P.a, ar, G, Ent, Pnt);
+        P.p += tdiv(Pnt, G);
+        U.put(lv.cs[PRESSURE], i, j, k, P.p);
+    }
+    FSt<NM, T> S;
+    p2c_sound<NM, T>(lv, P, S);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) U.put(lv.cs[ALPHARHO] + m, i, j, k, S.ar[m]);
+    U.put(lv.cs[RHO], i, j, k, S.rho);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) U.put(lv.cs[RHOU] + q, i, j, k, S.u[q] * S.rho);
+    U.put(lv.cs[TOTAL_E], i, j, k, S.E);
+}
+
 // Z = X over ncomp components of the valid cells (views of any strides)
 __global__ __launch_bounds__(256) void k_copy_mf(MF X, MF Z, int ncomp, int nx, int ny, int nz) {
     const long long ncell = (long long)nx * ny * nz;

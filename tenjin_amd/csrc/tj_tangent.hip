@@ -87,6 +87,9 @@ void fill_mats(const tj_tangent* tg, int first, int cnt, MatT<Dual<K>>* M) {
             case TJ_TP_S:
                 M[m].s.d[q] = 1.0;
                 break;
+            case TJ_TP_PREF:
+                M[m].pref.d[q] = 1.0;
+                break;
             default:
                 M[m].yield.d[q] = 1.0;
                 break;
@@ -116,22 +119,32 @@ int run_group(tj_tangent* tg, int first, int cnt, const tj_mf* P0, int shift, tj
                        ny, nz);
     HIPCHK(c, hipMemsetAsync(tg->d_tan[0], 0, sizeof(double) * bs * K, st));
     HIPCHK(c, hipMemsetAsync(tg->d_tan[1], 0, sizeof(double) * bs * K, st));
-    hipLaunchKernelGGL((k_tinit_p2c_s<2, K>), dim3(gc), dim3(256), 0, st, lv, dS, shift);
+    const bool solid = lv.solid != 0;
+    if (solid) hipLaunchKernelGGL((k_tinit_p2c_s<2, K>), dim3(gc), dim3(256), 0, st, lv, dS, shift);
+    else hipLaunchKernelGGL((k_tinit_p2c_f<2, K>), dim3(gc), dim3(256), 0, st, lv, dS, shift);
     // tj_begin_run: clock reset, final time, getTimeStep of the initial state
     int r;
     if ((r = tj_set_dt(c, 0.0, 1))) return r;
     hipLaunchKernelGGL(k_tclock_begin<K>, dim3(1), dim3(64), 0, st, c->d_clk, final_time, tg->d_arg, tg->d_tt);
-    hipLaunchKernelGGL((k_tsound_dt_s<2, K>), dim3(gc), dim3(256), 0, st, lv, dS, c->d_clk, tg->d_cand);
+    if (solid) hipLaunchKernelGGL((k_tsound_dt_s<2, K>), dim3(gc), dim3(256), 0, st, lv, dS, c->d_clk, tg->d_cand);
+    else hipLaunchKernelGGL((k_tsound_dt_f<2, K>), dim3(gc), dim3(256), 0, st, lv, dS, c->d_clk, tg->d_cand);
     hipLaunchKernelGGL(k_targmin, dim3(gc), dim3(256), 0, st, c->d_clk, tg->d_cand, nc, tg->d_arg);
-    unsigned ops = TJ_OP_CLEAN_ALPHA | TJ_OP_DT | TJ_OP_NANCHECK | TJ_OP_CLEAN_V;
+    unsigned ops = TJ_OP_CLEAN_ALPHA | TJ_OP_DT | TJ_OP_NANCHECK;
+    if (solid) ops |= TJ_OP_CLEAN_V;
     if (c->desc.PLASTIC) ops |= TJ_OP_PLASTIC;
     auto fluxes = [&](const DMF<K>& U) {
         for (int d = 0; d < lv.dim; ++d) {
             const Faces& F = tg->F[d];
             const unsigned g = grid1((long long)F.nx * F.ny * F.nz, 256);
-            if (d == 0) hipLaunchKernelGGL((k_tflux_s<2, K, 0>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
-            else if (d == 1) hipLaunchKernelGGL((k_tflux_s<2, K, 1>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
-            else hipLaunchKernelGGL((k_tflux_s<2, K, 2>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+            if (solid) {
+                if (d == 0) hipLaunchKernelGGL((k_tflux_s<2, K, 0>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+                else if (d == 1) hipLaunchKernelGGL((k_tflux_s<2, K, 1>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+                else hipLaunchKernelGGL((k_tflux_s<2, K, 2>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+            } else {
+                if (d == 0) hipLaunchKernelGGL((k_tflux_f<2, K, 0>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+                else if (d == 1) hipLaunchKernelGGL((k_tflux_f<2, K, 1>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+                else hipLaunchKernelGGL((k_tflux_f<2, K, 2>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+            }
         }
     };
     const Faces& F0 = tg->F[0];
@@ -141,11 +154,19 @@ int run_group(tj_tangent* tg, int first, int cnt, const tj_mf* P0, int shift, tj
         hipLaunchKernelGGL(k_tclock_dt<K>, dim3(1), dim3(64), 0, st, c->d_clk, lv.cfl, tg->d_cand, nc, tg->d_arg,
                            tg->d_tdt, tg->d_tt);
         fluxes(dS);
-        hipLaunchKernelGGL((k_tupdate_s<2, 1, K>), dim3(gc), dim3(256), 0, st, lv, dS, dS, dT, F0, F1, F2, c->d_clk,
-                           tg->d_tdt, 0u, tg->d_cand);
+        if (solid)
+            hipLaunchKernelGGL((k_tupdate_s<2, 1, K>), dim3(gc), dim3(256), 0, st, lv, dS, dS, dT, F0, F1, F2,
+                               c->d_clk, tg->d_tdt, 0u, tg->d_cand);
+        else
+            hipLaunchKernelGGL((k_tupdate_f<2, 1, K>), dim3(gc), dim3(256), 0, st, lv, dS, dS, dT, F0, F1, F2,
+                               c->d_clk, tg->d_tdt, 0u, tg->d_cand);
         fluxes(dT);
-        hipLaunchKernelGGL((k_tupdate_s<2, 2, K>), dim3(gc), dim3(256), 0, st, lv, dT, dS, dS, F0, F1, F2, c->d_clk,
-                           tg->d_tdt, ops, tg->d_cand);
+        if (solid)
+            hipLaunchKernelGGL((k_tupdate_s<2, 2, K>), dim3(gc), dim3(256), 0, st, lv, dT, dS, dS, F0, F1, F2,
+                               c->d_clk, tg->d_tdt, ops, tg->d_cand);
+        else
+            hipLaunchKernelGGL((k_tupdate_f<2, 2, K>), dim3(gc), dim3(256), 0, st, lv, dT, dS, dS, F0, F1, F2,
+                               c->d_clk, tg->d_tdt, ops, tg->d_cand);
         hipLaunchKernelGGL(k_targmin, dim3(gc), dim3(256), 0, st, c->d_clk, tg->d_cand, nc, tg->d_arg);
         hipLaunchKernelGGL(k_tclock_advance<K>, dim3(1), dim3(64), 0, st, c->d_clk, tg->d_tdt, tg->d_tt);
         hipLaunchKernelGGL(k_tprobe<K>, dim3(1), dim3(64), 0, st, dS, probe[0], probe[1], probe[2], probe[3],
@@ -179,8 +200,8 @@ int tj_tangent_create(tj_ctx* c, int ndir, const int* mat, const int* param, int
     if (!out) return TJ_EINVAL;
     *out = nullptr;
     if (!c || !c->bound) return TJ_ENOLEVEL;
-    if (!c->desc.SOLID || c->desc.RADIAL) {
-        c->err = "tangent: solid levels without the geometric source only";
+    if (c->desc.RADIAL || c->lv.mixm >= 0 || (!c->desc.SOLID && c->desc.THINC)) {
+        c->err = "tangent: solid levels, and fluid levels with MUSCL + HLLC (no THINC, mixture or geometric source)";
         return TJ_EUNSUPPORTED;
     }
     if (ndir < 1 || !mat || !param) { c->err = "tangent: no directions"; return TJ_EINVAL; }
@@ -192,12 +213,17 @@ int tj_tangent_create(tj_ctx* c, int ndir, const int* mat, const int* param, int
     }
     for (int q = 0; q < ndir; ++q) {
         const int m = mat[q], p = param[q];
-        if (m < 0 || m >= c->nmat || p < TJ_TP_GAMMA0 || p > TJ_TP_YIELD) {
+        if (m < 0 || m >= c->nmat || p < TJ_TP_GAMMA0 || p > TJ_TP_PREF) {
             c->err = "tangent: bad direction (material or parameter out of range)";
             return TJ_EINVAL;
         }
-        if (c->lv.m[m].eos != 1 || !c->lv.m[m].solid || (p == TJ_TP_YIELD && !c->lv.m[m].plastic)) {
-            c->err = "tangent: directions name Us-up solid materials (yield stress: plastic ones)";
+        const MatC& M = c->lv.m[m];
+        if ((p == TJ_TP_S || p == TJ_TP_YIELD) && (M.eos != 1 || !M.solid || (p == TJ_TP_YIELD && !M.plastic))) {
+            c->err = "tangent: s and the yield stress belong to Us-up solid materials (yield stress: plastic ones)";
+            return TJ_EINVAL;
+        }
+        if (p == TJ_TP_PREF && M.eos != 0) {
+            c->err = "tangent: pref belongs to MieGruneisen (stiffened-gas) materials";
             return TJ_EINVAL;
         }
     }
@@ -209,7 +235,7 @@ int tj_tangent_create(tj_ctx* c, int ndir, const int* mat, const int* param, int
     tg->par.assign(param, param + ndir);
     tg->ncomp = tj_ncomp(c);
     const long long nc = c->lv.ncells;
-    const int NF = 3 * c->nmat + 17;
+    const int NF = c->desc.SOLID ? 3 * c->nmat + 17 : 2 * c->nmat + 5;
     auto fail = [&](hipError_t e, const char* what) {
         c->err = std::string("tangent: ") + what + ": " + hipGetErrorString(e);
         tg_free(tg);

@@ -1,6 +1,6 @@
-"""Tangent-linear (forward-mode differentiated) form of the solid path, and the
-probe-trace calibration loss of the differentiated plate impact with a
-torch.autograd.Function over it.
+"""Tangent-linear (forward-mode differentiated) form of the solid path and of
+the MUSCL + HLLC fluid path, and the probe-trace calibration loss of the
+differentiated plate impact with a torch.autograd.Function over it.
 
 The reference has no differentiated form.  Its time loop (main.cpp:121-191)
 is what is differentiated: every value of a tangent run is the FP64 form's,
@@ -19,14 +19,17 @@ import ctypes
 
 import numpy as np
 
-from . import (TJ_TP_GAMMA0, TJ_TP_S, TJ_TP_YIELD, GpuLevel, TenjinError, configs)
+from . import (TJ_TP_GAMMA0, TJ_TP_PREF, TJ_TP_S, TJ_TP_YIELD, GpuLevel, TenjinError, configs)
 
-PARAM_CODE = {"Gamma0": TJ_TP_GAMMA0, "s": TJ_TP_S, "Y": TJ_TP_YIELD}
+# "Gamma0" is the Gruneisen coefficient adiabaticIndex - 1 of any material
+# (d/dGamma0 = d/d adiabaticIndex); "pref" the stiffened-gas reference pressure
+PARAM_CODE = {"Gamma0": TJ_TP_GAMMA0, "gamma": TJ_TP_GAMMA0, "s": TJ_TP_S, "Y": TJ_TP_YIELD, "pref": TJ_TP_PREF}
 
 
 class TangentRun:
-    """A solid level with its tangent-linear form.  directions: (material,
-    parameter) pairs, parameter one of "Gamma0", "s", "Y" (or TJ_TP_* codes).
+    """A level (solid, or fluid with MUSCL + HLLC) with its tangent-linear
+    form.  directions: (material, parameter) pairs, parameter one of
+    "Gamma0" (alias "gamma"), "s", "Y", "pref" (or TJ_TP_* codes).
     chunk: directions per pass group (1, 2, 3 or 6; 0 = all in one group up to
     6); every group re-runs the trajectory."""
 

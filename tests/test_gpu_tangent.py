@@ -371,11 +371,65 @@ def test_libtorch_autograd_function():
     assert np.allclose(theta.grad.numpy(), 3.0 * ref_grad, rtol=1e-10, atol=1e-12 * np.max(np.abs(3.0 * ref_grad)))
 
 
+def _fluid_fd(make, m, field, rel, steps, probe):
+    """Central difference of the oracle's probe trace in material m's
+    `field` ("gamma" = adiabaticIndex, i.e. Gamma0 + 1, or "pref")."""
+    pb0 = make()
+    v0 = getattr(pb0.materials[m], field)
+    h = rel * abs(v0)
+    us = []
+    for sgn in (1.0, -1.0):
+        pb = make()
+        setattr(pb.materials[m], field, v0 + sgn * h)
+        us.append(_oracle_trace(pb, steps, probe)[0])
+    return (us[0] - us[1]) / (2 * h)
+
+
+@pytest.mark.parametrize("case", ["tube1d", "inclusion2d"])
+def test_fluid_derivatives_against_oracle_fd(case):
+    """The MUSCL + HLLC fluid path: values bit-exact, derivatives with respect
+    to the Gruneisen coefficients and the water's pref against the oracle."""
+    from tenjin_amd import configs, tangent
+
+    if case == "tube1d":
+        make = lambda: configs.shock_tube_1d(200)  # noqa: E731
+        steps = 60
+        pb = make()
+        probe = (pb.layout.comp(configs.P), 150, 0, 0)
+        dirs = [(0, "Gamma0"), (0, "pref"), (1, "Gamma0")]
+    else:
+        make = lambda: configs.shock_inclusion_2d(nx=64, ny=32, box=16, thinc=0)  # noqa: E731
+        steps = 40
+        pb = make()
+        probe = (pb.layout.comp(configs.VELOCITY, 0, 0), 26, 13, 0)
+        dirs = [(0, "Gamma0"), (1, "Gamma0")]
+    print(f"\n[case] fluid tangent {case}, steps={steps}", flush=True)
+    tr = tangent.TangentRun(pb, dirs, chunk=len(dirs))
+    trace, done = tr.run(steps, probe)
+    st = tr.status()
+    S = tr.state()
+    tr.close()
+    ov, So = _oracle_trace(pb, steps, probe)
+    from tests.parity_util import exact_fraction
+
+    assert done == steps and st == 0
+    assert np.array_equal(trace[:, 0], ov) and exact_fraction(pb, So, S) == 1.0
+    for col, (m, par) in enumerate(dirs, start=1):
+        fd = _fluid_fd(make, m, "gamma" if par == "Gamma0" else "pref", 1e-6, steps, probe)
+        err = np.max(np.abs(trace[:, col] - fd)) / np.max(np.abs(fd))
+        print(f"[case] {case} material {m} {par}: {err:.3e} (scale {np.max(np.abs(fd)):.3e})", flush=True)
+        assert err <= FD_BAR
+
+
 def test_refusals():
     from tenjin_amd import TenjinError, configs, tangent
 
-    with pytest.raises(TenjinError, match="solid"):
-        tangent.TangentRun(configs.shock_tube_1d(64), [(0, "Gamma0")])
+    with pytest.raises(TenjinError, match="THINC"):
+        tangent.TangentRun(configs.shock_inclusion_2d(nx=32, ny=16, box=16, thinc=1), [(0, "Gamma0")])
+    with pytest.raises(TenjinError, match="Us-up"):
+        tangent.TangentRun(configs.shock_tube_1d(64), [(0, "s")])
+    with pytest.raises(TenjinError, match="pref"):
+        tangent.TangentRun(_problem(n=16), [(0, "pref")])
     with pytest.raises(TenjinError, match="geometric source"):
         tangent.TangentRun(configs.rod_impact_axisymmetric(16, 16), [(0, "s")])
     elastic = configs.plate_impact(dim=2, n=16, box=16, plastic=False)
