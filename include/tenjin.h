@@ -268,7 +268,8 @@ int tj_batch_synchronize(tj_batch* batch);
  * reference has no differentiated form; this serves gradient-based
  * calibration of material parameters against probe traces.
  * Levels: SOLID levels (5-wave path, no RADIAL), and fluid levels with MUSCL
- * + HLLC (no THINC, no mixture, no RADIAL).  Directions run in groups of
+ * (+ THINC/BVD, whose history planes carry derivative parts) and HLLC (no
+ * mixture, no RADIAL).  Directions run in groups of
  * `chunk` (1, 2, 3 or 6; 0 = all in one group up to 6): each group re-runs
  * the trajectory from the initial primitives.
  * On an error from tj_tangent_create, *tg is NULL. */

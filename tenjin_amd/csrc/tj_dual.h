@@ -196,6 +196,31 @@ TJ_DUAL_FN void muscl1(Dual<K> um, Dual<K> uc, Dual<K> up, Dual<K>& L, Dual<K>& 
         Rt.d[t] = uc.d[t] + 0.5 * gd;
     }
 }
+// THINC's exp / tanh / cosh: values by the project routines the FP64 form
+// uses; exp' = exp, tanh' = 1 - tanh^2, cosh' = sinh
+TJ_DUAL_FN Dual<K> texp(const Dual<K>& a) {
+    Dual<K> r;
+    r.v = pm_exp(a.v);
+#pragma unroll
+    for (int t = 0; t < K; ++t) r.d[t] = r.v * a.d[t];
+    return r;
+}
+TJ_DUAL_FN Dual<K> ttanh(const Dual<K>& a) {
+    Dual<K> r;
+    r.v = pm_tanh(a.v);
+    const double g = 1.0 - r.v * r.v;
+#pragma unroll
+    for (int t = 0; t < K; ++t) r.d[t] = g * a.d[t];
+    return r;
+}
+TJ_DUAL_FN Dual<K> tcosh(const Dual<K>& a) {
+    Dual<K> r;
+    r.v = pm_cosh(a.v);
+    const double g = 0.5 * (pm_exp(a.v) - pm_exp(-a.v));
+#pragma unroll
+    for (int t = 0; t < K; ++t) r.d[t] = g * a.d[t];
+    return r;
+}
 // true when any derivative part is NaN (the value part is checked separately)
 TJ_DUAL_FN bool tan_nan(const Dual<K>& a) {
     bool b = false;

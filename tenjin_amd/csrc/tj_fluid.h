@@ -430,9 +430,10 @@ __device__ __forceinline__ void flip_state(FSt<NM, R>& S, int d) { S.u[d] = -S.u
 
 // ------------------------------------------------------------------ THINC
 // thinc.cpp:54-88 (mixedCell, Youngs normal in 2D), 130-186 (pass 1).
-// cell (i,j,k) along direction d; alphas are read from U with the boundary map.
-template <int NM, typename R>
-__device__ inline void arec(const Level& lv, const MF& U, int i, int j, int k, int d, R ynx, R yny, ARec<NM, R>& A) {
+// cell (i,j,k) along direction d; alphas are read from U with the boundary map
+// (V: the state view, MF, or the tangent-linear form's value + derivative view).
+template <int NM, typename R, typename V = MF>
+__device__ inline void arec(const Level& lv, const V& U, int i, int j, int k, int d, R ynx, R yny, ARec<NM, R>& A) {
     int di = d == 0, dj = d == 1, dk = d == 2;
     A.mixed = 0;
 #pragma unroll

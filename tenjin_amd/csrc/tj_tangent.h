@@ -1,5 +1,5 @@
 // Tangent-linear (forward-mode differentiated) form: the FP64 step of the
-// solid kernels, and of the MUSCL + HLLC fluid kernels, evaluated on Dual<K>
+// solid kernels, and of the MUSCL(+THINC/BVD) + HLLC fluid kernels, on Dual<K>
 // numbers, carrying K directional derivatives of the whole state with respect
 // to per-material EOS / plasticity parameters (Gamma0, s, yield stress; for
 // fluids the Gruneisen coefficient adiabaticIndex - 1 and pref).  Each kernel mirrors
@@ -441,10 +441,44 @@ __global__ void k_tclock_advance(Clock* clk, const double* tdt, double* tt) {
 #pragma unroll
     for (int t = 0; t < K; ++t) tt[t] += tdt[1 + t];
 }
+// alpha of a cell with the boundary map (load_alpha of tj_fluid.h), for arec
+template <typename R, int K>
+__device__ __forceinline__ R load_alpha(const Level& lv, const DMF<K>& U, int m, int i, int j, int k) {
+    bool f[3];
+    map3(lv, i, j, k, f);
+    return U.get(lv.cs[ALPHA] + m, i, j, k);
+}
+
+// THINC/BVD history planes (Hist) with derivative parts: plane blocks of
+// values, then K blocks of derivatives, each `*blk` doubles apart.
+struct THist {
+    double* urstar;
+    const double* star_in;
+    double* star_out;
+    const double* edge_in;
+    double* edge_out;
+    long long pstride, estride;  // plane strides (cells, faces of the direction)
+    long long pblk, eblk;        // derivative block strides (NM planes)
+};
+template <int K>
+__device__ __forceinline__ Dual<K> hget(const double* p, long long blk, long long idx) {
+    Dual<K> x;
+    x.v = p[idx];
+#pragma unroll
+    for (int t = 0; t < K; ++t) x.d[t] = p[(t + 1) * blk + idx];
+    return x;
+}
+template <int K>
+__device__ __forceinline__ void hput(double* p, long long blk, long long idx, const Dual<K>& x) {
+    p[idx] = x.v;
+#pragma unroll
+    for (int t = 0; t < K; ++t) p[(t + 1) * blk + idx] = x.d[t];
+}
+
 // ------------------------------------------------------------------ fluid levels
-// The MUSCL + HLLC path of fluid levels (no THINC, no mixture, no geometric
-// source), mirroring k_flux_x / k_flux_march (MUSCL only), update_body with
-// finish_cell, init_p2c_body and sound_dt_body.
+// The MUSCL(+THINC/BVD) + HLLC path of fluid levels (no mixture, no geometric
+// source), mirroring k_flux_x / k_flux_march, update_body with finish_cell,
+// init_p2c_body and sound_dt_body.
 template <int NM, int K>
 __device__ __forceinline__ void load_tprim_f(const Level& lv, const DMF<K>& U, int i, int j, int k,
                                              Prim<NM, Dual<K>>& P) {
@@ -463,8 +497,14 @@ __device__ __forceinline__ void load_tprim_f(const Level& lv, const DMF<K>& U, i
     P.p = U.get(lv.cs[PRESSURE], i, j, k);
 }
 
-template <int NM, int K, int D>
-__global__ __launch_bounds__(256) void k_tflux_f(Level lv, DMF<K> U, Faces F, Clock* clk) {
+// One face per thread.  With THINC (TH) the alpha faces of each of the two
+// cells go through the BVD selection of flux_x_body / flux_march_body: the
+// cell's and its neighbours' THINC reconstructions, the box-edge semantics and
+// the stale history planes (read with their derivative parts); the face then
+// writes the history of this pass: urstar of its upper cell (where mixed), the
+// star alpha of its upper cell and, at box-high faces, the edge plane.
+template <int NM, int K, int D, bool TH>
+__global__ __launch_bounds__(256) void k_tflux_f(Level lv, DMF<K> U, Faces F, THist h, Clock* clk) {
     if (clk->done) return;
     constexpr int NF = 2 * NM + 5;
     using T = Dual<K>;
@@ -476,27 +516,82 @@ __global__ __launch_bounds__(256) void k_tflux_f(Level lv, DMF<K> U, Faces F, Cl
     const int k = (int)(id / ((long long)fx * fy));
     const int f = D == 0 ? i : (D == 1 ? j : k);
     const int nd = lv.n[D];
+    const int nx = lv.n[0], ny = lv.n[1];
     const int cl = f == 0 ? 0 : f - 1, cr = f == nd ? nd - 1 : f;
-    auto at = [&](int c, Prim<NM, T>& P) {
-        load_tprim_f<NM, K>(lv, U, D == 0 ? c : i, D == 1 ? c : j, D == 2 ? c : k, P);
+    auto ci = [&](int c) { return D == 0 ? c : i; };
+    auto cj = [&](int c) { return D == 1 ? c : j; };
+    auto ck = [&](int c) { return D == 2 ? c : k; };
+    auto cellidx = [&](int c) { return ((long long)ck(c) * ny + cj(c)) * nx + ci(c); };
+    const long long cstride = D == 0 ? 1 : (D == 1 ? (long long)nx : (long long)nx * ny);
+    // face index of face g along D in the face grid of this direction
+    auto faceidx = [&](int g) {
+        return D == 0 ? ((long long)k * ny + j) * (nx + 1) + g
+                      : (D == 1 ? ((long long)k * (ny + 1) + g) * nx + i : ((long long)g * ny + j) * nx + i);
+    };
+    unsigned st = 0;
+    // reconstruct cell c (MUSCL, then THINC/BVD and cleanUpAlpha of the alpha
+    // faces); want_right: the UR face state, else UL
+    auto recon = [&](int c, bool want_right, FSt<NM, T>& S) {
+        Prim<NM, T> Pm, Pc, Pp, PL, PR;
+        load_tprim_f<NM, K>(lv, U, ci(c - 1), cj(c - 1), ck(c - 1), Pm);
+        load_tprim_f<NM, K>(lv, U, ci(c), cj(c), ck(c), Pc);
+        load_tprim_f<NM, K>(lv, U, ci(c + 1), cj(c + 1), ck(c + 1), Pp);
+        muscl<NM, T>(Pm, Pc, Pp, PL, PR);
+        if (TH) {
+            const uint8_t ef = lv.edge[D][c];
+            const bool blo = ef & 1, bhi = ef & 2;
+            const T ynx = T(lv.yn[0]), yny = T(lv.yn[1]);
+            ARec<NM, T> Acur;
+            arec<NM, T>(lv, U, ci(c), cj(c), ck(c), D, ynx, yny, Acur);
+            const int mixedFlag = (Acur.mixed >> (NM - 1)) & 1;
+            const long long cc = cellidx(c);
+            if (mixedFlag) {
+                ARec<NM, T> Aprev, Anext;
+                if (!blo) arec<NM, T>(lv, U, ci(c - 1), cj(c - 1), ck(c - 1), D, ynx, yny, Aprev);
+                if (!bhi) arec<NM, T>(lv, U, ci(c + 1), cj(c + 1), ck(c + 1), D, ynx, yny, Anext);
+                T URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
+#pragma unroll
+                for (int m = 0; m < NM; ++m) {
+                    if (blo) {
+                        URl[m] = load_alpha<T>(lv, U, m, ci(c - 1), cj(c - 1), ck(c - 1));
+                        TRl[m] = T(0.0);
+                    } else {
+                        URl[m] = Aprev.ura[m];
+                        TRl[m] = ((Aprev.mixed >> m) & 1) ? Aprev.thr[m]
+                                                          : hget<K>(h.urstar, h.pblk, m * h.pstride + cc - cstride);
+                    }
+                    if (bhi) {
+                        ULr[m] = load_alpha<T>(lv, U, m, ci(c + 1), cj(c + 1), ck(c + 1));
+                        TLr[m] = hget<K>(h.edge_in, h.eblk, m * h.estride + faceidx(c + 1));
+                    } else {
+                        ULr[m] = Anext.ula[m];
+                        TLr[m] = ((Anext.mixed >> m) & 1) ? Anext.thl[m]
+                                                          : hget<K>(h.star_in, h.pblk, m * h.pstride + cc + cstride);
+                    }
+                    if ((Acur.mixed >> m) & 1) {
+                        thl[m] = Acur.thl[m];
+                        thr[m] = Acur.thr[m];
+                    } else {
+                        thl[m] = hget<K>(h.star_in, h.pblk, m * h.pstride + cc);
+                        thr[m] = hget<K>(h.urstar, h.pblk, m * h.pstride + cc);
+                    }
+                }
+                bvd<NM, T>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
+            }
+            st |= cleanup_alpha<NM, T>(PL.a);
+            st |= cleanup_alpha<NM, T>(PR.a);
+            // urstar of this face's upper cell (its owner)
+            if (!want_right && c == f && f < nd) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+                    if ((Acur.mixed >> m) & 1) hput<K>(h.urstar, h.pblk, m * h.pstride + cc, Acur.thr[m]);
+            }
+        }
+        p2c_sound<NM, T>(lv, want_right ? PR : PL, S);
     };
     FSt<NM, T> A, B;
-    {
-        Prim<NM, T> Pm, Pc, Pp, PL, PR;
-        at(cl - 1, Pm);
-        at(cl, Pc);
-        at(cl + 1, Pp);
-        muscl<NM, T>(Pm, Pc, Pp, PL, PR);
-        p2c_sound<NM, T>(lv, PR, A);
-    }
-    {
-        Prim<NM, T> Pm, Pc, Pp, PL, PR;
-        at(cr - 1, Pm);
-        at(cr, Pc);
-        at(cr + 1, Pp);
-        muscl<NM, T>(Pm, Pc, Pp, PL, PR);
-        p2c_sound<NM, T>(lv, PL, B);
-    }
+    recon(cl, true, A);
+    recon(cr, false, B);
     if (f == 0 && lv.bc_lo[D]) flip_state<NM, T>(A, D);
     if (f == nd && lv.bc_hi[D]) flip_state<NM, T>(B, D);
     T Fv[NF], sa[NM];
@@ -507,6 +602,17 @@ __global__ __launch_bounds__(256) void k_tflux_f(Level lv, DMF<K> U, Faces F, Cl
 #pragma unroll
         for (int t = 0; t < K; ++t) F.p[(NF * (t + 1) + q) * F.cs + id] = Fv[q].d[t];
     }
+    if (TH) {
+        if (f < nd) {
+#pragma unroll
+            for (int m = 0; m < NM; ++m) hput<K>(h.star_out, h.pblk, m * h.pstride + cellidx(f), sa[m]);
+        }
+        if (f == nd || (f > 0 && (lv.edge[D][f - 1] & 2))) {
+#pragma unroll
+            for (int m = 0; m < NM; ++m) hput<K>(h.edge_out, h.eblk, m * h.estride + faceidx(f), sa[m]);
+        }
+    }
+    if (st) atomicOr(&clk->status, st);
 }
 
 // conservativeToPrimitive (c2p of tj_fluid.h, no mixture) on dual numbers

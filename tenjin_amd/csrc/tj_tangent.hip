@@ -23,6 +23,11 @@ struct tj_tangent {
     double* d_tt = nullptr;           // derivatives of t (chunk)
     double* d_trace = nullptr;
     long long trace_cap = 0;
+    // THINC history planes with derivative parts ((1 + chunk) x NM planes each)
+    double* d_urstar = nullptr;
+    double* d_star[2] = {nullptr, nullptr};
+    double* d_estar[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    long long estride[3] = {0, 0, 0};
     int ncomp = 0;
     int group_first = -1, group_cnt = 0;
 };
@@ -42,6 +47,11 @@ void tg_free(tj_tangent* tg) {
     f(tg->d_tdt);
     f(tg->d_tt);
     f(tg->d_trace);
+    f(tg->d_urstar);
+    for (int q = 0; q < 2; ++q) {
+        f(tg->d_star[q]);
+        for (int a = 0; a < 3; ++a) f(tg->d_estar[a][q]);
+    }
 }
 
 // The materials of a group: values from the bound level; derivative part q
@@ -117,6 +127,18 @@ int run_group(tj_tangent* tg, int first, int cnt, const tj_mf* P0, int shift, tj
     // P0 -> S, zero derivatives, setInitialConditions
     hipLaunchKernelGGL(k_copy_mf, dim3(grid1(nc * tg->ncomp, 256)), dim3(256), 0, st, view(P0), view(S), tg->ncomp, nx,
                        ny, nz);
+    // tj_reset_history: THINC history planes zero, pass counters at 0
+    const bool thinc = c->desc.THINC != 0;
+    const long long hp = (long long)c->nmat * nc;
+    if (thinc) {
+        HIPCHK(c, hipMemsetAsync(tg->d_urstar, 0, sizeof(double) * hp * (K + 1), st));
+        for (int q = 0; q < 2; ++q) {
+            HIPCHK(c, hipMemsetAsync(tg->d_star[q], 0, sizeof(double) * hp * (K + 1), st));
+            for (int a = 0; a < lv0.dim; ++a)
+                HIPCHK(c, hipMemsetAsync(tg->d_estar[a][q], 0, sizeof(double) * c->nmat * tg->estride[a] * (K + 1), st));
+        }
+    }
+    long long pass = 0, dpass[3] = {0, 0, 0};
     HIPCHK(c, hipMemsetAsync(tg->d_tan[0], 0, sizeof(double) * bs * K, st));
     HIPCHK(c, hipMemsetAsync(tg->d_tan[1], 0, sizeof(double) * bs * K, st));
     const bool solid = lv.solid != 0;
@@ -136,14 +158,32 @@ int run_group(tj_tangent* tg, int first, int cnt, const tj_mf* P0, int shift, tj
         for (int d = 0; d < lv.dim; ++d) {
             const Faces& F = tg->F[d];
             const unsigned g = grid1((long long)F.nx * F.ny * F.nz, 256);
+            THist h{};
+            if (thinc) {  // hist_for of tj_lib.hip
+                const int q = (int)(pass & 1), e = (int)(dpass[d] & 1);
+                h.urstar = tg->d_urstar;
+                h.star_in = tg->d_star[q];
+                h.star_out = tg->d_star[q ^ 1];
+                h.edge_in = tg->d_estar[d][e];
+                h.edge_out = tg->d_estar[d][e ^ 1];
+                h.pstride = nc;
+                h.estride = tg->estride[d];
+                h.pblk = hp;
+                h.eblk = (long long)c->nmat * tg->estride[d];
+                pass += 1;
+                dpass[d] += 1;
+            }
             if (solid) {
                 if (d == 0) hipLaunchKernelGGL((k_tflux_s<2, K, 0>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
                 else if (d == 1) hipLaunchKernelGGL((k_tflux_s<2, K, 1>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
                 else hipLaunchKernelGGL((k_tflux_s<2, K, 2>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+            } else if (thinc) {  // THINC levels are 1D or 2D
+                if (d == 0) hipLaunchKernelGGL((k_tflux_f<2, K, 0, true>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
+                else hipLaunchKernelGGL((k_tflux_f<2, K, 1, true>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
             } else {
-                if (d == 0) hipLaunchKernelGGL((k_tflux_f<2, K, 0>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
-                else if (d == 1) hipLaunchKernelGGL((k_tflux_f<2, K, 1>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
-                else hipLaunchKernelGGL((k_tflux_f<2, K, 2>), dim3(g), dim3(256), 0, st, lv, U, F, c->d_clk);
+                if (d == 0) hipLaunchKernelGGL((k_tflux_f<2, K, 0, false>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
+                else if (d == 1) hipLaunchKernelGGL((k_tflux_f<2, K, 1, false>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
+                else hipLaunchKernelGGL((k_tflux_f<2, K, 2, false>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
             }
         }
     };
@@ -200,8 +240,8 @@ int tj_tangent_create(tj_ctx* c, int ndir, const int* mat, const int* param, int
     if (!out) return TJ_EINVAL;
     *out = nullptr;
     if (!c || !c->bound) return TJ_ENOLEVEL;
-    if (c->desc.RADIAL || c->lv.mixm >= 0 || (!c->desc.SOLID && c->desc.THINC)) {
-        c->err = "tangent: solid levels, and fluid levels with MUSCL + HLLC (no THINC, mixture or geometric source)";
+    if (c->desc.RADIAL || c->lv.mixm >= 0) {
+        c->err = "tangent: solid levels, and fluid levels with MUSCL(+THINC) + HLLC (no mixture or geometric source)";
         return TJ_EUNSUPPORTED;
     }
     if (ndir < 1 || !mat || !param) { c->err = "tangent: no directions"; return TJ_EINVAL; }
@@ -262,6 +302,19 @@ int tj_tangent_create(tj_ctx* c, int ndir, const int* mat, const int* param, int
     if ((e = hipMalloc(&tg->d_arg, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc");
     if ((e = hipMalloc(&tg->d_tdt, sizeof(double) * (chunk + 1))) != hipSuccess) return fail(e, "hipMalloc");
     if ((e = hipMalloc(&tg->d_tt, sizeof(double) * chunk)) != hipSuccess) return fail(e, "hipMalloc");
+    if (c->desc.THINC) {
+        const long long hp = (long long)c->nmat * nc * (chunk + 1);
+        if ((e = hipMalloc(&tg->d_urstar, sizeof(double) * hp)) != hipSuccess) return fail(e, "hipMalloc");
+        for (int q = 0; q < 2; ++q) {
+            if ((e = hipMalloc(&tg->d_star[q], sizeof(double) * hp)) != hipSuccess) return fail(e, "hipMalloc");
+            for (int a = 0; a < c->lv.dim; ++a) {
+                tg->estride[a] = tg->F[a].cs;
+                if ((e = hipMalloc(&tg->d_estar[a][q], sizeof(double) * c->nmat * tg->estride[a] * (chunk + 1))) !=
+                    hipSuccess)
+                    return fail(e, "hipMalloc");
+            }
+        }
+    }
     *out = tg;
     return TJ_OK;
 }

@@ -1,5 +1,5 @@
 """Tangent-linear (forward-mode differentiated) form of the solid path and of
-the MUSCL + HLLC fluid path, and the probe-trace calibration loss of the
+the MUSCL(+THINC/BVD) + HLLC fluid path, and the probe-trace calibration loss of the
 differentiated plate impact with a torch.autograd.Function over it.
 
 The reference has no differentiated form.  Its time loop (main.cpp:121-191)
@@ -27,7 +27,7 @@ PARAM_CODE = {"Gamma0": TJ_TP_GAMMA0, "gamma": TJ_TP_GAMMA0, "s": TJ_TP_S, "Y": 
 
 
 class TangentRun:
-    """A level (solid, or fluid with MUSCL + HLLC) with its tangent-linear
+    """A level (solid, or fluid with MUSCL(+THINC) + HLLC) with its tangent-linear
     form.  directions: (material, parameter) pairs, parameter one of
     "Gamma0" (alias "gamma"), "s", "Y", "pref" (or TJ_TP_* codes).
     chunk: directions per pass group (1, 2, 3 or 6; 0 = all in one group up to

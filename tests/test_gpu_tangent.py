@@ -385,20 +385,30 @@ def _fluid_fd(make, m, field, rel, steps, probe):
     return (us[0] - us[1]) / (2 * h)
 
 
-@pytest.mark.parametrize("case", ["tube1d", "inclusion2d"])
+def _tube_thinc():
+    from tenjin_amd import configs
+
+    pb = configs.shock_tube_1d(200, 0.7, 64)  # 64-cell boxes: box-edge history planes
+    pb.THINC = 1
+    return pb
+
+
+@pytest.mark.parametrize("case", ["tube1d", "inclusion2d", "tube1d_thinc", "inclusion2d_thinc"])
 def test_fluid_derivatives_against_oracle_fd(case):
-    """The MUSCL + HLLC fluid path: values bit-exact, derivatives with respect
-    to the Gruneisen coefficients and the water's pref against the oracle."""
+    """The fluid path (MUSCL, or MUSCL + THINC/BVD with its history planes,
+    and HLLC): values bit-exact, derivatives with respect to the Gruneisen
+    coefficients and the water's pref against the oracle."""
     from tenjin_amd import configs, tangent
 
-    if case == "tube1d":
-        make = lambda: configs.shock_tube_1d(200)  # noqa: E731
+    if case.startswith("tube1d"):
+        make = _tube_thinc if case.endswith("thinc") else (lambda: configs.shock_tube_1d(200))
         steps = 60
         pb = make()
         probe = (pb.layout.comp(configs.P), 150, 0, 0)
         dirs = [(0, "Gamma0"), (0, "pref"), (1, "Gamma0")]
     else:
-        make = lambda: configs.shock_inclusion_2d(nx=64, ny=32, box=16, thinc=0)  # noqa: E731
+        th = 1 if case.endswith("thinc") else 0
+        make = lambda: configs.shock_inclusion_2d(nx=64, ny=32, box=16, thinc=th)  # noqa: E731
         steps = 40
         pb = make()
         probe = (pb.layout.comp(configs.VELOCITY, 0, 0), 26, 13, 0)
@@ -421,11 +431,26 @@ def test_fluid_derivatives_against_oracle_fd(case):
         assert err <= FD_BAR
 
 
+def test_thinc_direction_groups_are_bit_identical():
+    from tenjin_amd import configs, tangent
+
+    pb = configs.shock_inclusion_2d(nx=64, ny=32, box=16, thinc=1)
+    probe = (pb.layout.comp(configs.VELOCITY, 0, 0), 26, 13, 0)
+    dirs = [(0, "Gamma0"), (1, "Gamma0"), (0, "pref")]
+    print("\n[case] THINC tangent chunk 1 vs 3", flush=True)
+    out = []
+    for chunk in (1, 3):
+        tr = tangent.TangentRun(pb, dirs, chunk=chunk)
+        out.append(tr.run(30, probe)[0])
+        tr.close()
+    assert np.array_equal(out[0], out[1])
+
+
 def test_refusals():
     from tenjin_amd import TenjinError, configs, tangent
 
-    with pytest.raises(TenjinError, match="THINC"):
-        tangent.TangentRun(configs.shock_inclusion_2d(nx=32, ny=16, box=16, thinc=1), [(0, "Gamma0")])
+    with pytest.raises(TenjinError, match="mixture"):
+        tangent.TangentRun(configs.reactive_tube(1, 64), [(1, "Gamma0")])
     with pytest.raises(TenjinError, match="Us-up"):
         tangent.TangentRun(configs.shock_tube_1d(64), [(0, "s")])
     with pytest.raises(TenjinError, match="pref"):
