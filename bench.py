@@ -94,7 +94,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=["inclusion2d", "plate3d", "tube1d", "ensemble", "explosion_rz",
-                                           "reactive2d", "plate2d_grad"], default="inclusion2d")
+                                           "reactive2d", "plate2d_grad", "inclusion2d_grad"], default="inclusion2d")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 500 for inclusion2d, 40 for plate3d)")
     ap.add_argument("--warmup", type=int, default=10)
@@ -116,7 +116,8 @@ def parse():
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"inclusion2d": 500, "plate3d": 40, "tube1d": 1280, "ensemble": 200,
-                      "explosion_rz": 500, "reactive2d": 200, "plate2d_grad": 200}[args.workload]
+                      "explosion_rz": 500, "reactive2d": 200, "plate2d_grad": 200,
+                      "inclusion2d_grad": 100}[args.workload]
     return args
 
 
@@ -452,6 +453,113 @@ def bench_plate2d_grad(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_inclusion2d_grad(args):
+    """The gradient of a probe-trace loss on the 2D shock/inclusion (THINC)
+    with respect to the Gruneisen coefficients (adiabaticIndex - 1) of both
+    gases, by the tangent-linear form (2 directions).  The target trace comes
+    from both coefficients x 1.05; the probe is u_x at x = 0.53, mid-height,
+    which the incoming shock crosses.
+    Reports the gradient run's throughput, the forward run's, their ratio,
+    and the gradient against central differences of the FP64 form."""
+    import numpy as np
+    import torch
+    from tenjin_amd import GpuLevel, configs, tangent
+
+    steps = args.steps
+
+    def make(theta):
+        pb = configs.shock_inclusion_2d(nx=args.nx, ny=args.ny, box=args.box)
+        for m in range(2):
+            pb.materials[m].gamma = theta[m] + 1.0
+        return pb
+
+    theta = [m.gamma - 1.0 for m in configs.shock_inclusion_2d(nx=64, ny=32, box=16).materials]
+    pb = make(theta)
+    # x = 0.53 at mid-height: the shock (from x = 0.5) crosses it within the run
+    probe = (pb.layout.comp(configs.VELOCITY, 0, 0), int(0.53 * args.ny), args.ny // 2, 0)
+    target = tangent.forward_trace(make([t * 1.05 for t in theta]), steps, probe)
+    P0 = torch.from_numpy(pb.prims).to("cuda")
+
+    def forward_once():
+        g = GpuLevel(pb)
+        g.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.init_from_device(P0, pb.shift_pressure)
+        g.begin_run(float("inf"))
+        g.advance(steps)
+        g.sync()
+        el = time.perf_counter() - t0
+        g.close()
+        return el
+
+    forward_once()
+    t_fwd = min(forward_once() for _ in range(3))
+    dirs = [(0, "Gamma0"), (1, "Gamma0")]
+    trial = {}
+    for chunk in (1, 2):
+        tr = tangent.TangentRun(pb, dirs, chunk=chunk)
+        tr.run(2, probe)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr.run(10, probe)
+        trial[chunk] = time.perf_counter() - t0
+        tr.close()
+    chunk = min(trial, key=trial.get)
+    tr = tangent.TangentRun(pb, dirs, chunk=chunk)
+    tr.run(2, probe)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    trace, done = tr.run(steps, probe)
+    t_grad = time.perf_counter() - t0
+    st = tr.status()
+    tr.close()
+    r = trace[:, 0] - target
+    grad = 2.0 * (r @ trace[:, 1:])
+    fd = []
+    for p in range(2):
+        h = 1e-6 * abs(theta[p])
+        lpm = []
+        for sgn in (1.0, -1.0):
+            th = list(theta)
+            th[p] += sgn * h
+            lpm.append(tangent.probe_loss(tangent.forward_trace(make(th), steps, probe), target))
+        fd.append((lpm[0] - lpm[1]) / (2 * h))
+    out = {
+        "metric": "Mcell-updates/s on 1 GPU",
+        "value": round(pb.ncells * steps / t_grad / 1e6, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 2,
+        "ms_per_step": round(1e3 * t_grad / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"gradient of a probe-trace loss on the 2D shock/inclusion, {args.nx}x{args.ny} cells, "
+                               f"{pb.max_grid_size}^2 boxes, MUSCL+THINC+HLLC, {steps} steps, w.r.t. the Gruneisen "
+                               "coefficients of both gases by the tangent-linear form",
+                   "cells": pb.ncells, "precision": "fp64", "directions": 2, "chunk": chunk,
+                   "chunk_trial_s": {str(k): round(v, 4) for k, v in trial.items()}, "parallelism": "single GPU"},
+        "adjoint_over_forward": round(t_grad / t_fwd, 3),
+        "per_direction_over_forward": round(t_grad / t_fwd / 2, 3),
+        "forward_value": round(pb.ncells * steps / t_fwd / 1e6, 2),
+        "forward_ms": round(1e3 * t_fwd, 3),
+        "gradient_ms": round(1e3 * t_grad, 3),
+        "loss": float(r @ r),
+        "gradient": [float(f"{v:.9e}") for v in grad],
+        "gradient_rel_err_vs_central_differences": [float(f"{abs(a - b) / max(abs(b), 1e-300):.3e}")
+                                                    for a, b in zip(grad, fd)],
+        "status": st,
+        "steps_done": done,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -477,6 +585,9 @@ def main():
         return
     if args.workload == "plate2d_grad":
         bench_plate2d_grad(args)
+        return
+    if args.workload == "inclusion2d_grad":
+        bench_inclusion2d_grad(args)
         return
     if args.workload == "inclusion2d":
         pb = configs.shock_inclusion_2d(nx=args.nx, ny=args.ny, box=args.box)
