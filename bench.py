@@ -166,6 +166,22 @@ def read_traffic(workload: str, name: str):
     return None
 
 
+def read_valu_gflop(workload: str, name: str):
+    """FP64 GFLOP per launch of `name` from the committed rocprofv3 VALU
+    counter summary (profiles/valu_fp64_kernels.txt)."""
+    path = os.path.join(HERE, "profiles", "valu_fp64_kernels.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "valu_gflop_per_launch" and parts[1] == workload and parts[2] == name:
+            return float(parts[3])
+    return None
+
+
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
 def bench_tube1d(args):
     """The 1D shock tube (1000 cells): latency-bound, so the loop bodies run as
     replays of one captured hipGraph of 64 bodies; the eager device loop is
@@ -654,6 +670,13 @@ def main():
                 "avg_launch_ms": round(davg_s * 1e3, 4),
             },
         }
+        gfl = read_valu_gflop(args.workload, dname) if prec == TJ_PREC_FP64 else None
+        if gfl is not None:
+            # the FP64 flux kernels are VALU/latency-bound: their compute roof
+            tf = gfl / davg_s / 1e3
+            res["roofline"]["valu"] = {"bound": "fp64-valu", "achieved": round(tf, 2), "peak": FP64_VALU_PEAK_TFLOPS,
+                                       "unit": "TFLOP/s", "frac": round(tf / FP64_VALU_PEAK_TFLOPS, 4),
+                                       "flops_per_launch": round(gfl * 1e9)}
         return g, res
 
     def rel_l1(a, b):
