@@ -221,6 +221,16 @@ TJ_DUAL_FN Dual<K> tcosh(const Dual<K>& a) {
     for (int t = 0; t < K; ++t) r.d[t] = g * a.d[t];
     return r;
 }
+// lane src's value of x within the wave (__shfl), for plain and dual numbers
+__device__ __forceinline__ double shfl_any(double x, int src) { return __shfl(x, src, 64); }
+__device__ __forceinline__ float shfl_any(float x, int src) { return __shfl(x, src, 64); }
+TJ_DUAL_FN Dual<K> shfl_any(const Dual<K>& x, int src) {
+    Dual<K> r;
+    r.v = __shfl(x.v, src, 64);
+#pragma unroll
+    for (int t = 0; t < K; ++t) r.d[t] = __shfl(x.d[t], src, 64);
+    return r;
+}
 // true when any derivative part is NaN (the value part is checked separately)
 TJ_DUAL_FN bool tan_nan(const Dual<K>& a) {
     bool b = false;

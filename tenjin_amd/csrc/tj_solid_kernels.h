@@ -11,21 +11,21 @@ __device__ __forceinline__ SSt<NM, R> shfl_state(const SSt<NM, R>& S, int src) {
     SSt<NM, R> T;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        T.al[m] = __shfl(S.al[m], src, 64);
-        T.ar[m] = __shfl(S.ar[m], src, 64);
-        T.are[m] = __shfl(S.are[m], src, 64);
+        T.al[m] = shfl_any(S.al[m], src);
+        T.ar[m] = shfl_any(S.ar[m], src);
+        T.are[m] = shfl_any(S.are[m], src);
     }
-    T.rho = __shfl(S.rho, src, 64);
+    T.rho = shfl_any(S.rho, src);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        T.u[q] = __shfl(S.u[q], src, 64);
-        T.sig[q] = __shfl(S.sig[q], src, 64);
+        T.u[q] = shfl_any(S.u[q], src);
+        T.sig[q] = shfl_any(S.sig[q], src);
     }
-    T.E = __shfl(S.E, src, 64);
-    T.p = __shfl(S.p, src, 64);
-    T.a = __shfl(S.a, src, 64);
+    T.E = shfl_any(S.E, src);
+    T.p = shfl_any(S.p, src);
+    T.a = shfl_any(S.a, src);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) T.V[q] = __shfl(S.V[q], src, 64);
+    for (int q = 0; q < 9; ++q) T.V[q] = shfl_any(S.V[q], src);
     return T;
 }
 

@@ -112,6 +112,61 @@ __global__ __launch_bounds__(256) void k_tflux_s(Level lv, DMF<K> U, Faces F, Cl
     if (st) atomicOr(&clk->status, st);
 }
 
+// The x faces with one reconstruction per cell (the lane layout of
+// k_flux_x_s): lanes along x, 63 faces per wave; face c takes its left state
+// UR(c-1) from the lane below, and face 0 its own lane's UR(0).  Every value
+// and derivative part is the one k_tflux_s<NM, K, 0> computes (the same
+// operations on the same inputs), at half its reconstructions.  The right
+// state is shuffled before the left one is built, so two face states are
+// live at a time, as in k_tflux_s.
+template <int NM, int K>
+__global__ __launch_bounds__(256) void k_tflux_xs(Level lv, DMF<K> U, Faces F, Clock* clk) {
+    if (clk->done) return;
+    constexpr int PER = 63;
+    constexpr int NF = 3 * NM + 17;
+    using T = Dual<K>;
+    const int lane = threadIdx.x & 63;
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const int nseg = (nx + 1 + PER - 1) / PER;
+    if (wave >= (long long)nseg * ny * nz) return;  // uniform per wave
+    const int seg = (int)(wave % nseg);
+    const long long row = wave / nseg;
+    const int j = (int)(row % ny), k = (int)(row / ny);
+    const int c = seg * PER - 1 + lane;
+    bool fl;
+    const int s = bcmap(c, nx, lv.bc_lo[0], lv.bc_hi[0], fl);
+    SPrim<NM, T> PL;
+    SSt<NM, T> Lf;
+    {
+        SPrim<NM, T> Pm, Pc, Pp, PR;
+        load_tprim<NM, K>(lv, U, s - 1, j, k, Pm);
+        load_tprim<NM, K>(lv, U, s, j, k, Pc);
+        load_tprim<NM, K>(lv, U, s + 1, j, k, Pp);
+        muscl_s<NM, T>(Pm, Pc, Pp, PL, PR);
+        SSt<NM, T> SRs;
+        p2c_sound_s<NM, T>(lv, PR, 0, SRs);
+        Lf = shfl_state<NM, T>(SRs, (c == 0 || lane == 0) ? lane : lane - 1);
+    }
+    if (c == 0 && lv.bc_lo[0]) flip_s<NM, T>(Lf, 0);
+    SSt<NM, T> Rf;
+    p2c_sound_s<NM, T>(lv, PL, 0, Rf);
+    if (c == nx && lv.bc_hi[0]) flip_s<NM, T>(Rf, 0);
+    unsigned st = 0;
+    if (lane >= 1 && c >= 0 && c <= nx) {
+        T Fv[NF];
+        wave5<NM, T>(lv, Lf, Rf, 0, Fv, st);
+        const long long id = ((long long)k * F.ny + j) * F.nx + c;
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            F.p[q * F.cs + id] = Fv[q].v;
+#pragma unroll
+            for (int t = 0; t < K; ++t) F.p[(NF * (t + 1) + q) * F.cs + id] = Fv[q].d[t];
+        }
+    }
+    if (st) atomicOr(&clk->status, st);
+}
+
 template <int NM, int K>
 struct TCell {
     Dual<K> al[NM], ar[NM], rk[NM], e[NM], are[NM], rho, ru[3], u[3], E, p, V[9], hj2;
