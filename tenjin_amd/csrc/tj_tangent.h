@@ -60,103 +60,69 @@ __device__ __forceinline__ void load_tprim(const Level& lv, const DMF<K>& U, int
     for (int q = 0; q < 9; ++q) P.V[q] = U.get(lv.cs[V_TENSOR] + q, i, j, k);
 }
 
-// One face per thread along D: left state UR(f-1), right state UL(f); at the
-// domain faces the image of the boundary cell's own reconstruction (flipped
-// normal velocity when reflective), as the FP64 passes do.
+// Faces along D with one reconstruction per cell.  Each wave covers a tile
+// of cells: 64 along x (D = 0), or 8 along x by 8 along D (D = 1, 2; lanes
+// x fastest, so a load reads 64-byte row pieces).  A lane reconstructs one
+// cell (MUSCL, then p2c + sound speed of both face states), and face f takes
+// its left state UR(f-1) from the lane one cell below along D by shuffle, so
+// a tile column of L cells gives L - 1 faces (L = 64 or 8).  At the domain
+// faces the left state of face 0 is the image of UR(0) and the right state
+// of face n the image of UL(n-1) (normal velocity flipped when reflective),
+// as in the FP64 passes.  The right state is shuffled before the left one is
+// built, so two face states are live at a time.
 template <int NM, int K, int D>
-__global__ __launch_bounds__(256) void k_tflux_s(Level lv, DMF<K> U, Faces F, Clock* clk) {
+__global__ __launch_bounds__(256) void k_tflux_ts(Level lv, DMF<K> U, Faces F, Clock* clk) {
     if (clk->done) return;
-    constexpr int NF = 3 * NM + 17;
-    using T = Dual<K>;
-    const int fx = F.nx, fy = F.ny, fz = F.nz;
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned st = 0;
-    if (id < (long long)fx * fy * fz) {
-        const int i = (int)(id % fx);
-        const int j = (int)((id / fx) % fy);
-        const int k = (int)(id / ((long long)fx * fy));
-        const int f = D == 0 ? i : (D == 1 ? j : k);
-        const int nd = lv.n[D];
-        const int cl = f == 0 ? 0 : f - 1, cr = f == nd ? nd - 1 : f;
-        auto at = [&](int c, SPrim<NM, T>& P) {
-            load_tprim<NM, K>(lv, U, D == 0 ? c : i, D == 1 ? c : j, D == 2 ? c : k, P);
-        };
-        SSt<NM, T> A, B;
-        {
-            SPrim<NM, T> Pm, Pc, Pp, PL, PR;
-            at(cl - 1, Pm);
-            at(cl, Pc);
-            at(cl + 1, Pp);
-            muscl_s<NM, T>(Pm, Pc, Pp, PL, PR);
-            p2c_sound_s<NM, T>(lv, PR, D, A);
-        }
-        {
-            SPrim<NM, T> Pm, Pc, Pp, PL, PR;
-            at(cr - 1, Pm);
-            at(cr, Pc);
-            at(cr + 1, Pp);
-            muscl_s<NM, T>(Pm, Pc, Pp, PL, PR);
-            p2c_sound_s<NM, T>(lv, PL, D, B);
-        }
-        if (f == 0 && lv.bc_lo[D]) flip_s<NM, T>(A, D);
-        if (f == nd && lv.bc_hi[D]) flip_s<NM, T>(B, D);
-        T Fv[NF];
-        wave5<NM, T>(lv, A, B, D, Fv, st);
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-            F.p[q * F.cs + id] = Fv[q].v;
-#pragma unroll
-            for (int t = 0; t < K; ++t) F.p[(NF * (t + 1) + q) * F.cs + id] = Fv[q].d[t];
-        }
-    }
-    if (st) atomicOr(&clk->status, st);
-}
-
-// The x faces with one reconstruction per cell (the lane layout of
-// k_flux_x_s): lanes along x, 63 faces per wave; face c takes its left state
-// UR(c-1) from the lane below, and face 0 its own lane's UR(0).  Every value
-// and derivative part is the one k_tflux_s<NM, K, 0> computes (the same
-// operations on the same inputs), at half its reconstructions.  The right
-// state is shuffled before the left one is built, so two face states are
-// live at a time, as in k_tflux_s.
-template <int NM, int K>
-__global__ __launch_bounds__(256) void k_tflux_xs(Level lv, DMF<K> U, Faces F, Clock* clk) {
-    if (clk->done) return;
-    constexpr int PER = 63;
+    constexpr int LN = D == 0 ? 64 : 8;  // lanes along D
+    constexpr int LT = 64 / LN;           // lanes along x (D = 1, 2)
+    constexpr int PER = LN - 1;           // faces per tile column
     constexpr int NF = 3 * NM + 17;
     using T = Dual<K>;
     const int lane = threadIdx.x & 63;
+    const int ld = lane / LT, lt = lane % LT;
     const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
-    const int nseg = (nx + 1 + PER - 1) / PER;
-    if (wave >= (long long)nseg * ny * nz) return;  // uniform per wave
-    const int seg = (int)(wave % nseg);
-    const long long row = wave / nseg;
-    const int j = (int)(row % ny), k = (int)(row / ny);
-    const int c = seg * PER - 1 + lane;
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2], nd = lv.n[D];
+    const int nseg = (nd + 1 + PER - 1) / PER;
+    const int nxt = D == 0 ? 1 : (nx + LT - 1) / LT;
+    const int no = D == 0 ? ny * nz : (D == 1 ? nz : ny);
+    if (wave >= (long long)nxt * nseg * no) return;  // uniform per wave
+    const int xt = (int)(wave % nxt);
+    const long long rest = wave / nxt;
+    const int seg = (int)(rest % nseg);
+    const int o = (int)(rest / nseg);
+    const int c = seg * PER - 1 + ld;  // the lane's cell along D
     bool fl;
-    const int s = bcmap(c, nx, lv.bc_lo[0], lv.bc_hi[0], fl);
+    const int s = bcmap(c, nd, lv.bc_lo[D], lv.bc_hi[D], fl);
+    const int i0 = xt * LT + lt;
+    const bool act = D == 0 || i0 < nx;
+    const int i = D == 0 ? s : (act ? i0 : nx - 1);
+    const int j = D == 0 ? o % ny : (D == 1 ? s : o);
+    const int k = D == 0 ? o / ny : (D == 1 ? o : s);
+    auto at = [&](int r, SPrim<NM, T>& P) {
+        load_tprim<NM, K>(lv, U, D == 0 ? r : i, D == 1 ? r : j, D == 2 ? r : k, P);
+    };
     SPrim<NM, T> PL;
     SSt<NM, T> Lf;
     {
         SPrim<NM, T> Pm, Pc, Pp, PR;
-        load_tprim<NM, K>(lv, U, s - 1, j, k, Pm);
-        load_tprim<NM, K>(lv, U, s, j, k, Pc);
-        load_tprim<NM, K>(lv, U, s + 1, j, k, Pp);
+        at(s - 1, Pm);
+        at(s, Pc);
+        at(s + 1, Pp);
         muscl_s<NM, T>(Pm, Pc, Pp, PL, PR);
         SSt<NM, T> SRs;
-        p2c_sound_s<NM, T>(lv, PR, 0, SRs);
-        Lf = shfl_state<NM, T>(SRs, (c == 0 || lane == 0) ? lane : lane - 1);
+        p2c_sound_s<NM, T>(lv, PR, D, SRs);
+        Lf = shfl_state<NM, T>(SRs, (c == 0 || ld == 0) ? lane : lane - LT);
     }
-    if (c == 0 && lv.bc_lo[0]) flip_s<NM, T>(Lf, 0);
+    if (c == 0 && lv.bc_lo[D]) flip_s<NM, T>(Lf, D);
     SSt<NM, T> Rf;
-    p2c_sound_s<NM, T>(lv, PL, 0, Rf);
-    if (c == nx && lv.bc_hi[0]) flip_s<NM, T>(Rf, 0);
+    p2c_sound_s<NM, T>(lv, PL, D, Rf);
+    if (c == nd && lv.bc_hi[D]) flip_s<NM, T>(Rf, D);
     unsigned st = 0;
-    if (lane >= 1 && c >= 0 && c <= nx) {
+    if (act && ld >= 1 && c >= 0 && c <= nd) {
         T Fv[NF];
-        wave5<NM, T>(lv, Lf, Rf, 0, Fv, st);
-        const long long id = ((long long)k * F.ny + j) * F.nx + c;
+        wave5<NM, T>(lv, Lf, Rf, D, Fv, st);
+        const int fi = D == 0 ? c : i, fj = D == 1 ? c : j, fk = D == 2 ? c : k;
+        const long long id = ((long long)fk * F.ny + fj) * F.nx + fi;
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
             F.p[q * F.cs + id] = Fv[q].v;
