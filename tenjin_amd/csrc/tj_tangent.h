@@ -518,119 +518,161 @@ __device__ __forceinline__ void load_tprim_f(const Level& lv, const DMF<K>& U, i
     P.p = U.get(lv.cs[PRESSURE], i, j, k);
 }
 
-// One face per thread.  With THINC (TH) the alpha faces of each of the two
-// cells go through the BVD selection of flux_x_body / flux_march_body: the
-// cell's and its neighbours' THINC reconstructions, the box-edge semantics and
-// the stale history planes (read with their derivative parts); the face then
-// writes the history of this pass: urstar of its upper cell (where mixed), the
-// star alpha of its upper cell and, at box-high faces, the edge plane.
+// lane src's fluid face state within the wave (no mixture part)
+template <int NM, int K>
+__device__ __forceinline__ FSt<NM, Dual<K>> shfl_tfst(const FSt<NM, Dual<K>>& S, int src) {
+    FSt<NM, Dual<K>> T;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        T.al[m] = shfl_any(S.al[m], src);
+        T.ar[m] = shfl_any(S.ar[m], src);
+    }
+    T.rho = shfl_any(S.rho, src);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) T.u[q] = shfl_any(S.u[q], src);
+    T.E = shfl_any(S.E, src);
+    T.p = shfl_any(S.p, src);
+    T.a = shfl_any(S.a, src);
+    return T;
+}
+
+// Faces along D with one reconstruction per cell, in the tiles of k_tflux_ts
+// (64 x 1 cells in x, 8 x 8 in y and z; face f takes UR(f-1) from the lane
+// one cell below by shuffle).  A lane reconstructs its cell: MUSCL, and with
+// THINC (TH) the BVD selection of the alpha faces of flux_x_body /
+// flux_march_body (the cell's and its neighbours' THINC reconstructions, the
+// box-edge semantics and the stale history planes, read with their
+// derivative parts) and cleanUpAlpha; then p2c + sound speed of both faces.
+// The lane of face f then writes the history of this pass: urstar of cell f
+// (where mixed), the star alpha of cell f and, at box-high faces, the edge
+// plane.  Within a pass urstar is read only where a cell is not mixed and
+// written only where it is, so the writes race with no read.
 template <int NM, int K, int D, bool TH>
-__global__ __launch_bounds__(256) void k_tflux_f(Level lv, DMF<K> U, Faces F, THist h, Clock* clk) {
+__global__ __launch_bounds__(256) void k_tflux_tf(Level lv, DMF<K> U, Faces F, THist h, Clock* clk) {
     if (clk->done) return;
+    constexpr int LN = D == 0 ? 64 : 8;  // lanes along D
+    constexpr int LT = 64 / LN;           // lanes along x (D = 1, 2)
+    constexpr int PER = LN - 1;           // faces per tile column
     constexpr int NF = 2 * NM + 5;
     using T = Dual<K>;
-    const int fx = F.nx, fy = F.ny, fz = F.nz;
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long long)fx * fy * fz) return;
-    const int i = (int)(id % fx);
-    const int j = (int)((id / fx) % fy);
-    const int k = (int)(id / ((long long)fx * fy));
-    const int f = D == 0 ? i : (D == 1 ? j : k);
-    const int nd = lv.n[D];
-    const int nx = lv.n[0], ny = lv.n[1];
-    const int cl = f == 0 ? 0 : f - 1, cr = f == nd ? nd - 1 : f;
-    auto ci = [&](int c) { return D == 0 ? c : i; };
-    auto cj = [&](int c) { return D == 1 ? c : j; };
-    auto ck = [&](int c) { return D == 2 ? c : k; };
-    auto cellidx = [&](int c) { return ((long long)ck(c) * ny + cj(c)) * nx + ci(c); };
+    const int lane = threadIdx.x & 63;
+    const int ld = lane / LT, lt = lane % LT;
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2], nd = lv.n[D];
+    const int nseg = (nd + 1 + PER - 1) / PER;
+    const int nxt = D == 0 ? 1 : (nx + LT - 1) / LT;
+    const int no = D == 0 ? ny * nz : (D == 1 ? nz : ny);
+    if (wave >= (long long)nxt * nseg * no) return;  // uniform per wave
+    const int xt = (int)(wave % nxt);
+    const long long rest = wave / nxt;
+    const int seg = (int)(rest % nseg);
+    const int o = (int)(rest / nseg);
+    const int c = seg * PER - 1 + ld;  // the lane's cell along D
+    bool fl;
+    const int s = bcmap(c, nd, lv.bc_lo[D], lv.bc_hi[D], fl);
+    const int i0 = xt * LT + lt;
+    const bool act = D == 0 || i0 < nx;
+    const int i = D == 0 ? s : (act ? i0 : nx - 1);
+    const int j = D == 0 ? o % ny : (D == 1 ? s : o);
+    const int k = D == 0 ? o / ny : (D == 1 ? o : s);
+    auto ci = [&](int r) { return D == 0 ? r : i; };
+    auto cj = [&](int r) { return D == 1 ? r : j; };
+    auto ck = [&](int r) { return D == 2 ? r : k; };
+    auto cellidx = [&](int r) { return ((long long)ck(r) * ny + cj(r)) * nx + ci(r); };
     const long long cstride = D == 0 ? 1 : (D == 1 ? (long long)nx : (long long)nx * ny);
     // face index of face g along D in the face grid of this direction
     auto faceidx = [&](int g) {
         return D == 0 ? ((long long)k * ny + j) * (nx + 1) + g
                       : (D == 1 ? ((long long)k * (ny + 1) + g) * nx + i : ((long long)g * ny + j) * nx + i);
     };
+    const bool face = act && ld >= 1 && c >= 0 && c <= nd;  // this lane computes face c
     unsigned st = 0;
-    // reconstruct cell c (MUSCL, then THINC/BVD and cleanUpAlpha of the alpha
-    // faces); want_right: the UR face state, else UL
-    auto recon = [&](int c, bool want_right, FSt<NM, T>& S) {
-        Prim<NM, T> Pm, Pc, Pp, PL, PR;
-        load_tprim_f<NM, K>(lv, U, ci(c - 1), cj(c - 1), ck(c - 1), Pm);
-        load_tprim_f<NM, K>(lv, U, ci(c), cj(c), ck(c), Pc);
-        load_tprim_f<NM, K>(lv, U, ci(c + 1), cj(c + 1), ck(c + 1), Pp);
+    Prim<NM, T> PL, PR;
+    {
+        Prim<NM, T> Pm, Pc, Pp;
+        load_tprim_f<NM, K>(lv, U, ci(s - 1), cj(s - 1), ck(s - 1), Pm);
+        load_tprim_f<NM, K>(lv, U, ci(s), cj(s), ck(s), Pc);
+        load_tprim_f<NM, K>(lv, U, ci(s + 1), cj(s + 1), ck(s + 1), Pp);
         muscl<NM, T>(Pm, Pc, Pp, PL, PR);
-        if (TH) {
-            const uint8_t ef = lv.edge[D][c];
-            const bool blo = ef & 1, bhi = ef & 2;
-            const T ynx = T(lv.yn[0]), yny = T(lv.yn[1]);
-            ARec<NM, T> Acur;
-            arec<NM, T>(lv, U, ci(c), cj(c), ck(c), D, ynx, yny, Acur);
-            const int mixedFlag = (Acur.mixed >> (NM - 1)) & 1;
-            const long long cc = cellidx(c);
-            if (mixedFlag) {
-                ARec<NM, T> Aprev, Anext;
-                if (!blo) arec<NM, T>(lv, U, ci(c - 1), cj(c - 1), ck(c - 1), D, ynx, yny, Aprev);
-                if (!bhi) arec<NM, T>(lv, U, ci(c + 1), cj(c + 1), ck(c + 1), D, ynx, yny, Anext);
-                T URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
-#pragma unroll
-                for (int m = 0; m < NM; ++m) {
-                    if (blo) {
-                        URl[m] = load_alpha<T>(lv, U, m, ci(c - 1), cj(c - 1), ck(c - 1));
-                        TRl[m] = T(0.0);
-                    } else {
-                        URl[m] = Aprev.ura[m];
-                        TRl[m] = ((Aprev.mixed >> m) & 1) ? Aprev.thr[m]
-                                                          : hget<K>(h.urstar, h.pblk, m * h.pstride + cc - cstride);
-                    }
-                    if (bhi) {
-                        ULr[m] = load_alpha<T>(lv, U, m, ci(c + 1), cj(c + 1), ck(c + 1));
-                        TLr[m] = hget<K>(h.edge_in, h.eblk, m * h.estride + faceidx(c + 1));
-                    } else {
-                        ULr[m] = Anext.ula[m];
-                        TLr[m] = ((Anext.mixed >> m) & 1) ? Anext.thl[m]
-                                                          : hget<K>(h.star_in, h.pblk, m * h.pstride + cc + cstride);
-                    }
-                    if ((Acur.mixed >> m) & 1) {
-                        thl[m] = Acur.thl[m];
-                        thr[m] = Acur.thr[m];
-                    } else {
-                        thl[m] = hget<K>(h.star_in, h.pblk, m * h.pstride + cc);
-                        thr[m] = hget<K>(h.urstar, h.pblk, m * h.pstride + cc);
-                    }
-                }
-                bvd<NM, T>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
-            }
-            st |= cleanup_alpha<NM, T>(PL.a);
-            st |= cleanup_alpha<NM, T>(PR.a);
-            // urstar of this face's upper cell (its owner)
-            if (!want_right && c == f && f < nd) {
-#pragma unroll
-                for (int m = 0; m < NM; ++m)
-                    if ((Acur.mixed >> m) & 1) hput<K>(h.urstar, h.pblk, m * h.pstride + cc, Acur.thr[m]);
-            }
-        }
-        p2c_sound<NM, T>(lv, want_right ? PR : PL, S);
-    };
-    FSt<NM, T> A, B;
-    recon(cl, true, A);
-    recon(cr, false, B);
-    if (f == 0 && lv.bc_lo[D]) flip_state<NM, T>(A, D);
-    if (f == nd && lv.bc_hi[D]) flip_state<NM, T>(B, D);
-    T Fv[NF], sa[NM];
-    hllc<NM, T>(A, B, D, Fv, sa);
-#pragma unroll
-    for (int q = 0; q < NF; ++q) {
-        F.p[q * F.cs + id] = Fv[q].v;
-#pragma unroll
-        for (int t = 0; t < K; ++t) F.p[(NF * (t + 1) + q) * F.cs + id] = Fv[q].d[t];
     }
     if (TH) {
-        if (f < nd) {
+        const uint8_t ef = lv.edge[D][s];
+        const bool blo = ef & 1, bhi = ef & 2;
+        const T ynx = T(lv.yn[0]), yny = T(lv.yn[1]);
+        ARec<NM, T> Acur;
+        arec<NM, T>(lv, U, ci(s), cj(s), ck(s), D, ynx, yny, Acur);
+        const int mixedFlag = (Acur.mixed >> (NM - 1)) & 1;
+        const long long cc = cellidx(s);
+        if (mixedFlag) {
+            ARec<NM, T> Aprev, Anext;
+            if (!blo) arec<NM, T>(lv, U, ci(s - 1), cj(s - 1), ck(s - 1), D, ynx, yny, Aprev);
+            if (!bhi) arec<NM, T>(lv, U, ci(s + 1), cj(s + 1), ck(s + 1), D, ynx, yny, Anext);
+            T URl[NM], TRl[NM], ULr[NM], TLr[NM], thl[NM], thr[NM];
 #pragma unroll
-            for (int m = 0; m < NM; ++m) hput<K>(h.star_out, h.pblk, m * h.pstride + cellidx(f), sa[m]);
+            for (int m = 0; m < NM; ++m) {
+                if (blo) {
+                    URl[m] = load_alpha<T>(lv, U, m, ci(s - 1), cj(s - 1), ck(s - 1));
+                    TRl[m] = T(0.0);
+                } else {
+                    URl[m] = Aprev.ura[m];
+                    TRl[m] = ((Aprev.mixed >> m) & 1) ? Aprev.thr[m]
+                                                      : hget<K>(h.urstar, h.pblk, m * h.pstride + cc - cstride);
+                }
+                if (bhi) {
+                    ULr[m] = load_alpha<T>(lv, U, m, ci(s + 1), cj(s + 1), ck(s + 1));
+                    TLr[m] = hget<K>(h.edge_in, h.eblk, m * h.estride + faceidx(s + 1));
+                } else {
+                    ULr[m] = Anext.ula[m];
+                    TLr[m] = ((Anext.mixed >> m) & 1) ? Anext.thl[m]
+                                                      : hget<K>(h.star_in, h.pblk, m * h.pstride + cc + cstride);
+                }
+                if ((Acur.mixed >> m) & 1) {
+                    thl[m] = Acur.thl[m];
+                    thr[m] = Acur.thr[m];
+                } else {
+                    thl[m] = hget<K>(h.star_in, h.pblk, m * h.pstride + cc);
+                    thr[m] = hget<K>(h.urstar, h.pblk, m * h.pstride + cc);
+                }
+            }
+            bvd<NM, T>(1, URl, TRl, ULr, TLr, thl, thr, PL.a, PR.a);
         }
-        if (f == nd || (f > 0 && (lv.edge[D][f - 1] & 2))) {
+        st |= cleanup_alpha<NM, T>(PL.a);
+        st |= cleanup_alpha<NM, T>(PR.a);
+        if (face && c < nd) {  // urstar of cell c, the upper cell of face c (c == s here)
 #pragma unroll
-            for (int m = 0; m < NM; ++m) hput<K>(h.edge_out, h.eblk, m * h.estride + faceidx(f), sa[m]);
+            for (int m = 0; m < NM; ++m)
+                if ((Acur.mixed >> m) & 1) hput<K>(h.urstar, h.pblk, m * h.pstride + cc, Acur.thr[m]);
+        }
+    }
+    FSt<NM, T> Lf;
+    {
+        FSt<NM, T> SRs;
+        p2c_sound<NM, T>(lv, PR, SRs);
+        Lf = shfl_tfst<NM, K>(SRs, (c == 0 || ld == 0) ? lane : lane - LT);
+    }
+    if (c == 0 && lv.bc_lo[D]) flip_state<NM, T>(Lf, D);
+    FSt<NM, T> Rf;
+    p2c_sound<NM, T>(lv, PL, Rf);
+    if (c == nd && lv.bc_hi[D]) flip_state<NM, T>(Rf, D);
+    if (face) {
+        T Fv[NF], sa[NM];
+        hllc<NM, T>(Lf, Rf, D, Fv, sa);
+        const long long fi = faceidx(c);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            F.p[q * F.cs + fi] = Fv[q].v;
+#pragma unroll
+            for (int t = 0; t < K; ++t) F.p[(NF * (t + 1) + q) * F.cs + fi] = Fv[q].d[t];
+        }
+        if (TH) {
+            if (c < nd) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m) hput<K>(h.star_out, h.pblk, m * h.pstride + cellidx(c), sa[m]);
+            }
+            if (c == nd || (c > 0 && (lv.edge[D][c - 1] & 2))) {
+#pragma unroll
+                for (int m = 0; m < NM; ++m) hput<K>(h.edge_out, h.eblk, m * h.estride + fi, sa[m]);
+            }
         }
     }
     if (st) atomicOr(&clk->status, st);

@@ -157,7 +157,6 @@ int run_group(tj_tangent* tg, int first, int cnt, const tj_mf* P0, int shift, tj
     auto fluxes = [&](const DMF<K>& U) {
         for (int d = 0; d < lv.dim; ++d) {
             const Faces& F = tg->F[d];
-            const unsigned g = grid1((long long)F.nx * F.ny * F.nz, 256);
             THist h{};
             if (thinc) {  // hist_for of tj_lib.hip
                 const int q = (int)(pass & 1), e = (int)(dpass[d] & 1);
@@ -173,22 +172,23 @@ int run_group(tj_tangent* tg, int first, int cnt, const tj_mf* P0, int shift, tj
                 pass += 1;
                 dpass[d] += 1;
             }
-            if (solid) {  // tiles of 64 x 1 (x pass) or 8 x 8 cells giving 63 or 8 x 7 faces, 4 waves per block
-                const int per = d == 0 ? 63 : 7;
-                const long long nseg = (lv.n[d] + 1 + per - 1) / per;
-                const long long nw = d == 0 ? nseg * lv.n[1] * lv.n[2]
-                                            : (long long)((lv.n[0] + 7) / 8) * nseg * (d == 1 ? lv.n[2] : lv.n[1]);
-                const unsigned gt = grid1(nw, 4);
+            // tiles of 64 x 1 (x pass) or 8 x 8 cells giving 63 or 8 x 7 faces, 4 waves per block
+            const int per = d == 0 ? 63 : 7;
+            const long long nseg = (lv.n[d] + 1 + per - 1) / per;
+            const long long nw = d == 0 ? nseg * lv.n[1] * lv.n[2]
+                                        : (long long)((lv.n[0] + 7) / 8) * nseg * (d == 1 ? lv.n[2] : lv.n[1]);
+            const unsigned gt = grid1(nw, 4);
+            if (solid) {
                 if (d == 0) hipLaunchKernelGGL((k_tflux_ts<2, K, 0>), dim3(gt), dim3(256), 0, st, lv, U, F, c->d_clk);
                 else if (d == 1) hipLaunchKernelGGL((k_tflux_ts<2, K, 1>), dim3(gt), dim3(256), 0, st, lv, U, F, c->d_clk);
                 else hipLaunchKernelGGL((k_tflux_ts<2, K, 2>), dim3(gt), dim3(256), 0, st, lv, U, F, c->d_clk);
             } else if (thinc) {  // THINC levels are 1D or 2D
-                if (d == 0) hipLaunchKernelGGL((k_tflux_f<2, K, 0, true>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
-                else hipLaunchKernelGGL((k_tflux_f<2, K, 1, true>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
+                if (d == 0) hipLaunchKernelGGL((k_tflux_tf<2, K, 0, true>), dim3(gt), dim3(256), 0, st, lv, U, F, h, c->d_clk);
+                else hipLaunchKernelGGL((k_tflux_tf<2, K, 1, true>), dim3(gt), dim3(256), 0, st, lv, U, F, h, c->d_clk);
             } else {
-                if (d == 0) hipLaunchKernelGGL((k_tflux_f<2, K, 0, false>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
-                else if (d == 1) hipLaunchKernelGGL((k_tflux_f<2, K, 1, false>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
-                else hipLaunchKernelGGL((k_tflux_f<2, K, 2, false>), dim3(g), dim3(256), 0, st, lv, U, F, h, c->d_clk);
+                if (d == 0) hipLaunchKernelGGL((k_tflux_tf<2, K, 0, false>), dim3(gt), dim3(256), 0, st, lv, U, F, h, c->d_clk);
+                else if (d == 1) hipLaunchKernelGGL((k_tflux_tf<2, K, 1, false>), dim3(gt), dim3(256), 0, st, lv, U, F, h, c->d_clk);
+                else hipLaunchKernelGGL((k_tflux_tf<2, K, 2, false>), dim3(gt), dim3(256), 0, st, lv, U, F, h, c->d_clk);
             }
         }
     };
