@@ -106,7 +106,9 @@ def parse():
     ap.add_argument("--nrz", type=int, default=1448, help="explosion_rz: cells per side")
     ap.add_argument("--n2g", type=int, default=512, help="plate2d_grad: cells per side")
     ap.add_argument("--box", type=int, default=64)
-    ap.add_argument("--seglen", type=int, default=0, help="rows per marching segment (0: chosen per pass by the library)")
+    ap.add_argument("--seglen", type=int, default=0,
+                    help="rows per marching segment: > 0 fixed; -1 the library's own pick per pass; 0 (default) the "
+                         "fastest of the library's pick, 8 and 12, timed on a scratch copy of the level first")
     ap.add_argument("--fuse", action="store_true",
                     help="run the update inside the y pass instead of as a kernel of its own")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -584,7 +586,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from tenjin_amd import GpuLevel, TJ_PREC_FP64, TJ_PREC_MIXED, configs
+    from tenjin_amd import GpuLevel, TJ_PREC_FP64, TJ_PREC_MIXED, autotune, configs
 
     if args.calib_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -627,10 +629,19 @@ def main():
                     "Us-up Mie-Gruneisen, MUSCL+5-wave HLLC, perfect plasticity, CFL 0.5")
     ncell = pb.ncells
 
+    tuned = {}
+
     def timed(prec):
-        """warm-up steps, then exactly args.steps timed steps of the device loop"""
+        """warm-up steps, then exactly args.steps timed steps of the device loop
+        (segment length: --seglen, else the fastest of a few timed on a scratch
+        copy of the level beforehand)"""
+        fuse = 1 if args.fuse else 0
+        seglen = max(args.seglen, 0)
+        if args.seglen == 0:
+            seglen, trial = autotune.tune_level_seglen(pb, prec, fuse)
+            tuned[prec] = {"seglen": seglen, "trial_s": {str(k): round(v, 5) for k, v in trial.items()}}
         g = GpuLevel(pb)
-        g.set_tuning(args.seglen, 1 if args.fuse else 0)
+        g.set_tuning(seglen, fuse)
         g.init_from_prims(pb.prims, pb.shift_pressure)
         g.begin_run(float("inf"))
         g.advance(args.warmup, prec)
@@ -752,7 +763,8 @@ def main():
             "cells": ncell,
             "precision": "fp64" if prec == TJ_PREC_FP64 else "mixed",
             "fused_update": bool(args.fuse),
-            "seglen": args.seglen,
+            "seglen": args.seglen if args.seglen else {("fp64" if k == TJ_PREC_FP64 else "mixed"): v
+                                                        for k, v in tuned.items()},
             "parallelism": "single GPU",
         },
         "roofline": roofline,

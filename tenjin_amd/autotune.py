@@ -1,12 +1,20 @@
-"""Autotuner of the batched ensemble: picks the marching segment length of
-the y pass for each size bucket of 2D problems by timing a few candidates on
-that bucket's problems alone (a separate batch, discarded afterwards)."""
+"""Autotuners of the marching segment length (rows per wave of the y / z
+passes).  Every segment length gives bit-identical results; it only moves
+time, and the best one depends on where the work is (THINC's interface rows
+make some waves much longer than others), which no static model sees.
+
+- tune_seglen: the batched ensemble, per size bucket of 2D problems, timing a
+  few candidates on that bucket's problems alone (a separate batch, discarded
+  afterwards).
+- tune_level_seglen: one level, timing a few candidates (0 = the library's own
+  pick) from the initial state on a scratch level, discarded afterwards."""
 from __future__ import annotations
 
 import time
 
 BUCKETS = ((0, 128), (128, 256), (256, 1 << 30))   # 2D side ranges [lo, hi)
 CANDIDATES = (4, 8, 16, 32, 64)
+LEVEL_CANDIDATES = (0, 8, 12)
 
 
 def bucket_of(pb):
@@ -54,3 +62,34 @@ def apply(batch, problems, table):
     for p, pb in enumerate(problems):
         if pb.dim == 2 and bucket_of(pb) in table:
             batch.set_seglen(p, table[bucket_of(pb)])
+
+
+def tune_level_seglen(pb, prec: int = 0, fuse: int = 0, steps: int = 6, warmup: int = 2,
+                      candidates=LEVEL_CANDIDATES, margin: float = 0.03, device: int = 0):
+    """(seglen, {candidate: seconds for `steps` loop bodies}) for the level of
+    problem pb: each candidate runs `warmup` untimed and `steps` timed loop
+    bodies from the initial state on one scratch level (re-set to the initial
+    state, THINC history included, for each).  The library's own pick (0) is
+    kept unless another candidate is faster by more than `margin`, about the
+    run-to-run spread of so short a trial."""
+    from . import GpuLevel
+
+    g = GpuLevel(pb, device)
+    timing = {}
+    try:
+        for cand in candidates:
+            g.set_tuning(cand, fuse)
+            g.init_from_prims(pb.prims, pb.shift_pressure)
+            g.begin_run(float("inf"))
+            g.advance(warmup, prec)
+            g.sync()
+            t0 = time.perf_counter()
+            g.advance(steps, prec)
+            g.sync()
+            timing[cand] = time.perf_counter() - t0
+    finally:
+        g.close()
+    best = min(timing, key=timing.get)
+    if 0 in timing and timing[best] * (1.0 + margin) >= timing[0]:
+        best = 0
+    return best, timing

@@ -69,3 +69,31 @@ def test_batch_seglen_and_graph_replays():
         for p in range(len(probs)):
             assert np.array_equal(e.state(p), g.state(p), equal_nan=True), f"replay {rep} problem {p}"
         assert e.clocks() == g.clocks()
+
+
+def test_level_seglen_tuner():
+    """The level tuner times its candidates on a scratch level and returns one
+    of them; a level run with the chosen segment length equals a run with the
+    library's own pick, bit for bit."""
+    from tenjin_amd import autotune, configs
+
+    pb = configs.shock_inclusion_2d(nx=96, ny=48, box=32)
+    print("\n[case] level seglen tuner 96x48", flush=True)
+    sl, timing = autotune.tune_level_seglen(pb, 0, steps=2, warmup=1)
+    assert set(timing) == set(autotune.LEVEL_CANDIDATES) and sl in timing
+    assert all(t > 0.0 for t in timing.values())
+
+    def run(seglen):
+        from tenjin_amd import GpuLevel
+
+        g = GpuLevel(pb)
+        g.set_tuning(seglen, 0)
+        g.init_from_prims(pb.prims, pb.shift_pressure)
+        g.begin_run()
+        g.advance(6)
+        out = g.get_state(), g.clock()
+        g.close()
+        return out
+
+    (s0, c0), (s1, c1) = run(0), run(max(sl, 8))
+    assert c0 == c1 and np.array_equal(s0, s1, equal_nan=True)
