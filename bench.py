@@ -108,7 +108,8 @@ def parse():
     ap.add_argument("--box", type=int, default=64)
     ap.add_argument("--seglen", type=int, default=0,
                     help="rows per marching segment: > 0 fixed; -1 the library's own pick per pass; 0 (default) the "
-                         "fastest of the library's pick, 8 and 12, timed on a scratch copy of the level first")
+                         "fastest of the library's pick, 8 and 12, timed on a scratch copy of the level first, "
+                         "around the middle of the run")
     ap.add_argument("--fuse", action="store_true",
                     help="run the update inside the y pass instead of as a kernel of its own")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -638,7 +639,9 @@ def main():
         fuse = 1 if args.fuse else 0
         seglen = max(args.seglen, 0)
         if args.seglen == 0:
-            seglen, trial = autotune.tune_level_seglen(pb, prec, fuse)
+            # timed around the middle of the run (at most 250 loop bodies in)
+            seglen, trial = autotune.tune_level_seglen(pb, prec, fuse,
+                                                       at_step=min(args.warmup + args.steps // 2, 250))
             tuned[prec] = {"seglen": seglen, "trial_s": {str(k): round(v, 5) for k, v in trial.items()}}
         g = GpuLevel(pb)
         g.set_tuning(seglen, fuse)

@@ -7,7 +7,8 @@ make some waves much longer than others), which no static model sees.
   few candidates on that bucket's problems alone (a separate batch, discarded
   afterwards).
 - tune_level_seglen: one level, timing a few candidates (0 = the library's own
-  pick) from the initial state on a scratch level, discarded afterwards."""
+  pick) in turns on a scratch level advanced to a given step, discarded
+  afterwards."""
 from __future__ import annotations
 
 import time
@@ -64,29 +65,34 @@ def apply(batch, problems, table):
             batch.set_seglen(p, table[bucket_of(pb)])
 
 
-def tune_level_seglen(pb, prec: int = 0, fuse: int = 0, steps: int = 6, warmup: int = 2,
+def tune_level_seglen(pb, prec: int = 0, fuse: int = 0, at_step: int = 0, steps: int = 3, rounds: int = 3,
                       candidates=LEVEL_CANDIDATES, margin: float = 0.03, device: int = 0):
-    """(seglen, {candidate: seconds for `steps` loop bodies}) for the level of
-    problem pb: each candidate runs `warmup` untimed and `steps` timed loop
-    bodies from the initial state on one scratch level (re-set to the initial
-    state, THINC history included, for each).  The library's own pick (0) is
-    kept unless another candidate is faster by more than `margin`, about the
-    run-to-run spread of so short a trial."""
+    """(seglen, {candidate: seconds for its rounds x steps loop bodies}) for
+    the level of problem pb, timed on a scratch level (discarded afterwards).
+    The scratch level first runs max(at_step, 1) loop bodies with the
+    library's own pick, so that the trial sees the state the run will spend
+    its time in (THINC's interface work moves as the flow develops); the
+    candidates then take turns, `steps` loop bodies each, for `rounds`
+    rounds, so that all of them see the same drift of the state.  The
+    library's own pick (0) is kept unless another candidate is faster by more
+    than `margin`, about the spread of so short a trial."""
     from . import GpuLevel
 
     g = GpuLevel(pb, device)
-    timing = {}
+    timing = {c: 0.0 for c in candidates}
     try:
-        for cand in candidates:
-            g.set_tuning(cand, fuse)
-            g.init_from_prims(pb.prims, pb.shift_pressure)
-            g.begin_run(float("inf"))
-            g.advance(warmup, prec)
-            g.sync()
-            t0 = time.perf_counter()
-            g.advance(steps, prec)
-            g.sync()
-            timing[cand] = time.perf_counter() - t0
+        g.set_tuning(0, fuse)
+        g.init_from_prims(pb.prims, pb.shift_pressure)
+        g.begin_run(float("inf"))
+        g.advance(max(at_step, 1), prec)
+        g.sync()
+        for _ in range(rounds):
+            for cand in candidates:
+                g.set_tuning(cand, fuse)
+                t0 = time.perf_counter()
+                g.advance(steps, prec)
+                g.sync()
+                timing[cand] += time.perf_counter() - t0
     finally:
         g.close()
     best = min(timing, key=timing.get)

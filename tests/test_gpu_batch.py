@@ -72,14 +72,14 @@ def test_batch_seglen_and_graph_replays():
 
 
 def test_level_seglen_tuner():
-    """The level tuner times its candidates on a scratch level and returns one
-    of them; a level run with the chosen segment length equals a run with the
-    library's own pick, bit for bit."""
+    """The level tuner times its candidates in turns on a scratch level and
+    returns one of them; a level run with the chosen segment length equals a
+    run with the library's own pick, bit for bit."""
     from tenjin_amd import autotune, configs
 
     pb = configs.shock_inclusion_2d(nx=96, ny=48, box=32)
     print("\n[case] level seglen tuner 96x48", flush=True)
-    sl, timing = autotune.tune_level_seglen(pb, 0, steps=2, warmup=1)
+    sl, timing = autotune.tune_level_seglen(pb, 0, at_step=4, steps=2, rounds=2)
     assert set(timing) == set(autotune.LEVEL_CANDIDATES) and sl in timing
     assert all(t > 0.0 for t in timing.values())
 
