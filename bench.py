@@ -583,7 +583,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the domain decomposition over several GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch
