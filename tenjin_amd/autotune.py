@@ -95,7 +95,13 @@ def tune_level_seglen(pb, prec: int = 0, fuse: int = 0, at_step: int = 0, steps:
                 timing[cand] += time.perf_counter() - t0
     finally:
         g.close()
+    return choose(timing, margin), timing
+
+
+def choose(timing, margin: float = 0.03):
+    """The fastest candidate of {candidate: seconds}, except that the library's
+    own pick (0) stays unless the fastest beats it by more than `margin`."""
     best = min(timing, key=timing.get)
     if 0 in timing and timing[best] * (1.0 + margin) >= timing[0]:
         best = 0
-    return best, timing
+    return best
