@@ -1,0 +1,43 @@
+"""bench.py's output contract on the GPU: one JSON line with the metric,
+throughput, step counts, the dominant kernel's roofline and the CPU baseline
+(a small 2D shock/inclusion level, run as a child process)."""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_bench_json_contract():
+    print("\n[case] bench.py --nx 256 --ny 128 --steps 3 --warmup 1", flush=True)
+    cmd = [sys.executable, os.path.join(ROOT, "bench.py"), "--nx", "256", "--ny", "128", "--box", "64",
+           "--steps", "3", "--warmup", "1", "--cpu-seconds", "1"]
+    r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    assert len(lines) == 1
+    d = json.loads(lines[0])
+    for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
+              "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
+        assert k in d, k
+    assert d["metric"] == "Mcell-updates/s on 1 GPU" and d["unit"] == "Mcell-updates/s"
+    assert d["n_gpus"] == 1 and d["steps"] == 3 and d["warmup"] == 1 and d["higher_is_better"] is True
+    assert d["value"] > 0 and d["ms_per_step"] > 0 and d["status"] == 0
+    assert abs(d["value"] - 256 * 128 / (d["ms_per_step"] * 1e3)) <= 0.02 * d["value"]
+    assert d["dtype"] == "f64" and "workload" in d["config"]
+    rf = d["roofline"]
+    for k in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
+        assert k in rf, k
+    assert rf["bound"] == "hbm" and rf["peak"] == 8000.0 and rf["unit"] == "GB/s"
+    assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3
+    cb = d["cpu_baseline"]
+    assert cb["value"] > 0 and cb["kind"] == "port" and cb["cores"] >= 1 and cb["sample"]
+    m = d["mixed"]
+    assert m["value"] > 0 and m["l1_rel_err_vs_fp64"]["rho"] <= 1e-5
