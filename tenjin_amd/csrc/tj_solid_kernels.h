@@ -255,29 +255,30 @@ __device__ __forceinline__ void load_scell(const Level& lv, const MF& U, int i, 
     for (int q = 0; q < 9; ++q) C.V[q] = U.at(lv.cs[V_TENSOR] + q, i, j, k);
     C.hj2 = U.at(lv.cs[HJ2], i, j, k);
 }
-template <int NM>
+// NT: non-temporal stores (put_state), for the end-of-step state
+template <int NM, bool NT = false>
 __device__ __forceinline__ void store_scell(const Level& lv, const MF& U, int i, int j, int k, const SCell<NM>& C) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        U.at(lv.cs[ALPHA] + m, i, j, k) = C.al[m];
-        U.at(lv.cs[ALPHARHO] + m, i, j, k) = C.ar[m];
-        U.at(lv.cs[RHO_K] + m, i, j, k) = C.rk[m];
+        put_state<NT>(U.at(lv.cs[ALPHA] + m, i, j, k), C.al[m]);
+        put_state<NT>(U.at(lv.cs[ALPHARHO] + m, i, j, k), C.ar[m]);
+        put_state<NT>(U.at(lv.cs[RHO_K] + m, i, j, k), C.rk[m]);
         if (lv.plastic) {
-            U.at(lv.cs[EPSILON] + m, i, j, k) = C.e[m];
-            U.at(lv.cs[ALPHARHOEPSILON] + m, i, j, k) = C.are[m];
+            put_state<NT>(U.at(lv.cs[EPSILON] + m, i, j, k), C.e[m]);
+            put_state<NT>(U.at(lv.cs[ALPHARHOEPSILON] + m, i, j, k), C.are[m]);
         }
     }
-    U.at(lv.cs[RHO], i, j, k) = C.rho;
+    put_state<NT>(U.at(lv.cs[RHO], i, j, k), C.rho);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        U.at(lv.cs[RHOU] + q, i, j, k) = C.ru[q];
-        U.at(lv.cs[VELOCITY] + q, i, j, k) = C.u[q];
+        put_state<NT>(U.at(lv.cs[RHOU] + q, i, j, k), C.ru[q]);
+        put_state<NT>(U.at(lv.cs[VELOCITY] + q, i, j, k), C.u[q]);
     }
-    U.at(lv.cs[TOTAL_E], i, j, k) = C.E;
-    U.at(lv.cs[PRESSURE], i, j, k) = C.p;
+    put_state<NT>(U.at(lv.cs[TOTAL_E], i, j, k), C.E);
+    put_state<NT>(U.at(lv.cs[PRESSURE], i, j, k), C.p);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) U.at(lv.cs[V_TENSOR] + q, i, j, k) = C.V[q];
-    U.at(lv.cs[HJ2], i, j, k) = C.hj2;
+    for (int q = 0; q < 9; ++q) put_state<NT>(U.at(lv.cs[V_TENSOR] + q, i, j, k), C.V[q]);
+    put_state<NT>(U.at(lv.cs[HJ2], i, j, k), C.hj2);
 }
 
 // geometricSourceTerm (radial.cpp:5-31, flux.cpp:51-77) of a solid cell
@@ -412,13 +413,13 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
             }
             if (ops & 8u) {
                 const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
-                Uo.at(lv.cs[SOUNDSPEED], i, j, k) = a;
+                put_state<STAGE == 2>(Uo.at(lv.cs[SOUNDSPEED], i, j, k), a);
                 #pragma unroll
                 for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
                     if (d < lv.dim) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
             }
         }
-        store_scell<NM>(lv, Uo, i, j, k, C);
+        store_scell<NM, STAGE == 2>(lv, Uo, i, j, k, C);
     }
     if (STAGE == 2 && (ops & 8u)) {
         block_min_to(cand, &clk->dtmin_bits);
