@@ -281,6 +281,10 @@ __device__ __forceinline__ void put_state(double& dst, double v) {
     if constexpr (NT) __builtin_nontemporal_store(v, &dst);
     else dst = v;
 }
+// A read of the step's starting state for the RK average of stage 2, its
+// only reader in the step: non-temporal, which keeps the caches for the
+// stage's other reads (same level: step +1.7 % FP64, +1.5 % mixed).
+__device__ __forceinline__ double get_base(const double& src) { return __builtin_nontemporal_load(&src); }
 
 template <int NM, int STAGE, bool MX = false>
 __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const MF& Uo, int i, int j, int k, double* al,
@@ -292,23 +296,23 @@ __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const
     if (STAGE == 2) {
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
-            al[m] = Ub.at(lv.cs[ALPHA] + m, i, j, k) * (1.0 / 2.0) + al[m] * (1.0 / 2.0);
-            ar[m] = Ub.at(lv.cs[ALPHARHO] + m, i, j, k) * (1.0 / 2.0) + ar[m] * (1.0 / 2.0);
-            rk[m] = Ub.at(lv.cs[RHO_K] + m, i, j, k) * (1.0 / 2.0) + rk[m] * (1.0 / 2.0);
+            al[m] = get_base(Ub.at(lv.cs[ALPHA] + m, i, j, k)) * (1.0 / 2.0) + al[m] * (1.0 / 2.0);
+            ar[m] = get_base(Ub.at(lv.cs[ALPHARHO] + m, i, j, k)) * (1.0 / 2.0) + ar[m] * (1.0 / 2.0);
+            rk[m] = get_base(Ub.at(lv.cs[RHO_K] + m, i, j, k)) * (1.0 / 2.0) + rk[m] * (1.0 / 2.0);
         }
-        rho = Ub.at(lv.cs[RHO], i, j, k) * (1.0 / 2.0) + rho * (1.0 / 2.0);
+        rho = get_base(Ub.at(lv.cs[RHO], i, j, k)) * (1.0 / 2.0) + rho * (1.0 / 2.0);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            ru[q] = Ub.at(lv.cs[RHOU] + q, i, j, k) * (1.0 / 2.0) + ru[q] * (1.0 / 2.0);
-            u[q] = Ub.at(lv.cs[VELOCITY] + q, i, j, k) * (1.0 / 2.0) + u[q] * (1.0 / 2.0);
+            ru[q] = get_base(Ub.at(lv.cs[RHOU] + q, i, j, k)) * (1.0 / 2.0) + ru[q] * (1.0 / 2.0);
+            u[q] = get_base(Ub.at(lv.cs[VELOCITY] + q, i, j, k)) * (1.0 / 2.0) + u[q] * (1.0 / 2.0);
         }
-        E = Ub.at(lv.cs[TOTAL_E], i, j, k) * (1.0 / 2.0) + E * (1.0 / 2.0);
-        p = Ub.at(lv.cs[PRESSURE], i, j, k) * (1.0 / 2.0) + p * (1.0 / 2.0);
+        E = get_base(Ub.at(lv.cs[TOTAL_E], i, j, k)) * (1.0 / 2.0) + E * (1.0 / 2.0);
+        p = get_base(Ub.at(lv.cs[PRESSURE], i, j, k)) * (1.0 / 2.0) + p * (1.0 / 2.0);
         if (MX) {
-            arl = Ub.at(lv.cs[ALPHARHOLAMBDA], i, j, k) * (1.0 / 2.0) + arl * (1.0 / 2.0);
-            x.lam = Ub.at(lv.cs[LAMBDA], i, j, k) * (1.0 / 2.0) + x.lam * (1.0 / 2.0);
-            x.ra = Ub.at(lv.cs[RHO_MIX], i, j, k) * (1.0 / 2.0) + x.ra * (1.0 / 2.0);
-            x.rb = Ub.at(lv.cs[RHO_MIX] + 1, i, j, k) * (1.0 / 2.0) + x.rb * (1.0 / 2.0);
+            arl = get_base(Ub.at(lv.cs[ALPHARHOLAMBDA], i, j, k)) * (1.0 / 2.0) + arl * (1.0 / 2.0);
+            x.lam = get_base(Ub.at(lv.cs[LAMBDA], i, j, k)) * (1.0 / 2.0) + x.lam * (1.0 / 2.0);
+            x.ra = get_base(Ub.at(lv.cs[RHO_MIX], i, j, k)) * (1.0 / 2.0) + x.ra * (1.0 / 2.0);
+            x.rb = get_base(Ub.at(lv.cs[RHO_MIX] + 1, i, j, k)) * (1.0 / 2.0) + x.rb * (1.0 / 2.0);
             if (ops & 64u) {  // the burn on the averaged state, then c2p
 #pragma unroll
                 for (int m = 0; m < NM; ++m)

@@ -233,27 +233,33 @@ struct SCell {
     double al[NM], ar[NM], rk[NM], e[NM], are[NM], rho, ru[3], u[3], E, p, V[9], hj2;
 };
 
-template <int NM>
+template <bool NT>
+__device__ __forceinline__ double ld(const double& src) {
+    if constexpr (NT) return get_base(src);
+    else return src;
+}
+// NT: non-temporal loads (get_base), for the step's starting state in stage 2
+template <int NM, bool NT = false>
 __device__ __forceinline__ void load_scell(const Level& lv, const MF& U, int i, int j, int k, SCell<NM>& C) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        C.al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
-        C.ar[m] = U.at(lv.cs[ALPHARHO] + m, i, j, k);
-        C.rk[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
-        C.e[m] = lv.plastic ? U.at(lv.cs[EPSILON] + m, i, j, k) : 0.0;
-        C.are[m] = lv.plastic ? U.at(lv.cs[ALPHARHOEPSILON] + m, i, j, k) : 0.0;
+        C.al[m] = ld<NT>(U.at(lv.cs[ALPHA] + m, i, j, k));
+        C.ar[m] = ld<NT>(U.at(lv.cs[ALPHARHO] + m, i, j, k));
+        C.rk[m] = ld<NT>(U.at(lv.cs[RHO_K] + m, i, j, k));
+        C.e[m] = lv.plastic ? ld<NT>(U.at(lv.cs[EPSILON] + m, i, j, k)) : 0.0;
+        C.are[m] = lv.plastic ? ld<NT>(U.at(lv.cs[ALPHARHOEPSILON] + m, i, j, k)) : 0.0;
     }
-    C.rho = U.at(lv.cs[RHO], i, j, k);
+    C.rho = ld<NT>(U.at(lv.cs[RHO], i, j, k));
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        C.ru[q] = U.at(lv.cs[RHOU] + q, i, j, k);
-        C.u[q] = U.at(lv.cs[VELOCITY] + q, i, j, k);
+        C.ru[q] = ld<NT>(U.at(lv.cs[RHOU] + q, i, j, k));
+        C.u[q] = ld<NT>(U.at(lv.cs[VELOCITY] + q, i, j, k));
     }
-    C.E = U.at(lv.cs[TOTAL_E], i, j, k);
-    C.p = U.at(lv.cs[PRESSURE], i, j, k);
+    C.E = ld<NT>(U.at(lv.cs[TOTAL_E], i, j, k));
+    C.p = ld<NT>(U.at(lv.cs[PRESSURE], i, j, k));
 #pragma unroll
-    for (int q = 0; q < 9; ++q) C.V[q] = U.at(lv.cs[V_TENSOR] + q, i, j, k);
-    C.hj2 = U.at(lv.cs[HJ2], i, j, k);
+    for (int q = 0; q < 9; ++q) C.V[q] = ld<NT>(U.at(lv.cs[V_TENSOR] + q, i, j, k));
+    C.hj2 = ld<NT>(U.at(lv.cs[HJ2], i, j, k));
 }
 // NT: non-temporal stores (put_state), for the end-of-step state
 template <int NM, bool NT = false>
@@ -370,7 +376,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
         C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
         if (STAGE == 2) {
             SCell<NM> B;
-            load_scell<NM>(lv, Ub, i, j, k, B);
+            load_scell<NM, true>(lv, Ub, i, j, k, B);
             // U1 = (U * 1/2) + (U2 * 1/2) over every component
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
