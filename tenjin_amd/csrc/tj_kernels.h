@@ -193,18 +193,38 @@ __device__ __forceinline__ void load_faces(const FacesT<T>& Fd, int i, int j, in
 #pragma unroll
     for (int q = 0; q < 2 * NM + 5 + (MX ? 1 : 0); ++q) Fv[q] = Fd.at(q, i, j, k);
 }
+// A state store of the update kernels.  The end-of-step state (stage 2) goes
+// out non-temporal: it is not read again until the next step's x pass, a
+// level's state is larger than the MALL, and keeping it out of the caches
+// leaves them to the stage's reads (measured on the 2048 x 1024
+// shock/inclusion level: stage-2 update 0.194 -> 0.188 ms FP64, 0.153 ->
+// 0.144 ms mixed, the flux passes 1-3 % faster; the same hint on the stage-1
+// store, whose output the stage-2 passes read at once, costs 3-8 %).  The
+// solid stage-2 update gains 1 % (FP64) and 1.5 % (mixed) on the 256^3 plate.
+template <bool NT>
+__device__ __forceinline__ void put_state(double& dst, double v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, &dst);
+    else dst = v;
+}
+// A state read of the update kernels that nothing else in the kernel or the
+// next one reads again: the conserved components of the stage's input and,
+// in stage 2, the step's starting state for the RK average.  Non-temporal,
+// which keeps the caches for the faces (same level: the base read in stage 2
+// +1.7 % FP64 and +1.5 % mixed on the step; the conserved reads then the
+// stage-1 update -5.6 % and stage-2 -3.5 % FP64, stage-2 mixed -8 %).
+__device__ __forceinline__ double get_base(const double& src) { return __builtin_nontemporal_load(&src); }
 template <int NM, bool MX = false>
 __device__ __forceinline__ void load_cons(const Level& lv, const MF& U, int i, int j, int k, double* al, double* ar,
                                           double* ru, double& E, double* arl = nullptr) {
-    if (MX) *arl = U.at(lv.cs[ALPHARHOLAMBDA], i, j, k);
+    if (MX) *arl = get_base(U.at(lv.cs[ALPHARHOLAMBDA], i, j, k));
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
-        ar[m] = U.at(lv.cs[ALPHARHO] + m, i, j, k);
+        al[m] = get_base(U.at(lv.cs[ALPHA] + m, i, j, k));
+        ar[m] = get_base(U.at(lv.cs[ALPHARHO] + m, i, j, k));
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) ru[q] = U.at(lv.cs[RHOU] + q, i, j, k);
-    E = U.at(lv.cs[TOTAL_E], i, j, k);
+    for (int q = 0; q < 3; ++q) ru[q] = get_base(U.at(lv.cs[RHOU] + q, i, j, k));
+    E = get_base(U.at(lv.cs[TOTAL_E], i, j, k));
 }
 // geometricSourceTerm (radial.cpp:5-31): the factor dt * multiplier /
 // ((i + 1/2) dx) of cell i, multiplier -2 in one dimension (spherical) and -1
@@ -268,24 +288,6 @@ __device__ __forceinline__ double burn_rk2(double alpha, double ar, double y, do
 // getTimeStep (main.cpp:128); then the store.  MX: the mixture's
 // alpha rho lambda (arl) rides along, lambda and the phase densities come
 // from rootFind in each c2p and are averaged with the rest.
-// A state store of the update kernels.  The end-of-step state (stage 2) goes
-// out non-temporal: it is not read again until the next step's x pass, a
-// level's state is larger than the MALL, and keeping it out of the caches
-// leaves them to the stage's reads (measured on the 2048 x 1024
-// shock/inclusion level: stage-2 update 0.194 -> 0.188 ms FP64, 0.153 ->
-// 0.144 ms mixed, the flux passes 1-3 % faster; the same hint on the stage-1
-// store, whose output the stage-2 passes read at once, costs 3-8 %).  The
-// solid stage-2 update gains 1 % (FP64) and 1.5 % (mixed) on the 256^3 plate.
-template <bool NT>
-__device__ __forceinline__ void put_state(double& dst, double v) {
-    if constexpr (NT) __builtin_nontemporal_store(v, &dst);
-    else dst = v;
-}
-// A read of the step's starting state for the RK average of stage 2, its
-// only reader in the step: non-temporal, which keeps the caches for the
-// stage's other reads (same level: step +1.7 % FP64, +1.5 % mixed).
-__device__ __forceinline__ double get_base(const double& src) { return __builtin_nontemporal_load(&src); }
-
 template <int NM, int STAGE, bool MX = false>
 __device__ __forceinline__ void finish_cell(const Level& lv, const MF& Ub, const MF& Uo, int i, int j, int k, double* al,
                                             double* ar, double* ru, double E, double dt, unsigned ops, double& cand,
