@@ -238,7 +238,8 @@ __device__ __forceinline__ double ld(const double& src) {
     if constexpr (NT) return get_base(src);
     else return src;
 }
-// NT: non-temporal loads (get_base), for the step's starting state in stage 2
+// NT: non-temporal loads (get_base), for the update's input and, in stage 2,
+// the step's starting state (256^3 plate: the update stages 0.3-0.8 % faster)
 template <int NM, bool NT = false>
 __device__ __forceinline__ void load_scell(const Level& lv, const MF& U, int i, int j, int k, SCell<NM>& C) {
 #pragma unroll
@@ -339,7 +340,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
         const int k = (int)(id / ((long long)nx * ny));
         const double dt = clk->dt;
         SCell<NM> C;
-        load_scell<NM>(lv, Uin, i, j, k, C);
+        load_scell<NM, true>(lv, Uin, i, j, k, C);
         double a0[NM], V0[9], u0[3];
 #pragma unroll
         for (int m = 0; m < NM; ++m) a0[m] = C.al[m];
