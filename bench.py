@@ -124,13 +124,32 @@ def parse():
     return args
 
 
-def cpu_baseline(pb, budget_s: float, max_steps: int = 20):
-    """The oracle (CPU restatement) on this host's cores: first steps of the
-    same workload from its initial condition, until the time budget is used."""
-    from oracle.oracle import Oracle
+def require_clean_run(status: int, steps_done: int, steps_expected: int, what: str) -> None:
+    """A timed run counts only if every loop body ran a real step: a status bit
+    (NaN stop, fatal states) or a loop that ended early makes the later
+    launches no-ops and the rate meaningless, so the bench stops instead."""
+    if status != 0 or steps_done != steps_expected:
+        raise SystemExit(f"bench: {what}: the timed run is invalid (status {status}, {steps_done} device steps "
+                         f"of {steps_expected})")
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_leg(pb, threads: int, budget_s: float, max_steps: int):
+    """The oracle on `threads` OpenMP threads: one warm-up step, then steps
+    until the time budget or max_steps is reached; (steps, seconds)."""
+    from oracle.oracle import Oracle, set_threads
+
+    set_threads(threads)
     o = Oracle(pb)
     o.step(0.2 * o.timestep())  # warm-up step (not timed)
     n = 0
@@ -142,13 +161,30 @@ def cpu_baseline(pb, budget_s: float, max_steps: int = 20):
         el = time.perf_counter() - t0
         if el >= budget_s or n >= max_steps:
             break
+    del o
+    return n, el
+
+
+def cpu_baseline(pb, budget_s: float, max_steps: int = 20):
+    """The oracle (CPU restatement) on this host's cores: first steps of the
+    same workload from its initial condition, until the time budget is used,
+    on the host's CPU share (OMP_NUM_THREADS, else all cores up to 16) and on
+    one core.  The budget is split between the two legs."""
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, nproc)
+    n, el = _cpu_leg(pb, threads, 0.6 * budget_s, max_steps)
+    n1, el1 = _cpu_leg(pb, 1, 0.4 * budget_s, max_steps)
+    size = "x".join(str(v) for v in pb.n[:pb.dim])
     return {
         "value": pb.ncells * n / el / 1e6,
         "unit": "Mcell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle restatement, {'x'.join(str(v) for v in pb.n[:pb.dim])} level, {n} steps after 1 warm-up "
-                  f"step, {el:.1f} s, {threads} OpenMP threads over boxes",
+        "one_core": {"value": pb.ncells * n1 / el1 / 1e6, "cores": 1, "steps": n1, "seconds": round(el1, 2)},
+        "cpu_model": cpu_model(),
+        "nproc": nproc,
+        "sample": f"oracle restatement, {size} level, {n} steps after 1 warm-up step, {el:.1f} s, {threads} OpenMP "
+                  f"threads over boxes (the host's CPU share; nproc {nproc}); one core: {n1} steps, {el1:.1f} s",
     }
 
 
@@ -216,7 +252,7 @@ def bench_tube1d(args):
                 g.advance(group, TJ_PREC_FP64)
         g.sync()
         el = time.perf_counter() - t0
-        assert g.status() == 0
+        require_clean_run(g.status(), g.clock()[2], group + steps, f"tube1d graph={graph}")
         del g
         return el
 
@@ -281,6 +317,8 @@ def bench_ensemble(args):
     b.sync()
     el = time.perf_counter() - t0
     t, n, st = b.clocks()
+    for p_, (sp, np_) in enumerate(zip(st, n)):
+        require_clean_run(int(sp), int(np_), steps, f"ensemble problem {p_}")
     cells = sum(pb.ncells for pb in probs)
     value = cells * steps / el / 1e6
     out = {
@@ -351,6 +389,7 @@ def bench_plate2d_grad(args):
 
     forward_once()  # warm-up
     t_fwd, st_fwd = min(forward_once() for _ in range(3))
+    require_clean_run(st_fwd, steps, steps, "plate2d_grad forward")
 
     # the mixed forward beside it: time, and relative L1 of rho, p, alpha and
     # of the probe trace against the FP64 run after the same steps
@@ -406,6 +445,7 @@ def bench_plate2d_grad(args):
     t_grad = time.perf_counter() - t0
     st_grad = tr.status()
     tr.close()
+    require_clean_run(st_grad, done, steps, "plate2d_grad gradient")
     r = trace[:, 0] - target
     loss = float(r @ r)
     grad = 2.0 * (r @ trace[:, 1:])
@@ -533,6 +573,7 @@ def bench_inclusion2d_grad(args):
     t_grad = time.perf_counter() - t0
     st = tr.status()
     tr.close()
+    require_clean_run(st, done, steps, "inclusion2d_grad gradient")
     r = trace[:, 0] - target
     grad = 2.0 * (r @ trace[:, 1:])
     fd = []
@@ -659,6 +700,7 @@ def main():
         g.prof_enable(False)
         status = g.status()
         t_end, _, nsteps_done = g.clock()
+        require_clean_run(status, nsteps_done, args.warmup + args.steps, f"{args.workload} prec {prec}")
         el = t1 - t0
         dom = max(prof.items(), key=lambda kv: kv[1][1])
         dname, (dcount, dms) = dom

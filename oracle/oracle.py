@@ -96,6 +96,7 @@ def lib():
         L.tjo_vanleer.restype = ctypes.c_double
         L.tjo_math.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.tjo_math.restype = ctypes.c_double
+        L.tjo_set_threads.argtypes = [ctypes.c_int]
         L.tjo_sqrtm.argtypes = [dp, dp]
         L.tjo_inv3.argtypes = [dp, dp]
         L.tjo_det3.argtypes = [dp]
@@ -227,3 +228,8 @@ class Oracle:
         if n < 5:
             dt *= 0.2
         return dt
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's box loops."""
+    lib().tjo_set_threads(int(n))

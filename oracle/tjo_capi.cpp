@@ -1,6 +1,9 @@
 // Test infrastructure only (the oracle): C entry points over the CPU
 // restatement in tjo_oracle.hpp, loaded by tests/ and bench.py through ctypes.
 #include <cstring>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <vector>
 
 #include "tjo_oracle.hpp"
@@ -269,4 +272,12 @@ int tjo_chop(int n, int chunk, int* starts) {
     return (int)c.size();
 }
 
+// OpenMP threads of the box loops (the CPU baseline times several counts in one process).
+void tjo_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 }  // extern "C"

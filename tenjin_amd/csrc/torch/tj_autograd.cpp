@@ -111,13 +111,16 @@ class ProbeLoss : public torch::autograd::Function<ProbeLoss> {
                                  torch::Tensor target, torch::Tensor desc, int64_t nsteps, std::vector<int64_t> probe,
                                  int64_t chunk, int64_t shift) {
         auto res = evaluate(theta, prims, target, desc, nsteps, probe, chunk, shift);
-        ctx->saved_data["grad"] = torch::tensor(res.second, torch::kFloat64);
-        return torch::tensor(res.first, torch::kFloat64).to(theta.dtype());
+        // the gradient takes theta's device, dtype and shape, the loss theta's device and dtype, so
+        // autograd accepts them for a theta of any shape (3 * nmat values) on the CPU or the GPU
+        ctx->saved_data["grad"] =
+            torch::tensor(res.second, torch::kFloat64).to(theta.device(), theta.scalar_type()).view(theta.sizes());
+        return torch::tensor(res.first, torch::kFloat64).to(theta.device(), theta.scalar_type());
     }
     static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
                                                    torch::autograd::variable_list grad_out) {
         const torch::Tensor g = ctx->saved_data["grad"].toTensor();
-        return {grad_out[0] * g.to(grad_out[0].dtype()), torch::Tensor(), torch::Tensor(), torch::Tensor(),
+        return {grad_out[0].to(g.device(), g.scalar_type()) * g, torch::Tensor(), torch::Tensor(), torch::Tensor(),
                 torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor()};
     }
 };

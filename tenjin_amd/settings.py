@@ -196,12 +196,18 @@ def problem_from_settings(cfg, max_grid_size: int = 64, dim: Optional[int] = Non
         mask = sel == s
         if not mask.any():
             continue
+        # getState (initialise.cpp:237-263) keeps one F per state, and each
+        # solid material's entry overwrites it: the last solid's F sets V and
+        # the density of every solid (initialise.cpp:27-46, 74-77)
+        Fs = None
+        for m in range(nm):
+            if mats[m].solid:
+                Fs = np.asarray(st["material"][m]["F"], dtype=np.float64).reshape(3, 3)
         for m in range(nm):
             sm = st["material"][m]
             U[Lay.comp(ALPHA, m)][mask] = float(sm["alpha"])
             if mats[m].solid:
-                F = np.asarray(sm["F"], dtype=np.float64).reshape(3, 3)
-                U[Lay.comp(RHO_K, m)][mask] = mats[m].rho0 / np.linalg.det(F)  # setRhoFromDeformationTensor
+                U[Lay.comp(RHO_K, m)][mask] = mats[m].rho0 / np.linalg.det(Fs)  # setRhoFromDeformationTensor
             elif mats[m].mixture:  # initialise.cpp:66-71: rho_k and both phase densities = rhoa
                 U[Lay.comp(RHO_K, m)][mask] = float(sm["rhoa"])
                 U[Lay.comp(RHO_MIX, m, 0)][mask] = float(sm["rhoa"])
@@ -215,8 +221,7 @@ def problem_from_settings(cfg, max_grid_size: int = 64, dim: Optional[int] = Non
         for r, key in enumerate(("u", "v", "w")):
             U[Lay.comp(VELOCITY, 0, r)][mask] = float(st.get(key, 0.0))
         if solid_run:
-            F = next((st["material"][m]["F"] for m in range(nm) if mats[m].solid), None)
-            V = _stretch(F) if F is not None else np.eye(3)
+            V = _stretch(Fs) if Fs is not None else np.eye(3)
             for r in range(3):
                 for c in range(3):
                     U[Lay.comp(V_TENSOR, 0, r, c)][mask] = V[r, c]

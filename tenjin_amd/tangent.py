@@ -142,14 +142,15 @@ def _autograd_function():
 
         @staticmethod
         def forward(ctx, theta, target, n, steps, chunk):
-            loss, grad = loss_and_grad(theta.detach().cpu().double().numpy(), target, n, steps, chunk)
-            ctx.save_for_backward(torch.as_tensor(grad, dtype=theta.dtype))
-            return torch.as_tensor(loss, dtype=theta.dtype)
+            loss, grad = loss_and_grad(theta.detach().cpu().double().numpy().reshape(-1), target, n, steps, chunk)
+            # gradient with theta's device, dtype and shape; the loss on theta's device
+            ctx.save_for_backward(torch.as_tensor(grad, dtype=theta.dtype, device=theta.device).view(theta.shape))
+            return torch.as_tensor(loss, dtype=theta.dtype, device=theta.device)
 
         @staticmethod
         def backward(ctx, g):
             (grad,) = ctx.saved_tensors
-            return g * grad, None, None, None, None
+            return g.to(grad.device) * grad, None, None, None, None
 
     return ProbeLoss
 

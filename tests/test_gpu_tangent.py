@@ -371,6 +371,33 @@ def test_libtorch_autograd_function():
     assert np.allclose(theta.grad.numpy(), 3.0 * ref_grad, rtol=1e-10, atol=1e-12 * np.max(np.abs(3.0 * ref_grad)))
 
 
+def test_autograd_theta_on_gpu_any_shape():
+    """Both autograd Functions accept theta on the GPU and of shape [2, 3]
+    (per material: Gamma0, s, Y): theta.grad has theta's device and shape
+    and equals loss_and_grad's gradient."""
+    import torch
+
+    from tenjin_amd import configs, tangent
+
+    steps = 8
+    pb = _problem(steps=steps)
+    probe = configs.plate_probe(pb)
+    target = tangent.forward_trace(_problem(configs.plate_target_theta(), steps=steps), steps, probe)
+    ref_loss, ref_grad = tangent.loss_and_grad(configs.plate_params(), target, n=N, steps=steps, chunk=1)
+    for name, fn in (("python", lambda th: tangent.probe_loss_torch(th, target, n=N, steps=steps, chunk=3)),
+                     ("libtorch", lambda th: tangent.probe_loss_libtorch(th, target, n=N, steps=steps, chunk=3))):
+        print(f"\n[case] theta on cuda, shape [2, 3], {name}", flush=True)
+        theta = torch.tensor(configs.plate_params(), dtype=torch.float64, device="cuda").view(2, 3)
+        theta.requires_grad_(True)
+        loss = fn(theta)
+        assert loss.device == theta.device
+        (2.0 * loss).backward()
+        assert theta.grad.shape == (2, 3) and theta.grad.device == theta.device
+        got = theta.grad.detach().cpu().numpy().reshape(-1)
+        assert abs(float(loss.detach().cpu()) - ref_loss) <= 1e-12 * ref_loss
+        assert np.allclose(got, 2.0 * ref_grad, rtol=1e-10, atol=1e-12 * np.max(np.abs(ref_grad)))
+
+
 def _fluid_fd(make, m, field, rel, steps, probe):
     """Central difference of the oracle's probe trace in material m's
     `field` ("gamma" = adiabaticIndex, i.e. Gamma0 + 1, or "pref")."""

@@ -95,3 +95,47 @@ def test_custom_geometry_and_rejections():
     bad["listOfMaterials"][1]["mixture"] = 1  # a mixture needs the products' *mix constants
     with pytest.raises(KeyError):
         settings.problem_from_settings(bad)
+
+
+SOLID_DECK = """
+numberOfxCells = 8;  numberOfyCells = 1;  numberOfzCells = 1;
+xdomainLength = 1.0;
+CFL = 0.5;  materials = 2;  states = 2;  interface = 0.5;
+SOLID = 1; THINC = 0; PLASTIC = 1; MUSCL = 1;
+listOfMaterials = (
+  { material = "solid"; EOS = "UsUpSolid"; rho0 = 8930.0; Gamma0 = 2.0; c0 = 3940.0; s = 1.49;
+    G0 = 45.0e9; yieldStress = 90.0e6; },
+  { material = "solid"; EOS = "UsUpSolid"; rho0 = 2703.0; Gamma0 = 2.0; c0 = 5328.0; s = 1.338;
+    G0 = 26.0e9; yieldStress = 300.0e6; }
+);
+listOfStates = (
+  { material = ( { alpha = 0.999999; F = [1.02, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]; },
+                 { alpha = 0.000001; F = [0.95, 0.01, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]; } );
+    p = 0.0; u = 100.0; v = 0.0; w = 0.0; },
+  { material = ( { alpha = 0.000001; F = [1.0, 0.0, 0.0, 0.0, 1.1, 0.0, 0.0, 0.0, 1.0]; },
+                 { alpha = 0.999999; F = [1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.9]; } );
+    p = 0.0; u = 0.0; v = 0.0; w = 0.0; }
+);
+"""
+
+
+def test_solid_states_take_the_last_solid_F():
+    """getState keeps one F per state and each solid material overwrites it
+    (initialise.cpp:237-263): the last solid's F sets V and the density of
+    every solid of the state (initialise.cpp:27-46, 74-77)."""
+    from tenjin_amd.configs import V_TENSOR
+
+    pb = settings.problem_from_settings(SOLID_DECK)
+    L, U = pb.layout, pb.prims
+    lastF = [np.array([0.95, 0.01, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]).reshape(3, 3),
+             np.diag([1.0, 1.0, 0.9])]
+    rho0 = [8930.0, 2703.0]
+    for s, i in ((0, 0), (1, 7)):
+        F = lastF[s]
+        for m in range(2):
+            assert U[L.comp(RHO_K, m)][0, 0, i] == rho0[m] / np.linalg.det(F)
+        V = np.array([[U[L.comp(V_TENSOR, 0, r, c)][0, 0, i] for c in range(3)] for r in range(3)])
+        # V = sqrt(F F^T) normalised to det 1 (closed form check: V V^T = B / det(B)^(1/3))
+        B = F @ F.T
+        assert np.allclose(V @ V.T, B / np.linalg.det(B) ** (1.0 / 3.0), rtol=0, atol=1e-14)
+        assert np.allclose(V, V.T, atol=1e-15)
