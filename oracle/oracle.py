@@ -97,6 +97,8 @@ def lib():
         L.tjo_math.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.tjo_math.restype = ctypes.c_double
         L.tjo_set_threads.argtypes = [ctypes.c_int]
+        L.tjo_transverse.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.tjo_transverse.restype = ctypes.c_double
         L.tjo_sqrtm.argtypes = [dp, dp]
         L.tjo_inv3.argtypes = [dp, dp]
         L.tjo_det3.argtypes = [dp]
@@ -183,6 +185,10 @@ class Oracle:
         """UL, UR of direction d from the current state (MUSCL [+THINC/BVD], p2c, sound speed)."""
         lib().tjo_reconstruct(self.h, d)
         return self.get(self.UL), self.get(self.UR)
+
+    def transverse(self, i: int, j: int = 0, k: int = 0) -> float:
+        """transverseWaveSpeed of valid cell (i, j, k) of U1."""
+        return lib().tjo_transverse(self.h, i, j, k)
 
     def face_flux(self, d: int, left: np.ndarray, right: np.ndarray) -> np.ndarray:
         l = np.ascontiguousarray(left, dtype=np.float64)

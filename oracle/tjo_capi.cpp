@@ -272,6 +272,17 @@ int tjo_chop(int n, int chunk, int* starts) {
     return (int)c.size();
 }
 
+// transverseWaveSpeed (boxaccesscellarray.cpp:287-324) of valid cell (i,j,k) of U1
+// (it recomputes that cell's HJ2 first, as the reference does).
+double tjo_transverse(void* h, int i, int j, int k) {
+    auto* s = static_cast<tjo::Solver*>(h);
+    for (size_t b = 0; b < s->boxes.size(); ++b) {
+        const tjo::IBox& bx = s->boxes[b];
+        if (i >= bx.lo[0] && i <= bx.hi[0] && j >= bx.lo[1] && j <= bx.hi[1] && k >= bx.lo[2] && k <= bx.hi[2])
+            return s->transverse(s->U1.f[b], i, j, k);
+    }
+    return -1.0;
+}
 // OpenMP threads of the box loops (the CPU baseline times several counts in one process).
 void tjo_set_threads(int n) {
 #ifdef _OPENMP
