@@ -83,7 +83,8 @@ enum {
     TJ_EINVAL = -1,
     TJ_EUNSUPPORTED = -2,
     TJ_EHIP = -3,
-    TJ_ENOLEVEL = -4
+    TJ_ENOLEVEL = -4,
+    TJ_ESCRATCH = -5  /* tj_run_static: the level's buffers exceed TJ_STATIC_SCRATCH_BYTES */
 };
 
 /* status bits (tj_status) */
@@ -195,6 +196,30 @@ int tj_graph_launch(tj_ctx* ctx);
  * NaN stop.  Returns steps taken in *n_out and final time in *t_out. */
 int tj_run(tj_ctx* ctx, tj_mf* S, tj_mf* T, double final_time, int max_steps, int prec,
            int check_every, double* t_out, int* n_out);
+
+/* ---- The time loop without device allocation --------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tj_run on the level described by desc, with no context.
+ * Every buffer tj_bind_level would allocate (the clock words, box-edge flags,
+ * flux planes, THINC history planes) lives in one device array of
+ * TJ_STATIC_SCRATCH_BYTES that the library holds on each GPU it runs on until
+ * the process ends; the call allocates nothing, on the device or the host.
+ *   - S: the initial state (after tj_init_from_prims), advanced in place; T: a
+ *     distinct stage buffer; both caller-owned views of the bound layout.
+ *   - Work runs on `stream` (a hipStream_t, NULL for the null stream) on the
+ *     current device, with the fixed marching segment length 16; results are
+ *     bit-identical to tj_bind_level + tj_run (every segment length gives the
+ *     same values).
+ *   - Returns TJ_ESCRATCH, before touching the device, when the level needs
+ *     more than TJ_STATIC_SCRATCH_BYTES (about 330 bytes per cell for a 2D
+ *     fluid level with THINC in the mixed form: a 512 x 256 level fits).
+ *   - Every call on a GPU shares that GPU's array: calls that overlap on one
+ *     GPU (other host threads, other streams) race.  The history planes are
+ *     cleared at the start of each call. */
+#define TJ_STATIC_SCRATCH_BYTES (64ll << 20)
+int tj_run_static(const tj_level_desc* desc, tj_mf* S, tj_mf* T, double final_time, int max_steps, int prec,
+                  int check_every, void* stream, double* t_out, int* n_out);
 
 /* primitiveToConservative over the valid cells (boxaccesscellarray.cpp:124-174),
  * with the thermal-to-total pressure shift of initialise.cpp:104 when

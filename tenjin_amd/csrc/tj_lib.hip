@@ -180,9 +180,8 @@ const char* tj_last_error(const tj_ctx* c) { return c ? c->err.c_str() : "null c
 void* tj_stream(tj_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 // Validates a level description and fills the device-side Level (without
-// the edge-flag pointers); edges[a] receives the box-edge flags of
-// coordinate a (bit 0: first cell of a box, bit 1: last cell of a box).
-static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::vector<uint8_t> edges[3]) {
+// the edge-flag pointers).  Allocates nothing: the error is a static string.
+static int level_from_desc(const char*& err, const tj_level_desc* d, Level& lv) {
     if (!d) return TJ_EINVAL;
     if (d->dim < 1 || d->dim > 3 || d->nmat < 1 || d->nmat > TJ_MAX_MATERIALS) {
         err = "bind_level: bad dim or nmat";
@@ -291,6 +290,18 @@ static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::
     }
     build_layout(*d, lv.cs);
     lv.ncells = (long long)lv.n[0] * lv.n[1] * lv.n[2];
+    return TJ_OK;
+}
+
+// level_from_desc, and edges[a] receives the box-edge flags of coordinate a
+// (bit 0: first cell of a box, bit 1: last cell of a box).
+static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::vector<uint8_t> edges[3]) {
+    const char* e = nullptr;
+    const int r = level_from_desc(e, d, lv);
+    if (r) {
+        if (e) err = e;
+        return r;
+    }
     for (int a = 0; a < 3; ++a) {
         std::vector<uint8_t>& fl = edges[a];
         fl.assign(lv.n[a], 0);
@@ -1017,3 +1028,187 @@ int tj_synchronize(tj_ctx* c) {
 }  // extern "C"
 
 #include "tj_batch.h"
+
+// ---------------------------------------------------------------------------
+// tj_run_static: the time loop of tj_run for callers that cannot allocate
+// device memory.  Every buffer tj_bind_level allocates -- the clock words,
+// the box-edge flags, the flux planes (FP64, and FP32 for the mixed form) and
+// the THINC history planes -- is carved out of `scratch`, a file-scope device
+// array of TJ_STATIC_SCRATCH_BYTES.  Each GPU the library runs on holds its
+// own copy until the process ends.  Host code never uses the array's name as
+// a device address: the address comes from hipGetSymbolAddress, once per
+// device with that device current, and is kept per device.
+#include <mutex>
+
+namespace {
+
+__device__ __align__(256) unsigned char scratch[TJ_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_addr[kMaxDevices] = {};
+
+// The device address of the current device's scratch (looked up on the first
+// call on that device).
+int scratch_for_device(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kMaxDevices) return TJ_EINVAL;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_addr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TJ_EHIP;
+        scratch_addr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(scratch_addr[dev]);
+    return TJ_OK;
+}
+
+// Does v belong to {ratio (k sz + min(k, extra)) : klo <= k <= khi}, the
+// cumulative sizes of the first k boxes counted from the high end?
+__device__ __forceinline__ bool chop_sum(long long v, int ratio, int sz, int extra, int klo, int khi) {
+    if (v < 0 || v % ratio) return false;
+    const long long w = v / ratio;
+    const long long big = (long long)extra * (sz + 1);
+    long long k;
+    if (w <= big) {
+        if (w % (sz + 1)) return false;
+        k = w / (sz + 1);
+    } else {
+        if ((w - big) % sz) return false;
+        k = extra + (w - big) / sz;
+    }
+    return k >= klo && k <= khi;
+}
+
+// Box-edge flags of one coordinate (bit 0: first cell of a box, bit 1: last
+// cell) by the BoxArray::maxSize rule of chop_dim, evaluated per cell: boxes
+// k = 0, 1, ... from the high end hold (sz + 1) ratio cells while k < extra,
+// then sz ratio, and the last box starts at 0.
+__global__ void k_edge_flags(uint8_t* fl, int n, int chunk) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    uint8_t f = 0;
+    if (n <= chunk) {
+        f = (uint8_t)((i == 0 ? 1 : 0) | (i == n - 1 ? 2 : 0));
+    } else {
+        int ratio = 1, bs = chunk, nlen = n;
+        while (bs % 2 == 0 && nlen % 2 == 0) { ratio *= 2; bs /= 2; nlen /= 2; }
+        const int numblk = nlen / bs + (nlen % bs ? 1 : 0);
+        const int sz = nlen / numblk, extra = nlen % numblk;
+        if (chop_sum((long long)n - 1 - i, ratio, sz, extra, 0, numblk - 1)) f |= 2;  // a box's last cell
+        if (i == 0 || chop_sum((long long)n - i, ratio, sz, extra, 1, numblk - 1)) f |= 1;  // a box's first cell
+    }
+    fl[i] = f;
+}
+
+// Bump allocation of the scratch: 256-byte aligned pieces; with base == null
+// it only measures.
+struct Carve {
+    unsigned char* base;
+    size_t off = 0;
+    template <typename T>
+    T* take(long long count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += (size_t)count * sizeof(T);
+        return p;
+    }
+};
+
+// The buffers of tj_bind_level for a level, carved from `base` (or only
+// measured, base == null): returns the bytes used.
+size_t carve_level(tj_ctx* c, const tj_level_desc* d, unsigned char* base, int prec) {
+    Carve cv{base};
+    Level& lv = c->lv;
+    c->d_clk = cv.take<Clock>(1);
+    for (int a = 0; a < 3; ++a) {
+        c->d_edge[a] = cv.take<uint8_t>(lv.n[a]);
+        lv.edge[a] = c->d_edge[a];
+    }
+    c->nfc = d->SOLID ? 3 * d->nmat + 17 : 2 * d->nmat + 5 + (lv.mixm >= 0 ? 1 : 0);
+    for (int a = 0; a < d->dim; ++a) {
+        Faces& F = c->F[a];
+        F.nx = lv.n[0] + (a == 0);
+        F.ny = lv.n[1] + (a == 1);
+        F.nz = lv.n[2] + (a == 2);
+        F.cs = (long long)F.nx * F.ny * F.nz;
+        c->d_flux[a] = cv.take<double>(F.cs * c->nfc);
+        F.p = c->d_flux[a];
+        c->estride[a] = F.cs;
+        if (prec == TJ_PREC_MIXED) {
+            FacesT<float>& G = c->Ff[a];
+            G.nx = F.nx;
+            G.ny = F.ny;
+            G.nz = F.nz;
+            G.cs = F.cs;
+            c->d_fluxf[a] = cv.take<float>(F.cs * c->nfc);
+            G.p = c->d_fluxf[a];
+        }
+    }
+    if (d->THINC) {
+        const long long pl = lv.ncells * d->nmat;
+        c->d_urstar = cv.take<double>(pl);
+        for (int q = 0; q < 2; ++q) c->d_star[q] = cv.take<double>(pl);
+        for (int a = 0; a < d->dim; ++a)
+            for (int q = 0; q < 2; ++q) c->d_estar[a][q] = cv.take<double>(c->estride[a] * d->nmat);
+    }
+    return cv.off;
+}
+
+// The library-owned pointers of a context over the scratch are not its own:
+// clear them so nothing frees them.
+struct StaticCtx {
+    tj_ctx c;
+    ~StaticCtx() {
+        for (int a = 0; a < 3; ++a) {
+            c.d_edge[a] = nullptr;
+            c.d_flux[a] = nullptr;
+            c.d_fluxf[a] = nullptr;
+            c.d_estar[a][0] = c.d_estar[a][1] = nullptr;
+        }
+        c.d_urstar = c.d_star[0] = c.d_star[1] = nullptr;
+        c.d_clk = nullptr;
+        c.stream = nullptr;
+    }
+};
+
+}  // namespace
+
+extern "C" int tj_run_static(const tj_level_desc* desc, tj_mf* S, tj_mf* T, double final_time, int max_steps,
+                             int prec, int check_every, void* stream, double* t_out, int* n_out) {
+    if (!desc || !S || !T || !S->data || !T->data || S->data == T->data) return TJ_EINVAL;
+    if (prec != TJ_PREC_FP64 && prec != TJ_PREC_MIXED) return TJ_EINVAL;
+    StaticCtx sc;
+    tj_ctx* c = &sc.c;
+    const char* err = nullptr;
+    int r = level_from_desc(err, desc, c->lv);
+    if (r) return r;
+    c->desc = *desc;
+    c->nmat = desc->nmat;
+    if (carve_level(c, desc, nullptr, prec) > (size_t)TJ_STATIC_SCRATCH_BYTES) return TJ_ESCRATCH;
+    int ncomp_cs[NVAR];
+    const int ncomp = build_layout(*desc, ncomp_cs);
+    const Level& lv = c->lv;
+    for (const tj_mf* m : {S, T}) {  // the checks of check_mf, without its error text
+        if (m->ncomp < ncomp || m->jstride < lv.n[0] || m->kstride < m->jstride * lv.n[1] ||
+            m->nstride < m->kstride * lv.n[2])
+            return TJ_EINVAL;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return TJ_EHIP;
+    unsigned char* base = nullptr;
+    if ((r = scratch_for_device(dev, &base))) return r;
+    carve_level(c, desc, base, prec);
+    c->device = dev;
+    c->stream = static_cast<hipStream_t>(stream);
+    c->seglen = 16;  // fixed segment length: the per-pass pick keeps a host-side cache
+    c->bound = true;
+    for (int a = 0; a < 3; ++a) {
+        hipLaunchKernelGGL(k_edge_flags, dim3(grid1(lv.n[a], 256)), dim3(256), 0, c->stream, c->d_edge[a], lv.n[a],
+                           a < desc->dim ? desc->max_grid_size : lv.n[a]);
+        if (hipGetLastError() != hipSuccess) return TJ_EHIP;
+    }
+    // the history planes start at zero on every call (they hold the last
+    // call's contents otherwise), as after tj_bind_level
+    if ((r = tj_reset_history(c))) return r;
+    return tj_run(c, S, T, final_time, max_steps, prec, check_every, t_out, n_out);
+}
