@@ -39,6 +39,18 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell of each kernel (unique state reads + writes, FP64,
 # two materials; halo re-reads and THINC history reads not counted).
 FLUID_KERNEL_BYTES = {
+    # stage kernels: x kernel = 8 primitives + 6 further conserved comps in, the
+    # partially updated conserved state (8 comps) and 2 star alphas out; y kernel
+    # = 8 primitives and the partial state (8) in, 2 star alphas and 15 comps
+    # out, in stage 2 also 15 base comps in and the sound speed out
+    "xstage1": 112 + 64 + 16,
+    "xstage2": 112 + 64 + 16,
+    "ystage1": 64 + 64 + 16 + 120,
+    "ystage2_post": 64 + 64 + 16 + 120 + 120 + 8,
+    "xstage1_mixed": 112 + 64 + 16,
+    "xstage2_mixed": 112 + 64 + 16,
+    "ystage1_mixed": 64 + 64 + 16 + 120,
+    "ystage2_post_mixed": 64 + 64 + 16 + 120 + 120 + 8,
     "flux_x": 64 + 72 + 16,            # 8 primitives in, 9 flux comps out, 2 star alphas out
     "flux_y": 64 + 72 + 16,
     "flux_z": 64 + 72,
@@ -86,6 +98,11 @@ SOLID_KERNEL_BYTES = {
 }
 # 8 (3 N_s + 2 N_c): N_s/N_c = 14/8 (two-material fluid), 27/19 (two-material solid)
 # reactive2d: N_s = 14 + 4 (alpha rho lambda, lambda, two phase densities), N_c = 9
+# stage forms of tj_set_tuning's fuse argument, and the workloads whose levels
+# the stage kernels serve (2D fluid, no mixture)
+FORMS = {"stage": 2, "planes": 0, "march": 1}
+STAGE2D_WORKLOADS = ("inclusion2d", "explosion_rz")
+
 STEP_BYTES = {"inclusion2d": 464, "plate3d": 952, "tube1d": 464, "ensemble": 464, "explosion_rz": 464,
               "reactive2d": 8 * (3 * 18 + 2 * 9)}
 
@@ -110,8 +127,10 @@ def parse():
                     help="rows per marching segment: > 0 fixed; -1 the library's own pick per pass; 0 (default) the "
                          "fastest of the library's pick, 8 and 12, timed on a scratch copy of the level first, "
                          "around the middle of the run")
-    ap.add_argument("--fuse", action="store_true",
-                    help="run the update inside the y pass instead of as a kernel of its own")
+    ap.add_argument("--form", choices=["stage", "planes", "march"], default="stage",
+                    help="RK stage form of a 2D fluid level: stage = the x and y stage kernels with no flux planes "
+                         "(default; levels with a mixture use planes), planes = flux planes and an update kernel, "
+                         "march = flux planes with the update inside the y pass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calib-copy", action="store_true",
@@ -677,9 +696,11 @@ def main():
         """warm-up steps, then exactly args.steps timed steps of the device loop
         (segment length: --seglen, else the fastest of a few timed on a scratch
         copy of the level beforehand)"""
-        fuse = 1 if args.fuse else 0
+        fuse = FORMS[args.form]
         seglen = max(args.seglen, 0)
-        if args.seglen == 0:
+        if fuse == 2 and args.workload in STAGE2D_WORKLOADS:
+            tuned[prec] = {"seglen": "unused by the stage kernels"}
+        elif args.seglen == 0:
             # timed around the middle of the run (at most 250 loop bodies in)
             seglen, trial = autotune.tune_level_seglen(pb, prec, fuse,
                                                        at_step=min(args.warmup + args.steps // 2, 250))
@@ -807,7 +828,7 @@ def main():
             "workload": workload,
             "cells": ncell,
             "precision": "fp64" if prec == TJ_PREC_FP64 else "mixed",
-            "fused_update": bool(args.fuse),
+            "stage_form": args.form if args.workload in STAGE2D_WORKLOADS else "planes",
             "seglen": args.seglen if args.seglen else {("fp64" if k == TJ_PREC_FP64 else "mixed"): v
                                                         for k, v in tuned.items()},
             "parallelism": "single GPU",

@@ -132,11 +132,15 @@ int tj_bind_level(tj_ctx* ctx, const tj_level_desc* desc);
 /* Tuning: rows per marching segment of the y/z flux passes (seglen > 0
  * fixes it; 0, the default, picks it per pass from the level's shape and the
  * kernel's occupancy so that the segments fill the device's wave slots
- * evenly), and whether the update of a fluid level with dim >= 2 runs inside
- * its last directional pass (fuse = 1) or as a kernel of its own (fuse = 0,
- * default: faster on gfx950 while the fused pass is limited to one wave per
- * SIMD by its register use).  Every choice gives identical results; a value
- * < 0 leaves that setting unchanged. */
+ * evenly), and the form of an RK stage of a fluid level:
+ *   fuse = 2 (default): on two-dimensional levels without a mixture, the
+ *            stage kernels -- the x pass with the x part of the update, then
+ *            the y pass over LDS tiles with the rest of the update -- and no
+ *            flux planes; other levels as fuse = 0;
+ *   fuse = 1: flux planes, the update inside the last directional pass;
+ *   fuse = 0: flux planes and an update kernel of its own.
+ * Every choice gives identical results; a value < 0 leaves that setting
+ * unchanged. */
 int tj_set_tuning(tj_ctx* ctx, int seglen, int fuse);
 /* Number of components of the bound layout and the first component of a
  * reference Variable (structdefinitions.h:10-32 numbering); -1 if absent. */

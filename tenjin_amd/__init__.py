@@ -327,8 +327,10 @@ class GpuLevel:
         return s.value
 
     def set_tuning(self, seglen: int = -1, fuse: int = -1):
-        """Rows per marching segment (-1: unchanged) and fusion of the update
-        into the last directional pass (1 on, 0 off, -1 unchanged)."""
+        """Rows per marching segment (-1: unchanged) and the form of a stage of a
+        two-dimensional fluid level: 2 the stage kernels (default), 1 flux
+        planes with the update inside the last directional pass, 0 flux planes
+        and a separate update kernel (-1 unchanged)."""
         self._chk(self.lib.tj_set_tuning(self.ctx, seglen, fuse), "tj_set_tuning")
 
     def prof_enable(self, on: bool = True):

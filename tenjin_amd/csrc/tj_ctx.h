@@ -10,6 +10,7 @@
 #include "../../include/tenjin.h"
 #include "tj_kernels.h"
 #include "tj_solid_kernels.h"
+#include "tj_stage2d.h"
 
 using namespace tj;
 
@@ -42,6 +43,11 @@ struct tj_ctx {
     int ncu = 0;  // compute units of the device
     std::map<std::pair<const void*, int>, int> seg_cache;
     bool fuse = false;
+    // two-dimensional fluid levels without a mixture: each RK stage as the x
+    // and y stage kernels (tj_stage2d.h) over the partially updated state
+    // d_ux, with no flux planes (on by default; off: the flux-plane passes)
+    bool stage2d = true;
+    double* d_ux = nullptr;
     // a captured run of loop bodies (tj_graph_capture) and the THINC pass
     // counters it advances each launch
     hipGraph_t graph = nullptr;

@@ -122,6 +122,8 @@ void free_all(tj_ctx* c) {
     for (int q = 0; q < 2; ++q)
         if (c->d_star[q]) hipFree(c->d_star[q]);
     c->d_urstar = c->d_star[0] = c->d_star[1] = nullptr;
+    if (c->d_ux) hipFree(c->d_ux);
+    c->d_ux = nullptr;
     c->bound = false;
 }
 
@@ -316,6 +318,9 @@ static int make_level(std::string& err, const tj_level_desc* d, Level& lv, std::
     return TJ_OK;
 }
 
+// A level the stage kernels serve: two-dimensional fluid, no mixture.
+static bool stage2d_level(const tj_ctx* c) { return !c->lv.solid && c->lv.dim == 2 && c->lv.mixm < 0; }
+
 int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
     if (!c || !d) return TJ_EINVAL;
     Level lvn{};
@@ -347,6 +352,7 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
         F.p = c->d_flux[a];
         c->estride[a] = F.cs;
     }
+    if (stage2d_level(c)) HIPCHK(c, hipMalloc(&c->d_ux, sizeof(double) * lv.ncells * (2 * d->nmat + 4)));
     if (d->THINC) {
         const long long pl = lv.ncells * d->nmat;
         HIPCHK(c, hipMalloc(&c->d_urstar, sizeof(double) * pl));
@@ -735,6 +741,70 @@ static int launch_fluxes(tj_ctx* c, const MF& Uin, const FuseArgs* fz, bool* fus
     return TJ_OK;
 }
 
+// One RK stage of a two-dimensional fluid level by the stage kernels
+// (tj_stage2d.h): the x kernel, then the y kernel with the update's
+// completion (stage 2: RK average and the post-step ops).
+extern "C++" {
+template <bool TH, typename R>
+static void launch_stage2d_t(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops,
+                             const Hist& hx, const Hist& hy, const char* nx_name, const char* ny_name) {
+    constexpr int TC = 16;
+    const Level& lv = c->lv;
+    const UxPlanes ux{c->d_ux, lv.ncells};
+    {
+        Timed tm(c, nx_name);
+        constexpr int CPW = XGeom<TH>::CPW;
+        const long long waves = (long long)((lv.n[0] + CPW - 1) / CPW) * lv.n[1];
+        hipLaunchKernelGGL((k_xstage<2, TH, R>), dim3(grid1(waves, 4)), dim3(256), 0, c->stream, lv, in, ux, hx, c->d_clk);
+    }
+    if (TH) {
+        c->pass += 1;
+        c->dpass[0] += 1;
+    }
+    {
+        Timed tm(c, ny_name);
+        constexpr int CPW = XGeom<TH>::CPW;
+        const long long tiles = (long long)((lv.n[0] + TC - 1) / TC) * ((lv.n[1] + CPW - 1) / CPW);
+        if (stage == 1)
+            hipLaunchKernelGGL((k_ystage<2, TH, TC, 1, R>), dim3((unsigned)tiles), dim3(TC * 64), 0, c->stream, lv, in, ux,
+                               base, out, hy, c->d_clk, ops);
+        else
+            hipLaunchKernelGGL((k_ystage<2, TH, TC, 2, R>), dim3((unsigned)tiles), dim3(TC * 64), 0, c->stream, lv, in, ux,
+                               base, out, hy, c->d_clk, ops);
+    }
+    if (TH) {
+        c->pass += 1;
+        c->dpass[1] += 1;
+    }
+}
+}  // extern "C++"
+
+static int launch_stage2d(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops, int prec) {
+    const bool mixed = prec == TJ_PREC_MIXED;
+    static const char* xn[2][2] = {{"xstage1", "xstage2"}, {"xstage1_mixed", "xstage2_mixed"}};
+    static const char* yn[2][2] = {{"ystage1", "ystage2_post"}, {"ystage1_mixed", "ystage2_post_mixed"}};
+    const Hist hx = hist_for(c, 0);
+    // the y kernel's history: the planes after the x kernel's pass
+    const long long p0 = c->pass, d1 = c->dpass[1];
+    if (c->desc.THINC) c->pass += 1;
+    const Hist hy = hist_for(c, 1);
+    c->pass = p0;
+    c->dpass[1] = d1;
+    const char* a = xn[mixed][stage - 1];
+    const char* b = yn[mixed][stage - 1];
+    if (c->desc.THINC) {
+        if (mixed) launch_stage2d_t<true, float>(c, stage, in, base, out, ops, hx, hy, a, b);
+        else launch_stage2d_t<true, double>(c, stage, in, base, out, ops, hx, hy, a, b);
+    } else {
+        if (mixed) launch_stage2d_t<false, float>(c, stage, in, base, out, ops, hx, hy, a, b);
+        else launch_stage2d_t<false, double>(c, stage, in, base, out, ops, hx, hy, a, b);
+    }
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
+static bool use_stage2d(const tj_ctx* c) { return c->stage2d && c->d_ux && stage2d_level(c); }
+
 int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_mf* Uout, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
@@ -744,6 +814,9 @@ int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_m
     HIPCHK(c, hipSetDevice(c->device));
     const MF in = view(Uin);
     const MF base = stage == 2 ? view(Ubase) : in;
+    // the y kernel reads rows of Uin around each tile while other tiles write
+    // Uout: an in-place stage (Uout == Uin) takes the flux-plane passes
+    if (use_stage2d(c) && view(Uout).p != in.p) return launch_stage2d(c, stage, in, base, view(Uout), 0u, prec);
     const FuseArgs fz{stage, base, view(Uout), 0u};
     bool fused = false;
     if ((r = launch_fluxes(c, in, &fz, &fused, prec))) return r;
@@ -801,6 +874,13 @@ int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
     const MF s = view(S), t = view(T);
     if (s.p == t.p) { c->err = "step: S and T must be distinct buffers"; return TJ_EINVAL; }
     const bool mixed = prec == TJ_PREC_MIXED;
+    if (use_stage2d(c)) {
+        if ((r = launch_stage2d(c, 1, s, s, t, 0u, prec))) return r;
+        if ((r = launch_stage2d(c, 2, t, s, s, step_ops(c), prec))) return r;
+        hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, c->stream, c->d_clk);
+        HIPCHK(c, hipGetLastError());
+        return TJ_OK;
+    }
     bool fused = false;
     const FuseArgs f1{1, s, t, 0u};
     if ((r = launch_fluxes(c, s, &f1, &fused, prec))) return r;
@@ -992,7 +1072,10 @@ int tj_status(tj_ctx* c, uint32_t* status, int clear) {
 int tj_set_tuning(tj_ctx* c, int seglen, int fuse) {
     if (!c) return TJ_EINVAL;
     if (seglen >= 0) c->seglen = seglen;
-    if (fuse >= 0) c->fuse = fuse != 0;
+    if (fuse >= 0) {
+        c->stage2d = fuse == 2;
+        c->fuse = fuse == 1;
+    }
     return TJ_OK;
 }
 
@@ -1144,6 +1227,7 @@ size_t carve_level(tj_ctx* c, const tj_level_desc* d, unsigned char* base, int p
             G.p = c->d_fluxf[a];
         }
     }
+    if (stage2d_level(c)) c->d_ux = cv.take<double>(lv.ncells * (2 * d->nmat + 4));
     if (d->THINC) {
         const long long pl = lv.ncells * d->nmat;
         c->d_urstar = cv.take<double>(pl);
@@ -1166,6 +1250,7 @@ struct StaticCtx {
             c.d_estar[a][0] = c.d_estar[a][1] = nullptr;
         }
         c.d_urstar = c.d_star[0] = c.d_star[1] = nullptr;
+        c.d_ux = nullptr;
         c.d_clk = nullptr;
         c.stream = nullptr;
     }

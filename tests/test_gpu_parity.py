@@ -83,9 +83,11 @@ def test_shock_tube_1d_full_run():
     assert exact_fraction(pb, so, sg) == 1.0
 
 
-@pytest.mark.parametrize("thinc,fuse", [(0, 1), (1, 1), (0, 0), (1, 0)])
+@pytest.mark.parametrize("thinc,fuse", [(0, 1), (1, 1), (0, 0), (1, 0), (0, 2), (1, 2)])
 def test_inclusion_2d_stepwise(thinc, fuse):
-    """fuse=1: the update runs inside the y pass; fuse=0: separate update kernel."""
+    """fuse=2: the stage kernels (x kernel with the x update, y kernel over LDS
+    tiles with the rest); fuse=1: flux planes, the update inside the y pass;
+    fuse=0: flux planes and a separate update kernel."""
     from tenjin_amd import configs
 
     pb = configs.shock_inclusion_2d(nx=128, ny=64, box=32, thinc=thinc, shock_x=0.55, drift=(150.0, 60.0))
@@ -196,7 +198,7 @@ def test_plate_impact_3d_full_size_invariants():
     assert n == 3
 
 
-@pytest.mark.parametrize("fuse", [1, 0])
+@pytest.mark.parametrize("fuse", [2, 1, 0])
 def test_stage_api_matches_step(fuse):
     """tj_rk_stage(1), tj_rk_stage(2) and tj_post_step (cleanUpAlpha, NaN scan,
     getTimeStep) reproduce tj_step bit for bit, and both match the oracle."""
@@ -224,3 +226,33 @@ def test_stage_api_matches_step(fuse):
         b.sync()
         assert torch.equal(a.S, b.S), f"step {n}"
     assert a.status() == 0 and b.status() == 0
+
+
+@pytest.mark.parametrize("thinc,prec", [(1, 0), (1, 1), (0, 0), (0, 1)])
+def test_stage_kernels_match_flux_planes(thinc, prec):
+    """The stage kernels (the default for two-dimensional fluid levels) and the
+    flux-plane passes give bit-identical states in both precision forms, on
+    uneven boxes (maxSize 24 on 100 x 70), mixed boundaries and several tiles
+    of the y kernel, over 30 device-resident loop bodies."""
+    import torch
+
+    from tenjin_amd import GpuLevel, configs
+
+    pb = configs.shock_inclusion_2d(nx=100, ny=70, box=24, thinc=thinc, shock_x=0.55, drift=(-120.0, 90.0))
+    pb.bc_lo = (1, 0, 0)
+    pb.bc_hi = (0, 1, 0)
+    print(f"\n[case] stage kernels vs flux planes thinc={thinc} prec={prec}", flush=True)
+    a, b = GpuLevel(pb), GpuLevel(pb)
+    a.set_tuning(-1, 2)
+    b.set_tuning(-1, 0)
+    for g in (a, b):
+        g.init_from_prims(pb.prims)
+        g.begin_run(float("inf"))
+    for n in range(3):
+        a.advance(10, prec)
+        b.advance(10, prec)
+        a.sync()
+        b.sync()
+        assert torch.equal(a.S, b.S), f"after {10 * (n + 1)} steps"
+    assert a.status() == 0 and b.status() == 0
+    assert a.clock() == b.clock()
