@@ -34,6 +34,7 @@ struct tj_ctx {
     double* d_estar[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     long long estride[3] = {0, 0, 0};
     Clock* d_clk = nullptr;
+    unsigned long long* d_dtpart = nullptr;  // DT_PARTS partial slots of the CFL minimum
     unsigned* d_status = nullptr;
     long long pass = 0;
     long long dpass[3] = {0, 0, 0};

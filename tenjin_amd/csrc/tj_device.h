@@ -55,6 +55,8 @@ struct Level {
     // materials with derivative parts (MatT<Dual<K>>[MAXM], device memory) for
     // the tangent-linear form; null elsewhere
     const void* md;
+    // partial slots of the CFL minimum (DT_PARTS words; null: the clock's word)
+    unsigned long long* dtpart;
 };
 
 template <typename R>

@@ -14,6 +14,16 @@ struct Clock {
     unsigned int status;
 };
 
+// Partial slots of the CFL minimum (Level::dtpart): kernels with many blocks
+// spread their per-block atomics over DT_PARTS words (block b on word b mod
+// DT_PARTS), so that they do not all contend for the clock's one word;
+// k_clock_dt folds them in.  Levels without slots use the clock's word.
+constexpr int DT_PARTS = 256;
+constexpr unsigned long long DT_NONE = 0x7FEFFFFFFFFFFFFFull;  // std::numeric_limits<double>::max()
+__device__ __forceinline__ unsigned long long* dt_slot(const Level& lv, Clock* clk) {
+    return lv.dtpart ? lv.dtpart + (blockIdx.x % DT_PARTS) : &clk->dtmin_bits;
+}
+
 // ------------------------------------------------------------------ x pass
 // One wave covers a row segment; lane l holds cell c = f0 - H + l.  Lanes
 // reconstruct their cell (at the boundary-mapped cell s), neighbours' data
@@ -680,7 +690,7 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
             finish_row(nd - 1, Flo, Ftop);
         }
     }
-    if (FUSE && STAGE == 2 && (ops & 8u)) wave_min_to(cand, &clk->dtmin_bits);
+    if (FUSE && STAGE == 2 && (ops & 8u)) wave_min_to(cand, dt_slot(lv, clk));
     if (st) atomicOr(&clk->status, st);
 }
 template <int NM, bool TH, int D, bool FUSE = false, int STAGE = 1, typename R = double, bool MX = false>
@@ -735,7 +745,7 @@ __device__ __forceinline__ void update_body(const Level& lv, const MF& Uin, cons
         }
         finish_cell<NM, STAGE, MX>(lv, Ub, Uo, i, j, k, al, ar, ru, E, dt, ops, cand, st, arl);
     }
-    if (STAGE == 2 && (ops & 8u)) block_min_to(cand, &clk->dtmin_bits);
+    if (STAGE == 2 && (ops & 8u)) block_min_to(cand, dt_slot(lv, clk));
     if (st) atomicOr(&clk->status, st);
 }
 template <int NM, int STAGE, typename T = double, bool MX = false>
@@ -779,7 +789,7 @@ __device__ __forceinline__ void sound_dt_body(const Level& lv, const MF& U, Cloc
         for (int d = 0; d < lv.dim; ++d)
             cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(U.at(lv.cs[VELOCITY] + d, i, j, k))));
     }
-    block_min_to(cand, &clk->dtmin_bits);
+    block_min_to(cand, dt_slot(lv, clk));
 }
 template <int NM, bool MX = false>
 __global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
@@ -788,9 +798,25 @@ __global__ __launch_bounds__(256) void k_sound_dt(Level lv, MF U, Clock* clk) {
 
 #ifndef TJ_TEMPLATE_KERNELS_ONLY  // non-template kernels: one definition, in tj_lib.hip
 // dt = CFL * min (timestep.cpp:61-69); with `driver` the loop test and the
-// clamps of main.cpp:121, 135-143.  Re-arms the min word.
-__global__ void k_clock_dt(Clock* clk, double cfl, int driver) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// clamps of main.cpp:121, 135-143.  Re-arms the min word and, with `part`
+// (DT_PARTS words, launched with one wave), first folds the partial slots
+// into it and re-arms them.
+__global__ void k_clock_dt(Clock* clk, double cfl, int driver, unsigned long long* part) {
+    if (blockIdx.x != 0) return;
+    if (part) {
+        unsigned long long b = DT_NONE;
+        for (int q = (int)threadIdx.x; q < DT_PARTS; q += (int)blockDim.x) {
+            const unsigned long long v = part[q];
+            b = v < b ? v : b;
+            part[q] = DT_NONE;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(b, off, 64);
+            b = o < b ? o : b;
+        }
+        if (threadIdx.x == 0 && b < clk->dtmin_bits) clk->dtmin_bits = b;
+    }
+    if (threadIdx.x != 0) return;
     if (driver) {
         if (clk->done) return;
         if (!(clk->t < clk->final_time)) {
@@ -805,7 +831,11 @@ __global__ void k_clock_dt(Clock* clk, double cfl, int driver) {
         if (clk->n < 5) dt *= 0.2;
     }
     clk->dt = dt;
-    clk->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;  // std::numeric_limits<double>::max()
+    clk->dtmin_bits = DT_NONE;
+}
+// fill of the partial CFL slots with the empty value
+__global__ void k_dtpart_reset(unsigned long long* part) {
+    for (int q = (int)threadIdx.x; q < DT_PARTS; q += (int)blockDim.x) part[q] = DT_NONE;
 }
 // end of a loop iteration: the NaN stop of main.cpp:183-187 (before t, n
 // advance), else t += dt, n++ (main.cpp:121).

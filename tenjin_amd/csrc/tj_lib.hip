@@ -154,6 +154,9 @@ int tj_create(tj_ctx** out, int device) {
     init.final_time = __builtin_huge_val();
     init.dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
     if (hipMemcpy(c->d_clk, &init, sizeof(Clock), hipMemcpyHostToDevice) != hipSuccess) { delete c; return TJ_EHIP; }
+    if (hipMalloc(&c->d_dtpart, sizeof(unsigned long long) * DT_PARTS) != hipSuccess) { delete c; return TJ_EHIP; }
+    hipLaunchKernelGGL(k_dtpart_reset, dim3(1), dim3(64), 0, c->stream, c->d_dtpart);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { delete c; return TJ_EHIP; }
     *out = c;
     return TJ_OK;
 }
@@ -174,6 +177,7 @@ void tj_destroy(tj_ctx* c) {
     prof_drain(c);
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     if (c->d_clk) hipFree(c->d_clk);
+    if (c->d_dtpart) hipFree(c->d_dtpart);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -334,6 +338,7 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
     c->desc = *d;
     Level& lv = c->lv;
     lv = lvn;
+    lv.dtpart = c->d_dtpart;
     c->nmat = d->nmat;
     for (int a = 0; a < 3; ++a) {
         const std::vector<uint8_t>& fl = edges[a];
@@ -465,7 +470,7 @@ int tj_compute_dt(tj_ctx* c, const tj_mf* U, double* dt_out) {
     if (r) return r;
     HIPCHK(c, hipSetDevice(c->device));
     launch_sound_dt(c, view(U));
-    hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 0);
+    hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 0, c->d_dtpart);
     HIPCHK(c, hipGetLastError());
     if (dt_out) {
         Clock h;
@@ -491,6 +496,8 @@ int tj_set_dt(tj_ctx* c, double dt, int reset_clock) {
         h.dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
     }
     HIPCHK(c, hipMemcpyAsync(c->d_clk, &h, sizeof(Clock), hipMemcpyHostToDevice, c->stream));
+    if (reset_clock && c->d_dtpart) hipLaunchKernelGGL(k_dtpart_reset, dim3(1), dim3(64), 0, c->stream, c->d_dtpart);
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return TJ_OK;
 }
@@ -509,7 +516,7 @@ int tj_get_clock(tj_ctx* c, double* t, double* dt, int* n) {
 int tj_driver_dt(tj_ctx* c, double final_time) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     hipLaunchKernelGGL(k_clock_final, dim3(1), dim3(64), 0, c->stream, c->d_clk, final_time);
-    hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 1);
+    hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 1, c->d_dtpart);
     HIPCHK(c, hipGetLastError());
     return TJ_OK;
 }
@@ -745,10 +752,16 @@ static int launch_fluxes(tj_ctx* c, const MF& Uin, const FuseArgs* fz, bool* fus
 // (tj_stage2d.h): the x kernel, then the y kernel with the update's
 // completion (stage 2: RK average and the post-step ops).
 extern "C++" {
+#ifndef TJ_STAGE_TC
+#define TJ_STAGE_TC 16
+#endif
+#ifndef TJ_STAGE_LPS
+#define TJ_STAGE_LPS 32
+#endif
 template <bool TH, typename R>
 static void launch_stage2d_t(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops,
                              const Hist& hx, const Hist& hy, const char* nx_name, const char* ny_name) {
-    constexpr int TC = 16;
+    constexpr int TC = TJ_STAGE_TC;
     const Level& lv = c->lv;
     const UxPlanes ux{c->d_ux, lv.ncells};
     {
@@ -763,14 +776,15 @@ static void launch_stage2d_t(tj_ctx* c, int stage, const MF& in, const MF& base,
     }
     {
         Timed tm(c, ny_name);
-        constexpr int CPW = XGeom<TH>::CPW;
+        constexpr int LPS = TJ_STAGE_LPS;
+        constexpr int CPW = YTile<2, TH, TC, LPS, R>::CPW;
         const long long tiles = (long long)((lv.n[0] + TC - 1) / TC) * ((lv.n[1] + CPW - 1) / CPW);
         if (stage == 1)
-            hipLaunchKernelGGL((k_ystage<2, TH, TC, 1, R>), dim3((unsigned)tiles), dim3(TC * 64), 0, c->stream, lv, in, ux,
-                               base, out, hy, c->d_clk, ops);
+            hipLaunchKernelGGL((k_ystage<2, TH, TC, LPS, 1, R>), dim3((unsigned)tiles), dim3(TC * LPS), 0, c->stream, lv,
+                               in, ux, base, out, hy, c->d_clk, ops);
         else
-            hipLaunchKernelGGL((k_ystage<2, TH, TC, 2, R>), dim3((unsigned)tiles), dim3(TC * 64), 0, c->stream, lv, in, ux,
-                               base, out, hy, c->d_clk, ops);
+            hipLaunchKernelGGL((k_ystage<2, TH, TC, LPS, 2, R>), dim3((unsigned)tiles), dim3(TC * LPS), 0, c->stream, lv,
+                               in, ux, base, out, hy, c->d_clk, ops);
     }
     if (TH) {
         c->pass += 1;
@@ -915,7 +929,7 @@ int tj_advance_steps(tj_ctx* c, tj_mf* S, tj_mf* T, int nsteps, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
     for (int q = 0; q < nsteps; ++q) {
-        hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 1);
+        hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 1, c->d_dtpart);
         if ((r = tj_step(c, S, T, prec))) return r;
     }
     return TJ_OK;
@@ -1203,6 +1217,8 @@ size_t carve_level(tj_ctx* c, const tj_level_desc* d, unsigned char* base, int p
     Carve cv{base};
     Level& lv = c->lv;
     c->d_clk = cv.take<Clock>(1);
+    c->d_dtpart = cv.take<unsigned long long>(DT_PARTS);
+    lv.dtpart = c->d_dtpart;
     for (int a = 0; a < 3; ++a) {
         c->d_edge[a] = cv.take<uint8_t>(lv.n[a]);
         lv.edge[a] = c->d_edge[a];
@@ -1252,6 +1268,7 @@ struct StaticCtx {
         c.d_urstar = c.d_star[0] = c.d_star[1] = nullptr;
         c.d_ux = nullptr;
         c.d_clk = nullptr;
+        c.d_dtpart = nullptr;
         c.stream = nullptr;
     }
 };

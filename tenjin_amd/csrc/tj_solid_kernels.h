@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
         store_scell<NM, STAGE == 2>(lv, Uo, i, j, k, C);
     }
     if (STAGE == 2 && (ops & 8u)) {
-        block_min_to(cand, &clk->dtmin_bits);
+        block_min_to(cand, dt_slot(lv, clk));
     }
     if (st) atomicOr(&clk->status, st);
 }
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void k_sound_dt_s(Level lv, MF U, Clock* clk) 
         for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
             if (d < lv.dim) cand = dmin(cand, lv.dx[d] / (a + __builtin_fabs(C.u[d])));
     }
-    block_min_to(cand, &clk->dtmin_bits);
+    block_min_to(cand, dt_slot(lv, clk));
 }
 
 // setInitialConditions for solid levels: thermal -> total pressure shift
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsi
         store_scell<NM>(lv, U, i, j, k, C);
     }
     if (ops & 8u) {
-        block_min_to(cand, &clk->dtmin_bits);
+        block_min_to(cand, dt_slot(lv, clk));
     }
     if (st) atomicOr(&clk->status, st);
 }

@@ -203,16 +203,22 @@ void k_xstage(Level lv, MF U, UxPlanes Ux, Hist h, Clock* clk) {
 }
 
 // ------------------------------------------------------------------ y kernel
-// A block of TC waves owns columns i0 .. i0 + TC - 1 and, along y, the wave
-// geometry of the x kernel: lane l holds row r = s0 - H + l, faces s0 .. s0 +
-// CPW, cells s0 .. s0 + CPW - 1.
-template <int NM, bool TH, int TC, typename R>
+// A block owns columns i0 .. i0 + TC - 1 and a segment of rows.  Lanes run
+// along y in segments of LPS lanes (64: one column per wave; 32: two columns
+// per wave, which halves the block and its LDS so that two blocks share a
+// compute unit and one's loads overlap the other's arithmetic): lane l of a
+// segment holds row r = s0 - H + l, the segment computes faces s0 .. s0 + CPW
+// and completes cells s0 .. s0 + CPW - 1.
+template <int NM, bool TH, int TC, int LPS, typename R>
 struct YTile {
-    static constexpr int H = XGeom<TH>::H, CPW = XGeom<TH>::CPW;
+    static constexpr int H = TH ? 2 : 1;
+    static constexpr int CPW = LPS - 2 * H;    // rows completed per tile
     static constexpr int NP = 2 * NM + 4;      // staged primitives: alpha, rho_k, u (3), p
-    static constexpr int ROWS = 66;            // rows s0 - H - 1 .. s0 - H + 64
+    static constexpr int ROWS = LPS + 2;       // rows s0 - H - 1 .. s0 - H + LPS
     static constexpr int NF = 2 * NM + 5;      // flux components per face
     static constexpr int FR = CPW + 1;         // faces per column
+    static constexpr int NT = TC * LPS;        // threads per block
+    static constexpr int NW = NT / 64;         // waves per block
     static constexpr int PRIMS = NP * TC * ROWS + (TH ? NM * 2 * ROWS : 0);
     static constexpr int FLUX = NF * TC * FR;
     static constexpr int WORDS = PRIMS > FLUX ? PRIMS : FLUX;
@@ -233,13 +239,13 @@ __device__ __forceinline__ void block_min_to_n(double cand, unsigned long long* 
     }
 }
 
-template <int NM, bool TH, int TC, int STAGE, typename R>
+template <int NM, bool TH, int TC, int LPS, int STAGE, typename R>
 __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const UxPlanes& Ux, const MF& Ub,
                                             const MF& Uo, const Hist& h, Clock* clk, unsigned ops, long long tile) {
-    using T = YTile<NM, TH, TC, R>;
-    constexpr int H = T::H, CPW = T::CPW, NP = T::NP, ROWS = T::ROWS, NF = T::NF, FR = T::FR;
+    using T = YTile<NM, TH, TC, LPS, R>;
+    constexpr int H = T::H, CPW = T::CPW, NP = T::NP, ROWS = T::ROWS, NF = T::NF, FR = T::FR, NT = T::NT;
     __shared__ R lds[T::WORDS];
-    __shared__ unsigned long long red[TC];
+    __shared__ unsigned long long red[T::NW];
     if (clk->done) return;  // uniform: the clock does not change during the launch
     const int nx = lv.n[0], ny = lv.n[1];
     const int ntc = (nx + TC - 1) / TC;
@@ -255,7 +261,7 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
     auto FL = [&](int q, int col, int f) -> R& { return lds[(q * TC + col) * FR + f]; };
 
     // ---- stage the tile's primitives (load_prim of every LDS row and column)
-    for (int e = tid; e < ROWS * TC; e += TC * 64) {
+    for (int e = tid; e < ROWS * TC; e += NT) {
         const int col = e % TC, t = e / TC;
         Prim<NM, R> Pq;
         load_prim<NM, R>(lv, U, i0 + col, base + t, 0, Pq);
@@ -269,7 +275,7 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
         P(2 * NM + 3, col, t) = Pq.p;
     }
     if (TH) {
-        for (int e = tid; e < ROWS * 2; e += TC * 64) {
+        for (int e = tid; e < ROWS * 2; e += NT) {
             const int side = e & 1, t = e >> 1;
 #pragma unroll
             for (int m = 0; m < NM; ++m) AX(m, side, t) = load_alpha<R>(lv, U, m, side ? i0 + TC : i0 - 1, base + t, 0);
@@ -277,11 +283,14 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
     }
     __syncthreads();
 
-    // ---- one column per wave, lanes along y
-    const int lane = tid & 63, col = tid >> 6;
+    // ---- one column per segment of LPS lanes, lanes along y
+    const int lane = tid & 63;
+    const int sl = lane % LPS;                 // lane within the segment
+    const int lo = lane - sl, hi = lo + LPS - 1;  // the segment's lanes
+    const int col = (tid >> 6) * (64 / LPS) + lane / LPS;
     const int i = i0 + col;
     const bool colok = i < nx;
-    const int r = s0 - H + lane;
+    const int r = s0 - H + sl;
     bool fl;
     const int s = bcmap(r, ny, lv.bc_lo[1], lv.bc_hi[1], fl);
     const int lofs = s - r;
@@ -318,7 +327,7 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
 
     unsigned st = 0;
     const long long cell_s = (long long)s * nx + i;
-    const bool own = colok && lane >= H && r >= s0 && r < s0 + CPW && r < ny;
+    const bool own = colok && sl >= H && r >= s0 && r < s0 + CPW && r < ny;
     if (TH) {
         // arec (thinc.cpp:54-88, 130-186) of cell (i, s) along y from the staged alphas
         ARec<NM, R> A;
@@ -357,8 +366,8 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
         const uint8_t eflag = lv.edge[1][s];
         const bool blo = eflag & 1, bhi = eflag & 2;
         int srcm = lane + lofs - 1, srcp = lane + lofs + 1;
-        srcm = srcm < 0 ? 0 : (srcm > 63 ? 63 : srcm);
-        srcp = srcp < 0 ? 0 : (srcp > 63 ? 63 : srcp);
+        srcm = srcm < lo ? lo : (srcm > hi ? hi : srcm);
+        srcp = srcp < lo ? lo : (srcp > hi ? hi : srcp);
         const int mixm = __shfl(A.mixed, srcm, 64);
         const int mixp = __shfl(A.mixed, srcp, 64);
         const int mixedFlag = (A.mixed >> (NM - 1)) & 1;
@@ -407,7 +416,7 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
     p2c_sound<NM, R>(lv, PL, SLs);
     p2c_sound<NM, R>(lv, PR, SRs);
     FSt<NM, R> Lf;
-    const int lm = lane > 0 ? lane - 1 : 0;
+    const int lm = lane > lo ? lane - 1 : lo;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         Lf.al[m] = __shfl(SRs.al[m], lm, 64);
@@ -420,7 +429,7 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
     Lf.p = __shfl(SRs.p, lm, 64);
     Lf.a = __shfl(SRs.a, lm, 64);
 
-    const bool face = colok && lane >= H && lane <= H + CPW && r >= 0 && r <= ny;
+    const bool face = colok && sl >= H && sl <= H + CPW && r >= 0 && r <= ny;
     R Fv[NF];
 #pragma unroll
     for (int q = 0; q < NF; ++q) Fv[q] = R(0);
@@ -445,16 +454,16 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
         }
     }
     __syncthreads();  // every wave has read its primitives: the flux staging may overwrite them
-    if (lane >= H && lane <= H + CPW) {
+    if (sl >= H && sl <= H + CPW) {
 #pragma unroll
-        for (int q = 0; q < NF; ++q) FL(q, col, lane - H) = Fv[q];
+        for (int q = 0; q < NF; ++q) FL(q, col, sl - H) = Fv[q];
     }
     __syncthreads();
 
     // ---- y part of update and completion, lanes along x
     double cand = __builtin_huge_val();
     const double dt = clk->dt;
-    for (int e = tid; e < CPW * TC; e += TC * 64) {
+    for (int e = tid; e < CPW * TC; e += NT) {
         const int cc = e % TC, f = e / TC;
         const int ii = i0 + cc, rr = s0 + f;
         if (ii < nx && rr < ny) {
@@ -479,14 +488,14 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
             finish_cell<NM, STAGE>(lv, Ub, Uo, ii, rr, 0, al, ar, ru, E, dt, ops, cand, st);
         }
     }
-    if (STAGE == 2 && (ops & 8u)) block_min_to_n<TC>(cand, &clk->dtmin_bits, red);
+    if (STAGE == 2 && (ops & 8u)) block_min_to_n<T::NW>(cand, dt_slot(lv, clk), red);
     if (st) atomicOr(&clk->status, st);
 }
 
-template <int NM, bool TH, int TC, int STAGE, typename R = double>
-__global__ __launch_bounds__(TC * 64) void k_ystage(Level lv, MF U, UxPlanes Ux, MF Ub, MF Uo, Hist h, Clock* clk,
-                                                    unsigned ops) {
-    ystage_body<NM, TH, TC, STAGE, R>(lv, U, Ux, Ub, Uo, h, clk, ops, blockIdx.x);
+template <int NM, bool TH, int TC, int LPS, int STAGE, typename R = double>
+__global__ __launch_bounds__(TC * LPS) void k_ystage(Level lv, MF U, UxPlanes Ux, MF Ub, MF Uo, Hist h, Clock* clk,
+                                                     unsigned ops) {
+    ystage_body<NM, TH, TC, LPS, STAGE, R>(lv, U, Ux, Ub, Uo, h, clk, ops, blockIdx.x);
 }
 
 }  // namespace tj

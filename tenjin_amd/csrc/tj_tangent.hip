@@ -118,6 +118,7 @@ int run_group(tj_tangent* tg, int first, int cnt, const tj_mf* P0, int shift, tj
     fill_mats<K>(tg, first, cnt, hm);
     HIPCHK(c, hipMemcpyAsync(tg->d_mat, hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
     Level lv = lv0;
+    lv.dtpart = nullptr;  // the tangent kernels reduce into the clock's word (k_tclock_dt reads it alone)
     lv.md = tg->d_mat;
     const long long bs = (long long)tg->ncomp * nc;
     DMF<K> dS{view(S), MF{tg->d_tan[0], nx, (long long)nx * ny, nc}, bs};
