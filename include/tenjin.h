@@ -29,7 +29,10 @@ extern "C" {
 
 /* EOS kinds: 0 = MieGruneisenEOS of equationofstate.cpp:5-115 (stiffened gas,
  * constant-Gamma Mie-Gruneisen with pref/eref); 1 = Us-up Mie-Gruneisen solid
- * (no reference class; solid-EOS interface of equationofstate.cpp:502-596). */
+ * (no reference class; solid-EOS interface of equationofstate.cpp:502-596);
+ * 2 = RomenskiiSolidEOS (equationofstate.cpp:403-495: rho0, K0, eos_alpha,
+ * eos_beta, G0); 3 = WilkinsSolidEOS (equationofstate.cpp:502-596: rho0,
+ * e1 .. e5, G0).  Kinds 1-3 are solids (solid = 1) on SOLID levels. */
 typedef struct tj_material {
     int eos;
     int solid;    /* MaterialDescriptor::phase == solid (structdefinitions.h:90-98) */
@@ -43,6 +46,8 @@ typedef struct tj_material {
     double yield_stress; /* PlasticEOS::yieldStress[m] (plastic.h:21) */
     double gamma_mix, pref_mix, eref_mix, CV_mix; /* the products' EOS (adiabaticIndexmix, prefmix,
                                                      erefmix, CVmix of initialise.cpp:189-192) */
+    double K0, eos_alpha, eos_beta;               /* RomenskiiSolidEOS (K0, EOSalpha, EOSbeta) */
+    double e1, e2, e3, e4, e5;                    /* WilkinsSolidEOS */
 } tj_material;
 
 /* Level description: what ParameterStruct (structdefinitions.h:105-144),

@@ -50,6 +50,14 @@ class Params(ctypes.Structure):
         ("pref2", ctypes.c_double * 4),
         ("eref2", ctypes.c_double * 4),
         ("CV2", ctypes.c_double * 4),
+        ("K0", ctypes.c_double * 4),
+        ("ealpha", ctypes.c_double * 4),
+        ("ebeta", ctypes.c_double * 4),
+        ("e1", ctypes.c_double * 4),
+        ("e2", ctypes.c_double * 4),
+        ("e3", ctypes.c_double * 4),
+        ("e4", ctypes.c_double * 4),
+        ("e5", ctypes.c_double * 4),
     ]
 
 
@@ -136,6 +144,8 @@ def make_params(pb, math_mode: int = 0) -> Params:
         q.rho0[m], q.c0[m], q.s[m], q.G0[m], q.yield_[m] = M.rho0, M.c0, M.s, M.G0, M.yield_stress
         q.mixture[m] = M.mixture
         q.gamma2[m], q.pref2[m], q.eref2[m], q.CV2[m] = M.gamma2, M.pref2, M.eref2, M.CV2
+        q.K0[m], q.ealpha[m], q.ebeta[m] = M.K0, M.eos_alpha, M.eos_beta
+        q.e1[m], q.e2[m], q.e3[m], q.e4[m], q.e5[m] = M.e1, M.e2, M.e3, M.e4, M.e5
     return q
 
 

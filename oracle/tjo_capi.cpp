@@ -31,6 +31,7 @@ struct tjo_params {
     int REACTIVE;
     int mixture[4];
     double gamma2[4], pref2[4], eref2[4], CV2[4];
+    double K0[4], ealpha[4], ebeta[4], e1[4], e2[4], e3[4], e4[4], e5[4];
 };
 
 static tjo::Params to_params(const tjo_params* q) {
@@ -73,6 +74,14 @@ static tjo::Params to_params(const tjo_params* q) {
         M.pref2 = q->pref2[m];
         M.eref2 = q->eref2[m];
         M.CV2 = q->CV2[m];
+        M.K0 = q->K0[m];
+        M.EOSalpha = q->ealpha[m];
+        M.EOSbeta = q->ebeta[m];
+        M.e1 = q->e1[m];
+        M.e2 = q->e2[m];
+        M.e3 = q->e3[m];
+        M.e4 = q->e4[m];
+        M.e5 = q->e5[m];
         if (m < p.nmat && M.mixture) p.nmix = 1;
     }
     return p;

@@ -64,6 +64,9 @@ struct Material {
     // EOS above (reactant a), `second` the one below (products b)
     int mixture = 0;
     double adiabaticIndex2 = 1.4, pref2 = 0.0, eref2 = 0.0, CV2 = 1.0;
+    // eos 2: RomenskiiSolidEOS (equationofstate.cpp:403-495), eos 3:
+    // WilkinsSolidEOS (:502-596), both with rho0 and G0 above
+    double K0 = 0.0, EOSalpha = 0.0, EOSbeta = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
     double GG() const { return adiabaticIndex - 1.0; }
     double GG2() const { return adiabaticIndex2 - 1.0; }
 };
@@ -395,9 +398,36 @@ class Solver {
         double detadrho = M.rho0 / (rho * rho);
         return (dpHdrho(M, rho) * eta + pH(M, rho) * detadrho) / (2.0 * M.rho0);
     }
+    // RomenskiiSolidEOS / WilkinsSolidEOS terms (equationofstate.cpp:420-495, 519-596);
+    // std::pow and std::log as the project's routines (D5)
+    double romG(const Material& M, double rk) const { return M.G0 * mth.pow((rk / M.rho0), M.EOSbeta + 1.0); }
+    double solidDGdrho(const Material& M, double rk) const {
+        if (M.eos == 2) return M.G0 * (M.EOSbeta + 1.0) * mth.pow((rk / M.rho0), M.EOSbeta) / M.rho0;
+        return 0.0;
+    }
+    double solidDG2drho2(const Material& M, double rk) const {
+        if (M.eos == 2) return M.G0 * (M.EOSbeta + 1.0) * (M.EOSbeta) * mth.pow((rk / M.rho0), M.EOSbeta - 1.0) / (M.rho0 * M.rho0);
+        return 0.0;
+    }
+    double solidDpcdrho(const Material& M, double rk) const {
+        const double x = rk / M.rho0;
+        if (M.eos == 2)
+            return (M.K0 / (M.EOSalpha * M.rho0)) * mth.pow(x, M.EOSalpha) *
+                   ((2.0 * M.EOSalpha + 1.0) * mth.pow(x, M.EOSalpha) - (M.EOSalpha + 1.0));
+        return (6.0 * M.e2 * x * x + 2.0 * M.e3 * x - M.e5) / M.rho0;
+    }
     double coldE(Fab& f, int i, int j, int k, int m) {
         const Material& M = p.mat[m];
         if (M.eos == 1) return A(f, i, j, k, ALPHARHO, m) * eH(M, A(f, i, j, k, RHO_K, m));
+        if (M.eos == 2) {
+            const double x = A(f, i, j, k, RHO_K, m) / M.rho0;
+            return A(f, i, j, k, ALPHARHO, m) * ((M.K0 / (2.0 * M.rho0 * M.EOSalpha * M.EOSalpha)) *
+                                                 (mth.pow(x, M.EOSalpha) - 1.0) * (mth.pow(x, M.EOSalpha) - 1.0));
+        }
+        if (M.eos == 3) {
+            const double x = A(f, i, j, k, RHO_K, m) / M.rho0;
+            return A(f, i, j, k, ALPHARHO, m) * ((M.e1 + M.e2 * x * x + M.e3 * x + M.e4 / x - M.e5 * mth.log(x)) / M.rho0);
+        }
         if (M.mixture) {  // equationofstate.cpp:158-162
             const double lam = A(f, i, j, k, LAMBDA, m);
             return lam * (M.eref * A(f, i, j, k, ALPHARHO, m)) + (1.0 - lam) * (M.eref2 * A(f, i, j, k, ALPHARHO, m));
@@ -407,6 +437,15 @@ class Solver {
     double coldP(Fab& f, int i, int j, int k, int m) {
         const Material& M = p.mat[m];
         if (M.eos == 1) return (A(f, i, j, k, ALPHA, m) / M.GG()) * pH(M, A(f, i, j, k, RHO_K, m));
+        if (M.eos == 2) {
+            const double x = A(f, i, j, k, RHO_K, m) / M.rho0;
+            return (A(f, i, j, k, ALPHA, m) / M.GG()) *
+                   ((M.K0 / M.EOSalpha) * mth.pow(x, M.EOSalpha + 1.0) * (mth.pow(x, M.EOSalpha) - 1.0));
+        }
+        if (M.eos == 3) {
+            const double x = A(f, i, j, k, RHO_K, m) / M.rho0;
+            return (A(f, i, j, k, ALPHA, m) / M.GG()) * ((2.0 * M.e2 * x * x * x + M.e3 * x * x - M.e4 - M.e5 * x));
+        }
         if (M.mixture) {  // equationofstate.cpp:164-168
             const double lam = A(f, i, j, k, LAMBDA, m), rk = A(f, i, j, k, RHO_K, m);
             const double pa = M.pref * A(f, i, j, k, ALPHA, m) / M.GG();
@@ -422,18 +461,24 @@ class Solver {
         const double lam = A(f, i, j, k, LAMBDA, m);
         return (lam * M.adiabaticIndex * M.CV + (1.0 - lam) * M.adiabaticIndex2 * M.CV2) / (lam * M.CV + (1.0 - lam) * M.CV2) - 1.0;
     }
-    double shearG(int m) const {
+    // componentShearModulus (equationofstate.cpp:77-80, 447-450, 548-551)
+    double shearG(Fab& f, int i, int j, int k, int m) {
         const Material& M = p.mat[m];
-        return M.eos == 1 ? M.G0 : 0.0;
+        if (M.eos == 1 || M.eos == 3) return M.G0;
+        if (M.eos == 2) return romG(M, A(f, i, j, k, RHO_K, m));
+        return 0.0;
     }
     double shearE(Fab& f, int i, int j, int k, int m) {
-        if (p.mat[m].eos == 1) return A(f, i, j, k, ALPHA, m) * (shearG(m) * A(f, i, j, k, HJ2));
+        if (p.mat[m].eos >= 1) return A(f, i, j, k, ALPHA, m) * (shearG(f, i, j, k, m) * A(f, i, j, k, HJ2));
         return 0.0 * A(f, i, j, k, ALPHA, m);
     }
     double shearP(Fab& f, int i, int j, int k, int m) {
         const Material& M = p.mat[m];
         if (M.eos == 1)
-            return (A(f, i, j, k, ALPHA, m) / M.GG()) * ((A(f, i, j, k, RHO_K, m) * 0.0 - shearG(m)) * A(f, i, j, k, HJ2));
+            return (A(f, i, j, k, ALPHA, m) / M.GG()) * ((A(f, i, j, k, RHO_K, m) * 0.0 - shearG(f, i, j, k, m)) * A(f, i, j, k, HJ2));
+        if (M.eos >= 2)
+            return (A(f, i, j, k, ALPHA, m) / M.GG()) *
+                   ((A(f, i, j, k, RHO_K, m) * solidDGdrho(M, A(f, i, j, k, RHO_K, m)) - shearG(f, i, j, k, m)) * A(f, i, j, k, HJ2));
         return 0.0 * A(f, i, j, k, ALPHA, m);
     }
     double invGr(Fab& f, int i, int j, int k, int m) {
@@ -454,6 +499,11 @@ class Solver {
             dedrho += (1.0 - lam) * ((M.pref2 - pp) / (M.GG2() * rk * rk));
             return (pp / (rk * rk) - dedrho) /
                    (lam / (A(f, i, j, k, RHO_MIX, m, 0) * M.GG()) + (1.0 - lam) / (A(f, i, j, k, RHO_MIX, m, 1) * M.GG2()));
+        }
+        if (M.eos >= 2) {  // equationofstate.cpp:435-438, 536-539
+            return (pp * (M.GG() + 1.0) - (coldP(f, i, j, k, m) + shearP(f, i, j, k, m)) * (M.GG() + 1.0)) / rk +
+                   solidDpcdrho(M, rk) + rk * solidDG2drho2(M, rk) * A(f, i, j, k, HJ2) +
+                   (4.0 / 3.0) * shearG(f, i, j, k, m) / rk;
         }
         if (M.eos == 1) {
             double gp1 = M.GG() + 1.0;
@@ -508,7 +558,7 @@ class Solver {
         for (int m = 0; m < p.nmat; ++m)
             if (p.mat[m].solid) {
                 solid = 1;
-                Gt += shearG(m) * invGr(f, i, j, k, m);
+                Gt += shearG(f, i, j, k, m) * invGr(f, i, j, k, m);
             }
         Gt = Gt / effInvG(f, i, j, k);
         for (int r = 0; r < 3; ++r)
@@ -668,7 +718,7 @@ class Solver {
         double b = 0.0;
         henckyJ2(f, i, j, k);
         for (int m = 0; m < p.nmat; ++m)
-            if (p.mat[m].solid) b += invGr(f, i, j, k, m) * shearG(m) / A(f, i, j, k, RHO_K, m);
+            if (p.mat[m].solid) b += invGr(f, i, j, k, m) * shearG(f, i, j, k, m) / A(f, i, j, k, RHO_K, m);
         if (std::isnan(b) || b < 0.0) b = 0.0;
         double t = std::sqrt(b / effInvG(f, i, j, k));
         if (std::isnan(t)) t = 0.0;
@@ -1204,7 +1254,7 @@ class Solver {
                     double J, Jn;
                     if (p.mat[m].solid) {
                         J = std::sqrt(A(f, i, j, k, HJ2));
-                        if (shearG(m) * J * s3 > p.mat[m].yield) Jn = p.mat[m].yield / (s3 * shearG(m));
+                        if (shearG(f, i, j, k, m) * J * s3 > p.mat[m].yield) Jn = p.mat[m].yield / (s3 * shearG(f, i, j, k, m));
                         else Jn = J;
                         A(f, i, j, k, EPSILON, m) += s23 * (J - Jn);
                         A(f, i, j, k, ALPHARHOEPSILON, m) = A(f, i, j, k, EPSILON, m) * A(f, i, j, k, ALPHARHO, m);

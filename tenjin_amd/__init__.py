@@ -46,6 +46,14 @@ class Material(ctypes.Structure):
         ("pref_mix", ctypes.c_double),
         ("eref_mix", ctypes.c_double),
         ("CV_mix", ctypes.c_double),
+        ("K0", ctypes.c_double),
+        ("eos_alpha", ctypes.c_double),
+        ("eos_beta", ctypes.c_double),
+        ("e1", ctypes.c_double),
+        ("e2", ctypes.c_double),
+        ("e3", ctypes.c_double),
+        ("e4", ctypes.c_double),
+        ("e5", ctypes.c_double),
     ]
 
 
@@ -186,6 +194,8 @@ def level_desc(pb) -> LevelDesc:
         t.rho0, t.c0, t.s, t.G0, t.yield_stress = M.rho0, M.c0, M.s, M.G0, M.yield_stress
         t.mixture = M.mixture
         t.gamma_mix, t.pref_mix, t.eref_mix, t.CV_mix = M.gamma2, M.pref2, M.eref2, M.CV2
+        t.K0, t.eos_alpha, t.eos_beta = M.K0, M.eos_alpha, M.eos_beta
+        t.e1, t.e2, t.e3, t.e4, t.e5 = M.e1, M.e2, M.e3, M.e4, M.e5
     return d
 
 

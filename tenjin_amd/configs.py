@@ -129,6 +129,15 @@ class Material:
     pref2: float = 0.0
     eref2: float = 0.0
     CV2: float = 1.0
+    # RomenskiiSolidEOS (eos 2) and WilkinsSolidEOS (eos 3), with rho0 and G0
+    K0: float = 0.0
+    eos_alpha: float = 0.0
+    eos_beta: float = 0.0
+    e1: float = 0.0
+    e2: float = 0.0
+    e3: float = 0.0
+    e4: float = 0.0
+    e5: float = 0.0
 
     @staticmethod
     def stiffened(gamma: float, p_inf: float = 0.0) -> "Material":
@@ -139,6 +148,20 @@ class Material:
         """Reactant/product mixture; a, b = (adiabaticIndex, pref, eref, CV)."""
         return Material(gamma=a[0], pref=a[1], eref=a[2], CV=a[3], mixture=1,
                         gamma2=b[0], pref2=b[1], eref2=b[2], CV2=b[3])
+
+    @staticmethod
+    def romenskii_solid(gamma, pref, eref, CV, rho0, K0, alpha, beta, G, Y=0.0, plastic=True) -> "Material":
+        """RomenskiiSolidEOS (equationofstate.cpp:403-495): adiabaticIndex,
+        pref, eref, CV, rho0, K0, EOSalpha, EOSbeta, G0 as in a settings deck."""
+        return Material(gamma=gamma, pref=pref, eref=eref, CV=CV, eos=2, solid=1, plastic=int(plastic), rho0=rho0,
+                        G0=G, yield_stress=Y, K0=K0, eos_alpha=alpha, eos_beta=beta)
+
+    @staticmethod
+    def wilkins_solid(gamma, pref, eref, CV, rho0, e, G, Y=0.0, plastic=True) -> "Material":
+        """WilkinsSolidEOS (equationofstate.cpp:502-596): adiabaticIndex, pref,
+        eref, CV, rho0, e1 .. e5, G0 as in a settings deck."""
+        return Material(gamma=gamma, pref=pref, eref=eref, CV=CV, eos=3, solid=1, plastic=int(plastic), rho0=rho0,
+                        G0=G, yield_stress=Y, e1=e[0], e2=e[1], e3=e[2], e4=e[3], e5=e[4])
 
     @staticmethod
     def usup_solid(rho0, Gamma0, c0, s, G, Y, plastic=True) -> "Material":
@@ -361,6 +384,33 @@ def plate_impact(dim: int = 3, n: int = 256, box: int = 64, seed: int = 3, speed
     return pb
 
 
+# The plate impact's copper and aluminium as the reference's other solid EOSs:
+# RomenskiiSolidEOS copper (K0 = rho0 c0^2, EOSalpha 1.3, EOSbeta 2.2) and a
+# WilkinsSolidEOS aluminium whose polynomial has e_c = p_c = 0 at rho0 and
+# bulk modulus rho0 c0^2 (e1 + e2 + e3 + e4 = 0, 2 e2 + e3 - e4 - e5 = 0,
+# 6 e2 + 2 e3 - e5 = K).
+def romenskii_copper(plastic: bool = True) -> Material:
+    cu = COPPER
+    return Material.romenskii_solid(cu["Gamma0"] + 1.0, 0.0, 0.0, 390.0, cu["rho0"], cu["rho0"] * cu["c0"] ** 2,
+                                    1.3, 2.2, cu["G"], cu["Y"], plastic)
+
+
+def wilkins_aluminium(plastic: bool = True) -> Material:
+    al = ALUMINIUM
+    K = al["rho0"] * al["c0"] ** 2
+    e = (0.5 * K, 0.5 * K, -0.85 * K, -0.15 * K, 0.3 * K)
+    return Material.wilkins_solid(al["Gamma0"] + 1.0, 0.0, 0.0, 900.0, al["rho0"], e, al["G"], al["Y"], plastic)
+
+
+def plate_impact_rw(dim: int = 2, n: int = 48, box: int = 24, **kw) -> Problem:
+    """plate_impact with the Romenskii copper flyer and the Wilkins aluminium
+    target (the densities, speeds and geometry unchanged)."""
+    pb = plate_impact(dim=dim, n=n, box=box, **kw)
+    pb.materials = [romenskii_copper(bool(pb.PLASTIC)), wilkins_aluminium(bool(pb.PLASTIC))]
+    pb.name = f"plate_impact_rw_{dim}d"
+    return pb
+
+
 PLATE_PARAMS = ("Gamma0", "s", "Y")  # per material: Us-up Gruneisen coefficient, Hugoniot slope, yield stress
 
 
@@ -504,14 +554,16 @@ def reactive_tube(dim: int = 1, n: int = 800, box: int = 1 << 30, thinc: int = 0
 
 def ensemble(seed: int = 6, n1d: int = 128, n2d: int = 128, n1d_range=(500, 8000), side_range=(64, 512),
              thinc: int = 1, steps: int = 200) -> List[Problem]:
-    """Ensemble of independent two-material stiffened-gas shock tubes: n1d 1D
-    tubes with N ~ U{n1d_range} cells and n2d 2D square tubes with side
-    ~ U{side_range} (planar interface at x = 0.5 with a one-mode sinusoidal
-    perturbation in y).  Per problem: gamma ~ U[1.2, 5] and p_inf ~ U[0, 1e9]
-    for each material, density ratio ~ logU[10, 1000] and pressure ratio
+    """Ensemble of independent two-material shock tubes: n1d 1D tubes with
+    N ~ U{n1d_range} cells and n2d 2D square tubes with side ~ U{side_range}
+    (planar interface at x = 0.5 with a one-mode sinusoidal perturbation in
+    y).  Per problem: gamma ~ U[1.2, 5] and p_inf ~ U[0, 1e9] for each
+    material, density ratio ~ logU[10, 1000] and pressure ratio
     ~ logU[10, 1e4] (the dense, high-pressure side on the left), unit domain,
-    64-cell boxes, transmissive ends; 2D tubes reflective in y.  Draws come from
-    numpy's RandomState(seed) (MT19937)."""
+    64-cell boxes, transmissive ends; 2D tubes reflective in y.  Even problems
+    are stiffened-gas pairs; odd ones Mie-Gruneisen pairs (MieGruneisenEOS with
+    a reference energy eref ~ U[-1e5, 1e5] J/kg per material as well).  Draws
+    come from numpy's RandomState(seed) (MT19937)."""
     rs = np.random.RandomState(seed)
     out = []
     for q in range(n1d + n2d):
@@ -523,7 +575,10 @@ def ensemble(seed: int = 6, n1d: int = 128, n2d: int = 128, n1d_range=(500, 8000
         drat = float(np.exp(rs.uniform(np.log(10.0), np.log(1000.0))))
         prat = float(np.exp(rs.uniform(np.log(10.0), np.log(1e4))))
         amp = float(rs.uniform(0.0, 0.05))
+        er0, er1 = rs.uniform(-1e5, 1e5, 2)
         mats = [Material.stiffened(float(g0), float(pi0)), Material.stiffened(float(g1), float(pi1))]
+        if q % 2:  # Mie-Gruneisen pair: the same closure with a reference energy
+            mats[0].eref, mats[1].eref = float(er0), float(er1)
         nn = (n, n if dim == 2 else 1, 1)
         pb = Problem(f"ensemble_{q}_{dim}d", dim, nn, (1.0, 1.0, 1.0), 64, mats, THINC=thinc, steps=steps,
                      bc_lo=(0, 1, 0), bc_hi=(0, 1, 0))

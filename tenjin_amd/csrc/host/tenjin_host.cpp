@@ -509,6 +509,20 @@ void bindLevel(ParameterStruct& parameters, const Geometry& geom, const Vector<B
             t.c0 = s->c0;
             t.s = s->s;
         }
+        if (const RomenskiiSolidEOS* r = dynamic_cast<const RomenskiiSolidEOS*>(md.EOS)) {
+            t.rho0 = r->rho0;
+            t.K0 = r->K0;
+            t.eos_alpha = r->EOSalpha;
+            t.eos_beta = r->EOSbeta;
+        }
+        if (const WilkinsSolidEOS* w = dynamic_cast<const WilkinsSolidEOS*>(md.EOS)) {
+            t.rho0 = w->rho0;
+            t.e1 = w->e1;
+            t.e2 = w->e2;
+            t.e3 = w->e3;
+            t.e4 = w->e4;
+            t.e5 = w->e5;
+        }
     }
     if (tj_create(&d->ctx, device) != TJ_OK) throw std::runtime_error("bindLevel: tj_create failed");
     parameters.gpu = d;

@@ -84,8 +84,48 @@ public:
         first.define(Vector<Real>(params.begin(), params.begin() + 4));
         second.define(Vector<Real>(params.begin() + 4, params.end()));
     }
-    int kind() const override { return 2; }
+    int kind() const override { return 0; }  // a fluid of the MieGruneisen family (tj_material::mixture)
     MieGruneisenEOS first, second;
+};
+
+// RomenskiiSolidEOS (equationofstate.h:108-143): define(params) of
+// equationofstate.cpp:403-417 reads adiabaticIndex, pref, eref, CV, rho0, K0,
+// EOSalpha, EOSbeta, G0 (initialise.cpp:149-160).
+class RomenskiiSolidEOS : public MieGruneisenEOS {
+public:
+    RomenskiiSolidEOS() = default;
+    void define(const Vector<Real>& params) override {
+        MieGruneisenEOS::define(params);
+        rho0 = params.at(4);
+        K0 = params.at(5);
+        EOSalpha = params.at(6);
+        EOSbeta = params.at(7);
+        G0 = params.at(8);
+        shearModulus = G0;
+    }
+    int kind() const override { return 2; }
+    Real rho0 = 0.0, K0 = 0.0, EOSalpha = 0.0, EOSbeta = 0.0, G0 = 0.0;
+};
+
+// WilkinsSolidEOS (equationofstate.h:145-178): define(params) of
+// equationofstate.cpp:502-518 reads adiabaticIndex, pref, eref, CV, rho0,
+// e1 .. e5, G0 (initialise.cpp:161-172).
+class WilkinsSolidEOS : public MieGruneisenEOS {
+public:
+    WilkinsSolidEOS() = default;
+    void define(const Vector<Real>& params) override {
+        MieGruneisenEOS::define(params);
+        rho0 = params.at(4);
+        e1 = params.at(5);
+        e2 = params.at(6);
+        e3 = params.at(7);
+        e4 = params.at(8);
+        e5 = params.at(9);
+        G0 = params.at(10);
+        shearModulus = G0;
+    }
+    int kind() const override { return 3; }
+    Real rho0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0, G0 = 0.0;
 };
 
 // Us-up Mie-Gruneisen solid (no reference class; the solid-EOS interface of

@@ -27,6 +27,9 @@ struct MatT {
     // adiabaticIndex as given (the mixture formulas read it, not Gg + 1);
     // g2, Gg2, pref2, eref2, CV2 the products' MieGruneisen EOS
     R ga, g2, Gg2, pref2, eref2, CV2;
+    // RomenskiiSolidEOS (K0, EOSalpha, EOSbeta; equationofstate.cpp:403-495)
+    // and WilkinsSolidEOS (e1 .. e5; :502-596), with rho0 and G0 above
+    R K0, ea, eb, e1, e2, e3, e4, e5;
 };
 using MatC = MatT<double>;
 
@@ -125,6 +128,13 @@ __device__ __forceinline__ R tsgn(R v) { return (R)((R(0) < v) - (v < R(0))); }
 // whose error is far below that form's accuracy bar.
 __device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// Correctly rounded in both forms: the HLLC star-state factor (S_K - u)/(S_K - S*)
+// (hllc.cpp:40-44), which must be exactly 1 for two identical states at rest.
+// With the 1-ulp reciprocal it is not, and every face of a quiet region then
+// carries a flux S_K (mult - 1) U: it cancels inside the region, but at an open
+// domain end it is a boundary flux that moves mass and energy in or out.
+__device__ __forceinline__ double tdiv_cr(double a, double b) { return a / b; }
+__device__ __forceinline__ float tdiv_cr(float a, float b) { return a / b; }
 __device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
@@ -257,6 +267,12 @@ __device__ inline double pm_pow(double x, double y) {
 
 // exp/tanh/cosh by real type: the project-written FP64 routines above, and
 // the device library's single-precision functions for the mixed form.
+// x^y and log x of the solid EOSs (std::pow, std::log in equationofstate.cpp:
+// 403-596): the project routines in FP64, the device library's in FP32.
+__device__ __forceinline__ double tpow(double x, double y) { return pm_pow(x, y); }
+__device__ __forceinline__ float tpow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double tlog(double x) { return pm_log(x); }
+__device__ __forceinline__ float tlog(float x) { return logf(x); }
 __device__ __forceinline__ double texp(double x) { return pm_exp(x); }
 __device__ __forceinline__ double ttanh(double x) { return pm_tanh(x); }
 __device__ __forceinline__ double tcosh(double x) { return pm_cosh(x); }

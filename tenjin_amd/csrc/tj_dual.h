@@ -131,6 +131,7 @@ TJ_DUAL_FN Dual<K> tdiv(double a, const Dual<K>& b) {
     for (int t = 0; t < K; ++t) r.d[t] = g * b.d[t];
     return r;
 }
+TJ_DUAL_FN Dual<K> tdiv_cr(const Dual<K>& a, const Dual<K>& b) { return tdiv(a, b); }
 TJ_DUAL_FN Dual<K> operator/(const Dual<K>& a, const Dual<K>& b) { return tdiv(a, b); }
 TJ_DUAL_FN Dual<K> operator/(const Dual<K>& a, double b) { return tdiv(a, b); }
 TJ_DUAL_FN Dual<K> operator/(double a, const Dual<K>& b) { return tdiv(a, b); }

@@ -386,7 +386,7 @@ __device__ __forceinline__ void hllc(const FSt<NM, R>& L, const FSt<NM, R>& Rs, 
     const R SK = useL ? SL : SR;
     const R sigK = useL ? sigL : sigR;
     // star state
-    const R mult = (tabs(tdiv(Ss - K.u[d], Ss)) < R(1E-20)) ? R(1) : tdiv(SK - K.u[d], SK - Ss);
+    const R mult = (tabs(tdiv(Ss - K.u[d], Ss)) < R(1E-20)) ? R(1) : tdiv_cr(SK - K.u[d], SK - Ss);
     R arS[NM], rhoS = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {

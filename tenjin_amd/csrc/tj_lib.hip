@@ -217,6 +217,17 @@ static int level_from_desc(const char*& err, const tj_level_desc* d, Level& lv) 
         err = "bind_level: at most one mixture material (all mixtures share mixtureIndex 0, initialise.cpp:187)";
         return TJ_EUNSUPPORTED;
     }
+    for (int m = 0; m < d->nmat; ++m) {
+        const tj_material& M = d->mat[m];
+        if (M.eos < 0 || M.eos > 3 || (M.eos >= 1 && !M.solid)) {
+            err = "bind_level: EOS kinds are 0 (MieGruneisen), 1 (Us-up), 2 (Romenskii), 3 (Wilkins); 1-3 are solids";
+            return TJ_EINVAL;
+        }
+        if (M.eos >= 1 && !d->SOLID) {
+            err = "bind_level: a solid EOS needs a SOLID level";
+            return TJ_EINVAL;
+        }
+    }
     if (nmixm && d->SOLID) {
         err = "bind_level: mixtures are available on fluid levels";
         return TJ_EUNSUPPORTED;
@@ -270,6 +281,14 @@ static int level_from_desc(const char*& err, const tj_level_desc* d, Level& lv) 
         C.pref2 = M.pref_mix;
         C.eref2 = M.eref_mix;
         C.CV2 = M.CV_mix;
+        C.K0 = M.K0;
+        C.ea = M.eos_alpha;
+        C.eb = M.eos_beta;
+        C.e1 = M.e1;
+        C.e2 = M.e2;
+        C.e3 = M.e3;
+        C.e4 = M.e4;
+        C.e5 = M.e5;
         if (C.mix) lv.mixm = m;
         MatT<float>& F = lv.mf[m];
         F.eos = C.eos;
@@ -293,6 +312,14 @@ static int level_from_desc(const char*& err, const tj_level_desc* d, Level& lv) 
         F.pref2 = (float)C.pref2;
         F.eref2 = (float)C.eref2;
         F.CV2 = (float)C.CV2;
+        F.K0 = (float)C.K0;
+        F.ea = (float)C.ea;
+        F.eb = (float)C.eb;
+        F.e1 = (float)C.e1;
+        F.e2 = (float)C.e2;
+        F.e3 = (float)C.e3;
+        F.e4 = (float)C.e4;
+        F.e5 = (float)C.e5;
     }
     build_layout(*d, lv.cs);
     lv.ncells = (long long)lv.n[0] * lv.n[1] * lv.n[2];
@@ -397,9 +424,18 @@ int tj_comp_start(const tj_ctx* c, int v) {
 }
 
 
+// A solid level holding a Romenskii or Wilkins material runs the solid kernels'
+// RW instantiation; Us-up levels run the one without their pow/log code.
+static bool rw_level(const Level& lv) {
+    for (int m = 0; m < lv.nmat; ++m)
+        if (lv.m[m].eos >= 2) return true;
+    return false;
+}
+
 static void launch_sound_dt(tj_ctx* c, const MF& U) {
     const unsigned g = grid1(c->lv.ncells, 256);
-    if (c->lv.solid) hipLaunchKernelGGL(k_sound_dt_s<2>, dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
+    if (c->lv.solid && rw_level(c->lv)) hipLaunchKernelGGL((k_sound_dt_s<2, true>), dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
+    else if (c->lv.solid) hipLaunchKernelGGL((k_sound_dt_s<2, false>), dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
     else if (c->lv.mixm >= 0) hipLaunchKernelGGL((k_sound_dt<2, true>), dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
     else hipLaunchKernelGGL(k_sound_dt<2>, dim3(g), dim3(256), 0, c->stream, c->lv, U, c->d_clk);
 }
@@ -410,10 +446,15 @@ static void launch_update_t(tj_ctx* c, int stage, const MF& in, const MF& base, 
                             const FacesT<T>* F) {
     const unsigned g = grid1(c->lv.ncells, 256);
     if (c->lv.solid) {
-        if (stage == 1)
-            hipLaunchKernelGGL((k_update_s<2, 1, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
+        const bool rw = rw_level(c->lv);
+        if (stage == 1 && rw)
+            hipLaunchKernelGGL((k_update_s<2, 1, true, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
+        else if (stage == 1)
+            hipLaunchKernelGGL((k_update_s<2, 1, false, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
+        else if (rw)
+            hipLaunchKernelGGL((k_update_s<2, 2, true, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
         else
-            hipLaunchKernelGGL((k_update_s<2, 2, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
+            hipLaunchKernelGGL((k_update_s<2, 2, false, T>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
     } else if (c->lv.mixm >= 0) {
         if (stage == 1)
             hipLaunchKernelGGL((k_update<2, 1, T, true>), dim3(g), dim3(256), 0, c->stream, c->lv, in, base, out, F[0], F[1], F[2], c->d_clk, ops);
@@ -639,24 +680,30 @@ static unsigned march_grid(const Level& lv, int d, int sl, bool fused) {
 
 // One directional flux pass (not fused) with face values of type R.
 extern "C++" {
+template <typename R, bool RW>
+static void launch_pass_solid(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F) {
+    const Level& lv = c->lv;
+    if (d == 0) {
+        const long long nseg = (lv.n[0] + 1 + 62) / 63;
+        const unsigned g = grid1(nseg * lv.n[1] * lv.n[2], 4);
+        hipLaunchKernelGGL((k_flux_x_s<2, RW, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], c->d_clk);
+    } else if (d == 1) {
+        const int sl = seg_for(c, TJ_KPTR(k_flux_lmarch_s<2, 1, RW, R>), 1, false);
+        hipLaunchKernelGGL((k_flux_lmarch_s<2, 1, RW, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0, c->stream,
+                           lv, Uin, F[1], sl, c->d_clk);
+    } else {
+        const int sl = seg_for(c, TJ_KPTR(k_flux_lmarch_s<2, 2, RW, R>), 2, false);
+        hipLaunchKernelGGL((k_flux_lmarch_s<2, 2, RW, R>), dim3(march_grid(lv, 2, sl, false)), dim3(256), 0, c->stream,
+                           lv, Uin, F[2], sl, c->d_clk);
+    }
+}
 template <typename R, bool MX>
 static void launch_pass_t(tj_ctx* c, const MF& Uin, int d, const FacesT<R>* F, const Hist& h) {
     const Level& lv = c->lv;
     const bool th = c->desc.THINC != 0;
     if (lv.solid) {
-        if (d == 0) {
-            const long long nseg = (lv.n[0] + 1 + 62) / 63;
-            const unsigned g = grid1(nseg * lv.n[1] * lv.n[2], 4);
-            hipLaunchKernelGGL((k_flux_x_s<2, R>), dim3(g), dim3(256), 0, c->stream, lv, Uin, F[0], c->d_clk);
-        } else if (d == 1) {
-            const int sl = seg_for(c, TJ_KPTR(k_flux_lmarch_s<2, 1, R>), 1, false);
-            hipLaunchKernelGGL((k_flux_lmarch_s<2, 1, R>), dim3(march_grid(lv, 1, sl, false)), dim3(256), 0, c->stream,
-                               lv, Uin, F[1], sl, c->d_clk);
-        } else {
-            const int sl = seg_for(c, TJ_KPTR(k_flux_lmarch_s<2, 2, R>), 2, false);
-            hipLaunchKernelGGL((k_flux_lmarch_s<2, 2, R>), dim3(march_grid(lv, 2, sl, false)), dim3(256), 0, c->stream,
-                               lv, Uin, F[2], sl, c->d_clk);
-        }
+        if (rw_level(lv)) launch_pass_solid<R, true>(c, Uin, d, F);
+        else launch_pass_solid<R, false>(c, Uin, d, F);
     } else if (d == 0) {
         const int per = th ? 61 : 63;
         const long long nseg = (lv.n[0] + 1 + per - 1) / per;
@@ -857,7 +904,8 @@ int tj_post_step(tj_ctx* c, tj_mf* U, unsigned ops) {
     }
     const unsigned g = grid1(c->lv.ncells, 256);
     if (c->lv.solid) {
-        hipLaunchKernelGGL(k_post_s<2>, dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk, ops);
+        if (rw_level(c->lv)) hipLaunchKernelGGL((k_post_s<2, true>), dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk, ops);
+        else hipLaunchKernelGGL((k_post_s<2, false>), dim3(g), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk, ops);
     } else {
         const bool mx = c->lv.mixm >= 0;
         if ((ops & TJ_OP_REACTIVE) && mx)  // reactiveUpdate acts on mixtures only (reactive.cpp:64)
@@ -1016,9 +1064,10 @@ int tj_init_from_prims(tj_ctx* c, tj_mf* U, int shift) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
     if ((r = check_mf(c, U))) return r;
-    if (c->lv.solid)
-        hipLaunchKernelGGL(k_init_p2c_s<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
-    else
+    if (c->lv.solid) {
+        if (rw_level(c->lv)) hipLaunchKernelGGL((k_init_p2c_s<2, true>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
+        else hipLaunchKernelGGL((k_init_p2c_s<2, false>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
+    } else
         if (c->lv.mixm >= 0)
             hipLaunchKernelGGL((k_init_p2c<2, true>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), shift);
         else
@@ -1031,9 +1080,10 @@ int tj_c2p(tj_ctx* c, tj_mf* U) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
     if ((r = check_mf(c, U))) return r;
-    if (c->lv.solid)
-        hipLaunchKernelGGL(k_c2p_s<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
-    else
+    if (c->lv.solid) {
+        if (rw_level(c->lv)) hipLaunchKernelGGL((k_c2p_s<2, true>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+        else hipLaunchKernelGGL((k_c2p_s<2, false>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+    } else
         if (c->lv.mixm >= 0)
             hipLaunchKernelGGL((k_c2p<2, true>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U), c->d_clk);
         else
@@ -1061,9 +1111,10 @@ int tj_derive(tj_ctx* c, tj_mf* U) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     int r;
     if ((r = check_mf(c, U))) return r;
-    if (c->lv.solid)
-        hipLaunchKernelGGL(k_derive_s<2>, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
-    else
+    if (c->lv.solid) {
+        if (rw_level(c->lv)) hipLaunchKernelGGL((k_derive_s<2, true>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+        else hipLaunchKernelGGL((k_derive_s<2, false>), dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
+    } else
         hipLaunchKernelGGL(k_derive_fluid, dim3(grid1(c->lv.ncells, 256)), dim3(256), 0, c->stream, c->lv, view(U));
     HIPCHK(c, hipGetLastError());
     return TJ_OK;

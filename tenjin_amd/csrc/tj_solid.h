@@ -202,11 +202,94 @@ __device__ __forceinline__ R deH(const MatT<R>& M, R rho) {
     const R detadrho = tdiv(M.rho0, rho * rho);
     return tdiv(dpH(M, rho) * eta + pH(M, rho) * detadrho, R(2) * M.rho0);
 }
+// componentShearModulus (equationofstate.cpp:77-80, 447-450, 548-551): G0 of
+// the Us-up and Wilkins solids, G0 (rho_k/rho0)^(beta + 1) of the Romenskii
+// solid, 0 for a MieGruneisen material.  RW: the level holds a Romenskii or
+// Wilkins material.  The kernels are instantiated both ways and the launch
+// picks one, so that the Us-up kernels carry none of the pow/log code (their
+// registers and spills double with it).
+template <bool RW, typename R>
+__device__ __forceinline__ R shearG(const MatT<R>& M, R rk) {
+    if (M.eos == 1 || M.eos == 3) return M.G0;
+    if constexpr (RW) {
+        if (M.eos == 2) return M.G0 * tpow(tdiv(rk, M.rho0), M.eb + R(1.0));
+    }
+    return R(0);
+}
+
+// RomenskiiSolidEOS (equationofstate.cpp:420-495) and WilkinsSolidEOS
+// (:519-596): cold energy and pressure, dG/drho, d2G/drho2, dp_c/drho and
+// dp_s/drho of material M at density rk; a = alpha, ar = alpha rho.
 template <typename R>
-__device__ __forceinline__ R shearG(const MatT<R>& M) { return M.eos == 1 ? M.G0 : R(0); }
+__device__ __forceinline__ R cold_e(const MatT<R>& M, R ar, R rk) {
+    const R x = tdiv(rk, M.rho0);
+    if (M.eos == 2) {
+        const R xa = tpow(x, M.ea);
+        return ar * (tdiv(M.K0, R(2.0) * M.rho0 * M.ea * M.ea) * (xa - R(1.0)) * (xa - R(1.0)));
+    }
+    return ar * tdiv(M.e1 + M.e2 * x * x + M.e3 * x + tdiv(M.e4, x) - M.e5 * tlog(x), M.rho0);
+}
+template <typename R>
+__device__ __forceinline__ R cold_p(const MatT<R>& M, R a, R rk) {
+    const R x = tdiv(rk, M.rho0);
+    if (M.eos == 2) return tdiv(a, M.Gg) * (tdiv(M.K0, M.ea) * tpow(x, M.ea + R(1.0)) * (tpow(x, M.ea) - R(1.0)));
+    return tdiv(a, M.Gg) * ((R(2.0) * M.e2 * x * x * x + M.e3 * x * x - M.e4 - M.e5 * x));
+}
+template <typename R>
+__device__ __forceinline__ R dGdrho(const MatT<R>& M, R rk) {
+    if (M.eos == 2) return tdiv(M.G0 * (M.eb + R(1.0)) * tpow(tdiv(rk, M.rho0), M.eb), M.rho0);
+    return R(0.0);
+}
+template <typename R>
+__device__ __forceinline__ R dG2drho2(const MatT<R>& M, R rk) {
+    if (M.eos == 2) return tdiv(M.G0 * (M.eb + R(1.0)) * (M.eb) * tpow(tdiv(rk, M.rho0), M.eb - R(1.0)), M.rho0 * M.rho0);
+    return R(0.0);
+}
+template <typename R>
+__device__ __forceinline__ R dpc_drho(const MatT<R>& M, R rk) {
+    const R x = tdiv(rk, M.rho0);
+    if (M.eos == 2) {
+        const R xa = tpow(x, M.ea);
+        return tdiv(M.K0, M.ea * M.rho0) * xa * ((R(2.0) * M.ea + R(1.0)) * xa - (M.ea + R(1.0)));
+    }
+    return tdiv(R(6.0) * M.e2 * x * x + R(2.0) * M.e3 * x - M.e5, M.rho0);
+}
+
+// coldCompressionInternalEnergy + shearInternalEnergy and coldCompressionPressure
+// + shearPressure of a Romenskii or Wilkins material (equationofstate.cpp:420-495, 519-596)
+template <bool RW, typename R>
+__device__ __forceinline__ R rw_energy(const MatT<R>& M, R a, R ar, R rk, R hj2) {
+    if constexpr (!RW) {
+        return R(0);
+    } else {
+        return (cold_e(M, ar, rk) + a * (shearG<true>(M, rk) * hj2));
+    }
+}
+template <bool RW, typename R>
+__device__ __forceinline__ R rw_pressure(const MatT<R>& M, R a, R rk, R hj2) {
+    if constexpr (!RW) {
+        return R(0);
+    } else {
+        return cold_p(M, a, rk) + tdiv(a, M.Gg) * ((rk * dGdrho(M, rk) - shearG<true>(M, rk)) * hj2);
+    }
+}
+// getSoundSpeedContribution of a Romenskii or Wilkins material
+// (equationofstate.cpp:435-438, 536-539), whose cold and shear pressures carry
+// the material's alpha / Gamma factor as the reference's do
+template <bool RW, typename R>
+__device__ __forceinline__ R rw_c2(const MatT<R>& M, R p, R a, R rk, R hj2) {
+    if constexpr (!RW) {
+        return R(0);
+    } else {
+        const R G = shearG<true>(M, rk);
+        const R ps = tdiv(a, M.Gg) * ((rk * dGdrho(M, rk) - G) * hj2);
+        return tdiv(p * (M.Gg + R(1.0)) - (cold_p(M, a, rk) + ps) * (M.Gg + R(1.0)), rk) + dpc_drho(M, rk) +
+               rk * dG2drho2(M, rk) * hj2 + tdiv((R(4.0) / R(3.0)) * G, rk);
+    }
+}
 
 // effective sums of boxaccesscellarray.cpp:218-252 with the solid terms
-template <int NM, typename R>
+template <int NM, bool RW, typename R>
 __device__ __forceinline__ void eos_sums_s(const Level& lv, const R* a, const R* ar, const R* rk, R hj2, R& G, R& Ent,
                                            R& Pnt) {
     G = R(0);
@@ -217,47 +300,54 @@ __device__ __forceinline__ void eos_sums_s(const Level& lv, const R* a, const R*
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         const MatT<R>& M = mat<R>(lv, m);
-        if (M.eos == 1) Ent += (ar[m] * eH(M, rk[m]) + a[m] * (shearG(M) * hj2));
+        if (M.eos == 1) Ent += (ar[m] * eH(M, rk[m]) + a[m] * (shearG<RW>(M, rk[m]) * hj2));
+        else if (RW && M.eos >= 2) Ent += rw_energy<RW>(M, a[m], ar[m], rk[m], hj2);
         else Ent += (M.eref * ar[m] + R(0) * a[m]);
     }
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         const MatT<R>& M = mat<R>(lv, m);
-        if (M.eos == 1) Pnt += tdiv(a[m], M.Gg) * pH(M, rk[m]) + tdiv(a[m], M.Gg) * ((rk[m] * R(0) - shearG(M)) * hj2);
+        if (M.eos == 1)
+            Pnt += tdiv(a[m], M.Gg) * pH(M, rk[m]) + tdiv(a[m], M.Gg) * ((rk[m] * R(0) - shearG<RW>(M, rk[m])) * hj2);
+        else if (RW && M.eos >= 2)
+            Pnt += rw_pressure<RW>(M, a[m], rk[m], hj2);
         else Pnt += tdiv(M.pref * a[m], M.Gg) + R(0) * a[m];
     }
 }
-// getSoundSpeedContribution: stiffened gas (equationofstate.cpp:61-64) or the
-// Us-up solid with c^2 = dp/drho|_s + 4G/(3 rho)
-template <typename R>
-__device__ __forceinline__ R c2_s(const MatT<R>& M, R p, R rk, R hj2) {
+// getSoundSpeedContribution: stiffened gas (equationofstate.cpp:61-64), the
+// Us-up solid with c^2 = dp/drho|_s + 4G/(3 rho), or the Romenskii / Wilkins
+// form of equationofstate.cpp:435-438, 536-539, whose cold and shear
+// pressures carry the material's alpha / Gamma factor as the reference's do.
+template <bool RW, typename R>
+__device__ __forceinline__ R c2_s(const MatT<R>& M, R p, R a, R rk, R hj2) {
     if (M.eos == 1)
         return tdiv(p * M.gp1 - pH(M, rk) + M.gp1 * M.G0 * hj2, rk) + dpH(M, rk) - M.Gg * rk * deH(M, rk) +
                tdiv((R(4.0) / R(3.0)) * M.G0, rk);
+    if (RW && M.eos >= 2) return rw_c2<RW>(M, p, a, rk, hj2);
     return tdiv(p * M.gp1, rk) - tdiv(M.pref, rk);
 }
-template <int NM, typename R>
+template <int NM, bool RW, typename R>
 __device__ __forceinline__ R sound_s(const Level& lv, const R* al, const R* ar, const R* rk, R rho, R p, R hj2) {
     R a = R(0), xt = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         const MatT<R>& M = mat<R>(lv, m);
-        a += tmax(R(0), tdiv(M.xi * c2_s(M, p, rk[m], hj2) * ar[m], rho));
+        a += tmax(R(0), tdiv(M.xi * c2_s<RW>(M, p, al[m], rk[m], hj2) * ar[m], rho));
         xt += M.xi * al[m];
     }
     if (a <= R(0)) a = R(1E-6);
     return tsqrt(tdiv(a, xt));
 }
 // stressTensor column d (boxaccesscellarray.cpp:176-216)
-template <int NM, typename R>
-__device__ __forceinline__ void stress_col(const Level& lv, const R* al, R p, const R* dH, int d, R* sig) {
+template <int NM, bool RW, typename R>
+__device__ __forceinline__ void stress_col(const Level& lv, const R* al, const R* rk, R p, const R* dH, int d, R* sig) {
     R Gt = R(0), Gi = R(0);
     int solid = 0;
 #pragma unroll
     for (int m = 0; m < NM; ++m)
         if (mat<R>(lv, m).solid) {
             solid = 1;
-            Gt += shearG(mat<R>(lv, m)) * tdiv(al[m], mat<R>(lv, m).Gg);
+            Gt += shearG<RW>(mat<R>(lv, m), rk[m]) * tdiv(al[m], mat<R>(lv, m).Gg);
         }
 #pragma unroll
     for (int m = 0; m < NM; ++m) Gi += tdiv(al[m], mat<R>(lv, m).Gg);
@@ -315,7 +405,7 @@ __device__ __forceinline__ void muscl_s(const SPrim<NM, Rt>& m_, const SPrim<NM,
 
 // primitiveToConservative + getSoundSpeed of a reconstructed solid state
 // (boxaccesscellarray.cpp:124-174, 254-285), keeping sigma column d.
-template <int NM, typename R>
+template <int NM, bool RW, typename R>
 __device__ __forceinline__ void p2c_sound_s(const Level& lv, const SPrim<NM, R>& P, int d, SSt<NM, R>& S) {
     R rho = R(0);
 #pragma unroll
@@ -334,19 +424,19 @@ __device__ __forceinline__ void p2c_sound_s(const Level& lv, const SPrim<NM, R>&
         ke += R(0.5) * rho * P.u[q] * P.u[q];
     }
     R G, Ent, Pnt;
-    eos_sums_s<NM, R>(lv, S.al, S.ar, P.r, hj2, G, Ent, Pnt);
+    eos_sums_s<NM, RW, R>(lv, S.al, S.ar, P.r, hj2, G, Ent, Pnt);
     S.E = P.p * G + Ent - Pnt + ke;
     S.p = P.p;
     S.rho = rho;
-    stress_col<NM, R>(lv, S.al, P.p, dH, d, S.sig);
+    stress_col<NM, RW, R>(lv, S.al, P.r, P.p, dH, d, S.sig);
 #pragma unroll
     for (int q = 0; q < 9; ++q) S.V[q] = P.V[q];
-    S.a = sound_s<NM, R>(lv, S.al, S.ar, P.r, rho, P.p, hj2);
+    S.a = sound_s<NM, RW, R>(lv, S.al, S.ar, P.r, rho, P.p, hj2);
 }
 
 // conservativeToPrimitive (boxaccesscellarray.cpp:67-122) of a solid cell;
 // returns HJ2 and the deviatoric Hencky strain in dH.
-template <int NM, typename R>
+template <int NM, bool RW, typename R>
 __device__ __forceinline__ R c2p_s(const Level& lv, const R* al, const R* ar, const R* are, const R* ru, R E, const R* V,
                                    R* rk, R* e, R& rho, R* u, R& p, R* dH) {
     rho = R(0);
@@ -364,18 +454,18 @@ __device__ __forceinline__ R c2p_s(const Level& lv, const R* al, const R* ar, co
         ke += R(0.5) * rho * u[q] * u[q];
     }
     R G, Ent, Pnt;
-    eos_sums_s<NM, R>(lv, al, ar, rk, hj2, G, Ent, Pnt);
+    eos_sums_s<NM, RW, R>(lv, al, ar, rk, hj2, G, Ent, Pnt);
     p = tdiv(E - ke - Ent + Pnt, G);
     return hj2;
 }
 
 // transverseWaveSpeed of a star state (boxaccesscellarray.cpp:287-324)
-template <int NM, typename R>
+template <int NM, bool RW, typename R>
 __device__ __forceinline__ R transverse(const Level& lv, const R* al, const R* rk) {
     R b = R(0), g = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m)
-        if (mat<R>(lv, m).solid) b += tdiv(tdiv(al[m], mat<R>(lv, m).Gg) * shearG(mat<R>(lv, m)), rk[m]);
+        if (mat<R>(lv, m).solid) b += tdiv(tdiv(al[m], mat<R>(lv, m).Gg) * shearG<RW>(mat<R>(lv, m), rk[m]), rk[m]);
     if (tisnan(b) || b < R(0)) b = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) g += tdiv(al[m], mat<R>(lv, m).Gg);
@@ -393,7 +483,7 @@ struct SStar {
 template <int NM, typename R>
 __device__ __forceinline__ void star_s(const Level& lv, const SSt<NM, R>& K, const R* ruK, R SK, R Ss, int d,
                                        SStar<NM, R>& T) {
-    const R mult = (tabs(tdiv(Ss - K.u[d], Ss)) < R(1E-20)) ? R(1) : tdiv(SK - K.u[d], SK - Ss);
+    const R mult = (tabs(tdiv(Ss - K.u[d], Ss)) < R(1E-20)) ? R(1) : tdiv_cr(SK - K.u[d], SK - Ss);
     T.rho = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
@@ -443,7 +533,7 @@ __device__ __forceinline__ void pflux_s(const SSt<NM, R>& K, const R* ruK, int d
 
 // calc_5Wave_fluxes for one face (hllc.cpp:75-380).  F layout: the conserved
 // vector (3 NM + 13), then S*, then VSTAR row d (3).
-template <int NM, typename Rt>
+template <int NM, bool RW, typename Rt>
 __device__ inline void wave5(const Level& lv, const SSt<NM, Rt>& L, const SSt<NM, Rt>& R, int d, Rt* F,
                              unsigned& st) {
     constexpr int NC = 3 * NM + 13;
@@ -471,22 +561,22 @@ __device__ inline void wave5(const Level& lv, const SSt<NM, Rt>& L, const SSt<NM
         left = true;
     } else if (Ss >= Rt(0)) {
         star_s<NM, Rt>(lv, L, ruL, SL, Ss, d, Ls);
-        SLT = Ss - transverse<NM, Rt>(lv, Ls.al, Ls.rk);
+        SLT = Ss - transverse<NM, RW, Rt>(lv, Ls.al, Ls.rk);
         left = true;
         if (SLT >= Rt(0)) {
             kind = 1;
         } else {
             star_s<NM, Rt>(lv, R, ruR, SR, Ss, d, Rs);
-            SRT = Ss + transverse<NM, Rt>(lv, Rs.al, Rs.rk);
+            SRT = Ss + transverse<NM, RW, Rt>(lv, Rs.al, Rs.rk);
             kind = 2;
         }
     } else if (SR >= Rt(0)) {
         star_s<NM, Rt>(lv, R, ruR, SR, Ss, d, Rs);
-        SRT = Ss + transverse<NM, Rt>(lv, Rs.al, Rs.rk);
+        SRT = Ss + transverse<NM, RW, Rt>(lv, Rs.al, Rs.rk);
         left = false;
         if (SRT >= Rt(0)) {
             star_s<NM, Rt>(lv, L, ruL, SL, Ss, d, Ls);
-            SLT = Ss - transverse<NM, Rt>(lv, Ls.al, Ls.rk);
+            SLT = Ss - transverse<NM, RW, Rt>(lv, Ls.al, Ls.rk);
             kind = 2;
         } else {
             kind = 1;
@@ -578,8 +668,8 @@ __device__ inline void wave5(const Level& lv, const SSt<NM, Rt>& L, const SSt<NM
 // ------------------------------------------------------------------ plastic
 // boxPlasticUpdate for one cell (plastic.cpp:83-225); V is overwritten with
 // the Cayley update even without yield.  Returns status bits.
-template <int NM, typename R>
-__device__ inline unsigned plastic_cell(const Level& lv, const R* al, const R* ar, R* e, R* are, R* V) {
+template <int NM, bool RW, typename R>
+__device__ inline unsigned plastic_cell(const Level& lv, const R* al, const R* ar, const R* rk, R* e, R* are, R* V) {
     R ts = R(0);
 #pragma unroll
     for (int m = 0; m < NM; ++m)
@@ -602,7 +692,7 @@ __device__ inline unsigned plastic_cell(const Level& lv, const R* al, const R* a
         R J, Jn;
         if (M.solid) {
             J = tsqrt(hj2);
-            if (shearG(M) * J * s3 > M.yield) Jn = tdiv(M.yield, s3 * shearG(M));
+            if (shearG<RW>(M, rk[m]) * J * s3 > M.yield) Jn = tdiv(M.yield, s3 * shearG<RW>(M, rk[m]));
             else Jn = J;
             e[m] += s23 * (J - Jn);
             are[m] = e[m] * ar[m];

@@ -35,7 +35,7 @@ template <int NM, typename R>
 __device__ __forceinline__ void flip_s(SSt<NM, R>& S, int d) { S.u[d] = -S.u[d]; }
 
 // x pass of the solid path (same wave layout as k_flux_x, no THINC).
-template <int NM, typename R = double>
+template <int NM, bool RW, typename R>
 __global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, FacesT<R> F, Clock* clk) {
     if (clk->done) return;
     constexpr int PER = 63;
@@ -58,8 +58,8 @@ __global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, FacesT<R> F, C
         load_sprim<NM, R>(lv, U, s, j, k, Pc);
         load_sprim<NM, R>(lv, U, s + 1, j, k, Pp);
         muscl_s<NM, R>(Pm, Pc, Pp, PL, PR);
-        p2c_sound_s<NM, R>(lv, PL, 0, SLs);
-        p2c_sound_s<NM, R>(lv, PR, 0, SRs);
+        p2c_sound_s<NM, RW, R>(lv, PL, 0, SLs);
+        p2c_sound_s<NM, RW, R>(lv, PR, 0, SRs);
     }
     SSt<NM, R> Lf = shfl_state<NM, R>(SRs, lane > 0 ? lane - 1 : 0);
     unsigned st = 0;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, FacesT<R> F, C
         SSt<NM, R> Rf = SLs;
         if (c == nx && lv.bc_hi[0]) flip_s<NM, R>(Rf, 0);
         R Fv[NF];
-        wave5<NM, R>(lv, Lf, Rf, 0, Fv, st);
+        wave5<NM, RW, R>(lv, Lf, Rf, 0, Fv, st);
 #pragma unroll
         for (int q = 0; q < NF; ++q) F.at(q, c, j, k) = Fv[q];
     }
@@ -134,7 +134,7 @@ __device__ __forceinline__ void sst_load(const R* sh, int lane, SSt<NM, R>& S) {
 // against the rows of all resident waves): 565 B/cell of HBM traffic against
 // 336, which made the FP32 (mixed) pass bandwidth-bound (measured: FP32 pass
 // 1.6 -> 1.1 ms, FP64 3.2 -> 3.0 ms on 256^3).
-template <int NM, int D, typename R>
+template <int NM, int D, bool RW, typename R>
 __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U, const FacesT<R>& F, int seglen, Clock* clk) {
     if (clk->done) return;
     constexpr int NF = 3 * NM + 17;
@@ -172,12 +172,12 @@ __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U,
         load_sprim<NM, R>(lv, U, i, cj(r), ck(r), Pc);
         load_sprim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), Pp);
         muscl_s<NM, R>(Pm, Pc, Pp, PL, PR);
-        p2c_sound_s<NM, R>(lv, PL, D, SL);
-        p2c_sound_s<NM, R>(lv, PR, D, SR);
+        p2c_sound_s<NM, RW, R>(lv, PL, D, SL);
+        p2c_sound_s<NM, RW, R>(lv, PR, D, SR);
     };
     auto emit = [&](int face, const SSt<NM, R>& A, const SSt<NM, R>& B) {
         R Fv[NF];
-        wave5<NM, R>(lv, A, B, D, Fv, st);
+        wave5<NM, RW, R>(lv, A, B, D, Fv, st);
         if (!act) return;
         const long long fi = faceidx(face);
 #pragma unroll
@@ -195,8 +195,8 @@ __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U,
             SPrim<NM, R> W2, PL, PR;
             load_sprim<NM, R>(lv, U, i, cj(r + 1), ck(r + 1), W2);
             muscl_s<NM, R>(W0, W1, W2, PL, PR);
-            p2c_sound_s<NM, R>(lv, PL, D, SLs);
-            p2c_sound_s<NM, R>(lv, PR, D, SRs);
+            p2c_sound_s<NM, RW, R>(lv, PL, D, SLs);
+            p2c_sound_s<NM, RW, R>(lv, PR, D, SRs);
             W0 = W1;
             W1 = W2;
         } else {
@@ -222,9 +222,9 @@ __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U,
     }
     if (st) atomicOr(&clk->status, st);
 }
-template <int NM, int D, typename R = double>
+template <int NM, int D, bool RW, typename R>
 __global__ __launch_bounds__(256) void k_flux_lmarch_s(Level lv, MF U, FacesT<R> F, int seglen, Clock* clk) {
-    flux_lmarch_s_body<NM, D, R>(lv, U, F, seglen, clk);
+    flux_lmarch_s_body<NM, D, RW, R>(lv, U, F, seglen, clk);
 }
 
 // Per-cell solid state held by the update kernel.
@@ -293,12 +293,12 @@ __device__ __forceinline__ void store_scell(const Level& lv, const MF& U, int i,
 // conservativeToPrimitive that ends plasticUpdate: sigma from p and the
 // deviatoric Hencky strain dH (stressTensor, boxaccesscellarray.cpp:176-203).
 // V rows 0 and 1 take +1/3 V u_x, row 2 -2/3 V u_x.
-template <int NM>
+template <int NM, bool RW>
 __device__ __forceinline__ void radial_solid(const Level& lv, double f, SCell<NM>& C, const double* dH) {
     double s0[3], s1[3], s2[3];
-    stress_col<NM>(lv, C.al, C.p, dH, 0, s0);
-    stress_col<NM>(lv, C.al, C.p, dH, 1, s1);
-    stress_col<NM>(lv, C.al, C.p, dH, 2, s2);
+    stress_col<NM, RW>(lv, C.al, C.rk, C.p, dH, 0, s0);
+    stress_col<NM, RW>(lv, C.al, C.rk, C.p, dH, 1, s1);
+    stress_col<NM, RW>(lv, C.al, C.rk, C.p, dH, 2, s2);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         C.ar[m] += f * (C.ar[m] * C.u[0]);
@@ -312,11 +312,11 @@ __device__ __forceinline__ void radial_solid(const Level& lv, double f, SCell<NM
 }
 // plasticUpdate's conservativeToPrimitive leaves p and dH of the state; a
 // stand-alone source recomputes dH from V (hencky does what getHenckyJ2 does).
-template <int NM>
+template <int NM, bool RW>
 __device__ __forceinline__ void radial_solid_c2p(const Level& lv, double f, SCell<NM>& C, double* dH, bool have_dH) {
     if (!have_dH) hencky(C.V, dH);
-    radial_solid<NM>(lv, f, C, dH);
-    C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+    radial_solid<NM, RW>(lv, f, C, dH);
+    C.hj2 = c2p_s<NM, RW>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
 }
 
 // update (hllc.cpp:493-523, with the V rows of :511-514) in x, y, z order,
@@ -324,7 +324,7 @@ __device__ __forceinline__ void radial_solid_c2p(const Level& lv, double f, SCel
 // plasticUpdate (main.cpp:154-157, plastic.cpp:227-244), the geometric source
 // (main.cpp:159-162), cleanUpV (main.cpp:164-167), cleanUpAlpha
 // (main.cpp:169-171), the NaN scan and the next getTimeStep.
-template <int NM, int STAGE, typename T = double>
+template <int NM, int STAGE, bool RW, typename T>
 __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo, FacesT<T> F0, FacesT<T> F1, FacesT<T> F2,
                                                   Clock* clk, unsigned ops) {
     if (clk->done) return;
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
                 }
         }
         double dH[9];
-        C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+        C.hj2 = c2p_s<NM, RW>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
         if (STAGE == 2) {
             SCell<NM> B;
             load_scell<NM, true>(lv, Ub, i, j, k, B);
@@ -399,10 +399,10 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
             for (int q = 0; q < 9; ++q) C.V[q] = B.V[q] * (1.0 / 2.0) + C.V[q] * (1.0 / 2.0);
             C.hj2 = B.hj2 * (1.0 / 2.0) + C.hj2 * (1.0 / 2.0);
             if (ops & 1u) {  // plastic relaxation, then conservativeToPrimitive
-                st |= plastic_cell<NM>(lv, C.al, C.ar, C.e, C.are, C.V);
-                C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+                st |= plastic_cell<NM, RW>(lv, C.al, C.ar, C.rk, C.e, C.are, C.V);
+                C.hj2 = c2p_s<NM, RW>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
             }
-            if (ops & 32u) radial_solid_c2p<NM>(lv, radial_factor(lv, i, dt), C, dH, (ops & 1u) != 0);
+            if (ops & 32u) radial_solid_c2p<NM, RW>(lv, radial_factor(lv, i, dt), C, dH, (ops & 1u) != 0);
             if (ops & 2u) cleanup_v<NM, T>(lv, C.al, C.V);  // FP32 in the mixed form
             if (ops & 4u) st |= cleanup_alpha<NM>(C.al);
             if (ops & 16u) {
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
                 if (bad) st |= 1u;
             }
             if (ops & 8u) {
-                const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
+                const double a = sound_s<NM, RW>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
                 put_state<STAGE == 2>(Uo.at(lv.cs[SOUNDSPEED], i, j, k), a);
                 #pragma unroll
                 for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
     if (st) atomicOr(&clk->status, st);
 }
 
-template <int NM>
+template <int NM, bool RW>
 __global__ __launch_bounds__(256) void k_sound_dt_s(Level lv, MF U, Clock* clk) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(256) void k_sound_dt_s(Level lv, MF U, Clock* clk) 
         const int k = (int)(id / ((long long)nx * ny));
         SCell<NM> C;
         load_scell<NM>(lv, U, i, j, k, C);
-        const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
+        const double a = sound_s<NM, RW>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
         U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
         #pragma unroll
         for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(256) void k_sound_dt_s(Level lv, MF U, Clock* clk) 
 // setInitialConditions for solid levels: thermal -> total pressure shift
 // (initialise.cpp:104, with the HJ2 component as given) then
 // primitiveToConservative (which computes HJ2 and devH).
-template <int NM>
+template <int NM, bool RW>
 __global__ __launch_bounds__(256) void k_init_p2c_s(Level lv, MF U, int shift) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -482,12 +482,12 @@ __global__ __launch_bounds__(256) void k_init_p2c_s(Level lv, MF U, int shift) {
         double ar[NM], G, Ent, Pnt;
 #pragma unroll
         for (int m = 0; m < NM; ++m) ar[m] = P.a[m] * P.r[m];
-        eos_sums_s<NM>(lv, P.a, ar, P.r, U.at(lv.cs[HJ2], i, j, k), G, Ent, Pnt);
+        eos_sums_s<NM, RW>(lv, P.a, ar, P.r, U.at(lv.cs[HJ2], i, j, k), G, Ent, Pnt);
         P.p += Pnt / G;
         U.at(lv.cs[PRESSURE], i, j, k) = P.p;
     }
     SSt<NM> S;
-    p2c_sound_s<NM>(lv, P, 0, S);
+    p2c_sound_s<NM, RW>(lv, P, 0, S);
     double dH[9];
     const double hj2 = hencky(P.V, dH);
 #pragma unroll
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(256) void k_init_p2c_s(Level lv, MF U, int shift) {
 
 // derived components of a solid state: devH, HJ2 from V and sigma
 // (boxaccesscellarray.cpp:176-216, 326-380)
-template <int NM>
+template <int NM, bool RW>
 __global__ __launch_bounds__(256) void k_derive_s(Level lv, MF U) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -514,11 +514,14 @@ __global__ __launch_bounds__(256) void k_derive_s(Level lv, MF U) {
     const int i = (int)(id % nx);
     const int j = (int)((id / nx) % ny);
     const int k = (int)(id / ((long long)nx * ny));
-    double V[9], dH[9], al[NM];
+    double V[9], dH[9], al[NM], rk[NM];
 #pragma unroll
     for (int q = 0; q < 9; ++q) V[q] = U.at(lv.cs[V_TENSOR] + q, i, j, k);
 #pragma unroll
-    for (int m = 0; m < NM; ++m) al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+    for (int m = 0; m < NM; ++m) {
+        al[m] = U.at(lv.cs[ALPHA] + m, i, j, k);
+        rk[m] = U.at(lv.cs[RHO_K] + m, i, j, k);
+    }
     const double hj2 = hencky(V, dH);
     const double p = U.at(lv.cs[PRESSURE], i, j, k);
     U.at(lv.cs[HJ2], i, j, k) = hj2;
@@ -526,7 +529,7 @@ __global__ __launch_bounds__(256) void k_derive_s(Level lv, MF U) {
     for (int q = 0; q < 9; ++q) U.at(lv.cs[DEVH] + q, i, j, k) = dH[q];
     for (int d = 0; d < 3; ++d) {
         double sig[3];
-        stress_col<NM>(lv, al, p, dH, d, sig);
+        stress_col<NM, RW>(lv, al, rk, p, dH, d, sig);
         for (int r = 0; r < 3; ++r) U.at(lv.cs[SIGMA] + 3 * r + d, i, j, k) = sig[r];
     }
 }
@@ -536,7 +539,7 @@ __global__ __launch_bounds__(256) void k_derive_s(Level lv, MF U) {
 namespace tj {
 
 // conservativeToPrimitive over the valid cells of a solid level.
-template <int NM>
+template <int NM, bool RW>
 __global__ __launch_bounds__(256) void k_c2p_s(Level lv, MF U) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -547,7 +550,7 @@ __global__ __launch_bounds__(256) void k_c2p_s(Level lv, MF U) {
     SCell<NM> C;
     load_scell<NM>(lv, U, i, j, k, C);
     double dH[9];
-    C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+    C.hj2 = c2p_s<NM, RW>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
     store_scell<NM>(lv, U, i, j, k, C);
 }
 
@@ -555,7 +558,7 @@ __global__ __launch_bounds__(256) void k_c2p_s(Level lv, MF U) {
 // (tj_post_step): plastic + c2p, the geometric source + c2p (with the clock's
 // dt), cleanUpV, cleanUpAlpha, NaN scan, sound speed and the CFL minimum, each
 // when its bit is set.
-template <int NM>
+template <int NM, bool RW>
 __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsigned ops) {
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -570,10 +573,10 @@ __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsi
         load_scell<NM>(lv, U, i, j, k, C);
         double dH[9];
         if (ops & 1u) {
-            st |= plastic_cell<NM>(lv, C.al, C.ar, C.e, C.are, C.V);
-            C.hj2 = c2p_s<NM>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+            st |= plastic_cell<NM, RW>(lv, C.al, C.ar, C.rk, C.e, C.are, C.V);
+            C.hj2 = c2p_s<NM, RW>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
         }
-        if (ops & 32u) radial_solid_c2p<NM>(lv, radial_factor(lv, i, clk->dt), C, dH, (ops & 1u) != 0);
+        if (ops & 32u) radial_solid_c2p<NM, RW>(lv, radial_factor(lv, i, clk->dt), C, dH, (ops & 1u) != 0);
         if (ops & 2u) cleanup_v<NM>(lv, C.al, C.V);
         if (ops & 4u) st |= cleanup_alpha<NM>(C.al);
         if (ops & 16u) {
@@ -590,7 +593,7 @@ __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsi
             if (bad) st |= 1u;
         }
         if (ops & 8u) {
-            const double a = sound_s<NM>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
+            const double a = sound_s<NM, RW>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
             U.at(lv.cs[SOUNDSPEED], i, j, k) = a;
             #pragma unroll
             for (int d = 0; d < 3; ++d)  // unrolled: constant indices keep the state in registers

@@ -131,17 +131,17 @@ __global__ __launch_bounds__(256) void k_tflux_ts(Level lv, DMF<K> U, Faces F, C
         at(s + 1, Pp);
         muscl_s<NM, T>(Pm, Pc, Pp, PL, PR);
         SSt<NM, T> SRs;
-        p2c_sound_s<NM, T>(lv, PR, D, SRs);
+        p2c_sound_s<NM, false, T>(lv, PR, D, SRs);
         Lf = shfl_state<NM, T>(SRs, (c == 0 || ld == 0) ? lane : lane - LT);
     }
     if (c == 0 && lv.bc_lo[D]) flip_s<NM, T>(Lf, D);
     SSt<NM, T> Rf;
-    p2c_sound_s<NM, T>(lv, PL, D, Rf);
+    p2c_sound_s<NM, false, T>(lv, PL, D, Rf);
     if (c == nd && lv.bc_hi[D]) flip_s<NM, T>(Rf, D);
     unsigned st = 0;
     if (act && ld >= 1 && c >= 0 && c <= nd) {
         T Fv[NF];
-        wave5<NM, T>(lv, Lf, Rf, D, Fv, st);
+        wave5<NM, false, T>(lv, Lf, Rf, D, Fv, st);
         const int fi = D == 0 ? c : i, fj = D == 1 ? c : j, fk = D == 2 ? c : k;
         const long long id = ((long long)fk * F.ny + fj) * F.nx + fi;
 #pragma unroll
@@ -211,7 +211,7 @@ template <int NM, int K>
 __device__ __forceinline__ double tsound_cand(const Level& lv, const DMF<K>& U, const TCell<NM, K>& C, int i, int j,
                                               int k, long long id, double* cand) {
     using T = Dual<K>;
-    const T a = sound_s<NM, T>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
+    const T a = sound_s<NM, false, T>(lv, C.al, C.ar, C.rk, C.rho, C.p, C.hj2);
     U.put(lv.cs[SOUNDSPEED], i, j, k, a);
     T c = T(__builtin_huge_val());
 #pragma unroll
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void k_tupdate_s(Level lv, DMF<K> Uin, DMF<K> 
                 }
         }
         T dH[9];
-        C.hj2 = c2p_s<NM, T>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+        C.hj2 = c2p_s<NM, false, T>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
         if (STAGE == 2) {
             TCell<NM, K> B;
             load_tcell<NM, K, true>(lv, Ub, i, j, k, B);
@@ -312,8 +312,8 @@ __global__ __launch_bounds__(256) void k_tupdate_s(Level lv, DMF<K> Uin, DMF<K> 
             for (int q = 0; q < 9; ++q) C.V[q] = B.V[q] * (1.0 / 2.0) + C.V[q] * (1.0 / 2.0);
             C.hj2 = B.hj2 * (1.0 / 2.0) + C.hj2 * (1.0 / 2.0);
             if (ops & 1u) {
-                st |= plastic_cell<NM, T>(lv, C.al, C.ar, C.e, C.are, C.V);
-                C.hj2 = c2p_s<NM, T>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+                st |= plastic_cell<NM, false, T>(lv, C.al, C.ar, C.rk, C.e, C.are, C.V);
+                C.hj2 = c2p_s<NM, false, T>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
             }
             if (ops & 2u) cleanup_v<NM, T>(lv, C.al, C.V);
             if (ops & 4u) st |= cleanup_alpha<NM, T>(C.al);
@@ -394,12 +394,12 @@ __global__ __launch_bounds__(256) void k_tinit_p2c_s(Level lv, DMF<K> U, int shi
         T ar[NM], G, Ent, Pnt;
 #pragma unroll
         for (int m = 0; m < NM; ++m) ar[m] = P.a[m] * P.r[m];
-        eos_sums_s<NM, T>(lv, P.a, ar, P.r, U.get(lv.cs[HJ2], i, j, k), G, Ent, Pnt);
+        eos_sums_s<NM, false, T>(lv, P.a, ar, P.r, U.get(lv.cs[HJ2], i, j, k), G, Ent, Pnt);
         P.p += tdiv(Pnt, G);
         U.put(lv.cs[PRESSURE], i, j, k, P.p);
     }
     SSt<NM, T> S;
-    p2c_sound_s<NM, T>(lv, P, 0, S);
+    p2c_sound_s<NM, false, T>(lv, P, 0, S);
     T dH[9];
     const T hj2 = hencky(P.V, dH);
 #pragma unroll

@@ -84,6 +84,14 @@ void fill_mats(const tj_tangent* tg, int first, int cnt, MatT<Dual<K>>* M) {
         d.pref2 = s.pref2;
         d.eref2 = s.eref2;
         d.CV2 = s.CV2;
+        d.K0 = s.K0;
+        d.ea = s.ea;
+        d.eb = s.eb;
+        d.e1 = s.e1;
+        d.e2 = s.e2;
+        d.e3 = s.e3;
+        d.e4 = s.e4;
+        d.e5 = s.e5;
     }
     for (int q = 0; q < cnt; ++q) {
         const int m = tg->mat[first + q];
@@ -257,6 +265,11 @@ int tj_tangent_create(tj_ctx* c, int ndir, const int* mat, const int* param, int
         c->err = "tangent: chunk must be 0, 1, 2, 3 or 6";
         return TJ_EINVAL;
     }
+    for (int m = 0; m < c->nmat; ++m)
+        if (c->lv.m[m].eos >= 2) {
+            c->err = "tangent: the Romenskii and Wilkins solid EOSs are not part of the tangent-linear form";
+            return TJ_EUNSUPPORTED;
+        }
     for (int q = 0; q < ndir; ++q) {
         const int m = mat[q], p = param[q];
         if (m < 0 || m >= c->nmat || p < TJ_TP_GAMMA0 || p > TJ_TP_PREF) {

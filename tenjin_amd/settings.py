@@ -9,9 +9,12 @@ optionally a mixture (mixture = 1: MixtureEOS with the products'
 adiabaticIndexmix, prefmix, erefmix, CVmix, initialise.cpp:183-193, and per
 state lambda and rhoa, :245-249); solid materials with EOS "UsUpSolid" (rho0,
 Gamma0, c0, s, G0, yieldStress; this project's Us-up Mie-Gruneisen solid,
-which has no reference class); RADIAL and REACTIVE (initialise.cpp:332-335)
-switch the geometric and the reactive source on.  A second mixture and the
-Romenskii and Wilkins solids are rejected with an error.  Host-side
+which has no reference class); solid materials with EOS "RomenskiiSolid"
+(adiabaticIndex, pref, eref, CV, rho0, K0, EOSalpha, EOSbeta, G0,
+yieldStress) and "WilkinsSolid" (adiabaticIndex, pref, eref, CV, rho0,
+e1 .. e5, G0, yieldStress), the keys of initialise.cpp:149-172, 205-215;
+RADIAL and REACTIVE (initialise.cpp:332-335) switch the geometric and the
+reactive source on.  A second mixture is rejected with an error.  Host-side
 set-up only.
 """
 from __future__ import annotations
@@ -172,6 +175,20 @@ def problem_from_settings(cfg, max_grid_size: int = 64, dim: Optional[int] = Non
         elif eos == "UsUpSolid" and phase == "solid":
             mats.append(Material.usup_solid(float(md["rho0"]), float(md["Gamma0"]), float(md["c0"]), float(md["s"]),
                                             float(md["G0"]), float(md.get("yieldStress", 0.0)), bool(plastic_run)))
+        elif eos in ("RomenskiiSolid", "WilkinsSolid"):
+            # getMaterialParameters (initialise.cpp:125-235): a solid EOS cannot be a
+            # mixture; G0 (and yieldStress on PLASTIC runs) come with phase "solid"
+            if phase != "solid":
+                raise ValueError(f"settings: material {m}: {eos} needs material = \"solid\"")
+            base = (float(md["adiabaticIndex"]), float(md["pref"]), float(md["eref"]), float(md["CV"]))
+            Y = float(md["yieldStress"]) if plastic_run else 0.0
+            if eos == "RomenskiiSolid":
+                mats.append(Material.romenskii_solid(*base, float(md["rho0"]), float(md["K0"]), float(md["EOSalpha"]),
+                                                     float(md["EOSbeta"]), float(md["G0"]), Y, bool(plastic_run)))
+            else:
+                mats.append(Material.wilkins_solid(*base, float(md["rho0"]),
+                                                   tuple(float(md[f"e{q}"]) for q in range(1, 6)), float(md["G0"]), Y,
+                                                   bool(plastic_run)))
         else:
             raise ValueError(f"settings: material {m}: EOS {eos!r} for a {phase} is not part of this path")
     low = [int(v) for v in cfg.get("lowBoundary", [0, 0, 0])] + [0, 0, 0]

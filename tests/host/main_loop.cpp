@@ -80,6 +80,14 @@ int main(int argc, char** argv) {
         const tj_material& t = desc.mat[m];
         if (t.eos == 1) {
             eos.emplace_back(new UsUpSolidEOS(t.rho0, t.gamma - 1.0, t.c0, t.s, t.G0, t.yield_stress));
+        } else if (t.eos == 2) {  // initialise.cpp:149-160 then G0 (:209)
+            eos.emplace_back(new RomenskiiSolidEOS());
+            eos.back()->define({t.gamma, t.pref, t.eref, t.CV, t.rho0, t.K0, t.eos_alpha, t.eos_beta, t.G0});
+            eos.back()->yieldStress = t.yield_stress;
+        } else if (t.eos == 3) {  // initialise.cpp:161-172 then G0
+            eos.emplace_back(new WilkinsSolidEOS());
+            eos.back()->define({t.gamma, t.pref, t.eref, t.CV, t.rho0, t.e1, t.e2, t.e3, t.e4, t.e5, t.G0});
+            eos.back()->yieldStress = t.yield_stress;
         } else if (t.mixture) {  // initialise.cpp:183-193: MixtureEOS::define with the eight numbers
             auto* mx = new MixtureEOS();
             mx->define({t.gamma, t.pref, t.eref, t.CV, t.gamma_mix, t.pref_mix, t.eref_mix, t.CV_mix});

@@ -171,6 +171,17 @@ def test_plate_impact_stepwise(dim, n, box, steps):
     _stepwise(configs.plate_impact(dim=dim, n=n, box=box), steps)
 
 
+@pytest.mark.parametrize("dim,n,box,steps", [(2, 48, 24, 12), (3, 20, 10, 6)])
+def test_plate_impact_romenskii_wilkins_stepwise(dim, n, box, steps):
+    """The reference's other solid EOSs (RomenskiiSolidEOS copper flyer,
+    WilkinsSolidEOS aluminium target, equationofstate.cpp:403-596): density-
+    dependent shear modulus in the stress, the 5-wave solver and the radial
+    return, bit-exact per step."""
+    from tenjin_amd import configs
+
+    _stepwise(configs.plate_impact_rw(dim=dim, n=n, box=box), steps)
+
+
 def test_plate_impact_3d_full_size_invariants():
     """256^3 elastoplastic impact, 3 device-resident steps: no status bit,
     det V = 1 after cleanUpV, plastic strain non-negative, mass positive."""

@@ -84,7 +84,11 @@ def test_custom_geometry_and_rejections():
     assert pb.dim == 2
     assert pb.prims[pb.layout.comp(P)][0, 0, 0] == 1e9 and pb.prims[pb.layout.comp(P)][0, 7, 0] == 1e5
     bad = settings.parse(DECK)
-    bad["listOfMaterials"][0]["EOS"] = "RomenskiiSolid"
+    bad["listOfMaterials"][0]["EOS"] = "RomenskiiSolid"  # a solid EOS on a fluid material
+    with pytest.raises(ValueError):
+        settings.problem_from_settings(bad)
+    bad = settings.parse(DECK)
+    bad["listOfMaterials"][0]["EOS"] = "Tillotson"  # not an EOS of the reference
     with pytest.raises(ValueError):
         settings.problem_from_settings(bad)
     bad = settings.parse(DECK)
@@ -139,3 +143,37 @@ def test_solid_states_take_the_last_solid_F():
         B = F @ F.T
         assert np.allclose(V @ V.T, B / np.linalg.det(B) ** (1.0 / 3.0), rtol=0, atol=1e-14)
         assert np.allclose(V, V.T, atol=1e-15)
+
+
+ROMENSKII_WILKINS_DECK = """
+numberOfxCells = 8;  numberOfyCells = 1;  numberOfzCells = 1;
+CFL = 0.5;  materials = 2;  states = 2;  interface = 0.5;
+SOLID = 1; THINC = 0; PLASTIC = 1; MUSCL = 1;
+listOfMaterials = (
+  { material = "solid"; mixture = 0; EOS = "RomenskiiSolid"; adiabaticIndex = 3.0; pref = 0.0; eref = 0.0; CV = 390.0;
+    rho0 = 8930.0; K0 = 1.36e11; EOSalpha = 1.3; EOSbeta = 2.2; G0 = 4.5e10; yieldStress = 9.0e7; },
+  { material = "solid"; mixture = 0; EOS = "WilkinsSolid"; adiabaticIndex = 3.0; pref = 0.0; eref = 0.0; CV = 900.0;
+    rho0 = 2703.0; e1 = 3.8e10; e2 = 3.8e10; e3 = -6.46e10; e4 = -1.14e10; e5 = 2.28e10; G0 = 2.6e10;
+    yieldStress = 3.0e8; }
+);
+listOfStates = (
+  { material = ( { alpha = 0.999999; F = [1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]; },
+                 { alpha = 0.000001; F = [1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]; } );
+    p = 0.0; u = 400.0; v = 0.0; w = 0.0; },
+  { material = ( { alpha = 0.000001; F = [1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]; },
+                 { alpha = 0.999999; F = [1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]; } );
+    p = 0.0; u = 0.0; v = 0.0; w = 0.0; }
+);
+"""
+
+
+def test_romenskii_and_wilkins_decks():
+    """The reference's keys for RomenskiiSolid and WilkinsSolid
+    (initialise.cpp:149-172) load into EOS kinds 2 and 3 with every constant."""
+    pb = settings.problem_from_settings(ROMENSKII_WILKINS_DECK)
+    r, w = pb.materials
+    assert (r.eos, r.solid, r.plastic, r.gamma, r.CV, r.rho0, r.K0, r.eos_alpha, r.eos_beta, r.G0, r.yield_stress) == (
+        2, 1, 1, 3.0, 390.0, 8930.0, 1.36e11, 1.3, 2.2, 4.5e10, 9.0e7)
+    assert (w.eos, w.rho0, w.e1, w.e2, w.e3, w.e4, w.e5, w.G0, w.yield_stress) == (
+        3, 2703.0, 3.8e10, 3.8e10, -6.46e10, -1.14e10, 2.28e10, 2.6e10, 3.0e8)
+    assert pb.shift_pressure == 1 and pb.SOLID == 1 and pb.PLASTIC == 1
