@@ -98,12 +98,16 @@ enum {
     TJ_ST_WAVESPEED = 2,    /* NaN wave speeds in the 5-wave solver (hllc.cpp:169-200, 367-373) */
     TJ_ST_ALPHA_NANS = 4,   /* "too many Nans" in cleanUpAlpha (boxaccesscellarray.cpp:559-561) */
     TJ_ST_ALPHA_SUM = 8,    /* total alpha <= 0 in cleanUpAlpha (boxaccesscellarray.cpp:583-588) */
-    TJ_ST_PLASTIC_DIV = 16, /* "Divide by zero error" in boxPlasticUpdate (plastic.cpp:220-224) */
+    TJ_ST_PLASTIC_DIV = 16, /* "Divide by zero error" in boxPlasticUpdate (plastic.cpp:188-193) */
     TJ_ST_MIX_BRACKET = 32, /* "Error in root Bisection" in MixtureEOS::rootFind (equationofstate.cpp:313-321) */
     TJ_ST_MIX_ITER = 64,    /* rootFind's bisection reached its bound of 2000 passes (the reference
                                loops without one, equationofstate.cpp:324-353) */
-    TJ_ST_TANGENT_NAN = 128 /* a NaN derivative part in the tangent-linear form (no reference
+    TJ_ST_TANGENT_NAN = 128, /* a NaN derivative part in the tangent-linear form (no reference
                                counterpart; the run goes on) */
+    /* The device time loop stops at the step that raises any of these (t and n
+     * are not advanced), where the reference stops its loop (NaN, main.cpp:
+     * 183-187) or calls exit(1) (the others; bit 64: its loop never ends). */
+    TJ_ST_STOP = 0x7f
 };
 
 /* precision forms */
@@ -256,6 +260,7 @@ int tj_status(tj_ctx* ctx, uint32_t* status, int clear);
  * allocation contents before their first write; zero is used here. */
 int tj_reset_history(tj_ctx* ctx);
 int tj_synchronize(tj_ctx* ctx);
+
 
 /* ---- Batched ensemble -------------------------------------------------
  * Many independent fluid levels (1D and 2D, uneven sizes, each with its own

@@ -539,7 +539,20 @@ void initialiseFromPrimitives(CellArray& U, bool shiftPressure) {
 bool checkStatus(ParameterStruct& parameters) {
     uint32_t st = 0;
     tj_check(parameters, tj_status(dev(parameters).ctx, &st, 1), "tj_status");
-    if (st & TJ_ST_PLASTIC_DIV) std::printf("Divide by zero error\n");  // plastic.cpp:191 (printed, not fatal)
+    if (st & TJ_ST_PLASTIC_DIV) {
+        std::printf("Divide by zero error\n");  // plastic.cpp:188-193
+        std::exit(1);
+    }
+    if (st & TJ_ST_MIX_BRACKET) {
+        std::printf("Error in root Bisection \n");  // equationofstate.cpp:336-343
+        std::exit(1);
+    }
+    if (st & TJ_ST_MIX_ITER) {
+        // the reference's bisection loop (equationofstate.cpp:347-373) has no
+        // bound and never returns here; the device form stops after 2000 passes
+        std::printf("Root finding is taking a while\n");
+        std::exit(1);
+    }
     if (st & TJ_ST_WAVESPEED) {
         std::printf("Nan in Wavespeeds \n");  // hllc.cpp:169-199, 367-372
         std::exit(1);

@@ -149,7 +149,7 @@ __global__ void k_b_clock_advance(const BProb* P, int np) {
     if (p >= np) return;
     Clock* clk = P[p].clk;
     if (clk->done) return;
-    if (clk->status & 1u) {
+    if (clk->status & ST_STOP) {
         clk->done = 1;
         return;
     }

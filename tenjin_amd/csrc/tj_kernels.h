@@ -13,6 +13,11 @@ struct Clock {
     unsigned long long dtmin_bits;  // min over cells of dx/(a+|u|), as ordered bits of a positive double
     unsigned int status;
 };
+// Status bits on which the loop stops at the step that raised them (t and n
+// stay): the NaN stop of main.cpp:183-187 and every condition on which the
+// reference calls exit(1) (5-wave NaN speeds, cleanUpAlpha, the plastic
+// divide, rootFind's bracket) or never returns (rootFind's unbounded loop).
+constexpr unsigned ST_STOP = 0x7fu;
 
 // Partial slots of the CFL minimum (Level::dtpart): kernels with many blocks
 // spread their per-block atomics over DT_PARTS words (block b on word b mod
@@ -23,6 +28,7 @@ constexpr unsigned long long DT_NONE = 0x7FEFFFFFFFFFFFFFull;  // std::numeric_l
 __device__ __forceinline__ unsigned long long* dt_slot(const Level& lv, Clock* clk) {
     return lv.dtpart ? lv.dtpart + (blockIdx.x % DT_PARTS) : &clk->dtmin_bits;
 }
+
 
 // ------------------------------------------------------------------ x pass
 // One wave covers a row segment; lane l holds cell c = f0 - H + l.  Lanes
@@ -842,7 +848,7 @@ __global__ void k_dtpart_reset(unsigned long long* part) {
 __global__ void k_clock_advance(Clock* clk) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (clk->done) return;
-    if (clk->status & 1u) {
+    if (clk->status & ST_STOP) {
         clk->done = 1;
         return;
     }

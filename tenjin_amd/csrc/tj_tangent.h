@@ -474,7 +474,7 @@ template <int K>
 __global__ void k_tclock_advance(Clock* clk, const double* tdt, double* tt) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (clk->done) return;
-    if (clk->status & 1u) {
+    if (clk->status & ST_STOP) {
         clk->done = 1;
         return;
     }
