@@ -71,15 +71,37 @@ typedef struct tj_level_desc {
                                 the mixture material */
 } tj_level_desc;
 
-/* Non-owning view of a CellArray on the device: component-major, i fastest.
- * data points at component 0 of valid cell (0,0,0); strides in doubles, so a
- * buffer with AMReX ghost cells is described by its own strides. */
+typedef struct tj_fabset tj_fabset;
+
+/* Non-owning view of a CellArray on the device, in one of two forms:
+ *   - fabs == NULL: one array over the level, component-major, i fastest.
+ *     data points at component 0 of valid cell (0,0,0); strides in doubles,
+ *     so a single ghost-grown FAB is described by its own strides;
+ *   - fabs != NULL: a MultiFab of one FAB per box (tj_fabset_create); data
+ *     and the strides are not read.
+ * Every entry point taking tj_mf accepts either form; the views of one call
+ * must all be of the same form. */
 typedef struct tj_mf {
     double* data;
     long long jstride, kstride, nstride;
     int ncomp;
     int pad_;
+    const tj_fabset* fabs;
 } tj_mf;
+
+/* One FAB of a MultiFab (an AMReX FArrayBox as its Array4 sees it: base
+ * pointer, first stored index and strides): device memory holding the cells glo ..
+ * glo + extent - 1 of the level's index space, component-major.  lo/hi is the
+ * box it owns (a box of the level's BoxArray::maxSize decomposition, inclusive);
+ * the stored cells beyond it are its ghost cells (glo = lo - nGrow for an
+ * AMReX FAB). */
+typedef struct tj_fab {
+    double* data;                         /* component 0 of stored cell glo */
+    int lo[3], hi[3];                     /* the valid box (level indices) */
+    int glo[3];                           /* first stored index per dimension */
+    int pad_;
+    long long jstride, kstride, nstride;  /* in doubles */
+} tj_fab;
 
 typedef struct tj_ctx tj_ctx;
 
@@ -219,7 +241,8 @@ int tj_run(tj_ctx* ctx, tj_mf* S, tj_mf* T, double final_time, int max_steps, in
  * TJ_STATIC_SCRATCH_BYTES that the library holds on each GPU it runs on until
  * the process ends; the call allocates nothing, on the device or the host.
  *   - S: the initial state (after tj_init_from_prims), advanced in place; T: a
- *     distinct stage buffer; both caller-owned views of the bound layout.
+ *     distinct stage buffer; both caller-owned level arrays of the bound
+ *     layout (a tj_fabset view needs a context: TJ_EINVAL).
  *   - Work runs on `stream` (a hipStream_t, NULL for the null stream) on the
  *     current device, with the fixed marching segment length 16; results are
  *     bit-identical to tj_bind_level + tj_run (every segment length gives the
@@ -261,6 +284,50 @@ int tj_status(tj_ctx* ctx, uint32_t* status, int clear);
 int tj_reset_history(tj_ctx* ctx);
 int tj_synchronize(tj_ctx* ctx);
 
+/* A MultiFab as one FAB per box: the library reads and writes each cell in
+ * the FAB that owns it, at that FAB's strides (no copy into a level array;
+ * the FAB buffers stay the caller's).  Replaces the per-box Array4 access of
+ * BoxAccessCellArray(MFIter&, const Box&, CellArray&) and
+ * BoxAccessCellArray(const Box&, FArrayBox&, CellArray&) (cellarray.h:46-72,
+ * boxaccesscellarray.cpp:3-23) for the whole level at once.
+ *   - the boxes must be exactly the bound level's BoxArray::maxSize
+ *     decomposition, in any order (TJ_EINVAL otherwise);
+ *   - ghost cells are neither read nor written: a stencil that leaves a box
+ *     reads the neighbouring box's valid cells, and the domain boundary is
+ *     applied on the fly, as for the level array (FillBoundary and
+ *     setBoundaryConditions are part of the kernels);
+ *   - ncomp: components in every FAB (at least tj_ncomp).
+ * The set holds device tables of the FAB records (about 40 B per box); it
+ * belongs to ctx's device and must be destroyed before ctx. */
+int tj_fabset_create(tj_ctx* ctx, int nfab, const tj_fab* fabs, int ncomp, tj_fabset** out);
+/* The bound level's boxes (BoxArray::maxSize of main.cpp:34-36, the cut the
+ * kernels' box-edge semantics follow): *nbox boxes, and with lohi != NULL six
+ * ints per box (lo[3], hi[3]), x fastest.  For allocating FABs that match. */
+int tj_level_boxes(const tj_ctx* ctx, int* nbox, int* lohi);
+/* Cell-wise operations on one box [lo, hi] of the level (inclusive level
+ * indices), the rest of the level untouched: the per-box methods of
+ * BoxAccessCellArray (cellarray.h:75-101: primitiveToConservative,
+ * conservativeToPrimitive, getSoundSpeed, stressTensor + getHenckyJ2 +
+ * getDeviatoricHenckyStrain, cleanUpV, cleanUpAlpha, contains_nan),
+ * TimeStep::boxTimeStepFunction (timestep.cpp:7-34) and
+ * PlasticEOS::boxPlasticUpdate (plastic.cpp:83-225).  ops: TJ_BOX_* and the
+ * cell-wise TJ_OP_* bits (PLASTIC, CLEAN_V, CLEAN_ALPHA, NANCHECK, REACTIVE,
+ * DT), applied in the order P2C, C2P, REACTIVE, PLASTIC, CLEAN_V, CLEAN_ALPHA,
+ * NANCHECK, DERIVE, DT.  With TJ_OP_DT, *dt_out = CFL x the box's minimum of
+ * dx/(|u| + a) (the sound speed written on the way); the level's clock
+ * (t, dt, n, CFL minimum) is left as it was, status bits accumulate as for
+ * any call.  U may be a level array or a tj_fabset view; with a set the box
+ * must lie in one FAB.  TJ_OP_RADIAL needs the cell's radius on the level:
+ * TJ_EUNSUPPORTED.  Synchronises. */
+enum {
+    TJ_BOX_P2C = 1 << 8,     /* primitiveToConservative (no thermal-pressure shift) */
+    TJ_BOX_C2P = 1 << 9,     /* conservativeToPrimitive */
+    TJ_BOX_DERIVE = 1 << 10  /* stress, HJ2 and devH (solid levels; sigma = -p I for fluids) */
+};
+int tj_box_op(tj_ctx* ctx, tj_mf* U, const int lo[3], const int hi[3], unsigned ops, double* dt_out);
+/* A tj_mf naming the set (fabs = set, ncomp = the set's). */
+int tj_fabset_view(const tj_fabset* set, tj_mf* out);
+void tj_fabset_destroy(tj_fabset* set);
 
 /* ---- Batched ensemble -------------------------------------------------
  * Many independent fluid levels (1D and 2D, uneven sizes, each with its own

@@ -18,6 +18,7 @@ from . import configs  # noqa: F401
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtenjin.so")
 
+TJ_BOX_P2C, TJ_BOX_C2P, TJ_BOX_DERIVE = 1 << 8, 1 << 9, 1 << 10
 TJ_PREC_FP64 = 0
 TJ_PREC_MIXED = 1
 TJ_ST_NAN, TJ_ST_WAVESPEED, TJ_ST_ALPHA_NANS, TJ_ST_ALPHA_SUM, TJ_ST_PLASTIC_DIV = 1, 2, 4, 8, 16
@@ -86,6 +87,21 @@ class MFView(ctypes.Structure):
         ("nstride", ctypes.c_longlong),
         ("ncomp", ctypes.c_int),
         ("pad_", ctypes.c_int),
+        ("fabs", ctypes.c_void_p),
+    ]
+
+
+class Fab(ctypes.Structure):
+    """tj_fab: one FAB of a MultiFab of one FAB per box."""
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("lo", ctypes.c_int * 3),
+        ("hi", ctypes.c_int * 3),
+        ("glo", ctypes.c_int * 3),
+        ("pad_", ctypes.c_int),
+        ("jstride", ctypes.c_longlong),
+        ("kstride", ctypes.c_longlong),
+        ("nstride", ctypes.c_longlong),
     ]
 
 
@@ -96,6 +112,7 @@ EXPORTS = [
     "tj_post_step", "tj_step", "tj_run", "tj_init_from_prims", "tj_c2p", "tj_derive", "tj_status",
     "tj_reset_history", "tj_synchronize", "tj_prof_enable", "tj_prof_report", "tj_set_tuning",
     "tj_begin_run", "tj_advance_steps", "tj_lincomb", "tj_graph_capture", "tj_graph_launch",
+    "tj_fabset_create", "tj_fabset_view", "tj_fabset_destroy", "tj_level_boxes", "tj_box_op",
     "tj_batch_create", "tj_batch_destroy", "tj_batch_last_error", "tj_batch_reset_history", "tj_batch_set_seglen",
     "tj_batch_init_from_prims", "tj_batch_begin_run", "tj_batch_advance", "tj_batch_graph_capture",
     "tj_batch_graph_launch", "tj_batch_clocks", "tj_batch_synchronize",
@@ -149,6 +166,12 @@ def load_library(path: str = LIB_PATH):
     L.tj_lincomb.argtypes = [vp, mfp, dp, mfp, dp, mfp]
     L.tj_graph_capture.argtypes = [vp, mfp, mfp, ip, ip]
     L.tj_graph_launch.argtypes = [vp]
+    L.tj_fabset_create.argtypes = [vp, ip, ctypes.POINTER(Fab), ip, ctypes.POINTER(vp)]
+    L.tj_fabset_view.argtypes = [vp, mfp]
+    L.tj_fabset_destroy.argtypes = [vp]
+    L.tj_fabset_destroy.restype = None
+    L.tj_level_boxes.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
+    L.tj_box_op.argtypes = [vp, mfp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.c_uint, ctypes.POINTER(dp)]
     L.tj_batch_create.argtypes = [ctypes.POINTER(vp), ip, ip, ctypes.POINTER(LevelDesc), mfp, mfp]
     L.tj_batch_destroy.argtypes = [vp]
     L.tj_batch_destroy.restype = None
@@ -361,6 +384,119 @@ class GpuLevel:
     def c2p(self):
         """conservativeToPrimitive of S (tj_c2p)."""
         self._chk(self.lib.tj_c2p(self.ctx, ctypes.byref(self.vS)), "tj_c2p")
+
+    def box_op(self, lo, hi, ops: int) -> float:
+        """Cell-wise ops (TJ_BOX_*, TJ_OP_* bits) on the box [lo, hi] of S
+        only (tj_box_op); returns the box's dt with TJ_OP_DT."""
+        L3 = ctypes.c_int * 3
+        dt = ctypes.c_double()
+        self._chk(self.lib.tj_box_op(self.ctx, ctypes.byref(self.vS), L3(*lo), L3(*hi), ops, ctypes.byref(dt)),
+                  "tj_box_op")
+        return dt.value
+
+
+class GpuFabLevel(GpuLevel):
+    """GpuLevel whose S and T are MultiFabs of one FAB per box, as an AMReX
+    MultiFab holds them: each box of the level's maxSize decomposition owns a
+    float64 device tensor [ncomp][nz + 2g][ny + 2g][nx + 2g + pad] (g ghost
+    cells in the active dimensions, `pad` extra doubles per row), and every
+    entry point runs on them through tj_fabset views, with no copy into a
+    level array.  Ghost cells are never read or written by the library; they
+    start as `ghost_fill`.  fabS[b] / fabT[b] and boxes[b] = (lo, hi) are the
+    FABs and their boxes."""
+
+    def __init__(self, pb, device: int = 0, ghost: int = 2, pad: int = 0, ghost_fill: float = float("nan")):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise TenjinError("no GPU visible: the tenjin_amd update runs on the GPU only")
+        self.torch = torch
+        self.lib = load_library()
+        self.pb = pb
+        self.ctx = ctypes.c_void_p()
+        self._sets = []
+        self._chk(self.lib.tj_create(ctypes.byref(self.ctx), device), "tj_create")
+        self._desc = level_desc(pb)
+        self._chk(self.lib.tj_bind_level(self.ctx, ctypes.byref(self._desc)), "tj_bind_level")
+        self.ncomp = self.lib.tj_ncomp(self.ctx)
+        self.shape = pb.shape
+        self.ghost = ghost
+        nb = ctypes.c_int()
+        self._chk(self.lib.tj_level_boxes(self.ctx, ctypes.byref(nb), None), "tj_level_boxes")
+        lohi = (ctypes.c_int * (6 * nb.value))()
+        self._chk(self.lib.tj_level_boxes(self.ctx, ctypes.byref(nb), lohi), "tj_level_boxes")
+        self.boxes = [(tuple(lohi[6 * q:6 * q + 3]), tuple(lohi[6 * q + 3:6 * q + 6])) for q in range(nb.value)]
+        dev = torch.device("cuda", device)
+        self.fabS, self.fabT = [], []
+        for fabs in (self.fabS, self.fabT):
+            recs = (Fab * len(self.boxes))()
+            for q, (lo, hi) in enumerate(self.boxes):
+                g = [ghost if d < pb.dim else 0 for d in range(3)]
+                ext = [hi[d] - lo[d] + 1 + 2 * g[d] for d in range(3)]
+                t = torch.full((self.ncomp, ext[2], ext[1], ext[0] + pad), ghost_fill, dtype=torch.float64, device=dev)
+                fabs.append(t)
+                r = recs[q]
+                r.data = t.data_ptr()
+                for d in range(3):
+                    r.lo[d], r.hi[d], r.glo[d] = lo[d], hi[d], lo[d] - g[d]
+                r.jstride = ext[0] + pad
+                r.kstride = r.jstride * ext[1]
+                r.nstride = r.kstride * ext[2]
+            fs = ctypes.c_void_p()
+            self._chk(self.lib.tj_fabset_create(self.ctx, len(self.boxes), recs, self.ncomp, ctypes.byref(fs)),
+                      "tj_fabset_create")
+            self._sets.append(fs)
+        self.vS, self.vT = MFView(), MFView()
+        self._chk(self.lib.tj_fabset_view(self._sets[0], ctypes.byref(self.vS)), "tj_fabset_view")
+        self._chk(self.lib.tj_fabset_view(self._sets[1], ctypes.byref(self.vT)), "tj_fabset_view")
+
+    def _valid(self, q):
+        lo, hi = self.boxes[q]
+        g = [self.ghost if d < self.pb.dim else 0 for d in range(3)]
+        return (slice(None), slice(g[2], g[2] + hi[2] - lo[2] + 1), slice(g[1], g[1] + hi[1] - lo[1] + 1),
+                slice(g[0], g[0] + hi[0] - lo[0] + 1))
+
+    def _level(self, q):
+        lo, hi = self.boxes[q]
+        return (slice(None), slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
+
+    def set_state(self, arr: np.ndarray):
+        self.torch.cuda.synchronize()
+        src = self.torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(self.fabS[0].device)
+        for q in range(len(self.boxes)):
+            self.fabS[q][self._valid(q)] = src[self._level(q)]
+        self.torch.cuda.synchronize()
+
+    def get_state(self, which: str = "S") -> np.ndarray:
+        self.sync()
+        fabs = self.fabS if which == "S" else self.fabT
+        out = np.empty((self.ncomp,) + self.shape)
+        for q in range(len(self.boxes)):
+            out[self._level(q)] = fabs[q][self._valid(q)].cpu().numpy()
+        return out
+
+    def ghosts(self):
+        """Every FAB's ghost cells (and row padding), flattened: what the
+        library must leave as it found it."""
+        self.sync()
+        parts = []
+        for fabs in (self.fabS, self.fabT):
+            for q, t in enumerate(fabs):
+                m = self.torch.ones_like(t, dtype=self.torch.bool)
+                m[self._valid(q)] = False
+                parts.append(t[m].cpu().numpy())
+        return np.concatenate(parts)
+
+    def init_from_device(self, prims, shift_pressure: int = 0):
+        raise TenjinError("GpuFabLevel: use init_from_prims")
+
+    def close(self):
+        if self.ctx:
+            self.lib.tj_synchronize(self.ctx)
+            for fs in self._sets:
+                self.lib.tj_fabset_destroy(fs)
+            self._sets = []
+        super().close()
 
 
 class GpuBatch:

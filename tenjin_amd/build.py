@@ -42,7 +42,11 @@ FLAGS = [
 
 
 def sources():
-    return [os.path.join(CSRC, "tj_lib.hip"), os.path.join(CSRC, "tj_tangent.hip")]
+    """(source, extra flags, object name): tj_lib.hip twice, the second time
+    with the state view as one FAB per box (TJ_FAB_VIEWS)."""
+    lib = os.path.join(CSRC, "tj_lib.hip")
+    return [(lib, [], "tj_lib.o"), (lib, ["-DTJ_FAB_VIEWS"], "tj_lib_fabs.o"),
+            (os.path.join(CSRC, "tj_tangent.hip"), [], "tj_tangent.o")]
 
 
 def _run(cmd, verbose):
@@ -59,9 +63,9 @@ def build(verbose: bool = True, extra=None) -> str:
     comp = [f for f in FLAGS if f != "-shared"] + list(extra or [])
     with tempfile.TemporaryDirectory(prefix="tenjin_build_") as tmp:
         objs, procs = [], []
-        for src in sources():
-            obj = os.path.join(tmp, os.path.basename(src) + ".o")
-            cmd = [HIPCC] + comp + ["-c", src, "-I", INCLUDE, "-o", obj]
+        for src, defs, name in sources():
+            obj = os.path.join(tmp, name)
+            cmd = [HIPCC] + comp + defs + ["-c", src, "-I", INCLUDE, "-o", obj]
             if verbose:
                 print(" ".join(cmd), flush=True)
             procs.append((cmd, subprocess.Popen(cmd)))

@@ -9,10 +9,13 @@
 //   * BCRec holds AMReX BCType codes (reflect_odd -1, reflect_even 1,
 //     foextrap 2);
 //   * a MultiFab's data are component-major with i fastest (FArrayBox order).
-// They differ where a GPU-resident level wants something else: a MultiFab is
-// one device array over the whole level's valid cells (its boxes are views
-// into it) and stores no ghost cells -- ghost values are mapped on the fly by
-// the kernels -- so nGrow() is recorded but not allocated.
+// They differ where a GPU-resident level wants something else: by default a
+// MultiFab is one device array over the whole level's valid cells (its boxes
+// are views into it) and stores no ghost cells -- ghost values are mapped on
+// the fly by the kernels -- so nGrow() is recorded but not allocated.  With
+// MFInfo().SetFabPerBox() it holds AMReX's own storage instead: one FArrayBox
+// per box, grown by nGrow() cells in the active directions, and the library
+// runs on those FABs in place (tj_fabset).
 #pragma once
 
 #include <array>
@@ -120,29 +123,88 @@ private:
     int hi_[3] = {BCType::foextrap, BCType::foextrap, BCType::foextrap};
 };
 
+// Host view of one FAB's cells (AMReX Array4): p at cell begin, strides in
+// elements; (i, j, k, n) in level indices.
+template <class T>
+struct Array4 {
+    T* p = nullptr;
+    long long jstride = 0, kstride = 0, nstride = 0;
+    IntVect begin, end;  // first cell, one past the last
+    int ncomp = 0;
+    T& operator()(int i, int j, int k, int n = 0) const {
+        return p[(i - begin[0]) + (j - begin[1]) * jstride + (k - begin[2]) * kstride + n * nstride];
+    }
+};
+
+// Storage options of a MultiFab (AMReX's MFInfo carries allocation options;
+// this one names the storage form).
+struct MFInfo {
+    bool fabPerBox = false;
+    MFInfo& SetFabPerBox(bool v = true) {
+        fabPerBox = v;
+        return *this;
+    }
+};
+
+// One box's cell data on the GPU (AMReX FArrayBox): ncomp components over the
+// cells of box() -- the valid box grown by the ghost width --, component-major,
+// i fastest.  host is a staging copy of the whole FAB (copyToHost/FromHost).
+class FArrayBox {
+public:
+    FArrayBox(const Box& grown, int ncomp);
+    ~FArrayBox();
+    FArrayBox(const FArrayBox&) = delete;
+    FArrayBox& operator=(const FArrayBox&) = delete;
+    const Box& box() const { return box_; }
+    int nComp() const { return ncomp_; }
+    Real* dataPtr() const { return data_; }
+    long long jstride() const { return box_.length(0); }
+    long long kstride() const { return (long long)box_.length(0) * box_.length(1); }
+    long long nstride() const { return box_.numPts(); }
+    void copyToHost();
+    void copyFromHost();
+    Vector<Real> host;
+    // Array4 over the host staging copy (after copyToHost)
+    Array4<Real> hostArray();
+
+private:
+    Box box_;
+    int ncomp_ = 0;
+    Real* data_ = nullptr;
+};
+
 // One level's cell data on the GPU: ncomp components over the valid cells of
-// the BoxArray's minimal box, component-major, i fastest.
+// the BoxArray's minimal box, component-major, i fastest -- or, with
+// MFInfo().SetFabPerBox(), one FArrayBox per box.
 class MultiFab {
 public:
     MultiFab() = default;
-    MultiFab(const BoxArray& ba, const DistributionMapping& dm, int ncomp, int ngrow) { define(ba, dm, ncomp, ngrow); }
+    MultiFab(const BoxArray& ba, const DistributionMapping& dm, int ncomp, int ngrow, const MFInfo& info = MFInfo()) {
+        define(ba, dm, ncomp, ngrow, info);
+    }
     ~MultiFab();
     MultiFab(const MultiFab&) = delete;
     MultiFab& operator=(const MultiFab&) = delete;
 
-    void define(const BoxArray& ba, const DistributionMapping& dm, int ncomp, int ngrow);
+    void define(const BoxArray& ba, const DistributionMapping& dm, int ncomp, int ngrow, const MFInfo& info = MFInfo());
+    bool fabPerBox() const { return !fabs_.empty(); }
+    // the FAB of box b (fabPerBox() only)
+    FArrayBox& fab(int b) { return *fabs_.at(b); }
+    const FArrayBox& fab(int b) const { return *fabs_.at(b); }
     int nComp() const { return ncomp_; }
     int nGrow() const { return ngrow_; }
     const BoxArray& boxArray() const { return ba_; }
     const Box& domain() const { return dom_; }
     long long numCells() const { return dom_.numPts(); }
     // device pointer to component 0 of cell (lo, lo, lo); strides in doubles
+    // (the level array; null with fabPerBox())
     Real* devicePtr() const { return data_; }
     long long jstride() const { return dom_.length(0); }
     long long kstride() const { return (long long)dom_.length(0) * dom_.length(1); }
     long long nstride() const { return numCells(); }
 
-    // host staging (synchronous): the whole level, same layout
+    // host staging (synchronous): the whole level's valid cells, level-array
+    // layout, whatever the storage
     void copyToHost(Vector<Real>& h) const;
     void copyFromHost(const Vector<Real>& h);
 
@@ -153,6 +215,7 @@ private:
     Box dom_;
     int ncomp_ = 0, ngrow_ = 0;
     Real* data_ = nullptr;
+    Vector<FArrayBox*> fabs_;
 };
 
 // Iterates over the boxes of a MultiFab's BoxArray.

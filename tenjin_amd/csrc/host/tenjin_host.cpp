@@ -9,6 +9,7 @@
 #include <cerrno>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -96,27 +97,106 @@ Box BoxArray::minimalBox() const {
     return m;
 }
 
-void MultiFab::define(const BoxArray& ba, const DistributionMapping&, int ncomp, int ngrow) {
+// ------------------------------------------------------------------ FArrayBox
+FArrayBox::FArrayBox(const Box& grown, int ncomp) : box_(grown), ncomp_(ncomp) {
+    const size_t n = sizeof(Real) * (size_t)ncomp * (size_t)grown.numPts();
+    hip_check(hipMalloc(&data_, n), "FArrayBox: hipMalloc");
+    hip_check(hipMemset(data_, 0, n), "FArrayBox: hipMemset");
+}
+FArrayBox::~FArrayBox() {
+    if (data_) (void)hipFree(data_);
+}
+void FArrayBox::copyToHost() {
+    host.resize((size_t)ncomp_ * (size_t)nstride());
+    hip_check(hipDeviceSynchronize(), "FArrayBox::copyToHost");
+    hip_check(hipMemcpy(host.data(), data_, sizeof(Real) * host.size(), hipMemcpyDeviceToHost), "FArrayBox::copyToHost");
+}
+void FArrayBox::copyFromHost() {
+    if (host.size() != (size_t)ncomp_ * (size_t)nstride()) throw std::runtime_error("FArrayBox::copyFromHost: stage first");
+    hip_check(hipDeviceSynchronize(), "FArrayBox::copyFromHost");
+    hip_check(hipMemcpy(data_, host.data(), sizeof(Real) * host.size(), hipMemcpyHostToDevice),
+              "FArrayBox::copyFromHost");
+}
+Array4<Real> FArrayBox::hostArray() {
+    Array4<Real> a;
+    a.p = host.data();
+    a.jstride = jstride();
+    a.kstride = kstride();
+    a.nstride = nstride();
+    a.begin = box_.lo;
+    a.end = IntVect(box_.hi[0] + 1, box_.hi[1] + 1, box_.hi[2] + 1);
+    a.ncomp = ncomp_;
+    return a;
+}
+
+// ------------------------------------------------------------------ MultiFab
+void MultiFab::define(const BoxArray& ba, const DistributionMapping&, int ncomp, int ngrow, const MFInfo& info) {
     ba_ = ba;
     dom_ = ba.minimalBox();
     ncomp_ = ncomp;
     ngrow_ = ngrow;
+    if (info.fabPerBox) {
+        // AMReX's storage: each box grown by ngrow in the directions the level
+        // extends in
+        for (int b = 0; b < ba.size(); ++b) {
+            Box g = ba[b];
+            for (int d = 0; d < 3; ++d)
+                if (dom_.length(d) > 1) {
+                    g.lo[d] -= ngrow;
+                    g.hi[d] += ngrow;
+                }
+            fabs_.push_back(new FArrayBox(g, ncomp));
+        }
+        return;
+    }
     hip_check(hipMalloc(&data_, sizeof(Real) * (size_t)ncomp * (size_t)numCells()), "MultiFab: hipMalloc");
     hip_check(hipMemset(data_, 0, sizeof(Real) * (size_t)ncomp * (size_t)numCells()), "MultiFab: hipMemset");
 }
 
 MultiFab::~MultiFab() {
+    for (FArrayBox* f : fabs_) delete f;
     if (data_) (void)hipFree(data_);
+}
+
+// the valid cells of box b between a FAB's host copy and a level-layout vector
+static void fab_level_copy(const Box& vb, FArrayBox& f, Vector<Real>& lvl, const Box& dom, int ncomp, bool to_level) {
+    const long long ljs = dom.length(0), lks = (long long)dom.length(0) * dom.length(1), lns = dom.numPts();
+    const Box& g = f.box();
+    for (int c = 0; c < ncomp; ++c)
+        for (int k = vb.lo[2]; k <= vb.hi[2]; ++k)
+            for (int j = vb.lo[1]; j <= vb.hi[1]; ++j) {
+                Real* fr = f.host.data() + c * f.nstride() + (k - g.lo[2]) * f.kstride() + (j - g.lo[1]) * f.jstride() +
+                           (vb.lo[0] - g.lo[0]);
+                Real* lr = lvl.data() + c * lns + (k - dom.lo[2]) * lks + (j - dom.lo[1]) * ljs + (vb.lo[0] - dom.lo[0]);
+                if (to_level) std::copy(fr, fr + vb.length(0), lr);
+                else std::copy(lr, lr + vb.length(0), fr);
+            }
 }
 
 void MultiFab::copyToHost(Vector<Real>& h) const {
     h.resize((size_t)ncomp_ * (size_t)numCells());
+    if (fabPerBox()) {
+        for (int b = 0; b < ba_.size(); ++b) {
+            fabs_[b]->copyToHost();
+            fab_level_copy(ba_[b], *fabs_[b], h, dom_, ncomp_, true);
+        }
+        return;
+    }
     hip_check(hipDeviceSynchronize(), "MultiFab::copyToHost");
     hip_check(hipMemcpy(h.data(), data_, sizeof(Real) * h.size(), hipMemcpyDeviceToHost), "MultiFab::copyToHost");
 }
 
 void MultiFab::copyFromHost(const Vector<Real>& h) {
     if (h.size() != (size_t)ncomp_ * (size_t)numCells()) throw std::runtime_error("MultiFab::copyFromHost: size");
+    if (fabPerBox()) {
+        Vector<Real>& lvl = const_cast<Vector<Real>&>(h);
+        for (int b = 0; b < ba_.size(); ++b) {
+            fabs_[b]->copyToHost();  // keeps the ghost cells as they are
+            fab_level_copy(ba_[b], *fabs_[b], lvl, dom_, ncomp_, false);
+            fabs_[b]->copyFromHost();
+        }
+        return;
+    }
     hip_check(hipDeviceSynchronize(), "MultiFab::copyFromHost");
     hip_check(hipMemcpy(data_, h.data(), sizeof(Real) * h.size(), hipMemcpyHostToDevice), "MultiFab::copyFromHost");
 }
@@ -124,6 +204,17 @@ void MultiFab::copyFromHost(const Vector<Real>& h) {
 void MultiFab::Copy(MultiFab& dst, const MultiFab& src, int srccomp, int dstcomp, int numcomp, int) {
     if (dst.numCells() != src.numCells()) throw std::runtime_error("MultiFab::Copy: different levels");
     hip_check(hipDeviceSynchronize(), "MultiFab::Copy");
+    if (dst.fabPerBox() != src.fabPerBox()) throw std::runtime_error("MultiFab::Copy: different storage forms");
+    if (dst.fabPerBox()) {
+        for (int b = 0; b < src.ba_.size(); ++b) {
+            const FArrayBox &fs = *src.fabs_[b], &fd = *dst.fabs_[b];
+            if (fs.nstride() != fd.nstride()) throw std::runtime_error("MultiFab::Copy: different ghost widths");
+            hip_check(hipMemcpy(fd.dataPtr() + (size_t)dstcomp * fd.nstride(), fs.dataPtr() + (size_t)srccomp * fs.nstride(),
+                                sizeof(Real) * (size_t)numcomp * (size_t)fs.nstride(), hipMemcpyDeviceToDevice),
+                      "MultiFab::Copy");
+        }
+        return;
+    }
     hip_check(hipMemcpy(dst.data_ + (size_t)dstcomp * dst.numCells(), src.data_ + (size_t)srccomp * src.numCells(),
                         sizeof(Real) * (size_t)numcomp * (size_t)src.numCells(), hipMemcpyDeviceToDevice),
               "MultiFab::Copy");
@@ -280,11 +371,43 @@ AccessPattern::AccessPattern(ParameterStruct& parameters) {
 CellArray::CellArray(BoxArray& ba, DistributionMapping& dm, const int Ncomp, const int Nghost,
                      AccessPattern& _accessPattern, ParameterStruct& parameters_)
     : accessPattern(_accessPattern), numberOfMaterials(parameters_.numberOfMaterials), parameters(parameters_) {
-    data.define(ba, dm, Ncomp, Nghost);
+    data.define(ba, dm, Ncomp, Nghost, MFInfo().SetFabPerBox(parameters_.fabPerBox != 0));
+}
+
+CellArray::~CellArray() {
+    if (fabset_) tj_fabset_destroy(fabset_);
 }
 
 tj_mf CellArray::view() {
     tj_mf v{};
+    if (data.fabPerBox()) {
+        tj_ctx* ctx = dev(parameters).ctx;
+        if (fabset_ && fabsetCtx_ != ctx) {  // the level was bound again
+            tj_fabset_destroy(fabset_);
+            fabset_ = nullptr;
+        }
+        if (!fabset_) {
+            const BoxArray& ba = data.boxArray();
+            std::vector<tj_fab> fabs(ba.size());
+            for (int b = 0; b < ba.size(); ++b) {
+                const FArrayBox& f = data.fab(b);
+                tj_fab& r = fabs[b];
+                r.data = f.dataPtr();
+                for (int d = 0; d < 3; ++d) {
+                    r.lo[d] = ba[b].lo[d];
+                    r.hi[d] = ba[b].hi[d];
+                    r.glo[d] = f.box().lo[d];
+                }
+                r.jstride = f.jstride();
+                r.kstride = f.kstride();
+                r.nstride = f.nstride();
+            }
+            tj_check(parameters, tj_fabset_create(ctx, ba.size(), fabs.data(), data.nComp(), &fabset_), "tj_fabset_create");
+            fabsetCtx_ = ctx;
+        }
+        tj_fabset_view(fabset_, &v);
+        return v;
+    }
     v.data = data.devicePtr();
     v.jstride = data.jstride();
     v.kstride = data.kstride();
@@ -293,8 +416,20 @@ tj_mf CellArray::view() {
     return v;
 }
 
-void CellArray::stage() { data.copyToHost(host); }
-void CellArray::commit() { data.copyFromHost(host); }
+void CellArray::stage() {
+    if (data.fabPerBox())
+        for (int b = 0; b < data.boxArray().size(); ++b) data.fab(b).copyToHost();
+    data.copyToHost(host);
+}
+void CellArray::commit() {
+    if (data.fabPerBox()) {
+        // the FABs' own staged copies carry the BoxAccessCellArray writes
+        for (int b = 0; b < data.boxArray().size(); ++b) data.fab(b).copyFromHost();
+        data.copyToHost(host);
+        return;
+    }
+    data.copyFromHost(host);
+}
 
 // cellarray.cpp:5-18
 void CellArray::conservativeToPrimitive() {
@@ -355,21 +490,39 @@ bool CellArray::contains_nan() {
 }
 
 // ------------------------------------------------------------------ BoxAccessCellArray
+// the FAB whose valid box contains bx (fabPerBox)
+static FArrayBox* owning_fab(CellArray& U, const Box& bx) {
+    const BoxArray& ba = U.data.boxArray();
+    for (int b = 0; b < ba.size(); ++b) {
+        bool in = true;
+        for (int d = 0; d < 3; ++d) in = in && ba[b].lo[d] <= bx.lo[d] && bx.hi[d] <= ba[b].hi[d];
+        if (in) return &U.data.fab(b);
+    }
+    throw std::runtime_error("BoxAccessCellArray: the box lies in no single FAB of the CellArray");
+}
+
 BoxAccessCellArray::BoxAccessCellArray(MFIter& mfi, const Box& bx, CellArray& U_)
     : box(bx), U(U_), accessPattern(U_.accessPattern), numberOfMaterials(U_.numberOfMaterials) {
-    (void)mfi;
-    if (U.host.empty()) throw std::runtime_error("BoxAccessCellArray: stage() the CellArray first");
+    if (U.data.fabPerBox()) fabp = &U.data.fab(mfi.index());
+    if (fabp ? fabp->host.empty() : U.host.empty())
+        throw std::runtime_error("BoxAccessCellArray: stage() the CellArray first");
+}
+
+BoxAccessCellArray::BoxAccessCellArray(const Box& bx, FArrayBox& fb, CellArray& U_)
+    : box(bx), U(U_), fabp(&fb), accessPattern(U_.accessPattern), numberOfMaterials(U_.numberOfMaterials) {
+    if (fb.host.empty()) throw std::runtime_error("BoxAccessCellArray: stage the FArrayBox first (copyToHost)");
 }
 
 BoxAccessCellArray::BoxAccessCellArray(const Box& bx, CellArray& U_)
     : box(bx), U(U_), accessPattern(U_.accessPattern), numberOfMaterials(U_.numberOfMaterials) {
-    if (U.host.empty()) throw std::runtime_error("BoxAccessCellArray: stage() the CellArray first");
+    if (U.data.fabPerBox()) fabp = owning_fab(U, bx);
+    if (fabp ? fabp->host.empty() : U.host.empty())
+        throw std::runtime_error("BoxAccessCellArray: stage() the CellArray first");
 }
 
 // component of (var, mat, row, col) as in boxaccesscellarray.cpp:19-63:
 // per-material variables add mat, vectors add row, 3x3 tensors add 3 row + col
 Real& BoxAccessCellArray::operator()(int i, int j, int k, Variable var, int mat, int row, int col) {
-    const Box& d = U.data.domain();
     int comp = accessPattern[var];
     switch (var) {
         case ALPHA: case ALPHARHO: case RHO_K: case EPSILON: case ALPHARHOEPSILON: comp += mat; break;
@@ -381,6 +534,13 @@ Real& BoxAccessCellArray::operator()(int i, int j, int k, Variable var, int mat,
         case RHO: case TOTAL_E: case P: case SOUNDSPEED: case USTAR: case HJ2: break;
         default: throw std::runtime_error("Incorrect Access variable " + std::to_string((int)var));
     }
+    if (fabp) {
+        const Box& g = fabp->box();
+        const long long cell = (long long)(k - g.lo[2]) * fabp->kstride() + (long long)(j - g.lo[1]) * fabp->jstride() +
+                               (i - g.lo[0]);
+        return fabp->host[(size_t)comp * (size_t)fabp->nstride() + (size_t)cell];
+    }
+    const Box& d = U.data.domain();
     const long long cell = ((long long)(k - d.lo[2]) * U.data.kstride()) + (long long)(j - d.lo[1]) * U.data.jstride() +
                            (i - d.lo[0]);
     return U.host[(size_t)comp * (size_t)U.data.nstride() + (size_t)cell];
@@ -389,8 +549,132 @@ Real& BoxAccessCellArray::operator()(int i, int j, int k, Variable var, int mat,
 Real& BoxAccessCellArray::operator()(int i, int j, int k, int var, int mat, int row, int col) {
     return (*this)(i, j, k, (Variable)var, mat, row, col);
 }
+Real& BoxAccessCellArray::operator()(int i, int j, int k, MaterialSpecifier& m) {
+    return (*this)(i, j, k, m.var, m.mat, m.row, m.col);
+}
+
+// boxaccesscellarray.cpp:382-430: the cell one step down or up along d
+Real& BoxAccessCellArray::left(Direction_enum d, int i, int j, int k, MaterialSpecifier& m) {
+    switch (d) {
+        case x: return (*this)(i - 1, j, k, m);
+        case y: return (*this)(i, j - 1, k, m);
+        case z: return (*this)(i, j, k - 1, m);
+    }
+    std::printf("Bad Direction in Left function\n");
+    std::exit(1);
+}
+Real& BoxAccessCellArray::right(Direction_enum d, int i, int j, int k, MaterialSpecifier& m) {
+    switch (d) {
+        case x: return (*this)(i + 1, j, k, m);
+        case y: return (*this)(i, j + 1, k, m);
+        case z: return (*this)(i, j, k + 1, m);
+    }
+    std::printf("Bad Direction in Right function\n");
+    std::exit(1);
+}
+Real& BoxAccessCellArray::left(Direction_enum d, int i, int j, int k, Variable var, int mat, int row, int col) {
+    MaterialSpecifier t(var, mat, row, col);
+    return left(d, i, j, k, t);
+}
+Real& BoxAccessCellArray::right(Direction_enum d, int i, int j, int k, Variable var, int mat, int row, int col) {
+    MaterialSpecifier t(var, mat, row, col);
+    return right(d, i, j, k, t);
+}
+Real& BoxAccessCellArray::neighbour(int di, int dj, int dk, int i, int j, int k, MaterialSpecifier& m) {
+    return (*this)(i + di, j + dj, k + dk, m);
+}
+Real& BoxAccessCellArray::neighbour(int di, int dj, int dk, int i, int j, int k, Variable var, int mat, int row,
+                                    int col) {
+    MaterialSpecifier t(var, mat, row, col);
+    return neighbour(di, dj, dk, i, j, k, t);
+}
+// boxaccesscellarray.cpp:432-444
+void BoxAccessCellArray::amrexToArray(int i, int j, int k, Variable var, int m, double* copy, int nx, int ny) {
+    for (int row = 0; row < nx; ++row)
+        for (int col = 0; col < ny; ++col) copy[row * nx + col] = (*this)(i, j, k, var, m, row, col);
+}
+
+// The region's staged cells go to the device, the op runs on the region there,
+// and the region's cells are staged again.  Staged copies hold the whole FAB
+// (fabPerBox) or the whole level; only the region's rows move.
+Real BoxAccessCellArray::deviceOp(const Box& region, unsigned ops, bool* nan) {
+    ParameterStruct& prm = U.parameters;
+    const int nc = U.data.nComp();
+    Real* dptr;
+    Real* hptr;
+    Box st;  // the box the staged copy and the device buffer cover
+    long long js, ks, ns;
+    if (fabp) {
+        st = fabp->box();
+        dptr = fabp->dataPtr();
+        hptr = fabp->host.data();
+        js = fabp->jstride(), ks = fabp->kstride(), ns = fabp->nstride();
+    } else {
+        st = U.data.domain();
+        dptr = U.data.devicePtr();
+        hptr = U.host.data();
+        js = U.data.jstride(), ks = U.data.kstride(), ns = U.data.nstride();
+    }
+    auto rows = [&](bool to_device) {
+        for (int c = 0; c < nc; ++c)
+            for (int k = region.lo[2]; k <= region.hi[2]; ++k) {
+                const size_t off = (size_t)c * ns + (size_t)(k - st.lo[2]) * ks + (size_t)(region.lo[1] - st.lo[1]) * js +
+                                   (size_t)(region.lo[0] - st.lo[0]);
+                const size_t w = sizeof(Real) * region.length(0), pitch = sizeof(Real) * js;
+                hip_check(to_device ? hipMemcpy2D(dptr + off, pitch, hptr + off, pitch, w, region.length(1),
+                                                  hipMemcpyHostToDevice)
+                                    : hipMemcpy2D(hptr + off, pitch, dptr + off, pitch, w, region.length(1),
+                                                  hipMemcpyDeviceToHost),
+                          "BoxAccessCellArray: region copy");
+            }
+    };
+    hip_check(hipDeviceSynchronize(), "BoxAccessCellArray");
+    rows(true);
+    tj_mf v = U.view();
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = region.lo[d];
+        hi[d] = region.hi[d];
+    }
+    double dt = 0.0;
+    tj_check(prm, tj_box_op(dev(prm).ctx, &v, lo, hi, ops, &dt), "tj_box_op");
+    hip_check(hipDeviceSynchronize(), "BoxAccessCellArray");
+    rows(false);
+    const bool n = tenjin_host::checkStatus(prm);
+    if (nan) *nan = n;
+    return dt;
+}
+
+void BoxAccessCellArray::conservativeToPrimitive() { deviceOp(box, TJ_BOX_C2P); }
+void BoxAccessCellArray::primitiveToConservative() { deviceOp(box, TJ_BOX_P2C); }
+void BoxAccessCellArray::getSoundSpeed() { deviceOp(box, TJ_OP_DT); }
+void BoxAccessCellArray::cleanUpV() { deviceOp(box, TJ_OP_CLEAN_V); }
+void BoxAccessCellArray::cleanUpAlpha() { deviceOp(box, TJ_OP_CLEAN_ALPHA); }
+bool BoxAccessCellArray::contains_nan() {
+    bool nan = false;
+    deviceOp(box, TJ_OP_NANCHECK, &nan);
+    return nan;
+}
+void BoxAccessCellArray::stressTensor(int i, int j, int k) { deviceOp(Box(IntVect(i, j, k), IntVect(i, j, k)), TJ_BOX_DERIVE); }
+void BoxAccessCellArray::getHenckyJ2(int i, int j, int k) { deviceOp(Box(IntVect(i, j, k), IntVect(i, j, k)), TJ_BOX_DERIVE); }
+void BoxAccessCellArray::getDeviatoricHenckyStrain(int i, int j, int k) {
+    deviceOp(Box(IntVect(i, j, k), IntVect(i, j, k)), TJ_BOX_DERIVE);
+}
 
 // ------------------------------------------------------------------ TimeStep, PlasticEOS, advance
+// timestep.cpp:7-34 over the staged cells of U.box
+void TimeStep::boxTimeStepFunction(BoxAccessCellArray& U, Array4<Real> const& prop_arr_time, ParameterStruct& parameters) {
+    const int dim = parameters.gpu ? dev(parameters).desc.dim : 3;
+    for (int k = U.box.lo[2]; k <= U.box.hi[2]; ++k)
+        for (int j = U.box.lo[1]; j <= U.box.hi[1]; ++j)
+            for (int i = U.box.lo[0]; i <= U.box.hi[0]; ++i)
+                for (int n = 0; n < dim; ++n)
+                    prop_arr_time(i, j, k, n) = parameters.dx[n] / (U(i, j, k, SOUNDSPEED) + std::fabs(U(i, j, k, VELOCITY, 0, n)));
+}
+
+// plastic.cpp:83-225 on U.box
+void PlasticEOS::boxPlasticUpdate(BoxAccessCellArray& U, ParameterStruct&) { U.deviceOp(U.box, TJ_OP_PLASTIC); }
+
 // timestep.cpp:36-70: CFL * min over the valid cells of dx_d / (a + |u_d|)
 Real TimeStep::getTimeStep(CellArray& U, ParameterStruct& parameters) {
     tj_mf v = U.view();

@@ -172,6 +172,11 @@ struct ParameterStruct {
 
     int max_grid_size = 64;
 
+    // host layer: CellArrays hold AMReX's storage, one ghost-grown FArrayBox
+    // per box (MFInfo().SetFabPerBox()), and the library runs on those FABs
+    // in place; 0: one level array per CellArray
+    int fabPerBox = 0;
+
     Real CFL = 0.5;
     Real x0 = 0.0;
     Real dt = 0.0;
@@ -197,6 +202,9 @@ class CellArray {
 public:
     CellArray(BoxArray& ba, DistributionMapping& dm, const int Ncomp, const int Nghost, AccessPattern& _accessPattern,
               ParameterStruct& parameters);
+    ~CellArray();
+    CellArray(const CellArray&) = delete;
+    CellArray& operator=(const CellArray&) = delete;
 
     void primitiveToConservative();
     void conservativeToPrimitive();
@@ -214,27 +222,75 @@ public:
     int numberOfMaterials;
 
     // host staging for BoxAccessCellArray: stage() copies the level to the
-    // host, commit() copies it back
+    // host (with fabPerBox: each FAB to its own host copy), commit() copies it
+    // back
     void stage();
     void commit();
     Vector<Real> host;
 
+    // the library's view: the level array, or (fabPerBox) a tj_fabset over the
+    // FABs, made on first use after bindLevel
     tj_mf view();
     ParameterStruct& parameters;
+
+private:
+    tj_fabset* fabset_ = nullptr;
+    tj_ctx* fabsetCtx_ = nullptr;
 };
 
-// Per-box accessor over a CellArray's host staging copy (cellarray.h:46-107:
-// operator()(i, j, k, Variable, mat, row, col) with i, j, k in level indices).
+// structdefinitions.h:63-80: a component named by variable, material, row, col
+struct MaterialSpecifier {
+    Variable var;
+    int mat, row, col;
+    MaterialSpecifier(Variable v = RHO, int m = 0, int r = 0, int c = 0) : var(v), mat(m), row(r), col(c) {}
+};
+
+// Per-box accessor (cellarray.h:46-107, boxaccesscellarray.cpp:3-23, 382-430)
+// over a host staging copy: the CellArray's (stage() first), or with fabPerBox
+// the box's FArrayBox's; i, j, k are level indices.  The per-box methods run on
+// the GPU for this box alone (tj_box_op): they copy the box's staged cells to
+// the device, run, and stage the box's cells again, so the staged copy and the
+// device agree for the box afterwards.
 class BoxAccessCellArray {
 public:
     BoxAccessCellArray(MFIter& mfi, const Box& bx, CellArray& U);
+    BoxAccessCellArray(const Box& bx, FArrayBox& fb, CellArray& U);
     BoxAccessCellArray(const Box& bx, CellArray& U);
     const Box& box;
     CellArray& U;
+    FArrayBox* fabp = nullptr;  // the FAB the accessor reads (fabPerBox), else null
     AccessPattern& accessPattern;
     int numberOfMaterials;
+    static const int numberOfComponents = 3;
+
+    Real& operator()(int i, int j, int k, MaterialSpecifier& m);
     Real& operator()(int i, int j, int k, Variable var, int mat = 0, int row = 0, int col = 0);
     Real& operator()(int i, int j, int k, int var, int mat = 0, int row = 0, int col = 0);
+    Real& left(Direction_enum d, int i, int j, int k, MaterialSpecifier& m);
+    Real& right(Direction_enum d, int i, int j, int k, MaterialSpecifier& m);
+    Real& left(Direction_enum d, int i, int j, int k, Variable var, int mat = 0, int row = 0, int col = 0);
+    Real& right(Direction_enum d, int i, int j, int k, Variable var, int mat = 0, int row = 0, int col = 0);
+    Real& neighbour(int di, int dj, int dk, int i, int j, int k, MaterialSpecifier& m);
+    Real& neighbour(int di, int dj, int dk, int i, int j, int k, Variable var, int mat = 0, int row = 0, int col = 0);
+    void amrexToArray(int i, int j, int k, Variable var, int m, double* copy, int nx = 3, int ny = 3);
+
+    // per-box methods on the GPU (boxaccesscellarray.cpp:65-599)
+    void conservativeToPrimitive();
+    void primitiveToConservative();
+    void getSoundSpeed();
+    void cleanUpV();
+    void cleanUpAlpha();
+    bool contains_nan();
+    // per-cell: stress, HenckyJ2 and devH of cell (i, j, k) (stressTensor,
+    // getHenckyJ2, getDeviatoricHenckyStrain; boxaccesscellarray.cpp:176-216, 326-380)
+    void stressTensor(int i, int j, int k);
+    void getHenckyJ2(int i, int j, int k);
+    void getDeviatoricHenckyStrain(int i, int j, int k);
+
+    // copy [lo, hi] of the staged cells to the device, run tj_box_op with ops
+    // there, stage [lo, hi] again; returns the op's dt with TJ_OP_DT, and in
+    // *nan whether the NaN bit was raised (the reference's fatal bits exit)
+    Real deviceOp(const Box& region, unsigned ops, bool* nan = nullptr);
 };
 
 // The reference's THINC flag array; the library keeps the THINC state itself.
@@ -248,6 +304,10 @@ class TimeStep {
 public:
     TimeStep() = default;
     TimeStep(BoxArray&, DistributionMapping&, const int, const int) {}
+    // timestep.cpp:7-34: prop_arr_time(i, j, k, n) = dx[n] / (a + |u_n|) over
+    // U.box, from U's staged cells (host arithmetic over the staged copy; the
+    // time loop's getTimeStep runs on the GPU)
+    void boxTimeStepFunction(BoxAccessCellArray& U, Array4<Real> const& prop_arr_time, ParameterStruct& parameters);
     Real getTimeStep(CellArray& U, ParameterStruct& parameters);
 };
 
@@ -255,6 +315,8 @@ class PlasticEOS {
 public:
     PlasticEOS() = default;
     void plasticUpdate(CellArray& U, ParameterStruct& parameters);
+    // plastic.cpp:83-225 on U.box, on the GPU (the box's staged cells in and out)
+    void boxPlasticUpdate(BoxAccessCellArray& U, ParameterStruct& parameters);
     Vector<Real> yieldStress;
 };
 

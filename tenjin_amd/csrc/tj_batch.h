@@ -8,6 +8,7 @@
 #pragma once
 
 namespace tj {
+inline namespace TJ_VIEW_NS {
 
 // Everything one problem's passes need, in device memory.
 struct BProb {
@@ -169,6 +170,7 @@ __global__ void k_b_clock_begin(const BProb* P, int np, double final_time) {
     clk->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
 }
 
+}  // inline namespace TJ_VIEW_NS
 }  // namespace tj
 
 struct tj_batch {
@@ -381,6 +383,11 @@ int tj_batch_create(tj_batch** out, int device, int nprob, const tj_level_desc* 
         }
         int cs[NVAR];
         const int nc = build_layout(d, cs);
+        if (S[p].fabs || T[p].fabs) {
+            b->err = "batch: state views must be level arrays (tj_fabset views are for single levels)";
+            *out = b;
+            return TJ_EUNSUPPORTED;
+        }
         if (S[p].ncomp < nc || T[p].ncomp < nc) {
             b->err = "batch: state view with too few components";
             *out = b;

@@ -72,12 +72,47 @@ struct tj_ctx {
         }                                                                                   \
     } while (0)
 
-inline MF view(const tj_mf* m) { return MF{m->data, m->jstride, m->kstride, m->nstride}; }
+// A MultiFab of one FAB per box (tj_fabset_create): the FAB records in
+// box-grid order and the per-dimension box maps, in device memory.
+struct tj_fabset {
+    tj_ctx* ctx = nullptr;
+    int ncomp = 0;
+    int nb[3] = {1, 1, 1};  // boxes per dimension
+    int n[3] = {1, 1, 1};   // the level's cells per dimension when the set was made
+    tj::FabRec* d_rec = nullptr;
+    int* d_map[3] = {nullptr, nullptr, nullptr};      // cell index -> box offset (d_mapbase + pad)
+    int* d_mapbase[3] = {nullptr, nullptr, nullptr};  // the allocations
+    std::vector<tj_fab> fabs;  // the caller's records, in box-grid order
+};
 
 inline unsigned grid1(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+namespace tj {
+inline namespace TJ_VIEW_NS {
+#ifdef TJ_FAB_VIEWS
+inline MF view(const tj_mf* m) {
+    const tj_fabset* f = m->fabs;
+    return MF{f->d_rec, f->d_map[0], f->d_map[1], f->d_map[2]};
+}
+inline int check_mf(tj_ctx* c, const tj_mf* m) {
+    if (!m || !m->fabs) { c->err = "null tj_mf"; return TJ_EINVAL; }
+    const tj_fabset* f = m->fabs;
+    if (f->ctx != c) { c->err = "tj_fabset made for another context"; return TJ_EINVAL; }
+    if (f->n[0] != c->lv.n[0] || f->n[1] != c->lv.n[1] || f->n[2] != c->lv.n[2]) {
+        c->err = "tj_fabset made for another level";
+        return TJ_EINVAL;
+    }
+    if (m->ncomp < tj_ncomp(c) || f->ncomp < tj_ncomp(c)) {
+        c->err = "tj_mf has fewer components than the bound layout";
+        return TJ_EINVAL;
+    }
+    return TJ_OK;
+}
+#else
+inline MF view(const tj_mf* m) { return MF{m->data, m->jstride, m->kstride, m->nstride}; }
 inline int check_mf(tj_ctx* c, const tj_mf* m) {
     if (!m || !m->data) { c->err = "null tj_mf"; return TJ_EINVAL; }
+    if (m->fabs) { c->err = "tj_mf names a tj_fabset where a level array is expected"; return TJ_EINVAL; }
     if (m->ncomp < tj_ncomp(c)) { c->err = "tj_mf has fewer components than the bound layout"; return TJ_EINVAL; }
     if (m->jstride < c->lv.n[0] || m->kstride < m->jstride * c->lv.n[1] || m->nstride < m->kstride * c->lv.n[2]) {
         c->err = "tj_mf strides do not cover the level";
@@ -85,3 +120,6 @@ inline int check_mf(tj_ctx* c, const tj_mf* m) {
     }
     return TJ_OK;
 }
+#endif
+}  // inline namespace TJ_VIEW_NS
+}  // namespace tj

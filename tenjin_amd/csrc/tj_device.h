@@ -71,6 +71,40 @@ __device__ __forceinline__ const MatT<double>& mat<double>(const Level& lv, int 
 template <>
 __device__ __forceinline__ const MatT<float>& mat<float>(const Level& lv, int m) { return lv.mf[m]; }
 
+// One FAB of a per-box MultiFab: p0 is the address level cell (0,0,0) would
+// have in this FAB's strides (an address outside the FAB when the box does
+// not start at the origin; only cells of the box itself are ever formed).
+struct FabRec {
+    double* p0;
+    long long js, ks, ns;
+};
+
+// The kernels reach the state only through MF::at(c, i, j, k) with level cell
+// indices.  The library is compiled twice, once per form of MF: a level array,
+// or one FAB per box.  Each compilation puts MF and everything that takes it
+// (kernels, device bodies, launch code) in its own inline namespace, so the
+// two sets of symbols stay apart; the types above and below are shared.
+#ifdef TJ_FAB_VIEWS
+#define TJ_VIEW_NS fab_view
+#else
+#define TJ_VIEW_NS level_view
+#endif
+inline namespace TJ_VIEW_NS {
+#ifdef TJ_FAB_VIEWS
+// Caller-owned MultiFab of one FAB per box (tj_fabset): the cell's box from the
+// per-dimension box maps (cell index -> box coordinate), then its FAB.
+struct MF {
+    const FabRec* rec;                // boxes in box-grid order, x fastest
+    const int* map0;                  // per x index: box column
+    const int* map1;                  // per y index: box row * boxes along x
+    const int* map2;                  // per z index: box layer * boxes per layer
+    __device__ __forceinline__ double& at(int c, int i, int j, int k) const {
+        const FabRec& r = rec[map2[k] + map1[j] + map0[i]];
+        return r.p0[(long long)c * r.ns + (long long)k * r.ks + (long long)j * r.js + i];
+    }
+};
+__host__ __device__ __forceinline__ bool same_view(const MF& a, const MF& b) { return a.rec == b.rec; }
+#else
 // Caller-owned CellArray view (component-major, i fastest).
 struct MF {
     double* p;
@@ -79,6 +113,9 @@ struct MF {
         return p[(long long)c * ns + (long long)k * ks + (long long)j * js + i];
     }
 };
+__host__ __device__ __forceinline__ bool same_view(const MF& a, const MF& b) { return a.p == b.p; }
+#endif
+}  // inline namespace TJ_VIEW_NS
 
 // Face-centred flux planes of one direction: lower face of cell (i,j,k),
 // extent n_d + 1 along d; component-major.  FP64, or FP32 in the

@@ -4,6 +4,7 @@
 #include "tj_device.h"
 
 namespace tj {
+inline namespace TJ_VIEW_NS {
 
 // ------------------------------------------------------------------ states
 // R: the real type of reconstruction and flux work -- double in the FP64
@@ -530,4 +531,5 @@ __device__ __forceinline__ void bvd(int mixedFlag, const R* URl, const R* TRl, c
     }
 }
 
+}  // inline namespace TJ_VIEW_NS
 }  // namespace tj

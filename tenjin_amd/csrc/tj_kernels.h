@@ -29,6 +29,8 @@ __device__ __forceinline__ unsigned long long* dt_slot(const Level& lv, Clock* c
     return lv.dtpart ? lv.dtpart + (blockIdx.x % DT_PARTS) : &clk->dtmin_bits;
 }
 
+// Everything below takes or launches with the state view MF (see tj_device.h).
+inline namespace TJ_VIEW_NS {
 
 // ------------------------------------------------------------------ x pass
 // One wave covers a row segment; lane l holds cell c = f0 - H + l.  Lanes
@@ -1097,4 +1099,5 @@ __global__ void k_zero(double* p, long long n) {
 }
 #endif
 
+}  // inline namespace TJ_VIEW_NS
 }  // namespace tj

@@ -6,6 +6,7 @@
 #include <map>
 #include <queue>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -138,7 +139,62 @@ int zero(tj_ctx* c, double* p, long long n) {
 
 }  // namespace
 
+// The library is compiled twice (see tj_device.h): this file as it stands, and
+// with TJ_FAB_VIEWS, where MF is a MultiFab of one FAB per box.  In the second
+// compilation the entry points below are C++ functions of namespace tjfab;
+// here each entry point taking state views forwards to its tjfab twin when the
+// views name tj_fabsets.
+// An entry point calling another: the one of its own compilation (qualified,
+// since argument-dependent lookup on tj_ctx* also finds the global ones).
+#ifdef TJ_FAB_VIEWS
+#define TJ_SELF(f) tjfab::f
+#else
+#define TJ_SELF(f) ::f
+#endif
+#ifdef TJ_FAB_VIEWS
+#define TJ_FORWARD_FABS(c, call, ...) \
+    do {                              \
+    } while (0)
+namespace tjfab {
+#else
+namespace tjfab {
+int tj_compute_dt(tj_ctx*, const tj_mf*, double*);
+int tj_rk_stage(tj_ctx*, int, const tj_mf*, const tj_mf*, tj_mf*, int);
+int tj_post_step(tj_ctx*, tj_mf*, unsigned);
+int tj_step(tj_ctx*, tj_mf*, tj_mf*, int);
+int tj_begin_run(tj_ctx*, const tj_mf*, double);
+int tj_advance_steps(tj_ctx*, tj_mf*, tj_mf*, int, int);
+int tj_graph_capture(tj_ctx*, tj_mf*, tj_mf*, int, int);
+int tj_run(tj_ctx*, tj_mf*, tj_mf*, double, int, int, int, double*, int*);
+int tj_init_from_prims(tj_ctx*, tj_mf*, int);
+int tj_c2p(tj_ctx*, tj_mf*);
+int tj_lincomb(tj_ctx*, const tj_mf*, double, const tj_mf*, double, tj_mf*);
+int tj_derive(tj_ctx*, tj_mf*);
+}  // namespace tjfab
+namespace {
+// 0: every view is a level array, 1: every view names a tj_fabset, -1: mixed
+int view_form(std::initializer_list<const tj_mf*> ms) {
+    int f = -2;
+    for (const tj_mf* m : ms) {
+        if (!m) continue;
+        const int g = m->fabs ? 1 : 0;
+        if (f == -2) f = g;
+        else if (f != g) return -1;
+    }
+    return f < 0 ? 0 : f;
+}
+}  // namespace
+#define TJ_FORWARD_FABS(c, call, ...)                                                         \
+    do {                                                                                     \
+        const int f_ = view_form({__VA_ARGS__});                                             \
+        if (f_ < 0) {                                                                        \
+            (c)->err = "the views of one call must all be level arrays or all tj_fabsets";   \
+            return TJ_EINVAL;                                                                \
+        }                                                                                    \
+        if (f_ > 0) return tjfab::call;                                                      \
+    } while (0)
 extern "C" {
+#endif
 
 int tj_create(tj_ctx** out, int device) {
     if (!out) return TJ_EINVAL;
@@ -393,7 +449,7 @@ int tj_bind_level(tj_ctx* c, const tj_level_desc* d) {
             for (int q = 0; q < 2; ++q) HIPCHK(c, hipMalloc(&c->d_estar[a][q], sizeof(double) * c->estride[a] * d->nmat));
     }
     c->bound = true;
-    return tj_reset_history(c);
+    return ::tj_reset_history(c);
 }
 
 int tj_reset_history(tj_ctx* c) {
@@ -507,6 +563,7 @@ static unsigned step_ops(const tj_ctx* c) {
 
 int tj_compute_dt(tj_ctx* c, const tj_mf* U, double* dt_out) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_compute_dt(c, U, dt_out), U);
     int r = check_mf(c, U);
     if (r) return r;
     HIPCHK(c, hipSetDevice(c->device));
@@ -774,7 +831,7 @@ static void launch_fused(tj_ctx* c, const MF& Uin, int d, const Hist& h, const F
 static int launch_fluxes(tj_ctx* c, const MF& Uin, const FuseArgs* fz, bool* fused, int prec) {
     const Level& lv = c->lv;
     const bool mixed = prec == TJ_PREC_MIXED;
-    const bool fuse = fz && c->fuse && !mixed && !lv.solid && lv.mixm < 0 && lv.dim >= 2 && fz->out.p != Uin.p;
+    const bool fuse = fz && c->fuse && !mixed && !lv.solid && lv.mixm < 0 && lv.dim >= 2 && !same_view(fz->out, Uin);
     if (fused) *fused = fuse;
     for (int d = 0; d < lv.dim; ++d) {
         const Hist h = hist_for(c, d);
@@ -868,6 +925,7 @@ static bool use_stage2d(const tj_ctx* c) { return c->stage2d && c->d_ux && stage
 
 int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_mf* Uout, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_rk_stage(c, stage, Uin, Ubase, Uout, prec), Uin, Ubase, Uout);
     int r;
     if ((r = check_prec(c, prec))) return r;
     if ((r = check_mf(c, Uin)) || (r = check_mf(c, Uout))) return r;
@@ -877,7 +935,7 @@ int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_m
     const MF base = stage == 2 ? view(Ubase) : in;
     // the y kernel reads rows of Uin around each tile while other tiles write
     // Uout: an in-place stage (Uout == Uin) takes the flux-plane passes
-    if (use_stage2d(c) && view(Uout).p != in.p) return launch_stage2d(c, stage, in, base, view(Uout), 0u, prec);
+    if (use_stage2d(c) && !same_view(view(Uout), in)) return launch_stage2d(c, stage, in, base, view(Uout), 0u, prec);
     const FuseArgs fz{stage, base, view(Uout), 0u};
     bool fused = false;
     if ((r = launch_fluxes(c, in, &fz, &fused, prec))) return r;
@@ -892,6 +950,7 @@ int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_m
 
 int tj_post_step(tj_ctx* c, tj_mf* U, unsigned ops) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_post_step(c, U, ops), U);
     int r;
     if ((r = check_mf(c, U))) return r;
     if ((ops & (TJ_OP_PLASTIC | TJ_OP_CLEAN_V)) && !c->lv.solid) {
@@ -929,12 +988,13 @@ int tj_post_step(tj_ctx* c, tj_mf* U, unsigned ops) {
 
 int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_step(c, S, T, prec), S, T);
     int r;
     if ((r = check_prec(c, prec))) return r;
     if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
     HIPCHK(c, hipSetDevice(c->device));
     const MF s = view(S), t = view(T);
-    if (s.p == t.p) { c->err = "step: S and T must be distinct buffers"; return TJ_EINVAL; }
+    if (same_view(s, t)) { c->err = "step: S and T must be distinct buffers"; return TJ_EINVAL; }
     const bool mixed = prec == TJ_PREC_MIXED;
     if (use_stage2d(c)) {
         if ((r = launch_stage2d(c, 1, s, s, t, 0u, prec))) return r;
@@ -963,9 +1023,10 @@ int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
 
 int tj_begin_run(tj_ctx* c, const tj_mf* S, double final_time) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_begin_run(c, S, final_time), S);
     int r;
     if ((r = check_mf(c, S))) return r;
-    if ((r = tj_set_dt(c, 0.0, 1))) return r;
+    if ((r = TJ_SELF(tj_set_dt)(c, 0.0, 1))) return r;
     hipLaunchKernelGGL(k_clock_final, dim3(1), dim3(64), 0, c->stream, c->d_clk, final_time);
     // getTimeStep of the initial state feeds the first driver dt
     launch_sound_dt(c, view(S));
@@ -975,16 +1036,18 @@ int tj_begin_run(tj_ctx* c, const tj_mf* S, double final_time) {
 
 int tj_advance_steps(tj_ctx* c, tj_mf* S, tj_mf* T, int nsteps, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_advance_steps(c, S, T, nsteps, prec), S, T);
     int r;
     for (int q = 0; q < nsteps; ++q) {
         hipLaunchKernelGGL(k_clock_dt, dim3(1), dim3(64), 0, c->stream, c->d_clk, c->lv.cfl, 1, c->d_dtpart);
-        if ((r = tj_step(c, S, T, prec))) return r;
+        if ((r = TJ_SELF(tj_step)(c, S, T, prec))) return r;
     }
     return TJ_OK;
 }
 
 int tj_graph_capture(tj_ctx* c, tj_mf* S, tj_mf* T, int nsteps, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_graph_capture(c, S, T, nsteps, prec), S, T);
     if (nsteps < 1) { c->err = "graph_capture: nsteps < 1"; return TJ_EINVAL; }
     int r;
     if ((r = check_mf(c, S)) || (r = check_mf(c, T))) return r;
@@ -996,7 +1059,7 @@ int tj_graph_capture(tj_ctx* c, tj_mf* S, tj_mf* T, int nsteps, int prec) {
     c->prof = false;
     const long long p0 = c->pass, d0[3] = {c->dpass[0], c->dpass[1], c->dpass[2]};
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int rr = tj_advance_steps(c, S, T, nsteps, prec);
+    const int rr = TJ_SELF(tj_advance_steps)(c, S, T, nsteps, prec);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->prof = prof;
@@ -1038,14 +1101,15 @@ int tj_graph_launch(tj_ctx* c) {
 int tj_run(tj_ctx* c, tj_mf* S, tj_mf* T, double final_time, int max_steps, int prec, int check_every,
            double* t_out, int* n_out) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_run(c, S, T, final_time, max_steps, prec, check_every, t_out, n_out), S, T);
     int r;
     if ((r = check_mf(c, T))) return r;
-    if ((r = tj_begin_run(c, S, final_time))) return r;
+    if ((r = TJ_SELF(tj_begin_run)(c, S, final_time))) return r;
     if (check_every < 1) check_every = 1;
     int steps = 0;
     while (steps < max_steps) {
         const int chunk = std::min(check_every, max_steps - steps);
-        if ((r = tj_advance_steps(c, S, T, chunk, prec))) return r;
+        if ((r = TJ_SELF(tj_advance_steps)(c, S, T, chunk, prec))) return r;
         steps += chunk;
         Clock h;
         HIPCHK(c, hipMemcpyAsync(&h, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
@@ -1062,6 +1126,7 @@ int tj_run(tj_ctx* c, tj_mf* S, tj_mf* T, double final_time, int max_steps, int 
 
 int tj_init_from_prims(tj_ctx* c, tj_mf* U, int shift) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_init_from_prims(c, U, shift), U);
     int r;
     if ((r = check_mf(c, U))) return r;
     if (c->lv.solid) {
@@ -1078,6 +1143,7 @@ int tj_init_from_prims(tj_ctx* c, tj_mf* U, int shift) {
 
 int tj_c2p(tj_ctx* c, tj_mf* U) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_c2p(c, U), U);
     int r;
     if ((r = check_mf(c, U))) return r;
     if (c->lv.solid) {
@@ -1094,6 +1160,7 @@ int tj_c2p(tj_ctx* c, tj_mf* U) {
 
 int tj_lincomb(tj_ctx* c, const tj_mf* X, double a, const tj_mf* Y, double b, tj_mf* Z) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_lincomb(c, X, a, Y, b, Z), X, Y, Z);
     int r;
     if ((r = check_mf(c, X)) || (r = check_mf(c, Z))) return r;
     if (Y && (r = check_mf(c, Y))) return r;
@@ -1109,6 +1176,7 @@ int tj_lincomb(tj_ctx* c, const tj_mf* X, double a, const tj_mf* Y, double b, tj
 
 int tj_derive(tj_ctx* c, tj_mf* U) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
+    TJ_FORWARD_FABS(c, tj_derive(c, U), U);
     int r;
     if ((r = check_mf(c, U))) return r;
     if (c->lv.solid) {
@@ -1173,8 +1241,219 @@ int tj_synchronize(tj_ctx* c) {
     return TJ_OK;
 }
 
-}  // extern "C"
+#ifndef TJ_FAB_VIEWS
+// A MultiFab of one FAB per box (BoxAccessCellArray(const Box&, FArrayBox&,
+// CellArray&), boxaccesscellarray.cpp:3-23, for every box at once).
+int tj_fabset_create(tj_ctx* c, int nfab, const tj_fab* fabs, int ncomp, tj_fabset** out) {
+    if (!out) return TJ_EINVAL;
+    *out = nullptr;
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    if (nfab < 1 || !fabs) { c->err = "fabset: no FABs"; return TJ_EINVAL; }
+    if (ncomp < TJ_SELF(tj_ncomp)(c)) { c->err = "fabset: fewer components than the bound layout"; return TJ_EINVAL; }
+    // the level's BoxArray::maxSize decomposition, as tj_bind_level cuts it
+    std::vector<int> starts[3];
+    int nb[3];
+    for (int d = 0; d < 3; ++d) {
+        starts[d] = d < c->lv.dim ? chop_dim(c->lv.n[d], c->desc.max_grid_size) : std::vector<int>{0};
+        nb[d] = (int)starts[d].size();
+    }
+    const long long nbox = (long long)nb[0] * nb[1] * nb[2];
+    if (nfab != nbox) {
+        c->err = "fabset: the FABs must be the level's maxSize boxes (" + std::to_string(nbox) + ")";
+        return TJ_EINVAL;
+    }
+    auto box_end = [&](int d, int b) { return b + 1 < nb[d] ? starts[d][b + 1] - 1 : c->lv.n[d] - 1; };
+    std::vector<int> slot(nbox, -1);
+    std::vector<FabRec> rec(nbox);
+    for (int q = 0; q < nfab; ++q) {
+        const tj_fab& f = fabs[q];
+        int bc[3];
+        for (int d = 0; d < 3; ++d) {
+            const auto it = std::lower_bound(starts[d].begin(), starts[d].end(), f.lo[d]);
+            if (it == starts[d].end() || *it != f.lo[d] || box_end(d, (int)(it - starts[d].begin())) != f.hi[d]) {
+                c->err = "fabset: FAB " + std::to_string(q) + " is not a box of the level's decomposition";
+                return TJ_EINVAL;
+            }
+            bc[d] = (int)(it - starts[d].begin());
+            if (f.glo[d] > f.lo[d]) {
+                c->err = "fabset: FAB " + std::to_string(q) + " does not store its valid box (glo > lo)";
+                return TJ_EINVAL;
+            }
+        }
+        const long long e0 = f.hi[0] - f.glo[0] + 1, e1 = f.hi[1] - f.glo[1] + 1, e2 = f.hi[2] - f.glo[2] + 1;
+        if (!f.data || f.jstride < e0 || f.kstride < f.jstride * e1 || f.nstride < f.kstride * e2) {
+            c->err = "fabset: FAB " + std::to_string(q) + " has no data or strides that do not cover its box";
+            return TJ_EINVAL;
+        }
+        const long long b = ((long long)bc[2] * nb[1] + bc[1]) * nb[0] + bc[0];
+        if (slot[b] >= 0) {
+            c->err = "fabset: two FABs for one box";
+            return TJ_EINVAL;
+        }
+        slot[b] = q;
+        // address of level cell (0,0,0) in this FAB's strides (integer arithmetic:
+        // the address may lie outside the FAB; only the box's own cells are formed)
+        const long long off = f.glo[0] + f.glo[1] * f.jstride + f.glo[2] * f.kstride;
+        rec[b] = FabRec{reinterpret_cast<double*>(reinterpret_cast<uintptr_t>(f.data) - (uintptr_t)(off * 8)),
+                        f.jstride, f.kstride, f.nstride};
+    }
+    tj_fabset* fs = new tj_fabset();
+    fs->ctx = c;
+    fs->ncomp = ncomp;
+    for (int d = 0; d < 3; ++d) {
+        fs->nb[d] = nb[d];
+        fs->n[d] = c->lv.n[d];
+    }
+    for (long long b = 0; b < nbox; ++b) fs->fabs.push_back(fabs[slot[b]]);
+    // the maps carry kMapPad entries beyond each end that repeat the edge box, so
+    // an index a few cells outside the level still names a FAB (no kernel forms
+    // one: every stencil index is mapped into the domain first)
+    constexpr int kMapPad = 8;
+    std::vector<int> map[3];
+    const int mul[3] = {1, nb[0], nb[0] * nb[1]};
+    for (int d = 0; d < 3; ++d) {
+        const int n = c->lv.n[d];
+        map[d].resize(n + 2 * kMapPad);
+        for (int b = 0; b < nb[d]; ++b)
+            for (int i = starts[d][b]; i <= box_end(d, b); ++i) map[d][kMapPad + i] = b * mul[d];
+        for (int q = 0; q < kMapPad; ++q) {
+            map[d][q] = map[d][kMapPad];
+            map[d][kMapPad + n + q] = map[d][kMapPad + n - 1];
+        }
+    }
+    auto fail = [&](hipError_t e) {
+        c->err = std::string("fabset: ") + hipGetErrorString(e);
+        TJ_SELF(tj_fabset_destroy)(fs);
+        return TJ_EHIP;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(c->device)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&fs->d_rec, sizeof(FabRec) * nbox)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpy(fs->d_rec, rec.data(), sizeof(FabRec) * nbox, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+    for (int d = 0; d < 3; ++d) {
+        if ((e = hipMalloc(&fs->d_mapbase[d], sizeof(int) * map[d].size())) != hipSuccess) return fail(e);
+        if ((e = hipMemcpy(fs->d_mapbase[d], map[d].data(), sizeof(int) * map[d].size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return fail(e);
+        fs->d_map[d] = fs->d_mapbase[d] + kMapPad;
+    }
+    *out = fs;
+    return TJ_OK;
+}
 
+int tj_level_boxes(const tj_ctx* c, int* nbox, int* lohi) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    if (!nbox) return TJ_EINVAL;
+    std::vector<int> starts[3];
+    for (int d = 0; d < 3; ++d)
+        starts[d] = d < c->lv.dim ? chop_dim(c->lv.n[d], c->desc.max_grid_size) : std::vector<int>{0};
+    *nbox = (int)(starts[0].size() * starts[1].size() * starts[2].size());
+    if (!lohi) return TJ_OK;
+    int q = 0;
+    for (size_t bk = 0; bk < starts[2].size(); ++bk)
+        for (size_t bj = 0; bj < starts[1].size(); ++bj)
+            for (size_t bi = 0; bi < starts[0].size(); ++bi, ++q) {
+                const size_t b[3] = {bi, bj, bk};
+                for (int d = 0; d < 3; ++d) {
+                    lohi[6 * q + d] = starts[d][b[d]];
+                    lohi[6 * q + 3 + d] = b[d] + 1 < starts[d].size() ? starts[d][b[d] + 1] - 1 : c->lv.n[d] - 1;
+                }
+            }
+    return TJ_OK;
+}
+
+int tj_box_op(tj_ctx* c, tj_mf* U, const int lo[3], const int hi[3], unsigned ops, double* dt_out) {
+    if (!c || !c->bound) return TJ_ENOLEVEL;
+    if (!U || !lo || !hi) return TJ_EINVAL;
+    for (int d = 0; d < 3; ++d)
+        if (lo[d] < 0 || hi[d] >= c->lv.n[d] || lo[d] > hi[d]) {
+            c->err = "box_op: the box is not inside the level";
+            return TJ_EINVAL;
+        }
+    const unsigned post = TJ_OP_PLASTIC | TJ_OP_CLEAN_V | TJ_OP_CLEAN_ALPHA | TJ_OP_NANCHECK | TJ_OP_REACTIVE;
+    if (ops & TJ_OP_RADIAL) {
+        c->err = "box_op: the geometric source needs each cell's radius on the level (tj_post_step)";
+        return TJ_EUNSUPPORTED;
+    }
+    if (ops & ~(post | TJ_OP_DT | TJ_BOX_P2C | TJ_BOX_C2P | TJ_BOX_DERIVE)) {
+        c->err = "box_op: unknown op bits";
+        return TJ_EINVAL;
+    }
+    // the box as a level array whose cell (0,0,0) is lo: inside the level
+    // array, or inside the FAB that owns the box
+    tj_mf sub{};
+    sub.ncomp = U->ncomp;
+    if (U->fabs) {
+        const tj_fab* own = nullptr;
+        for (const tj_fab& f : U->fabs->fabs) {
+            bool in = true;
+            for (int d = 0; d < 3; ++d) in = in && f.lo[d] <= lo[d] && hi[d] <= f.hi[d];
+            if (in) { own = &f; break; }
+        }
+        if (!own || U->fabs->ctx != c) {
+            c->err = "box_op: with a tj_fabset the box must lie in one FAB of this context's level";
+            return TJ_EINVAL;
+        }
+        sub.data = own->data + (lo[0] - own->glo[0]) + (lo[1] - own->glo[1]) * own->jstride +
+                   (lo[2] - own->glo[2]) * own->kstride;
+        sub.jstride = own->jstride;
+        sub.kstride = own->kstride;
+        sub.nstride = own->nstride;
+        sub.ncomp = U->fabs->ncomp;
+    } else {
+        int r;
+        if ((r = check_mf(c, U))) return r;
+        sub = *U;
+        sub.data = U->data + lo[0] + lo[1] * U->jstride + lo[2] * U->kstride;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    Clock saved;
+    HIPCHK(c, hipMemcpyAsync(&saved, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the cell-wise kernels index cells from the level's extent only: narrow it
+    const Level whole = c->lv;
+    for (int d = 0; d < 3; ++d) c->lv.n[d] = hi[d] - lo[d] + 1;
+    c->lv.ncells = (long long)c->lv.n[0] * c->lv.n[1] * c->lv.n[2];
+    int r = TJ_OK;
+    if (!r && (ops & TJ_BOX_P2C)) r = ::tj_init_from_prims(c, &sub, 0);
+    if (!r && (ops & TJ_BOX_C2P)) r = ::tj_c2p(c, &sub);
+    if (!r && (ops & post)) r = ::tj_post_step(c, &sub, ops & post);
+    if (!r && (ops & TJ_BOX_DERIVE)) r = ::tj_derive(c, &sub);
+    if (!r && (ops & TJ_OP_DT)) r = ::tj_compute_dt(c, &sub, dt_out);
+    c->lv = whole;
+    // the level's clock as it was; status bits raised meanwhile stay
+    Clock now;
+    HIPCHK(c, hipMemcpyAsync(&now, c->d_clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    saved.status = now.status;
+    HIPCHK(c, hipMemcpyAsync(c->d_clk, &saved, sizeof(Clock), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return r;
+}
+
+int tj_fabset_view(const tj_fabset* fs, tj_mf* out) {
+    if (!fs || !out) return TJ_EINVAL;
+    *out = tj_mf{};
+    out->ncomp = fs->ncomp;
+    out->fabs = fs;
+    return TJ_OK;
+}
+
+void tj_fabset_destroy(tj_fabset* fs) {
+    if (!fs) return;
+    if (fs->ctx) {
+        (void)hipSetDevice(fs->ctx->device);
+        (void)hipStreamSynchronize(fs->ctx->stream);
+    }
+    (void)hipFree(fs->d_rec);
+    for (int d = 0; d < 3; ++d) (void)hipFree(fs->d_mapbase[d]);
+    delete fs;
+}
+#endif
+
+}  // extern "C" (namespace tjfab with TJ_FAB_VIEWS)
+
+#ifndef TJ_FAB_VIEWS
 #include "tj_batch.h"
 
 // ---------------------------------------------------------------------------
@@ -1362,6 +1641,8 @@ extern "C" int tj_run_static(const tj_level_desc* desc, tj_mf* S, tj_mf* T, doub
     }
     // the history planes start at zero on every call (they hold the last
     // call's contents otherwise), as after tj_bind_level
-    if ((r = tj_reset_history(c))) return r;
+    if ((r = TJ_SELF(tj_reset_history)(c))) return r;
     return tj_run(c, S, T, final_time, max_steps, prec, check_every, t_out, n_out);
 }
+
+#endif  // !TJ_FAB_VIEWS

@@ -8,6 +8,7 @@
 #include "tj_dual.h"
 
 namespace tj {
+inline namespace TJ_VIEW_NS {
 
 // ------------------------------------------------------------------ 3x3 algebra
 // tensor.cpp:81-115 (products: k = 0,1,2 summed left to right), 140-150
@@ -753,4 +754,5 @@ __device__ inline void cleanup_v(const Level& lv, const S* al, S* V) {
     }
 }
 
+}  // inline namespace TJ_VIEW_NS
 }  // namespace tj

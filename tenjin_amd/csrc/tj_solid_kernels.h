@@ -5,6 +5,7 @@
 #include "tj_solid.h"
 
 namespace tj {
+inline namespace TJ_VIEW_NS {
 
 template <int NM, typename R>
 __device__ __forceinline__ SSt<NM, R> shfl_state(const SSt<NM, R>& S, int src) {
@@ -534,9 +535,11 @@ __global__ __launch_bounds__(256) void k_derive_s(Level lv, MF U) {
     }
 }
 
+}  // inline namespace TJ_VIEW_NS
 }  // namespace tj
 
 namespace tj {
+inline namespace TJ_VIEW_NS {
 
 // conservativeToPrimitive over the valid cells of a solid level.
 template <int NM, bool RW>
@@ -607,4 +610,5 @@ __global__ __launch_bounds__(256) void k_post_s(Level lv, MF U, Clock* clk, unsi
     if (st) atomicOr(&clk->status, st);
 }
 
+}  // inline namespace TJ_VIEW_NS
 }  // namespace tj

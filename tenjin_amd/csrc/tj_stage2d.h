@@ -27,6 +27,7 @@
 #include "tj_kernels.h"
 
 namespace tj {
+inline namespace TJ_VIEW_NS {
 
 // Partially updated conserved state between the two kernels of a stage:
 // component-major planes of FP64 over the level's cells, in the order
@@ -498,4 +499,5 @@ __global__ __launch_bounds__(TC * LPS) void k_ystage(Level lv, MF U, UxPlanes Ux
     ystage_body<NM, TH, TC, LPS, STAGE, R>(lv, U, Ux, Ub, Uo, h, clk, ops, blockIdx.x);
 }
 
+}  // inline namespace TJ_VIEW_NS
 }  // namespace tj

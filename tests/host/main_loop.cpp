@@ -3,10 +3,18 @@
 // tests: the level description, the primitive initial state, finalT and a step
 // cap.  Writes (t, n) and the final state of U1.
 //
-//   main_loop <input> <output> [plotfile root]
+//   main_loop [--fab-per-box] [--box-checks] <input> <output> [plotfile root]
 //
 // With a plotfile root, the initial and the final state of U1 are written as
 // AMReX plotfiles <root>00000 and <root><n> (main.cpp:104-107, 195-199).
+// --fab-per-box: every CellArray holds one ghost-grown FArrayBox per box, as an
+// AMReX MultiFab does, and the library runs on those FABs in place.
+// --box-checks: after the loop, the per-box tier against the level-wide calls
+// on copies of U1 (U2 box by box, U level-wide): boxPlasticUpdate / cleanUpV /
+// cleanUpAlpha / conservativeToPrimitive / getSoundSpeed of each
+// BoxAccessCellArray and boxTimeStepFunction's minimum must give the same bits
+// as plasticUpdate / cleanUpV / cleanUpAlpha / conservativeToPrimitive and
+// getTimeStep; exit status 3 if not.
 //
 // input : "TJIN" | int32 1 | tj_level_desc | double finalT | int32 max_steps
 //         | int32 shift_pressure | int64 ncomp | int64 ncells | ncomp*ncells doubles
@@ -16,7 +24,10 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <limits>
 #include <memory>
+#include <string>
+#include <vector>
 
 #include "tenjin_host.h"
 
@@ -31,12 +42,20 @@ void get(std::ifstream& f, T& v) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3 && argc != 4) {
-        std::fprintf(stderr, "usage: main_loop <input> <output> [plotfile root]\n");
+    bool fab_per_box = false, box_checks = false;
+    std::vector<std::string> pos;
+    for (int a = 1; a < argc; ++a) {
+        const std::string s = argv[a];
+        if (s == "--fab-per-box") fab_per_box = true;
+        else if (s == "--box-checks") box_checks = true;
+        else pos.push_back(s);
+    }
+    if (pos.size() != 2 && pos.size() != 3) {
+        std::fprintf(stderr, "usage: main_loop [--fab-per-box] [--box-checks] <input> <output> [plotfile root]\n");
         return 2;
     }
-    const std::string plotroot = argc == 4 ? argv[3] : "";
-    std::ifstream in(argv[1], std::ios::binary);
+    const std::string plotroot = pos.size() == 3 ? pos[2] : "";
+    std::ifstream in(pos[0], std::ios::binary);
     char magic[4];
     in.read(magic, 4);
     int32_t version = 0;
@@ -75,6 +94,7 @@ int main(int argc, char** argv) {
     parameters.max_grid_size = desc.max_grid_size;
     parameters.numberOfMaterials = desc.nmat;
     parameters.REACTIVE = desc.REACTIVE;
+    parameters.fabPerBox = fab_per_box ? 1 : 0;
     std::vector<std::unique_ptr<MieGruneisenEOS>> eos;
     for (int m = 0; m < desc.nmat; ++m) {
         const tj_material& t = desc.mat[m];
@@ -219,8 +239,57 @@ int main(int argc, char** argv) {
 
     if (!plotroot.empty())
         WriteSingleLevelPlotfile(Concatenate(plotroot, n, 5), U1.data, U1.accessPattern.variableNames, geom, t, n);
+
+    int status = 0;
+    if (box_checks) {
+        // level-wide on U
+        U = U1;
+        if (parameters.PLASTIC) plastic.plasticUpdate(U, parameters);
+        if (parameters.SOLID) U.cleanUpV();
+        U.cleanUpAlpha();
+        U.conservativeToPrimitive();
+        const Real dt_level = timeStep.getTimeStep(U, parameters);
+        // box by box on U2, through the per-box tier
+        U2 = U1;
+        U2.stage();
+        Real tmin = std::numeric_limits<Real>::max();
+        const int dim = desc.dim;
+        for (MFIter mfi(U2.data); mfi.isValid(); ++mfi) {
+            const Box& bx = mfi.validbox();
+            BoxAccessCellArray baca(mfi, bx, U2);
+            if (parameters.PLASTIC) plastic.boxPlasticUpdate(baca, parameters);
+            if (parameters.SOLID) baca.cleanUpV();
+            baca.cleanUpAlpha();
+            baca.conservativeToPrimitive();
+            baca.getSoundSpeed();
+            // prop_arr_time over the box, as TimeStep's own MultiFab holds it
+            std::vector<Real> dtb((size_t)bx.numPts() * dim);
+            Array4<Real> arr;
+            arr.p = dtb.data();
+            arr.jstride = bx.length(0);
+            arr.kstride = (long long)bx.length(0) * bx.length(1);
+            arr.nstride = bx.numPts();
+            arr.begin = bx.lo;
+            arr.end = IntVect(bx.hi[0] + 1, bx.hi[1] + 1, bx.hi[2] + 1);
+            arr.ncomp = dim;
+            timeStep.boxTimeStepFunction(baca, arr, parameters);
+            for (Real v : dtb) tmin = v < tmin ? v : tmin;
+            if (baca.contains_nan()) status = 3;
+        }
+        U2.commit();
+        U.stage();
+        U2.stage();
+        bool same = U.host.size() == U2.host.size();
+        for (size_t q = 0; same && q < U.host.size(); ++q)
+            same = std::memcmp(&U.host[q], &U2.host[q], sizeof(Real)) == 0;
+        const Real dt_box = parameters.CFL * tmin;
+        std::cout << "box checks: state " << (same ? "equal" : "DIFFERENT") << ", dt " << dt_box << " vs "
+                  << dt_level << std::endl;
+        if (!same || std::memcmp(&dt_box, &dt_level, sizeof(Real)) != 0) status = 3;
+    }
+
     U1.stage();
-    std::ofstream out(argv[2], std::ios::binary);
+    std::ofstream out(pos[1], std::ios::binary);
     out.write("TJOU", 4);
     out.write(reinterpret_cast<const char*>(&t), sizeof t);
     int32_t n32 = n, ns = nan_stop;
@@ -230,5 +299,5 @@ int main(int argc, char** argv) {
     out.write(reinterpret_cast<const char*>(&count), sizeof count);
     out.write(reinterpret_cast<const char*>(U1.host.data()), sizeof(Real) * U1.host.size());
     std::printf("main_loop: n=%d t=%.17g\n", n, t);
-    return 0;
+    return status;
 }

@@ -37,14 +37,20 @@ def read_output(path: str, pb):
     return t, n, nan_stop, data.reshape((-1,) + pb.shape)
 
 
-def run(pb, final_time: float, max_steps: int, workdir: str, timeout: float = 300.0, plotroot: str = ""):
+def run(pb, final_time: float, max_steps: int, workdir: str, timeout: float = 300.0, plotroot: str = "",
+        fab_per_box: bool = False, box_checks: bool = False):
     if not os.path.exists(DRIVER):
         raise RuntimeError(f"{DRIVER} is missing: build it with `python tenjin_amd/build.py`")
     src = os.path.join(workdir, "in.bin")
     dst = os.path.join(workdir, "out.bin")
     write_input(src, pb, final_time, max_steps)
-    r = subprocess.run([DRIVER, src, dst] + ([plotroot] if plotroot else []), capture_output=True, text=True,
+    flags = (["--fab-per-box"] if fab_per_box else []) + (["--box-checks"] if box_checks else [])
+    r = subprocess.run([DRIVER] + flags + [src, dst] + ([plotroot] if plotroot else []), capture_output=True, text=True,
                        timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"main_loop failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
-    return read_output(dst, pb)
+    out = read_output(dst, pb)
+    if box_checks:
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("box checks:")]
+        assert line and "state equal" in line[0], r.stdout[-2000:]
+    return out

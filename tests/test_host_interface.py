@@ -22,15 +22,15 @@ def test_driver_and_host_library_built():
     assert os.path.exists(os.path.join(os.path.dirname(DRIVER), "..", "..", "tenjin_amd", "libtenjin_host.so"))
 
 
-def _check(pb, final_time, max_steps, tmp_path):
+def _check(pb, final_time, max_steps, tmp_path, **kw):
     from oracle.oracle import Oracle
     from tests import host_driver
 
     print(f"\n[case] host interface {pb.name} n={pb.n[:pb.dim]} thinc={pb.THINC} solid={pb.SOLID} "
-          f"final={final_time} max_steps={max_steps}", flush=True)
+          f"final={final_time} max_steps={max_steps} {kw}", flush=True)
     o = Oracle(pb)
     n_o, t_o = o.run(final_time, max_steps)
-    t_g, n_g, nan_stop, sg = host_driver.run(pb, final_time, max_steps, str(tmp_path))
+    t_g, n_g, nan_stop, sg = host_driver.run(pb, final_time, max_steps, str(tmp_path), **kw)
     assert nan_stop == 0
     assert (n_g, t_g) == (n_o, t_o)
     so = o.get()
@@ -60,6 +60,34 @@ def test_host_interface_plate_impact_2d(tmp_path):
 
     pb = configs.plate_impact(dim=2, n=48, box=24)
     _check(pb, 1.0, 10, tmp_path)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["tube_thinc", "inclusion", "plate2d", "plate3d_rw"])
+@pytest.mark.parametrize("fab_per_box", [True, False])
+def test_host_interface_per_box_tier(tmp_path, kind, fab_per_box):
+    """The reference's loop on CellArrays of one ghost-grown FArrayBox per box
+    (AMReX's MultiFab storage; the library works on the FABs in place) or of
+    one level array, bit-exact against the oracle; then the per-box tier
+    (BoxAccessCellArray per-box methods, boxPlasticUpdate, boxTimeStepFunction)
+    box by box against the level-wide calls, bit for bit (the driver's
+    --box-checks)."""
+    from tenjin_amd import configs
+
+    if kind == "tube_thinc":
+        pb = configs.shock_tube_1d(n=300, max_grid_size=40)
+        pb.THINC = 1
+        steps = 30
+    elif kind == "inclusion":
+        pb = configs.shock_inclusion_2d(nx=96, ny=40, box=24, thinc=1, shock_x=0.55, drift=(150.0, 60.0))
+        steps = 15
+    elif kind == "plate2d":
+        pb = configs.plate_impact(dim=2, n=48, box=24)
+        steps = 8
+    else:
+        pb = configs.plate_impact_rw(dim=3, n=20, box=10)
+        steps = 4
+    _check(pb, 1.0, steps, tmp_path, fab_per_box=fab_per_box, box_checks=True)
 
 
 @pytest.mark.gpu
