@@ -361,10 +361,47 @@ def bench_ensemble(args):
                    "parallelism": "single GPU"},
         "steps_done_min_max": [min(n), max(n)],
         "status_nonzero": sum(1 for x in st if x),
-        "roofline": None,
-        "cpu_baseline": None,
+        # the batch runs as graph replays, whose kernels are not timed one by
+        # one: the roofline is the whole step's (464 algorithmic bytes per
+        # cell-update, FP64 state; the FP32 flux planes are not counted)
+        "roofline": {"bound": "hbm", "kernel": "whole step (graph replays of all k_b_* launches)",
+                     "achieved": round(value * 1e6 * STEP_BYTES["ensemble"] / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(value * 1e6 * STEP_BYTES["ensemble"] / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_ensemble(probs, args.cpu_seconds),
     }
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_ensemble(probs, budget_s: float):
+    """The oracle on every 16th problem of the ensemble (8 1D and 8 2D
+    problems), each from its initial condition for as many steps as its share
+    of the budget allows (at most 20), on the host's CPU share; the rate is
+    the sample's cell-updates over its time."""
+    from oracle.oracle import Oracle, set_threads
+
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, nproc)
+    set_threads(threads)
+    sample = probs[::16]
+    work, el, steps = 0, 0.0, 0
+    for pb in sample:
+        o = Oracle(pb)
+        o.step(0.2 * o.timestep())  # warm-up step (not timed)
+        t0 = time.perf_counter()
+        n = 0
+        while n < 20 and time.perf_counter() - t0 < budget_s / len(sample):
+            dt = o.timestep()
+            o.step(dt if n + 1 >= 5 else 0.2 * dt)
+            n += 1
+        el += time.perf_counter() - t0
+        work += pb.ncells * n
+        steps += n
+        del o
+    return {"value": work / el / 1e6, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": nproc,
+            "sample": f"oracle restatement (FP64), {len(sample)} of the {len(probs)} problems (every 16th), "
+                      f"{steps} steps in all after one warm-up step each, {el:.1f} s, {threads} OpenMP threads"}
 
 
 def bench_plate2d_grad(args):
