@@ -306,18 +306,19 @@ def bench_ensemble(args):
     """256 independent two-material stiffened-gas tubes (128 1D with 500-8000
     cells, 128 2D with sides 64-512, seed 6), MUSCL+THINC+HLLC, mixed
     precision with FP64 update, each with its own dt: the batch runs as
-    replays of one captured hipGraph of 20 loop bodies, with the y-pass
-    segment length autotuned per 2D size bucket.  Work = sum of cells x steps."""
+    replays of one captured hipGraph of 20 loop bodies, the 2D problems on the
+    stage kernels with the y-tile shape autotuned per 2D size bucket.  Work =
+    sum of cells x steps."""
     import torch
-    from tenjin_amd import GpuBatch, TJ_PREC_MIXED, autotune, configs
+    from tenjin_amd import BATCH_TILES, GpuBatch, TJ_PREC_MIXED, autotune, configs
 
     probs = configs.ensemble()
     prec = TJ_PREC_MIXED
     t0 = time.perf_counter()
-    table = autotune.tune_seglen(probs, prec)
+    table = autotune.tune_tiles(probs, prec)
     t_tune = time.perf_counter() - t0
     b = GpuBatch(probs)
-    autotune.apply(b, probs, table)
+    autotune.apply_tiles(b, probs, table)
     b.init_from_prims()
     b.begin_run()
     group = 20
@@ -357,7 +358,9 @@ def bench_ensemble(args):
                                "sides 64-512), MUSCL+THINC+HLLC, per-problem dt",
                    "problems": len(probs), "cells": cells, "precision": "mixed",
                    "launch": f"hipGraph of {group} loop bodies, {nrep} replays",
-                   "seglen_by_bucket": {str(k): v for k, v in table.items()}, "autotune_s": round(t_tune, 2),
+                   "form": "stage kernels (2D), flux-plane passes (1D)",
+                   "tile_by_bucket": {str(autotune.BUCKETS[k]): list(BATCH_TILES[v]) for k, v in table.items()},
+                   "autotune_s": round(t_tune, 2),
                    "parallelism": "single GPU"},
         "steps_done_min_max": [min(n), max(n)],
         "status_nonzero": sum(1 for x in st if x),

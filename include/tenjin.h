@@ -349,8 +349,18 @@ void tj_batch_destroy(tj_batch* batch);
 const char* tj_batch_last_error(const tj_batch* batch);
 /* Zeroes every problem's THINC history planes (as tj_reset_history). */
 int tj_batch_reset_history(tj_batch* batch);
-/* Rows per marching segment of problem p's y pass (tuning; default 16). */
+/* Rows per marching segment of problem p's y pass in the flux-plane form
+ * (tuning; default 16). */
 int tj_batch_set_seglen(tj_batch* batch, int prob, int seglen);
+/* The y-tile shape of problem prob's stage kernels (2D problems): 0 = 16
+ * columns x 32 rows per lane segment (the default), 1 = 8 x 32, 2 = 16 x 16.
+ * Every shape gives the same results; the autotuner picks one per size
+ * bucket.  Drops a captured graph. */
+int tj_batch_set_tile(tj_batch* batch, int prob, int tile);
+/* 2D problems: the stage kernels (on = 1, the default: each RK stage as an
+ * x and a y launch over the tiles, no face fluxes in HBM) or the flux-plane
+ * passes (on = 0).  The same results either way.  Drops a captured graph. */
+int tj_batch_set_stage2d(tj_batch* batch, int on);
 /* tj_init_from_prims for every problem. */
 int tj_batch_init_from_prims(tj_batch* batch, int shift_pressure);
 /* tj_begin_run for every problem (one final time for all). */

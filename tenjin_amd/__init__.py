@@ -19,6 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtenjin.so")
 
 TJ_BOX_P2C, TJ_BOX_C2P, TJ_BOX_DERIVE = 1 << 8, 1 << 9, 1 << 10
+# y-tile shapes of the batch's stage kernels (columns, rows per lane segment)
+BATCH_TILES = ((16, 32), (8, 32), (16, 16))
 TJ_PREC_FP64 = 0
 TJ_PREC_MIXED = 1
 TJ_ST_NAN, TJ_ST_WAVESPEED, TJ_ST_ALPHA_NANS, TJ_ST_ALPHA_SUM, TJ_ST_PLASTIC_DIV = 1, 2, 4, 8, 16
@@ -113,6 +115,7 @@ EXPORTS = [
     "tj_reset_history", "tj_synchronize", "tj_prof_enable", "tj_prof_report", "tj_set_tuning",
     "tj_begin_run", "tj_advance_steps", "tj_lincomb", "tj_graph_capture", "tj_graph_launch",
     "tj_fabset_create", "tj_fabset_view", "tj_fabset_destroy", "tj_level_boxes", "tj_box_op",
+    "tj_batch_set_tile", "tj_batch_set_stage2d",
     "tj_batch_create", "tj_batch_destroy", "tj_batch_last_error", "tj_batch_reset_history", "tj_batch_set_seglen",
     "tj_batch_init_from_prims", "tj_batch_begin_run", "tj_batch_advance", "tj_batch_graph_capture",
     "tj_batch_graph_launch", "tj_batch_clocks", "tj_batch_synchronize",
@@ -179,6 +182,8 @@ def load_library(path: str = LIB_PATH):
     L.tj_batch_last_error.restype = ctypes.c_char_p
     L.tj_batch_reset_history.argtypes = [vp]
     L.tj_batch_set_seglen.argtypes = [vp, ip, ip]
+    L.tj_batch_set_tile.argtypes = [vp, ip, ip]
+    L.tj_batch_set_stage2d.argtypes = [vp, ip]
     L.tj_batch_init_from_prims.argtypes = [vp, ip]
     L.tj_batch_begin_run.argtypes = [vp, dp]
     L.tj_batch_advance.argtypes = [vp, ip, ip]
@@ -565,6 +570,14 @@ class GpuBatch:
 
     def set_seglen(self, prob: int, seglen: int):
         self._chk(self.lib.tj_batch_set_seglen(self.b, prob, seglen), "tj_batch_set_seglen")
+
+    def set_tile(self, prob: int, tile: int):
+        """y-tile shape of problem prob's stage kernels (BATCH_TILES index)."""
+        self._chk(self.lib.tj_batch_set_tile(self.b, prob, tile), "tj_batch_set_tile")
+
+    def set_stage2d(self, on: bool):
+        """2D problems on the stage kernels (True, default) or the flux-plane passes."""
+        self._chk(self.lib.tj_batch_set_stage2d(self.b, int(on)), "tj_batch_set_stage2d")
 
     def begin_run(self, final_time: float = float("inf")):
         self._chk(self.lib.tj_batch_begin_run(self.b, final_time), "tj_batch_begin_run")

@@ -1,11 +1,13 @@
-"""Autotuners of the marching segment length (rows per wave of the y / z
-passes).  Every segment length gives bit-identical results; it only moves
-time, and the best one depends on where the work is (THINC's interface rows
-make some waves much longer than others), which no static model sees.
+"""Autotuners of launch shapes: the y-tile shape of the stage kernels and the
+marching segment length (rows per wave of the y / z passes of the flux-plane
+form).  Every choice gives bit-identical results; it only moves time, and the
+best one depends on where the work is (THINC's interface rows make some
+waves much longer than others), which no static model sees.
 
-- tune_seglen: the batched ensemble, per size bucket of 2D problems, timing a
-  few candidates on that bucket's problems alone (a separate batch, discarded
-  afterwards).
+- tune_tiles: the batched ensemble's stage kernels, per size bucket of 2D
+  problems, timing each tile shape (BATCH_TILES) on that bucket's problems
+  alone (a separate batch, discarded afterwards).
+- tune_seglen: the same for the segment length of the flux-plane form.
 - tune_level_seglen: one level, timing a few candidates (0 = the library's own
   pick) in turns on a scratch level advanced to a given step, discarded
   afterwards."""
@@ -26,10 +28,10 @@ def bucket_of(pb):
     return len(BUCKETS) - 1
 
 
-def tune_seglen(problems, prec: int = 0, steps: int = 6, warmup: int = 2, candidates=CANDIDATES,
-                max_per_bucket: int = 16):
-    """{bucket: best seglen} for the 2D problems of `problems` (time per step
-    of the bucket's batch, median of the timed steps)."""
+def _tune_buckets(problems, prec, steps, warmup, candidates, max_per_bucket, setter, stage2d):
+    """{bucket: best candidate}: each 2D bucket's problems alone in a batch,
+    every candidate set on all of them by setter(batch, p, cand), timed as
+    the median of `steps` single steps after `warmup` steps."""
     from . import GpuBatch
 
     best = {}
@@ -38,10 +40,11 @@ def tune_seglen(problems, prec: int = 0, steps: int = 6, warmup: int = 2, candid
         if not group:
             continue
         b = GpuBatch(group)
+        b.set_stage2d(stage2d)
         timing = {}
         for cand in candidates:
             for p in range(len(group)):
-                b.set_seglen(p, cand)
+                setter(b, p, cand)
             b.init_from_prims()
             b.begin_run()
             b.advance(warmup, prec)
@@ -58,11 +61,36 @@ def tune_seglen(problems, prec: int = 0, steps: int = 6, warmup: int = 2, candid
     return best
 
 
+def tune_tiles(problems, prec: int = 0, steps: int = 6, warmup: int = 2, max_per_bucket: int = 16):
+    """{bucket: best y-tile shape (index into BATCH_TILES)} for the stage
+    kernels of the 2D problems of `problems`."""
+    from . import BATCH_TILES
+
+    return _tune_buckets(problems, prec, steps, warmup, range(len(BATCH_TILES)), max_per_bucket,
+                         lambda b, p, c: b.set_tile(p, c), True)
+
+
+def tune_seglen(problems, prec: int = 0, steps: int = 6, warmup: int = 2, candidates=CANDIDATES,
+                max_per_bucket: int = 16):
+    """{bucket: best seglen} for the flux-plane form of the 2D problems of
+    `problems` (time per step of the bucket's batch, median of the timed
+    steps)."""
+    return _tune_buckets(problems, prec, steps, warmup, candidates, max_per_bucket,
+                         lambda b, p, c: b.set_seglen(p, c), False)
+
+
 def apply(batch, problems, table):
     """Sets each 2D problem's segment length from the bucket table."""
     for p, pb in enumerate(problems):
         if pb.dim == 2 and bucket_of(pb) in table:
             batch.set_seglen(p, table[bucket_of(pb)])
+
+
+def apply_tiles(batch, problems, table):
+    """Sets each 2D problem's y-tile shape from the bucket table."""
+    for p, pb in enumerate(problems):
+        if pb.dim == 2 and bucket_of(pb) in table:
+            batch.set_tile(p, table[bucket_of(pb)])
 
 
 def tune_level_seglen(pb, prec: int = 0, fuse: int = 0, at_step: int = 0, steps: int = 3, rounds: int = 3,

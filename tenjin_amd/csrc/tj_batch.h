@@ -16,6 +16,8 @@ struct BProb {
     MF S, T;
     Faces F[2];
     FacesT<float> Ff[2];
+    UxPlanes ux;  // partial state of the stage kernels (2D problems)
+    int tile;     // y-tile shape of the stage kernels (BATCH_TILES)
     double* urstar;
     double* star[2];
     double* estar[2][2];
@@ -54,6 +56,12 @@ __device__ __forceinline__ Hist bhist(const BProb& P, int d, int q, int e) {
     h.estride = P.estride[d];
     return h;
 }
+
+// y-tile shapes of the stage kernels a 2D problem can take (columns x rows
+// per lane segment); the autotuner picks one per size bucket
+constexpr int NBATCH_TILES = 3;
+constexpr int BATCH_TILE_TC[NBATCH_TILES] = {16, 8, 16};
+constexpr int BATCH_TILE_LPS[NBATCH_TILES] = {32, 32, 16};
 
 template <typename R>
 __device__ __forceinline__ const FacesT<R>& bfaces(const BProb& P, int d);
@@ -106,6 +114,29 @@ __global__ __launch_bounds__(256) void k_b_update(BTab t, unsigned ops) {
     const FacesT<T>& F1 = P.lv.dim > 1 ? bfaces<T>(P, 1) : F0;
     if (STAGE == 1) update_body<2, 1, T>(P.lv, P.S, P.S, P.T, F0, F1, F0, P.clk, ops, gb - t.start[p]);
     else update_body<2, 2, T>(P.lv, P.T, P.S, P.S, F0, F1, F0, P.clk, ops, gb - t.start[p]);
+}
+
+// stage kernels of the 2D problems (tj_stage2d.h): x part over all problems,
+// then the y part over the problems of one tile shape
+template <bool TH, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH && sizeof(R) == 8 ? 4 : 1)))
+void k_b_xstage(BTab t, int which, int q, int e) {
+    const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (gw >= t.start[t.np]) return;
+    const int p = bfind(t.start, t.np, gw);
+    const BProb& P = t.p[p];
+    xstage_body<2, TH, R>(P.lv, which ? P.T : P.S, P.ux, bhist(P, 0, q, e), P.clk, gw - t.start[p]);
+}
+template <bool TH, int TC, int LPS, int STAGE, typename R>
+__global__ __launch_bounds__(TC * LPS) void k_b_ystage(BTab t, int q, int e, unsigned ops) {
+    const long long gb = blockIdx.x;
+    if (gb >= t.start[t.np]) return;
+    const int p = bfind(t.start, t.np, gb);
+    const BProb& P = t.p[p];
+    if (STAGE == 1)
+        ystage_body<2, TH, TC, LPS, 1, R>(P.lv, P.S, P.ux, P.S, P.T, bhist(P, 1, q, e), P.clk, ops, gb - t.start[p]);
+    else
+        ystage_body<2, TH, TC, LPS, 2, R>(P.lv, P.T, P.ux, P.S, P.S, bhist(P, 1, q, e), P.clk, ops, gb - t.start[p]);
 }
 
 __global__ __launch_bounds__(256) void k_b_sound_dt(BTab t) {
@@ -188,8 +219,13 @@ struct tj_batch {
     // dimension groups: records [g0, g1) are 1D, [g1, np) are 2D
     int g1 = 0;
     long long* d_start = nullptr;       // prefix tables, see build_tables
-    long long tot[2][3] = {{0, 0, 0}, {0, 0, 0}};  // [group][x waves, y waves, cell blocks]
-    long long off[2][3] = {{0, 0, 0}, {0, 0, 0}};  // offsets of the tables in d_start
+    // [group][kind]: x waves, y waves, cell blocks of the flux-plane passes;
+    // x-stage waves and y tiles of each shape of the stage kernels
+    static constexpr int NKIND = 4 + tj::NBATCH_TILES;
+    long long tot[2][NKIND] = {};
+    long long off[2][NKIND] = {};
+    // 2D problems: the stage kernels (default) or the flux-plane passes
+    bool stage2d = true;
     long long pass[2] = {0, 0}, dpass[2][2] = {{0, 0}, {0, 0}};
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -214,17 +250,26 @@ int balloc(tj_batch* b, void** p, size_t bytes) {
     return TJ_OK;
 }
 
-// Units per problem: x-pass waves, y-pass waves, 256-cell blocks.
-void units(const tj::BProb& P, int thinc, long long u[3]) {
+// rows a y tile completes (YTile::CPW) for a shape
+int tile_rows(int thinc, int s) { return tj::BATCH_TILE_LPS[s] - 2 * (thinc ? 2 : 1); }
+
+// Units per problem: x-pass waves, y-pass waves, 256-cell blocks; x-stage
+// waves, y tiles of shape 0 .. NBATCH_TILES - 1 (only the problem's own shape).
+void units(const tj::BProb& P, int thinc, bool stage2d, long long u[tj_batch::NKIND]) {
     const tj::Level& lv = P.lv;
+    for (int k = 0; k < tj_batch::NKIND; ++k) u[k] = 0;
     const int per = thinc ? 61 : 63;
     u[0] = (long long)((lv.n[0] + 1 + per - 1) / per) * lv.n[1] * lv.n[2];
     if (lv.dim > 1) {
         const long long nstrip = (lv.n[0] + 63) / 64;
         const long long nseg = (lv.n[1] + 1 + P.seglen - 1) / P.seglen;
         u[1] = nstrip * nseg * lv.n[2];
-    } else {
-        u[1] = 0;
+        if (stage2d) {
+            const int cpw = thinc ? tj::XGeom<true>::CPW : tj::XGeom<false>::CPW;
+            u[3] = (long long)((lv.n[0] + cpw - 1) / cpw) * lv.n[1];
+            const int tc = tj::BATCH_TILE_TC[P.tile], rows = tile_rows(thinc, P.tile);
+            u[4 + P.tile] = (long long)((lv.n[0] + tc - 1) / tc) * ((lv.n[1] + rows - 1) / rows);
+        }
     }
     u[2] = (lv.ncells + 255) / 256;
 }
@@ -234,13 +279,13 @@ int build_tables(tj_batch* b) {
     std::vector<long long> all;
     for (int g = 0; g < 2; ++g) {
         const int lo = g == 0 ? 0 : b->g1, hi = g == 0 ? b->g1 : b->np;
-        for (int kind = 0; kind < 3; ++kind) {
+        for (int kind = 0; kind < tj_batch::NKIND; ++kind) {
             b->off[g][kind] = (long long)all.size();
             long long acc = 0;
             all.push_back(0);
             for (int r = lo; r < hi; ++r) {
-                long long u[3];
-                units(b->h[r], b->thinc, u);
+                long long u[tj_batch::NKIND];
+                units(b->h[r], b->thinc, b->stage2d, u);
                 acc += u[kind];
                 all.push_back(acc);
             }
@@ -271,6 +316,49 @@ void batch_clear_graph(tj_batch* b) {
     b->graph = nullptr;
 }
 
+// The stage kernels of group g (2D): x stage, then y stage per tile shape.
+template <bool TH, typename R>
+void launch_b_stage2d(tj_batch* b, int g, int stage, unsigned ops) {
+    using namespace tj;
+    const int which = stage == 1 ? 0 : 1;
+    {
+        const int q = (int)(b->pass[g] & 1), e = (int)(b->dpass[g][0] & 1);
+        hipLaunchKernelGGL((k_b_xstage<TH, R>), dim3(grid1(b->tot[g][3], 4)), dim3(256), 0, b->stream, tab(b, g, 3), which,
+                           q, e);
+        if (b->thinc) {
+            b->pass[g] += 1;
+            b->dpass[g][0] += 1;
+        }
+    }
+    const int q = (int)(b->pass[g] & 1), e = (int)(b->dpass[g][1] & 1);
+    const unsigned o = stage == 1 ? 0u : ops;
+    for (int sh = 0; sh < NBATCH_TILES; ++sh) {
+        const long long n = b->tot[g][4 + sh];
+        if (!n) continue;
+        const BTab t = tab(b, g, 4 + sh);
+#define TJ_B_YSTAGE(TC, LPS)                                                                                       \
+    do {                                                                                                           \
+        if (stage == 1)                                                                                            \
+            hipLaunchKernelGGL((k_b_ystage<TH, TC, LPS, 1, R>), dim3((unsigned)n), dim3(TC * LPS), 0, b->stream, t, q, \
+                               e, o);                                                                              \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_b_ystage<TH, TC, LPS, 2, R>), dim3((unsigned)n), dim3(TC * LPS), 0, b->stream, t, q, \
+                               e, o);                                                                              \
+    } while (0)
+        static_assert(BATCH_TILE_TC[0] == 16 && BATCH_TILE_LPS[0] == 32 && BATCH_TILE_TC[1] == 8 &&
+                          BATCH_TILE_LPS[1] == 32 && BATCH_TILE_TC[2] == 16 && BATCH_TILE_LPS[2] == 16,
+                      "tile shapes of the launch switch");
+        if (sh == 0) TJ_B_YSTAGE(16, 32);
+        else if (sh == 1) TJ_B_YSTAGE(8, 32);
+        else TJ_B_YSTAGE(16, 16);
+#undef TJ_B_YSTAGE
+    }
+    if (b->thinc) {
+        b->pass[g] += 1;
+        b->dpass[g][1] += 1;
+    }
+}
+
 // One loop body for every problem: driver dt, two RK stages (the post-step
 // chain fused into stage 2), clock advance.
 int batch_step(tj_batch* b, int prec) {
@@ -283,6 +371,13 @@ int batch_step(tj_batch* b, int prec) {
         const int which = stage == 1 ? 0 : 1;
         for (int g = 0; g < 2; ++g) {
             if (b->tot[g][2] == 0) continue;
+            if (g == 1 && b->stage2d) {
+                if (b->thinc && mixed) launch_b_stage2d<true, float>(b, g, stage, ops);
+                else if (b->thinc) launch_b_stage2d<true, double>(b, g, stage, ops);
+                else if (mixed) launch_b_stage2d<false, float>(b, g, stage, ops);
+                else launch_b_stage2d<false, double>(b, g, stage, ops);
+                continue;
+            }
             // x pass
             {
                 const int q = (int)(b->pass[g] & 1), e = (int)(b->dpass[g][0] & 1);
@@ -421,6 +516,12 @@ int tj_batch_create(tj_batch** out, int device, int nprob, const tj_level_desc* 
                     if ((rc = balloc(b, (void**)&P.estar[a][q], sizeof(double) * P.estride[a] * d.nmat)))
                         return *out = b, rc;
         }
+        if (d.dim == 2) {  // the stage kernels' partial state: alpha, alpha rho, rho u (3), E
+            P.ux.cs = P.lv.ncells;
+            if ((rc = balloc(b, (void**)&P.ux.p, sizeof(double) * P.ux.cs * (2 * d.nmat + 4))))
+                return *out = b, rc;
+        }
+        P.tile = 0;
         P.clk = b->d_clk + r;
         P.seglen = 16;
     }
@@ -566,6 +667,24 @@ int tj_batch_graph_launch(tj_batch* b) {
         for (int a = 0; a < 2; ++a) b->dpass[g][a] += b->g_ndpass[g][a];
     }
     return TJ_OK;
+}
+
+int tj_batch_set_tile(tj_batch* b, int prob, int tile) {
+    if (!b || prob < 0 || prob >= b->np || tile < 0 || tile >= tj::NBATCH_TILES) return TJ_EINVAL;
+    for (int r = 0; r < b->np; ++r)
+        if (b->order[r] == prob) b->h[r].tile = tile;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    BHIPCHK(b, hipMemcpy(b->d_p, b->h.data(), sizeof(tj::BProb) * b->np, hipMemcpyHostToDevice));
+    batch_clear_graph(b);
+    return build_tables(b);
+}
+
+int tj_batch_set_stage2d(tj_batch* b, int on) {
+    if (!b) return TJ_EINVAL;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    b->stage2d = on != 0;
+    batch_clear_graph(b);
+    return build_tables(b);
 }
 
 int tj_batch_clocks(tj_batch* b, double* t, int* n, uint32_t* status) {

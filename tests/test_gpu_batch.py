@@ -2,7 +2,8 @@
 uneven size, each with its own materials, states and time step, advanced
 together; every problem must equal the same problem run alone through the
 single-level device loop, bit for bit (state, t, n), in both precision forms,
-with any marching segment length, and through graph replays."""
+on the stage kernels with any y-tile shape and on the flux-plane passes with
+any marching segment length, and through graph replays."""
 from __future__ import annotations
 
 import numpy as np
@@ -48,16 +49,18 @@ def test_batch_matches_single_levels(prec):
 
 
 def test_batch_seglen_and_graph_replays():
-    """Per-problem segment lengths change the schedule only; graph replays of
-    the batch equal eager batch steps."""
+    """The flux-plane form with per-problem segment lengths against the stage
+    kernels (the default for the 2D problems): the schedule changes only;
+    graph replays of the batch equal eager batch steps."""
     from tenjin_amd import GpuBatch
 
     probs = _small_ensemble()
-    print("\n[case] batch seglen + graph", flush=True)
+    print("\n[case] batch flux planes with seglens + graph vs stage kernels", flush=True)
     e = GpuBatch(probs)
     e.init_from_prims()
     e.begin_run()
     g = GpuBatch(probs)
+    g.set_stage2d(False)
     for p in range(len(probs)):
         g.set_seglen(p, 3 + 5 * p)
     g.init_from_prims()
@@ -65,6 +68,31 @@ def test_batch_seglen_and_graph_replays():
     g.graph_capture(4)
     for rep in range(3):
         e.advance(4)
+        g.graph_launch()
+        for p in range(len(probs)):
+            assert np.array_equal(e.state(p), g.state(p), equal_nan=True), f"replay {rep} problem {p}"
+        assert e.clocks() == g.clocks()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_tile_shapes_and_graph_replays(prec):
+    """Every y-tile shape of the stage kernels, mixed over the 2D problems,
+    gives the default's bits; graph replays equal eager steps."""
+    from tenjin_amd import BATCH_TILES, GpuBatch
+
+    probs = _small_ensemble()
+    print(f"\n[case] batch tile shapes {BATCH_TILES} prec={prec} + graph", flush=True)
+    e = GpuBatch(probs)
+    e.init_from_prims()
+    e.begin_run()
+    g = GpuBatch(probs)
+    for p in range(len(probs)):
+        g.set_tile(p, (p + 1) % len(BATCH_TILES))
+    g.init_from_prims()
+    g.begin_run()
+    g.graph_capture(5, prec)
+    for rep in range(3):
+        e.advance(5, prec)
         g.graph_launch()
         for p in range(len(probs)):
             assert np.array_equal(e.state(p), g.state(p), equal_nan=True), f"replay {rep} problem {p}"
