@@ -293,10 +293,16 @@ def bench_tube1d(args):
         "data": "synthetic",
         "config": {"workload": "1D two-material stiffened-gas shock tube, 1000 cells, MUSCL+HLLC, CFL 0.5",
                    "cells": pb.ncells, "precision": "fp64", "launch": f"hipGraph of {group} loop bodies",
+                   "loop_body": "k_clock_dt, k_xstage1d stage 1, k_xstage1d stage 2 (+ clock advance)",
                    "parallelism": "single GPU"},
         "eager_ms_per_step": round(1e3 * el_eager / steps, 5),
         "graph_speedup": round(el_eager / el_graph, 3),
-        "roofline": None,
+        # 1000 cells: each launch is a few waves, so the step is bound by launch
+        # and dependency latency; the whole-step bandwidth shows how far from HBM
+        "roofline": {"bound": "hbm", "kernel": "whole step (graph replays; launch- and latency-bound at 1000 cells)",
+                     "achieved": round(value * 1e6 * STEP_BYTES["tube1d"] / 1e9, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(value * 1e6 * STEP_BYTES["tube1d"] / 1e9 / HBM_PEAK_GBS, 6),
+                     "traffic": None},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(pb, args.cpu_seconds, 1000),
     }
     print(json.dumps(out), flush=True)

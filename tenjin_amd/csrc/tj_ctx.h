@@ -35,6 +35,7 @@ struct tj_ctx {
     long long estride[3] = {0, 0, 0};
     Clock* d_clk = nullptr;
     unsigned long long* d_dtpart = nullptr;  // DT_PARTS partial slots of the CFL minimum
+    unsigned* d_ticket = nullptr;            // wave tickets of the one-launch 1D stage (k_xstage1d), 0 between launches
     unsigned* d_status = nullptr;
     long long pass = 0;
     long long dpass[3] = {0, 0, 0};

@@ -212,6 +212,8 @@ int tj_create(tj_ctx** out, int device) {
     if (hipMemcpy(c->d_clk, &init, sizeof(Clock), hipMemcpyHostToDevice) != hipSuccess) { delete c; return TJ_EHIP; }
     if (hipMalloc(&c->d_dtpart, sizeof(unsigned long long) * DT_PARTS) != hipSuccess) { delete c; return TJ_EHIP; }
     hipLaunchKernelGGL(k_dtpart_reset, dim3(1), dim3(64), 0, c->stream, c->d_dtpart);
+    if (hipMalloc(&c->d_ticket, sizeof(unsigned long long)) != hipSuccess) { delete c; return TJ_EHIP; }
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, c->stream, reinterpret_cast<double*>(c->d_ticket), 1ll);
     if (hipStreamSynchronize(c->stream) != hipSuccess) { delete c; return TJ_EHIP; }
     *out = c;
     return TJ_OK;
@@ -234,6 +236,7 @@ void tj_destroy(tj_ctx* c) {
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
     if (c->d_clk) hipFree(c->d_clk);
     if (c->d_dtpart) hipFree(c->d_dtpart);
+    if (c->d_ticket) hipFree(c->d_ticket);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -923,6 +926,54 @@ static int launch_stage2d(tj_ctx* c, int stage, const MF& in, const MF& base, co
 
 static bool use_stage2d(const tj_ctx* c) { return c->stage2d && c->d_ux && stage2d_level(c); }
 
+// One-dimensional fluid levels without a mixture: each RK stage is one launch
+// of k_xstage1d (the x stage completing its cells); with `tail` stage 2 also
+// advances the clock (k_clock_advance's work).
+static bool use_stage1d(const tj_ctx* c) {
+    return c->stage2d && c->d_ticket && !c->lv.solid && c->lv.dim == 1 && c->lv.mixm < 0;
+}
+extern "C++" {
+template <bool TH, typename R>
+static void launch_stage1d_t(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops, bool tail,
+                             const char* name) {
+    const Level& lv = c->lv;
+    const Hist h = hist_for(c, 0);
+    {
+        Timed tm(c, name);
+        constexpr int CPW = XGeom<TH>::CPW;
+        const unsigned g = grid1((long long)((lv.n[0] + CPW - 1) / CPW) * lv.n[1], 4);
+        if (stage == 1)
+            hipLaunchKernelGGL((k_xstage1d<2, TH, 1, false, R>), dim3(g), dim3(256), 0, c->stream, lv, in, base, out, h,
+                               c->d_clk, ops, c->d_ticket);
+        else if (tail)
+            hipLaunchKernelGGL((k_xstage1d<2, TH, 2, true, R>), dim3(g), dim3(256), 0, c->stream, lv, in, base, out, h,
+                               c->d_clk, ops, c->d_ticket);
+        else
+            hipLaunchKernelGGL((k_xstage1d<2, TH, 2, false, R>), dim3(g), dim3(256), 0, c->stream, lv, in, base, out, h,
+                               c->d_clk, ops, c->d_ticket);
+    }
+    if (TH) {
+        c->pass += 1;
+        c->dpass[0] += 1;
+    }
+}
+}  // extern "C++"
+static int launch_stage1d(tj_ctx* c, int stage, const MF& in, const MF& base, const MF& out, unsigned ops, bool tail,
+                          int prec) {
+    const bool mixed = prec == TJ_PREC_MIXED;
+    static const char* nm[2][2] = {{"xstage1d_1", "xstage1d_2_post"}, {"xstage1d_1_mixed", "xstage1d_2_post_mixed"}};
+    const char* name = nm[mixed][stage - 1];
+    if (c->desc.THINC) {
+        if (mixed) launch_stage1d_t<true, float>(c, stage, in, base, out, ops, tail, name);
+        else launch_stage1d_t<true, double>(c, stage, in, base, out, ops, tail, name);
+    } else {
+        if (mixed) launch_stage1d_t<false, float>(c, stage, in, base, out, ops, tail, name);
+        else launch_stage1d_t<false, double>(c, stage, in, base, out, ops, tail, name);
+    }
+    HIPCHK(c, hipGetLastError());
+    return TJ_OK;
+}
+
 int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_mf* Uout, int prec) {
     if (!c || !c->bound) return TJ_ENOLEVEL;
     TJ_FORWARD_FABS(c, tj_rk_stage(c, stage, Uin, Ubase, Uout, prec), Uin, Ubase, Uout);
@@ -936,6 +987,7 @@ int tj_rk_stage(tj_ctx* c, int stage, const tj_mf* Uin, const tj_mf* Ubase, tj_m
     // the y kernel reads rows of Uin around each tile while other tiles write
     // Uout: an in-place stage (Uout == Uin) takes the flux-plane passes
     if (use_stage2d(c) && !same_view(view(Uout), in)) return launch_stage2d(c, stage, in, base, view(Uout), 0u, prec);
+    if (use_stage1d(c) && !same_view(view(Uout), in)) return launch_stage1d(c, stage, in, base, view(Uout), 0u, false, prec);
     const FuseArgs fz{stage, base, view(Uout), 0u};
     bool fused = false;
     if ((r = launch_fluxes(c, in, &fz, &fused, prec))) return r;
@@ -1002,6 +1054,10 @@ int tj_step(tj_ctx* c, tj_mf* S, tj_mf* T, int prec) {
         hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, c->stream, c->d_clk);
         HIPCHK(c, hipGetLastError());
         return TJ_OK;
+    }
+    if (use_stage1d(c)) {  // two launches; the second advances the clock
+        if ((r = launch_stage1d(c, 1, s, s, t, 0u, false, prec))) return r;
+        return launch_stage1d(c, 2, t, s, s, step_ops(c), true, prec);
     }
     bool fused = false;
     const FuseArgs f1{1, s, t, 0u};
@@ -1549,6 +1605,7 @@ size_t carve_level(tj_ctx* c, const tj_level_desc* d, unsigned char* base, int p
     c->d_clk = cv.take<Clock>(1);
     c->d_dtpart = cv.take<unsigned long long>(DT_PARTS);
     lv.dtpart = c->d_dtpart;
+    c->d_ticket = reinterpret_cast<unsigned*>(cv.take<unsigned long long>(1));
     for (int a = 0; a < 3; ++a) {
         c->d_edge[a] = cv.take<uint8_t>(lv.n[a]);
         lv.edge[a] = c->d_edge[a];
@@ -1599,6 +1656,7 @@ struct StaticCtx {
         c.d_ux = nullptr;
         c.d_clk = nullptr;
         c.d_dtpart = nullptr;
+        c.d_ticket = nullptr;
         c.stream = nullptr;
     }
 };
@@ -1642,6 +1700,8 @@ extern "C" int tj_run_static(const tj_level_desc* desc, tj_mf* S, tj_mf* T, doub
     // the history planes start at zero on every call (they hold the last
     // call's contents otherwise), as after tj_bind_level
     if ((r = TJ_SELF(tj_reset_history)(c))) return r;
+    // the 1D stage's wave tickets start at zero (a fill kernel: no memset node)
+    if ((r = zero(c, reinterpret_cast<double*>(c->d_ticket), 1))) return r;
     return tj_run(c, S, T, final_time, max_steps, prec, check_every, t_out, n_out);
 }
 

@@ -47,9 +47,17 @@ struct XGeom {
     static constexpr int CPW = TH ? 60 : 62;
 };
 
-template <int NM, bool TH, typename R>
+// With FIN = 1 or 2 (one-dimensional levels) the x part is the whole stage:
+// the wave completes its cells itself (finish_cell: c2p; in stage 2 the RK
+// average with Ub and the post-step chain) into Uo instead of writing Ux, and
+// in stage 2 puts its CFL candidate into the partial slots.
+struct XFin {
+    MF Ub, Uo;
+    unsigned ops;
+};
+template <int NM, bool TH, typename R, int FIN = 0>
 __device__ __forceinline__ void xstage_body(const Level& lv, const MF& U, const UxPlanes& Ux, const Hist& h, Clock* clk,
-                                            long long wave) {
+                                            long long wave, const XFin& fin = XFin{}) {
     constexpr int NF = 2 * NM + 5;
     if (clk->done) return;
     constexpr int H = XGeom<TH>::H, CPW = XGeom<TH>::CPW;
@@ -176,6 +184,7 @@ __device__ __forceinline__ void xstage_body(const Level& lv, const MF& U, const 
     const int lp = lane < 63 ? lane + 1 : 63;
 #pragma unroll
     for (int q = 0; q < NF; ++q) Fh[q] = __shfl(Fv[q], lp, 64);
+    double cand = __builtin_huge_val();
     if (own) {
         const long long cell = (long long)j * nx + c;
         double al[NM], ar[NM], ru[3], E, a0[NM];
@@ -183,14 +192,21 @@ __device__ __forceinline__ void xstage_body(const Level& lv, const MF& U, const 
 #pragma unroll
         for (int m = 0; m < NM; ++m) a0[m] = al[m];
         accum_dir<NM, R>(al, ar, ru, E, a0, clk->dt / lv.dx[0], Fv, Fh);
+        if constexpr (FIN == 0) {
 #pragma unroll
-        for (int m = 0; m < NM; ++m) {
-            Ux.at(m, cell) = al[m];
-            Ux.at(NM + m, cell) = ar[m];
+            for (int m = 0; m < NM; ++m) {
+                Ux.at(m, cell) = al[m];
+                Ux.at(NM + m, cell) = ar[m];
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Ux.at(2 * NM + q, cell) = ru[q];
+            Ux.at(2 * NM + 3, cell) = E;
+        } else {
+            finish_cell<NM, FIN>(lv, fin.Ub, fin.Uo, c, j, k, al, ar, ru, E, clk->dt, fin.ops, cand, st);
         }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) Ux.at(2 * NM + q, cell) = ru[q];
-        Ux.at(2 * NM + 3, cell) = E;
+    }
+    if constexpr (FIN == 2) {
+        if (fin.ops & 8u) wave_min_to(cand, dt_slot(lv, clk));
     }
     if (st) atomicOr(&clk->status, st);
 }
@@ -201,6 +217,36 @@ void k_xstage(Level lv, MF U, UxPlanes Ux, Hist h, Clock* clk) {
     const long long nb = gridDim.x;
     const long long b = xcd_block(blockIdx.x, nb, 2);
     xstage_body<NM, TH, R>(lv, U, Ux, h, clk, b * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
+
+// One RK stage of a one-dimensional fluid level in one launch (FIN = STAGE).
+// With TAIL the launch also ends the step: every wave, done or not, takes a
+// ticket after its work, and the wave holding the last one advances the clock
+// as k_clock_advance does and re-arms the ticket (no wave waits on another).
+template <int NM, bool TH, int STAGE, bool TAIL, typename R = double>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH && sizeof(R) == 8 ? 4 : 1)))
+void k_xstage1d(Level lv, MF U, MF Ub, MF Uo, Hist h, Clock* clk, unsigned ops, unsigned* ticket) {
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    xstage_body<NM, TH, R, STAGE>(lv, U, UxPlanes{}, h, clk, wave, XFin{Ub, Uo, ops});
+    if constexpr (TAIL) {
+        if ((threadIdx.x & 63) == 0) {
+            __threadfence();
+            const unsigned waves = gridDim.x * (blockDim.x >> 6);
+            if (atomicAdd(ticket, 1u) == waves - 1) {
+                __threadfence();
+                *ticket = 0u;
+                if (!clk->done) {
+                    if (atomicOr(&clk->status, 0u) & ST_STOP) {
+                        clk->done = 1;
+                    } else {
+                        clk->t += clk->dt;
+                        clk->n += 1;
+                    }
+                }
+                __threadfence();
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ y kernel
