@@ -338,11 +338,13 @@ void tj_fabset_destroy(tj_fabset* set);
  * tj_run.  The reference has no batched driver; this serves ensembles of
  * small problems that would each leave most of the GPU idle.
  * Restrictions: fluid levels (SOLID = 0), two materials, one THINC setting
- * for the whole batch.  S[p] and T[p] are caller-owned device views (state
+ * for the whole batch, at most TJ_BATCH_MAX_PROBLEMS problems (TJ_EINVAL
+ * beyond).  S[p] and T[p] are caller-owned device views (state
  * and stage buffer) of problem p, held by address until tj_batch_destroy.
  * On an error from tj_batch_create, *batch may hold a batch for
  * tj_batch_last_error; destroy it. */
 typedef struct tj_batch tj_batch;
+#define TJ_BATCH_MAX_PROBLEMS 65535
 int tj_batch_create(tj_batch** batch, int device, int nprob, const tj_level_desc* descs, const tj_mf* S,
                     const tj_mf* T);
 void tj_batch_destroy(tj_batch* batch);

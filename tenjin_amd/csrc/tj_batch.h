@@ -33,16 +33,35 @@ struct BTab {
     int np;
 };
 
+// The problem records and prefix tables are read-only while a kernel runs.
+// Reading them through the constant address space, at a wave-uniform index,
+// lets the backend use scalar loads and keep or re-load the fields in SGPRs
+// across the kernel's own global stores.  Through a plain pointer at an index
+// derived from threadIdx.x, every field of the problem's Level was a per-lane
+// vector load (measured: 188 vector loads per x-stage wave against 75 on a
+// single level, whose Level is a kernel argument).
+template <typename T>
+__device__ __forceinline__ const T& ro_at(const T* base, long long i) {
+    using CT = const __attribute__((address_space(4))) T;
+    return *(const T*)((CT*)base + i);
+}
+// the wave's index in the grid, known to the compiler to be wave-uniform
+__device__ __forceinline__ long long bwave() {
+    return (long long)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // largest q with start[q] <= u (start[0] = 0, u < start[np])
 __device__ __forceinline__ int bfind(const long long* start, int np, long long u) {
     int lo = 0, hi = np - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= u) lo = mid;
+        if (ro_at(start, mid) <= u) lo = mid;
         else hi = mid - 1;
     }
     return lo;
 }
+// the number of work units of a table (its last prefix entry)
+__device__ __forceinline__ long long bunits(const BTab& t) { return ro_at(t.start, t.np); }
 
 __device__ __forceinline__ Hist bhist(const BProb& P, int d, int q, int e) {
     Hist h{};
@@ -73,24 +92,24 @@ __device__ __forceinline__ const FacesT<float>& bfaces<float>(const BProb& P, in
 // x pass over all problems of a group; which = 0 reads S, 1 reads T
 template <bool TH, typename R>
 __global__ __launch_bounds__(256) void k_b_flux_x(BTab t, int which, int q, int e) {
-    const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (gw >= t.start[t.np]) return;
+    const long long gw = bwave();
+    if (gw >= bunits(t)) return;
     const int p = bfind(t.start, t.np, gw);
-    const BProb& P = t.p[p];
-    flux_x_body<2, TH, R>(P.lv, which ? P.T : P.S, bfaces<R>(P, 0), bhist(P, 0, q, e), P.clk, gw - t.start[p]);
+    const BProb& P = ro_at(t.p, p);
+    flux_x_body<2, TH, R>(P.lv, which ? P.T : P.S, bfaces<R>(P, 0), bhist(P, 0, q, e), P.clk, gw - ro_at(t.start, p));
 }
 
 // y pass (marching) over all problems of a 2D group
 template <bool TH, typename R>
 __device__ __forceinline__ void b_flux_y(const BTab& t, int which, int q, int e) {
-    const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (gw >= t.start[t.np]) return;
+    const long long gw = bwave();
+    if (gw >= bunits(t)) return;
     const int p = bfind(t.start, t.np, gw);
-    const BProb& P = t.p[p];
+    const BProb& P = ro_at(t.p, p);
     const MF z{};
     const FacesT<R> zf{};
     flux_march_body<2, TH, 1, false, 1, R>(P.lv, which ? P.T : P.S, bfaces<R>(P, 1), bhist(P, 1, q, e), P.seglen,
-                                           P.clk, z, z, zf, zf, 0u, gw - t.start[p]);
+                                           P.clk, z, z, zf, zf, 0u, gw - ro_at(t.start, p));
 }
 template <bool TH, typename R>
 __global__ __launch_bounds__(256) void k_b_flux_y(BTab t, int which, int q, int e) {
@@ -107,13 +126,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
 template <int STAGE, typename T>
 __global__ __launch_bounds__(256) void k_b_update(BTab t, unsigned ops) {
     const long long gb = blockIdx.x;
-    if (gb >= t.start[t.np]) return;  // uniform per block
+    if (gb >= bunits(t)) return;  // uniform per block
     const int p = bfind(t.start, t.np, gb);
-    const BProb& P = t.p[p];
+    const BProb& P = ro_at(t.p, p);
     const FacesT<T>& F0 = bfaces<T>(P, 0);
     const FacesT<T>& F1 = P.lv.dim > 1 ? bfaces<T>(P, 1) : F0;
-    if (STAGE == 1) update_body<2, 1, T>(P.lv, P.S, P.S, P.T, F0, F1, F0, P.clk, ops, gb - t.start[p]);
-    else update_body<2, 2, T>(P.lv, P.T, P.S, P.S, F0, F1, F0, P.clk, ops, gb - t.start[p]);
+    if (STAGE == 1) update_body<2, 1, T>(P.lv, P.S, P.S, P.T, F0, F1, F0, P.clk, ops, gb - ro_at(t.start, p));
+    else update_body<2, 2, T>(P.lv, P.T, P.S, P.S, F0, F1, F0, P.clk, ops, gb - ro_at(t.start, p));
 }
 
 // stage kernels of the 2D problems (tj_stage2d.h): x part over all problems,
@@ -121,38 +140,41 @@ __global__ __launch_bounds__(256) void k_b_update(BTab t, unsigned ops) {
 template <bool TH, typename R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH && sizeof(R) == 8 ? 4 : 1)))
 void k_b_xstage(BTab t, int which, int q, int e) {
-    const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (gw >= t.start[t.np]) return;
+    // blocks renumbered per XCD as on a single 2D level (xcd_block): each
+    // XCD's L2 then holds neighbouring rows of one problem
+    const long long gw = xcd_block(blockIdx.x, gridDim.x, 2) * (blockDim.x >> 6) +
+                         __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (gw >= bunits(t)) return;
     const int p = bfind(t.start, t.np, gw);
-    const BProb& P = t.p[p];
-    xstage_body<2, TH, R>(P.lv, which ? P.T : P.S, P.ux, bhist(P, 0, q, e), P.clk, gw - t.start[p]);
+    const BProb& P = ro_at(t.p, p);
+    xstage_body<2, TH, R>(P.lv, which ? P.T : P.S, P.ux, bhist(P, 0, q, e), P.clk, gw - ro_at(t.start, p));
 }
 template <bool TH, int TC, int LPS, int STAGE, typename R>
 __global__ __launch_bounds__(TC * LPS) void k_b_ystage(BTab t, int q, int e, unsigned ops) {
     const long long gb = blockIdx.x;
-    if (gb >= t.start[t.np]) return;
+    if (gb >= bunits(t)) return;
     const int p = bfind(t.start, t.np, gb);
-    const BProb& P = t.p[p];
+    const BProb& P = ro_at(t.p, p);
     if (STAGE == 1)
-        ystage_body<2, TH, TC, LPS, 1, R>(P.lv, P.S, P.ux, P.S, P.T, bhist(P, 1, q, e), P.clk, ops, gb - t.start[p]);
+        ystage_body<2, TH, TC, LPS, 1, R>(P.lv, P.S, P.ux, P.S, P.T, bhist(P, 1, q, e), P.clk, ops, gb - ro_at(t.start, p));
     else
-        ystage_body<2, TH, TC, LPS, 2, R>(P.lv, P.T, P.ux, P.S, P.S, bhist(P, 1, q, e), P.clk, ops, gb - t.start[p]);
+        ystage_body<2, TH, TC, LPS, 2, R>(P.lv, P.T, P.ux, P.S, P.S, bhist(P, 1, q, e), P.clk, ops, gb - ro_at(t.start, p));
 }
 
 __global__ __launch_bounds__(256) void k_b_sound_dt(BTab t) {
     const long long gb = blockIdx.x;
-    if (gb >= t.start[t.np]) return;
+    if (gb >= bunits(t)) return;
     const int p = bfind(t.start, t.np, gb);
-    const BProb& P = t.p[p];
-    sound_dt_body<2>(P.lv, P.S, P.clk, gb - t.start[p]);
+    const BProb& P = ro_at(t.p, p);
+    sound_dt_body<2>(P.lv, P.S, P.clk, gb - ro_at(t.start, p));
 }
 
 __global__ __launch_bounds__(256) void k_b_init_p2c(BTab t, int shift) {
     const long long gb = blockIdx.x;
-    if (gb >= t.start[t.np]) return;
+    if (gb >= bunits(t)) return;
     const int p = bfind(t.start, t.np, gb);
-    const BProb& P = t.p[p];
-    init_p2c_body<2>(P.lv, P.S, shift, gb - t.start[p]);
+    const BProb& P = ro_at(t.p, p);
+    init_p2c_body<2>(P.lv, P.S, shift, gb - ro_at(t.start, p));
 }
 
 // the clock kernels of the single level, one thread per problem
@@ -187,6 +209,26 @@ __global__ void k_b_clock_advance(const BProb* P, int np) {
     }
     clk->t += clk->dt;
     clk->n += 1;
+}
+// zeroes the THINC history planes of every problem in one launch: block
+// row y is problem y, its blocks stride over that problem's planes
+__global__ __launch_bounds__(256) void k_b_zero_hist(const BProb* P) {
+    const BProb& Q = P[blockIdx.y];
+    if (!Q.lv.thinc) return;
+    const long long pl = Q.lv.ncells * Q.lv.nmat;
+    const long long id0 = (long long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = id0; i < pl; i += stride) {
+        Q.urstar[i] = 0.0;
+        Q.star[0][i] = 0.0;
+        Q.star[1][i] = 0.0;
+    }
+    for (int a = 0; a < Q.lv.dim; ++a) {
+        const long long n = Q.estride[a] * Q.lv.nmat;
+        for (long long i = id0; i < n; i += stride) {
+            Q.estar[a][0][i] = 0.0;
+            Q.estar[a][1][i] = 0.0;
+        }
+    }
 }
 __global__ void k_b_clock_begin(const BProb* P, int np, double final_time) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -429,6 +471,7 @@ int tj_batch_create(tj_batch** out, int device, int nprob, const tj_level_desc* 
     using namespace tj;
     if (!out || nprob < 1 || !descs || !S || !T) return TJ_EINVAL;
     *out = nullptr;
+    if (nprob > TJ_BATCH_MAX_PROBLEMS) return TJ_EINVAL;  // one launch row per problem (grid y)
     tj_batch* b = new tj_batch();
     b->device = device;
     auto fail = [&](int r) {
@@ -547,23 +590,16 @@ int tj_batch_reset_history(tj_batch* b) {
     using namespace tj;
     if (!b || !b->d_p) return TJ_EINVAL;
     BHIPCHK(b, hipSetDevice(b->device));
-    for (int r = 0; r < b->np; ++r) {
-        BProb& P = b->h[r];
-        if (!P.lv.thinc) continue;
-        const long long pl = P.lv.ncells * P.lv.nmat;
-        hipLaunchKernelGGL(k_zero, dim3(grid1(pl, 256)), dim3(256), 0, b->stream, P.urstar, pl);
-        for (int q = 0; q < 2; ++q) hipLaunchKernelGGL(k_zero, dim3(grid1(pl, 256)), dim3(256), 0, b->stream, P.star[q], pl);
-        for (int a = 0; a < P.lv.dim; ++a)
-            for (int q = 0; q < 2; ++q) {
-                const long long n = P.estride[a] * P.lv.nmat;
-                hipLaunchKernelGGL(k_zero, dim3(grid1(n, 256)), dim3(256), 0, b->stream, P.estar[a][q], n);
-            }
-    }
     for (int g = 0; g < 2; ++g) {
         b->pass[g] = 0;
         b->dpass[g][0] = b->dpass[g][1] = 0;
     }
     BHIPCHK(b, hipMemcpyAsync(b->d_p, b->h.data(), sizeof(BProb) * b->np, hipMemcpyHostToDevice, b->stream));
+    // the history planes of all problems in one launch (the records are on the device now)
+    if (b->thinc) {
+        hipLaunchKernelGGL(k_b_zero_hist, dim3(64, (unsigned)b->np), dim3(256), 0, b->stream, b->d_p);
+        BHIPCHK(b, hipGetLastError());
+    }
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     return build_tables(b);
 }

@@ -181,6 +181,31 @@ __device__ __forceinline__ bool tisnan(float x) { return __builtin_isnan(x); }
 __device__ __forceinline__ bool tisinf(double x) { return __builtin_isinf(x); }
 __device__ __forceinline__ bool tisinf(float x) { return __builtin_isinf(x); }
 
+// ------------------------------------------------------------------ neighbour-lane moves
+// x of lane - 1 (lane_dn) or lane + 1 (lane_up) within the wave, lane 0 (63)
+// keeping its own: DPP wave_shr:1 / wave_shl:1 row moves (a VALU mov on the
+// register file) in place of __shfl's ds_bpermute through the LDS crossbar.
+// Every lane of the wave must be active at the call.  bound_ctrl off: the lane
+// with no source keeps `old`, which is x itself.
+__device__ __forceinline__ int lane_dn(int x) { return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int lane_up(int x) { return __builtin_amdgcn_update_dpp(x, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_dn(float x) {
+    return __builtin_bit_cast(float, lane_dn(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ float lane_up(float x) {
+    return __builtin_bit_cast(float, lane_up(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double lane_dn(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)lane_dn((int)b), hi = (unsigned)lane_dn((int)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double lane_up(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)lane_up((int)b), hi = (unsigned)lane_up((int)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // ------------------------------------------------------------------ math
 // Project-written exp/expm1/tanh/cosh/x^(-1/3): Cody-Waite reduction with the
 // fdlibm split of ln2 and Taylor/Horner polynomials; the oracle evaluates the

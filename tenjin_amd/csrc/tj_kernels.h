@@ -126,16 +126,16 @@ __device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const 
     FSt<NM, R> Lf;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        Lf.al[m] = __shfl(SRs.al[m], lane > 0 ? lane - 1 : 0, 64);
-        Lf.ar[m] = __shfl(SRs.ar[m], lane > 0 ? lane - 1 : 0, 64);
+        Lf.al[m] = lane_dn(SRs.al[m]);
+        Lf.ar[m] = lane_dn(SRs.ar[m]);
     }
-    Lf.rho = __shfl(SRs.rho, lane > 0 ? lane - 1 : 0, 64);
+    Lf.rho = lane_dn(SRs.rho);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) Lf.u[q] = __shfl(SRs.u[q], lane > 0 ? lane - 1 : 0, 64);
-    Lf.E = __shfl(SRs.E, lane > 0 ? lane - 1 : 0, 64);
-    Lf.p = __shfl(SRs.p, lane > 0 ? lane - 1 : 0, 64);
-    Lf.a = __shfl(SRs.a, lane > 0 ? lane - 1 : 0, 64);
-    if (MX) Lf.arl = __shfl(SRs.arl, lane > 0 ? lane - 1 : 0, 64);
+    for (int q = 0; q < 3; ++q) Lf.u[q] = lane_dn(SRs.u[q]);
+    Lf.E = lane_dn(SRs.E);
+    Lf.p = lane_dn(SRs.p);
+    Lf.a = lane_dn(SRs.a);
+    if (MX) Lf.arl = lane_dn(SRs.arl);
 
     if (lane >= H && lane < H + PER && c >= 0 && c <= nx) {
         if (c == 0) {

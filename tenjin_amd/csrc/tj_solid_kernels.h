@@ -30,6 +30,30 @@ __device__ __forceinline__ SSt<NM, R> shfl_state(const SSt<NM, R>& S, int src) {
     return T;
 }
 
+// lane - 1's state (lane 0 keeps its own) by DPP row moves, as lane_dn
+template <int NM, typename R>
+__device__ __forceinline__ SSt<NM, R> state_dn(const SSt<NM, R>& S) {
+    SSt<NM, R> T;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        T.al[m] = lane_dn(S.al[m]);
+        T.ar[m] = lane_dn(S.ar[m]);
+        T.are[m] = lane_dn(S.are[m]);
+    }
+    T.rho = lane_dn(S.rho);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        T.u[q] = lane_dn(S.u[q]);
+        T.sig[q] = lane_dn(S.sig[q]);
+    }
+    T.E = lane_dn(S.E);
+    T.p = lane_dn(S.p);
+    T.a = lane_dn(S.a);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) T.V[q] = lane_dn(S.V[q]);
+    return T;
+}
+
 // The left image of a reconstructed state across a reflective face:
 // reflect_odd flips the normal velocity and momentum only (boundary.cpp:3-42).
 template <int NM, typename R>
@@ -62,7 +86,7 @@ __global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, FacesT<R> F, C
         p2c_sound_s<NM, RW, R>(lv, PL, 0, SLs);
         p2c_sound_s<NM, RW, R>(lv, PR, 0, SRs);
     }
-    SSt<NM, R> Lf = shfl_state<NM, R>(SRs, lane > 0 ? lane - 1 : 0);
+    SSt<NM, R> Lf = state_dn<NM, R>(SRs);
     unsigned st = 0;
     if (lane >= 1 && c >= 0 && c <= nx) {
         if (c == 0) {

@@ -141,18 +141,17 @@ __device__ __forceinline__ void xstage_body(const Level& lv, const MF& U, const 
 
     // left state of face c: UR(c-1) from the neighbouring lane
     FSt<NM, R> Lf;
-    const int lm = lane > 0 ? lane - 1 : 0;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        Lf.al[m] = __shfl(SRs.al[m], lm, 64);
-        Lf.ar[m] = __shfl(SRs.ar[m], lm, 64);
+        Lf.al[m] = lane_dn(SRs.al[m]);
+        Lf.ar[m] = lane_dn(SRs.ar[m]);
     }
-    Lf.rho = __shfl(SRs.rho, lm, 64);
+    Lf.rho = lane_dn(SRs.rho);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) Lf.u[q] = __shfl(SRs.u[q], lm, 64);
-    Lf.E = __shfl(SRs.E, lm, 64);
-    Lf.p = __shfl(SRs.p, lm, 64);
-    Lf.a = __shfl(SRs.a, lm, 64);
+    for (int q = 0; q < 3; ++q) Lf.u[q] = lane_dn(SRs.u[q]);
+    Lf.E = lane_dn(SRs.E);
+    Lf.p = lane_dn(SRs.p);
+    Lf.a = lane_dn(SRs.a);
 
     // face c: lanes H .. H + CPW (faces s0 .. s0 + CPW), up to the domain's top face
     const bool face = lane >= H && lane <= H + CPW && c >= 0 && c <= nx;
@@ -181,9 +180,8 @@ __device__ __forceinline__ void xstage_body(const Level& lv, const MF& U, const 
     }
     // upper face of cell c from lane + 1
     R Fh[NF];
-    const int lp = lane < 63 ? lane + 1 : 63;
 #pragma unroll
-    for (int q = 0; q < NF; ++q) Fh[q] = __shfl(Fv[q], lp, 64);
+    for (int q = 0; q < NF; ++q) Fh[q] = lane_up(Fv[q]);
     double cand = __builtin_huge_val();
     if (own) {
         const long long cell = (long long)j * nx + c;
@@ -462,19 +460,21 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
     FSt<NM, R> SLs, SRs;
     p2c_sound<NM, R>(lv, PL, SLs);
     p2c_sound<NM, R>(lv, PR, SRs);
+    // UR of the row below from lane - 1; the segment's first lane takes the
+    // previous segment's value, but it is a halo lane (sl = 0 < H) whose Lf is
+    // never read
     FSt<NM, R> Lf;
-    const int lm = lane > lo ? lane - 1 : lo;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-        Lf.al[m] = __shfl(SRs.al[m], lm, 64);
-        Lf.ar[m] = __shfl(SRs.ar[m], lm, 64);
+        Lf.al[m] = lane_dn(SRs.al[m]);
+        Lf.ar[m] = lane_dn(SRs.ar[m]);
     }
-    Lf.rho = __shfl(SRs.rho, lm, 64);
+    Lf.rho = lane_dn(SRs.rho);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) Lf.u[q] = __shfl(SRs.u[q], lm, 64);
-    Lf.E = __shfl(SRs.E, lm, 64);
-    Lf.p = __shfl(SRs.p, lm, 64);
-    Lf.a = __shfl(SRs.a, lm, 64);
+    for (int q = 0; q < 3; ++q) Lf.u[q] = lane_dn(SRs.u[q]);
+    Lf.E = lane_dn(SRs.E);
+    Lf.p = lane_dn(SRs.p);
+    Lf.a = lane_dn(SRs.a);
 
     const bool face = colok && sl >= H && sl <= H + CPW && r >= 0 && r <= ny;
     R Fv[NF];
