@@ -47,7 +47,7 @@ __device__ __forceinline__ const T& ro_at(const T* base, long long i) {
 }
 // the wave's index in the grid, known to the compiler to be wave-uniform
 __device__ __forceinline__ long long bwave() {
-    return (long long)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    return (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
 }
 
 // largest q with start[q] <= u (start[0] = 0, u < start[np])
@@ -143,7 +143,7 @@ void k_b_xstage(BTab t, int which, int q, int e) {
     // blocks renumbered per XCD as on a single 2D level (xcd_block): each
     // XCD's L2 then holds neighbouring rows of one problem
     const long long gw = xcd_block(blockIdx.x, gridDim.x, 2) * (blockDim.x >> 6) +
-                         __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+                         wave_in_block();
     if (gw >= bunits(t)) return;
     const int p = bfind(t.start, t.np, gw);
     const BProb& P = ro_at(t.p, p);

@@ -181,6 +181,12 @@ __device__ __forceinline__ bool tisnan(float x) { return __builtin_isnan(x); }
 __device__ __forceinline__ bool tisinf(double x) { return __builtin_isinf(x); }
 __device__ __forceinline__ bool tisinf(float x) { return __builtin_isinf(x); }
 
+// ------------------------------------------------------------------ wave index
+// The wave's index within its block, marked wave-uniform for the compiler
+// (readfirstlane): what is derived from it (rows, segments, row addresses,
+// per-wave LDS bases) is then scalar work in SGPRs, not per-lane VALU work.
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // ------------------------------------------------------------------ neighbour-lane moves
 // x of lane - 1 (lane_dn) or lane + 1 (lane_up) within the wave, lane 0 (63)
 // keeping its own: DPP wave_shr:1 / wave_shl:1 row moves (a VALU mov on the

@@ -169,7 +169,7 @@ __device__ __forceinline__ void flux_x_body(const Level& lv, const MF& U, const 
 template <int NM, bool TH, typename R = double, bool MX = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH && !MX && sizeof(R) == 8 ? 4 : 1)))
 void k_flux_x(Level lv, MF U, FacesT<R> F, Hist h, Clock* clk) {
-    flux_x_body<NM, TH, R, MX>(lv, U, F, h, clk, (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    flux_x_body<NM, TH, R, MX>(lv, U, F, h, clk, (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block());
 }
 
 // ------------------------------------------------------------------ XCD-aware block order
@@ -513,7 +513,7 @@ __device__ __forceinline__ void flux_march_body(const Level& lv, const MF& U, co
     __shared__ R shm[4 * Slots::N * 64];
     if (clk->done) return;
     const int lane = threadIdx.x & 63;
-    R* const shR = shm + (threadIdx.x >> 6) * Slots::N * 64;  // right state of row r-1
+    R* const shR = shm + wave_in_block() * Slots::N * 64;  // right state of row r-1
     R* const shAp = shR + Slots::FST * 64;                      // alpha reconstruction of row r-1
     R* const shAc = shAp + Slots::AREC * 64;                    // alpha reconstruction of row r
     R* const shF = shR + (Slots::FST + (TH ? 2 * Slots::AREC : 0)) * 64;  // lower face flux (fused)
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(256) void k_flux_march(Level lv, MF U, FacesT<R> F,
                                                     MF Ub = MF{}, MF Uo = MF{}, FacesT<R> Fx = FacesT<R>{},
                                                     FacesT<R> Fy = FacesT<R>{}, unsigned ops = 0u) {
     flux_march_body<NM, TH, D, FUSE, STAGE, R, MX>(lv, U, F, h, seglen, clk, Ub, Uo, Fx, Fy, ops,
-                                                   (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+                                                   (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block());
 }
 // The same pass held to two waves per SIMD (the FP64 THINC pass: its few
 // registers beyond 256 spill to scratch on rarely taken paths, and the second
@@ -715,7 +715,7 @@ template <int NM, bool TH, int D, typename R = double, bool MX = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_flux_march_w2(
     Level lv, MF U, FacesT<R> F, Hist h, int seglen, Clock* clk) {
     flux_march_body<NM, TH, D, false, 1, R, MX>(lv, U, F, h, seglen, clk, MF{}, MF{}, FacesT<R>{}, FacesT<R>{}, 0u,
-                                                (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+                                                (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block());
 }
 
 // ------------------------------------------------------------------ update

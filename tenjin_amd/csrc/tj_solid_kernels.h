@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_flux_x_s(Level lv, MF U, FacesT<R> F, C
     constexpr int PER = 63;
     constexpr int NF = 3 * NM + 17;
     const int lane = threadIdx.x & 63;
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const int nseg = (nx + 1 + PER - 1) / PER;
     if (wave >= (long long)nseg * ny * nz) return;
@@ -166,8 +166,8 @@ __device__ __forceinline__ void flux_lmarch_s_body(const Level& lv, const MF& U,
     constexpr int NS = SStSlots<NM, R>::N;
     __shared__ R shm[4 * NS * 64];
     const int lane = threadIdx.x & 63;
-    R* sh = shm + (threadIdx.x >> 6) * NS * 64;
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    R* sh = shm + wave_in_block() * NS * 64;
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     const int nd = lv.n[D];
     const int no = (D == 1) ? nz : ny;

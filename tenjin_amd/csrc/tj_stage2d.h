@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH && sizeo
 void k_xstage(Level lv, MF U, UxPlanes Ux, Hist h, Clock* clk) {
     const long long nb = gridDim.x;
     const long long b = xcd_block(blockIdx.x, nb, 2);
-    xstage_body<NM, TH, R>(lv, U, Ux, h, clk, b * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    xstage_body<NM, TH, R>(lv, U, Ux, h, clk, b * (blockDim.x >> 6) + wave_in_block());
 }
 
 // One RK stage of a one-dimensional fluid level in one launch (FIN = STAGE).
@@ -224,7 +224,7 @@ void k_xstage(Level lv, MF U, UxPlanes Ux, Hist h, Clock* clk) {
 template <int NM, bool TH, int STAGE, bool TAIL, typename R = double>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH && sizeof(R) == 8 ? 4 : 1)))
 void k_xstage1d(Level lv, MF U, MF Ub, MF Uo, Hist h, Clock* clk, unsigned ops, unsigned* ticket) {
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
     xstage_body<NM, TH, R, STAGE>(lv, U, UxPlanes{}, h, clk, wave, XFin{Ub, Uo, ops});
     if constexpr (TAIL) {
         if ((threadIdx.x & 63) == 0) {

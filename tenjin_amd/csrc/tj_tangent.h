@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_tflux_ts(Level lv, DMF<K> U, Faces F, C
     using T = Dual<K>;
     const int lane = threadIdx.x & 63;
     const int ld = lane / LT, lt = lane % LT;
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2], nd = lv.n[D];
     const int nseg = (nd + 1 + PER - 1) / PER;
     const int nxt = D == 0 ? 1 : (nx + LT - 1) / LT;
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256) void k_tflux_tf(Level lv, DMF<K> U, Faces F, T
     using T = Dual<K>;
     const int lane = threadIdx.x & 63;
     const int ld = lane / LT, lt = lane % LT;
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2], nd = lv.n[D];
     const int nseg = (nd + 1 + PER - 1) / PER;
     const int nxt = D == 0 ? 1 : (nx + LT - 1) / LT;
