@@ -327,7 +327,8 @@ __device__ __forceinline__ void radial_solid(const Level& lv, double f, SCell<NM
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         C.ar[m] += f * (C.ar[m] * C.u[0]);
-        if (lv.plastic) C.are[m] += f * (C.are[m] * C.u[0]);
+        // a select, not a conditional store: the latter put C.are in scratch
+        C.are[m] = lv.plastic ? C.are[m] + f * (C.are[m] * C.u[0]) : C.are[m];
     }
     radial_momentum_energy(f, C.u, s0, s1[0], s2[0], s2[2], C.ru, C.E);
 #pragma unroll
@@ -373,7 +374,11 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
         for (int q = 0; q < 9; ++q) V0[q] = C.V[q];
 #pragma unroll
         for (int q = 0; q < 3; ++q) u0[q] = C.u[q];
-        for (int d = 0; d < lv.dim; ++d) {
+        // unrolled over the three directions (a loop to lv.dim kept C.are in
+        // scratch memory: the plastic-only update made its index dynamic)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d >= lv.dim) break;
             const FacesT<T>& Fd = d == 0 ? F0 : (d == 1 ? F1 : F2);
             const int i1 = i + (d == 0), j1 = j + (d == 1), k1 = k + (d == 2);
             const double dtdx = dt / lv.dx[d];
@@ -382,8 +387,10 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
             for (int m = 0; m < NM; ++m) {
                 C.al[m] += dtdx * ((double)Fd.at(m, i, j, k) - (double)Fd.at(m, i1, j1, k1) - a0[m] * (s0 - s1));
                 C.ar[m] += dtdx * ((double)Fd.at(NM + m, i, j, k) - (double)Fd.at(NM + m, i1, j1, k1));
-                if (lv.plastic)
-                    C.are[m] += dtdx * ((double)Fd.at(2 * NM + 13 + m, i, j, k) - (double)Fd.at(2 * NM + 13 + m, i1, j1, k1));
+                if (lv.plastic) {
+                    const double fl = (double)Fd.at(2 * NM + 13 + m, i, j, k), fh = (double)Fd.at(2 * NM + 13 + m, i1, j1, k1);
+                    C.are[m] += dtdx * (fl - fh);
+                }
             }
 #pragma unroll
             for (int q = 0; q < 3; ++q) C.ru[q] += dtdx * ((double)Fd.at(2 * NM + q, i, j, k) - (double)Fd.at(2 * NM + q, i1, j1, k1));
