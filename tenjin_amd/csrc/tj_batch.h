@@ -151,7 +151,7 @@ void k_b_xstage(BTab t, int which, int q, int e) {
 }
 template <bool TH, int TC, int LPS, int STAGE, typename R>
 __global__ __launch_bounds__(TC * LPS) void k_b_ystage(BTab t, int q, int e, unsigned ops) {
-    const long long gb = blockIdx.x;
+    const long long gb = xcd_block(blockIdx.x, gridDim.x, 2);  // as k_ystage
     if (gb >= bunits(t)) return;
     const int p = bfind(t.start, t.np, gb);
     const BProb& P = ro_at(t.p, p);

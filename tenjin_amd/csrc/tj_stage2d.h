@@ -545,7 +545,10 @@ __device__ __forceinline__ void ystage_body(const Level& lv, const MF& U, const 
 template <int NM, bool TH, int TC, int LPS, int STAGE, typename R = double>
 __global__ __launch_bounds__(TC * LPS) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1))) void k_ystage(Level lv, MF U, UxPlanes Ux, MF Ub, MF Uo, Hist h, Clock* clk,
                                                      unsigned ops) {
-    ystage_body<NM, TH, TC, LPS, STAGE, R>(lv, U, Ux, Ub, Uo, h, clk, ops, blockIdx.x);
+    // tiles renumbered per XCD (xcd_block): neighbouring column tiles, which
+    // share cache lines of rows whose pitch is not a multiple of the line,
+    // and the row bands above and below, which share halo rows, run in one L2
+    ystage_body<NM, TH, TC, LPS, STAGE, R>(lv, U, Ux, Ub, Uo, h, clk, ops, xcd_block(blockIdx.x, gridDim.x, 2));
 }
 
 }  // inline namespace TJ_VIEW_NS
