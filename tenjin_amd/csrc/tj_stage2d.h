@@ -249,9 +249,11 @@ void k_xstage1d(Level lv, MF U, MF Ub, MF Uo, Hist h, Clock* clk, unsigned ops, 
 
 // ------------------------------------------------------------------ y kernel
 // A block owns columns i0 .. i0 + TC - 1 and a segment of rows.  Lanes run
-// along y in segments of LPS lanes (64: one column per wave; 32: two columns
-// per wave, which halves the block and its LDS so that two blocks share a
-// compute unit and one's loads overlap the other's arithmetic): lane l of a
+// along y in segments of LPS lanes (64: one column per wave; 32 or 16: two
+// or four columns per wave).  Shorter segments recompute more halo rows
+// (THINC: 12 of 16 lanes complete a row, against 28 of 32) but make smaller
+// blocks with less LDS, so that more blocks share a compute unit and one's
+// loads after its barrier overlap the others' arithmetic: lane l of a
 // segment holds row r = s0 - H + l, the segment computes faces s0 .. s0 + CPW
 // and completes cells s0 .. s0 + CPW - 1.
 template <int NM, bool TH, int TC, int LPS, typename R>
