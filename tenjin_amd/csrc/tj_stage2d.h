@@ -7,10 +7,11 @@
 //             part of update (hllc.cpp:493-523) for each cell whose two faces
 //             the wave holds, the upper face arriving by a lane shuffle.  Out:
 //             the partially updated conserved state Ux (8 FP64 components).
-//   y kernel  a block owns TC columns x 64 rows.  The primitives of the tile
-//             (rows with a 3-row halo, and the alphas of the two neighbouring
-//             columns for Youngs' stencil) are staged in LDS with coalesced
-//             loads; each wave then reconstructs one column with lanes along y
+//   y kernel  a block owns TC columns x a band of rows (16 x 12 by
+//             default).  The primitives of the tile (the band with its halo
+//             rows, and the alphas of the two neighbouring columns for
+//             Youngs' stencil) are staged in LDS with coalesced
+//             loads; each lane segment then reconstructs one column with lanes along y
 //             (neighbours by shuffle, as in the x kernel) and computes the
 //             y faces; the face fluxes go to LDS, and a lanes-along-x phase
 //             adds the y part of update to Ux and completes the cell:
