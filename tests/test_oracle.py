@@ -452,6 +452,27 @@ def test_c_abi_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_batch_create_refuses_more_problems_than_its_limit():
+    """tj_batch_create refuses a batch of more than TJ_BATCH_MAX_PROBLEMS
+    problems (the batch kernels take one launch row per problem) with
+    TJ_EINVAL, before it touches a device."""
+    import ctypes
+
+    from tenjin_amd import LIB_PATH, LevelDesc, MFView
+
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("libtenjin.so not built")
+    header = open(os.path.join(ROOT, "include", "tenjin.h")).read()
+    limit = int(re.search(r"#define TJ_BATCH_MAX_PROBLEMS (\d+)", header).group(1))
+    einval = int(re.search(r"TJ_EINVAL = (-?\d+)", header).group(1))
+    assert limit == 65535  # the grid's y extent
+    lib_ = ctypes.CDLL(LIB_PATH)
+    descs, S, T = (LevelDesc * 1)(), (MFView * 1)(), (MFView * 1)()
+    out = ctypes.c_void_p(1)
+    rc = lib_.tj_batch_create(ctypes.byref(out), 0, limit + 1, descs, S, T)
+    assert rc == einval and not out.value
+
+
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode != 0 and "one GPU" in r.stderr
