@@ -276,6 +276,11 @@ int tj_derive(tj_ctx* ctx, tj_mf* U);
  * hook).  tj_prof_report writes "name count total_ms\n" lines; synchronises. */
 int tj_prof_enable(tj_ctx* ctx, int on);
 int tj_prof_report(tj_ctx* ctx, char* buf, int len);
+/* Restrict the event timing to launches of one name (as tj_prof_report names
+ * them); NULL or "" times every launch again.  Each timed launch adds two
+ * event records to the stream (a few microseconds), so a measurement of
+ * throughput times only the kernel it needs (no reference counterpart). */
+int tj_prof_only(tj_ctx* ctx, const char* name);
 
 /* Status word (TJ_ST_*), OR-ed since the last call with clear != 0. Synchronises. */
 int tj_status(tj_ctx* ctx, uint32_t* status, int clear);

@@ -58,6 +58,7 @@ struct tj_ctx {
     long long g_npass = 0, g_ndpass[3] = {0, 0, 0};
     // per-kernel event timing
     bool prof = false;
+    std::string prof_only;  // when set, only launches of this name are timed
     struct Rec { const char* name; hipEvent_t a, b; };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;

@@ -24,11 +24,12 @@ struct Timed {
     tj_ctx* c;
     const char* name;
     hipEvent_t a{}, b{};
-    Timed(tj_ctx* c_, const char* n) : c(c_), name(n) {
-        if (c->prof) { a = ev_get(c); hipEventRecord(a, c->stream); }
+    bool on;
+    Timed(tj_ctx* c_, const char* n) : c(c_), name(n), on(c_->prof && (c_->prof_only.empty() || c_->prof_only == n)) {
+        if (on) { a = ev_get(c); hipEventRecord(a, c->stream); }
     }
     ~Timed() {
-        if (c->prof) { b = ev_get(c); hipEventRecord(b, c->stream); c->pending.push_back({name, a, b}); }
+        if (on) { b = ev_get(c); hipEventRecord(b, c->stream); c->pending.push_back({name, a, b}); }
     }
 };
 void prof_drain(tj_ctx* c) {
@@ -1276,6 +1277,12 @@ int tj_prof_enable(tj_ctx* c, int on) {
     prof_drain(c);
     c->acc.clear();
     c->prof = on != 0;
+    return TJ_OK;
+}
+
+int tj_prof_only(tj_ctx* c, const char* name) {
+    if (!c) return TJ_EINVAL;
+    c->prof_only = name ? name : "";
     return TJ_OK;
 }
 
