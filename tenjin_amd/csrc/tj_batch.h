@@ -149,6 +149,19 @@ void k_b_xstage(BTab t, int which, int q, int e) {
     const BProb& P = ro_at(t.p, p);
     xstage_body<2, TH, R>(P.lv, which ? P.T : P.S, P.ux, bhist(P, 0, q, e), P.clk, gw - ro_at(t.start, p));
 }
+// one RK stage of every 1D problem in one launch (k_xstage1d of a single
+// level): reconstruction, fluxes and the completion of each wave's cells
+template <bool TH, int STAGE, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH && sizeof(R) == 8 ? 4 : 1)))
+void k_b_xstage1d(BTab t, int q, int e, unsigned ops) {
+    const long long gw = bwave();
+    if (gw >= bunits(t)) return;
+    const int p = bfind(t.start, t.np, gw);
+    const BProb& P = ro_at(t.p, p);
+    const long long w = gw - ro_at(t.start, p);
+    if (STAGE == 1) xstage_body<2, TH, R, 1>(P.lv, P.S, UxPlanes{}, bhist(P, 0, q, e), P.clk, w, XFin{P.S, P.T, ops});
+    else xstage_body<2, TH, R, 2>(P.lv, P.T, UxPlanes{}, bhist(P, 0, q, e), P.clk, w, XFin{P.S, P.S, ops});
+}
 template <bool TH, int TC, int LPS, int STAGE, typename R>
 __global__ __launch_bounds__(TC * LPS) void k_b_ystage(BTab t, int q, int e, unsigned ops) {
     const long long gb = xcd_block(blockIdx.x, gridDim.x, 2);  // as k_ystage
@@ -302,13 +315,15 @@ void units(const tj::BProb& P, int thinc, bool stage2d, long long u[tj_batch::NK
     for (int k = 0; k < tj_batch::NKIND; ++k) u[k] = 0;
     const int per = thinc ? 61 : 63;
     u[0] = (long long)((lv.n[0] + 1 + per - 1) / per) * lv.n[1] * lv.n[2];
+    if (stage2d) {  // x-stage waves (2D: x part of the stage kernels; 1D: the whole stage)
+        const int cpw = thinc ? tj::XGeom<true>::CPW : tj::XGeom<false>::CPW;
+        u[3] = (long long)((lv.n[0] + cpw - 1) / cpw) * lv.n[1];
+    }
     if (lv.dim > 1) {
         const long long nstrip = (lv.n[0] + 63) / 64;
         const long long nseg = (lv.n[1] + 1 + P.seglen - 1) / P.seglen;
         u[1] = nstrip * nseg * lv.n[2];
         if (stage2d) {
-            const int cpw = thinc ? tj::XGeom<true>::CPW : tj::XGeom<false>::CPW;
-            u[3] = (long long)((lv.n[0] + cpw - 1) / cpw) * lv.n[1];
             const int tc = tj::BATCH_TILE_TC[P.tile], rows = tile_rows(thinc, P.tile);
             u[4 + P.tile] = (long long)((lv.n[0] + tc - 1) / tc) * ((lv.n[1] + rows - 1) / rows);
         }
@@ -418,6 +433,27 @@ int batch_step(tj_batch* b, int prec) {
                 else if (b->thinc) launch_b_stage2d<true, double>(b, g, stage, ops);
                 else if (mixed) launch_b_stage2d<false, float>(b, g, stage, ops);
                 else launch_b_stage2d<false, double>(b, g, stage, ops);
+                continue;
+            }
+            if (g == 0 && b->stage2d) {  // 1D problems: one launch per stage, no flux planes
+                const int q = (int)(b->pass[g] & 1), e = (int)(b->dpass[g][0] & 1);
+                const unsigned gx = grid1(b->tot[g][3], 4);
+                const BTab t = tab(b, g, 3);
+                const unsigned o = stage == 1 ? 0u : ops;
+#define TJ_B_X1D(TH, R)                                                                                            \
+    do {                                                                                                           \
+        if (stage == 1) hipLaunchKernelGGL((k_b_xstage1d<TH, 1, R>), dim3(gx), dim3(256), 0, b->stream, t, q, e, o); \
+        else hipLaunchKernelGGL((k_b_xstage1d<TH, 2, R>), dim3(gx), dim3(256), 0, b->stream, t, q, e, o);           \
+    } while (0)
+                if (b->thinc && mixed) TJ_B_X1D(true, float);
+                else if (b->thinc) TJ_B_X1D(true, double);
+                else if (mixed) TJ_B_X1D(false, float);
+                else TJ_B_X1D(false, double);
+#undef TJ_B_X1D
+                if (b->thinc) {
+                    b->pass[g] += 1;
+                    b->dpass[g][0] += 1;
+                }
                 continue;
             }
             // x pass
