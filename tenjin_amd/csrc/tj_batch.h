@@ -190,11 +190,29 @@ __global__ __launch_bounds__(256) void k_b_init_p2c(BTab t, int shift) {
     init_p2c_body<2>(P.lv, P.S, shift, gb - ro_at(t.start, p));
 }
 
-// the clock kernels of the single level, one thread per problem
-__global__ void k_b_clock_dt(const BProb* P, int np, int driver) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// the clock kernels of the single level, one 64-thread block per problem.
+// Each problem has its own DT_PARTS partial slots of the CFL minimum (its
+// Level::dtpart), as a single level does: the blocks of one problem's stage
+// kernels spread their atomics over them (on one word, the 1365 blocks of a
+// 511^2 problem's stage-2 y kernel serialised to twice the kernel's time).
+// k_b_clock_dt folds and re-arms them before it takes the minimum.
+__global__ __launch_bounds__(64) void k_b_clock_dt(const BProb* P, int np, int driver) {
+    const int p = blockIdx.x;
     if (p >= np) return;
     Clock* clk = P[p].clk;
+    unsigned long long* part = P[p].lv.dtpart;
+    unsigned long long b = DT_NONE;
+    for (int q = (int)threadIdx.x; q < DT_PARTS; q += 64) {
+        const unsigned long long v = part[q];
+        b = v < b ? v : b;
+        part[q] = DT_NONE;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(b, off, 64);
+        b = o < b ? o : b;
+    }
+    if (threadIdx.x != 0) return;
+    if (b < clk->dtmin_bits) clk->dtmin_bits = b;
     if (driver) {
         if (clk->done) return;
         if (!(clk->t < clk->final_time)) {
@@ -243,9 +261,11 @@ __global__ __launch_bounds__(256) void k_b_zero_hist(const BProb* P) {
         }
     }
 }
-__global__ void k_b_clock_begin(const BProb* P, int np, double final_time) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_b_clock_begin(const BProb* P, int np, double final_time) {
+    const int p = blockIdx.x;
     if (p >= np) return;
+    for (int q = (int)threadIdx.x; q < DT_PARTS; q += 64) P[p].lv.dtpart[q] = DT_NONE;
+    if (threadIdx.x != 0) return;
     Clock* clk = P[p].clk;
     clk->t = 0.0;
     clk->dt = 0.0;
@@ -274,6 +294,7 @@ struct tj_batch {
     // dimension groups: records [g0, g1) are 1D, [g1, np) are 2D
     int g1 = 0;
     long long* d_start = nullptr;       // prefix tables, see build_tables
+    unsigned long long* d_dtpart = nullptr;  // DT_PARTS partial CFL slots per problem (in allocs)
     // [group][kind]: x waves, y waves, cell blocks of the flux-plane passes;
     // x-stage waves and y tiles of each shape of the stage kernels
     static constexpr int NKIND = 4 + tj::NBATCH_TILES;
@@ -423,7 +444,7 @@ int batch_step(tj_batch* b, int prec) {
     const bool mixed = prec == TJ_PREC_MIXED;
     const unsigned ops = TJ_OP_CLEAN_ALPHA | TJ_OP_DT | TJ_OP_NANCHECK;
     const unsigned gp = grid1(b->np, 64);
-    hipLaunchKernelGGL(k_b_clock_dt, dim3(gp), dim3(64), 0, b->stream, b->d_p, b->np, 1);
+    hipLaunchKernelGGL(k_b_clock_dt, dim3((unsigned)b->np), dim3(64), 0, b->stream, b->d_p, b->np, 1);
     for (int stage = 1; stage <= 2; ++stage) {
         const int which = stage == 1 ? 0 : 1;
         for (int g = 0; g < 2; ++g) {
@@ -533,6 +554,7 @@ int tj_batch_create(tj_batch** out, int device, int nprob, const tj_level_desc* 
         if (descs[b->order[r]].dim == 1) b->g1 = r + 1;
     int rc = TJ_OK;
     if ((rc = balloc(b, (void**)&b->d_clk, sizeof(Clock) * nprob))) return *out = b, rc;
+    if ((rc = balloc(b, (void**)&b->d_dtpart, sizeof(unsigned long long) * DT_PARTS * nprob))) return *out = b, rc;
     for (int r = 0; r < nprob; ++r) {
         const int p = b->order[r];
         const tj_level_desc& d = descs[p];
@@ -548,6 +570,7 @@ int tj_batch_create(tj_batch** out, int device, int nprob, const tj_level_desc* 
         P = BProb{};
         std::vector<uint8_t> edges[3];
         if ((rc = make_level(b->err, &d, P.lv, edges))) return *out = b, rc;
+        P.lv.dtpart = b->d_dtpart + (long long)r * DT_PARTS;
         for (int a = 0; a < 3; ++a) {
             void* e = nullptr;
             if ((rc = balloc(b, &e, edges[a].size()))) return *out = b, rc;
@@ -605,6 +628,8 @@ int tj_batch_create(tj_batch** out, int device, int nprob, const tj_level_desc* 
         P.seglen = 16;
     }
     if ((rc = balloc(b, (void**)&b->d_p, sizeof(BProb) * nprob))) return *out = b, rc;
+    for (int r = 0; r < nprob; ++r)  // the partial CFL slots start empty (begin_run re-arms them)
+        hipLaunchKernelGGL(k_dtpart_reset, dim3(1), dim3(64), 0, b->stream, b->d_dtpart + (long long)r * DT_PARTS);
     *out = b;
     return tj_batch_reset_history(b);
 }
@@ -664,7 +689,7 @@ int tj_batch_init_from_prims(tj_batch* b, int shift_pressure) {
 int tj_batch_begin_run(tj_batch* b, double final_time) {
     using namespace tj;
     if (!b) return TJ_EINVAL;
-    hipLaunchKernelGGL(k_b_clock_begin, dim3(grid1(b->np, 64)), dim3(64), 0, b->stream, b->d_p, b->np, final_time);
+    hipLaunchKernelGGL(k_b_clock_begin, dim3((unsigned)b->np), dim3(64), 0, b->stream, b->d_p, b->np, final_time);
     for (int g = 0; g < 2; ++g)
         if (b->tot[g][2])
             hipLaunchKernelGGL(k_b_sound_dt, dim3((unsigned)b->tot[g][2]), dim3(256), 0, b->stream, tab(b, g, 2));
