@@ -81,3 +81,35 @@ def test_graph_refuses_other_history_phase():
     g.stage(1)  # one direction pass in 1D: odd phase
     with pytest.raises(TenjinError):
         g.graph_launch()
+
+
+def test_prof_only_times_the_named_launches_alone():
+    """tj_prof_only: with a name, only that kernel's launches carry events (one
+    per loop body for the stage-2 y kernel of a 2D level); with None, every
+    launch again.  Timing changes no result."""
+    from tenjin_amd import GpuLevel, configs
+
+    pb = configs.shock_inclusion_2d(nx=128, ny=64, box=32)
+    print(f"\n[case] prof_only {pb.name} n={pb.n[:pb.dim]}", flush=True)
+    g = GpuLevel(pb)
+    g.init_from_prims(pb.prims, pb.shift_pressure)
+    g.begin_run()
+    ref = GpuLevel(pb)
+    ref.init_from_prims(pb.prims, pb.shift_pressure)
+    ref.begin_run()
+    g.prof_enable(True)
+    g.prof_only("ystage2_post")
+    g.advance(3)
+    only = g.prof_report()
+    assert set(only) == {"ystage2_post"} and only["ystage2_post"][0] == 3
+    g.prof_enable(True)  # clears the accumulators
+    g.prof_only(None)
+    g.advance(2)
+    every = g.prof_report()
+    g.prof_enable(False)
+    assert {"xstage1", "xstage2", "ystage1", "ystage2_post"} <= set(every)
+    assert all(cnt == 2 for cnt, _ in every.values())
+    ref.advance(5)
+    assert np.array_equal(g.get_state(), ref.get_state()) and g.clock() == ref.clock()
+    g.close()
+    ref.close()
