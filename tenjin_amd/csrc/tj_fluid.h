@@ -261,13 +261,13 @@ __device__ __forceinline__ unsigned root_find(const Level& lv, const double* al,
         x.rb = mix_rhob(lam, x.rk, x.ra);
         return 0u;
     }
-    if (lam > 1.0 - 1E-3) {
+    // lambda at either end: both phases at the mixture's density.  One branch
+    // with one store order: two branches storing ra and rb in opposite orders
+    // were merged into a store through a select of the two fields' addresses,
+    // which kept x in scratch memory in every mixture kernel.
+    if (lam > 1.0 - 1E-3 || lam < 1E-3) {
         x.ra = x.rk;
         x.rb = x.rk;
-        return 0u;
-    } else if (lam < 1E-3) {
-        x.rb = x.rk;
-        x.ra = x.rk;
         return 0u;
     }
     double a = lam * x.rk + 1E-10;
