@@ -435,6 +435,17 @@ __device__ __forceinline__ void p2c_sound_s(const Level& lv, const SPrim<NM, R>&
     S.a = sound_s<NM, RW, R>(lv, S.al, S.ar, P.r, rho, P.p, hj2);
 }
 
+// The phase densities and strains of conservativeToPrimitive alone, formed
+// as c2p_s forms them.
+template <int NM, typename R>
+__device__ __forceinline__ void c2p_rk_e(const Level& lv, const R* al, const R* ar, const R* are, R* rk, R* e) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        rk[m] = tdiv(ar[m], al[m]);
+        e[m] = mat<R>(lv, m).plastic ? tdiv(are[m], ar[m]) : R(0);
+    }
+}
+
 // conservativeToPrimitive (boxaccesscellarray.cpp:67-122) of a solid cell;
 // returns HJ2 and the deviatoric Hencky strain in dH.
 template <int NM, bool RW, typename R>

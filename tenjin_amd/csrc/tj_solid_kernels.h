@@ -406,10 +406,18 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
                 }
         }
         double dH[9];
-        C.hj2 = c2p_s<NM, RW>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
+        // Stage 2 with the plastic relaxation: the conservativeToPrimitive that
+        // ends it rewrites rho, u, p, HJ2 and dH, so this one forms only what
+        // the relaxation reads (rho_k and epsilon, as c2p_s does).  The values
+        // are those of the full sequence.
+        const bool relax = STAGE == 2 && (ops & 1u) != 0;
+        SCell<NM> B;
+        if (STAGE == 2) load_scell<NM, true>(lv, Ub, i, j, k, B);  // in flight across the branch
+        if (relax) {
+            c2p_rk_e<NM>(lv, C.al, C.ar, C.are, C.rk, C.e);
+            C.rho = C.p = C.hj2 = 0.0;  // rewritten after the relaxation (no loads of Uin's values)
+        } else C.hj2 = c2p_s<NM, RW>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
         if (STAGE == 2) {
-            SCell<NM> B;
-            load_scell<NM, true>(lv, Ub, i, j, k, B);
             // U1 = (U * 1/2) + (U2 * 1/2) over every component
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
@@ -430,7 +438,7 @@ __global__ __launch_bounds__(256) void k_update_s(Level lv, MF Uin, MF Ub, MF Uo
 #pragma unroll
             for (int q = 0; q < 9; ++q) C.V[q] = B.V[q] * (1.0 / 2.0) + C.V[q] * (1.0 / 2.0);
             C.hj2 = B.hj2 * (1.0 / 2.0) + C.hj2 * (1.0 / 2.0);
-            if (ops & 1u) {  // plastic relaxation, then conservativeToPrimitive
+            if (relax) {  // plastic relaxation, then conservativeToPrimitive
                 st |= plastic_cell<NM, RW>(lv, C.al, C.ar, C.rk, C.e, C.are, C.V);
                 C.hj2 = c2p_s<NM, RW>(lv, C.al, C.ar, C.are, C.ru, C.E, C.V, C.rk, C.e, C.rho, C.u, C.p, dH);
             }
