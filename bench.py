@@ -22,13 +22,17 @@ Prints one JSON line:  metric/value (Mcell-updates/s = cells x steps / s),
 roofline of the dominant kernel (HIP-event times on the library's stream over
 the timed region), step roofline (8 (3 N_s + 2 N_c) bytes per cell-update) and
 the CPU baseline (the oracle restatement on this host's cores, a bounded sample
-of the same workload).
+of the same workload).  The full-size default run also reports, under
+`companions`, the 3D plate impact's mixed-over-FP64 speed-up with its L1 error
+and the 2D plate impact's gradient cost over the forward run, each from a
+child run of this script (`--companions on|off` forces them either way).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -133,6 +137,10 @@ def parse():
                          "march = flux planes with the update inside the y pass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--companions", choices=["auto", "on", "off"], default="auto",
+                    help="also run the 3D plate impact (FP64 and mixed) and the 2D plate impact's gradient as child "
+                         "runs and report them in the JSON line; auto = on the full-size default workload with "
+                         "the CPU baseline")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a plain 1 GiB device copy first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -923,7 +931,58 @@ def main():
                                                args.cpu_seconds, 3)
     else:
         out["cpu_baseline"] = None
+    if args.companions == "on" or (args.companions == "auto" and args.workload == "inclusion2d" and
+                                   (args.nx, args.ny) == (2048, 1024) and not args.no_cpu and not args.calib_copy):
+        out["companions"] = companion_runs(args)
     print(json.dumps(out), flush=True)
+
+
+def _child_line(cmd, timeout_s: float):
+    """Runs bench.py as a child process (not an exec) and returns its JSON
+    line, or an error record."""
+    try:
+        r = subprocess.run(cmd, cwd=os.path.dirname(os.path.abspath(__file__)), capture_output=True, text=True,
+                           timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {timeout_s:.0f} s"}
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit status {r.returncode}: {r.stderr.strip()[-300:]}"}
+    return json.loads(lines[-1])
+
+
+def companion_runs(args):
+    """The two other headline figures beside the 2D level's rate, each from a
+    bench.py child run with its own JSON line: the mixed form's speed-up
+    over FP64 on the 3D elastoplastic plate impact (with the density, pressure
+    and volume-fraction L1 error between the two forms), and the cost of the
+    six-parameter gradient of the 2D plate impact's probe loss over one
+    forward run (tangent-linear form)."""
+    me = os.path.abspath(__file__)
+    comp = {}
+    t0 = time.perf_counter()
+    d = _child_line([sys.executable, me, "--workload", "plate3d", "--n3", str(args.n3), "--box", str(args.box),
+                     "--steps", "40", "--warmup", "2", "--no-cpu"], 600)
+    if "error" in d:
+        comp["plate3d"] = d
+    else:
+        m = d["mixed"]
+        comp["plate3d"] = {"workload": d["config"]["workload"], "steps": d["steps"],
+                           "fp64": d["value"], "mixed": m["value"], "mixed_over_fp64": m["speedup_vs_fp64"],
+                           "l1_rel_err_vs_fp64": m["l1_rel_err_vs_fp64"], "status": [d["status"], m["status"]]}
+    d = _child_line([sys.executable, me, "--workload", "plate2d_grad", "--n2g", str(args.n2g), "--no-cpu"], 600)
+    if "error" in d:
+        comp["plate2d_grad"] = d
+    else:
+        comp["plate2d_grad"] = {"workload": d["config"]["workload"], "steps": d["steps"],
+                                "form": "tangent-linear (forward mode), one pass per group of directions",
+                                "directions": d["config"]["directions"], "forward_ms": d["forward_ms"],
+                                "gradient_ms": d["gradient_ms"], "gradient_over_forward": d["adjoint_over_forward"],
+                                "per_direction_over_forward": d["per_direction_over_forward"],
+                                "gradient_rel_err_vs_central_differences":
+                                    d["gradient_rel_err_vs_central_differences"], "status": d["status"]}
+    comp["wall_s"] = round(time.perf_counter() - t0, 1)
+    return comp
 
 
 if __name__ == "__main__":

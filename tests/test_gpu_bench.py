@@ -41,3 +41,26 @@ def test_bench_json_contract():
     assert cb["value"] > 0 and cb["kind"] == "port" and cb["cores"] >= 1 and cb["sample"]
     m = d["mixed"]
     assert m["value"] > 0 and m["l1_rel_err_vs_fp64"]["rho"] <= 1e-5
+
+
+def test_bench_companion_runs():
+    """--companions on: the 3D plate impact's mixed/FP64 figures and the 2D
+    plate impact's gradient cost arrive in the same JSON line, from child runs
+    at small sizes here (32^3, 64^2)."""
+    print("\n[case] bench.py --companions on --n3 32 --n2g 64", flush=True)
+    cmd = [sys.executable, os.path.join(ROOT, "bench.py"), "--nx", "256", "--ny", "128", "--box", "64",
+           "--steps", "3", "--warmup", "1", "--no-cpu", "--companions", "on", "--n3", "32", "--n2g", "64"]
+    r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    assert len(lines) == 1
+    c = json.loads(lines[0])["companions"]
+    p3, g = c["plate3d"], c["plate2d_grad"]
+    assert "error" not in p3, p3
+    assert "error" not in g, g
+    assert p3["fp64"] > 0 and p3["mixed"] > 0 and p3["status"] == [0, 0] and p3["steps"] == 40
+    assert abs(p3["mixed_over_fp64"] - p3["mixed"] / p3["fp64"]) <= 0.01 * p3["mixed_over_fp64"]
+    assert p3["l1_rel_err_vs_fp64"]["rho"] <= 1e-5
+    assert g["directions"] == 6 and g["forward_ms"] > 0 and g["gradient_ms"] > g["forward_ms"]
+    assert abs(g["gradient_over_forward"] - g["gradient_ms"] / g["forward_ms"]) <= 0.01 * g["gradient_over_forward"]
+    assert g["status"] == [0, 0]
