@@ -25,7 +25,9 @@ the CPU baseline (the oracle restatement on this host's cores, a bounded sample
 of the same workload).  The full-size default run also reports, under
 `companions`, the 3D plate impact's mixed-over-FP64 speed-up with its L1 error
 and the 2D plate impact's gradient cost over the forward run, each from a
-child run of this script (`--companions on|off` forces them either way).
+child run of this script (`--companions on|off` forces them either way),
+and under `machine` the HBM and FMA rates that plain kernels reach on this
+device (tenjin_amd/csrc/tj_peaks.hip; `--peaks` asks for them on any run).
 """
 from __future__ import annotations
 
@@ -141,6 +143,9 @@ def parse():
                     help="also run the 3D plate impact (FP64 and mixed) and the 2D plate impact's gradient as child "
                          "runs and report them in the JSON line; auto = on the full-size default workload with "
                          "the CPU baseline")
+    ap.add_argument("--peaks", action="store_true",
+                    help="measure the device's HBM and FMA rates with plain kernels and report them (always on the "
+                         "full-size default run)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a plain 1 GiB device copy first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -931,10 +936,51 @@ def main():
                                                args.cpu_seconds, 3)
     else:
         out["cpu_baseline"] = None
-    if args.companions == "on" or (args.companions == "auto" and args.workload == "inclusion2d" and
-                                   (args.nx, args.ny) == (2048, 1024) and not args.no_cpu and not args.calib_copy):
+    full_default = (args.workload == "inclusion2d" and (args.nx, args.ny) == (2048, 1024) and not args.no_cpu and
+                    not args.calib_copy)
+    if args.peaks or full_default:
+        peaks = machine_peaks()
+        out["machine"] = peaks
+        if isinstance(peaks.get("hbm_copy_gbs"), float):
+            out["roofline"]["frac_of_measured_copy"] = round(out["roofline"]["achieved"] / peaks["hbm_copy_gbs"], 4)
+            out["step_roofline"]["frac_of_measured_copy"] = round(step_gbs / peaks["hbm_copy_gbs"], 4)
+        if isinstance(peaks.get("fp64_fma_tflops"), float) and "valu" in out["roofline"]:
+            v = out["roofline"]["valu"]
+            v["frac_of_measured_fma"] = round(v["achieved"] / peaks["fp64_fma_tflops"], 4)
+    if args.companions == "on" or (args.companions == "auto" and full_default):
         out["companions"] = companion_runs(args)
     print(json.dumps(out), flush=True)
+
+
+def machine_peaks(nbytes: int = 4 << 30):
+    """What plain kernels reach on this device (tenjin_amd/libtenjin_peaks.so,
+    kernels of csrc/tj_peaks.hip): HBM read, write and copy rates over a
+    buffer of `nbytes` (copy counts the bytes read plus written), and the FMA
+    throughput in FP64 and in FP32 without packed instructions.  The
+    datasheet peaks stay the rooflines' `peak`; these give the same fractions
+    against what the device delivers."""
+    import ctypes
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tenjin_amd", "libtenjin_peaks.so")
+    if not os.path.exists(path):
+        return {"error": "tenjin_amd/libtenjin_peaks.so is not built"}
+    lib = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.tjp_hbm.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp, dp]
+    lib.tjp_fma.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+    lib.tjp_hbm.restype = lib.tjp_fma.restype = ctypes.c_int
+    out = {"buffer_bytes": nbytes}
+    for kind, name in ((0, "hbm_read_gbs"), (1, "hbm_write_gbs"), (2, "hbm_copy_gbs")):
+        rate, ms = ctypes.c_double(), ctypes.c_double()
+        rc = lib.tjp_hbm(kind, nbytes, 20, ctypes.byref(rate), ctypes.byref(ms))
+        out[name] = round(rate.value, 1) if rc == 0 else f"error {rc}"
+    for fp64, name in ((1, "fp64_fma_tflops"), (0, "fp32_fma_tflops")):
+        rate, ms = ctypes.c_double(), ctypes.c_double()
+        rc = lib.tjp_fma(fp64, 4096, 10, ctypes.byref(rate), ctypes.byref(ms))
+        out[name] = round(rate.value, 2) if rc == 0 else f"error {rc}"
+    out["kernels"] = ("read: one sum per block; write: fill; copy: read + write; 16-byte accesses, grid-stride, "
+                      "2048 blocks of 256; FMA: 8 chains per lane, 8 waves per SIMD, FP32 unpacked")
+    return out
 
 
 def _child_line(cmd, timeout_s: float):

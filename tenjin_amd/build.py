@@ -3,9 +3,10 @@ FP64 / mixed forms and the tangent-linear form) into
 tenjin_amd/libtenjin.so with hipcc (in-tree, so the built library travels with
 the source tree), the C++ host interface tenjin_amd/csrc/host/ into
 tenjin_amd/libtenjin_host.so, and the C++ time-loop driver used by the tests
-(tests/host/main_loop.cpp) into tests/host/main_loop, and the libtorch
+(tests/host/main_loop.cpp) into tests/host/main_loop, the libtorch
 autograd Function over the tangent-linear form (csrc/torch/) into
-tenjin_amd/tenjin_torch.so.
+tenjin_amd/tenjin_torch.so, and the device-peak kernels (csrc/tj_peaks.hip)
+into tenjin_amd/libtenjin_peaks.so.
 
 Flags:
   --offload-arch=gfx950            MI355X (CDNA4) only
@@ -76,6 +77,19 @@ def build(verbose: bool = True, extra=None) -> str:
         _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
     return LIB
+
+
+PEAKS_LIB = os.path.join(HERE, "libtenjin_peaks.so")
+
+
+def build_peaks(verbose: bool = True) -> str:
+    """The device-peak measurement kernels (csrc/tj_peaks.hip ->
+    tenjin_amd/libtenjin_peaks.so, loaded by bench.py), with the library's
+    flags."""
+    src = os.path.join(CSRC, "tj_peaks.hip")
+    _run([HIPCC] + FLAGS + [src, "-o", PEAKS_LIB + ".tmp"], verbose)
+    os.replace(PEAKS_LIB + ".tmp", PEAKS_LIB)
+    return PEAKS_LIB
 
 
 TORCH_LIB = os.path.join(HERE, "tenjin_torch.so")

@@ -64,3 +64,21 @@ def test_bench_companion_runs():
     assert g["directions"] == 6 and g["forward_ms"] > 0 and g["gradient_ms"] > g["forward_ms"]
     assert abs(g["gradient_over_forward"] - g["gradient_ms"] / g["forward_ms"]) <= 0.01 * g["gradient_over_forward"]
     assert g["status"] == [0, 0]
+
+
+def test_machine_peaks_are_physical():
+    """The streaming and FMA kernels of libtenjin_peaks.so report rates that a
+    device can deliver: positive, and at most the MI355X datasheet figures
+    (8 TB/s HBM3E; 78.6 TFLOP/s FP64 vector; FP32 without packed
+    instructions at most the packed 157.3) over a 1 GiB buffer, which no
+    cache holds."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    print("\n[case] machine peaks, 1 GiB buffers", flush=True)
+    p = bench.machine_peaks(1 << 30)
+    assert "error" not in p, p
+    for k in ("hbm_read_gbs", "hbm_write_gbs", "hbm_copy_gbs"):
+        assert isinstance(p[k], float) and 500.0 < p[k] <= 8000.0, (k, p[k])
+    assert isinstance(p["fp64_fma_tflops"], float) and 5.0 < p["fp64_fma_tflops"] <= 78.6
+    assert isinstance(p["fp32_fma_tflops"], float) and 5.0 < p["fp32_fma_tflops"] <= 157.3

@@ -452,6 +452,26 @@ def test_c_abi_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_peaks_library_loads_and_rejects_bad_arguments():
+    """The device-peak kernels' library (bench.py's machine figures) loads
+    without a device and refuses invalid arguments before any HIP call."""
+    import ctypes
+
+    path = os.path.join(ROOT, "tenjin_amd", "libtenjin_peaks.so")
+    if not os.path.exists(path):
+        pytest.skip("libtenjin_peaks.so not built")
+    lib_ = ctypes.CDLL(path)
+    assert hasattr(lib_, "tjp_hbm") and hasattr(lib_, "tjp_fma")
+    lib_.tjp_hbm.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    lib_.tjp_fma.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    r, t = ctypes.c_double(), ctypes.c_double()
+    assert lib_.tjp_hbm(3, 1 << 20, 1, ctypes.byref(r), ctypes.byref(t)) == 1  # no such kind
+    assert lib_.tjp_hbm(0, 16, 1, ctypes.byref(r), ctypes.byref(t)) == 1  # smaller than one block's reads
+    assert lib_.tjp_hbm(0, 1 << 20, 0, ctypes.byref(r), ctypes.byref(t)) == 1
+    assert lib_.tjp_fma(1, 0, 1, ctypes.byref(r), ctypes.byref(t)) == 1
+    assert lib_.tjp_fma(1, 8, 1, None, ctypes.byref(t)) == 1
+
+
 def test_batch_create_refuses_more_problems_than_its_limit():
     """tj_batch_create refuses a batch of more than TJ_BATCH_MAX_PROBLEMS
     problems (the batch kernels take one launch row per problem) with
