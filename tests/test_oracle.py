@@ -493,6 +493,22 @@ def test_batch_create_refuses_more_problems_than_its_limit():
     assert rc == einval and not out.value
 
 
+def test_bench_companion_child_errors_become_records():
+    """A companion run that fails, prints no JSON line or overruns its time
+    gives an error record in the bench line instead of ending the bench."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    r = bench._child_line([sys.executable, "-c", "import sys; sys.stderr.write('boom'); sys.exit(3)"], 60)
+    assert r["error"].startswith("exit status 3") and "boom" in r["error"]
+    r = bench._child_line([sys.executable, "-c", "print('no json here')"], 60)
+    assert r["error"].startswith("exit status 0")
+    r = bench._child_line([sys.executable, "-c", "import time; time.sleep(5)"], 0.5)
+    assert r["error"].startswith("timed out")
+    r = bench._child_line([sys.executable, "-c", "print('x'); print('{\"a\": 1}')"], 60)
+    assert r == {"a": 1}
+
+
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode != 0 and "one GPU" in r.stderr
