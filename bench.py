@@ -754,6 +754,7 @@ def main():
     ncell = pb.ncells
 
     tuned = {}
+    sampled = {}
 
     def timed(prec):
         """warm-up steps, then exactly args.steps timed steps of the device loop
@@ -784,16 +785,23 @@ def main():
         # the timed steps: events around the dominant kernel's launches only
         # (each event record costs the stream a few microseconds)
         dname = max(warm.items(), key=lambda kv: kv[1][1])[0] if warm else None
+        # events on about 50 of its launches: every launch in short runs, one
+        # in ten over C2's 500 steps
+        per_step = warm[dname][0] / max(1, args.warmup - first) if warm else 1.0
+        every = max(1, int(args.steps * per_step) // 50)
         g.prof_enable(True)
         g.prof_only(dname)
+        g.prof_sample(every)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         g.advance(args.steps, prec)
         g.sync()
         t1 = time.perf_counter()
         prof = g.prof_report()
+        g.prof_sample(1)
         g.prof_only(None)
         g.prof_enable(False)
+        sampled[prec] = every
         status = g.status()
         t_end, _, nsteps_done = g.clock()
         require_clean_run(status, nsteps_done, args.warmup + args.steps, f"{args.workload} prec {prec}")
@@ -921,6 +929,7 @@ def main():
         "kernel_avg_ms": kernel_ms,
         "kernel_avg_ms_over": ("warm-up steps after the first two (HIP events on every launch); the roofline "
                                "kernel's avg_launch_ms is over the timed steps, which carry events on that kernel alone"
+                               f" (on one launch in {sampled.get(prec, 1)} of it, the first included)"
                                if args.warmup > 2 else "the timed steps (HIP events on every launch)"),
         "status": status,
         "device_steps": nsteps_done,

@@ -281,6 +281,11 @@ int tj_prof_report(tj_ctx* ctx, char* buf, int len);
  * event records to the stream (a few microseconds), so a measurement of
  * throughput times only the kernel it needs (no reference counterpart). */
 int tj_prof_only(tj_ctx* ctx, const char* name);
+/* Of the launches that tj_prof_enable / tj_prof_only would time, time the
+ * first and then every `every`-th one (1: all of them, the default), so a
+ * long run carries fewer event records; the count in tj_prof_report is the
+ * number timed.  TJ_EINVAL for every < 1 (no reference counterpart). */
+int tj_prof_sample(tj_ctx* ctx, int every);
 
 /* Status word (TJ_ST_*), OR-ed since the last call with clear != 0. Synchronises. */
 int tj_status(tj_ctx* ctx, uint32_t* status, int clear);

@@ -112,7 +112,7 @@ EXPORTS = [
     "tj_create", "tj_destroy", "tj_last_error", "tj_stream", "tj_bind_level", "tj_ncomp",
     "tj_comp_start", "tj_compute_dt", "tj_driver_dt", "tj_set_dt", "tj_get_clock", "tj_rk_stage",
     "tj_post_step", "tj_step", "tj_run", "tj_init_from_prims", "tj_c2p", "tj_derive", "tj_status",
-    "tj_reset_history", "tj_synchronize", "tj_prof_enable", "tj_prof_report", "tj_prof_only", "tj_set_tuning",
+    "tj_reset_history", "tj_synchronize", "tj_prof_enable", "tj_prof_report", "tj_prof_only", "tj_prof_sample", "tj_set_tuning",
     "tj_begin_run", "tj_advance_steps", "tj_lincomb", "tj_graph_capture", "tj_graph_launch",
     "tj_fabset_create", "tj_fabset_view", "tj_fabset_destroy", "tj_level_boxes", "tj_box_op",
     "tj_batch_set_tile", "tj_batch_set_stage2d",
@@ -164,6 +164,7 @@ def load_library(path: str = LIB_PATH):
     L.tj_prof_enable.argtypes = [vp, ip]
     L.tj_prof_report.argtypes = [vp, ctypes.c_char_p, ip]
     L.tj_prof_only.argtypes = [vp, ctypes.c_char_p]
+    L.tj_prof_sample.argtypes = [vp, ctypes.c_int]
     L.tj_set_tuning.argtypes = [vp, ip, ip]
     L.tj_begin_run.argtypes = [vp, mfp, dp]
     L.tj_advance_steps.argtypes = [vp, mfp, mfp, ip, ip]
@@ -378,6 +379,10 @@ class GpuLevel:
     def prof_only(self, name=None):
         """Time only the launches named `name` (None: every launch)."""
         self._chk(self.lib.tj_prof_only(self.ctx, name.encode() if name else None), "tj_prof_only")
+
+    def prof_sample(self, every: int = 1):
+        """Time the first and then every `every`-th qualifying launch."""
+        self._chk(self.lib.tj_prof_sample(self.ctx, int(every)), "tj_prof_sample")
 
     def prof_report(self) -> dict:
         buf = ctypes.create_string_buffer(1 << 16)

@@ -59,6 +59,8 @@ struct tj_ctx {
     // per-kernel event timing
     bool prof = false;
     std::string prof_only;  // when set, only launches of this name are timed
+    int prof_every = 1;     // of the launches that qualify, every prof_every-th is timed
+    long long prof_seen = 0;
     struct Rec { const char* name; hipEvent_t a, b; };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;

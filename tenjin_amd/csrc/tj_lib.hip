@@ -25,9 +25,11 @@ struct Timed {
     const char* name;
     hipEvent_t a{}, b{};
     bool on;
-    Timed(tj_ctx* c_, const char* n) : c(c_), name(n), on(c_->prof && (c_->prof_only.empty() || c_->prof_only == n)) {
+    Timed(tj_ctx* c_, const char* n)
+        : c(c_), name(n), on(c_->prof && (c_->prof_only.empty() || c_->prof_only == n) && sample(c_)) {
         if (on) { a = ev_get(c); hipEventRecord(a, c->stream); }
     }
+    static bool sample(tj_ctx* c) { return c->prof_every <= 1 || c->prof_seen++ % c->prof_every == 0; }
     ~Timed() {
         if (on) { b = ev_get(c); hipEventRecord(b, c->stream); c->pending.push_back({name, a, b}); }
     }
@@ -1283,6 +1285,14 @@ int tj_prof_enable(tj_ctx* c, int on) {
 int tj_prof_only(tj_ctx* c, const char* name) {
     if (!c) return TJ_EINVAL;
     c->prof_only = name ? name : "";
+    c->prof_seen = 0;
+    return TJ_OK;
+}
+
+int tj_prof_sample(tj_ctx* c, int every) {
+    if (!c || every < 1) return TJ_EINVAL;
+    c->prof_every = every;
+    c->prof_seen = 0;
     return TJ_OK;
 }
 

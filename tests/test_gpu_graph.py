@@ -106,10 +106,21 @@ def test_prof_only_times_the_named_launches_alone():
     g.prof_only(None)
     g.advance(2)
     every = g.prof_report()
-    g.prof_enable(False)
     assert {"xstage1", "xstage2", "ystage1", "ystage2_post"} <= set(every)
     assert all(cnt == 2 for cnt, _ in every.values())
-    ref.advance(5)
+    # tj_prof_sample: the first and then every 4th named launch carries events
+    g.prof_enable(True)
+    g.prof_only("ystage2_post")
+    g.prof_sample(4)
+    g.advance(9)
+    sampled = g.prof_report()
+    g.prof_sample(1)
+    g.prof_only(None)
+    g.prof_enable(False)
+    assert set(sampled) == {"ystage2_post"} and sampled["ystage2_post"][0] == 3  # launches 1, 5, 9
+    with pytest.raises(RuntimeError):
+        g.prof_sample(0)
+    ref.advance(14)
     assert np.array_equal(g.get_state(), ref.get_state()) and g.clock() == ref.clock()
     g.close()
     ref.close()
